@@ -1,0 +1,153 @@
+/*
+ * fgssjoin.h -- C ABI of the MI355X-native fgssjoin set-similarity self-join.
+ *
+ * The reference (RafaelQuirino/new_fgssjoin) has no library: its "operator"
+ * boundary is four C++ host functions called by its CLI main
+ * (src/fgssjoin_cli.cu:61,86,91,100).  Each entry point below names the
+ * reference interface it replaces.  Plain pointers and sizes only; every call
+ * returns an fgss_status and never exits the process (the reference calls
+ * exit(1) from gpuAssert, src/util.cu:43-49, and from its loader).
+ *
+ * Data model.  A collection is a list of records; each record is a set of
+ * int32 token ids (ascending, unique).  Internally records are kept in the
+ * reference's order: the .bin file order reversed (src/fgssjoin/data.cu:99-112),
+ * i.e. descending length for files written by the q-gram tokenizer.  A result
+ * pair (q, s) always has internal index q < s and is printed as
+ * "id[q] id[s] similarity" (src/fgssjoin/block.cu:193).
+ *
+ * Threading.  Handles are thread-compatible: one thread per handle at a time.
+ * All device work runs on the current HIP device.
+ */
+#ifndef FGSSJOIN_H
+#define FGSSJOIN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    FGSS_OK = 0,
+    FGSS_ERR_ARG = 1,         /* invalid argument: null, t outside (0,1), q < 1 */
+    FGSS_ERR_IO = 2,          /* file missing or unreadable ("File error")      */
+    FGSS_ERR_FORMAT = 3,      /* truncated or malformed .bin ("Reading error.") */
+    FGSS_ERR_DEVICE = 4,      /* a HIP runtime call failed                      */
+    FGSS_ERR_NOMEM = 5,       /* host or device allocation failed               */
+    FGSS_ERR_UNSUPPORTED = 6  /* collection breaks a documented limit           */
+} fgss_status;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *fgss_last_error(void);
+/* Library version string. */
+const char *fgss_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Collections (replaces sets_t / ppjoin_format / prepare_data)       */
+/* ------------------------------------------------------------------ */
+typedef struct fgss_collection fgss_collection;
+
+typedef struct {
+    int64_t n_sets;        /* "# Records"  (src/fgssjoin_cli.cu:62)  */
+    int64_t n_tokens;      /* "# Tokens"   (src/fgssjoin_cli.cu:64)  */
+    int32_t n_terms;       /* "# Terms" = max token + 1 (data.cu:77-78) */
+    int32_t max_len;       /* longest record                          */
+    double average_size;   /* "# Average size" (data.cu:100-104)      */
+    int32_t sorted_desc;   /* 1 if lengths are non-increasing in internal order */
+    int32_t on_device;     /* 1 once fgss_collection_upload has run   */
+} fgss_collection_info;
+
+/* Load a reference ".bin" file: per record [int32 id][int32 len][len x int32].
+ * Replaces ppjoin_format (include/fgssjoin/data.cuh:65-66, src/fgssjoin/data.cu:31-126). */
+int fgss_collection_from_bin(const char *path, fgss_collection **out);
+/* Same, from an in-memory int32 stream of n_ints values. */
+int fgss_collection_from_bin_buffer(const int32_t *ints, int64_t n_ints, fgss_collection **out);
+
+/* Tokenize a text file (one record per non-empty line) into q-gram sets with
+ * global-frequency token ids, as the README's "-f <file> -q <q>" front end
+ * (README.md:12-15) and the preprocess library (src/preprocess/string/qgram.cpp:20-49,
+ * src/preprocess_cli.cpp:266-340) describe.  Record ids are 1-based line numbers
+ * among non-empty lines; lines shorter than q chars yield no record. */
+int fgss_collection_from_text(const char *path, int q, fgss_collection **out);
+int fgss_collection_from_text_buffer(const char *text, int64_t nbytes, int q, fgss_collection **out);
+
+/* Write the collection as a reference .bin (records by size ascending, i.e.
+ * internal order reversed) and, when loc_path is non-null and the collection
+ * came from text, the .loc stream (q-gram start offset per stored token). */
+int fgss_collection_write_bin(const fgss_collection *c, const char *path, const char *loc_path);
+
+int fgss_collection_get_info(const fgss_collection *c, fgss_collection_info *info);
+/* Host views of the internal-order arrays (valid until the collection is freed). */
+int fgss_collection_arrays(const fgss_collection *c, const int32_t **id, const int32_t **len,
+                           const int64_t **pos, const int32_t **tokens);
+
+/* Copy the collection to HBM on the current device and keep it resident.
+ * Replaces the H2D part of prepare_data (src/fgssjoin/data.cu:133-153). */
+int fgss_collection_upload(fgss_collection *c);
+
+void fgss_collection_free(fgss_collection *c);
+
+/* ------------------------------------------------------------------ */
+/* Self-join (replaces inverted_index + process_blocks)               */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int64_t index_entries;     /* "# Inverted Index Entries" (fgssjoin_cli.cu:92)    */
+    int64_t probe_tokens;      /* sum of max_prefix(|q|)                              */
+    int64_t postings_scanned;  /* postings with s > q inside the size window          */
+    int64_t candidates;        /* "# Candidates"    (block.cu:209)                    */
+    int64_t intersections;     /* "# Intersections" (block.cu:210)                    */
+    int64_t results;           /* "# Results"       (block.cu:211)                    */
+    int64_t verify_tokens;     /* token reads of suffix-merge verification            */
+    double ms_index;           /* mid-prefix selection + inverted index build         */
+    double ms_join;            /* candidate generation + verification + compaction    */
+    double ms_copy;            /* device -> host copy of the result pairs             */
+    double ms_total;           /* whole call                                          */
+} fgss_join_stats;
+
+typedef struct {
+    int32_t query_chunk;       /* queries per work item, 0 = library default          */
+    int32_t table_slots;       /* LDS candidate table slots per work item, 0 = default */
+    int32_t verify_only;       /* reserved, must be 0                                 */
+    int32_t reserved;
+} fgss_join_opts;
+
+typedef struct {
+    int64_t n_pairs;
+    int32_t *q;             /* internal index of the query record (q < s)      */
+    int32_t *s;             /* internal index of the source record             */
+    int32_t *rid;           /* id[q] as printed                                */
+    int32_t *sid;           /* id[s] as printed                                */
+    int32_t *overlap;       /* |Q intersect S|                                 */
+    float *similarity;      /* (float)ov / (float)(|Q|+|S|-ov)                  */
+    fgss_join_stats stats;
+} fgss_pairs;
+
+/* Self-join at Jaccard threshold t in (0,1).  The collection is uploaded on
+ * first use.  Pairs are returned sorted by (q, s).  Replaces inverted_index
+ * (include/fgssjoin/index.cuh:99-100) + process_blocks
+ * (include/fgssjoin/block.cuh:13-14), whose results were printed instead of
+ * returned.  opts may be null. */
+int fgss_selfjoin(fgss_collection *c, float threshold, const fgss_join_opts *opts, fgss_pairs **out);
+void fgss_pairs_free(fgss_pairs *p);
+
+/* Print pairs as the reference does: "%d %d %f\n" = id[q] id[s] similarity
+ * (src/fgssjoin/block.cu:193). */
+int fgss_pairs_print(const fgss_pairs *p, void *file /* FILE* */);
+
+/* ------------------------------------------------------------------ */
+/* Synthetic workloads (no reference counterpart; used by the bench)  */
+/* ------------------------------------------------------------------ */
+typedef enum {
+    FGSS_GEN_TITLES = 0,   /* 4-15 Zipf words, 20-120 chars, 10% copies with 1-3 edits  */
+    FGSS_GEN_LONG = 1      /* 200-1200 chars of Zipf words, 5% copies with 1-10 edits  */
+} fgss_gen_kind;
+
+/* Generate n_records lines of seeded synthetic text ('\n'-terminated) into a
+ * malloc'd buffer; release with fgss_free. */
+int fgss_generate_text(int kind, int64_t n_records, uint64_t seed, char **text, int64_t *nbytes);
+void fgss_free(void *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FGSSJOIN_H */

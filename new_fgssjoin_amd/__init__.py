@@ -1,0 +1,266 @@
+"""new_fgssjoin_amd -- MI355X-native fgssjoin (set-similarity self-join).
+
+Python mirror of the reference's host interface over the C ABI in
+``include/fgssjoin.h`` (library ``libfgssjoin.so``, built in-tree by
+``new_fgssjoin_amd/build.py``).  The reference's host functions map as:
+
+  ppjoin_format(path)            -> Collection.from_bin(path)      (src/fgssjoin/data.cu:31-126)
+  text front end (-f, -q)        -> Collection.from_text(path, q)  (README.md:12-15)
+  prepare_data(sets, t)          -> Collection.upload()            (src/fgssjoin/data.cu:133-199)
+  inverted_index + process_blocks-> Collection.selfjoin(t)         (src/fgssjoin/index.cu:169-211,
+                                                                    src/fgssjoin/block.cu:61-213)
+
+Errors raise FgssError carrying the C status code; nothing here falls back to
+a CPU implementation: every join runs the HIP kernels of libfgssjoin.so, and
+importing the package fails when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfgssjoin.so")
+CLI_PATH = os.path.join(_HERE, "fgssjoin")
+
+FGSS_OK, FGSS_ERR_ARG, FGSS_ERR_IO, FGSS_ERR_FORMAT, FGSS_ERR_DEVICE, FGSS_ERR_NOMEM, FGSS_ERR_UNSUPPORTED = range(7)
+GEN_TITLES, GEN_LONG = 0, 1
+
+# every symbol declared in include/fgssjoin.h
+EXPORTS = (
+    "fgss_last_error", "fgss_version",
+    "fgss_collection_from_bin", "fgss_collection_from_bin_buffer",
+    "fgss_collection_from_text", "fgss_collection_from_text_buffer",
+    "fgss_collection_write_bin", "fgss_collection_get_info", "fgss_collection_arrays",
+    "fgss_collection_upload", "fgss_collection_free",
+    "fgss_selfjoin", "fgss_pairs_free", "fgss_pairs_print",
+    "fgss_generate_text", "fgss_free",
+)
+
+
+class FgssError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+
+
+class CollectionInfo(C.Structure):
+    _fields_ = [("n_sets", C.c_int64), ("n_tokens", C.c_int64), ("n_terms", C.c_int32),
+                ("max_len", C.c_int32), ("average_size", C.c_double), ("sorted_desc", C.c_int32),
+                ("on_device", C.c_int32)]
+
+
+class JoinStats(C.Structure):
+    _fields_ = [("index_entries", C.c_int64), ("probe_tokens", C.c_int64), ("postings_scanned", C.c_int64),
+                ("candidates", C.c_int64), ("intersections", C.c_int64), ("results", C.c_int64),
+                ("verify_tokens", C.c_int64), ("ms_index", C.c_double), ("ms_join", C.c_double),
+                ("ms_copy", C.c_double), ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class JoinOpts(C.Structure):
+    _fields_ = [("query_chunk", C.c_int32), ("table_slots", C.c_int32), ("verify_only", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class _Pairs(C.Structure):
+    _fields_ = [("n_pairs", C.c_int64), ("q", C.POINTER(C.c_int32)), ("s", C.POINTER(C.c_int32)),
+                ("rid", C.POINTER(C.c_int32)), ("sid", C.POINTER(C.c_int32)),
+                ("overlap", C.POINTER(C.c_int32)), ("similarity", C.POINTER(C.c_float)),
+                ("stats", JoinStats)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libfgssjoin.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m new_fgssjoin_amd.build` "
+                          "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp, i32p, i64p = C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+    L.fgss_last_error.restype = C.c_char_p
+    L.fgss_version.restype = C.c_char_p
+    for name, args in {
+        "fgss_collection_from_bin": [C.c_char_p, C.POINTER(vp)],
+        "fgss_collection_from_bin_buffer": [i32p, C.c_int64, C.POINTER(vp)],
+        "fgss_collection_from_text": [C.c_char_p, C.c_int, C.POINTER(vp)],
+        "fgss_collection_from_text_buffer": [C.c_char_p, C.c_int64, C.c_int, C.POINTER(vp)],
+        "fgss_collection_write_bin": [vp, C.c_char_p, C.c_char_p],
+        "fgss_collection_get_info": [vp, C.POINTER(CollectionInfo)],
+        "fgss_collection_arrays": [vp, C.POINTER(i32p), C.POINTER(i32p), C.POINTER(i64p), C.POINTER(i32p)],
+        "fgss_collection_upload": [vp],
+        "fgss_selfjoin": [vp, C.c_float, C.POINTER(JoinOpts), C.POINTER(C.POINTER(_Pairs))],
+        "fgss_pairs_print": [C.POINTER(_Pairs), vp],
+        "fgss_generate_text": [C.c_int, C.c_int64, C.c_uint64, C.POINTER(C.c_void_p), i64p],
+    }.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.fgss_collection_free.argtypes = [vp]
+    L.fgss_collection_free.restype = None
+    L.fgss_pairs_free.argtypes = [C.POINTER(_Pairs)]
+    L.fgss_pairs_free.restype = None
+    L.fgss_free.argtypes = [vp]
+    L.fgss_free.restype = None
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != FGSS_OK:
+        raise FgssError(rc, lib().fgss_last_error().decode(errors="replace"))
+
+
+def version() -> str:
+    return lib().fgss_version().decode()
+
+
+def parse_threshold(s: str) -> float:
+    """The CLI's threshold parse: atof to double, stored in a float (src/fgssjoin_cli.cu:31)."""
+    return float(np.float32(float(s)))
+
+
+def generate_text(kind: int, n_records: int, seed: int) -> bytes:
+    """Seeded synthetic records, one per line (GEN_TITLES or GEN_LONG)."""
+    L = lib()
+    p, n = C.c_void_p(), C.c_int64()
+    _check(L.fgss_generate_text(kind, n_records, seed, C.byref(p), C.byref(n)))
+    try:
+        return C.string_at(p, n.value)
+    finally:
+        L.fgss_free(p)
+
+
+@dataclass
+class Pairs:
+    """Result pairs sorted by internal (q, s); rid/sid are the printed record ids."""
+    q: np.ndarray
+    s: np.ndarray
+    rid: np.ndarray
+    sid: np.ndarray
+    overlap: np.ndarray
+    similarity: np.ndarray
+    stats: dict
+
+    def __len__(self):
+        return int(self.q.size)
+
+    def lines(self) -> list[str]:
+        """Reference stdout lines: "%d %d %f" (src/fgssjoin/block.cu:193)."""
+        return ["%d %d %f" % (a, b, float(c)) for a, b, c in zip(self.rid, self.sid, self.similarity)]
+
+
+class Collection:
+    """A set collection in the reference's internal order (file order reversed)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    @classmethod
+    def from_bin(cls, path: str) -> "Collection":
+        h = C.c_void_p()
+        _check(lib().fgss_collection_from_bin(os.fsencode(path), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_bin_ints(cls, ints) -> "Collection":
+        a = np.ascontiguousarray(ints, dtype=np.int32)
+        h = C.c_void_p()
+        _check(lib().fgss_collection_from_bin_buffer(a.ctypes.data_as(C.POINTER(C.c_int32)), a.size, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_text(cls, path: str, q: int) -> "Collection":
+        h = C.c_void_p()
+        _check(lib().fgss_collection_from_text(os.fsencode(path), q, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_text_bytes(cls, text: bytes, q: int) -> "Collection":
+        h = C.c_void_p()
+        _check(lib().fgss_collection_from_text_buffer(text, len(text), q, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def generate(cls, kind: int, n_records: int, seed: int, q: int = 3) -> "Collection":
+        return cls.from_text_bytes(generate_text(kind, n_records, seed), q)
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().fgss_collection_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def info(self) -> CollectionInfo:
+        i = CollectionInfo()
+        _check(lib().fgss_collection_get_info(self._h, C.byref(i)))
+        return i
+
+    def arrays(self):
+        """(id, len, pos, tokens) numpy copies in internal order."""
+        i32p, i64p = C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+        idp, lp, pp, tp = i32p(), i32p(), i64p(), i32p()
+        _check(lib().fgss_collection_arrays(self._h, C.byref(idp), C.byref(lp), C.byref(pp), C.byref(tp)))
+        inf = self.info
+        n, nt = inf.n_sets, inf.n_tokens
+
+        def cp(p, k, dt):
+            return np.ctypeslib.as_array(p, shape=(k,)).copy() if k > 0 else np.zeros(0, dt)
+        return cp(idp, n, np.int32), cp(lp, n, np.int32), cp(pp, n, np.int64), cp(tp, nt, np.int32)
+
+    def bin_ints(self) -> np.ndarray:
+        """The collection as a reference .bin int32 stream (file order)."""
+        ids, lens, pos, tok = self.arrays()
+        parts = []
+        for k in range(ids.size - 1, -1, -1):
+            parts.append(np.array([ids[k], lens[k]], np.int32))
+            parts.append(tok[pos[k]:pos[k] + lens[k]])
+        return np.concatenate(parts) if parts else np.zeros(0, np.int32)
+
+    def write_bin(self, path: str, loc_path: str | None = None):
+        _check(lib().fgss_collection_write_bin(self._h, os.fsencode(path),
+                                               os.fsencode(loc_path) if loc_path else None))
+
+    def upload(self):
+        _check(lib().fgss_collection_upload(self._h))
+
+    def selfjoin(self, threshold: float, table_slots: int = 0) -> Pairs:
+        L = lib()
+        opts = JoinOpts(0, table_slots, 0, 0)
+        out = C.POINTER(_Pairs)()
+        _check(L.fgss_selfjoin(self._h, C.c_float(threshold), C.byref(opts), C.byref(out)))
+        try:
+            p = out.contents
+            n = p.n_pairs
+
+            def cp(ptr, dt):
+                return np.ctypeslib.as_array(ptr, shape=(n,)).copy() if n > 0 else np.zeros(0, dt)
+            return Pairs(cp(p.q, np.int32), cp(p.s, np.int32), cp(p.rid, np.int32), cp(p.sid, np.int32),
+                         cp(p.overlap, np.int32), cp(p.similarity, np.float32), p.stats.as_dict())
+        finally:
+            L.fgss_pairs_free(out)
+
+
+lib()

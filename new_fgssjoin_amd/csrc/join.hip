@@ -1,0 +1,832 @@
+// join.hip -- the fgssjoin self-join on MI355X (gfx950).
+//
+// Pipeline per threshold t (all device work on one HIP stream):
+//   1. k_midlen        mid_prefix(|s|) per record          (src/fgssjoin/data.cu:241-257)
+//   2. exclusive_scan   -> entry offsets                     (data.cu:181, scan.cuh:39-105)
+//   3. k_entries       (term, entry) for every mid-prefix token (index.cu:276-296)
+//   4. radix_sort_pairs stable by term -> postings ascending by record id
+//                        (replaces df_count_kernel + inverted_index_kernel, index.cu:303-342,
+//                         whose list order was arbitrary)
+//   5. k_postings      SoA postings (sid, pos) + per-term list offsets
+//   6. k_lmin          per length l: the smallest partner length passing the size filter
+//   7. k_plan          per query: source window (q, hi] and the number of useful postings
+//   8. k_items         split heavy queries into source sub-windows ("work items")
+//   9. k_join          one wavefront per work item: candidate generation with overlap
+//                      accumulation in an LDS hash table (filtering.cu:76-141), positional
+//                      filter, verification against an LDS bitmap of the query tokens with
+//                      ballot/popcount (checking.cu:93-175), float32 final test
+//                      (block.cu:183-197) and compaction of result pairs.
+//
+// Result semantics (identical to the reference's printed pair set): a pair (q, s),
+// q < s in internal order, is a result iff
+//   size filter holds, m = |Q[0,qp) n S[0,sp)| > 0, the positional filter passes,
+//   ov = |Q n S| >= floor(bound) and (float)ov/(float)(|Q|+|S|-ov) >= t.
+// The positional filter depends only on (m, last matched query position, last matched
+// source position) because both records share one global token order, so it can be
+// accumulated in any order with an atomic add and an atomic max.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <vector>
+
+#include "fgss_bounds.h"
+#include "fgss_internal.h"
+#include "prims.h"
+
+using namespace fgss;
+
+#define HIPCK(x)                                                                                    \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess) return fgss_fail(FGSS_ERR_DEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr int kJoinWaves = 4;              // wavefronts per block in k_join
+constexpr int kTokBatch = 256;             // probe tokens whose posting ranges are staged at once
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+// device counters, one 64-bit word each
+enum Ctr {
+    C_PQ = 0,
+    C_ESCAN,
+    C_CAND,
+    C_INTER,
+    C_VTOK,
+    C_RESULTS,       // pairs emitted (may exceed the output capacity)
+    C_OVERFLOW,      // work items whose LDS table overflowed
+    C_ITEMS_DONE,
+    C_QUEUE,         // work-queue head
+    C_COUNT
+};
+
+struct Item {
+    uint32_t q, a, b, pad;   // query and source sub-window [a, b)
+};
+
+}  // namespace
+
+struct fgss_device_sets {
+    int device = 0;
+    hipStream_t st = nullptr;
+    int64_t n = 0, ntok = 0;
+    int32_t n_terms = 0, max_len = 0;
+    uint32_t *tok = nullptr, *len = nullptr, *pos = nullptr;
+    DevBuf b_midlen, b_midoff, b_ka, b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_psid, b_pspos, b_loff, b_lmin,
+        b_hi, b_nparts, b_partoff, b_scanscr, b_items, b_items2, b_ovf, b_out, b_ctr, b_misc;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint64_t *h_ctr = nullptr;   // pinned host mirror of the counters
+};
+
+static int ensure(DevBuf &b, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    if (b.cap >= bytes) return FGSS_OK;
+    if (b.p) HIPCK(hipFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = bytes + bytes / 8;
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) return fgss_fail(FGSS_ERR_NOMEM, "hipMalloc(%zu): %s", want, hipGetErrorString(e));
+    b.cap = want;
+    return FGSS_OK;
+}
+
+template <class T>
+static T *P(DevBuf &b) { return (T *)b.p; }
+
+#define ENSURE(buf, bytes)                \
+    do {                                  \
+        int rc_ = ensure((buf), (bytes)); \
+        if (rc_) return rc_;              \
+    } while (0)
+
+void fgss_device_release(fgss_collection *c)
+{
+    if (!c || !c->dev) return;
+    fgss_device_sets *d = c->dev;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(d->device);
+    if (d->st) (void)hipStreamSynchronize(d->st);
+    DevBuf *bufs[] = {&d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib, &d->b_sortscr,
+                      &d->b_sid_of, &d->b_psid, &d->b_pspos, &d->b_loff, &d->b_lmin, &d->b_hi, &d->b_nparts,
+                      &d->b_partoff, &d->b_scanscr, &d->b_items, &d->b_items2, &d->b_ovf, &d->b_out, &d->b_ctr,
+                      &d->b_misc};
+    for (DevBuf *b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    if (d->tok) (void)hipFree(d->tok);
+    if (d->len) (void)hipFree(d->len);
+    if (d->pos) (void)hipFree(d->pos);
+    for (auto &e : d->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (d->h_ctr) (void)hipHostFree(d->h_ctr);
+    if (d->st) (void)hipStreamDestroy(d->st);
+    (void)hipSetDevice(cur);
+    delete d;
+    c->dev = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// upload (the H2D part of prepare_data, src/fgssjoin/data.cu:133-153)
+// ---------------------------------------------------------------------------
+static int validate_collection(const fgss_collection *c)
+{
+    const int64_t n = (int64_t)c->len.size();
+    if (n >= (int64_t)INT32_MAX) return fgss_fail(FGSS_ERR_UNSUPPORTED, "more than 2^31-1 records");
+    if ((int64_t)c->tok.size() >= (int64_t)UINT32_MAX)
+        return fgss_fail(FGSS_ERR_UNSUPPORTED, "more than 2^32-1 tokens (the reference's unsigned offsets)");
+    if (c->max_len >= 65536) return fgss_fail(FGSS_ERR_UNSUPPORTED, "records longer than 65535 tokens");
+    if (!c->sorted_desc)
+        return fgss_fail(FGSS_ERR_UNSUPPORTED,
+                         "record lengths must be non-increasing in internal order (a size-ascending .bin)");
+    for (int64_t i = 0; i < n; i++) {
+        const int32_t *r = c->tok.data() + c->pos[i];
+        for (int32_t k = 0; k < c->len[i]; k++) {
+            if (r[k] < 0) return fgss_fail(FGSS_ERR_UNSUPPORTED, "negative token id in record %lld", (long long)i);
+            if (k > 0 && r[k] <= r[k - 1])
+                return fgss_fail(FGSS_ERR_UNSUPPORTED, "record %lld: tokens must be strictly ascending", (long long)i);
+        }
+    }
+    return FGSS_OK;
+}
+
+extern "C" int fgss_collection_upload(fgss_collection *c)
+{
+    if (!c) return fgss_fail(FGSS_ERR_ARG, "null collection");
+    if (c->dev) return FGSS_OK;
+    int rc = validate_collection(c);
+    if (rc) return rc;
+    auto *d = new fgss_device_sets();
+    c->dev = d;
+    HIPCK(hipGetDevice(&d->device));
+    HIPCK(hipStreamCreateWithFlags(&d->st, hipStreamNonBlocking));
+    for (auto &e : d->ev) HIPCK(hipEventCreate(&e));
+    HIPCK(hipHostMalloc((void **)&d->h_ctr, C_COUNT * sizeof(uint64_t), hipHostMallocDefault));
+    d->n = (int64_t)c->len.size();
+    d->ntok = (int64_t)c->tok.size();
+    d->n_terms = c->n_terms;
+    d->max_len = c->max_len;
+    const size_t nb = (size_t)std::max<int64_t>(d->n, 1) * 4, tb = (size_t)std::max<int64_t>(d->ntok, 1) * 4;
+    if (hipMalloc(&d->tok, tb) != hipSuccess || hipMalloc(&d->len, nb) != hipSuccess ||
+        hipMalloc(&d->pos, nb) != hipSuccess) {
+        fgss_device_release(c);
+        return fgss_fail(FGSS_ERR_NOMEM, "device allocation for the collection failed");
+    }
+    std::vector<uint32_t> pos32((size_t)d->n);
+    for (int64_t i = 0; i < d->n; i++) pos32[(size_t)i] = (uint32_t)c->pos[(size_t)i];
+    HIPCK(hipMemcpyAsync(d->tok, c->tok.data(), (size_t)d->ntok * 4, hipMemcpyHostToDevice, d->st));
+    HIPCK(hipMemcpyAsync(d->len, c->len.data(), (size_t)d->n * 4, hipMemcpyHostToDevice, d->st));
+    HIPCK(hipMemcpyAsync(d->pos, pos32.data(), (size_t)d->n * 4, hipMemcpyHostToDevice, d->st));
+    HIPCK(hipStreamSynchronize(d->st));
+    return FGSS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// index build kernels
+// ---------------------------------------------------------------------------
+__global__ void k_midlen(const uint32_t *__restrict__ len, uint32_t *__restrict__ midlen, int64_t n, float t)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        uint32_t l = len[i];
+        uint32_t mp = fgss_mid_prefix(l, t);
+        midlen[i] = mp < l ? mp : l;   // an empty record contributes nothing
+    }
+}
+
+// thread per record: keys[e] = token, sid_of[e] = record, for e in its mid-prefix range
+__global__ void k_entries(const uint32_t *__restrict__ tok, const uint32_t *__restrict__ pos,
+                          const uint32_t *__restrict__ midlen, const uint32_t *__restrict__ midoff, int64_t n,
+                          uint32_t *__restrict__ keys, uint32_t *__restrict__ sid_of)
+{
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    uint32_t m = midlen[s], o = midoff[s], p = pos[s];
+    for (uint32_t j = 0; j < m; j++) {
+        keys[o + j] = tok[p + j];
+        sid_of[o + j] = (uint32_t)s;
+    }
+}
+
+// postings in term order: psid[i] = record, pspos[i] = position of the token in it
+__global__ void k_postings(const uint32_t *__restrict__ sidx, const uint32_t *__restrict__ sid_of,
+                           const uint32_t *__restrict__ midoff, uint32_t nE, uint32_t *__restrict__ psid,
+                           uint32_t *__restrict__ pspos)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nE) return;
+    uint32_t e = sidx[i];
+    uint32_t s = sid_of[e];
+    psid[i] = s;
+    pspos[i] = e - midoff[s];
+}
+
+// loff[t] = first posting of term t; loff[n_terms] = nE
+__global__ void k_list_offsets(const uint32_t *__restrict__ skeys, uint32_t nE, uint32_t n_terms,
+                               uint32_t *__restrict__ loff)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nE) return;
+    int64_t cur = i < nE ? (int64_t)skeys[i] : (int64_t)n_terms;
+    int64_t prev = i > 0 ? (int64_t)skeys[i - 1] : -1;
+    for (int64_t tt = prev + 1; tt <= cur; tt++) loff[tt] = i;
+}
+
+// lmin[l] = smallest partner length ls <= l with size_ok(l, ls); the valid partner
+// lengths of l form an interval that contains l.
+__global__ void k_lmin(uint32_t *__restrict__ lmin, int32_t max_len, float t)
+{
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l > max_len) return;
+    uint32_t ls = (uint32_t)l;
+    while (ls > 0 && fgss_size_ok((uint32_t)l, ls - 1, t)) ls--;
+    lmin[l] = ls;
+}
+
+// first index in [lo, hi) with a[index] >= v (a ascending)
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *__restrict__ a, uint32_t lo, uint32_t hi, uint32_t v)
+{
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// wave per query: window hi(q) and number of useful postings E(q) -> work item count
+__global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, const uint32_t *__restrict__ pos,
+                                              const uint32_t *__restrict__ len, int64_t n,
+                                              const uint32_t *__restrict__ lmin, const uint32_t *__restrict__ loff,
+                                              const uint32_t *__restrict__ psid, float t, uint32_t target,
+                                              uint32_t *__restrict__ hi_out, uint32_t *__restrict__ nparts,
+                                              unsigned long long *__restrict__ ctr)
+{
+    const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    if (q >= n) return;
+    const uint32_t lq = len[q];
+    uint32_t qp = fgss_max_prefix(lq, t);
+    if (qp > lq) qp = lq;
+    // window: all s > q with len[s] >= lmin[lq]; lengths are non-increasing
+    const uint32_t need = lmin[lq];
+    uint32_t lo = (uint32_t)q + 1, hi = (uint32_t)n;   // first s with len[s] < need
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (len[mid] >= need) lo = mid + 1;
+        else hi = mid;
+    }
+    const uint32_t wend = lo;   // window is [q+1, wend)
+    uint32_t e = 0;
+    for (uint32_t j = lane_id(); j < qp; j += kWave) {
+        uint32_t term = tok[pos[q] + j];
+        uint32_t L0 = loff[term], L1 = loff[term + 1];
+        uint32_t a = lower_bound_u32(psid, L0, L1, (uint32_t)q + 1);
+        uint32_t b = lower_bound_u32(psid, a, L1, wend);
+        e += b - a;
+    }
+    e = wave_sum(e);
+    if (lane_id() == 0) {
+        hi_out[q] = wend;
+        nparts[q] = e == 0 ? 0u : (e + target - 1) / target;
+        atomicAdd(&ctr[C_PQ], (unsigned long long)qp);
+    }
+}
+
+__global__ void k_items(const uint32_t *__restrict__ hi, const uint32_t *__restrict__ nparts,
+                        const uint32_t *__restrict__ partoff, int64_t n, Item *__restrict__ items)
+{
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    uint32_t np = nparts[q];
+    if (np == 0) return;
+    uint64_t a0 = (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
+    uint32_t o = partoff[q];
+    for (uint32_t p = 0; p < np; p++) {
+        Item it;
+        it.q = (uint32_t)q;
+        it.a = (uint32_t)(a0 + W * p / np);
+        it.b = (uint32_t)(a0 + W * (p + 1) / np);
+        it.pad = 0;
+        items[o + p] = it;
+    }
+}
+
+// split every overflowed item into `ways` equal source sub-windows
+__global__ void k_resplit(const Item *__restrict__ in, const uint32_t *__restrict__ ovf, uint32_t novf, uint32_t ways,
+                          Item *__restrict__ out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= novf) return;
+    Item it = in[ovf[i]];
+    uint64_t W = (uint64_t)it.b - it.a;
+    for (uint32_t p = 0; p < ways; p++) {
+        Item o;
+        o.q = it.q;
+        o.a = (uint32_t)(it.a + W * p / ways);
+        o.b = (uint32_t)(it.a + W * (p + 1) / ways);
+        o.pad = 0;
+        out[i * ways + p] = o;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the join kernel: one wavefront per work item, items taken from a queue
+// ---------------------------------------------------------------------------
+struct JoinArgs {
+    const uint32_t *tok, *pos, *len;
+    const uint32_t *loff, *psid, *pspos;
+    const Item *items;
+    uint32_t n_items;
+    float t;
+    uint32_t bm_words;          // query bitmap words (n_terms / 32 rounded up)
+    int32_t *out;               // (q, s, overlap) triples
+    uint32_t out_cap;
+    uint32_t *ovf;              // indices of overflowed items
+    unsigned long long *ctr;
+};
+
+__device__ __forceinline__ uint32_t slot_hash(uint32_t s, int log2cap)
+{
+    return (s * 0x9E3779B1u) >> (32 - log2cap);
+}
+
+// Order LDS traffic between lanes of one wavefront: a compiler barrier plus a
+// wavefront-scope fence (the hardware executes a wave's LDS operations in order).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int CAP>
+__global__ __launch_bounds__(kJoinWaves * kWave) void k_join(JoinArgs A)
+{
+    constexpr int LOG2CAP = CAP == 512 ? 9 : CAP == 1024 ? 10 : CAP == 2048 ? 11 : 12;
+    static_assert((1 << LOG2CAP) == CAP, "CAP must be a power of two");
+    extern __shared__ uint32_t smem[];
+    const int w = (int)threadIdx.x / kWave;
+    const int lane = lane_id();
+    const uint32_t per_wave = 3 * CAP + 2 * (kTokBatch + 1) + A.bm_words;
+    uint32_t *tkey = smem + (size_t)w * per_wave;
+    uint32_t *tcnt = tkey + CAP;
+    uint32_t *tlast = tcnt + CAP;
+    uint32_t *rlo = tlast + CAP;
+    uint32_t *rpre = rlo + kTokBatch + 1;
+    uint32_t *bm = rpre + kTokBatch + 1;
+    const float t = A.t;
+
+    for (uint32_t i = lane; i < A.bm_words; i += kWave) bm[i] = 0;
+    wave_sync();
+
+    unsigned long long s_escan = 0, s_cand = 0, s_inter = 0, s_vtok = 0, s_done = 0;
+    for (;;) {
+        uint32_t it = 0;
+        if (lane == 0) it = atomicAdd((unsigned int *)&A.ctr[C_QUEUE], 1u);
+        it = __shfl(it, 0, kWave);
+        if (it >= A.n_items) break;
+        const Item item = A.items[it];
+        const uint32_t q = item.q, wa = item.a, wb = item.b;
+        const uint32_t lq = A.len[q];
+        const uint32_t qpos = A.pos[q];
+        uint32_t qp = fgss_max_prefix(lq, t);
+        if (qp > lq) qp = lq;
+
+        for (uint32_t i = lane; i < CAP; i += kWave) { tkey[i] = kEmpty; tcnt[i] = 0; tlast[i] = 0; }
+        for (uint32_t i = lane; i < lq; i += kWave) {
+            uint32_t tk = A.tok[qpos + i];
+            atomicOr(&bm[tk >> 5], 1u << (tk & 31));
+        }
+        wave_sync();
+
+        // ---- candidate generation (filtering.cu:94-139) ----
+        bool overflow = false;
+        uint32_t escan = 0;
+        for (uint32_t jb = 0; jb < qp; jb += kTokBatch) {
+            const uint32_t nb = min((uint32_t)kTokBatch, qp - jb);
+            for (uint32_t jj = lane; jj < nb; jj += kWave) {
+                uint32_t term = A.tok[qpos + jb + jj];
+                uint32_t L0 = A.loff[term], L1 = A.loff[term + 1];
+                uint32_t lo = lower_bound_u32(A.psid, L0, L1, wa);
+                uint32_t hi = lower_bound_u32(A.psid, lo, L1, wb);
+                rlo[jj] = lo;
+                rpre[jj] = hi - lo;
+            }
+            wave_sync();
+            // exclusive scan of the segment lengths, kTokBatch/64 entries per lane
+            constexpr int PER = kTokBatch / kWave;
+            uint32_t v[PER], sum = 0;
+#pragma unroll
+            for (int k = 0; k < PER; k++) {
+                uint32_t jj = (uint32_t)lane * PER + k;
+                v[k] = jj < nb ? rpre[jj] : 0u;
+                sum += v[k];
+            }
+            uint32_t inc = wave_inclusive_scan(sum);
+            uint32_t run = inc - sum;
+#pragma unroll
+            for (int k = 0; k < PER; k++) {
+                uint32_t jj = (uint32_t)lane * PER + k;
+                if (jj < nb) rpre[jj] = run;
+                run += v[k];
+            }
+            const uint32_t total = __shfl(inc, kWave - 1, kWave);
+            if (lane == 0) rpre[nb] = total;
+            wave_sync();
+            escan += total;
+            for (uint32_t e0 = 0; e0 < total; e0 += kWave) {
+                const uint32_t e = e0 + lane;
+                bool ok = true;
+                if (e < total) {
+                    uint32_t lo = 0, hi = nb;   // largest jj with rpre[jj] <= e
+                    while (hi - lo > 1) {
+                        uint32_t mid = (lo + hi) >> 1;
+                        if (rpre[mid] <= e) lo = mid;
+                        else hi = mid;
+                    }
+                    const uint32_t idx = rlo[lo] + (e - rpre[lo]);
+                    const uint32_t s = A.psid[idx];
+                    const uint32_t sp = A.pspos[idx];
+                    uint32_t h = slot_hash(s, LOG2CAP);
+                    int probes = 0;
+                    for (;;) {
+                        uint32_t prev = atomicCAS(&tkey[h], kEmpty, s);
+                        if (prev == kEmpty || prev == s) break;
+                        h = (h + 1) & (CAP - 1);
+                        if (++probes >= CAP) { ok = false; break; }
+                    }
+                    if (ok) {
+                        atomicAdd(&tcnt[h], 1u);
+                        atomicMax(&tlast[h], ((jb + lo) << 16) | sp);
+                    }
+                }
+                if (__ballot(!ok)) { overflow = true; break; }
+            }
+            wave_sync();
+            if (overflow) break;
+        }
+        if (overflow) {
+            if (lane == 0) {
+                uint32_t k = (uint32_t)atomicAdd(&A.ctr[C_OVERFLOW], 1ull);
+                A.ovf[k] = it;
+            }
+            for (uint32_t i = lane; i < lq; i += kWave) bm[A.tok[qpos + i] >> 5] = 0;
+            wave_sync();
+            continue;
+        }
+
+        // ---- positional filter + compaction of the candidates into the table front ----
+        uint32_t ncand = 0, cand = 0;
+        for (uint32_t base = 0; base < CAP; base += kWave) {
+            const uint32_t i = base + lane;
+            const uint32_t s = tkey[i];
+            const uint32_t m = tcnt[i];
+            const uint32_t last = tlast[i];
+            bool verify = false;
+            if (s != kEmpty) {
+                const uint32_t ls = A.len[s];
+                const float bound = fgss_min_overlap(lq, ls, t);
+                const uint32_t js = last >> 16, ss = last & 0xFFFFu;
+                const uint32_t rq = lq - js, rs = ls - ss;
+                const uint32_t minrem = rq < rs ? rq : rs;
+                if (!((float)(m + minrem) < bound)) {          // filtering.cu:127-133 at the last match
+                    cand++;
+                    // overlap can still reach floor(bound): m + min(|Q|-js-1, |S|-ss-1)
+                    verify = (int)(m + minrem - 1) >= (int)floorf(bound);
+                }
+            }
+            const uint64_t bal = __ballot(verify);
+            if (verify) {
+                const uint32_t d = ncand + (uint32_t)__popcll(bal & lanemask_lt());
+                tkey[d] = s;
+                tcnt[d] = m;
+                tlast[d] = last;
+            }
+            ncand += (uint32_t)__popcll(bal);
+        }
+        cand = wave_sum(cand);
+        wave_sync();
+
+        // ---- verification (checking.cu:93-175): overlap of S after the last prefix match ----
+        uint32_t inter = 0, vtok = 0;
+        for (uint32_t c = 0; c < ncand; c++) {
+            const uint32_t s = tkey[c];
+            const uint32_t m = tcnt[c];
+            const uint32_t ss = tlast[c] & 0xFFFFu;
+            const uint32_t ls = A.len[s];
+            const uint32_t spos = A.pos[s];
+            const float bound = fgss_min_overlap(lq, ls, t);
+            const int floorb = (int)floorf(bound);
+            uint32_t ov = m;
+            uint32_t base = ss + 1;
+            for (; base < ls; base += kWave) {
+                const uint32_t i = base + lane;
+                bool hit = false;
+                if (i < ls) {
+                    uint32_t tk = A.tok[spos + i];
+                    hit = (bm[tk >> 5] >> (tk & 31)) & 1u;
+                }
+                ov += (uint32_t)__popcll(__ballot(hit));
+                const int rem = (int)ls - (int)(base + kWave);
+                if ((int)ov + (rem > 0 ? rem : 0) < floorb) { base += kWave; break; }
+            }
+            vtok += min(base, ls) - (ss + 1);
+            if ((int)ov >= floorb) {
+                inter++;
+                if (fgss_similarity(lq, ls, ov) >= t) {            // block.cu:189-192
+                    if (lane == 0) {
+                        uint32_t r = (uint32_t)atomicAdd(&A.ctr[C_RESULTS], 1ull);
+                        if (r < A.out_cap) {
+                            A.out[3 * (size_t)r] = (int32_t)q;
+                            A.out[3 * (size_t)r + 1] = (int32_t)s;
+                            A.out[3 * (size_t)r + 2] = (int32_t)ov;
+                        }
+                    }
+                }
+            }
+        }
+        for (uint32_t i = lane; i < lq; i += kWave) bm[A.tok[qpos + i] >> 5] = 0;
+        wave_sync();
+        s_escan += escan;
+        s_cand += cand;
+        s_inter += inter;
+        s_vtok += vtok;
+        s_done += 1;
+    }
+    if (lane == 0) {
+        atomicAdd(&A.ctr[C_ESCAN], s_escan);
+        atomicAdd(&A.ctr[C_CAND], s_cand);
+        atomicAdd(&A.ctr[C_INTER], s_inter);
+        atomicAdd(&A.ctr[C_VTOK], s_vtok);
+        atomicAdd(&A.ctr[C_ITEMS_DONE], s_done);
+    }
+}
+
+template <int CAP>
+static size_t join_smem_bytes(uint32_t bm_words)
+{
+    return (size_t)kJoinWaves * (3 * CAP + 2 * (kTokBatch + 1) + bm_words) * sizeof(uint32_t);
+}
+
+template <int CAP>
+static int launch_join(fgss_device_sets *d, JoinArgs A, int *blocks_out)
+{
+    const size_t smem = join_smem_bytes<CAP>(A.bm_words);
+    if (smem > 160 * 1024) return fgss_fail(FGSS_ERR_UNSUPPORTED, "query bitmap does not fit in LDS");
+    HIPCK(hipFuncSetAttribute((const void *)k_join<CAP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    int per_cu = 0;
+    HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_join<CAP>, kJoinWaves * kWave, smem));
+    int cus = 0;
+    HIPCK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device));
+    if (per_cu < 1) per_cu = 1;
+    int blocks = per_cu * cus;
+    const int need = (int)((A.n_items + kJoinWaves - 1) / kJoinWaves);
+    if (blocks > need) blocks = std::max(need, 1);
+    hipLaunchKernelGGL(k_join<CAP>, dim3(blocks), dim3(kJoinWaves * kWave), smem, d->st, A);
+    HIPCK(hipGetLastError());
+    if (blocks_out) *blocks_out = blocks;
+    return FGSS_OK;
+}
+
+static int run_join(fgss_device_sets *d, JoinArgs A, int cap)
+{
+    switch (cap) {
+    case 512: return launch_join<512>(d, A, nullptr);
+    case 1024: return launch_join<1024>(d, A, nullptr);
+    case 2048: return launch_join<2048>(d, A, nullptr);
+    case 4096: return launch_join<4096>(d, A, nullptr);
+    default: return fgss_fail(FGSS_ERR_ARG, "table_slots must be 512, 1024, 2048 or 4096");
+    }
+}
+
+static inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+// ---------------------------------------------------------------------------
+// fgss_selfjoin
+// ---------------------------------------------------------------------------
+static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *opts, std::vector<int32_t> &tri,
+                           fgss_join_stats &stats)
+{
+    fgss_device_sets *d = c->dev;
+    hipStream_t st = d->st;
+    const int64_t n = d->n;
+    int cap = opts && opts->table_slots ? opts->table_slots : 1024;
+    uint32_t target = (uint32_t)(cap / 2);
+    memset(&stats, 0, sizeof(stats));
+    tri.clear();
+    if (n == 0) return FGSS_OK;
+
+    HIPCK(hipEventRecord(d->ev[0], st));
+    ENSURE(d->b_ctr, C_COUNT * sizeof(uint64_t));
+    unsigned long long *ctr = P<unsigned long long>(d->b_ctr);
+    HIPCK(hipMemsetAsync(ctr, 0, C_COUNT * sizeof(uint64_t), st));
+
+    // 1-2: mid-prefix lengths and entry offsets
+    ENSURE(d->b_midlen, (size_t)n * 4);
+    ENSURE(d->b_midoff, (size_t)(n + 1) * 4);
+    ENSURE(d->b_scanscr, (size_t)scan_scratch_elems(std::max<int64_t>(n, 1 << 20)) * 8);
+    ENSURE(d->b_misc, 64);
+    uint32_t *midlen = P<uint32_t>(d->b_midlen), *midoff = P<uint32_t>(d->b_midoff);
+    uint32_t *misc = P<uint32_t>(d->b_misc);
+    hipLaunchKernelGGL(k_midlen, dim3(grid_for(n, 256)), dim3(256), 0, st, d->len, midlen, n, t);
+    HIPCK(hipGetLastError());
+    HIPCK(exclusive_scan<uint32_t>(midlen, midoff, n, misc, P<uint32_t>(d->b_scanscr), st));
+    uint32_t nE = 0;
+    HIPCK(hipMemcpyAsync(&nE, misc, 4, hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    stats.index_entries = nE;
+
+    // 3-5: entries, stable sort by term, postings and list offsets
+    const size_t eb = (size_t)std::max<uint32_t>(nE, 1) * 4;
+    ENSURE(d->b_ka, eb);
+    ENSURE(d->b_kb, eb);
+    ENSURE(d->b_ia, eb);
+    ENSURE(d->b_ib, eb);
+    ENSURE(d->b_sid_of, eb);
+    ENSURE(d->b_psid, eb + 4);
+    ENSURE(d->b_pspos, eb + 4);
+    ENSURE(d->b_loff, ((size_t)d->n_terms + 2) * 4);
+    ENSURE(d->b_sortscr, (size_t)radix_scratch_elems(std::max<uint32_t>(nE, 1)) * 4);
+    uint32_t *ka = P<uint32_t>(d->b_ka), *kb = P<uint32_t>(d->b_kb), *ia = P<uint32_t>(d->b_ia),
+             *ib = P<uint32_t>(d->b_ib), *sid_of = P<uint32_t>(d->b_sid_of), *psid = P<uint32_t>(d->b_psid),
+             *pspos = P<uint32_t>(d->b_pspos), *loff = P<uint32_t>(d->b_loff);
+    hipLaunchKernelGGL(k_entries, dim3(grid_for(n, 256)), dim3(256), 0, st, d->tok, d->pos, midlen, midoff, n, ka,
+                       sid_of);
+    HIPCK(hipGetLastError());
+    int bits = 1;
+    while (bits < 32 && (1ll << bits) < (long long)d->n_terms) bits++;
+    int which = 0;
+    HIPCK(radix_sort_pairs(ka, ia, kb, ib, nE, bits, P<uint32_t>(d->b_sortscr), st, &which));
+    const uint32_t *skeys = which ? kb : ka;
+    const uint32_t *sidx = which ? ib : ia;
+    if (nE > 0) {
+        hipLaunchKernelGGL(k_postings, dim3(grid_for(nE, 256)), dim3(256), 0, st, sidx, sid_of, midoff, nE, psid,
+                           pspos);
+        HIPCK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_list_offsets, dim3(grid_for((int64_t)nE + 1, 256)), dim3(256), 0, st, skeys, nE,
+                       (uint32_t)d->n_terms, loff);
+    HIPCK(hipGetLastError());
+    HIPCK(hipEventRecord(d->ev[1], st));
+
+    // 6-8: plan work items
+    ENSURE(d->b_lmin, ((size_t)d->max_len + 1) * 4);
+    ENSURE(d->b_hi, (size_t)n * 4);
+    ENSURE(d->b_nparts, (size_t)n * 4);
+    ENSURE(d->b_partoff, (size_t)(n + 1) * 4);
+    uint32_t *lmin = P<uint32_t>(d->b_lmin), *hi = P<uint32_t>(d->b_hi), *nparts = P<uint32_t>(d->b_nparts),
+             *partoff = P<uint32_t>(d->b_partoff);
+    hipLaunchKernelGGL(k_lmin, dim3(grid_for(d->max_len + 1, 256)), dim3(256), 0, st, lmin, d->max_len, t);
+    HIPCK(hipGetLastError());
+    hipLaunchKernelGGL(k_plan, dim3(grid_for(n * kWave, 256)), dim3(256), 0, st, d->tok, d->pos, d->len, n, lmin,
+                       loff, psid, t, target, hi, nparts, ctr);
+    HIPCK(hipGetLastError());
+    HIPCK(exclusive_scan<uint32_t>(nparts, partoff, n, misc + 1, P<uint32_t>(d->b_scanscr), st));
+    uint32_t n_items = 0;
+    HIPCK(hipMemcpyAsync(&n_items, misc + 1, 4, hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    ENSURE(d->b_items, (size_t)std::max<uint32_t>(n_items, 1) * sizeof(Item));
+    Item *items = P<Item>(d->b_items);
+    hipLaunchKernelGGL(k_items, dim3(grid_for(n, 256)), dim3(256), 0, st, hi, nparts, partoff, n, items);
+    HIPCK(hipGetLastError());
+
+    // 9: join; overflowed items are split into finer source windows and run again,
+    // and a launch whose result buffer was too small is redone with a larger one
+    ENSURE(d->b_ovf, (size_t)std::max<uint32_t>(n_items, 1) * 4);
+    if (d->b_out.cap < (size_t)65536 * 12) ENSURE(d->b_out, (size_t)65536 * 12);
+    uint64_t acc[C_COUNT] = {0};
+    const Item *cur_items = items;
+    uint32_t cur_n = n_items;
+    for (int round = 0; cur_n > 0; round++) {
+        if (round >= 48) return fgss_fail(FGSS_ERR_DEVICE, "internal: work-item splitting did not converge");
+        for (;;) {
+            const uint32_t out_cap = (uint32_t)std::min<size_t>(d->b_out.cap / 12, UINT32_MAX);
+            JoinArgs A;
+            A.tok = d->tok; A.pos = d->pos; A.len = d->len;
+            A.loff = loff; A.psid = psid; A.pspos = pspos;
+            A.items = cur_items; A.n_items = cur_n; A.t = t;
+            A.bm_words = (uint32_t)(((int64_t)d->n_terms + 31) / 32 + 1);
+            A.out = P<int32_t>(d->b_out); A.out_cap = out_cap; A.ovf = P<uint32_t>(d->b_ovf); A.ctr = ctr;
+            HIPCK(hipMemsetAsync(ctr + C_ESCAN, 0, (C_COUNT - C_ESCAN) * sizeof(uint64_t), st));
+            int rc = run_join(d, A, cap);
+            if (rc) return rc;
+            HIPCK(hipMemcpyAsync(d->h_ctr, ctr, C_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+            HIPCK(hipStreamSynchronize(st));
+            if (d->h_ctr[C_RESULTS] <= out_cap) break;
+            ENSURE(d->b_out, (size_t)(d->h_ctr[C_RESULTS] + d->h_ctr[C_RESULTS] / 2) * 12);
+        }
+        for (int k = C_ESCAN; k <= C_VTOK; k++) acc[k] += d->h_ctr[k];
+        const uint64_t nres = d->h_ctr[C_RESULTS], novf = d->h_ctr[C_OVERFLOW];
+        const size_t old = tri.size();
+        tri.resize(old + 3 * (size_t)nres);
+        if (nres) HIPCK(hipMemcpyAsync(tri.data() + old, d->b_out.p, (size_t)nres * 12, hipMemcpyDeviceToHost, st));
+        if (novf == 0) break;
+        const uint32_t ways = 4;
+        ENSURE(d->b_items2, (size_t)novf * ways * sizeof(Item));
+        hipLaunchKernelGGL(k_resplit, dim3(grid_for((int64_t)novf, 256)), dim3(256), 0, st, cur_items,
+                           P<uint32_t>(d->b_ovf), (uint32_t)novf, ways, P<Item>(d->b_items2));
+        HIPCK(hipGetLastError());
+        std::swap(d->b_items, d->b_items2);   // the old list stays allocated in b_items2
+        cur_items = P<Item>(d->b_items);
+        cur_n = (uint32_t)(novf * ways);
+        ENSURE(d->b_ovf, (size_t)cur_n * 4);
+    }
+    HIPCK(hipEventRecord(d->ev[3], st));
+    HIPCK(hipEventSynchronize(d->ev[3]));
+    stats.probe_tokens = (int64_t)d->h_ctr[C_PQ];
+    stats.postings_scanned = (int64_t)acc[C_ESCAN];
+    stats.candidates = (int64_t)acc[C_CAND];
+    stats.intersections = (int64_t)acc[C_INTER];
+    stats.verify_tokens = (int64_t)acc[C_VTOK];
+    stats.results = (int64_t)(tri.size() / 3);
+    float ms = 0;
+    HIPCK(hipEventElapsedTime(&ms, d->ev[0], d->ev[1]));
+    stats.ms_index = ms;
+    HIPCK(hipEventElapsedTime(&ms, d->ev[1], d->ev[3]));
+    stats.ms_join = ms;
+    return FGSS_OK;
+}
+
+extern "C" int fgss_selfjoin(fgss_collection *c, float threshold, const fgss_join_opts *opts, fgss_pairs **out)
+{
+    if (!c || !out) return fgss_fail(FGSS_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (!(threshold > 0.0f) || !(threshold < 1.0f))   // fgssjoin_cli.cu:32-35
+        return fgss_fail(FGSS_ERR_ARG, "Threshold must be a float in the range (0.0..1.0).");
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = fgss_collection_upload(c);
+    if (rc) return rc;
+    std::vector<int32_t> tri;
+    fgss_join_stats stats;
+    auto t1 = std::chrono::steady_clock::now();
+    rc = selfjoin_device(c, threshold, opts, tri, stats);
+    if (rc) return rc;
+    auto t2 = std::chrono::steady_clock::now();
+    const int64_t R = (int64_t)(tri.size() / 3);
+    // sort by (q, s); the device emits in completion order
+    std::vector<int64_t> order((size_t)R);
+    for (int64_t i = 0; i < R; i++) order[(size_t)i] = i;
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        if (tri[3 * a] != tri[3 * b]) return tri[3 * a] < tri[3 * b];
+        return tri[3 * a + 1] < tri[3 * b + 1];
+    });
+    auto *p = (fgss_pairs *)calloc(1, sizeof(fgss_pairs));
+    const size_t nb = (size_t)std::max<int64_t>(R, 1);
+    p->q = (int32_t *)malloc(nb * 4);
+    p->s = (int32_t *)malloc(nb * 4);
+    p->rid = (int32_t *)malloc(nb * 4);
+    p->sid = (int32_t *)malloc(nb * 4);
+    p->overlap = (int32_t *)malloc(nb * 4);
+    p->similarity = (float *)malloc(nb * 4);
+    if (!p->q || !p->s || !p->rid || !p->sid || !p->overlap || !p->similarity) {
+        fgss_pairs_free(p);
+        return fgss_fail(FGSS_ERR_NOMEM, "out of host memory");
+    }
+    for (int64_t k = 0; k < R; k++) {
+        int64_t i = order[(size_t)k];
+        int32_t q = tri[3 * i], s = tri[3 * i + 1], ov = tri[3 * i + 2];
+        p->q[k] = q;
+        p->s[k] = s;
+        p->rid[k] = c->id[(size_t)q];
+        p->sid[k] = c->id[(size_t)s];
+        p->overlap[k] = ov;
+        p->similarity[k] = fgss_similarity((uint32_t)c->len[(size_t)q], (uint32_t)c->len[(size_t)s], (uint32_t)ov);
+    }
+    p->n_pairs = R;
+    auto t3 = std::chrono::steady_clock::now();
+    stats.ms_copy = std::chrono::duration<double, std::milli>(t3 - t2).count();
+    stats.ms_total = std::chrono::duration<double, std::milli>(t3 - t1).count();
+    (void)t0;
+    p->stats = stats;
+    *out = p;
+    return FGSS_OK;
+}
+
+extern "C" void fgss_pairs_free(fgss_pairs *p)
+{
+    if (!p) return;
+    free(p->q);
+    free(p->s);
+    free(p->rid);
+    free(p->sid);
+    free(p->overlap);
+    free(p->similarity);
+    free(p);
+}
+
+extern "C" int fgss_pairs_print(const fgss_pairs *p, void *file)
+{
+    if (!p || !file) return fgss_fail(FGSS_ERR_ARG, "null argument");
+    FILE *f = (FILE *)file;
+    for (int64_t k = 0; k < p->n_pairs; k++)
+        fprintf(f, "%d %d %f\n", p->rid[k], p->sid[k], (double)p->similarity[k]);   // block.cu:193
+    return FGSS_OK;
+}

@@ -1,0 +1,46 @@
+"""Shared helpers for the parity tests (product vs oracle on identical inputs)."""
+from __future__ import annotations
+
+import gzip
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def golden_path(name: str) -> str:
+    return os.path.join(GOLDEN, name)
+
+
+def dblp_text() -> bytes:
+    with gzip.open(golden_path("dblp_t_18k.txt.gz"), "rb") as f:
+        return f.read()
+
+
+def oracle_sets(oracle, coll):
+    """Oracle view of a product collection (same internal-order arrays)."""
+    ids, lens, pos, tok = coll.arrays()
+    return oracle.Sets.from_arrays(ids, lens, pos, tok, coll.info.n_terms)
+
+
+def assert_join_equal(oracle, sets, pairs, t, counters=None):
+    """GPU pairs == oracle triples, bit for bit, with the printed fields and stats."""
+    tri, ctr = oracle.selfjoin(sets, t) if counters is None else counters
+    got = np.stack([pairs.q, pairs.s, pairs.overlap], axis=1) if len(pairs) else np.zeros((0, 3), np.int32)
+    assert got.shape == tri.shape, f"t={t}: {got.shape[0]} pairs vs oracle {tri.shape[0]}"
+    assert np.array_equal(got, tri), f"t={t}: pair sets differ"
+    if len(pairs):
+        assert np.array_equal(pairs.rid, sets.id[tri[:, 0]])
+        assert np.array_equal(pairs.sid, sets.id[tri[:, 1]])
+        sims = np.array([oracle.similarity(int(sets.len[q]), int(sets.len[s]), int(o)) for q, s, o in tri],
+                        np.float32)
+        assert np.array_equal(pairs.similarity.view(np.uint32), sims.view(np.uint32))
+    st = pairs.stats
+    assert st["index_entries"] == ctr["e_idx"]
+    assert st["probe_tokens"] == ctr["p_q"]
+    assert st["postings_scanned"] == ctr["e_scan"]
+    assert st["candidates"] == ctr["candidates"]
+    assert st["intersections"] == ctr["intersections"]
+    assert st["results"] == ctr["results"]
+    return tri, ctr
