@@ -87,6 +87,10 @@ int fgss_collection_upload(fgss_collection *c);
 
 void fgss_collection_free(fgss_collection *c);
 
+/* Block until all device work issued by this process has finished (the
+ * reference's cudaDeviceSynchronize after each launch, src/fgssjoin/filtering.cu:38). */
+int fgss_device_synchronize(void);
+
 /* ------------------------------------------------------------------ */
 /* Self-join (replaces inverted_index + process_blocks)               */
 /* ------------------------------------------------------------------ */
@@ -102,6 +106,8 @@ typedef struct {
     double ms_join;            /* candidate generation + verification + compaction    */
     double ms_copy;            /* device -> host copy of the result pairs             */
     double ms_total;           /* whole call                                          */
+    double ms_kernel;          /* device time of the join kernel launches (HIP events) */
+    int64_t kernel_launches;   /* join kernel launches (1 unless work items were re-split) */
 } fgss_join_stats;
 
 typedef struct {

@@ -35,7 +35,7 @@ EXPORTS = (
     "fgss_collection_from_bin", "fgss_collection_from_bin_buffer",
     "fgss_collection_from_text", "fgss_collection_from_text_buffer",
     "fgss_collection_write_bin", "fgss_collection_get_info", "fgss_collection_arrays",
-    "fgss_collection_upload", "fgss_collection_free",
+    "fgss_collection_upload", "fgss_collection_free", "fgss_device_synchronize",
     "fgss_selfjoin", "fgss_pairs_free", "fgss_pairs_print",
     "fgss_generate_text", "fgss_free",
 )
@@ -57,7 +57,8 @@ class JoinStats(C.Structure):
     _fields_ = [("index_entries", C.c_int64), ("probe_tokens", C.c_int64), ("postings_scanned", C.c_int64),
                 ("candidates", C.c_int64), ("intersections", C.c_int64), ("results", C.c_int64),
                 ("verify_tokens", C.c_int64), ("ms_index", C.c_double), ("ms_join", C.c_double),
-                ("ms_copy", C.c_double), ("ms_total", C.c_double)]
+                ("ms_copy", C.c_double), ("ms_total", C.c_double),
+                ("ms_kernel", C.c_double), ("kernel_launches", C.c_int64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -99,6 +100,7 @@ def lib():
         "fgss_collection_get_info": [vp, C.POINTER(CollectionInfo)],
         "fgss_collection_arrays": [vp, C.POINTER(i32p), C.POINTER(i32p), C.POINTER(i64p), C.POINTER(i32p)],
         "fgss_collection_upload": [vp],
+        "fgss_device_synchronize": [],
         "fgss_selfjoin": [vp, C.c_float, C.POINTER(JoinOpts), C.POINTER(C.POINTER(_Pairs))],
         "fgss_pairs_print": [C.POINTER(_Pairs), vp],
         "fgss_generate_text": [C.c_int, C.c_int64, C.c_uint64, C.POINTER(C.c_void_p), i64p],
@@ -123,6 +125,11 @@ def _check(rc: int):
 
 def version() -> str:
     return lib().fgss_version().decode()
+
+
+def device_synchronize() -> None:
+    """Wait for all device work of this process."""
+    _check(lib().fgss_device_synchronize())
 
 
 def parse_threshold(s: str) -> float:

@@ -441,9 +441,17 @@ void edit(std::string &s, int n_edits, const Vocab &v, Rng &r)
     for (int e = 0; e < n_edits; e++) {
         int op = (int)r.below(3);
         if (s.empty()) op = 1;
-        if (op == 0) s[(size_t)r.below(s.size())] = v.letter(r);                          // substitute
-        else if (op == 1) s.insert(s.begin() + (long)r.below(s.size() + 1), v.letter(r));  // insert
-        else s.erase(s.begin() + (long)r.below(s.size()));                                 // delete
+        // draw the position before the letter: one fixed order on every compiler
+        if (op == 0) {                                                   // substitute
+            size_t at = (size_t)r.below(s.size());
+            s[at] = v.letter(r);
+        } else if (op == 1) {                                            // insert
+            size_t at = (size_t)r.below(s.size() + 1);
+            char ch = v.letter(r);
+            s.insert(s.begin() + (long)at, ch);
+        } else {                                                         // delete
+            s.erase(s.begin() + (long)r.below(s.size()));
+        }
     }
 }
 }  // namespace

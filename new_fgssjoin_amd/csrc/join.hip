@@ -46,7 +46,8 @@ using namespace fgss;
 namespace {
 
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
-constexpr int kJoinWaves = 4;              // wavefronts per block in k_join
+// wavefronts per block in k_join: four, or two for the largest LDS tables
+constexpr int join_waves(int cap) { return cap >= 4096 ? 2 : 4; }
 constexpr int kTokBatch = 256;             // probe tokens whose posting ranges are staged at once
 
 struct DevBuf {
@@ -82,7 +83,7 @@ struct fgss_device_sets {
     uint32_t *tok = nullptr, *len = nullptr, *pos = nullptr;
     DevBuf b_midlen, b_midoff, b_ka, b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_psid, b_pspos, b_loff, b_lmin,
         b_hi, b_nparts, b_partoff, b_scanscr, b_items, b_items2, b_ovf, b_out, b_ctr, b_misc;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     uint64_t *h_ctr = nullptr;   // pinned host mirror of the counters
 };
 
@@ -368,7 +369,7 @@ __device__ __forceinline__ void wave_sync()
 }
 
 template <int CAP>
-__global__ __launch_bounds__(kJoinWaves * kWave) void k_join(JoinArgs A)
+__global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
 {
     constexpr int LOG2CAP = CAP == 512 ? 9 : CAP == 1024 ? 10 : CAP == 2048 ? 11 : 12;
     static_assert((1 << LOG2CAP) == CAP, "CAP must be a power of two");
@@ -573,7 +574,7 @@ __global__ __launch_bounds__(kJoinWaves * kWave) void k_join(JoinArgs A)
 template <int CAP>
 static size_t join_smem_bytes(uint32_t bm_words)
 {
-    return (size_t)kJoinWaves * (3 * CAP + 2 * (kTokBatch + 1) + bm_words) * sizeof(uint32_t);
+    return (size_t)join_waves(CAP) * (3 * CAP + 2 * (kTokBatch + 1) + bm_words) * sizeof(uint32_t);
 }
 
 template <int CAP>
@@ -583,14 +584,14 @@ static int launch_join(fgss_device_sets *d, JoinArgs A, int *blocks_out)
     if (smem > 160 * 1024) return fgss_fail(FGSS_ERR_UNSUPPORTED, "query bitmap does not fit in LDS");
     HIPCK(hipFuncSetAttribute((const void *)k_join<CAP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     int per_cu = 0;
-    HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_join<CAP>, kJoinWaves * kWave, smem));
+    HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_join<CAP>, join_waves(CAP) * kWave, smem));
     int cus = 0;
     HIPCK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device));
     if (per_cu < 1) per_cu = 1;
     int blocks = per_cu * cus;
-    const int need = (int)((A.n_items + kJoinWaves - 1) / kJoinWaves);
+    const int need = (int)((A.n_items + join_waves(CAP) - 1) / join_waves(CAP));
     if (blocks > need) blocks = std::max(need, 1);
-    hipLaunchKernelGGL(k_join<CAP>, dim3(blocks), dim3(kJoinWaves * kWave), smem, d->st, A);
+    hipLaunchKernelGGL(k_join<CAP>, dim3(blocks), dim3(join_waves(CAP) * kWave), smem, d->st, A);
     HIPCK(hipGetLastError());
     if (blocks_out) *blocks_out = blocks;
     return FGSS_OK;
@@ -691,7 +692,9 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     HIPCK(hipGetLastError());
     HIPCK(exclusive_scan<uint32_t>(nparts, partoff, n, misc + 1, P<uint32_t>(d->b_scanscr), st));
     uint32_t n_items = 0;
+    unsigned long long probe_tokens = 0;
     HIPCK(hipMemcpyAsync(&n_items, misc + 1, 4, hipMemcpyDeviceToHost, st));
+    HIPCK(hipMemcpyAsync(&probe_tokens, ctr + C_PQ, 8, hipMemcpyDeviceToHost, st));
     HIPCK(hipStreamSynchronize(st));
     ENSURE(d->b_items, (size_t)std::max<uint32_t>(n_items, 1) * sizeof(Item));
     Item *items = P<Item>(d->b_items);
@@ -703,6 +706,8 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     ENSURE(d->b_ovf, (size_t)std::max<uint32_t>(n_items, 1) * 4);
     if (d->b_out.cap < (size_t)65536 * 12) ENSURE(d->b_out, (size_t)65536 * 12);
     uint64_t acc[C_COUNT] = {0};
+    double kernel_ms = 0;
+    int kernel_launches = 0;
     const Item *cur_items = items;
     uint32_t cur_n = n_items;
     for (int round = 0; cur_n > 0; round++) {
@@ -716,10 +721,16 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
             A.bm_words = (uint32_t)(((int64_t)d->n_terms + 31) / 32 + 1);
             A.out = P<int32_t>(d->b_out); A.out_cap = out_cap; A.ovf = P<uint32_t>(d->b_ovf); A.ctr = ctr;
             HIPCK(hipMemsetAsync(ctr + C_ESCAN, 0, (C_COUNT - C_ESCAN) * sizeof(uint64_t), st));
+            HIPCK(hipEventRecord(d->ev[4], st));
             int rc = run_join(d, A, cap);
             if (rc) return rc;
+            HIPCK(hipEventRecord(d->ev[5], st));
             HIPCK(hipMemcpyAsync(d->h_ctr, ctr, C_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
             HIPCK(hipStreamSynchronize(st));
+            float kms = 0;
+            HIPCK(hipEventElapsedTime(&kms, d->ev[4], d->ev[5]));
+            kernel_ms += kms;
+            kernel_launches++;
             if (d->h_ctr[C_RESULTS] <= out_cap) break;
             ENSURE(d->b_out, (size_t)(d->h_ctr[C_RESULTS] + d->h_ctr[C_RESULTS] / 2) * 12);
         }
@@ -741,7 +752,7 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     }
     HIPCK(hipEventRecord(d->ev[3], st));
     HIPCK(hipEventSynchronize(d->ev[3]));
-    stats.probe_tokens = (int64_t)d->h_ctr[C_PQ];
+    stats.probe_tokens = (int64_t)probe_tokens;
     stats.postings_scanned = (int64_t)acc[C_ESCAN];
     stats.candidates = (int64_t)acc[C_CAND];
     stats.intersections = (int64_t)acc[C_INTER];
@@ -752,6 +763,8 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     stats.ms_index = ms;
     HIPCK(hipEventElapsedTime(&ms, d->ev[1], d->ev[3]));
     stats.ms_join = ms;
+    stats.ms_kernel = kernel_ms;
+    stats.kernel_launches = kernel_launches;
     return FGSS_OK;
 }
 
@@ -828,5 +841,13 @@ extern "C" int fgss_pairs_print(const fgss_pairs *p, void *file)
     FILE *f = (FILE *)file;
     for (int64_t k = 0; k < p->n_pairs; k++)
         fprintf(f, "%d %d %f\n", p->rid[k], p->sid[k], (double)p->similarity[k]);   // block.cu:193
+    return FGSS_OK;
+}
+
+// Wait for all device work of this process (the reference synchronizes the whole
+// device after every launch, e.g. src/fgssjoin/filtering.cu:38).
+extern "C" int fgss_device_synchronize(void)
+{
+    HIPCK(hipDeviceSynchronize());
     return FGSS_OK;
 }

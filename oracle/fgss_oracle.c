@@ -290,6 +290,12 @@ int orc_bin_load(const int32_t *bin, int64_t nints, int32_t *id, int32_t *len,
 /* Join helpers                                                              */
 /* ------------------------------------------------------------------------ */
 
+/* Prefix lengths are clamped to the record length.  For every non-empty record
+ * max_prefix and mid_prefix are already <= |x|; an empty record would make the
+ * reference read one token past its end (max_prefix(0) = 1, similarity.cuh:44-47,
+ * checking.cu:126), so here it simply has no prefix and never joins. */
+static uint32_t prefix_clamp(uint32_t p, uint32_t len) { return p < len ? p : len; }
+
 /* Inverted index over mid-prefix tokens (index.cu:169-211 with the entry
  * kernel :276-296).  Lists hold (set, pos) grouped by term, sets ascending. */
 typedef struct { int64_t *off; int32_t *sid; int32_t *spos; } orc_index;
@@ -301,7 +307,7 @@ static void build_index(int64_t n, const int32_t *len, const int64_t *pos,
     ix->off = (int64_t *)calloc((size_t)n_terms + 1, sizeof(int64_t));
     int64_t E = 0;
     for (int64_t s = 0; s < n; s++) {
-        uint32_t mp = orc_mid_prefix((uint32_t)len[s], t);   /* data.cu:241-257 */
+        uint32_t mp = prefix_clamp(orc_mid_prefix((uint32_t)len[s], t), (uint32_t)len[s]);   /* data.cu:241-257 */
         for (uint32_t j = 0; j < mp; j++) ix->off[tok[pos[s] + j] + 1]++;
         E += mp;
     }
@@ -311,7 +317,7 @@ static void build_index(int64_t n, const int32_t *len, const int64_t *pos,
     int64_t *fill = (int64_t *)malloc((size_t)(n_terms > 0 ? n_terms : 1) * sizeof(int64_t));
     memcpy(fill, ix->off, (size_t)n_terms * sizeof(int64_t));
     for (int64_t s = 0; s < n; s++) {
-        uint32_t mp = orc_mid_prefix((uint32_t)len[s], t);
+        uint32_t mp = prefix_clamp(orc_mid_prefix((uint32_t)len[s], t), (uint32_t)len[s]);
         for (uint32_t j = 0; j < mp; j++) {
             int64_t at = fill[tok[pos[s] + j]]++;
             ix->sid[at] = (int32_t)s; ix->spos[at] = (int32_t)j;
@@ -437,7 +443,7 @@ static void *cf_run(void *arg)
         for (int64_t q = q0; q < q1; q++) {
             const uint32_t lq = (uint32_t)sh->len[q];
             const int32_t *Q = sh->tok + sh->pos[q];
-            const uint32_t qp = orc_max_prefix(lq, t);
+            const uint32_t qp = prefix_clamp(orc_max_prefix(lq, t), lq);
             int64_t nt = 0;
             w->c.p_q += qp;
             /* filtering.cu:94-139: probe-prefix token j, every posting (s, spos) */
@@ -498,7 +504,6 @@ int64_t orc_selfjoin(int64_t n_sets, const int32_t *len, const int64_t *pos,
     orc_index ix;
     orc_counters total;
     memset(&total, 0, sizeof(total));
-    for (int64_t s = 0; s < n_sets; s++) if (len[s] < 1) return -1;   /* D3: empty sets unsupported */
     build_index(n_sets, len, pos, tok, n_terms, t, &ix, &total.e_idx);
     cf_shared sh;
     sh.n = n_sets; sh.len = len; sh.pos = pos; sh.tok = tok; sh.t = t; sh.ix = &ix;
@@ -534,7 +539,6 @@ int64_t orc_selfjoin_literal(int64_t n_sets, const int32_t *len, const int64_t *
 {
     orc_counters c;
     memset(&c, 0, sizeof(c));
-    for (int64_t s = 0; s < n_sets; s++) if (len[s] < 1) return -1;
     if (block_size < 1) return -1;
     if (block_size > n_sets) block_size = n_sets;          /* block.cu:50-51 */
     orc_index ix;
@@ -560,7 +564,7 @@ int64_t orc_selfjoin_literal(int64_t n_sets, const int32_t *len, const int64_t *
                 int64_t query = qq + q_off;
                 uint32_t lq = (uint32_t)len[query];
                 const int32_t *Q = tok + pos[query];
-                uint32_t qp = orc_max_prefix(lq, t);
+                uint32_t qp = prefix_clamp(orc_max_prefix(lq, t), lq);
                 for (uint32_t jj = 0; jj < qp; jj++) {
                     int32_t tk = Q[jj];
                     for (int64_t e = ix.off[tk]; e < ix.off[tk + 1]; e++) {
@@ -604,7 +608,7 @@ int64_t orc_selfjoin_literal(int64_t n_sets, const int32_t *len, const int64_t *
             }
         }
     }
-    for (int64_t q = 0; q < n_sets; q++) c.p_q += orc_max_prefix((uint32_t)len[q], t);
+    for (int64_t q = 0; q < n_sets; q++) c.p_q += prefix_clamp(orc_max_prefix((uint32_t)len[q], t), (uint32_t)len[q]);
     free(scores); free(cand); free_index(&ix);
     qsort(res.v, (size_t)res.n, 3 * sizeof(int32_t), cmp_tri);
     *out = res.v ? res.v : (int32_t *)malloc(sizeof(int32_t));

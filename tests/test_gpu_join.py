@@ -1,0 +1,103 @@
+"""GPU parity: the HIP self-join through the C ABI vs the CPU oracle, bit for bit.
+
+Every case compares the sorted (q, s, overlap) triples, the printed ids and
+float32 similarities, and the reference's counters (# Inverted Index Entries,
+# Candidates, # Intersections, # Results) plus the probe/posting counts.
+"""
+import json
+import sys
+
+import numpy as np
+import pytest
+
+from _util import assert_join_equal, dblp_text, golden_path, oracle_sets
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_fixture_known_answers(fg, oracle_mod):
+    coll = fg.Collection.from_bin(golden_path("small.txt.3gram.bin"))
+    sets = oracle_sets(oracle_mod, coll)
+    ka = json.load(open(golden_path("small_known_answers.json")))
+    for ts, lines in ka["answers"].items():
+        t = fg.parse_threshold(ts)
+        _say(f"fixture t={ts}")
+        p = coll.selfjoin(t)
+        assert sorted(p.lines()) == sorted(lines)
+        assert_join_equal(oracle_mod, sets, p, t)
+
+
+@pytest.mark.parametrize("t", [0.5, 0.7, 0.8, 0.9, 0.95])
+def test_titles_4096(fg, oracle_mod, t):
+    _say(f"titles n=4096 seed=1 t={t}")
+    coll = fg.Collection.generate(fg.GEN_TITLES, 4096, 1)
+    sets = oracle_sets(oracle_mod, coll)
+    assert_join_equal(oracle_mod, sets, coll.selfjoin(t), t)
+
+
+@pytest.mark.parametrize("slots", [512, 1024, 2048, 4096])
+def test_table_sizes_and_splitting(fg, oracle_mod, slots):
+    """Tiny LDS tables force source-window splits and overflow retries; results must not change."""
+    _say(f"titles n=4096 seed=7 t=0.5 slots={slots}")
+    coll = fg.Collection.generate(fg.GEN_TITLES, 4096, 7)
+    sets = oracle_sets(oracle_mod, coll)
+    assert_join_equal(oracle_mod, sets, coll.selfjoin(0.5, table_slots=slots), 0.5)
+
+
+@pytest.mark.parametrize("t", [0.6, 0.75, 0.9])
+def test_titles_100k(fg, oracle_mod, t):
+    _say(f"titles n=100000 seed=2 t={t}")
+    coll = fg.Collection.generate(fg.GEN_TITLES, 100000, 2)
+    sets = oracle_sets(oracle_mod, coll)
+    assert_join_equal(oracle_mod, sets, coll.selfjoin(t), t)
+
+
+@pytest.mark.parametrize("t", [0.5, 0.8])
+def test_dblp_18k(fg, oracle_mod, t):
+    _say(f"dblp_t_18k q=3 t={t}")
+    coll = fg.Collection.from_text_bytes(dblp_text(), 3)
+    sets = oracle_sets(oracle_mod, coll)
+    assert_join_equal(oracle_mod, sets, coll.selfjoin(t), t)
+
+
+@pytest.mark.parametrize("t", [0.5, 0.8])
+def test_long_records(fg, oracle_mod, t):
+    """200-1200 char records: probe prefixes longer than one staged token batch at t=0.5."""
+    _say(f"long n=1500 seed=3 t={t}")
+    coll = fg.Collection.generate(fg.GEN_LONG, 1500, 3)
+    sets = oracle_sets(oracle_mod, coll)
+    assert_join_equal(oracle_mod, sets, coll.selfjoin(t), t)
+
+
+def test_edge_collections(fg, oracle_mod):
+    cases = {
+        "empty": np.zeros(0, np.int32),
+        "single": np.array([7, 3, 1, 2, 3], np.int32),
+        "identical": np.array([1, 3, 1, 2, 3, 2, 3, 1, 2, 3], np.int32),
+        "len1": np.array([1, 1, 5, 2, 1, 5, 3, 1, 6], np.int32),
+        "with_empty_record": np.array([9, 0, 1, 2, 4, 5, 2, 2, 4, 5], np.int32),
+        # t=0.8 boundary: |Q|=|S|=9, overlap 8 -> min_overlap = 8.000001f shortens the mid prefix
+        "prefix_boundary": np.array([1, 9, 0] + list(range(2, 10)) + [2, 9] + list(range(1, 10)), np.int32),
+        "prefix_hit": np.array([1, 9] + list(range(9)) + [2, 9] + list(range(8)) + [9], np.int32),
+        # t=0.6 boundary: floor(9/0.6f) = 14 < 15 drops a subset pair
+        "size_boundary": np.array([1, 9] + list(range(9)) + [2, 15] + list(range(15)), np.int32),
+    }
+    for name, ints in cases.items():
+        for t in (0.3, 0.6, 0.8, 0.9):
+            _say(f"edge {name} t={t}")
+            coll = fg.Collection.from_bin_ints(ints)
+            sets = oracle_mod.Sets(ints) if ints.size else oracle_sets(oracle_mod, coll)
+            assert_join_equal(oracle_mod, sets, coll.selfjoin(t), t)
+
+
+def test_repeatable(fg):
+    coll = fg.Collection.generate(fg.GEN_TITLES, 20000, 11)
+    a = coll.selfjoin(0.7)
+    b = coll.selfjoin(0.7)
+    assert np.array_equal(a.q, b.q) and np.array_equal(a.s, b.s) and np.array_equal(a.overlap, b.overlap)
+    assert a.stats["candidates"] == b.stats["candidates"]
