@@ -108,6 +108,8 @@ typedef struct {
     double ms_total;           /* whole call                                          */
     double ms_kernel;          /* device time of the join kernel launches (HIP events) */
     int64_t kernel_launches;   /* join kernel launches (1 unless work items were re-split) */
+    int64_t signature_checks;  /* candidates tested against the bucket-count overlap bound */
+    int64_t verified;          /* candidates whose tokens were merged                  */
 } fgss_join_stats;
 
 typedef struct {

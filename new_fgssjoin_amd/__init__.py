@@ -58,7 +58,8 @@ class JoinStats(C.Structure):
                 ("candidates", C.c_int64), ("intersections", C.c_int64), ("results", C.c_int64),
                 ("verify_tokens", C.c_int64), ("ms_index", C.c_double), ("ms_join", C.c_double),
                 ("ms_copy", C.c_double), ("ms_total", C.c_double),
-                ("ms_kernel", C.c_double), ("kernel_launches", C.c_int64)]
+                ("ms_kernel", C.c_double), ("kernel_launches", C.c_int64),
+                ("signature_checks", C.c_int64), ("verified", C.c_int64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

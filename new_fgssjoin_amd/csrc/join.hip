@@ -49,6 +49,8 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 // wavefronts per block in k_join: four, or two for the largest LDS tables
 constexpr int join_waves(int cap) { return cap >= 4096 ? 2 : 4; }
 constexpr int kTokBatch = 256;             // probe tokens whose posting ranges are staged at once
+constexpr int kSigBuckets = 128;           // token buckets of a record signature
+constexpr int kSigWords = kSigBuckets / 4; // one byte-wide count per bucket
 
 struct DevBuf {
     void *p = nullptr;
@@ -62,15 +64,17 @@ enum Ctr {
     C_CAND,
     C_INTER,
     C_VTOK,
+    C_SIGCHK,        // candidates tested against the signature bound
+    C_VERIFIED,      // candidates whose tokens were merged
     C_RESULTS,       // pairs emitted (may exceed the output capacity)
     C_OVERFLOW,      // work items whose LDS table overflowed
-    C_ITEMS_DONE,
     C_QUEUE,         // work-queue head
     C_COUNT
 };
 
 struct Item {
-    uint32_t q, a, b, pad;   // query and source sub-window [a, b)
+    uint32_t q, a, b;   // query and source sub-window [a, b)
+    uint32_t pad;       // expected useful postings (sizes the LDS table)
 };
 
 }  // namespace
@@ -81,7 +85,9 @@ struct fgss_device_sets {
     int64_t n = 0, ntok = 0;
     int32_t n_terms = 0, max_len = 0;
     uint32_t *tok = nullptr, *len = nullptr, *pos = nullptr;
-    DevBuf b_midlen, b_midoff, b_ka, b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_psid, b_pspos, b_loff, b_lmin,
+    uint32_t *sig = nullptr;    // kSigWords words per record (token-bucket byte counts)
+    uint8_t *sigbad = nullptr;  // per record: a count did not fit in a byte
+    DevBuf b_eq, b_midlen, b_midoff, b_ka, b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_psid, b_pspos, b_loff, b_lmin,
         b_hi, b_nparts, b_partoff, b_scanscr, b_items, b_items2, b_ovf, b_out, b_ctr, b_misc;
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     uint64_t *h_ctr = nullptr;   // pinned host mirror of the counters
@@ -118,7 +124,7 @@ void fgss_device_release(fgss_collection *c)
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(d->device);
     if (d->st) (void)hipStreamSynchronize(d->st);
-    DevBuf *bufs[] = {&d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib, &d->b_sortscr,
+    DevBuf *bufs[] = {&d->b_eq, &d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib, &d->b_sortscr,
                       &d->b_sid_of, &d->b_psid, &d->b_pspos, &d->b_loff, &d->b_lmin, &d->b_hi, &d->b_nparts,
                       &d->b_partoff, &d->b_scanscr, &d->b_items, &d->b_items2, &d->b_ovf, &d->b_out, &d->b_ctr,
                       &d->b_misc};
@@ -127,6 +133,8 @@ void fgss_device_release(fgss_collection *c)
     if (d->tok) (void)hipFree(d->tok);
     if (d->len) (void)hipFree(d->len);
     if (d->pos) (void)hipFree(d->pos);
+    if (d->sig) (void)hipFree(d->sig);
+    if (d->sigbad) (void)hipFree(d->sigbad);
     for (auto &e : d->ev)
         if (e) (void)hipEventDestroy(e);
     if (d->h_ctr) (void)hipHostFree(d->h_ctr);
@@ -134,6 +142,44 @@ void fgss_device_release(fgss_collection *c)
     (void)hipSetDevice(cur);
     delete d;
     c->dev = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// record signatures: per record, how many of its tokens fall in each of 128
+// hash buckets, as bytes.  For any two records sum_b min(cQ[b], cS[b]) bounds
+// their overlap from above, and equals (|Q| + |S| - sum_b |cQ[b] - cS[b]|) / 2,
+// which v_sad_u8 evaluates four buckets per instruction.  Built once per upload.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t sig_bucket(uint32_t tok) { return (tok * 0x9E3779B1u) >> (32 - 7); }
+
+__global__ __launch_bounds__(256) void k_signature(const uint32_t *__restrict__ tok, const uint32_t *__restrict__ pos,
+                                                   const uint32_t *__restrict__ len, int64_t n,
+                                                   uint32_t *__restrict__ sig, uint8_t *__restrict__ sigbad)
+{
+    __shared__ uint32_t hist[256 / kWave][kSigBuckets];
+    const int w = (int)threadIdx.x / kWave;
+    const int lane = lane_id();
+    uint32_t *h = hist[w];
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x / kWave) + w; r < n; r += nwaves) {
+        for (int b = lane; b < kSigBuckets; b += kWave) h[b] = 0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t l = len[r], p = pos[r];
+        for (uint32_t i = lane; i < l; i += kWave) atomicAdd(&h[sig_bucket(tok[p + i])], 1u);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        bool bad = false;
+        if (lane < kSigWords) {
+            uint32_t c0 = h[4 * lane], c1 = h[4 * lane + 1], c2 = h[4 * lane + 2], c3 = h[4 * lane + 3];
+            bad = (c0 | c1 | c2 | c3) > 255u;
+            sig[(size_t)r * kSigWords + lane] = (c0 & 255u) | (c1 & 255u) << 8 | (c2 & 255u) << 16 | (c3 & 255u) << 24;
+        }
+        const uint64_t anybad = __ballot(bad);
+        if (lane == 0) sigbad[r] = anybad ? 1 : 0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -178,7 +224,8 @@ extern "C" int fgss_collection_upload(fgss_collection *c)
     d->max_len = c->max_len;
     const size_t nb = (size_t)std::max<int64_t>(d->n, 1) * 4, tb = (size_t)std::max<int64_t>(d->ntok, 1) * 4;
     if (hipMalloc(&d->tok, tb) != hipSuccess || hipMalloc(&d->len, nb) != hipSuccess ||
-        hipMalloc(&d->pos, nb) != hipSuccess) {
+        hipMalloc(&d->pos, nb) != hipSuccess || hipMalloc(&d->sig, nb * kSigWords) != hipSuccess ||
+        hipMalloc(&d->sigbad, nb) != hipSuccess) {
         fgss_device_release(c);
         return fgss_fail(FGSS_ERR_NOMEM, "device allocation for the collection failed");
     }
@@ -187,6 +234,12 @@ extern "C" int fgss_collection_upload(fgss_collection *c)
     HIPCK(hipMemcpyAsync(d->tok, c->tok.data(), (size_t)d->ntok * 4, hipMemcpyHostToDevice, d->st));
     HIPCK(hipMemcpyAsync(d->len, c->len.data(), (size_t)d->n * 4, hipMemcpyHostToDevice, d->st));
     HIPCK(hipMemcpyAsync(d->pos, pos32.data(), (size_t)d->n * 4, hipMemcpyHostToDevice, d->st));
+    if (d->n > 0) {
+        const int64_t blocks = std::min<int64_t>((d->n + 3) / 4, 65536);
+        hipLaunchKernelGGL(k_signature, dim3((unsigned)blocks), dim3(256), 0, d->st, d->tok, d->pos, d->len, d->n,
+                           d->sig, d->sigbad);
+        HIPCK(hipGetLastError());
+    }
     HIPCK(hipStreamSynchronize(d->st));
     return FGSS_OK;
 }
@@ -270,7 +323,7 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
                                               const uint32_t *__restrict__ lmin, const uint32_t *__restrict__ loff,
                                               const uint32_t *__restrict__ psid, float t, uint32_t target,
                                               uint32_t *__restrict__ hi_out, uint32_t *__restrict__ nparts,
-                                              unsigned long long *__restrict__ ctr)
+                                              uint32_t *__restrict__ eq_out, unsigned long long *__restrict__ ctr)
 {
     const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     if (q >= n) return;
@@ -297,13 +350,15 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
     e = wave_sum(e);
     if (lane_id() == 0) {
         hi_out[q] = wend;
+        eq_out[q] = e;
         nparts[q] = e == 0 ? 0u : (e + target - 1) / target;
         atomicAdd(&ctr[C_PQ], (unsigned long long)qp);
     }
 }
 
 __global__ void k_items(const uint32_t *__restrict__ hi, const uint32_t *__restrict__ nparts,
-                        const uint32_t *__restrict__ partoff, int64_t n, Item *__restrict__ items)
+                        const uint32_t *__restrict__ eq, const uint32_t *__restrict__ partoff, int64_t n,
+                        Item *__restrict__ items)
 {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
@@ -311,12 +366,13 @@ __global__ void k_items(const uint32_t *__restrict__ hi, const uint32_t *__restr
     if (np == 0) return;
     uint64_t a0 = (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
     uint32_t o = partoff[q];
+    const uint32_t per = (eq[q] + np - 1) / np;   // expected postings per part: sizes its LDS table
     for (uint32_t p = 0; p < np; p++) {
         Item it;
         it.q = (uint32_t)q;
         it.a = (uint32_t)(a0 + W * p / np);
         it.b = (uint32_t)(a0 + W * (p + 1) / np);
-        it.pad = 0;
+        it.pad = per;
         items[o + p] = it;
     }
 }
@@ -334,7 +390,7 @@ __global__ void k_resplit(const Item *__restrict__ in, const uint32_t *__restric
         o.q = it.q;
         o.a = (uint32_t)(it.a + W * p / ways);
         o.b = (uint32_t)(it.a + W * (p + 1) / ways);
-        o.pad = 0;
+        o.pad = it.pad;   // same table size as the parent: four times the room per posting
         out[i * ways + p] = o;
     }
 }
@@ -345,6 +401,8 @@ __global__ void k_resplit(const Item *__restrict__ in, const uint32_t *__restric
 struct JoinArgs {
     const uint32_t *tok, *pos, *len;
     const uint32_t *loff, *psid, *pspos;
+    const uint32_t *sig;        // per record: kSigWords words of byte counts per token bucket
+    const uint8_t *sigbad;      // 1 when a bucket count did not fit in a byte
     const Item *items;
     uint32_t n_items;
     float t;
@@ -368,6 +426,21 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
+__device__ __forceinline__ int ceil_log2(uint32_t x)
+{
+    return x <= 1 ? 0 : 32 - __clz(x - 1);
+}
+
+// Work item = (query q, source window [a, b)).  Phases, all inside one wavefront:
+//  A  candidate generation: the postings of the probe-prefix tokens whose record id
+//     lies in the window, flattened over all tokens; per source an LDS hash slot keeps
+//     the match count and the last (query position, source position).
+//  B  positional filter (counted as the reference's "# Candidates"), then the bound
+//     m + min(|Q|-j-1, |S|-s-1) >= floor(bound); survivors compacted in place.
+//  C  signature bound: overlap <= (|Q| + |S| - SAD(countsQ, countsS)) / 2 over 128
+//     byte-wide token-bucket counts, one lane per candidate.
+//  D  exact overlap for the rest: the source tokens after the last prefix match,
+//     64 per step, tested against an LDS bitmap of the query, ballot + popcount.
 template <int CAP>
 __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
 {
@@ -388,7 +461,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
     for (uint32_t i = lane; i < A.bm_words; i += kWave) bm[i] = 0;
     wave_sync();
 
-    unsigned long long s_escan = 0, s_cand = 0, s_inter = 0, s_vtok = 0, s_done = 0;
+    unsigned long long s_escan = 0, s_cand = 0, s_sig = 0, s_ver = 0, s_inter = 0, s_vtok = 0;
     for (;;) {
         uint32_t it = 0;
         if (lane == 0) it = atomicAdd((unsigned int *)&A.ctr[C_QUEUE], 1u);
@@ -400,15 +473,19 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
         const uint32_t qpos = A.pos[q];
         uint32_t qp = fgss_max_prefix(lq, t);
         if (qp > lq) qp = lq;
+        // table sized for twice the expected postings (64 .. CAP slots)
+        int tlog = ceil_log2(2 * item.pad);
+        tlog = tlog < 6 ? 6 : (tlog > LOG2CAP ? LOG2CAP : tlog);
+        const uint32_t tsize = 1u << tlog;
 
-        for (uint32_t i = lane; i < CAP; i += kWave) { tkey[i] = kEmpty; tcnt[i] = 0; tlast[i] = 0; }
+        for (uint32_t i = lane; i < tsize; i += kWave) { tkey[i] = kEmpty; tcnt[i] = 0; tlast[i] = 0; }
         for (uint32_t i = lane; i < lq; i += kWave) {
             uint32_t tk = A.tok[qpos + i];
             atomicOr(&bm[tk >> 5], 1u << (tk & 31));
         }
         wave_sync();
 
-        // ---- candidate generation (filtering.cu:94-139) ----
+        // ---- A: candidate generation (filtering.cu:94-139) ----
         bool overflow = false;
         uint32_t escan = 0;
         for (uint32_t jb = 0; jb < qp; jb += kTokBatch) {
@@ -456,13 +533,13 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
                     const uint32_t idx = rlo[lo] + (e - rpre[lo]);
                     const uint32_t s = A.psid[idx];
                     const uint32_t sp = A.pspos[idx];
-                    uint32_t h = slot_hash(s, LOG2CAP);
-                    int probes = 0;
+                    uint32_t h = slot_hash(s, tlog);
+                    uint32_t probes = 0;
                     for (;;) {
                         uint32_t prev = atomicCAS(&tkey[h], kEmpty, s);
                         if (prev == kEmpty || prev == s) break;
-                        h = (h + 1) & (CAP - 1);
-                        if (++probes >= CAP) { ok = false; break; }
+                        h = (h + 1) & (tsize - 1);
+                        if (++probes >= tsize) { ok = false; break; }
                     }
                     if (ok) {
                         atomicAdd(&tcnt[h], 1u);
@@ -484,14 +561,14 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
             continue;
         }
 
-        // ---- positional filter + compaction of the candidates into the table front ----
+        // ---- B: positional filter, compaction of the candidates into the table front ----
         uint32_t ncand = 0, cand = 0;
-        for (uint32_t base = 0; base < CAP; base += kWave) {
+        for (uint32_t base = 0; base < tsize; base += kWave) {
             const uint32_t i = base + lane;
-            const uint32_t s = tkey[i];
-            const uint32_t m = tcnt[i];
-            const uint32_t last = tlast[i];
-            bool verify = false;
+            const uint32_t s = i < tsize ? tkey[i] : kEmpty;
+            const uint32_t m = i < tsize ? tcnt[i] : 0u;
+            const uint32_t last = i < tsize ? tlast[i] : 0u;
+            bool keep = false;
             if (s != kEmpty) {
                 const uint32_t ls = A.len[s];
                 const float bound = fgss_min_overlap(lq, ls, t);
@@ -501,11 +578,11 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
                 if (!((float)(m + minrem) < bound)) {          // filtering.cu:127-133 at the last match
                     cand++;
                     // overlap can still reach floor(bound): m + min(|Q|-js-1, |S|-ss-1)
-                    verify = (int)(m + minrem - 1) >= (int)floorf(bound);
+                    keep = (int)(m + minrem - 1) >= (int)floorf(bound);
                 }
             }
-            const uint64_t bal = __ballot(verify);
-            if (verify) {
+            const uint64_t bal = __ballot(keep);
+            if (keep) {
                 const uint32_t d = ncand + (uint32_t)__popcll(bal & lanemask_lt());
                 tkey[d] = s;
                 tcnt[d] = m;
@@ -516,9 +593,55 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
         cand = wave_sum(cand);
         wave_sync();
 
-        // ---- verification (checking.cu:93-175): overlap of S after the last prefix match ----
+        // ---- C: signature bound, one lane per candidate ----
+        const uint32_t nsig = ncand;
+        uint32_t nver = 0;
+        {
+            const uint4 *qs4 = (const uint4 *)(A.sig + (size_t)q * kSigWords);
+            const bool qbad = A.sigbad[q] != 0;
+            for (uint32_t base = 0; base < nsig; base += kWave) {
+                const uint32_t c = base + lane;
+                uint32_t s = 0, m = 0, last = 0;
+                bool keep = false;
+                if (c < nsig) {
+                    s = tkey[c];
+                    m = tcnt[c];
+                    last = tlast[c];
+                    const uint32_t ls = A.len[s];
+                    if (qbad || A.sigbad[s]) {
+                        keep = true;
+                    } else {
+                        const uint4 *ss4 = (const uint4 *)(A.sig + (size_t)s * kSigWords);
+                        uint32_t sad = 0;
+#pragma unroll
+                        for (int k = 0; k < kSigWords / 4; k++) {
+                            const uint4 a4 = qs4[k], b4 = ss4[k];
+                            sad = __builtin_amdgcn_sad_u8(a4.x, b4.x, sad);
+                            sad = __builtin_amdgcn_sad_u8(a4.y, b4.y, sad);
+                            sad = __builtin_amdgcn_sad_u8(a4.z, b4.z, sad);
+                            sad = __builtin_amdgcn_sad_u8(a4.w, b4.w, sad);
+                        }
+                        // sum over buckets of min(cQ, cS) = (|Q| + |S| - SAD) / 2 >= |Q n S|
+                        const int ub = ((int)(lq + ls) - (int)sad) / 2;
+                        keep = ub >= (int)floorf(fgss_min_overlap(lq, ls, t));
+                    }
+                }
+                const uint64_t bal = __ballot(keep);
+                wave_sync();
+                if (keep) {
+                    const uint32_t d = nver + (uint32_t)__popcll(bal & lanemask_lt());
+                    tkey[d] = s;
+                    tcnt[d] = m;
+                    tlast[d] = last;
+                }
+                nver += (uint32_t)__popcll(bal);
+            }
+            wave_sync();
+        }
+
+        // ---- D: verification (checking.cu:93-175): overlap of S after the last prefix match ----
         uint32_t inter = 0, vtok = 0;
-        for (uint32_t c = 0; c < ncand; c++) {
+        for (uint32_t c = 0; c < nver; c++) {
             const uint32_t s = tkey[c];
             const uint32_t m = tcnt[c];
             const uint32_t ss = tlast[c] & 0xFFFFu;
@@ -558,16 +681,18 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
         wave_sync();
         s_escan += escan;
         s_cand += cand;
+        s_sig += nsig;
+        s_ver += nver;
         s_inter += inter;
         s_vtok += vtok;
-        s_done += 1;
     }
     if (lane == 0) {
         atomicAdd(&A.ctr[C_ESCAN], s_escan);
         atomicAdd(&A.ctr[C_CAND], s_cand);
+        atomicAdd(&A.ctr[C_SIGCHK], s_sig);
+        atomicAdd(&A.ctr[C_VERIFIED], s_ver);
         atomicAdd(&A.ctr[C_INTER], s_inter);
         atomicAdd(&A.ctr[C_VTOK], s_vtok);
-        atomicAdd(&A.ctr[C_ITEMS_DONE], s_done);
     }
 }
 
@@ -681,6 +806,7 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     // 6-8: plan work items
     ENSURE(d->b_lmin, ((size_t)d->max_len + 1) * 4);
     ENSURE(d->b_hi, (size_t)n * 4);
+    ENSURE(d->b_eq, (size_t)n * 4);
     ENSURE(d->b_nparts, (size_t)n * 4);
     ENSURE(d->b_partoff, (size_t)(n + 1) * 4);
     uint32_t *lmin = P<uint32_t>(d->b_lmin), *hi = P<uint32_t>(d->b_hi), *nparts = P<uint32_t>(d->b_nparts),
@@ -688,7 +814,7 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     hipLaunchKernelGGL(k_lmin, dim3(grid_for(d->max_len + 1, 256)), dim3(256), 0, st, lmin, d->max_len, t);
     HIPCK(hipGetLastError());
     hipLaunchKernelGGL(k_plan, dim3(grid_for(n * kWave, 256)), dim3(256), 0, st, d->tok, d->pos, d->len, n, lmin,
-                       loff, psid, t, target, hi, nparts, ctr);
+                       loff, psid, t, target, hi, nparts, P<uint32_t>(d->b_eq), ctr);
     HIPCK(hipGetLastError());
     HIPCK(exclusive_scan<uint32_t>(nparts, partoff, n, misc + 1, P<uint32_t>(d->b_scanscr), st));
     uint32_t n_items = 0;
@@ -698,7 +824,8 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     HIPCK(hipStreamSynchronize(st));
     ENSURE(d->b_items, (size_t)std::max<uint32_t>(n_items, 1) * sizeof(Item));
     Item *items = P<Item>(d->b_items);
-    hipLaunchKernelGGL(k_items, dim3(grid_for(n, 256)), dim3(256), 0, st, hi, nparts, partoff, n, items);
+    hipLaunchKernelGGL(k_items, dim3(grid_for(n, 256)), dim3(256), 0, st, hi, nparts, P<uint32_t>(d->b_eq), partoff, n,
+                       items);
     HIPCK(hipGetLastError());
 
     // 9: join; overflowed items are split into finer source windows and run again,
@@ -717,6 +844,7 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
             JoinArgs A;
             A.tok = d->tok; A.pos = d->pos; A.len = d->len;
             A.loff = loff; A.psid = psid; A.pspos = pspos;
+            A.sig = d->sig; A.sigbad = d->sigbad;
             A.items = cur_items; A.n_items = cur_n; A.t = t;
             A.bm_words = (uint32_t)(((int64_t)d->n_terms + 31) / 32 + 1);
             A.out = P<int32_t>(d->b_out); A.out_cap = out_cap; A.ovf = P<uint32_t>(d->b_ovf); A.ctr = ctr;
@@ -734,7 +862,7 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
             if (d->h_ctr[C_RESULTS] <= out_cap) break;
             ENSURE(d->b_out, (size_t)(d->h_ctr[C_RESULTS] + d->h_ctr[C_RESULTS] / 2) * 12);
         }
-        for (int k = C_ESCAN; k <= C_VTOK; k++) acc[k] += d->h_ctr[k];
+        for (int k = C_ESCAN; k <= C_VERIFIED; k++) acc[k] += d->h_ctr[k];
         const uint64_t nres = d->h_ctr[C_RESULTS], novf = d->h_ctr[C_OVERFLOW];
         const size_t old = tri.size();
         tri.resize(old + 3 * (size_t)nres);
@@ -757,6 +885,8 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     stats.candidates = (int64_t)acc[C_CAND];
     stats.intersections = (int64_t)acc[C_INTER];
     stats.verify_tokens = (int64_t)acc[C_VTOK];
+    stats.signature_checks = (int64_t)acc[C_SIGCHK];
+    stats.verified = (int64_t)acc[C_VERIFIED];
     stats.results = (int64_t)(tri.size() / 3);
     float ms = 0;
     HIPCK(hipEventElapsedTime(&ms, d->ev[0], d->ev[1]));
