@@ -33,9 +33,10 @@ def log(msg):
 
 def kernel_bytes(st):
     """Algorithmic bytes of the join kernel for one join:
-    4 B per probe token + 8 B per useful posting (sid, pos) + 4 B per verified
-    source token + 12 B per result triple."""
-    return 4 * st["probe_tokens"] + 8 * st["postings_scanned"] + 4 * st["verify_tokens"] + 12 * st["results"]
+    4 B per probe token + 8 B per useful posting (record id, position) + 128 B per
+    signature tested + 4 B per verified source token + 12 B per result triple."""
+    return (4 * st["probe_tokens"] + 8 * st["postings_scanned"] + 128 * st["signature_checks"]
+            + 4 * st["verify_tokens"] + 12 * st["results"])
 
 
 def cpu_baseline(coll, gpu_pairs):
@@ -110,7 +111,8 @@ def main():
     per_t = {str(t): {"join_ms": round(p.stats["ms_total"], 3), "kernel_ms": round(p.stats["ms_kernel"], 3),
                       "index_ms": round(p.stats["ms_index"], 3), "pairs": len(p),
                       "candidates": p.stats["candidates"], "postings": p.stats["postings_scanned"],
-                      "verify_tokens": p.stats["verify_tokens"]}
+                      "verify_tokens": p.stats["verify_tokens"], "sig_checks": p.stats["signature_checks"],
+                      "verified": p.stats["verified"], "phase_cycles": p.stats["phase_cycles"]}
              for t, p in zip(THRESHOLDS, last)}
     achieved = kbytes / (kms / 1e3) / 1e9 if kms > 0 else 0.0
     out = {

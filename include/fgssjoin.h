@@ -110,6 +110,10 @@ typedef struct {
     int64_t kernel_launches;   /* join kernel launches (1 unless work items were re-split) */
     int64_t signature_checks;  /* candidates tested against the bucket-count overlap bound */
     int64_t verified;          /* candidates whose tokens were merged                  */
+    int64_t phase_cycles[6];   /* join kernel shader cycles summed over wavefronts: item setup,
+                                  candidate generation, positional filter, signature bound,
+                                  token verification, and the part of candidate generation
+                                  spent locating posting ranges                        */
 } fgss_join_stats;
 
 typedef struct {

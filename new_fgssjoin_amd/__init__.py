@@ -59,10 +59,13 @@ class JoinStats(C.Structure):
                 ("verify_tokens", C.c_int64), ("ms_index", C.c_double), ("ms_join", C.c_double),
                 ("ms_copy", C.c_double), ("ms_total", C.c_double),
                 ("ms_kernel", C.c_double), ("kernel_launches", C.c_int64),
-                ("signature_checks", C.c_int64), ("verified", C.c_int64)]
+                ("signature_checks", C.c_int64), ("verified", C.c_int64),
+                ("phase_cycles", C.c_int64 * 6)]
 
     def as_dict(self):
-        return {n: getattr(self, n) for n, _ in self._fields_}
+        d = {n: getattr(self, n) for n, _ in self._fields_}
+        d["phase_cycles"] = list(self.phase_cycles)
+        return d
 
 
 class JoinOpts(C.Structure):

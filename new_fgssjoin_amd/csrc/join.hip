@@ -48,7 +48,8 @@ namespace {
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 // wavefronts per block in k_join: four, or two for the largest LDS tables
 constexpr int join_waves(int cap) { return cap >= 4096 ? 2 : 4; }
-constexpr int kTokBatch = 256;             // probe tokens whose posting ranges are staged at once
+constexpr int kTokBatch = 128;             // probe tokens whose posting ranges are staged at once
+constexpr int kQueueBatch = 16;            // work items claimed per queue atomic
 constexpr int kSigBuckets = 128;           // token buckets of a record signature
 constexpr int kSigWords = kSigBuckets / 4; // one byte-wide count per bucket
 
@@ -69,13 +70,20 @@ enum Ctr {
     C_RESULTS,       // pairs emitted (may exceed the output capacity)
     C_OVERFLOW,      // work items whose LDS table overflowed
     C_QUEUE,         // work-queue head
+    C_CYC_SETUP,     // shader-clock cycles summed over waves, per k_join phase
+    C_CYC_A,
+    C_CYC_B,
+    C_CYC_C,
+    C_CYC_D,
+    C_CYC_RANGES,    // part of C_CYC_A spent locating posting ranges
     C_COUNT
 };
 
 struct Item {
     uint32_t q, a, b;   // query and source sub-window [a, b)
-    uint32_t pad;       // expected useful postings (sizes the LDS table)
+    uint32_t pad;       // expected useful postings (sizes the LDS table) | kWholeWindow
 };
+constexpr uint32_t kWholeWindow = 0x80000000u;   // the item covers the query's whole window
 
 }  // namespace
 
@@ -87,7 +95,7 @@ struct fgss_device_sets {
     uint32_t *tok = nullptr, *len = nullptr, *pos = nullptr;
     uint32_t *sig = nullptr;    // kSigWords words per record (token-bucket byte counts)
     uint8_t *sigbad = nullptr;  // per record: a count did not fit in a byte
-    DevBuf b_eq, b_midlen, b_midoff, b_ka, b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_psid, b_pspos, b_loff, b_lmin,
+    DevBuf b_probelen, b_poff, b_rng, b_eq, b_midlen, b_midoff, b_ka, b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_post, b_loff, b_lmin,
         b_hi, b_nparts, b_partoff, b_scanscr, b_items, b_items2, b_ovf, b_out, b_ctr, b_misc;
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     uint64_t *h_ctr = nullptr;   // pinned host mirror of the counters
@@ -124,8 +132,8 @@ void fgss_device_release(fgss_collection *c)
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(d->device);
     if (d->st) (void)hipStreamSynchronize(d->st);
-    DevBuf *bufs[] = {&d->b_eq, &d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib, &d->b_sortscr,
-                      &d->b_sid_of, &d->b_psid, &d->b_pspos, &d->b_loff, &d->b_lmin, &d->b_hi, &d->b_nparts,
+    DevBuf *bufs[] = {&d->b_probelen, &d->b_poff, &d->b_rng, &d->b_eq, &d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib, &d->b_sortscr,
+                      &d->b_sid_of, &d->b_post, &d->b_loff, &d->b_lmin, &d->b_hi, &d->b_nparts,
                       &d->b_partoff, &d->b_scanscr, &d->b_items, &d->b_items2, &d->b_ovf, &d->b_out, &d->b_ctr,
                       &d->b_misc};
     for (DevBuf *b : bufs)
@@ -247,13 +255,18 @@ extern "C" int fgss_collection_upload(fgss_collection *c)
 // ---------------------------------------------------------------------------
 // index build kernels
 // ---------------------------------------------------------------------------
-__global__ void k_midlen(const uint32_t *__restrict__ len, uint32_t *__restrict__ midlen, int64_t n, float t)
+// per record: mid-prefix length (indexed tokens) and probe-prefix length, both
+// clamped to the record length so that an empty record contributes nothing
+__global__ void k_midlen(const uint32_t *__restrict__ len, uint32_t *__restrict__ midlen,
+                         uint32_t *__restrict__ probelen, int64_t n, float t)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         uint32_t l = len[i];
         uint32_t mp = fgss_mid_prefix(l, t);
-        midlen[i] = mp < l ? mp : l;   // an empty record contributes nothing
+        uint32_t qp = fgss_max_prefix(l, t);
+        midlen[i] = mp < l ? mp : l;
+        probelen[i] = qp < l ? qp : l;
     }
 }
 
@@ -271,17 +284,16 @@ __global__ void k_entries(const uint32_t *__restrict__ tok, const uint32_t *__re
     }
 }
 
-// postings in term order: psid[i] = record, pspos[i] = position of the token in it
+// postings in term order, packed: x = record id, y = token position | record length << 16
 __global__ void k_postings(const uint32_t *__restrict__ sidx, const uint32_t *__restrict__ sid_of,
-                           const uint32_t *__restrict__ midoff, uint32_t nE, uint32_t *__restrict__ psid,
-                           uint32_t *__restrict__ pspos)
+                           const uint32_t *__restrict__ midoff, const uint32_t *__restrict__ len, uint32_t nE,
+                           uint2 *__restrict__ post)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nE) return;
     uint32_t e = sidx[i];
     uint32_t s = sid_of[e];
-    psid[i] = s;
-    pspos[i] = e - midoff[s];
+    post[i] = make_uint2(s, (e - midoff[s]) | (len[s] << 16));
 }
 
 // loff[t] = first posting of term t; loff[n_terms] = nE
@@ -306,6 +318,17 @@ __global__ void k_lmin(uint32_t *__restrict__ lmin, int32_t max_len, float t)
     lmin[l] = ls;
 }
 
+// first index in [lo, hi) with post[index].x (record id) >= v (ascending within a list)
+__device__ __forceinline__ uint32_t lower_bound_sid(const uint2 *__restrict__ post, uint32_t lo, uint32_t hi, uint32_t v)
+{
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (post[mid].x < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // first index in [lo, hi) with a[index] >= v (a ascending)
 __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *__restrict__ a, uint32_t lo, uint32_t hi, uint32_t v)
 {
@@ -317,42 +340,46 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *__restrict__
     return lo;
 }
 
-// wave per query: window hi(q) and number of useful postings E(q) -> work item count
+// Waves loop over queries: the source window [q+1, hi) and, for each probe-prefix
+// token, the range of its posting list inside the window (stored for k_join), whose
+// total E(q) sets the number of work items.
 __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, const uint32_t *__restrict__ pos,
                                               const uint32_t *__restrict__ len, int64_t n,
-                                              const uint32_t *__restrict__ lmin, const uint32_t *__restrict__ loff,
-                                              const uint32_t *__restrict__ psid, float t, uint32_t target,
-                                              uint32_t *__restrict__ hi_out, uint32_t *__restrict__ nparts,
-                                              uint32_t *__restrict__ eq_out, unsigned long long *__restrict__ ctr)
+                                              const uint32_t *__restrict__ probelen,
+                                              const uint32_t *__restrict__ poff, const uint32_t *__restrict__ lmin,
+                                              const uint32_t *__restrict__ loff, const uint2 *__restrict__ post,
+                                              uint32_t target, uint2 *__restrict__ rng, uint32_t *__restrict__ hi_out,
+                                              uint32_t *__restrict__ nparts, uint32_t *__restrict__ eq_out)
 {
-    const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    if (q >= n) return;
-    const uint32_t lq = len[q];
-    uint32_t qp = fgss_max_prefix(lq, t);
-    if (qp > lq) qp = lq;
-    // window: all s > q with len[s] >= lmin[lq]; lengths are non-increasing
-    const uint32_t need = lmin[lq];
-    uint32_t lo = (uint32_t)q + 1, hi = (uint32_t)n;   // first s with len[s] < need
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (len[mid] >= need) lo = mid + 1;
-        else hi = mid;
-    }
-    const uint32_t wend = lo;   // window is [q+1, wend)
-    uint32_t e = 0;
-    for (uint32_t j = lane_id(); j < qp; j += kWave) {
-        uint32_t term = tok[pos[q] + j];
-        uint32_t L0 = loff[term], L1 = loff[term + 1];
-        uint32_t a = lower_bound_u32(psid, L0, L1, (uint32_t)q + 1);
-        uint32_t b = lower_bound_u32(psid, a, L1, wend);
-        e += b - a;
-    }
-    e = wave_sum(e);
-    if (lane_id() == 0) {
-        hi_out[q] = wend;
-        eq_out[q] = e;
-        nparts[q] = e == 0 ? 0u : (e + target - 1) / target;
-        atomicAdd(&ctr[C_PQ], (unsigned long long)qp);
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < n; q += nwaves) {
+        const uint32_t lq = len[q];
+        const uint32_t qp = probelen[q];
+        // window: all s > q with len[s] >= lmin[lq]; lengths are non-increasing
+        const uint32_t need = lmin[lq];
+        uint32_t lo = (uint32_t)q + 1, hi = (uint32_t)n;   // first s with len[s] < need
+        while (lo < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (len[mid] >= need) lo = mid + 1;
+            else hi = mid;
+        }
+        const uint32_t wend = lo;   // window is [q+1, wend)
+        const uint32_t ro = poff[q];
+        uint32_t e = 0;
+        for (uint32_t j = lane_id(); j < qp; j += kWave) {
+            uint32_t term = tok[pos[q] + j];
+            uint32_t L0 = loff[term], L1 = loff[term + 1];
+            uint32_t a = lower_bound_sid(post, L0, L1, (uint32_t)q + 1);
+            uint32_t b = lower_bound_sid(post, a, L1, wend);
+            rng[ro + j] = make_uint2(a, b);
+            e += b - a;
+        }
+        e = wave_sum(e);
+        if (lane_id() == 0) {
+            hi_out[q] = wend;
+            eq_out[q] = e;
+            nparts[q] = e == 0 ? 0u : (e + target - 1) / target;
+        }
     }
 }
 
@@ -372,7 +399,7 @@ __global__ void k_items(const uint32_t *__restrict__ hi, const uint32_t *__restr
         it.q = (uint32_t)q;
         it.a = (uint32_t)(a0 + W * p / np);
         it.b = (uint32_t)(a0 + W * (p + 1) / np);
-        it.pad = per;
+        it.pad = per | (np == 1 ? kWholeWindow : 0u);
         items[o + p] = it;
     }
 }
@@ -390,7 +417,7 @@ __global__ void k_resplit(const Item *__restrict__ in, const uint32_t *__restric
         o.q = it.q;
         o.a = (uint32_t)(it.a + W * p / ways);
         o.b = (uint32_t)(it.a + W * (p + 1) / ways);
-        o.pad = it.pad;   // same table size as the parent: four times the room per posting
+        o.pad = it.pad & ~kWholeWindow;   // same table size as the parent: four times the room per posting
         out[i * ways + p] = o;
     }
 }
@@ -400,13 +427,15 @@ __global__ void k_resplit(const Item *__restrict__ in, const uint32_t *__restric
 // ---------------------------------------------------------------------------
 struct JoinArgs {
     const uint32_t *tok, *pos, *len;
-    const uint32_t *loff, *psid, *pspos;
+    const uint32_t *loff;
+    const uint2 *post;          // postings (record id, token position), grouped by term
     const uint32_t *sig;        // per record: kSigWords words of byte counts per token bucket
     const uint8_t *sigbad;      // 1 when a bucket count did not fit in a byte
+    const uint2 *rng;           // per probe token: posting range inside the query's window
+    const uint32_t *poff;       // per query: first entry of its probe tokens in rng
     const Item *items;
     uint32_t n_items;
     float t;
-    uint32_t bm_words;          // query bitmap words (n_terms / 32 rounded up)
     int32_t *out;               // (q, s, overlap) triples
     uint32_t out_cap;
     uint32_t *ovf;              // indices of overflowed items
@@ -433,40 +462,59 @@ __device__ __forceinline__ int ceil_log2(uint32_t x)
 
 // Work item = (query q, source window [a, b)).  Phases, all inside one wavefront:
 //  A  candidate generation: the postings of the probe-prefix tokens whose record id
-//     lies in the window, flattened over all tokens; per source an LDS hash slot keeps
-//     the match count and the last (query position, source position).
-//  B  positional filter (counted as the reference's "# Candidates"), then the bound
-//     m + min(|Q|-j-1, |S|-s-1) >= floor(bound); survivors compacted in place.
+//     lies in the window, flattened over all tokens, kUnroll per lane in flight; per
+//     source an LDS hash slot keeps the match count and the last (query position,
+//     source position); each new source is appended to a compact list together with
+//     its length (carried by the posting).
+//  B  positional filter over the list (counted as the reference's "# Candidates"),
+//     then the bound m + min(|Q|-j-1, |S|-s-1) >= floor(bound).
 //  C  signature bound: overlap <= (|Q| + |S| - SAD(countsQ, countsS)) / 2 over 128
 //     byte-wide token-bucket counts, one lane per candidate.
 //  D  exact overlap for the rest: the source tokens after the last prefix match,
-//     64 per step, tested against an LDS bitmap of the query, ballot + popcount.
+//     64 per step, each looked up in the query (staged in LDS when it fits),
+//     ballot + popcount.
+// Each phase compacts the list in place and resets the table slots it drops, so the
+// table is empty again when the item ends: no per-item clear.
+constexpr int kUnroll = 4;
+constexpr int kQStage = 192;   // query tokens staged in LDS for verification (longer queries stay in HBM)
+// LDS words per wavefront: table (key, count, last match), list, token ranges, query stage
+constexpr uint32_t join_wave_words(int cap) { return 4u * cap + 2u * kTokBatch + 1u + kQStage; }
+
 template <int CAP>
 __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
 {
-    constexpr int LOG2CAP = CAP == 512 ? 9 : CAP == 1024 ? 10 : CAP == 2048 ? 11 : 12;
+    constexpr int LOG2CAP = CAP == 256 ? 8 : CAP == 512 ? 9 : CAP == 1024 ? 10 : CAP == 2048 ? 11 : 12;
     static_assert((1 << LOG2CAP) == CAP, "CAP must be a power of two");
     extern __shared__ uint32_t smem[];
     const int w = (int)threadIdx.x / kWave;
     const int lane = lane_id();
-    const uint32_t per_wave = 3 * CAP + 2 * (kTokBatch + 1) + A.bm_words;
-    uint32_t *tkey = smem + (size_t)w * per_wave;
+    uint32_t *tkey = smem + (size_t)w * join_wave_words(CAP);
     uint32_t *tcnt = tkey + CAP;
     uint32_t *tlast = tcnt + CAP;
-    uint32_t *rlo = tlast + CAP;
-    uint32_t *rpre = rlo + kTokBatch + 1;
-    uint32_t *bm = rpre + kTokBatch + 1;
+    uint32_t *clist = tlast + CAP;        // slot | |S| << 16, one per distinct source
+    uint32_t *rlo = clist + CAP;
+    uint32_t *rpre = rlo + kTokBatch;     // kTokBatch + 1 entries
+    uint32_t *qstage = rpre + kTokBatch + 1;
     const float t = A.t;
 
-    for (uint32_t i = lane; i < A.bm_words; i += kWave) bm[i] = 0;
+    for (uint32_t i = lane; i < CAP; i += kWave) { tkey[i] = kEmpty; tcnt[i] = 0; tlast[i] = 0; }
     wave_sync();
 
     unsigned long long s_escan = 0, s_cand = 0, s_sig = 0, s_ver = 0, s_inter = 0, s_vtok = 0;
+    unsigned long long cyc[6] = {0, 0, 0, 0, 0, 0};
+    unsigned long long c0 = clock64(), c1;
+    // items are claimed kQueueBatch at a time: one global atomic per batch
+    uint32_t batch_next = 0, batch_end = 0;
     for (;;) {
-        uint32_t it = 0;
-        if (lane == 0) it = atomicAdd((unsigned int *)&A.ctr[C_QUEUE], 1u);
-        it = __shfl(it, 0, kWave);
-        if (it >= A.n_items) break;
+        if (batch_next >= batch_end) {
+            uint32_t b0 = 0;
+            if (lane == 0) b0 = atomicAdd((unsigned int *)&A.ctr[C_QUEUE], (unsigned)kQueueBatch);
+            b0 = __shfl(b0, 0, kWave);
+            batch_next = b0;
+            batch_end = min(b0 + (uint32_t)kQueueBatch, A.n_items);
+        }
+        if (batch_next >= A.n_items) break;
+        const uint32_t it = batch_next++;
         const Item item = A.items[it];
         const uint32_t q = item.q, wa = item.a, wb = item.b;
         const uint32_t lq = A.len[q];
@@ -474,27 +522,27 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
         uint32_t qp = fgss_max_prefix(lq, t);
         if (qp > lq) qp = lq;
         // table sized for twice the expected postings (64 .. CAP slots)
-        int tlog = ceil_log2(2 * item.pad);
+        int tlog = ceil_log2(2 * (item.pad & ~kWholeWindow));
         tlog = tlog < 6 ? 6 : (tlog > LOG2CAP ? LOG2CAP : tlog);
         const uint32_t tsize = 1u << tlog;
 
-        for (uint32_t i = lane; i < tsize; i += kWave) { tkey[i] = kEmpty; tcnt[i] = 0; tlast[i] = 0; }
-        for (uint32_t i = lane; i < lq; i += kWave) {
-            uint32_t tk = A.tok[qpos + i];
-            atomicOr(&bm[tk >> 5], 1u << (tk & 31));
-        }
-        wave_sync();
+        const bool whole = (item.pad & kWholeWindow) != 0;
+        const uint2 *qrng = A.rng + A.poff[q];
+        c1 = clock64(); cyc[0] += c1 - c0; c0 = c1;
 
         // ---- A: candidate generation (filtering.cu:94-139) ----
         bool overflow = false;
-        uint32_t escan = 0;
-        for (uint32_t jb = 0; jb < qp; jb += kTokBatch) {
+        uint32_t escan = 0, nlist = 0;
+        for (uint32_t jb = 0; jb < qp && !overflow; jb += kTokBatch) {
             const uint32_t nb = min((uint32_t)kTokBatch, qp - jb);
+            const unsigned long long r0 = clock64();
             for (uint32_t jj = lane; jj < nb; jj += kWave) {
-                uint32_t term = A.tok[qpos + jb + jj];
-                uint32_t L0 = A.loff[term], L1 = A.loff[term + 1];
-                uint32_t lo = lower_bound_u32(A.psid, L0, L1, wa);
-                uint32_t hi = lower_bound_u32(A.psid, lo, L1, wb);
+                const uint2 r = qrng[jb + jj];   // the token's postings inside the whole window
+                uint32_t lo = r.x, hi = r.y;
+                if (!whole) {                    // narrow to this item's sub-window
+                    lo = lower_bound_sid(A.post, r.x, r.y, wa);
+                    hi = lower_bound_sid(A.post, lo, r.y, wb);
+                }
                 rlo[jj] = lo;
                 rpre[jj] = hi - lo;
             }
@@ -519,58 +567,88 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
             const uint32_t total = __shfl(inc, kWave - 1, kWave);
             if (lane == 0) rpre[nb] = total;
             wave_sync();
+            cyc[5] += clock64() - r0;
             escan += total;
-            for (uint32_t e0 = 0; e0 < total; e0 += kWave) {
-                const uint32_t e = e0 + lane;
-                bool ok = true;
-                if (e < total) {
-                    uint32_t lo = 0, hi = nb;   // largest jj with rpre[jj] <= e
-                    while (hi - lo > 1) {
-                        uint32_t mid = (lo + hi) >> 1;
-                        if (rpre[mid] <= e) lo = mid;
-                        else hi = mid;
-                    }
-                    const uint32_t idx = rlo[lo] + (e - rpre[lo]);
-                    const uint32_t s = A.psid[idx];
-                    const uint32_t sp = A.pspos[idx];
-                    uint32_t h = slot_hash(s, tlog);
-                    uint32_t probes = 0;
-                    for (;;) {
-                        uint32_t prev = atomicCAS(&tkey[h], kEmpty, s);
-                        if (prev == kEmpty || prev == s) break;
-                        h = (h + 1) & (tsize - 1);
-                        if (++probes >= tsize) { ok = false; break; }
-                    }
-                    if (ok) {
-                        atomicAdd(&tcnt[h], 1u);
-                        atomicMax(&tlast[h], ((jb + lo) << 16) | sp);
+            const int steps = ceil_log2(nb);   // wave-uniform search depth
+            for (uint32_t e0 = 0; e0 < total; e0 += kWave * kUnroll) {
+                uint2 pe[kUnroll];
+                uint32_t jv[kUnroll], seg[kUnroll], e[kUnroll];
+                // segment of each flattened posting: the largest jj with rpre[jj] <= e,
+                // found by a fixed-depth branchless search so the kUnroll searches interleave
+#pragma unroll
+                for (int k = 0; k < kUnroll; k++) { e[k] = e0 + (uint32_t)(k * kWave) + lane; seg[k] = 0; }
+                for (int st = steps - 1; st >= 0; st--) {
+#pragma unroll
+                    for (int k = 0; k < kUnroll; k++) {
+                        const uint32_t mid = seg[k] + (1u << st);
+                        if (mid < nb && rpre[mid] <= e[k]) seg[k] = mid;
                     }
                 }
-                if (__ballot(!ok)) { overflow = true; break; }
+                // load kUnroll postings per lane before touching the table
+#pragma unroll
+                for (int k = 0; k < kUnroll; k++) {
+                    jv[k] = kEmpty;
+                    if (e[k] < total) {
+                        jv[k] = jb + seg[k];
+                        pe[k] = A.post[rlo[seg[k]] + (e[k] - rpre[seg[k]])];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kUnroll; k++) {
+                    const bool valid = jv[k] != kEmpty;
+                    bool ok = true, isnew = false;
+                    uint32_t h = 0;
+                    if (valid) {
+                        const uint32_t s = pe[k].x;
+                        h = slot_hash(s, tlog);
+                        uint32_t probes = 0;
+                        for (;;) {
+                            const uint32_t prev = atomicCAS(&tkey[h], kEmpty, s);
+                            if (prev == kEmpty) { isnew = true; break; }
+                            if (prev == s) break;
+                            h = (h + 1) & (tsize - 1);
+                            if (++probes >= tsize) { ok = false; break; }
+                        }
+                        if (ok) {
+                            atomicAdd(&tcnt[h], 1u);
+                            atomicMax(&tlast[h], (jv[k] << 16) | (pe[k].y & 0xFFFFu));
+                        }
+                    }
+                    const uint64_t nm = __ballot(isnew);
+                    if (isnew) clist[nlist + (uint32_t)__popcll(nm & lanemask_lt())] = h | (pe[k].y & 0xFFFF0000u);
+                    nlist += (uint32_t)__popcll(nm);
+                    if (__ballot(!ok)) overflow = true;
+                }
+                if (overflow) break;
             }
             wave_sync();
-            if (overflow) break;
         }
         if (overflow) {
             if (lane == 0) {
                 uint32_t k = (uint32_t)atomicAdd(&A.ctr[C_OVERFLOW], 1ull);
                 A.ovf[k] = it;
             }
-            for (uint32_t i = lane; i < lq; i += kWave) bm[A.tok[qpos + i] >> 5] = 0;
+            for (uint32_t k = lane; k < nlist; k += kWave) {
+                const uint32_t h = clist[k] & 0xFFFFu;
+                tkey[h] = kEmpty; tcnt[h] = 0; tlast[h] = 0;
+            }
             wave_sync();
             continue;
         }
+        c1 = clock64(); cyc[1] += c1 - c0; c0 = c1;
 
-        // ---- B: positional filter, compaction of the candidates into the table front ----
+        // ---- B: positional filter over the distinct sources ----
         uint32_t ncand = 0, cand = 0;
-        for (uint32_t base = 0; base < tsize; base += kWave) {
-            const uint32_t i = base + lane;
-            const uint32_t s = i < tsize ? tkey[i] : kEmpty;
-            const uint32_t m = i < tsize ? tcnt[i] : 0u;
-            const uint32_t last = i < tsize ? tlast[i] : 0u;
+        for (uint32_t base = 0; base < nlist; base += kWave) {
+            const uint32_t k = base + lane;
             bool keep = false;
-            if (s != kEmpty) {
-                const uint32_t ls = A.len[s];
+            uint32_t ent = 0, h = 0;
+            if (k < nlist) {
+                ent = clist[k];
+                h = ent & 0xFFFFu;
+                const uint32_t ls = ent >> 16;
+                const uint32_t m = tcnt[h];
+                const uint32_t last = tlast[h];
                 const float bound = fgss_min_overlap(lq, ls, t);
                 const uint32_t js = last >> 16, ss = last & 0xFFFFu;
                 const uint32_t rq = lq - js, rs = ls - ss;
@@ -580,18 +658,16 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
                     // overlap can still reach floor(bound): m + min(|Q|-js-1, |S|-ss-1)
                     keep = (int)(m + minrem - 1) >= (int)floorf(bound);
                 }
+                if (!keep) { tkey[h] = kEmpty; tcnt[h] = 0; tlast[h] = 0; }
             }
             const uint64_t bal = __ballot(keep);
-            if (keep) {
-                const uint32_t d = ncand + (uint32_t)__popcll(bal & lanemask_lt());
-                tkey[d] = s;
-                tcnt[d] = m;
-                tlast[d] = last;
-            }
+            wave_sync();
+            if (keep) clist[ncand + (uint32_t)__popcll(bal & lanemask_lt())] = ent;
             ncand += (uint32_t)__popcll(bal);
         }
         cand = wave_sum(cand);
         wave_sync();
+        c1 = clock64(); cyc[2] += c1 - c0; c0 = c1;
 
         // ---- C: signature bound, one lane per candidate ----
         const uint32_t nsig = ncand;
@@ -601,13 +677,12 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
             const bool qbad = A.sigbad[q] != 0;
             for (uint32_t base = 0; base < nsig; base += kWave) {
                 const uint32_t c = base + lane;
-                uint32_t s = 0, m = 0, last = 0;
+                uint32_t ent = 0;
                 bool keep = false;
                 if (c < nsig) {
-                    s = tkey[c];
-                    m = tcnt[c];
-                    last = tlast[c];
-                    const uint32_t ls = A.len[s];
+                    ent = clist[c];
+                    const uint32_t h = ent & 0xFFFFu, ls = ent >> 16;
+                    const uint32_t s = tkey[h];
                     if (qbad || A.sigbad[s]) {
                         keep = true;
                     } else {
@@ -625,27 +700,31 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
                         const int ub = ((int)(lq + ls) - (int)sad) / 2;
                         keep = ub >= (int)floorf(fgss_min_overlap(lq, ls, t));
                     }
+                    if (!keep) { tkey[h] = kEmpty; tcnt[h] = 0; tlast[h] = 0; }
                 }
                 const uint64_t bal = __ballot(keep);
                 wave_sync();
-                if (keep) {
-                    const uint32_t d = nver + (uint32_t)__popcll(bal & lanemask_lt());
-                    tkey[d] = s;
-                    tcnt[d] = m;
-                    tlast[d] = last;
-                }
+                if (keep) clist[nver + (uint32_t)__popcll(bal & lanemask_lt())] = ent;
                 nver += (uint32_t)__popcll(bal);
             }
             wave_sync();
         }
+        c1 = clock64(); cyc[3] += c1 - c0; c0 = c1;
 
         // ---- D: verification (checking.cu:93-175): overlap of S after the last prefix match ----
         uint32_t inter = 0, vtok = 0;
+        const bool staged = lq <= (uint32_t)kQStage;
+        if (nver > 0 && staged) {
+            for (uint32_t i = lane; i < lq; i += kWave) qstage[i] = A.tok[qpos + i];
+            wave_sync();
+        }
+        const uint32_t *Qv = staged ? qstage : A.tok + qpos;
         for (uint32_t c = 0; c < nver; c++) {
-            const uint32_t s = tkey[c];
-            const uint32_t m = tcnt[c];
-            const uint32_t ss = tlast[c] & 0xFFFFu;
-            const uint32_t ls = A.len[s];
+            const uint32_t ent = clist[c];
+            const uint32_t h = ent & 0xFFFFu, ls = ent >> 16;
+            const uint32_t s = tkey[h];
+            const uint32_t m = tcnt[h];
+            const uint32_t js = tlast[h] >> 16, ss = tlast[h] & 0xFFFFu;
             const uint32_t spos = A.pos[s];
             const float bound = fgss_min_overlap(lq, ls, t);
             const int floorb = (int)floorf(bound);
@@ -655,8 +734,15 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
                 const uint32_t i = base + lane;
                 bool hit = false;
                 if (i < ls) {
-                    uint32_t tk = A.tok[spos + i];
-                    hit = (bm[tk >> 5] >> (tk & 31)) & 1u;
+                    // tokens of S after position ss exceed Q[js]: search Q(js, lq)
+                    const uint32_t tk = A.tok[spos + i];
+                    uint32_t lo = js + 1, hi = lq;
+                    while (lo < hi) {
+                        uint32_t mid = (lo + hi) >> 1;
+                        if (Qv[mid] < tk) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    hit = lo < lq && Qv[lo] == tk;
                 }
                 ov += (uint32_t)__popcll(__ballot(hit));
                 const int rem = (int)ls - (int)(base + kWave);
@@ -677,8 +763,13 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
                 }
             }
         }
-        for (uint32_t i = lane; i < lq; i += kWave) bm[A.tok[qpos + i] >> 5] = 0;
         wave_sync();
+        for (uint32_t c = lane; c < nver; c += kWave) {
+            const uint32_t h = clist[c] & 0xFFFFu;
+            tkey[h] = kEmpty; tcnt[h] = 0; tlast[h] = 0;
+        }
+        wave_sync();
+        c1 = clock64(); cyc[4] += c1 - c0; c0 = c1;
         s_escan += escan;
         s_cand += cand;
         s_sig += nsig;
@@ -693,20 +784,20 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
         atomicAdd(&A.ctr[C_VERIFIED], s_ver);
         atomicAdd(&A.ctr[C_INTER], s_inter);
         atomicAdd(&A.ctr[C_VTOK], s_vtok);
+        for (int k = 0; k < 6; k++) atomicAdd(&A.ctr[C_CYC_SETUP + k], cyc[k]);
     }
 }
 
 template <int CAP>
-static size_t join_smem_bytes(uint32_t bm_words)
+static size_t join_smem_bytes()
 {
-    return (size_t)join_waves(CAP) * (3 * CAP + 2 * (kTokBatch + 1) + bm_words) * sizeof(uint32_t);
+    return (size_t)join_waves(CAP) * join_wave_words(CAP) * sizeof(uint32_t);
 }
 
 template <int CAP>
 static int launch_join(fgss_device_sets *d, JoinArgs A, int *blocks_out)
 {
-    const size_t smem = join_smem_bytes<CAP>(A.bm_words);
-    if (smem > 160 * 1024) return fgss_fail(FGSS_ERR_UNSUPPORTED, "query bitmap does not fit in LDS");
+    const size_t smem = join_smem_bytes<CAP>();
     HIPCK(hipFuncSetAttribute((const void *)k_join<CAP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     int per_cu = 0;
     HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_join<CAP>, join_waves(CAP) * kWave, smem));
@@ -725,11 +816,12 @@ static int launch_join(fgss_device_sets *d, JoinArgs A, int *blocks_out)
 static int run_join(fgss_device_sets *d, JoinArgs A, int cap)
 {
     switch (cap) {
+    case 256: return launch_join<256>(d, A, nullptr);
     case 512: return launch_join<512>(d, A, nullptr);
     case 1024: return launch_join<1024>(d, A, nullptr);
     case 2048: return launch_join<2048>(d, A, nullptr);
     case 4096: return launch_join<4096>(d, A, nullptr);
-    default: return fgss_fail(FGSS_ERR_ARG, "table_slots must be 512, 1024, 2048 or 4096");
+    default: return fgss_fail(FGSS_ERR_ARG, "table_slots must be 256, 512, 1024, 2048 or 4096");
     }
 }
 
@@ -744,7 +836,7 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     fgss_device_sets *d = c->dev;
     hipStream_t st = d->st;
     const int64_t n = d->n;
-    int cap = opts && opts->table_slots ? opts->table_slots : 1024;
+    int cap = opts && opts->table_slots ? opts->table_slots : 512;
     uint32_t target = (uint32_t)(cap / 2);
     memset(&stats, 0, sizeof(stats));
     tri.clear();
@@ -762,11 +854,16 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     ENSURE(d->b_misc, 64);
     uint32_t *midlen = P<uint32_t>(d->b_midlen), *midoff = P<uint32_t>(d->b_midoff);
     uint32_t *misc = P<uint32_t>(d->b_misc);
-    hipLaunchKernelGGL(k_midlen, dim3(grid_for(n, 256)), dim3(256), 0, st, d->len, midlen, n, t);
+    ENSURE(d->b_probelen, (size_t)n * 4);
+    ENSURE(d->b_poff, (size_t)(n + 1) * 4);
+    uint32_t *probelen = P<uint32_t>(d->b_probelen), *poff = P<uint32_t>(d->b_poff);
+    hipLaunchKernelGGL(k_midlen, dim3(grid_for(n, 256)), dim3(256), 0, st, d->len, midlen, probelen, n, t);
     HIPCK(hipGetLastError());
     HIPCK(exclusive_scan<uint32_t>(midlen, midoff, n, misc, P<uint32_t>(d->b_scanscr), st));
-    uint32_t nE = 0;
+    HIPCK(exclusive_scan<uint32_t>(probelen, poff, n, misc + 2, P<uint32_t>(d->b_scanscr), st));
+    uint32_t nE = 0, nP = 0;
     HIPCK(hipMemcpyAsync(&nE, misc, 4, hipMemcpyDeviceToHost, st));
+    HIPCK(hipMemcpyAsync(&nP, misc + 2, 4, hipMemcpyDeviceToHost, st));
     HIPCK(hipStreamSynchronize(st));
     stats.index_entries = nE;
 
@@ -777,13 +874,12 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     ENSURE(d->b_ia, eb);
     ENSURE(d->b_ib, eb);
     ENSURE(d->b_sid_of, eb);
-    ENSURE(d->b_psid, eb + 4);
-    ENSURE(d->b_pspos, eb + 4);
+    ENSURE(d->b_post, 2 * eb + 8);
     ENSURE(d->b_loff, ((size_t)d->n_terms + 2) * 4);
     ENSURE(d->b_sortscr, (size_t)radix_scratch_elems(std::max<uint32_t>(nE, 1)) * 4);
     uint32_t *ka = P<uint32_t>(d->b_ka), *kb = P<uint32_t>(d->b_kb), *ia = P<uint32_t>(d->b_ia),
-             *ib = P<uint32_t>(d->b_ib), *sid_of = P<uint32_t>(d->b_sid_of), *psid = P<uint32_t>(d->b_psid),
-             *pspos = P<uint32_t>(d->b_pspos), *loff = P<uint32_t>(d->b_loff);
+             *ib = P<uint32_t>(d->b_ib), *sid_of = P<uint32_t>(d->b_sid_of), *loff = P<uint32_t>(d->b_loff);
+    uint2 *post = P<uint2>(d->b_post);
     hipLaunchKernelGGL(k_entries, dim3(grid_for(n, 256)), dim3(256), 0, st, d->tok, d->pos, midlen, midoff, n, ka,
                        sid_of);
     HIPCK(hipGetLastError());
@@ -794,8 +890,8 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     const uint32_t *skeys = which ? kb : ka;
     const uint32_t *sidx = which ? ib : ia;
     if (nE > 0) {
-        hipLaunchKernelGGL(k_postings, dim3(grid_for(nE, 256)), dim3(256), 0, st, sidx, sid_of, midoff, nE, psid,
-                           pspos);
+        hipLaunchKernelGGL(k_postings, dim3(grid_for(nE, 256)), dim3(256), 0, st, sidx, sid_of, midoff, d->len, nE,
+                           post);
         HIPCK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_list_offsets, dim3(grid_for((int64_t)nE + 1, 256)), dim3(256), 0, st, skeys, nE,
@@ -813,14 +909,15 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
              *partoff = P<uint32_t>(d->b_partoff);
     hipLaunchKernelGGL(k_lmin, dim3(grid_for(d->max_len + 1, 256)), dim3(256), 0, st, lmin, d->max_len, t);
     HIPCK(hipGetLastError());
-    hipLaunchKernelGGL(k_plan, dim3(grid_for(n * kWave, 256)), dim3(256), 0, st, d->tok, d->pos, d->len, n, lmin,
-                       loff, psid, t, target, hi, nparts, P<uint32_t>(d->b_eq), ctr);
+    ENSURE(d->b_rng, (size_t)std::max<uint32_t>(nP, 1) * sizeof(uint2));
+    hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 4096)), dim3(256), 0, st,
+                       d->tok, d->pos, d->len, n, probelen, poff, lmin, loff, post, target, P<uint2>(d->b_rng), hi,
+                       nparts, P<uint32_t>(d->b_eq));
     HIPCK(hipGetLastError());
     HIPCK(exclusive_scan<uint32_t>(nparts, partoff, n, misc + 1, P<uint32_t>(d->b_scanscr), st));
     uint32_t n_items = 0;
-    unsigned long long probe_tokens = 0;
+    const unsigned long long probe_tokens = nP;
     HIPCK(hipMemcpyAsync(&n_items, misc + 1, 4, hipMemcpyDeviceToHost, st));
-    HIPCK(hipMemcpyAsync(&probe_tokens, ctr + C_PQ, 8, hipMemcpyDeviceToHost, st));
     HIPCK(hipStreamSynchronize(st));
     ENSURE(d->b_items, (size_t)std::max<uint32_t>(n_items, 1) * sizeof(Item));
     Item *items = P<Item>(d->b_items);
@@ -843,10 +940,10 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
             const uint32_t out_cap = (uint32_t)std::min<size_t>(d->b_out.cap / 12, UINT32_MAX);
             JoinArgs A;
             A.tok = d->tok; A.pos = d->pos; A.len = d->len;
-            A.loff = loff; A.psid = psid; A.pspos = pspos;
+            A.loff = loff; A.post = post;
             A.sig = d->sig; A.sigbad = d->sigbad;
             A.items = cur_items; A.n_items = cur_n; A.t = t;
-            A.bm_words = (uint32_t)(((int64_t)d->n_terms + 31) / 32 + 1);
+            A.rng = P<uint2>(d->b_rng); A.poff = poff;
             A.out = P<int32_t>(d->b_out); A.out_cap = out_cap; A.ovf = P<uint32_t>(d->b_ovf); A.ctr = ctr;
             HIPCK(hipMemsetAsync(ctr + C_ESCAN, 0, (C_COUNT - C_ESCAN) * sizeof(uint64_t), st));
             HIPCK(hipEventRecord(d->ev[4], st));
@@ -863,6 +960,7 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
             ENSURE(d->b_out, (size_t)(d->h_ctr[C_RESULTS] + d->h_ctr[C_RESULTS] / 2) * 12);
         }
         for (int k = C_ESCAN; k <= C_VERIFIED; k++) acc[k] += d->h_ctr[k];
+        for (int k = C_CYC_SETUP; k <= C_CYC_RANGES; k++) acc[k] += d->h_ctr[k];
         const uint64_t nres = d->h_ctr[C_RESULTS], novf = d->h_ctr[C_OVERFLOW];
         const size_t old = tri.size();
         tri.resize(old + 3 * (size_t)nres);
@@ -887,6 +985,7 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     stats.verify_tokens = (int64_t)acc[C_VTOK];
     stats.signature_checks = (int64_t)acc[C_SIGCHK];
     stats.verified = (int64_t)acc[C_VERIFIED];
+    for (int k = 0; k < 6; k++) stats.phase_cycles[k] = (int64_t)acc[C_CYC_SETUP + k];
     stats.results = (int64_t)(tri.size() / 3);
     float ms = 0;
     HIPCK(hipEventElapsedTime(&ms, d->ev[0], d->ev[1]));

@@ -40,7 +40,7 @@ def test_titles_4096(fg, oracle_mod, t):
     assert_join_equal(oracle_mod, sets, coll.selfjoin(t), t)
 
 
-@pytest.mark.parametrize("slots", [512, 1024, 2048, 4096])
+@pytest.mark.parametrize("slots", [256, 512, 1024, 2048, 4096])
 def test_table_sizes_and_splitting(fg, oracle_mod, slots):
     """Tiny LDS tables force source-window splits and overflow retries; results must not change."""
     _say(f"titles n=4096 seed=7 t=0.5 slots={slots}")
