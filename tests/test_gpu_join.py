@@ -101,3 +101,38 @@ def test_repeatable(fg):
     b = coll.selfjoin(0.7)
     assert np.array_equal(a.q, b.q) and np.array_equal(a.s, b.s) and np.array_equal(a.overlap, b.overlap)
     assert a.stats["candidates"] == b.stats["candidates"]
+
+
+def test_cli_outputs(fg, oracle_mod, tmp_path):
+    """The CLI's stdout lines and stderr counters, both input forms."""
+    import subprocess
+    ka = json.load(open(golden_path("small_known_answers.json")))
+    for form in ("bin", "text"):
+        for ts in ("0.25", "0.5", "0.75", "0.9"):
+            _say(f"cli {form} t={ts}")
+            if form == "bin":
+                args = [fg.CLI_PATH, golden_path("small.txt.3gram.bin"), ts]
+            else:
+                out = tmp_path / "o.bin"
+                args = [fg.CLI_PATH, "-f", golden_path("small.txt"), "-q", "3", "-t", ts, "-o", str(out)]
+            r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+            assert r.returncode == 0, r.stderr
+            assert sorted(r.stdout.splitlines()) == sorted(ka["answers"][ts])
+            assert f"# Results: {len(ka['answers'][ts])}" in r.stderr
+            assert "# Records: 5" in r.stderr and "# Terms: 42" in r.stderr and "# Tokens: 88" in r.stderr
+            if form == "text":
+                assert out.read_bytes() == open(golden_path("small.txt.3gram.bin"), "rb").read()
+    # counters against the oracle on a larger input
+    coll = fg.Collection.generate(fg.GEN_TITLES, 3000, 4)
+    path = tmp_path / "g.bin"
+    coll.write_bin(str(path))
+    sets = oracle_mod.Sets(np.fromfile(str(path), "<i4"))
+    tri, ctr = oracle_mod.selfjoin(sets, np.float32(0.7))
+    r = subprocess.run([fg.CLI_PATH, str(path), "0.7"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0
+    assert f"# Candidates: {ctr['candidates']}" in r.stderr
+    assert f"# Intersections: {ctr['intersections']}" in r.stderr
+    assert f"# Inverted Index Entries: {ctr['e_idx']}" in r.stderr
+    want = sorted("%d %d %f" % (sets.id[q], sets.id[s], oracle_mod.similarity(int(sets.len[q]), int(sets.len[s]), int(o)))
+                  for q, s, o in tri)
+    assert sorted(r.stdout.splitlines()) == want

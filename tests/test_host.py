@@ -1,0 +1,134 @@
+"""Host-side checks that need no GPU: the C ABI library, its q-gram front end,
+.bin reader/writer, generator, error paths and the CLI's argument handling."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from _util import dblp_text, golden_path
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol(fg):
+    header = open(os.path.join(ROOT, "include", "fgssjoin.h")).read()
+    declared = set(re.findall(r"\b(fgss_[a-z_0-9]+)\s*\(", header))
+    assert declared, "no declarations parsed"
+    L = fg.lib()
+    for name in sorted(declared):
+        assert hasattr(L, name), name
+    assert declared == set(fg.EXPORTS)
+
+
+def test_text_front_end_matches_fixture(fg, tmp_path):
+    coll = fg.Collection.from_text(golden_path("small.txt"), 3)
+    out, loc = tmp_path / "s.bin", tmp_path / "s.bin.loc"
+    coll.write_bin(str(out), str(loc))
+    assert out.read_bytes() == open(golden_path("small.txt.3gram.bin"), "rb").read()
+    assert loc.read_bytes() == open(golden_path("small.txt.3gram.bin.loc"), "rb").read()
+
+
+def test_bin_reader_reverses_file_order(fg):
+    coll = fg.Collection.from_bin(golden_path("small.txt.3gram.bin"))
+    ids, lens, pos, tok = coll.arrays()
+    assert ids.tolist() == [4, 2, 3, 1, 5]
+    assert lens.tolist() == [24, 19, 18, 16, 11]
+    assert pos.tolist() == [0, 24, 43, 61, 77]
+    info = coll.info
+    assert (info.n_sets, info.n_tokens, info.n_terms, info.max_len) == (5, 88, 42, 24)
+    assert info.sorted_desc == 1 and info.on_device == 0
+    assert np.array_equal(coll.bin_ints(), np.fromfile(golden_path("small.txt.3gram.bin"), "<i4"))
+
+
+@pytest.mark.parametrize("q", [2, 3, 5])
+def test_tokenizer_matches_oracle_dblp(fg, oracle_mod, q):
+    text = dblp_text()
+    b, _ = oracle_mod.tokenize(text, q)
+    coll = fg.Collection.from_text_bytes(text, q)
+    assert np.array_equal(coll.bin_ints(), b)
+
+
+@pytest.mark.parametrize("kind,n", [(0, 4096), (1, 300)])
+def test_tokenizer_matches_oracle_generated(fg, oracle_mod, kind, n):
+    text = fg.generate_text(kind, n, 9)
+    b, loc = oracle_mod.tokenize(text, 3)
+    coll = fg.Collection.from_text_bytes(text, 3)
+    assert np.array_equal(coll.bin_ints(), b)
+
+
+def test_tokenizer_long_q_and_odd_bytes(fg, oracle_mod):
+    text = b"ab\n\nA\x00B\x00CdEf\r\nzzzzzzzzzzzzzzzzzzzz\n\xff\xfe\xfd\xfc\xfb\xfa\xf9\xf8\xf7\n"
+    for q in (1, 2, 3, 9, 12):
+        b, _ = oracle_mod.tokenize(text, q)
+        coll = fg.Collection.from_text_bytes(text, q)
+        assert np.array_equal(coll.bin_ints(), b), q
+
+
+def test_generator_is_seeded_and_shaped(fg):
+    a = fg.generate_text(fg.GEN_TITLES, 5000, 2)
+    assert a == fg.generate_text(fg.GEN_TITLES, 5000, 2)
+    assert a != fg.generate_text(fg.GEN_TITLES, 5000, 3)
+    lines = a.split(b"\n")[:-1]
+    lens = np.array([len(x) for x in lines])
+    assert len(lines) == 5000
+    assert 55 < lens.mean() < 85 and lens.max() <= 125
+    assert all(re.fullmatch(rb"[a-z ]+", x) for x in lines[:500])
+    long = fg.generate_text(fg.GEN_LONG, 200, 3).split(b"\n")[:-1]
+    ll = np.array([len(x) for x in long])
+    assert 180 <= ll.min() and ll.max() <= 1215 and 550 < ll.mean() < 850
+
+
+def test_errors_are_status_codes(fg, tmp_path):
+    with pytest.raises(fg.FgssError) as e:
+        fg.Collection.from_bin(str(tmp_path / "missing.bin"))
+    assert e.value.status == fg.FGSS_ERR_IO
+    bad = tmp_path / "trunc.bin"
+    bad.write_bytes(np.array([1, 5, 1, 2], "<i4").tobytes())
+    with pytest.raises(fg.FgssError) as e:
+        fg.Collection.from_bin(str(bad))
+    assert e.value.status == fg.FGSS_ERR_FORMAT
+    with pytest.raises(fg.FgssError) as e:
+        fg.Collection.from_text_bytes(b"abc\n", 0)
+    assert e.value.status == fg.FGSS_ERR_ARG
+    coll = fg.Collection.from_bin(golden_path("small.txt.3gram.bin"))
+    for t in (0.0, 1.0, -0.5, 1.5):
+        with pytest.raises(fg.FgssError) as e:
+            coll.selfjoin(t)
+        assert e.value.status == fg.FGSS_ERR_ARG
+    # unsorted lengths and non-ascending tokens are refused before any device work
+    unsorted = fg.Collection.from_bin_ints(np.array([1, 3, 1, 2, 3, 2, 1, 5], np.int32))
+    with pytest.raises(fg.FgssError) as e:
+        unsorted.upload()
+    assert e.value.status == fg.FGSS_ERR_UNSUPPORTED
+    dup = fg.Collection.from_bin_ints(np.array([1, 3, 1, 1, 3], np.int32))
+    with pytest.raises(fg.FgssError) as e:
+        dup.upload()
+    assert e.value.status == fg.FGSS_ERR_UNSUPPORTED
+
+
+def test_trailing_partial_int_is_ignored(fg, tmp_path):
+    raw = open(golden_path("small.txt.3gram.bin"), "rb").read()
+    p = tmp_path / "tail.bin"
+    p.write_bytes(raw + b"\x01\x02")
+    assert fg.Collection.from_bin(str(p)).info.n_sets == 5
+    p.write_bytes(raw + b"\x01\x00\x00\x00\x02")   # an id without a size: read error
+    with pytest.raises(fg.FgssError):
+        fg.Collection.from_bin(str(p))
+
+
+def _cli(fg, *args):
+    return subprocess.run([fg.CLI_PATH, *args], capture_output=True, text=True, timeout=60)
+
+
+def test_cli_argument_errors(fg, tmp_path):
+    r = _cli(fg)
+    assert r.returncode == 1 and "Usage" in r.stdout
+    r = _cli(fg, str(tmp_path / "nope.bin"), "0.5")
+    assert r.returncode == 1 and "does not exist" in r.stderr
+    for t in ("0", "1", "1.5", "-2", "abc"):
+        r = _cli(fg, golden_path("small.txt.3gram.bin"), t)
+        assert r.returncode == 1 and "Threshold must be" in r.stderr, t
+    r = _cli(fg, "-f", golden_path("small.txt"), "-q", "0", "-t", "0.5")
+    assert r.returncode == 1
