@@ -25,6 +25,7 @@
 // source position) because both records share one global token order, so it can be
 // accumulated in any order with an atomic add and an atomic max.
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -50,6 +51,7 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int join_waves(int cap) { return cap >= 4096 ? 2 : 4; }
 constexpr int kTokBatch = 128;             // probe tokens whose posting ranges are staged at once
 constexpr int kQueueBatch = 16;            // work items claimed per queue atomic
+constexpr uint32_t kFence = 32;            // postings per fence block (one 128-byte line of ids)
 constexpr int kSigBuckets = 128;           // token buckets of a record signature
 constexpr int kSigWords = kSigBuckets / 4; // one byte-wide count per bucket
 
@@ -81,9 +83,14 @@ enum Ctr {
 
 struct Item {
     uint32_t q, a, b;   // query and source sub-window [a, b)
-    uint32_t pad;       // expected useful postings (sizes the LDS table) | kWholeWindow
+    uint32_t pad;       // expected useful postings (sizes the LDS table) | kWholeWindow | kPlanned
+    uint32_t part, nparts;  // planned items: part index and part count of the query
+    uint32_t bofs;          // planned items: first boundary position of the query in the bounds table
+    uint32_t spare;
 };
 constexpr uint32_t kWholeWindow = 0x80000000u;   // the item covers the query's whole window
+constexpr uint32_t kPlanned = 0x40000000u;       // part of a planned split: ranges come from the bounds table
+constexpr uint32_t kPadMask = 0x3FFFFFFFu;
 
 }  // namespace
 
@@ -95,7 +102,7 @@ struct fgss_device_sets {
     uint32_t *tok = nullptr, *len = nullptr, *pos = nullptr;
     uint32_t *sig = nullptr;    // kSigWords words per record (token-bucket byte counts)
     uint8_t *sigbad = nullptr;  // per record: a count did not fit in a byte
-    DevBuf b_probelen, b_poff, b_rng, b_eq, b_midlen, b_midoff, b_ka, b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_post, b_loff, b_lmin,
+    DevBuf b_psid, b_fence, b_bcnt, b_bofs, b_bounds, b_probelen, b_poff, b_rng, b_eq, b_midlen, b_midoff, b_ka, b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_post, b_loff, b_lmin,
         b_hi, b_nparts, b_partoff, b_scanscr, b_items, b_items2, b_ovf, b_out, b_ctr, b_misc;
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     uint64_t *h_ctr = nullptr;   // pinned host mirror of the counters
@@ -132,7 +139,7 @@ void fgss_device_release(fgss_collection *c)
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(d->device);
     if (d->st) (void)hipStreamSynchronize(d->st);
-    DevBuf *bufs[] = {&d->b_probelen, &d->b_poff, &d->b_rng, &d->b_eq, &d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib, &d->b_sortscr,
+    DevBuf *bufs[] = {&d->b_psid, &d->b_fence, &d->b_bcnt, &d->b_bofs, &d->b_bounds, &d->b_probelen, &d->b_poff, &d->b_rng, &d->b_eq, &d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib, &d->b_sortscr,
                       &d->b_sid_of, &d->b_post, &d->b_loff, &d->b_lmin, &d->b_hi, &d->b_nparts,
                       &d->b_partoff, &d->b_scanscr, &d->b_items, &d->b_items2, &d->b_ovf, &d->b_out, &d->b_ctr,
                       &d->b_misc};
@@ -287,13 +294,15 @@ __global__ void k_entries(const uint32_t *__restrict__ tok, const uint32_t *__re
 // postings in term order, packed: x = record id, y = token position | record length << 16
 __global__ void k_postings(const uint32_t *__restrict__ sidx, const uint32_t *__restrict__ sid_of,
                            const uint32_t *__restrict__ midoff, const uint32_t *__restrict__ len, uint32_t nE,
-                           uint2 *__restrict__ post)
+                           uint2 *__restrict__ post, uint32_t *__restrict__ psid, uint32_t *__restrict__ fence)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nE) return;
     uint32_t e = sidx[i];
     uint32_t s = sid_of[e];
     post[i] = make_uint2(s, (e - midoff[s]) | (len[s] << 16));
+    psid[i] = s;
+    if ((i & (kFence - 1)) == 0) fence[i / kFence] = s;
 }
 
 // loff[t] = first posting of term t; loff[n_terms] = nE
@@ -318,15 +327,31 @@ __global__ void k_lmin(uint32_t *__restrict__ lmin, int32_t max_len, float t)
     lmin[l] = ls;
 }
 
-// first index in [lo, hi) with post[index].x (record id) >= v (ascending within a list)
-__device__ __forceinline__ uint32_t lower_bound_sid(const uint2 *__restrict__ post, uint32_t lo, uint32_t hi, uint32_t v)
+// First index in [lo, hi) whose record id is >= v, within one posting list (ids
+// ascending).  Two levels: fence[f] = psid[f * kFence] for every aligned block of
+// kFence postings (small, stays in L2) narrows the search to one block, which is
+// one 128-byte line of psid.
+__device__ __forceinline__ uint32_t lower_bound_sid(const uint32_t *__restrict__ psid, const uint32_t *__restrict__ fence,
+                                                    uint32_t lo, uint32_t hi, uint32_t v)
 {
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (post[mid].x < v) lo = mid + 1;
-        else hi = mid;
+    if (lo >= hi) return lo;
+    // fence entries whose positions lie in [lo, hi)
+    uint32_t flo = (lo + kFence - 1) / kFence, fhi = (hi - 1) / kFence + 1;
+    while (flo < fhi) {                  // first fence entry with id >= v
+        uint32_t mid = (flo + fhi) >> 1;
+        if (fence[mid] < v) flo = mid + 1;
+        else fhi = mid;
     }
-    return lo;
+    // the answer lies in [max(lo, (flo-1)*kFence), min(hi, flo*kFence)]
+    uint32_t a = flo > 0 ? (flo - 1) * kFence : 0, b = flo * kFence;
+    if (a < lo) a = lo;
+    if (b > hi) b = hi;
+    while (a < b) {
+        uint32_t mid = (a + b) >> 1;
+        if (psid[mid] < v) a = mid + 1;
+        else b = mid;
+    }
+    return a;
 }
 
 // first index in [lo, hi) with a[index] >= v (a ascending)
@@ -347,9 +372,11 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
                                               const uint32_t *__restrict__ len, int64_t n,
                                               const uint32_t *__restrict__ probelen,
                                               const uint32_t *__restrict__ poff, const uint32_t *__restrict__ lmin,
-                                              const uint32_t *__restrict__ loff, const uint2 *__restrict__ post,
-                                              uint32_t target, uint2 *__restrict__ rng, uint32_t *__restrict__ hi_out,
-                                              uint32_t *__restrict__ nparts, uint32_t *__restrict__ eq_out)
+                                              const uint32_t *__restrict__ loff, const uint32_t *__restrict__ psid,
+                                              const uint32_t *__restrict__ fence, uint32_t target,
+                                              uint2 *__restrict__ rng, uint32_t *__restrict__ hi_out,
+                                              uint32_t *__restrict__ nparts, uint32_t *__restrict__ eq_out,
+                                              uint32_t *__restrict__ bcnt)
 {
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < n; q += nwaves) {
@@ -369,8 +396,8 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
         for (uint32_t j = lane_id(); j < qp; j += kWave) {
             uint32_t term = tok[pos[q] + j];
             uint32_t L0 = loff[term], L1 = loff[term + 1];
-            uint32_t a = lower_bound_sid(post, L0, L1, (uint32_t)q + 1);
-            uint32_t b = lower_bound_sid(post, a, L1, wend);
+            uint32_t a = lower_bound_sid(psid, fence, L0, L1, (uint32_t)q + 1);
+            uint32_t b = lower_bound_sid(psid, fence, a, L1, wend);
             rng[ro + j] = make_uint2(a, b);
             e += b - a;
         }
@@ -378,14 +405,49 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
         if (lane_id() == 0) {
             hi_out[q] = wend;
             eq_out[q] = e;
-            nparts[q] = e == 0 ? 0u : (e + target - 1) / target;
+            const uint32_t np = e == 0 ? 0u : (e + target - 1) / target;
+            nparts[q] = np;
+            bcnt[q] = np > 1 ? (np - 1) * qp : 0u;   // internal boundaries x probe tokens
+        }
+    }
+}
+
+// sid of the start of part p of a window [a0, a0 + W) split into np parts
+__device__ __forceinline__ uint32_t part_start(uint64_t a0, uint64_t W, uint32_t p, uint32_t np)
+{
+    return (uint32_t)(a0 + W * p / np);
+}
+
+// For every query split into np > 1 parts: the position of each internal part
+// boundary in each probe token's posting range, computed once here so that k_join
+// reads its ranges instead of searching.  A wave per query; its lanes take the
+// (boundary, token) pairs, one independent binary search each.
+__global__ __launch_bounds__(256) void k_bounds(int64_t n, const uint32_t *__restrict__ probelen,
+                                                const uint32_t *__restrict__ poff, const uint2 *__restrict__ rng,
+                                                const uint32_t *__restrict__ hi, const uint32_t *__restrict__ nparts,
+                                                const uint32_t *__restrict__ bofs, const uint32_t *__restrict__ psid,
+                                                const uint32_t *__restrict__ fence, uint32_t *__restrict__ bounds)
+{
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < n; q += nwaves) {
+        const uint32_t np = nparts[q];
+        if (np < 2) continue;
+        const uint32_t qp = probelen[q];
+        const uint64_t a0 = (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
+        uint32_t *B = bounds + bofs[q];
+        const uint2 *R = rng + poff[q];
+        const uint32_t total = (np - 1) * qp;
+        for (uint32_t k = lane_id(); k < total; k += kWave) {
+            const uint32_t p = k / qp + 1, j = k - (p - 1) * qp;
+            const uint2 r = R[j];
+            B[k] = lower_bound_sid(psid, fence, r.x, r.y, part_start(a0, W, p, np));
         }
     }
 }
 
 __global__ void k_items(const uint32_t *__restrict__ hi, const uint32_t *__restrict__ nparts,
-                        const uint32_t *__restrict__ eq, const uint32_t *__restrict__ partoff, int64_t n,
-                        Item *__restrict__ items)
+                        const uint32_t *__restrict__ eq, const uint32_t *__restrict__ partoff,
+                        const uint32_t *__restrict__ bofs, int64_t n, Item *__restrict__ items)
 {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
@@ -397,9 +459,13 @@ __global__ void k_items(const uint32_t *__restrict__ hi, const uint32_t *__restr
     for (uint32_t p = 0; p < np; p++) {
         Item it;
         it.q = (uint32_t)q;
-        it.a = (uint32_t)(a0 + W * p / np);
-        it.b = (uint32_t)(a0 + W * (p + 1) / np);
-        it.pad = per | (np == 1 ? kWholeWindow : 0u);
+        it.a = part_start(a0, W, p, np);
+        it.b = part_start(a0, W, p + 1, np);
+        it.pad = per | (np == 1 ? kWholeWindow : kPlanned);
+        it.part = p;
+        it.nparts = np;
+        it.bofs = bofs[q];
+        it.spare = 0;
         items[o + p] = it;
     }
 }
@@ -417,7 +483,11 @@ __global__ void k_resplit(const Item *__restrict__ in, const uint32_t *__restric
         o.q = it.q;
         o.a = (uint32_t)(it.a + W * p / ways);
         o.b = (uint32_t)(it.a + W * (p + 1) / ways);
-        o.pad = it.pad & ~kWholeWindow;   // same table size as the parent: four times the room per posting
+        o.pad = it.pad & kPadMask;   // same table size as the parent: four times the room per posting
+        o.part = 0;
+        o.nparts = 0;
+        o.bofs = 0;
+        o.spare = 0;
         out[i * ways + p] = o;
     }
 }
@@ -428,11 +498,14 @@ __global__ void k_resplit(const Item *__restrict__ in, const uint32_t *__restric
 struct JoinArgs {
     const uint32_t *tok, *pos, *len;
     const uint32_t *loff;
-    const uint2 *post;          // postings (record id, token position), grouped by term
+    const uint2 *post;          // postings (record id, token position | length << 16), grouped by term
+    const uint32_t *psid;       // record ids of the postings alone
+    const uint32_t *fence;      // psid[f * kFence]
     const uint32_t *sig;        // per record: kSigWords words of byte counts per token bucket
     const uint8_t *sigbad;      // 1 when a bucket count did not fit in a byte
     const uint2 *rng;           // per probe token: posting range inside the query's window
     const uint32_t *poff;       // per query: first entry of its probe tokens in rng
+    const uint32_t *bounds;     // planned split boundaries (see k_bounds)
     const Item *items;
     uint32_t n_items;
     float t;
@@ -522,12 +595,16 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
         uint32_t qp = fgss_max_prefix(lq, t);
         if (qp > lq) qp = lq;
         // table sized for twice the expected postings (64 .. CAP slots)
-        int tlog = ceil_log2(2 * (item.pad & ~kWholeWindow));
+        int tlog = ceil_log2(2 * (item.pad & kPadMask));
         tlog = tlog < 6 ? 6 : (tlog > LOG2CAP ? LOG2CAP : tlog);
         const uint32_t tsize = 1u << tlog;
 
         const bool whole = (item.pad & kWholeWindow) != 0;
+        const bool planned = (item.pad & kPlanned) != 0;
         const uint2 *qrng = A.rng + A.poff[q];
+        const uint32_t *lo_row = planned && item.part > 0 ? A.bounds + item.bofs + (size_t)(item.part - 1) * qp : nullptr;
+        const uint32_t *hi_row = planned && item.part + 1 < item.nparts ? A.bounds + item.bofs + (size_t)item.part * qp
+                                                                        : nullptr;
         c1 = clock64(); cyc[0] += c1 - c0; c0 = c1;
 
         // ---- A: candidate generation (filtering.cu:94-139) ----
@@ -539,9 +616,12 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
             for (uint32_t jj = lane; jj < nb; jj += kWave) {
                 const uint2 r = qrng[jb + jj];   // the token's postings inside the whole window
                 uint32_t lo = r.x, hi = r.y;
-                if (!whole) {                    // narrow to this item's sub-window
-                    lo = lower_bound_sid(A.post, r.x, r.y, wa);
-                    hi = lower_bound_sid(A.post, lo, r.y, wb);
+                if (planned) {                   // boundaries precomputed by k_bounds
+                    if (lo_row) lo = lo_row[jb + jj];
+                    if (hi_row) hi = hi_row[jb + jj];
+                } else if (!whole) {             // a re-split item: narrow by search
+                    lo = lower_bound_sid(A.psid, A.fence, r.x, r.y, wa);
+                    hi = lower_bound_sid(A.psid, A.fence, lo, r.y, wb);
                 }
                 rlo[jj] = lo;
                 rpre[jj] = hi - lo;
@@ -797,14 +877,20 @@ static size_t join_smem_bytes()
 template <int CAP>
 static int launch_join(fgss_device_sets *d, JoinArgs A, int *blocks_out)
 {
+    // occupancy-derived persistent grid, computed once per table size
+    static thread_local int cached_blocks[2] = {-1, -1};
     const size_t smem = join_smem_bytes<CAP>();
-    HIPCK(hipFuncSetAttribute((const void *)k_join<CAP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-    int per_cu = 0;
-    HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_join<CAP>, join_waves(CAP) * kWave, smem));
-    int cus = 0;
-    HIPCK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device));
-    if (per_cu < 1) per_cu = 1;
-    int blocks = per_cu * cus;
+    int &blocks_all = cached_blocks[0];
+    if (cached_blocks[1] != d->device || blocks_all < 0) {
+        HIPCK(hipFuncSetAttribute((const void *)k_join<CAP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+        int per_cu = 0;
+        HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_join<CAP>, join_waves(CAP) * kWave, smem));
+        int cus = 0;
+        HIPCK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device));
+        blocks_all = std::max(per_cu, 1) * cus;
+        cached_blocks[1] = d->device;
+    }
+    int blocks = blocks_all;
     const int need = (int)((A.n_items + join_waves(CAP) - 1) / join_waves(CAP));
     if (blocks > need) blocks = std::max(need, 1);
     hipLaunchKernelGGL(k_join<CAP>, dim3(blocks), dim3(join_waves(CAP) * kWave), smem, d->st, A);
@@ -830,6 +916,18 @@ static inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n +
 // ---------------------------------------------------------------------------
 // fgss_selfjoin
 // ---------------------------------------------------------------------------
+// FGSS_TRACE=1 prints a host-side timeline of each join to stderr
+static void trace_mark(const char *what)
+{
+    static const bool on = getenv("FGSS_TRACE") && getenv("FGSS_TRACE")[0] == '1';
+    static thread_local std::chrono::steady_clock::time_point last;
+    if (!on) return;
+    auto now = std::chrono::steady_clock::now();
+    if (strcmp(what, "begin") != 0)
+        fprintf(stderr, "[fgss] %-12s +%8.3f ms\n", what, std::chrono::duration<double, std::milli>(now - last).count());
+    last = now;
+}
+
 static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *opts, std::vector<int32_t> &tri,
                            fgss_join_stats &stats)
 {
@@ -841,6 +939,7 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     memset(&stats, 0, sizeof(stats));
     tri.clear();
     if (n == 0) return FGSS_OK;
+    trace_mark("begin");
 
     HIPCK(hipEventRecord(d->ev[0], st));
     ENSURE(d->b_ctr, C_COUNT * sizeof(uint64_t));
@@ -865,6 +964,7 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     HIPCK(hipMemcpyAsync(&nE, misc, 4, hipMemcpyDeviceToHost, st));
     HIPCK(hipMemcpyAsync(&nP, misc + 2, 4, hipMemcpyDeviceToHost, st));
     HIPCK(hipStreamSynchronize(st));
+    trace_mark("after_sizes");
     stats.index_entries = nE;
 
     // 3-5: entries, stable sort by term, postings and list offsets
@@ -875,6 +975,9 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     ENSURE(d->b_ib, eb);
     ENSURE(d->b_sid_of, eb);
     ENSURE(d->b_post, 2 * eb + 8);
+    ENSURE(d->b_psid, eb + 4);
+    ENSURE(d->b_fence, (size_t)(nE / kFence + 2) * 4);
+    uint32_t *psid = P<uint32_t>(d->b_psid), *fence = P<uint32_t>(d->b_fence);
     ENSURE(d->b_loff, ((size_t)d->n_terms + 2) * 4);
     ENSURE(d->b_sortscr, (size_t)radix_scratch_elems(std::max<uint32_t>(nE, 1)) * 4);
     uint32_t *ka = P<uint32_t>(d->b_ka), *kb = P<uint32_t>(d->b_kb), *ia = P<uint32_t>(d->b_ia),
@@ -891,7 +994,7 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     const uint32_t *sidx = which ? ib : ia;
     if (nE > 0) {
         hipLaunchKernelGGL(k_postings, dim3(grid_for(nE, 256)), dim3(256), 0, st, sidx, sid_of, midoff, d->len, nE,
-                           post);
+                           post, psid, fence);
         HIPCK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_list_offsets, dim3(grid_for((int64_t)nE + 1, 256)), dim3(256), 0, st, skeys, nE,
@@ -905,24 +1008,37 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     ENSURE(d->b_eq, (size_t)n * 4);
     ENSURE(d->b_nparts, (size_t)n * 4);
     ENSURE(d->b_partoff, (size_t)(n + 1) * 4);
+    ENSURE(d->b_bcnt, (size_t)n * 4);
+    ENSURE(d->b_bofs, (size_t)(n + 1) * 4);
+    uint32_t *bcnt = P<uint32_t>(d->b_bcnt), *bofs = P<uint32_t>(d->b_bofs);
     uint32_t *lmin = P<uint32_t>(d->b_lmin), *hi = P<uint32_t>(d->b_hi), *nparts = P<uint32_t>(d->b_nparts),
              *partoff = P<uint32_t>(d->b_partoff);
     hipLaunchKernelGGL(k_lmin, dim3(grid_for(d->max_len + 1, 256)), dim3(256), 0, st, lmin, d->max_len, t);
     HIPCK(hipGetLastError());
     ENSURE(d->b_rng, (size_t)std::max<uint32_t>(nP, 1) * sizeof(uint2));
     hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 4096)), dim3(256), 0, st,
-                       d->tok, d->pos, d->len, n, probelen, poff, lmin, loff, post, target, P<uint2>(d->b_rng), hi,
-                       nparts, P<uint32_t>(d->b_eq));
+                       d->tok, d->pos, d->len, n, probelen, poff, lmin, loff, psid, fence, target, P<uint2>(d->b_rng), hi,
+                       nparts, P<uint32_t>(d->b_eq), bcnt);
     HIPCK(hipGetLastError());
     HIPCK(exclusive_scan<uint32_t>(nparts, partoff, n, misc + 1, P<uint32_t>(d->b_scanscr), st));
-    uint32_t n_items = 0;
+    HIPCK(exclusive_scan<uint32_t>(bcnt, bofs, n, misc + 3, P<uint32_t>(d->b_scanscr), st));
+    uint32_t n_items = 0, nB = 0;
     const unsigned long long probe_tokens = nP;
     HIPCK(hipMemcpyAsync(&n_items, misc + 1, 4, hipMemcpyDeviceToHost, st));
+    HIPCK(hipMemcpyAsync(&nB, misc + 3, 4, hipMemcpyDeviceToHost, st));
     HIPCK(hipStreamSynchronize(st));
+    trace_mark("after_plan");
+    ENSURE(d->b_bounds, (size_t)std::max<uint32_t>(nB, 1) * 4);
+    if (nB > 0) {
+        hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 8192)), dim3(256), 0,
+                           st, n, probelen, poff, P<uint2>(d->b_rng), hi, nparts, bofs, psid, fence,
+                           P<uint32_t>(d->b_bounds));
+        HIPCK(hipGetLastError());
+    }
     ENSURE(d->b_items, (size_t)std::max<uint32_t>(n_items, 1) * sizeof(Item));
     Item *items = P<Item>(d->b_items);
-    hipLaunchKernelGGL(k_items, dim3(grid_for(n, 256)), dim3(256), 0, st, hi, nparts, P<uint32_t>(d->b_eq), partoff, n,
-                       items);
+    hipLaunchKernelGGL(k_items, dim3(grid_for(n, 256)), dim3(256), 0, st, hi, nparts, P<uint32_t>(d->b_eq), partoff, bofs,
+                       n, items);
     HIPCK(hipGetLastError());
 
     // 9: join; overflowed items are split into finer source windows and run again,
@@ -940,10 +1056,10 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
             const uint32_t out_cap = (uint32_t)std::min<size_t>(d->b_out.cap / 12, UINT32_MAX);
             JoinArgs A;
             A.tok = d->tok; A.pos = d->pos; A.len = d->len;
-            A.loff = loff; A.post = post;
+            A.loff = loff; A.post = post; A.psid = psid; A.fence = fence;
             A.sig = d->sig; A.sigbad = d->sigbad;
             A.items = cur_items; A.n_items = cur_n; A.t = t;
-            A.rng = P<uint2>(d->b_rng); A.poff = poff;
+            A.rng = P<uint2>(d->b_rng); A.poff = poff; A.bounds = P<uint32_t>(d->b_bounds);
             A.out = P<int32_t>(d->b_out); A.out_cap = out_cap; A.ovf = P<uint32_t>(d->b_ovf); A.ctr = ctr;
             HIPCK(hipMemsetAsync(ctr + C_ESCAN, 0, (C_COUNT - C_ESCAN) * sizeof(uint64_t), st));
             HIPCK(hipEventRecord(d->ev[4], st));
@@ -952,6 +1068,7 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
             HIPCK(hipEventRecord(d->ev[5], st));
             HIPCK(hipMemcpyAsync(d->h_ctr, ctr, C_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
             HIPCK(hipStreamSynchronize(st));
+            trace_mark("after_join");
             float kms = 0;
             HIPCK(hipEventElapsedTime(&kms, d->ev[4], d->ev[5]));
             kernel_ms += kms;
@@ -978,6 +1095,7 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     }
     HIPCK(hipEventRecord(d->ev[3], st));
     HIPCK(hipEventSynchronize(d->ev[3]));
+    trace_mark("end");
     stats.probe_tokens = (int64_t)probe_tokens;
     stats.postings_scanned = (int64_t)acc[C_ESCAN];
     stats.candidates = (int64_t)acc[C_CAND];
@@ -997,6 +1115,36 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     return FGSS_OK;
 }
 
+// Order of the (q, s, overlap) triples by (q, s): an LSD radix sort (11-bit digits)
+// of 64-bit keys (q, s) carried with the triple index; passes whose digit is
+// constant are skipped.  out[k] low 32 bits = index of the k-th triple.
+static void sort_pairs_by_key(const std::vector<int32_t> &tri, std::vector<uint64_t> &out)
+{
+    const size_t R = tri.size() / 3;
+    std::vector<uint64_t> key(R), idx(R), k2(R), i2(R);
+    for (size_t i = 0; i < R; i++) {
+        key[i] = ((uint64_t)(uint32_t)tri[3 * i] << 31) | (uint32_t)tri[3 * i + 1];
+        idx[i] = i;
+    }
+    for (int shift = 0; shift < 62; shift += 11) {
+        size_t cnt[2049] = {0};
+        for (size_t i = 0; i < R; i++) cnt[((key[i] >> shift) & 2047) + 1]++;
+        bool constant = false;
+        for (int d = 1; d <= 2048; d++)
+            if (cnt[d] == R) constant = true;
+        if (constant) continue;
+        for (int d = 0; d < 2048; d++) cnt[d + 1] += cnt[d];
+        for (size_t i = 0; i < R; i++) {
+            size_t at = cnt[(key[i] >> shift) & 2047]++;
+            k2[at] = key[i];
+            i2[at] = idx[i];
+        }
+        key.swap(k2);
+        idx.swap(i2);
+    }
+    out.swap(idx);
+}
+
 extern "C" int fgss_selfjoin(fgss_collection *c, float threshold, const fgss_join_opts *opts, fgss_pairs **out)
 {
     if (!c || !out) return fgss_fail(FGSS_ERR_ARG, "null argument");
@@ -1014,12 +1162,8 @@ extern "C" int fgss_selfjoin(fgss_collection *c, float threshold, const fgss_joi
     auto t2 = std::chrono::steady_clock::now();
     const int64_t R = (int64_t)(tri.size() / 3);
     // sort by (q, s); the device emits in completion order
-    std::vector<int64_t> order((size_t)R);
-    for (int64_t i = 0; i < R; i++) order[(size_t)i] = i;
-    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
-        if (tri[3 * a] != tri[3 * b]) return tri[3 * a] < tri[3 * b];
-        return tri[3 * a + 1] < tri[3 * b + 1];
-    });
+    std::vector<uint64_t> order;
+    sort_pairs_by_key(tri, order);
     auto *p = (fgss_pairs *)calloc(1, sizeof(fgss_pairs));
     const size_t nb = (size_t)std::max<int64_t>(R, 1);
     p->q = (int32_t *)malloc(nb * 4);
@@ -1033,7 +1177,7 @@ extern "C" int fgss_selfjoin(fgss_collection *c, float threshold, const fgss_joi
         return fgss_fail(FGSS_ERR_NOMEM, "out of host memory");
     }
     for (int64_t k = 0; k < R; k++) {
-        int64_t i = order[(size_t)k];
+        int64_t i = (int64_t)(order[(size_t)k] & 0xFFFFFFFFull);
         int32_t q = tri[3 * i], s = tri[3 * i + 1], ov = tri[3 * i + 2];
         p->q[k] = q;
         p->s[k] = s;
@@ -1043,6 +1187,7 @@ extern "C" int fgss_selfjoin(fgss_collection *c, float threshold, const fgss_joi
         p->similarity[k] = fgss_similarity((uint32_t)c->len[(size_t)q], (uint32_t)c->len[(size_t)s], (uint32_t)ov);
     }
     p->n_pairs = R;
+    trace_mark("host_pairs");
     auto t3 = std::chrono::steady_clock::now();
     stats.ms_copy = std::chrono::duration<double, std::milli>(t3 - t2).count();
     stats.ms_total = std::chrono::duration<double, std::milli>(t3 - t1).count();
