@@ -32,11 +32,11 @@ def log(msg):
 
 
 def kernel_bytes(st):
-    """Algorithmic bytes of the join kernel for one join:
-    4 B per probe token + 8 B per useful posting (record id, position) + 128 B per
-    signature tested + 4 B per verified source token + 12 B per result triple."""
-    return (4 * st["probe_tokens"] + 8 * st["postings_scanned"] + 128 * st["signature_checks"]
-            + 4 * st["verify_tokens"] + 12 * st["results"])
+    """Bytes the join kernel's algorithm must move for one join: 4 B per probe token,
+    8 B per useful posting (record id, position|length), 64 B per 64-bucket signature
+    test, 128 B per 128-bucket test, 4 B per verified source token, 12 B per result."""
+    return (4 * st["probe_tokens"] + 8 * st["postings_scanned"] + 64 * st["signature_checks"]
+            + 128 * st["signature_checks_128"] + 4 * st["verify_tokens"] + 12 * st["results"])
 
 
 def cpu_baseline(coll, gpu_pairs):
@@ -69,6 +69,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--table-slots", type=int, default=0)
+    ap.add_argument("--load-pct", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the join does not shard across GPUs "
@@ -89,7 +90,7 @@ def main():
     coll.upload()
 
     def step():
-        return [coll.selfjoin(t, table_slots=args.table_slots) for t in THRESHOLDS]
+        return [coll.selfjoin(t, table_slots=args.table_slots, load_pct=args.load_pct) for t in THRESHOLDS]
 
     for i in range(args.warmup):
         step()

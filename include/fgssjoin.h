@@ -108,7 +108,8 @@ typedef struct {
     double ms_total;           /* whole call                                          */
     double ms_kernel;          /* device time of the join kernel launches (HIP events) */
     int64_t kernel_launches;   /* join kernel launches (1 unless work items were re-split) */
-    int64_t signature_checks;  /* candidates tested against the bucket-count overlap bound */
+    int64_t signature_checks;  /* candidates tested against the 64-bucket overlap bound */
+    int64_t signature_checks_128; /* of those, tested again with 128 buckets           */
     int64_t verified;          /* candidates whose tokens were merged                  */
     int64_t phase_cycles[6];   /* join kernel shader cycles summed over wavefronts: item setup,
                                   candidate generation, positional filter, signature bound,
@@ -117,8 +118,8 @@ typedef struct {
 } fgss_join_stats;
 
 typedef struct {
-    int32_t query_chunk;       /* queries per work item, 0 = library default          */
-    int32_t table_slots;       /* LDS candidate table slots per work item, 0 = default */
+    int32_t load_pct;          /* postings per work item as % of its table slots, 0 = default (50) */
+    int32_t table_slots;       /* LDS candidate table slots per work item: 256..4096, 0 = default (512) */
     int32_t verify_only;       /* reserved, must be 0                                 */
     int32_t reserved;
 } fgss_join_opts;

@@ -59,7 +59,7 @@ class JoinStats(C.Structure):
                 ("verify_tokens", C.c_int64), ("ms_index", C.c_double), ("ms_join", C.c_double),
                 ("ms_copy", C.c_double), ("ms_total", C.c_double),
                 ("ms_kernel", C.c_double), ("kernel_launches", C.c_int64),
-                ("signature_checks", C.c_int64), ("verified", C.c_int64),
+                ("signature_checks", C.c_int64), ("signature_checks_128", C.c_int64), ("verified", C.c_int64),
                 ("phase_cycles", C.c_int64 * 6)]
 
     def as_dict(self):
@@ -69,7 +69,7 @@ class JoinStats(C.Structure):
 
 
 class JoinOpts(C.Structure):
-    _fields_ = [("query_chunk", C.c_int32), ("table_slots", C.c_int32), ("verify_only", C.c_int32),
+    _fields_ = [("load_pct", C.c_int32), ("table_slots", C.c_int32), ("verify_only", C.c_int32),
                 ("reserved", C.c_int32)]
 
 
@@ -257,9 +257,9 @@ class Collection:
     def upload(self):
         _check(lib().fgss_collection_upload(self._h))
 
-    def selfjoin(self, threshold: float, table_slots: int = 0) -> Pairs:
+    def selfjoin(self, threshold: float, table_slots: int = 0, load_pct: int = 0) -> Pairs:
         L = lib()
-        opts = JoinOpts(0, table_slots, 0, 0)
+        opts = JoinOpts(load_pct, table_slots, 0, 0)
         out = C.POINTER(_Pairs)()
         _check(L.fgss_selfjoin(self._h, C.c_float(threshold), C.byref(opts), C.byref(out)))
         try:

@@ -54,6 +54,7 @@ constexpr int kQueueBatch = 16;            // work items claimed per queue atomi
 constexpr uint32_t kFence = 32;            // postings per fence block (one 128-byte line of ids)
 constexpr int kSigBuckets = 128;           // token buckets of a record signature
 constexpr int kSigWords = kSigBuckets / 4; // one byte-wide count per bucket
+constexpr int kSig64Words = 16;            // the same counts merged pairwise into 64 buckets
 
 struct DevBuf {
     void *p = nullptr;
@@ -69,6 +70,7 @@ enum Ctr {
     C_VTOK,
     C_SIGCHK,        // candidates tested against the signature bound
     C_VERIFIED,      // candidates whose tokens were merged
+    C_SIG2,          // candidates that needed the 128-bucket signature
     C_RESULTS,       // pairs emitted (may exceed the output capacity)
     C_OVERFLOW,      // work items whose LDS table overflowed
     C_QUEUE,         // work-queue head
@@ -101,6 +103,7 @@ struct fgss_device_sets {
     int32_t n_terms = 0, max_len = 0;
     uint32_t *tok = nullptr, *len = nullptr, *pos = nullptr;
     uint32_t *sig = nullptr;    // kSigWords words per record (token-bucket byte counts)
+    uint32_t *sig64 = nullptr;  // kSig64Words words per record (64-bucket byte counts)
     uint8_t *sigbad = nullptr;  // per record: a count did not fit in a byte
     DevBuf b_psid, b_fence, b_bcnt, b_bofs, b_bounds, b_probelen, b_poff, b_rng, b_eq, b_midlen, b_midoff, b_ka, b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_post, b_loff, b_lmin,
         b_hi, b_nparts, b_partoff, b_scanscr, b_items, b_items2, b_ovf, b_out, b_ctr, b_misc;
@@ -149,6 +152,7 @@ void fgss_device_release(fgss_collection *c)
     if (d->len) (void)hipFree(d->len);
     if (d->pos) (void)hipFree(d->pos);
     if (d->sig) (void)hipFree(d->sig);
+    if (d->sig64) (void)hipFree(d->sig64);
     if (d->sigbad) (void)hipFree(d->sigbad);
     for (auto &e : d->ev)
         if (e) (void)hipEventDestroy(e);
@@ -169,7 +173,8 @@ __device__ __forceinline__ uint32_t sig_bucket(uint32_t tok) { return (tok * 0x9
 
 __global__ __launch_bounds__(256) void k_signature(const uint32_t *__restrict__ tok, const uint32_t *__restrict__ pos,
                                                    const uint32_t *__restrict__ len, int64_t n,
-                                                   uint32_t *__restrict__ sig, uint8_t *__restrict__ sigbad)
+                                                   uint32_t *__restrict__ sig, uint32_t *__restrict__ sig64,
+                                                   uint8_t *__restrict__ sigbad)
 {
     __shared__ uint32_t hist[256 / kWave][kSigBuckets];
     const int w = (int)threadIdx.x / kWave;
@@ -187,8 +192,14 @@ __global__ __launch_bounds__(256) void k_signature(const uint32_t *__restrict__ 
         bool bad = false;
         if (lane < kSigWords) {
             uint32_t c0 = h[4 * lane], c1 = h[4 * lane + 1], c2 = h[4 * lane + 2], c3 = h[4 * lane + 3];
-            bad = (c0 | c1 | c2 | c3) > 255u;
             sig[(size_t)r * kSigWords + lane] = (c0 & 255u) | (c1 & 255u) << 8 | (c2 & 255u) << 16 | (c3 & 255u) << 24;
+        }
+        if (lane < kSig64Words) {   // bucket b of 64 = buckets 2b, 2b+1 of 128; every count <= its merge
+            uint32_t m[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) m[k] = h[8 * lane + 2 * k] + h[8 * lane + 2 * k + 1];
+            bad = (m[0] | m[1] | m[2] | m[3]) > 255u;
+            sig64[(size_t)r * kSig64Words + lane] = m[0] | m[1] << 8 | m[2] << 16 | m[3] << 24;
         }
         const uint64_t anybad = __ballot(bad);
         if (lane == 0) sigbad[r] = anybad ? 1 : 0;
@@ -240,6 +251,7 @@ extern "C" int fgss_collection_upload(fgss_collection *c)
     const size_t nb = (size_t)std::max<int64_t>(d->n, 1) * 4, tb = (size_t)std::max<int64_t>(d->ntok, 1) * 4;
     if (hipMalloc(&d->tok, tb) != hipSuccess || hipMalloc(&d->len, nb) != hipSuccess ||
         hipMalloc(&d->pos, nb) != hipSuccess || hipMalloc(&d->sig, nb * kSigWords) != hipSuccess ||
+        hipMalloc(&d->sig64, nb * kSig64Words) != hipSuccess ||
         hipMalloc(&d->sigbad, nb) != hipSuccess) {
         fgss_device_release(c);
         return fgss_fail(FGSS_ERR_NOMEM, "device allocation for the collection failed");
@@ -252,7 +264,7 @@ extern "C" int fgss_collection_upload(fgss_collection *c)
     if (d->n > 0) {
         const int64_t blocks = std::min<int64_t>((d->n + 3) / 4, 65536);
         hipLaunchKernelGGL(k_signature, dim3((unsigned)blocks), dim3(256), 0, d->st, d->tok, d->pos, d->len, d->n,
-                           d->sig, d->sigbad);
+                           d->sig, d->sig64, d->sigbad);
         HIPCK(hipGetLastError());
     }
     HIPCK(hipStreamSynchronize(d->st));
@@ -502,6 +514,7 @@ struct JoinArgs {
     const uint32_t *psid;       // record ids of the postings alone
     const uint32_t *fence;      // psid[f * kFence]
     const uint32_t *sig;        // per record: kSigWords words of byte counts per token bucket
+    const uint32_t *sig64;      // per record: kSig64Words words, 64 merged buckets
     const uint8_t *sigbad;      // 1 when a bucket count did not fit in a byte
     const uint2 *rng;           // per probe token: posting range inside the query's window
     const uint32_t *poff;       // per query: first entry of its probe tokens in rng
@@ -573,7 +586,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
     for (uint32_t i = lane; i < CAP; i += kWave) { tkey[i] = kEmpty; tcnt[i] = 0; tlast[i] = 0; }
     wave_sync();
 
-    unsigned long long s_escan = 0, s_cand = 0, s_sig = 0, s_ver = 0, s_inter = 0, s_vtok = 0;
+    unsigned long long s_escan = 0, s_cand = 0, s_sig = 0, s_ver = 0, s_inter = 0, s_vtok = 0, s_sig2 = 0;
     unsigned long long cyc[6] = {0, 0, 0, 0, 0, 0};
     unsigned long long c0 = clock64(), c1;
     // items are claimed kQueueBatch at a time: one global atomic per batch
@@ -749,23 +762,44 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
         wave_sync();
         c1 = clock64(); cyc[2] += c1 - c0; c0 = c1;
 
-        // ---- C: signature bound, one lane per candidate ----
+        // ---- C: signature bound, one lane per candidate: 64 buckets, then 128 for the rest ----
         const uint32_t nsig = ncand;
         uint32_t nver = 0;
         {
             const uint4 *qs4 = (const uint4 *)(A.sig + (size_t)q * kSigWords);
+            const uint4 *qh4 = (const uint4 *)(A.sig64 + (size_t)q * kSig64Words);
             const bool qbad = A.sigbad[q] != 0;
             for (uint32_t base = 0; base < nsig; base += kWave) {
                 const uint32_t c = base + lane;
-                uint32_t ent = 0;
-                bool keep = false;
+                uint32_t ent = 0, h = 0, s = 0, ls = 0;
+                int floorb = 0;
+                bool keep = false, exact = false;
                 if (c < nsig) {
                     ent = clist[c];
-                    const uint32_t h = ent & 0xFFFFu, ls = ent >> 16;
-                    const uint32_t s = tkey[h];
+                    h = ent & 0xFFFFu;
+                    ls = ent >> 16;
+                    s = tkey[h];
                     if (qbad || A.sigbad[s]) {
                         keep = true;
                     } else {
+                        floorb = (int)floorf(fgss_min_overlap(lq, ls, t));
+                        const uint4 *sh4 = (const uint4 *)(A.sig64 + (size_t)s * kSig64Words);
+                        uint32_t sad = 0;
+#pragma unroll
+                        for (int k = 0; k < kSig64Words / 4; k++) {
+                            const uint4 a4 = qh4[k], b4 = sh4[k];
+                            sad = __builtin_amdgcn_sad_u8(a4.x, b4.x, sad);
+                            sad = __builtin_amdgcn_sad_u8(a4.y, b4.y, sad);
+                            sad = __builtin_amdgcn_sad_u8(a4.z, b4.z, sad);
+                            sad = __builtin_amdgcn_sad_u8(a4.w, b4.w, sad);
+                        }
+                        // sum over buckets of min(cQ, cS) = (|Q| + |S| - SAD) / 2 >= |Q n S|
+                        exact = ((int)(lq + ls) - (int)sad) / 2 >= floorb;
+                    }
+                }
+                if (__ballot(exact)) {
+                    s_sig2 += (unsigned long long)__popcll(__ballot(exact));
+                    if (exact) {
                         const uint4 *ss4 = (const uint4 *)(A.sig + (size_t)s * kSigWords);
                         uint32_t sad = 0;
 #pragma unroll
@@ -776,12 +810,10 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
                             sad = __builtin_amdgcn_sad_u8(a4.z, b4.z, sad);
                             sad = __builtin_amdgcn_sad_u8(a4.w, b4.w, sad);
                         }
-                        // sum over buckets of min(cQ, cS) = (|Q| + |S| - SAD) / 2 >= |Q n S|
-                        const int ub = ((int)(lq + ls) - (int)sad) / 2;
-                        keep = ub >= (int)floorf(fgss_min_overlap(lq, ls, t));
+                        keep = ((int)(lq + ls) - (int)sad) / 2 >= floorb;
                     }
-                    if (!keep) { tkey[h] = kEmpty; tcnt[h] = 0; tlast[h] = 0; }
                 }
+                if (c < nsig && !keep) { tkey[h] = kEmpty; tcnt[h] = 0; tlast[h] = 0; }
                 const uint64_t bal = __ballot(keep);
                 wave_sync();
                 if (keep) clist[nver + (uint32_t)__popcll(bal & lanemask_lt())] = ent;
@@ -862,6 +894,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
         atomicAdd(&A.ctr[C_CAND], s_cand);
         atomicAdd(&A.ctr[C_SIGCHK], s_sig);
         atomicAdd(&A.ctr[C_VERIFIED], s_ver);
+        atomicAdd(&A.ctr[C_SIG2], s_sig2);
         atomicAdd(&A.ctr[C_INTER], s_inter);
         atomicAdd(&A.ctr[C_VTOK], s_vtok);
         for (int k = 0; k < 6; k++) atomicAdd(&A.ctr[C_CYC_SETUP + k], cyc[k]);
@@ -935,7 +968,8 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     hipStream_t st = d->st;
     const int64_t n = d->n;
     int cap = opts && opts->table_slots ? opts->table_slots : 512;
-    uint32_t target = (uint32_t)(cap / 2);
+    const int load = opts && opts->load_pct > 0 ? std::min(opts->load_pct, 100) : 65;
+    uint32_t target = std::max<uint32_t>(1u, (uint32_t)(cap * load / 100));
     memset(&stats, 0, sizeof(stats));
     tri.clear();
     if (n == 0) return FGSS_OK;
@@ -1057,7 +1091,7 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
             JoinArgs A;
             A.tok = d->tok; A.pos = d->pos; A.len = d->len;
             A.loff = loff; A.post = post; A.psid = psid; A.fence = fence;
-            A.sig = d->sig; A.sigbad = d->sigbad;
+            A.sig = d->sig; A.sig64 = d->sig64; A.sigbad = d->sigbad;
             A.items = cur_items; A.n_items = cur_n; A.t = t;
             A.rng = P<uint2>(d->b_rng); A.poff = poff; A.bounds = P<uint32_t>(d->b_bounds);
             A.out = P<int32_t>(d->b_out); A.out_cap = out_cap; A.ovf = P<uint32_t>(d->b_ovf); A.ctr = ctr;
@@ -1076,7 +1110,7 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
             if (d->h_ctr[C_RESULTS] <= out_cap) break;
             ENSURE(d->b_out, (size_t)(d->h_ctr[C_RESULTS] + d->h_ctr[C_RESULTS] / 2) * 12);
         }
-        for (int k = C_ESCAN; k <= C_VERIFIED; k++) acc[k] += d->h_ctr[k];
+        for (int k = C_ESCAN; k <= C_SIG2; k++) acc[k] += d->h_ctr[k];
         for (int k = C_CYC_SETUP; k <= C_CYC_RANGES; k++) acc[k] += d->h_ctr[k];
         const uint64_t nres = d->h_ctr[C_RESULTS], novf = d->h_ctr[C_OVERFLOW];
         const size_t old = tri.size();
@@ -1103,6 +1137,7 @@ static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *op
     stats.verify_tokens = (int64_t)acc[C_VTOK];
     stats.signature_checks = (int64_t)acc[C_SIGCHK];
     stats.verified = (int64_t)acc[C_VERIFIED];
+    stats.signature_checks_128 = (int64_t)acc[C_SIG2];
     for (int k = 0; k < 6; k++) stats.phase_cycles[k] = (int64_t)acc[C_CYC_SETUP + k];
     stats.results = (int64_t)(tri.size() / 3);
     float ms = 0;
