@@ -148,6 +148,39 @@ void fgss_pairs_free(fgss_pairs *p);
 int fgss_pairs_print(const fgss_pairs *p, void *file /* FILE* */);
 
 /* ------------------------------------------------------------------ */
+/* Batches: many independent self-joins in one pipeline run            */
+/* (no reference counterpart; the reference runs one join per process) */
+/* ------------------------------------------------------------------ */
+typedef struct fgss_batch fgss_batch;
+
+typedef struct {
+    double ms_total;          /* whole fgss_batch_run call                          */
+    double ms_pipeline;       /* device pipeline including its final synchronization */
+    int64_t n_pairs;          /* result pairs over all jobs                          */
+    int64_t candidates;       /* "# Candidates" summed over all jobs                 */
+    int32_t graph;            /* 1 when the pipeline ran as a replayed HIP graph     */
+    int32_t kernel_launches;  /* join kernel launches (1 unless items were re-split) */
+} fgss_batch_stats;
+
+/* Create a batch of njobs self-joins: job j joins colls[j] at thresholds[j].
+ * The collections are copied (concatenated, each keeping a private token range)
+ * and uploaded once; the caller keeps ownership of colls and must keep them alive
+ * while it reads results (record ids come from them). */
+int fgss_batch_create(fgss_collection *const *colls, const float *thresholds, int32_t njobs, fgss_batch **out);
+/* Change the jobs' thresholds for the next run (a captured graph stays valid). */
+int fgss_batch_set_thresholds(fgss_batch *b, const float *thresholds);
+/* Run every job: one launch of each pipeline stage for the whole batch.  With
+ * use_graph the stages are captured once into a HIP graph and replayed. */
+int fgss_batch_run(fgss_batch *b, const fgss_join_opts *opts, int32_t use_graph, fgss_batch_stats *stats);
+/* Pairs of one job from the last run, exactly as fgss_selfjoin would return them
+ * (sorted, indices and ids of that job's collection). */
+int fgss_batch_pairs(fgss_batch *b, int32_t job, fgss_pairs **out);
+/* Autotuner: time the batch for each LDS table size x work-item load and return
+ * the fastest options (graph replays, median of reps). */
+int fgss_batch_autotune(fgss_batch *b, int32_t reps, fgss_join_opts *best, double *best_ms);
+void fgss_batch_free(fgss_batch *b);
+
+/* ------------------------------------------------------------------ */
 /* Synthetic workloads (no reference counterpart; used by the bench)  */
 /* ------------------------------------------------------------------ */
 typedef enum {

@@ -38,6 +38,8 @@ EXPORTS = (
     "fgss_collection_upload", "fgss_collection_free", "fgss_device_synchronize",
     "fgss_selfjoin", "fgss_pairs_free", "fgss_pairs_print",
     "fgss_generate_text", "fgss_free",
+    "fgss_batch_create", "fgss_batch_set_thresholds", "fgss_batch_run", "fgss_batch_pairs",
+    "fgss_batch_autotune", "fgss_batch_free",
 )
 
 
@@ -71,6 +73,14 @@ class JoinStats(C.Structure):
 class JoinOpts(C.Structure):
     _fields_ = [("load_pct", C.c_int32), ("table_slots", C.c_int32), ("verify_only", C.c_int32),
                 ("reserved", C.c_int32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("ms_total", C.c_double), ("ms_pipeline", C.c_double), ("n_pairs", C.c_int64),
+                ("candidates", C.c_int64), ("graph", C.c_int32), ("kernel_launches", C.c_int32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 class _Pairs(C.Structure):
@@ -108,6 +118,11 @@ def lib():
         "fgss_selfjoin": [vp, C.c_float, C.POINTER(JoinOpts), C.POINTER(C.POINTER(_Pairs))],
         "fgss_pairs_print": [C.POINTER(_Pairs), vp],
         "fgss_generate_text": [C.c_int, C.c_int64, C.c_uint64, C.POINTER(C.c_void_p), i64p],
+        "fgss_batch_create": [C.POINTER(vp), C.POINTER(C.c_float), C.c_int32, C.POINTER(vp)],
+        "fgss_batch_set_thresholds": [vp, C.POINTER(C.c_float)],
+        "fgss_batch_run": [vp, C.POINTER(JoinOpts), C.c_int32, C.POINTER(BatchStats)],
+        "fgss_batch_pairs": [vp, C.c_int32, C.POINTER(C.POINTER(_Pairs))],
+        "fgss_batch_autotune": [vp, C.c_int32, C.POINTER(JoinOpts), C.POINTER(C.c_double)],
     }.items():
         f = getattr(L, name)
         f.argtypes = args
@@ -118,6 +133,8 @@ def lib():
     L.fgss_pairs_free.restype = None
     L.fgss_free.argtypes = [vp]
     L.fgss_free.restype = None
+    L.fgss_batch_free.argtypes = [vp]
+    L.fgss_batch_free.restype = None
     _lib = L
     return L
 
@@ -262,16 +279,68 @@ class Collection:
         opts = JoinOpts(load_pct, table_slots, 0, 0)
         out = C.POINTER(_Pairs)()
         _check(L.fgss_selfjoin(self._h, C.c_float(threshold), C.byref(opts), C.byref(out)))
-        try:
-            p = out.contents
-            n = p.n_pairs
+        return _take_pairs(out)
 
-            def cp(ptr, dt):
-                return np.ctypeslib.as_array(ptr, shape=(n,)).copy() if n > 0 else np.zeros(0, dt)
-            return Pairs(cp(p.q, np.int32), cp(p.s, np.int32), cp(p.rid, np.int32), cp(p.sid, np.int32),
-                         cp(p.overlap, np.int32), cp(p.similarity, np.float32), p.stats.as_dict())
-        finally:
-            L.fgss_pairs_free(out)
+
+def _take_pairs(out) -> Pairs:
+    try:
+        p = out.contents
+        n = p.n_pairs
+
+        def cp(ptr, dt):
+            return np.ctypeslib.as_array(ptr, shape=(n,)).copy() if n > 0 else np.zeros(0, dt)
+        return Pairs(cp(p.q, np.int32), cp(p.s, np.int32), cp(p.rid, np.int32), cp(p.sid, np.int32),
+                     cp(p.overlap, np.int32), cp(p.similarity, np.float32), p.stats.as_dict())
+    finally:
+        lib().fgss_pairs_free(out)
+
+
+class Batch:
+    """Many independent self-joins (own collection, own threshold) run as one pipeline.
+
+    The collections must stay alive while the batch is used (record ids come from them)."""
+
+    def __init__(self, collections, thresholds):
+        self.collections = list(collections)
+        n = len(self.collections)
+        hs = (C.c_void_p * n)(*[c._h.value for c in self.collections])
+        ts = (C.c_float * n)(*[float(t) for t in thresholds])
+        h = C.c_void_p()
+        _check(lib().fgss_batch_create(hs, ts, n, C.byref(h)))
+        self._h = h
+        self.njobs = n
+
+    def set_thresholds(self, thresholds):
+        ts = (C.c_float * self.njobs)(*[float(t) for t in thresholds])
+        _check(lib().fgss_batch_set_thresholds(self._h, ts))
+
+    def run(self, graph: bool = True, table_slots: int = 0, load_pct: int = 0) -> dict:
+        st = BatchStats()
+        opts = JoinOpts(load_pct, table_slots, 0, 0)
+        _check(lib().fgss_batch_run(self._h, C.byref(opts), 1 if graph else 0, C.byref(st)))
+        return st.as_dict()
+
+    def pairs(self, job: int) -> Pairs:
+        out = C.POINTER(_Pairs)()
+        _check(lib().fgss_batch_pairs(self._h, job, C.byref(out)))
+        return _take_pairs(out)
+
+    def autotune(self, reps: int = 3):
+        best = JoinOpts()
+        ms = C.c_double()
+        _check(lib().fgss_batch_autotune(self._h, reps, C.byref(best), C.byref(ms)))
+        return {"table_slots": best.table_slots, "load_pct": best.load_pct, "ms": ms.value}
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().fgss_batch_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 lib()

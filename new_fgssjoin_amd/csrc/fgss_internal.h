@@ -21,6 +21,8 @@ struct fgss_collection {
     std::vector<int64_t> pos;       // exclusive scan of len
     std::vector<int32_t> tok;       // concatenated records, each ascending
     std::vector<int32_t> loc;       // q-gram offsets (text front end only), parallel to tok
+    std::vector<uint32_t> job;      // batches only: job id per record
+    std::vector<uint32_t> jend;     // batches only: per job, one past its last record
     int32_t n_terms = 0;
     int32_t max_len = 0;
     bool sorted_desc = true;

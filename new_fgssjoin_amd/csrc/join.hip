@@ -51,6 +51,7 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr int join_waves(int cap) { return cap >= 4096 ? 2 : 4; }
 constexpr int kTokBatch = 128;             // probe tokens whose posting ranges are staged at once
 constexpr int kQueueBatch = 16;            // work items claimed per queue atomic
+constexpr int kJobCtr = 4;                 // per-job counters: postings, candidates, intersections, results
 constexpr uint32_t kFence = 32;            // postings per fence block (one 128-byte line of ids)
 constexpr int kSigBuckets = 128;           // token buckets of a record signature
 constexpr int kSigWords = kSigBuckets / 4; // one byte-wide count per bucket
@@ -88,7 +89,9 @@ struct Item {
     uint32_t pad;       // expected useful postings (sizes the LDS table) | kWholeWindow | kPlanned
     uint32_t part, nparts;  // planned items: part index and part count of the query
     uint32_t bofs;          // planned items: first boundary position of the query in the bounds table
-    uint32_t spare;
+    uint32_t job;           // the query's job
+    float t;                // its threshold
+    uint32_t lq, qpos, qpoff;   // the query's length, token offset and probe-range offset
 };
 constexpr uint32_t kWholeWindow = 0x80000000u;   // the item covers the query's whole window
 constexpr uint32_t kPlanned = 0x40000000u;       // part of a planned split: ranges come from the bounds table
@@ -96,19 +99,34 @@ constexpr uint32_t kPadMask = 0x3FFFFFFFu;
 
 }  // namespace
 
+// Records resident on one device: one collection, or the concatenation of the
+// collections of a batch (each record carries the id of its job; a job's records
+// are contiguous and end at jend[job]).
 struct fgss_device_sets {
     int device = 0;
     hipStream_t st = nullptr;
     int64_t n = 0, ntok = 0;
-    int32_t n_terms = 0, max_len = 0;
+    uint32_t n_terms = 0, njobs = 1;
+    int32_t max_len = 0;
     uint32_t *tok = nullptr, *len = nullptr, *pos = nullptr;
     uint32_t *sig = nullptr;    // kSigWords words per record (token-bucket byte counts)
     uint32_t *sig64 = nullptr;  // kSig64Words words per record (64-bucket byte counts)
     uint8_t *sigbad = nullptr;  // per record: a count did not fit in a byte
-    DevBuf b_psid, b_fence, b_bcnt, b_bofs, b_bounds, b_probelen, b_poff, b_rng, b_eq, b_midlen, b_midoff, b_ka, b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_post, b_loff, b_lmin,
-        b_hi, b_nparts, b_partoff, b_scanscr, b_items, b_items2, b_ovf, b_out, b_ctr, b_misc;
+    uint32_t *job = nullptr;    // per record: job id
+    uint32_t *jend = nullptr;   // per job: one past its last record
+    float *tj = nullptr;        // per job: threshold of the current run
+    DevBuf b_psid, b_fence, b_bcnt, b_bofs, b_bounds, b_probelen, b_poff, b_rng, b_eq, b_midlen, b_midoff, b_ka,
+        b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_post, b_loff, b_hi, b_nparts, b_partoff, b_scanscr, b_items,
+        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc;
+    uint32_t cap_items = 0, cap_bounds = 0, cap_out = 0;   // current capacities (elements)
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    uint64_t *h_ctr = nullptr;   // pinned host mirror of the counters
+    uint64_t *h_ctr = nullptr;    // pinned mirror of the counters
+    uint32_t *h_misc = nullptr;   // pinned mirror of the device totals
+    uint64_t *h_jctr = nullptr;   // pinned mirror of the per-job counters
+    hipGraph_t graph = nullptr;   // captured pipeline (batch runs)
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_key[3] = {0, 0, 0}; // table slots, load, capacity generation it was captured with
+    int cap_gen = 0;              // bumped whenever a buffer the pipeline uses is reallocated
 };
 
 static int ensure(DevBuf &b, size_t bytes)
@@ -142,21 +160,23 @@ void fgss_device_release(fgss_collection *c)
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(d->device);
     if (d->st) (void)hipStreamSynchronize(d->st);
-    DevBuf *bufs[] = {&d->b_psid, &d->b_fence, &d->b_bcnt, &d->b_bofs, &d->b_bounds, &d->b_probelen, &d->b_poff, &d->b_rng, &d->b_eq, &d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib, &d->b_sortscr,
-                      &d->b_sid_of, &d->b_post, &d->b_loff, &d->b_lmin, &d->b_hi, &d->b_nparts,
-                      &d->b_partoff, &d->b_scanscr, &d->b_items, &d->b_items2, &d->b_ovf, &d->b_out, &d->b_ctr,
+    if (d->graph_exec) (void)hipGraphExecDestroy(d->graph_exec);
+    if (d->graph) (void)hipGraphDestroy(d->graph);
+    DevBuf *bufs[] = {&d->b_psid, &d->b_fence, &d->b_bcnt, &d->b_bofs, &d->b_bounds, &d->b_probelen, &d->b_poff,
+                      &d->b_rng, &d->b_eq, &d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib,
+                      &d->b_sortscr, &d->b_sid_of, &d->b_post, &d->b_loff, &d->b_hi, &d->b_nparts, &d->b_partoff,
+                      &d->b_scanscr, &d->b_items, &d->b_items2, &d->b_ovf, &d->b_out, &d->b_ctr, &d->b_jctr,
                       &d->b_misc};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
-    if (d->tok) (void)hipFree(d->tok);
-    if (d->len) (void)hipFree(d->len);
-    if (d->pos) (void)hipFree(d->pos);
-    if (d->sig) (void)hipFree(d->sig);
-    if (d->sig64) (void)hipFree(d->sig64);
-    if (d->sigbad) (void)hipFree(d->sigbad);
+    void *arrs[] = {d->tok, d->len, d->pos, d->sig, d->sig64, d->sigbad, d->job, d->jend, d->tj};
+    for (void *a : arrs)
+        if (a) (void)hipFree(a);
     for (auto &e : d->ev)
         if (e) (void)hipEventDestroy(e);
     if (d->h_ctr) (void)hipHostFree(d->h_ctr);
+    if (d->h_misc) (void)hipHostFree(d->h_misc);
+    if (d->h_jctr) (void)hipHostFree(d->h_jctr);
     if (d->st) (void)hipStreamDestroy(d->st);
     (void)hipSetDevice(cur);
     delete d;
@@ -218,9 +238,19 @@ static int validate_collection(const fgss_collection *c)
     if ((int64_t)c->tok.size() >= (int64_t)UINT32_MAX)
         return fgss_fail(FGSS_ERR_UNSUPPORTED, "more than 2^32-1 tokens (the reference's unsigned offsets)");
     if (c->max_len >= 65536) return fgss_fail(FGSS_ERR_UNSUPPORTED, "records longer than 65535 tokens");
-    if (!c->sorted_desc)
-        return fgss_fail(FGSS_ERR_UNSUPPORTED,
-                         "record lengths must be non-increasing in internal order (a size-ascending .bin)");
+    {
+        // lengths non-increasing within each job (the whole collection is one job)
+        int64_t start = 0;
+        const size_t nj = c->jend.empty() ? 1 : c->jend.size();
+        for (size_t j = 0; j < nj; j++) {
+            const int64_t end = c->jend.empty() ? n : (int64_t)c->jend[j];
+            for (int64_t i = start + 1; i < end; i++)
+                if (c->len[(size_t)i] > c->len[(size_t)i - 1])
+                    return fgss_fail(FGSS_ERR_UNSUPPORTED,
+                                     "record lengths must be non-increasing in internal order (a size-ascending .bin)");
+            start = end;
+        }
+    }
     for (int64_t i = 0; i < n; i++) {
         const int32_t *r = c->tok.data() + c->pos[i];
         for (int32_t k = 0; k < c->len[i]; k++) {
@@ -243,18 +273,32 @@ extern "C" int fgss_collection_upload(fgss_collection *c)
     HIPCK(hipGetDevice(&d->device));
     HIPCK(hipStreamCreateWithFlags(&d->st, hipStreamNonBlocking));
     for (auto &e : d->ev) HIPCK(hipEventCreate(&e));
-    HIPCK(hipHostMalloc((void **)&d->h_ctr, C_COUNT * sizeof(uint64_t), hipHostMallocDefault));
     d->n = (int64_t)c->len.size();
     d->ntok = (int64_t)c->tok.size();
-    d->n_terms = c->n_terms;
+    d->n_terms = (uint32_t)c->n_terms;
     d->max_len = c->max_len;
+    d->njobs = c->jend.empty() ? 1u : (uint32_t)c->jend.size();
+    HIPCK(hipHostMalloc((void **)&d->h_ctr, C_COUNT * sizeof(uint64_t), hipHostMallocDefault));
+    HIPCK(hipHostMalloc((void **)&d->h_misc, 16 * sizeof(uint32_t), hipHostMallocDefault));
+    HIPCK(hipHostMalloc((void **)&d->h_jctr, (size_t)d->njobs * kJobCtr * sizeof(uint64_t), hipHostMallocDefault));
     const size_t nb = (size_t)std::max<int64_t>(d->n, 1) * 4, tb = (size_t)std::max<int64_t>(d->ntok, 1) * 4;
+    const size_t jb = (size_t)d->njobs * 4;
     if (hipMalloc(&d->tok, tb) != hipSuccess || hipMalloc(&d->len, nb) != hipSuccess ||
         hipMalloc(&d->pos, nb) != hipSuccess || hipMalloc(&d->sig, nb * kSigWords) != hipSuccess ||
-        hipMalloc(&d->sig64, nb * kSig64Words) != hipSuccess ||
-        hipMalloc(&d->sigbad, nb) != hipSuccess) {
+        hipMalloc(&d->sig64, nb * kSig64Words) != hipSuccess || hipMalloc(&d->sigbad, nb) != hipSuccess ||
+        hipMalloc(&d->job, nb) != hipSuccess || hipMalloc(&d->jend, jb) != hipSuccess ||
+        hipMalloc(&d->tj, jb) != hipSuccess) {
         fgss_device_release(c);
         return fgss_fail(FGSS_ERR_NOMEM, "device allocation for the collection failed");
+    }
+    if (c->jend.empty()) {
+        HIPCK(fill<uint32_t>(d->job, d->n, 0u, d->st));
+        const uint32_t e = (uint32_t)d->n;
+        HIPCK(hipMemcpyAsync(d->jend, &e, 4, hipMemcpyHostToDevice, d->st));
+        HIPCK(hipStreamSynchronize(d->st));   // e lives on this stack frame
+    } else {
+        HIPCK(hipMemcpyAsync(d->job, c->job.data(), (size_t)d->n * 4, hipMemcpyHostToDevice, d->st));
+        HIPCK(hipMemcpyAsync(d->jend, c->jend.data(), jb, hipMemcpyHostToDevice, d->st));
     }
     std::vector<uint32_t> pos32((size_t)d->n);
     for (int64_t i = 0; i < d->n; i++) pos32[(size_t)i] = (uint32_t)c->pos[(size_t)i];
@@ -276,11 +320,13 @@ extern "C" int fgss_collection_upload(fgss_collection *c)
 // ---------------------------------------------------------------------------
 // per record: mid-prefix length (indexed tokens) and probe-prefix length, both
 // clamped to the record length so that an empty record contributes nothing
-__global__ void k_midlen(const uint32_t *__restrict__ len, uint32_t *__restrict__ midlen,
-                         uint32_t *__restrict__ probelen, int64_t n, float t)
+__global__ void k_midlen(const uint32_t *__restrict__ len, const uint32_t *__restrict__ job,
+                         const float *__restrict__ tj, uint32_t *__restrict__ midlen,
+                         uint32_t *__restrict__ probelen, int64_t n)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
+        const float t = tj[job[i]];
         uint32_t l = len[i];
         uint32_t mp = fgss_mid_prefix(l, t);
         uint32_t qp = fgss_max_prefix(l, t);
@@ -305,9 +351,11 @@ __global__ void k_entries(const uint32_t *__restrict__ tok, const uint32_t *__re
 
 // postings in term order, packed: x = record id, y = token position | record length << 16
 __global__ void k_postings(const uint32_t *__restrict__ sidx, const uint32_t *__restrict__ sid_of,
-                           const uint32_t *__restrict__ midoff, const uint32_t *__restrict__ len, uint32_t nE,
-                           uint2 *__restrict__ post, uint32_t *__restrict__ psid, uint32_t *__restrict__ fence)
+                           const uint32_t *__restrict__ midoff, const uint32_t *__restrict__ len,
+                           const uint32_t *__restrict__ nE_dev, uint2 *__restrict__ post, uint32_t *__restrict__ psid,
+                           uint32_t *__restrict__ fence)
 {
+    const uint32_t nE = *nE_dev;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nE) return;
     uint32_t e = sidx[i];
@@ -318,9 +366,10 @@ __global__ void k_postings(const uint32_t *__restrict__ sidx, const uint32_t *__
 }
 
 // loff[t] = first posting of term t; loff[n_terms] = nE
-__global__ void k_list_offsets(const uint32_t *__restrict__ skeys, uint32_t nE, uint32_t n_terms,
-                               uint32_t *__restrict__ loff)
+__global__ void k_list_offsets(const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ nE_dev,
+                               uint32_t n_terms, uint32_t *__restrict__ loff)
 {
+    const uint32_t nE = *nE_dev;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > nE) return;
     int64_t cur = i < nE ? (int64_t)skeys[i] : (int64_t)n_terms;
@@ -328,15 +377,19 @@ __global__ void k_list_offsets(const uint32_t *__restrict__ skeys, uint32_t nE, 
     for (int64_t tt = prev + 1; tt <= cur; tt++) loff[tt] = i;
 }
 
-// lmin[l] = smallest partner length ls <= l with size_ok(l, ls); the valid partner
-// lengths of l form an interval that contains l.
-__global__ void k_lmin(uint32_t *__restrict__ lmin, int32_t max_len, float t)
+// The smallest partner length ls <= l with size_ok(l, ls).  The partner lengths that
+// pass the 4-way size filter form an interval containing l (each of its four bounds
+// is monotone in ls), so on [0, l] the filter is false below the answer and true from
+// it on: a binary search finds it.
+__device__ __forceinline__ uint32_t partner_lmin(uint32_t l, float t)
 {
-    int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l > max_len) return;
-    uint32_t ls = (uint32_t)l;
-    while (ls > 0 && fgss_size_ok((uint32_t)l, ls - 1, t)) ls--;
-    lmin[l] = ls;
+    uint32_t lo = 0, hi = l;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (fgss_size_ok(l, mid, t)) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
 }
 
 // First index in [lo, hi) whose record id is >= v, within one posting list (ids
@@ -382,8 +435,10 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *__restrict__
 // total E(q) sets the number of work items.
 __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, const uint32_t *__restrict__ pos,
                                               const uint32_t *__restrict__ len, int64_t n,
+                                              const uint32_t *__restrict__ job, const float *__restrict__ tj,
+                                              const uint32_t *__restrict__ jend,
                                               const uint32_t *__restrict__ probelen,
-                                              const uint32_t *__restrict__ poff, const uint32_t *__restrict__ lmin,
+                                              const uint32_t *__restrict__ poff,
                                               const uint32_t *__restrict__ loff, const uint32_t *__restrict__ psid,
                                               const uint32_t *__restrict__ fence, uint32_t target,
                                               uint2 *__restrict__ rng, uint32_t *__restrict__ hi_out,
@@ -394,9 +449,11 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
     for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < n; q += nwaves) {
         const uint32_t lq = len[q];
         const uint32_t qp = probelen[q];
-        // window: all s > q with len[s] >= lmin[lq]; lengths are non-increasing
-        const uint32_t need = lmin[lq];
-        uint32_t lo = (uint32_t)q + 1, hi = (uint32_t)n;   // first s with len[s] < need
+        const uint32_t jb = job[q];
+        // window: the s > q of the same job with len[s] >= the smallest partner length;
+        // lengths are non-increasing within a job
+        const uint32_t need = partner_lmin(lq, tj[jb]);
+        uint32_t lo = (uint32_t)q + 1, hi = jend[jb];   // first s with len[s] < need
         while (lo < hi) {
             uint32_t mid = (lo + hi) >> 1;
             if (len[mid] >= need) lo = mid + 1;
@@ -438,7 +495,8 @@ __global__ __launch_bounds__(256) void k_bounds(int64_t n, const uint32_t *__res
                                                 const uint32_t *__restrict__ poff, const uint2 *__restrict__ rng,
                                                 const uint32_t *__restrict__ hi, const uint32_t *__restrict__ nparts,
                                                 const uint32_t *__restrict__ bofs, const uint32_t *__restrict__ psid,
-                                                const uint32_t *__restrict__ fence, uint32_t *__restrict__ bounds)
+                                                const uint32_t *__restrict__ fence, uint32_t *__restrict__ bounds,
+                                                uint32_t cap_bounds)
 {
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < n; q += nwaves) {
@@ -449,6 +507,7 @@ __global__ __launch_bounds__(256) void k_bounds(int64_t n, const uint32_t *__res
         uint32_t *B = bounds + bofs[q];
         const uint2 *R = rng + poff[q];
         const uint32_t total = (np - 1) * qp;
+        if ((uint64_t)bofs[q] + total > cap_bounds) continue;   // too small: the host resizes and reruns
         for (uint32_t k = lane_id(); k < total; k += kWave) {
             const uint32_t p = k / qp + 1, j = k - (p - 1) * qp;
             const uint2 r = R[j];
@@ -459,7 +518,10 @@ __global__ __launch_bounds__(256) void k_bounds(int64_t n, const uint32_t *__res
 
 __global__ void k_items(const uint32_t *__restrict__ hi, const uint32_t *__restrict__ nparts,
                         const uint32_t *__restrict__ eq, const uint32_t *__restrict__ partoff,
-                        const uint32_t *__restrict__ bofs, int64_t n, Item *__restrict__ items)
+                        const uint32_t *__restrict__ bofs, const uint32_t *__restrict__ job,
+                        const float *__restrict__ tj, const uint32_t *__restrict__ len,
+                        const uint32_t *__restrict__ pos, const uint32_t *__restrict__ poff, int64_t n,
+                        Item *__restrict__ items, uint32_t cap_items)
 {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
@@ -467,6 +529,7 @@ __global__ void k_items(const uint32_t *__restrict__ hi, const uint32_t *__restr
     if (np == 0) return;
     uint64_t a0 = (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
     uint32_t o = partoff[q];
+    if ((uint64_t)o + np > cap_items) return;   // too small: the host resizes and reruns
     const uint32_t per = (eq[q] + np - 1) / np;   // expected postings per part: sizes its LDS table
     for (uint32_t p = 0; p < np; p++) {
         Item it;
@@ -477,7 +540,11 @@ __global__ void k_items(const uint32_t *__restrict__ hi, const uint32_t *__restr
         it.part = p;
         it.nparts = np;
         it.bofs = bofs[q];
-        it.spare = 0;
+        it.job = job[q];
+        it.t = tj[it.job];
+        it.lq = len[q];
+        it.qpos = pos[q];
+        it.qpoff = poff[q];
         items[o + p] = it;
     }
 }
@@ -491,15 +558,13 @@ __global__ void k_resplit(const Item *__restrict__ in, const uint32_t *__restric
     Item it = in[ovf[i]];
     uint64_t W = (uint64_t)it.b - it.a;
     for (uint32_t p = 0; p < ways; p++) {
-        Item o;
-        o.q = it.q;
+        Item o = it;
         o.a = (uint32_t)(it.a + W * p / ways);
         o.b = (uint32_t)(it.a + W * (p + 1) / ways);
         o.pad = it.pad & kPadMask;   // same table size as the parent: four times the room per posting
         o.part = 0;
         o.nparts = 0;
         o.bofs = 0;
-        o.spare = 0;
         out[i * ways + p] = o;
     }
 }
@@ -520,13 +585,24 @@ struct JoinArgs {
     const uint32_t *poff;       // per query: first entry of its probe tokens in rng
     const uint32_t *bounds;     // planned split boundaries (see k_bounds)
     const Item *items;
-    uint32_t n_items;
-    float t;
+    const uint32_t *n_items_dev;  // number of items (device); the launch is void if above n_items_cap
+    uint32_t n_items_cap;
+    const uint32_t *nb_dev;       // entries of the boundary table (device); void if above nb_cap
+    uint32_t nb_cap;
+    const uint32_t *job;        // per record: job id
+    const float *tj;            // per job: Jaccard threshold
+    unsigned long long *ctr_job;  // per job: kJobCtr counters
     int32_t *out;               // (q, s, overlap) triples
     uint32_t out_cap;
     uint32_t *ovf;              // indices of overflowed items
     unsigned long long *ctr;
 };
+
+__device__ __forceinline__ void flush_job(unsigned long long *ctr_job, uint32_t j, const volatile unsigned long long *acc)
+{
+    for (int k = 0; k < kJobCtr; k++)
+        if (acc[k]) atomicAdd(&ctr_job[(size_t)j * kJobCtr + k], acc[k]);
+}
 
 __device__ __forceinline__ uint32_t slot_hash(uint32_t s, int log2cap)
 {
@@ -562,49 +638,73 @@ __device__ __forceinline__ int ceil_log2(uint32_t x)
 // Each phase compacts the list in place and resets the table slots it drops, so the
 // table is empty again when the item ends: no per-item clear.
 constexpr int kUnroll = 4;
-constexpr int kQStage = 192;   // query tokens staged in LDS for verification (longer queries stay in HBM)
-// LDS words per wavefront: table (key, count, last match), list, token ranges, query stage
-constexpr uint32_t join_wave_words(int cap) { return 4u * cap + 2u * kTokBatch + 1u + kQStage; }
+constexpr int kQStage = 160;   // query tokens staged in LDS for verification (longer queries stay in HBM)
+// per-wave counters kept in LDS (not registers): totals, shader-clock phases, current job
+enum WaveAcc { W_ESCAN = 0, W_CAND, W_SIG, W_VER, W_SIG2, W_INTER, W_VTOK, W_CYC0, W_JOB0 = W_CYC0 + 6,
+               kAccSlots = W_JOB0 + kJobCtr };
+// LDS words per wavefront: counters (u64), table (key, count, last match), list,
+// token ranges, query stage; even so that the next wave's counters stay 8-byte aligned
+constexpr uint32_t join_wave_words(int cap)
+{
+    return (2u * kAccSlots + 4u * cap + 2u * kTokBatch + 1u + kQStage + 1u) & ~1u;
+}
 
 template <int CAP>
-__global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
+__global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
 {
     constexpr int LOG2CAP = CAP == 256 ? 8 : CAP == 512 ? 9 : CAP == 1024 ? 10 : CAP == 2048 ? 11 : 12;
     static_assert((1 << LOG2CAP) == CAP, "CAP must be a power of two");
     extern __shared__ uint32_t smem[];
     const int w = (int)threadIdx.x / kWave;
     const int lane = lane_id();
-    uint32_t *tkey = smem + (size_t)w * join_wave_words(CAP);
+    unsigned long long *wacc = (unsigned long long *)(smem + (size_t)w * join_wave_words(CAP));
+    uint32_t *tkey = (uint32_t *)(wacc + kAccSlots);
     uint32_t *tcnt = tkey + CAP;
     uint32_t *tlast = tcnt + CAP;
     uint32_t *clist = tlast + CAP;        // slot | |S| << 16, one per distinct source
     uint32_t *rlo = clist + CAP;
     uint32_t *rpre = rlo + kTokBatch;     // kTokBatch + 1 entries
     uint32_t *qstage = rpre + kTokBatch + 1;
-    const float t = A.t;
+    // a launch whose item list or boundary table did not fit does nothing: the host
+    // sees the totals, grows the buffers and runs the pipeline again
+    if (*A.n_items_dev > A.n_items_cap || *A.nb_dev > A.nb_cap) return;
+    const uint32_t n_items = *A.n_items_dev;
 
     for (uint32_t i = lane; i < CAP; i += kWave) { tkey[i] = kEmpty; tcnt[i] = 0; tlast[i] = 0; }
+    if (lane < kAccSlots) wacc[lane] = 0;
     wave_sync();
+#define WACC(k, v)                                     \
+    do {                                               \
+        if (lane == 0) wacc[(k)] += (unsigned long long)(v); \
+    } while (0)
 
-    unsigned long long s_escan = 0, s_cand = 0, s_sig = 0, s_ver = 0, s_inter = 0, s_vtok = 0, s_sig2 = 0;
-    unsigned long long cyc[6] = {0, 0, 0, 0, 0, 0};
     unsigned long long c0 = clock64(), c1;
     // items are claimed kQueueBatch at a time: one global atomic per batch
     uint32_t batch_next = 0, batch_end = 0;
+    uint32_t cur_job = kEmpty;
     for (;;) {
         if (batch_next >= batch_end) {
             uint32_t b0 = 0;
             if (lane == 0) b0 = atomicAdd((unsigned int *)&A.ctr[C_QUEUE], (unsigned)kQueueBatch);
             b0 = __shfl(b0, 0, kWave);
             batch_next = b0;
-            batch_end = min(b0 + (uint32_t)kQueueBatch, A.n_items);
+            batch_end = min(b0 + (uint32_t)kQueueBatch, n_items);
         }
-        if (batch_next >= A.n_items) break;
+        if (batch_next >= n_items) break;
         const uint32_t it = batch_next++;
         const Item item = A.items[it];
         const uint32_t q = item.q, wa = item.a, wb = item.b;
-        const uint32_t lq = A.len[q];
-        const uint32_t qpos = A.pos[q];
+        const uint32_t jq = item.job;
+        const float t = item.t;
+        if (jq != cur_job) {   // per-job counters: flush when the job changes
+            if (lane == 0) {
+                if (cur_job != kEmpty) flush_job(A.ctr_job, cur_job, wacc + W_JOB0);
+                for (int k = 0; k < kJobCtr; k++) wacc[W_JOB0 + k] = 0;
+            }
+            cur_job = jq;
+        }
+        const uint32_t lq = item.lq;
+        const uint32_t qpos = item.qpos;
         uint32_t qp = fgss_max_prefix(lq, t);
         if (qp > lq) qp = lq;
         // table sized for twice the expected postings (64 .. CAP slots)
@@ -614,11 +714,11 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
 
         const bool whole = (item.pad & kWholeWindow) != 0;
         const bool planned = (item.pad & kPlanned) != 0;
-        const uint2 *qrng = A.rng + A.poff[q];
+        const uint2 *qrng = A.rng + item.qpoff;
         const uint32_t *lo_row = planned && item.part > 0 ? A.bounds + item.bofs + (size_t)(item.part - 1) * qp : nullptr;
         const uint32_t *hi_row = planned && item.part + 1 < item.nparts ? A.bounds + item.bofs + (size_t)item.part * qp
                                                                         : nullptr;
-        c1 = clock64(); cyc[0] += c1 - c0; c0 = c1;
+        c1 = clock64(); WACC(W_CYC0 + 0, c1 - c0); c0 = c1;
 
         // ---- A: candidate generation (filtering.cu:94-139) ----
         bool overflow = false;
@@ -660,7 +760,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
             const uint32_t total = __shfl(inc, kWave - 1, kWave);
             if (lane == 0) rpre[nb] = total;
             wave_sync();
-            cyc[5] += clock64() - r0;
+            WACC(W_CYC0 + 5, clock64() - r0);
             escan += total;
             const int steps = ceil_log2(nb);   // wave-uniform search depth
             for (uint32_t e0 = 0; e0 < total; e0 += kWave * kUnroll) {
@@ -728,7 +828,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
             wave_sync();
             continue;
         }
-        c1 = clock64(); cyc[1] += c1 - c0; c0 = c1;
+        c1 = clock64(); WACC(W_CYC0 + 1, c1 - c0); c0 = c1;
 
         // ---- B: positional filter over the distinct sources ----
         uint32_t ncand = 0, cand = 0;
@@ -760,7 +860,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
         }
         cand = wave_sum(cand);
         wave_sync();
-        c1 = clock64(); cyc[2] += c1 - c0; c0 = c1;
+        c1 = clock64(); WACC(W_CYC0 + 2, c1 - c0); c0 = c1;
 
         // ---- C: signature bound, one lane per candidate: 64 buckets, then 128 for the rest ----
         const uint32_t nsig = ncand;
@@ -798,7 +898,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
                     }
                 }
                 if (__ballot(exact)) {
-                    s_sig2 += (unsigned long long)__popcll(__ballot(exact));
+                    WACC(W_SIG2, __popcll(__ballot(exact)));
                     if (exact) {
                         const uint4 *ss4 = (const uint4 *)(A.sig + (size_t)s * kSigWords);
                         uint32_t sad = 0;
@@ -821,10 +921,10 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
             }
             wave_sync();
         }
-        c1 = clock64(); cyc[3] += c1 - c0; c0 = c1;
+        c1 = clock64(); WACC(W_CYC0 + 3, c1 - c0); c0 = c1;
 
         // ---- D: verification (checking.cu:93-175): overlap of S after the last prefix match ----
-        uint32_t inter = 0, vtok = 0;
+        uint32_t inter = 0, vtok = 0, nres = 0;
         const bool staged = lq <= (uint32_t)kQStage;
         if (nver > 0 && staged) {
             for (uint32_t i = lane; i < lq; i += kWave) qstage[i] = A.tok[qpos + i];
@@ -864,6 +964,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
             if ((int)ov >= floorb) {
                 inter++;
                 if (fgss_similarity(lq, ls, ov) >= t) {            // block.cu:189-192
+                    nres++;
                     if (lane == 0) {
                         uint32_t r = (uint32_t)atomicAdd(&A.ctr[C_RESULTS], 1ull);
                         if (r < A.out_cap) {
@@ -881,23 +982,28 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave) void k_join(JoinArgs A)
             tkey[h] = kEmpty; tcnt[h] = 0; tlast[h] = 0;
         }
         wave_sync();
-        c1 = clock64(); cyc[4] += c1 - c0; c0 = c1;
-        s_escan += escan;
-        s_cand += cand;
-        s_sig += nsig;
-        s_ver += nver;
-        s_inter += inter;
-        s_vtok += vtok;
+        c1 = clock64(); WACC(W_CYC0 + 4, c1 - c0); c0 = c1;
+        WACC(W_JOB0 + 0, escan);
+        WACC(W_JOB0 + 1, cand);
+        WACC(W_JOB0 + 2, inter);
+        WACC(W_JOB0 + 3, nres);
+        WACC(W_ESCAN, escan);
+        WACC(W_CAND, cand);
+        WACC(W_SIG, nsig);
+        WACC(W_VER, nver);
+        WACC(W_INTER, inter);
+        WACC(W_VTOK, vtok);
     }
     if (lane == 0) {
-        atomicAdd(&A.ctr[C_ESCAN], s_escan);
-        atomicAdd(&A.ctr[C_CAND], s_cand);
-        atomicAdd(&A.ctr[C_SIGCHK], s_sig);
-        atomicAdd(&A.ctr[C_VERIFIED], s_ver);
-        atomicAdd(&A.ctr[C_SIG2], s_sig2);
-        atomicAdd(&A.ctr[C_INTER], s_inter);
-        atomicAdd(&A.ctr[C_VTOK], s_vtok);
-        for (int k = 0; k < 6; k++) atomicAdd(&A.ctr[C_CYC_SETUP + k], cyc[k]);
+        atomicAdd(&A.ctr[C_ESCAN], wacc[W_ESCAN]);
+        atomicAdd(&A.ctr[C_CAND], wacc[W_CAND]);
+        atomicAdd(&A.ctr[C_SIGCHK], wacc[W_SIG]);
+        atomicAdd(&A.ctr[C_VERIFIED], wacc[W_VER]);
+        atomicAdd(&A.ctr[C_SIG2], wacc[W_SIG2]);
+        atomicAdd(&A.ctr[C_INTER], wacc[W_INTER]);
+        atomicAdd(&A.ctr[C_VTOK], wacc[W_VTOK]);
+        for (int k = 0; k < 6; k++) atomicAdd(&A.ctr[C_CYC_SETUP + k], wacc[W_CYC0 + k]);
+        if (cur_job != kEmpty) flush_job(A.ctr_job, cur_job, wacc + W_JOB0);
     }
 }
 
@@ -907,48 +1013,196 @@ static size_t join_smem_bytes()
     return (size_t)join_waves(CAP) * join_wave_words(CAP) * sizeof(uint32_t);
 }
 
+// persistent grid: as many blocks as fit on the device, each wave loops over items
 template <int CAP>
-static int launch_join(fgss_device_sets *d, JoinArgs A, int *blocks_out)
+static int launch_join(fgss_device_sets *d, const JoinArgs &A, bool dry = false)
 {
-    // occupancy-derived persistent grid, computed once per table size
-    static thread_local int cached_blocks[2] = {-1, -1};
+    static thread_local int cached[2] = {-1, -1};   // blocks, device
     const size_t smem = join_smem_bytes<CAP>();
-    int &blocks_all = cached_blocks[0];
-    if (cached_blocks[1] != d->device || blocks_all < 0) {
+    if (cached[0] < 0 || cached[1] != d->device) {   // runtime queries: never inside a graph capture
         HIPCK(hipFuncSetAttribute((const void *)k_join<CAP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-        int per_cu = 0;
+        int per_cu = 0, cus = 0;
         HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_join<CAP>, join_waves(CAP) * kWave, smem));
-        int cus = 0;
         HIPCK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device));
-        blocks_all = std::max(per_cu, 1) * cus;
-        cached_blocks[1] = d->device;
+        cached[0] = std::max(per_cu, 1) * cus;
+        cached[1] = d->device;
     }
-    int blocks = blocks_all;
-    const int need = (int)((A.n_items + join_waves(CAP) - 1) / join_waves(CAP));
-    if (blocks > need) blocks = std::max(need, 1);
-    hipLaunchKernelGGL(k_join<CAP>, dim3(blocks), dim3(join_waves(CAP) * kWave), smem, d->st, A);
+    if (dry) return FGSS_OK;
+    hipLaunchKernelGGL(k_join<CAP>, dim3(cached[0]), dim3(join_waves(CAP) * kWave), smem, d->st, A);
     HIPCK(hipGetLastError());
-    if (blocks_out) *blocks_out = blocks;
     return FGSS_OK;
 }
 
-static int run_join(fgss_device_sets *d, JoinArgs A, int cap)
+static int launch_join_cap(fgss_device_sets *d, const JoinArgs &A, int cap, bool dry = false)
 {
     switch (cap) {
-    case 256: return launch_join<256>(d, A, nullptr);
-    case 512: return launch_join<512>(d, A, nullptr);
-    case 1024: return launch_join<1024>(d, A, nullptr);
-    case 2048: return launch_join<2048>(d, A, nullptr);
-    case 4096: return launch_join<4096>(d, A, nullptr);
+    case 256: return launch_join<256>(d, A, dry);
+    case 512: return launch_join<512>(d, A, dry);
+    case 1024: return launch_join<1024>(d, A, dry);
+    case 2048: return launch_join<2048>(d, A, dry);
+    case 4096: return launch_join<4096>(d, A, dry);
     default: return fgss_fail(FGSS_ERR_ARG, "table_slots must be 256, 512, 1024, 2048 or 4096");
     }
 }
 
-static inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+static inline unsigned grid_for(int64_t n, int threads) { return (unsigned)std::max<int64_t>((n + threads - 1) / threads, 1); }
 
-// ---------------------------------------------------------------------------
-// fgss_selfjoin
-// ---------------------------------------------------------------------------
+// device totals written by the scans (indices into b_misc)
+enum Misc { M_NE = 0, M_NITEMS, M_NP, M_NB, M_NRESPLIT, M_COUNT = 16 };
+
+struct RunCfg {
+    int cap = 512;          // LDS table slots per wave
+    uint32_t target = 332;  // postings per work item
+};
+
+static RunCfg make_cfg(const fgss_join_opts *opts)
+{
+    RunCfg c;
+    c.cap = opts && opts->table_slots ? opts->table_slots : 512;
+    const int load = opts && opts->load_pct > 0 ? std::min(opts->load_pct, 100) : 65;
+    c.target = std::max<uint32_t>(1u, (uint32_t)(c.cap * load / 100));
+    return c;
+}
+
+// Grow a pipeline buffer; any reallocation invalidates a captured graph.
+static int grow(fgss_device_sets *d, DevBuf &b, size_t bytes)
+{
+    const void *before = b.p;
+    int rc = ensure(b, bytes);
+    if (rc) return rc;
+    if (b.p != before) d->cap_gen++;
+    return FGSS_OK;
+}
+
+#define GROW(buf, bytes)                      \
+    do {                                      \
+        int rc_ = grow(d, (buf), (bytes));    \
+        if (rc_) return rc_;                  \
+    } while (0)
+
+// Every buffer the pipeline touches, sized for upper bounds where one exists (index
+// entries and probe tokens <= tokens) and for the current capacities otherwise.
+static int prepare_buffers(fgss_device_sets *d)
+{
+    const int64_t n = std::max<int64_t>(d->n, 1), E = std::max<int64_t>(d->ntok, 1);
+    if (d->cap_items == 0) d->cap_items = (uint32_t)std::min<int64_t>(2 * n + E / 64 + 4096, UINT32_MAX / 2);
+    if (d->cap_bounds == 0) d->cap_bounds = (uint32_t)std::min<int64_t>(E + 4096, UINT32_MAX / 2);
+    if (d->cap_out == 0) d->cap_out = 65536;
+    GROW(d->b_midlen, (size_t)n * 4);
+    GROW(d->b_midoff, (size_t)(n + 1) * 4);
+    GROW(d->b_probelen, (size_t)n * 4);
+    GROW(d->b_poff, (size_t)(n + 1) * 4);
+    GROW(d->b_scanscr, (size_t)scan_scratch_elems(n + 1) * 4);
+    GROW(d->b_ka, (size_t)E * 4);
+    GROW(d->b_kb, (size_t)E * 4);
+    GROW(d->b_ia, (size_t)E * 4);
+    GROW(d->b_ib, (size_t)E * 4);
+    GROW(d->b_sid_of, (size_t)E * 4);
+    GROW(d->b_post, (size_t)(E + 1) * 8);
+    GROW(d->b_psid, (size_t)(E + 1) * 4);
+    GROW(d->b_fence, (size_t)(E / kFence + 2) * 4);
+    GROW(d->b_loff, ((size_t)d->n_terms + 2) * 4);
+    GROW(d->b_sortscr, (size_t)radix_scratch_elems(E) * 4);
+    GROW(d->b_rng, (size_t)E * sizeof(uint2));
+    GROW(d->b_hi, (size_t)n * 4);
+    GROW(d->b_eq, (size_t)n * 4);
+    GROW(d->b_nparts, (size_t)n * 4);
+    GROW(d->b_partoff, (size_t)(n + 1) * 4);
+    GROW(d->b_bcnt, (size_t)n * 4);
+    GROW(d->b_bofs, (size_t)(n + 1) * 4);
+    GROW(d->b_bounds, (size_t)d->cap_bounds * 4);
+    GROW(d->b_items, (size_t)d->cap_items * sizeof(Item));
+    GROW(d->b_ovf, (size_t)d->cap_items * 4);
+    GROW(d->b_out, (size_t)d->cap_out * 12);
+    GROW(d->b_ctr, C_COUNT * sizeof(uint64_t));
+    GROW(d->b_jctr, (size_t)d->njobs * kJobCtr * sizeof(uint64_t));
+    GROW(d->b_misc, M_COUNT * 4);
+    return FGSS_OK;
+}
+
+static JoinArgs join_args(fgss_device_sets *d)
+{
+    JoinArgs A;
+    A.tok = d->tok; A.pos = d->pos; A.len = d->len;
+    A.loff = P<uint32_t>(d->b_loff); A.post = P<uint2>(d->b_post);
+    A.psid = P<uint32_t>(d->b_psid); A.fence = P<uint32_t>(d->b_fence);
+    A.sig = d->sig; A.sig64 = d->sig64; A.sigbad = d->sigbad;
+    A.rng = P<uint2>(d->b_rng); A.poff = P<uint32_t>(d->b_poff); A.bounds = P<uint32_t>(d->b_bounds);
+    A.items = P<Item>(d->b_items);
+    A.n_items_dev = P<uint32_t>(d->b_misc) + M_NITEMS; A.n_items_cap = d->cap_items;
+    A.nb_dev = P<uint32_t>(d->b_misc) + M_NB; A.nb_cap = d->cap_bounds;
+    A.job = d->job; A.tj = d->tj; A.ctr_job = P<unsigned long long>(d->b_jctr);
+    A.out = P<int32_t>(d->b_out); A.out_cap = d->cap_out; A.ovf = P<uint32_t>(d->b_ovf);
+    A.ctr = P<unsigned long long>(d->b_ctr);
+    return A;
+}
+
+// Enqueue the whole join on d->st with no host synchronization (so it can be
+// captured into a HIP graph): index build, planning, the join kernel, and the
+// device->host copies of all counters into pinned memory.
+static int enqueue_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool timing)
+{
+    hipStream_t st = d->st;
+    const int64_t n = d->n, E = std::max<int64_t>(d->ntok, 1);
+    uint32_t *misc = P<uint32_t>(d->b_misc), *scr = P<uint32_t>(d->b_scanscr);
+    uint32_t *midlen = P<uint32_t>(d->b_midlen), *midoff = P<uint32_t>(d->b_midoff);
+    uint32_t *probelen = P<uint32_t>(d->b_probelen), *poff = P<uint32_t>(d->b_poff);
+    uint32_t *ka = P<uint32_t>(d->b_ka), *kb = P<uint32_t>(d->b_kb), *ia = P<uint32_t>(d->b_ia),
+             *ib = P<uint32_t>(d->b_ib), *sid_of = P<uint32_t>(d->b_sid_of), *loff = P<uint32_t>(d->b_loff);
+    uint32_t *psid = P<uint32_t>(d->b_psid), *fence = P<uint32_t>(d->b_fence);
+    uint2 *post = P<uint2>(d->b_post);
+    uint32_t *hi = P<uint32_t>(d->b_hi), *eq = P<uint32_t>(d->b_eq), *nparts = P<uint32_t>(d->b_nparts),
+             *partoff = P<uint32_t>(d->b_partoff), *bcnt = P<uint32_t>(d->b_bcnt), *bofs = P<uint32_t>(d->b_bofs);
+    unsigned long long *ctr = P<unsigned long long>(d->b_ctr), *jctr = P<unsigned long long>(d->b_jctr);
+
+    if (timing) HIPCK(hipEventRecord(d->ev[0], st));
+    HIPCK(fill<unsigned long long>(ctr, C_COUNT, 0ull, st));
+    HIPCK(fill<unsigned long long>(jctr, (int64_t)d->njobs * kJobCtr, 0ull, st));
+    HIPCK(fill<uint32_t>(misc, M_COUNT, 0u, st));
+    // 1-5: mid-prefix entries, stable sort by term, postings and list offsets
+    hipLaunchKernelGGL(k_midlen, dim3(grid_for(n, 256)), dim3(256), 0, st, d->len, d->job, d->tj, midlen, probelen, n);
+    HIPCK(hipGetLastError());
+    HIPCK(exclusive_scan<uint32_t>(midlen, midoff, n, misc + M_NE, scr, st));
+    HIPCK(exclusive_scan<uint32_t>(probelen, poff, n, misc + M_NP, scr, st));
+    hipLaunchKernelGGL(k_entries, dim3(grid_for(n, 256)), dim3(256), 0, st, d->tok, d->pos, midlen, midoff, n, ka, sid_of);
+    HIPCK(hipGetLastError());
+    int bits = 1;
+    while (bits < 32 && (1ll << bits) < (long long)d->n_terms) bits++;
+    int which = 0;
+    HIPCK(radix_sort_pairs(ka, ia, kb, ib, E, misc + M_NE, bits, P<uint32_t>(d->b_sortscr), st, &which));
+    const uint32_t *skeys = which ? kb : ka;
+    const uint32_t *sidx = which ? ib : ia;
+    hipLaunchKernelGGL(k_postings, dim3(grid_for(E, 256)), dim3(256), 0, st, sidx, sid_of, midoff, d->len, misc + M_NE,
+                       post, psid, fence);
+    HIPCK(hipGetLastError());
+    hipLaunchKernelGGL(k_list_offsets, dim3(grid_for(E + 1, 256)), dim3(256), 0, st, skeys, misc + M_NE, d->n_terms, loff);
+    HIPCK(hipGetLastError());
+    if (timing) HIPCK(hipEventRecord(d->ev[1], st));
+    // 6-8: plan work items
+    hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 4096)), dim3(256), 0, st,
+                       d->tok, d->pos, d->len, n, d->job, d->tj, d->jend, probelen, poff, loff, psid, fence, cfg.target,
+                       P<uint2>(d->b_rng), hi, nparts, eq, bcnt);
+    HIPCK(hipGetLastError());
+    HIPCK(exclusive_scan<uint32_t>(nparts, partoff, n, misc + M_NITEMS, scr, st));
+    HIPCK(exclusive_scan<uint32_t>(bcnt, bofs, n, misc + M_NB, scr, st));
+    hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 8192)), dim3(256), 0, st, n,
+                       probelen, poff, P<uint2>(d->b_rng), hi, nparts, bofs, psid, fence, P<uint32_t>(d->b_bounds),
+                       d->cap_bounds);
+    HIPCK(hipGetLastError());
+    hipLaunchKernelGGL(k_items, dim3(grid_for(n, 256)), dim3(256), 0, st, hi, nparts, eq, partoff, bofs, d->job, d->tj,
+                       d->len, d->pos, poff, n, P<Item>(d->b_items), d->cap_items);
+    HIPCK(hipGetLastError());
+    // 9: the join
+    if (timing) HIPCK(hipEventRecord(d->ev[4], st));
+    int rc = launch_join_cap(d, join_args(d), cfg.cap);
+    if (rc) return rc;
+    if (timing) HIPCK(hipEventRecord(d->ev[5], st));
+    HIPCK(hipMemcpyAsync(d->h_ctr, ctr, C_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    HIPCK(hipMemcpyAsync(d->h_misc, misc, M_COUNT * 4, hipMemcpyDeviceToHost, st));
+    HIPCK(hipMemcpyAsync(d->h_jctr, jctr, (size_t)d->njobs * kJobCtr * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    return FGSS_OK;
+}
+
 // FGSS_TRACE=1 prints a host-side timeline of each join to stderr
 static void trace_mark(const char *what)
 {
@@ -961,192 +1215,143 @@ static void trace_mark(const char *what)
     last = now;
 }
 
-static int selfjoin_device(fgss_collection *c, float t, const fgss_join_opts *opts, std::vector<int32_t> &tri,
-                           fgss_join_stats &stats)
+struct RunTotals {
+    uint64_t acc[C_COUNT] = {0};   // global counters summed over launches
+    std::vector<uint64_t> jctr;    // per job: kJobCtr counters
+    uint32_t index_entries = 0, probe_tokens = 0;
+    double ms_index = 0, ms_join = 0, ms_kernel = 0;
+    int launches = 0;
+    bool graph = false;
+};
+
+static int capture_graph(fgss_device_sets *d, const RunCfg &cfg)
 {
-    fgss_device_sets *d = c->dev;
+    if (d->graph_exec) { HIPCK(hipGraphExecDestroy(d->graph_exec)); d->graph_exec = nullptr; }
+    if (d->graph) { HIPCK(hipGraphDestroy(d->graph)); d->graph = nullptr; }
+    HIPCK(hipStreamBeginCapture(d->st, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_pipeline(d, cfg, false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(d->st, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fgss_fail(FGSS_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    d->graph = g;
+    HIPCK(hipGraphInstantiate(&d->graph_exec, g, nullptr, nullptr, 0));
+    d->graph_key[0] = cfg.cap;
+    d->graph_key[1] = (int)cfg.target;
+    d->graph_key[2] = d->cap_gen;
+    return FGSS_OK;
+}
+
+// Run the join for the thresholds already in d->tj: the pipeline (eager, or as a
+// replayed graph), grown and rerun until every capacity holds, then host-driven
+// rounds for work items whose LDS table overflowed (split four ways each time).
+// Appends (q, s, overlap) triples in device emission order.
+static int run_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool use_graph, std::vector<int32_t> &tri, RunTotals &T)
+{
     hipStream_t st = d->st;
-    const int64_t n = d->n;
-    int cap = opts && opts->table_slots ? opts->table_slots : 512;
-    const int load = opts && opts->load_pct > 0 ? std::min(opts->load_pct, 100) : 65;
-    uint32_t target = std::max<uint32_t>(1u, (uint32_t)(cap * load / 100));
-    memset(&stats, 0, sizeof(stats));
     tri.clear();
-    if (n == 0) return FGSS_OK;
+    T = RunTotals();
+    T.jctr.assign((size_t)d->njobs * kJobCtr, 0);
+    if (d->n == 0) return FGSS_OK;
     trace_mark("begin");
-
-    HIPCK(hipEventRecord(d->ev[0], st));
-    ENSURE(d->b_ctr, C_COUNT * sizeof(uint64_t));
-    unsigned long long *ctr = P<unsigned long long>(d->b_ctr);
-    HIPCK(hipMemsetAsync(ctr, 0, C_COUNT * sizeof(uint64_t), st));
-
-    // 1-2: mid-prefix lengths and entry offsets
-    ENSURE(d->b_midlen, (size_t)n * 4);
-    ENSURE(d->b_midoff, (size_t)(n + 1) * 4);
-    ENSURE(d->b_scanscr, (size_t)scan_scratch_elems(std::max<int64_t>(n, 1 << 20)) * 8);
-    ENSURE(d->b_misc, 64);
-    uint32_t *midlen = P<uint32_t>(d->b_midlen), *midoff = P<uint32_t>(d->b_midoff);
-    uint32_t *misc = P<uint32_t>(d->b_misc);
-    ENSURE(d->b_probelen, (size_t)n * 4);
-    ENSURE(d->b_poff, (size_t)(n + 1) * 4);
-    uint32_t *probelen = P<uint32_t>(d->b_probelen), *poff = P<uint32_t>(d->b_poff);
-    hipLaunchKernelGGL(k_midlen, dim3(grid_for(n, 256)), dim3(256), 0, st, d->len, midlen, probelen, n, t);
-    HIPCK(hipGetLastError());
-    HIPCK(exclusive_scan<uint32_t>(midlen, midoff, n, misc, P<uint32_t>(d->b_scanscr), st));
-    HIPCK(exclusive_scan<uint32_t>(probelen, poff, n, misc + 2, P<uint32_t>(d->b_scanscr), st));
-    uint32_t nE = 0, nP = 0;
-    HIPCK(hipMemcpyAsync(&nE, misc, 4, hipMemcpyDeviceToHost, st));
-    HIPCK(hipMemcpyAsync(&nP, misc + 2, 4, hipMemcpyDeviceToHost, st));
-    HIPCK(hipStreamSynchronize(st));
-    trace_mark("after_sizes");
-    stats.index_entries = nE;
-
-    // 3-5: entries, stable sort by term, postings and list offsets
-    const size_t eb = (size_t)std::max<uint32_t>(nE, 1) * 4;
-    ENSURE(d->b_ka, eb);
-    ENSURE(d->b_kb, eb);
-    ENSURE(d->b_ia, eb);
-    ENSURE(d->b_ib, eb);
-    ENSURE(d->b_sid_of, eb);
-    ENSURE(d->b_post, 2 * eb + 8);
-    ENSURE(d->b_psid, eb + 4);
-    ENSURE(d->b_fence, (size_t)(nE / kFence + 2) * 4);
-    uint32_t *psid = P<uint32_t>(d->b_psid), *fence = P<uint32_t>(d->b_fence);
-    ENSURE(d->b_loff, ((size_t)d->n_terms + 2) * 4);
-    ENSURE(d->b_sortscr, (size_t)radix_scratch_elems(std::max<uint32_t>(nE, 1)) * 4);
-    uint32_t *ka = P<uint32_t>(d->b_ka), *kb = P<uint32_t>(d->b_kb), *ia = P<uint32_t>(d->b_ia),
-             *ib = P<uint32_t>(d->b_ib), *sid_of = P<uint32_t>(d->b_sid_of), *loff = P<uint32_t>(d->b_loff);
-    uint2 *post = P<uint2>(d->b_post);
-    hipLaunchKernelGGL(k_entries, dim3(grid_for(n, 256)), dim3(256), 0, st, d->tok, d->pos, midlen, midoff, n, ka,
-                       sid_of);
-    HIPCK(hipGetLastError());
-    int bits = 1;
-    while (bits < 32 && (1ll << bits) < (long long)d->n_terms) bits++;
-    int which = 0;
-    HIPCK(radix_sort_pairs(ka, ia, kb, ib, nE, bits, P<uint32_t>(d->b_sortscr), st, &which));
-    const uint32_t *skeys = which ? kb : ka;
-    const uint32_t *sidx = which ? ib : ia;
-    if (nE > 0) {
-        hipLaunchKernelGGL(k_postings, dim3(grid_for(nE, 256)), dim3(256), 0, st, sidx, sid_of, midoff, d->len, nE,
-                           post, psid, fence);
-        HIPCK(hipGetLastError());
+    for (int attempt = 0;; attempt++) {
+        if (attempt > 8) return fgss_fail(FGSS_ERR_DEVICE, "internal: buffer sizing did not converge");
+        int rc = prepare_buffers(d);
+        if (rc) return rc;
+        if (use_graph) {
+            rc = launch_join_cap(d, join_args(d), cfg.cap, true);   // launch configuration, outside the capture
+            if (rc) return rc;
+            if (!d->graph_exec || d->graph_key[0] != cfg.cap || d->graph_key[1] != (int)cfg.target ||
+                d->graph_key[2] != d->cap_gen) {
+                rc = capture_graph(d, cfg);
+                if (rc) return rc;
+            }
+            HIPCK(hipGraphLaunch(d->graph_exec, st));
+        } else {
+            rc = enqueue_pipeline(d, cfg, true);
+            if (rc) return rc;
+        }
+        HIPCK(hipStreamSynchronize(st));
+        trace_mark("pipeline");
+        const uint32_t n_items = d->h_misc[M_NITEMS], nB = d->h_misc[M_NB];
+        const uint64_t nres = d->h_ctr[C_RESULTS];
+        bool again = false;
+        if (n_items > d->cap_items) { d->cap_items = n_items + n_items / 4 + 1024; again = true; }
+        if (nB > d->cap_bounds) { d->cap_bounds = nB + nB / 4 + 1024; again = true; }
+        if (nres > d->cap_out) { d->cap_out = (uint32_t)std::min<uint64_t>(nres + nres / 2 + 1024, UINT32_MAX / 16); again = true; }
+        if (!again) break;
     }
-    hipLaunchKernelGGL(k_list_offsets, dim3(grid_for((int64_t)nE + 1, 256)), dim3(256), 0, st, skeys, nE,
-                       (uint32_t)d->n_terms, loff);
-    HIPCK(hipGetLastError());
-    HIPCK(hipEventRecord(d->ev[1], st));
-
-    // 6-8: plan work items
-    ENSURE(d->b_lmin, ((size_t)d->max_len + 1) * 4);
-    ENSURE(d->b_hi, (size_t)n * 4);
-    ENSURE(d->b_eq, (size_t)n * 4);
-    ENSURE(d->b_nparts, (size_t)n * 4);
-    ENSURE(d->b_partoff, (size_t)(n + 1) * 4);
-    ENSURE(d->b_bcnt, (size_t)n * 4);
-    ENSURE(d->b_bofs, (size_t)(n + 1) * 4);
-    uint32_t *bcnt = P<uint32_t>(d->b_bcnt), *bofs = P<uint32_t>(d->b_bofs);
-    uint32_t *lmin = P<uint32_t>(d->b_lmin), *hi = P<uint32_t>(d->b_hi), *nparts = P<uint32_t>(d->b_nparts),
-             *partoff = P<uint32_t>(d->b_partoff);
-    hipLaunchKernelGGL(k_lmin, dim3(grid_for(d->max_len + 1, 256)), dim3(256), 0, st, lmin, d->max_len, t);
-    HIPCK(hipGetLastError());
-    ENSURE(d->b_rng, (size_t)std::max<uint32_t>(nP, 1) * sizeof(uint2));
-    hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 4096)), dim3(256), 0, st,
-                       d->tok, d->pos, d->len, n, probelen, poff, lmin, loff, psid, fence, target, P<uint2>(d->b_rng), hi,
-                       nparts, P<uint32_t>(d->b_eq), bcnt);
-    HIPCK(hipGetLastError());
-    HIPCK(exclusive_scan<uint32_t>(nparts, partoff, n, misc + 1, P<uint32_t>(d->b_scanscr), st));
-    HIPCK(exclusive_scan<uint32_t>(bcnt, bofs, n, misc + 3, P<uint32_t>(d->b_scanscr), st));
-    uint32_t n_items = 0, nB = 0;
-    const unsigned long long probe_tokens = nP;
-    HIPCK(hipMemcpyAsync(&n_items, misc + 1, 4, hipMemcpyDeviceToHost, st));
-    HIPCK(hipMemcpyAsync(&nB, misc + 3, 4, hipMemcpyDeviceToHost, st));
-    HIPCK(hipStreamSynchronize(st));
-    trace_mark("after_plan");
-    ENSURE(d->b_bounds, (size_t)std::max<uint32_t>(nB, 1) * 4);
-    if (nB > 0) {
-        hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 8192)), dim3(256), 0,
-                           st, n, probelen, poff, P<uint2>(d->b_rng), hi, nparts, bofs, psid, fence,
-                           P<uint32_t>(d->b_bounds));
-        HIPCK(hipGetLastError());
+    T.graph = use_graph;
+    T.launches = 1;
+    T.index_entries = d->h_misc[M_NE];
+    T.probe_tokens = d->h_misc[M_NP];
+    if (!use_graph) {
+        float ms = 0;
+        HIPCK(hipEventElapsedTime(&ms, d->ev[0], d->ev[1]));
+        T.ms_index = ms;
+        HIPCK(hipEventElapsedTime(&ms, d->ev[1], d->ev[5]));
+        T.ms_join = ms;
+        HIPCK(hipEventElapsedTime(&ms, d->ev[4], d->ev[5]));
+        T.ms_kernel = ms;
     }
-    ENSURE(d->b_items, (size_t)std::max<uint32_t>(n_items, 1) * sizeof(Item));
-    Item *items = P<Item>(d->b_items);
-    hipLaunchKernelGGL(k_items, dim3(grid_for(n, 256)), dim3(256), 0, st, hi, nparts, P<uint32_t>(d->b_eq), partoff, bofs,
-                       n, items);
-    HIPCK(hipGetLastError());
+    for (int k = C_ESCAN; k <= C_CYC_RANGES; k++) T.acc[k] += d->h_ctr[k];
+    uint64_t nres = d->h_ctr[C_RESULTS], novf = d->h_ctr[C_OVERFLOW];
+    tri.resize(3 * (size_t)nres);
+    if (nres) HIPCK(hipMemcpy(tri.data(), d->b_out.p, (size_t)nres * 12, hipMemcpyDeviceToHost));
 
-    // 9: join; overflowed items are split into finer source windows and run again,
-    // and a launch whose result buffer was too small is redone with a larger one
-    ENSURE(d->b_ovf, (size_t)std::max<uint32_t>(n_items, 1) * 4);
-    if (d->b_out.cap < (size_t)65536 * 12) ENSURE(d->b_out, (size_t)65536 * 12);
-    uint64_t acc[C_COUNT] = {0};
-    double kernel_ms = 0;
-    int kernel_launches = 0;
-    const Item *cur_items = items;
-    uint32_t cur_n = n_items;
-    for (int round = 0; cur_n > 0; round++) {
-        if (round >= 48) return fgss_fail(FGSS_ERR_DEVICE, "internal: work-item splitting did not converge");
+    // overflowed work items: split each four ways and run them again
+    const Item *cur = P<Item>(d->b_items);
+    int round = 0;
+    while (novf > 0) {
+        if (++round > 48) return fgss_fail(FGSS_ERR_DEVICE, "internal: work-item splitting did not converge");
+        const uint32_t ways = 4, cnt = (uint32_t)(novf * ways);
+        int rc = ensure(d->b_items2, (size_t)cnt * sizeof(Item));
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_resplit, dim3(grid_for((int64_t)novf, 256)), dim3(256), 0, st, cur, P<uint32_t>(d->b_ovf),
+                           (uint32_t)novf, ways, P<Item>(d->b_items2));
+        HIPCK(hipGetLastError());
+        std::swap(d->b_items, d->b_items2);   // the parent list stays alive in b_items2
+        d->cap_gen++;
+        cur = P<Item>(d->b_items);
+        rc = ensure(d->b_ovf, (size_t)cnt * 4);
+        if (rc) return rc;
         for (;;) {
-            const uint32_t out_cap = (uint32_t)std::min<size_t>(d->b_out.cap / 12, UINT32_MAX);
-            JoinArgs A;
-            A.tok = d->tok; A.pos = d->pos; A.len = d->len;
-            A.loff = loff; A.post = post; A.psid = psid; A.fence = fence;
-            A.sig = d->sig; A.sig64 = d->sig64; A.sigbad = d->sigbad;
-            A.items = cur_items; A.n_items = cur_n; A.t = t;
-            A.rng = P<uint2>(d->b_rng); A.poff = poff; A.bounds = P<uint32_t>(d->b_bounds);
-            A.out = P<int32_t>(d->b_out); A.out_cap = out_cap; A.ovf = P<uint32_t>(d->b_ovf); A.ctr = ctr;
-            HIPCK(hipMemsetAsync(ctr + C_ESCAN, 0, (C_COUNT - C_ESCAN) * sizeof(uint64_t), st));
+            unsigned long long *ctr = P<unsigned long long>(d->b_ctr);
+            uint32_t *misc = P<uint32_t>(d->b_misc);
+            HIPCK(fill<unsigned long long>(ctr, C_COUNT, 0ull, st));
+            HIPCK(fill<uint32_t>(misc + M_NRESPLIT, 1, cnt, st));
+            JoinArgs A = join_args(d);
+            A.items = cur;
+            A.n_items_dev = misc + M_NRESPLIT;
+            A.n_items_cap = cnt;
+            A.nb_cap = UINT32_MAX;   // re-split items never read the boundary table
             HIPCK(hipEventRecord(d->ev[4], st));
-            int rc = run_join(d, A, cap);
+            rc = launch_join_cap(d, A, cfg.cap);
             if (rc) return rc;
             HIPCK(hipEventRecord(d->ev[5], st));
             HIPCK(hipMemcpyAsync(d->h_ctr, ctr, C_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
             HIPCK(hipStreamSynchronize(st));
-            trace_mark("after_join");
-            float kms = 0;
-            HIPCK(hipEventElapsedTime(&kms, d->ev[4], d->ev[5]));
-            kernel_ms += kms;
-            kernel_launches++;
-            if (d->h_ctr[C_RESULTS] <= out_cap) break;
-            ENSURE(d->b_out, (size_t)(d->h_ctr[C_RESULTS] + d->h_ctr[C_RESULTS] / 2) * 12);
+            T.launches++;
+            float ms = 0;
+            HIPCK(hipEventElapsedTime(&ms, d->ev[4], d->ev[5]));
+            T.ms_kernel += ms;
+            if (d->h_ctr[C_RESULTS] <= d->cap_out) break;
+            d->cap_out = (uint32_t)std::min<uint64_t>(d->h_ctr[C_RESULTS] * 2 + 1024, UINT32_MAX / 16);
+            rc = ensure(d->b_out, (size_t)d->cap_out * 12);
+            if (rc) return rc;
+            d->cap_gen++;
         }
-        for (int k = C_ESCAN; k <= C_SIG2; k++) acc[k] += d->h_ctr[k];
-        for (int k = C_CYC_SETUP; k <= C_CYC_RANGES; k++) acc[k] += d->h_ctr[k];
-        const uint64_t nres = d->h_ctr[C_RESULTS], novf = d->h_ctr[C_OVERFLOW];
+        for (int k = C_ESCAN; k <= C_CYC_RANGES; k++) T.acc[k] += d->h_ctr[k];
+        nres = d->h_ctr[C_RESULTS];
+        novf = d->h_ctr[C_OVERFLOW];
         const size_t old = tri.size();
         tri.resize(old + 3 * (size_t)nres);
-        if (nres) HIPCK(hipMemcpyAsync(tri.data() + old, d->b_out.p, (size_t)nres * 12, hipMemcpyDeviceToHost, st));
-        if (novf == 0) break;
-        const uint32_t ways = 4;
-        ENSURE(d->b_items2, (size_t)novf * ways * sizeof(Item));
-        hipLaunchKernelGGL(k_resplit, dim3(grid_for((int64_t)novf, 256)), dim3(256), 0, st, cur_items,
-                           P<uint32_t>(d->b_ovf), (uint32_t)novf, ways, P<Item>(d->b_items2));
-        HIPCK(hipGetLastError());
-        std::swap(d->b_items, d->b_items2);   // the old list stays allocated in b_items2
-        cur_items = P<Item>(d->b_items);
-        cur_n = (uint32_t)(novf * ways);
-        ENSURE(d->b_ovf, (size_t)cur_n * 4);
+        if (nres) HIPCK(hipMemcpy(tri.data() + old, d->b_out.p, (size_t)nres * 12, hipMemcpyDeviceToHost));
     }
-    HIPCK(hipEventRecord(d->ev[3], st));
-    HIPCK(hipEventSynchronize(d->ev[3]));
-    trace_mark("end");
-    stats.probe_tokens = (int64_t)probe_tokens;
-    stats.postings_scanned = (int64_t)acc[C_ESCAN];
-    stats.candidates = (int64_t)acc[C_CAND];
-    stats.intersections = (int64_t)acc[C_INTER];
-    stats.verify_tokens = (int64_t)acc[C_VTOK];
-    stats.signature_checks = (int64_t)acc[C_SIGCHK];
-    stats.verified = (int64_t)acc[C_VERIFIED];
-    stats.signature_checks_128 = (int64_t)acc[C_SIG2];
-    for (int k = 0; k < 6; k++) stats.phase_cycles[k] = (int64_t)acc[C_CYC_SETUP + k];
-    stats.results = (int64_t)(tri.size() / 3);
-    float ms = 0;
-    HIPCK(hipEventElapsedTime(&ms, d->ev[0], d->ev[1]));
-    stats.ms_index = ms;
-    HIPCK(hipEventElapsedTime(&ms, d->ev[1], d->ev[3]));
-    stats.ms_join = ms;
-    stats.ms_kernel = kernel_ms;
-    stats.kernel_launches = kernel_launches;
+    // per-job counters (overflowed items never flushed theirs)
+    HIPCK(hipMemcpy(d->h_jctr, d->b_jctr.p, (size_t)d->njobs * kJobCtr * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < T.jctr.size(); k++) T.jctr[k] = d->h_jctr[k];
+    trace_mark("collect");
     return FGSS_OK;
 }
 
@@ -1165,10 +1370,10 @@ static void sort_pairs_by_key(const std::vector<int32_t> &tri, std::vector<uint6
         size_t cnt[2049] = {0};
         for (size_t i = 0; i < R; i++) cnt[((key[i] >> shift) & 2047) + 1]++;
         bool constant = false;
-        for (int d = 1; d <= 2048; d++)
-            if (cnt[d] == R) constant = true;
+        for (int dd = 1; dd <= 2048; dd++)
+            if (cnt[dd] == R) constant = true;
         if (constant) continue;
-        for (int d = 0; d < 2048; d++) cnt[d + 1] += cnt[d];
+        for (int dd = 0; dd < 2048; dd++) cnt[dd + 1] += cnt[dd];
         for (size_t i = 0; i < R; i++) {
             size_t at = cnt[(key[i] >> shift) & 2047]++;
             k2[at] = key[i];
@@ -1180,27 +1385,14 @@ static void sort_pairs_by_key(const std::vector<int32_t> &tri, std::vector<uint6
     out.swap(idx);
 }
 
-extern "C" int fgss_selfjoin(fgss_collection *c, float threshold, const fgss_join_opts *opts, fgss_pairs **out)
+// Build a result object from triples (all with query index in [q0, q1) of the
+// device record set, whose records q0.. are the records 0.. of collection c).
+static int make_pairs(const fgss_collection *c, const std::vector<int32_t> &tri, const std::vector<uint64_t> &order,
+                      size_t first, size_t count, int64_t q0, fgss_pairs **out)
 {
-    if (!c || !out) return fgss_fail(FGSS_ERR_ARG, "null argument");
-    *out = nullptr;
-    if (!(threshold > 0.0f) || !(threshold < 1.0f))   // fgssjoin_cli.cu:32-35
-        return fgss_fail(FGSS_ERR_ARG, "Threshold must be a float in the range (0.0..1.0).");
-    auto t0 = std::chrono::steady_clock::now();
-    int rc = fgss_collection_upload(c);
-    if (rc) return rc;
-    std::vector<int32_t> tri;
-    fgss_join_stats stats;
-    auto t1 = std::chrono::steady_clock::now();
-    rc = selfjoin_device(c, threshold, opts, tri, stats);
-    if (rc) return rc;
-    auto t2 = std::chrono::steady_clock::now();
-    const int64_t R = (int64_t)(tri.size() / 3);
-    // sort by (q, s); the device emits in completion order
-    std::vector<uint64_t> order;
-    sort_pairs_by_key(tri, order);
     auto *p = (fgss_pairs *)calloc(1, sizeof(fgss_pairs));
-    const size_t nb = (size_t)std::max<int64_t>(R, 1);
+    if (!p) return fgss_fail(FGSS_ERR_NOMEM, "out of host memory");
+    const size_t nb = std::max<size_t>(count, 1);
     p->q = (int32_t *)malloc(nb * 4);
     p->s = (int32_t *)malloc(nb * 4);
     p->rid = (int32_t *)malloc(nb * 4);
@@ -1211,9 +1403,9 @@ extern "C" int fgss_selfjoin(fgss_collection *c, float threshold, const fgss_joi
         fgss_pairs_free(p);
         return fgss_fail(FGSS_ERR_NOMEM, "out of host memory");
     }
-    for (int64_t k = 0; k < R; k++) {
-        int64_t i = (int64_t)(order[(size_t)k] & 0xFFFFFFFFull);
-        int32_t q = tri[3 * i], s = tri[3 * i + 1], ov = tri[3 * i + 2];
+    for (size_t k = 0; k < count; k++) {
+        const size_t i = (size_t)(order[first + k] & 0xFFFFFFFFull);
+        const int32_t q = (int32_t)(tri[3 * i] - q0), s = (int32_t)(tri[3 * i + 1] - q0), ov = tri[3 * i + 2];
         p->q[k] = q;
         p->s[k] = s;
         p->rid[k] = c->id[(size_t)q];
@@ -1221,13 +1413,70 @@ extern "C" int fgss_selfjoin(fgss_collection *c, float threshold, const fgss_joi
         p->overlap[k] = ov;
         p->similarity[k] = fgss_similarity((uint32_t)c->len[(size_t)q], (uint32_t)c->len[(size_t)s], (uint32_t)ov);
     }
-    p->n_pairs = R;
+    p->n_pairs = (int64_t)count;
+    *out = p;
+    return FGSS_OK;
+}
+
+static void fill_stats(const RunTotals &T, size_t job, fgss_join_stats &st)
+{
+    memset(&st, 0, sizeof(st));
+    st.index_entries = T.index_entries;
+    st.probe_tokens = T.probe_tokens;
+    st.postings_scanned = (int64_t)T.jctr[job * kJobCtr + 0];
+    st.candidates = (int64_t)T.jctr[job * kJobCtr + 1];
+    st.intersections = (int64_t)T.jctr[job * kJobCtr + 2];
+    st.results = (int64_t)T.jctr[job * kJobCtr + 3];
+    st.verify_tokens = (int64_t)T.acc[C_VTOK];
+    st.signature_checks = (int64_t)T.acc[C_SIGCHK];
+    st.signature_checks_128 = (int64_t)T.acc[C_SIG2];
+    st.verified = (int64_t)T.acc[C_VERIFIED];
+    for (int k = 0; k < 6; k++) st.phase_cycles[k] = (int64_t)T.acc[C_CYC_SETUP + k];
+    st.ms_index = T.ms_index;
+    st.ms_join = T.ms_join;
+    st.ms_kernel = T.ms_kernel;
+    st.kernel_launches = T.launches;
+}
+
+static int set_thresholds(fgss_device_sets *d, const float *t, uint32_t njobs)
+{
+    HIPCK(hipMemcpyAsync(d->tj, t, (size_t)njobs * sizeof(float), hipMemcpyHostToDevice, d->st));
+    HIPCK(hipStreamSynchronize(d->st));
+    return FGSS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fgss_selfjoin
+// ---------------------------------------------------------------------------
+extern "C" int fgss_selfjoin(fgss_collection *c, float threshold, const fgss_join_opts *opts, fgss_pairs **out)
+{
+    if (!c || !out) return fgss_fail(FGSS_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (!(threshold > 0.0f) || !(threshold < 1.0f))   // fgssjoin_cli.cu:32-35
+        return fgss_fail(FGSS_ERR_ARG, "Threshold must be a float in the range (0.0..1.0).");
+    if (!c->jend.empty()) return fgss_fail(FGSS_ERR_ARG, "a batch collection is run with fgss_batch_run");
+    const RunCfg cfg = make_cfg(opts);
+    int rc = fgss_collection_upload(c);
+    if (rc) return rc;
+    auto t1 = std::chrono::steady_clock::now();
+    fgss_device_sets *d = c->dev;
+    rc = set_thresholds(d, &threshold, 1);
+    if (rc) return rc;
+    std::vector<int32_t> tri;
+    RunTotals T;
+    rc = run_pipeline(d, cfg, false, tri, T);
+    if (rc) return rc;
+    auto t2 = std::chrono::steady_clock::now();
+    std::vector<uint64_t> order;
+    sort_pairs_by_key(tri, order);
+    fgss_pairs *p = nullptr;
+    rc = make_pairs(c, tri, order, 0, order.size(), 0, &p);
+    if (rc) return rc;
     trace_mark("host_pairs");
     auto t3 = std::chrono::steady_clock::now();
-    stats.ms_copy = std::chrono::duration<double, std::milli>(t3 - t2).count();
-    stats.ms_total = std::chrono::duration<double, std::milli>(t3 - t1).count();
-    (void)t0;
-    p->stats = stats;
+    fill_stats(T, 0, p->stats);
+    p->stats.ms_copy = std::chrono::duration<double, std::milli>(t3 - t2).count();
+    p->stats.ms_total = std::chrono::duration<double, std::milli>(t3 - t1).count();
     *out = p;
     return FGSS_OK;
 }
@@ -1258,5 +1507,170 @@ extern "C" int fgss_pairs_print(const fgss_pairs *p, void *file)
 extern "C" int fgss_device_synchronize(void)
 {
     HIPCK(hipDeviceSynchronize());
+    return FGSS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batches: many independent self-joins (own collection, own threshold) run as
+// one pipeline over the concatenated records -- one launch of every kernel for
+// the whole batch, optionally replayed from a captured HIP graph
+// ---------------------------------------------------------------------------
+struct fgss_batch {
+    std::vector<const fgss_collection *> colls;
+    std::vector<float> t;
+    std::vector<int64_t> rec_begin;   // first record of each job in the merged set
+    fgss_collection merged;
+    std::vector<int32_t> tri;         // last run: triples sorted by (q, s), merged indices
+    std::vector<uint64_t> order;
+    std::vector<size_t> job_first;    // last run: first sorted triple of each job
+    RunTotals totals;
+    bool has_run = false;
+};
+
+extern "C" int fgss_batch_create(fgss_collection *const *colls, const float *thresholds, int32_t njobs,
+                                 fgss_batch **out)
+{
+    if (!colls || !thresholds || !out || njobs < 1) return fgss_fail(FGSS_ERR_ARG, "null argument or no jobs");
+    *out = nullptr;
+    auto *b = new fgss_batch();
+    fgss_collection &m = b->merged;
+    int64_t nrec = 0, ntok = 0;
+    for (int32_t j = 0; j < njobs; j++) {
+        if (!colls[j]) { delete b; return fgss_fail(FGSS_ERR_ARG, "null collection for job %d", j); }
+        if (!(thresholds[j] > 0.0f) || !(thresholds[j] < 1.0f)) {
+            delete b;
+            return fgss_fail(FGSS_ERR_ARG, "Threshold must be a float in the range (0.0..1.0).");
+        }
+        nrec += (int64_t)colls[j]->len.size();
+        ntok += (int64_t)colls[j]->tok.size();
+    }
+    if (nrec >= (int64_t)INT32_MAX || ntok >= (int64_t)UINT32_MAX) {
+        delete b;
+        return fgss_fail(FGSS_ERR_UNSUPPORTED, "batch exceeds 2^31 records or 2^32 tokens");
+    }
+    m.id.reserve((size_t)nrec);
+    m.len.reserve((size_t)nrec);
+    m.tok.reserve((size_t)ntok);
+    m.job.reserve((size_t)nrec);
+    int64_t term_off = 0;
+    for (int32_t j = 0; j < njobs; j++) {
+        const fgss_collection *c = colls[j];
+        b->colls.push_back(c);
+        b->t.push_back(thresholds[j]);
+        b->rec_begin.push_back((int64_t)m.len.size());
+        m.id.insert(m.id.end(), c->id.begin(), c->id.end());
+        m.len.insert(m.len.end(), c->len.begin(), c->len.end());
+        for (int32_t v : c->tok) m.tok.push_back((int32_t)(v + term_off));   // private term range per job
+        m.job.insert(m.job.end(), c->len.size(), (uint32_t)j);
+        m.jend.push_back((uint32_t)m.len.size());
+        term_off += c->n_terms;
+    }
+    if (term_off >= (int64_t)INT32_MAX) {
+        delete b;
+        return fgss_fail(FGSS_ERR_UNSUPPORTED, "batch exceeds 2^31 distinct terms");
+    }
+    b->rec_begin.push_back((int64_t)m.len.size());
+    fgss_finish_collection(&m);
+    m.n_terms = (int32_t)term_off;
+    int rc = fgss_collection_upload(&m);
+    if (rc) { fgss_device_release(&m); delete b; return rc; }
+    *out = b;
+    return FGSS_OK;
+}
+
+extern "C" void fgss_batch_free(fgss_batch *b)
+{
+    if (!b) return;
+    fgss_device_release(&b->merged);
+    delete b;
+}
+
+extern "C" int fgss_batch_set_thresholds(fgss_batch *b, const float *thresholds)
+{
+    if (!b || !thresholds) return fgss_fail(FGSS_ERR_ARG, "null argument");
+    for (size_t j = 0; j < b->t.size(); j++) {
+        if (!(thresholds[j] > 0.0f) || !(thresholds[j] < 1.0f))
+            return fgss_fail(FGSS_ERR_ARG, "Threshold must be a float in the range (0.0..1.0).");
+        b->t[j] = thresholds[j];
+    }
+    return FGSS_OK;
+}
+
+extern "C" int fgss_batch_run(fgss_batch *b, const fgss_join_opts *opts, int32_t use_graph, fgss_batch_stats *stats)
+{
+    if (!b) return fgss_fail(FGSS_ERR_ARG, "null batch");
+    auto t0 = std::chrono::steady_clock::now();
+    fgss_device_sets *d = b->merged.dev;
+    const RunCfg cfg = make_cfg(opts);
+    int rc = set_thresholds(d, b->t.data(), (uint32_t)b->t.size());
+    if (rc) return rc;
+    rc = run_pipeline(d, cfg, use_graph != 0, b->tri, b->totals);
+    if (rc) return rc;
+    auto t1 = std::chrono::steady_clock::now();
+    sort_pairs_by_key(b->tri, b->order);
+    // jobs are contiguous record ranges, so the sorted triples are grouped by job
+    const size_t nj = b->colls.size();
+    b->job_first.assign(nj + 1, b->order.size());
+    size_t k = 0;
+    for (size_t j = 0; j < nj; j++) {
+        while (k < b->order.size() && b->tri[3 * (size_t)(b->order[k] & 0xFFFFFFFFull)] < b->rec_begin[j]) k++;
+        b->job_first[j] = k;
+    }
+    b->has_run = true;
+    auto t2 = std::chrono::steady_clock::now();
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->ms_total = std::chrono::duration<double, std::milli>(t2 - t0).count();
+        stats->ms_pipeline = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        stats->n_pairs = (int64_t)b->order.size();
+        for (size_t j = 0; j < nj; j++) stats->candidates += (int64_t)b->totals.jctr[j * kJobCtr + 1];
+        stats->graph = b->totals.graph ? 1 : 0;
+        stats->kernel_launches = b->totals.launches;
+    }
+    return FGSS_OK;
+}
+
+extern "C" int fgss_batch_pairs(fgss_batch *b, int32_t job, fgss_pairs **out)
+{
+    if (!b || !out) return fgss_fail(FGSS_ERR_ARG, "null argument");
+    if (!b->has_run) return fgss_fail(FGSS_ERR_ARG, "the batch has not been run");
+    if (job < 0 || (size_t)job >= b->colls.size()) return fgss_fail(FGSS_ERR_ARG, "job %d out of range", job);
+    const size_t first = b->job_first[(size_t)job], last = b->job_first[(size_t)job + 1];
+    int rc = make_pairs(b->colls[(size_t)job], b->tri, b->order, first, last - first, b->rec_begin[(size_t)job], out);
+    if (rc) return rc;
+    fill_stats(b->totals, (size_t)job, (*out)->stats);
+    return FGSS_OK;
+}
+
+// Autotuner: time the batch under each (table slots, item load) choice, replaying
+// captured graphs, and return the fastest.  reps runs per choice, the median counts.
+extern "C" int fgss_batch_autotune(fgss_batch *b, int32_t reps, fgss_join_opts *best, double *best_ms)
+{
+    if (!b || !best) return fgss_fail(FGSS_ERR_ARG, "null argument");
+    if (reps < 1) reps = 3;
+    const int slots[] = {512, 1024};
+    const int loads[] = {50, 65, 80};
+    double bms = 1e300;
+    fgss_join_opts bo{};
+    for (int s : slots)
+        for (int l : loads) {
+            fgss_join_opts o{};
+            o.table_slots = s;
+            o.load_pct = l;
+            fgss_batch_stats st;
+            int rc = fgss_batch_run(b, &o, 1, &st);   // capture (and warm up)
+            if (rc) return rc;
+            std::vector<double> ms;
+            for (int r = 0; r < reps; r++) {
+                rc = fgss_batch_run(b, &o, 1, &st);
+                if (rc) return rc;
+                ms.push_back(st.ms_total);
+            }
+            std::sort(ms.begin(), ms.end());
+            const double med = ms[ms.size() / 2];
+            if (med < bms) { bms = med; bo = o; }
+        }
+    *best = bo;
+    if (best_ms) *best_ms = bms;
     return FGSS_OK;
 }

@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 namespace fgss {
 
 constexpr int kWave = 64;
@@ -32,6 +34,24 @@ __device__ __forceinline__ T wave_sum(T v)
 #pragma unroll
     for (int d = kWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWave);
     return v;
+}
+
+// dst[i] = v for i < n (used instead of memset so that the work can be captured
+// into a HIP graph)
+template <class T>
+__global__ void k_fill(T *__restrict__ dst, int64_t n, T v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = v;
+}
+
+template <class T>
+inline hipError_t fill(T *dst, int64_t n, T v, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill<T>, dim3(blocks), dim3(256), 0, st, dst, n, v);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -127,10 +147,7 @@ inline int64_t scan_scratch_elems(int64_t n)
 template <class T>
 hipError_t exclusive_scan(const T *in, T *out, int64_t n, T *total, T *scratch, hipStream_t st)
 {
-    if (n <= 0) {
-        if (total) return hipMemsetAsync(total, 0, sizeof(T), st);
-        return hipSuccess;
-    }
+    if (n <= 0) return total ? fill<T>(total, 1, (T)0, st) : hipSuccess;
     const int64_t nb = (n + kScanTile - 1) / kScanTile;
     if (nb == 1) {
         hipLaunchKernelGGL(k_scan_tile<T>, dim3(1), dim3(kScanThreads), 0, st, in, out, n, (const T *)nullptr, total);
@@ -155,9 +172,17 @@ constexpr int kSortWaveSpan = kWave * kSortItems;     // 1024 consecutive elemen
 constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 elements per block
 
 // hist[d * ntiles + tile] = number of keys in the tile with digit d
-__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t *__restrict__ keys, int64_t n, int shift,
+// n_dev, when given, holds the element count (<= n, the capacity the grid was sized for)
+__device__ __forceinline__ int64_t dev_count(int64_t n, const uint32_t *n_dev)
+{
+    return n_dev ? min(n, (int64_t)*n_dev) : n;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t *__restrict__ keys, int64_t n,
+                                                             const uint32_t *__restrict__ n_dev, int shift,
                                                              uint32_t *__restrict__ hist, int ntiles)
 {
+    n = dev_count(n, n_dev);
     __shared__ uint32_t h[kSortWaves][256];
     const int w = (int)threadIdx.x / kWave;
     for (int i = threadIdx.x; i < kSortWaves * 256; i += kSortThreads) (&h[0][0])[i] = 0;
@@ -180,11 +205,13 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t *__r
 // Stable scatter.  offs[d * ntiles + tile] = exclusive-scanned histogram.
 // If idx_in is null the index of element i is i itself.
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t *__restrict__ keys_in,
-                                                                const uint32_t *__restrict__ idx_in, int64_t n, int shift,
+                                                                const uint32_t *__restrict__ idx_in, int64_t n,
+                                                                const uint32_t *__restrict__ n_dev, int shift,
                                                                 const uint32_t *__restrict__ offs, int ntiles,
                                                                 uint32_t *__restrict__ keys_out,
                                                                 uint32_t *__restrict__ idx_out)
 {
+    n = dev_count(n, n_dev);
     __shared__ uint32_t h[kSortWaves][256];
     const int w = (int)threadIdx.x / kWave;
     for (int i = threadIdx.x; i < kSortWaves * 256; i += kSortThreads) (&h[0][0])[i] = 0;
@@ -244,9 +271,10 @@ inline int64_t radix_scratch_elems(int64_t n)
 
 // Sort n (key, index) pairs by the low `bits` bits of key, stably; indices start as 0..n-1.
 // Result in (keys_b, idx_b) after an odd number of passes, else (keys_a, idx_a); the
-// function returns which: 0 -> a, 1 -> b.  keys_a holds the input keys.
+// function returns which: 0 -> a, 1 -> b.  keys_a holds the input keys.  With n_dev the
+// element count is read on the device (n is then the capacity the launch is sized for).
 inline hipError_t radix_sort_pairs(uint32_t *keys_a, uint32_t *idx_a, uint32_t *keys_b, uint32_t *idx_b, int64_t n,
-                                   int bits, uint32_t *scratch, hipStream_t st, int *which)
+                                   const uint32_t *n_dev, int bits, uint32_t *scratch, hipStream_t st, int *which)
 {
     *which = 0;
     if (n <= 0) return hipSuccess;
@@ -263,11 +291,11 @@ inline hipError_t radix_sort_pairs(uint32_t *keys_a, uint32_t *idx_a, uint32_t *
         const uint32_t *ii = p == 0 ? nullptr : (even ? idx_a : idx_b);
         uint32_t *ko = even ? keys_b : keys_a;
         uint32_t *io = even ? idx_b : idx_a;
-        hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kSortThreads), 0, st, ki, n, 8 * p, hist, ntiles);
+        hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kSortThreads), 0, st, ki, n, n_dev, 8 * p, hist, ntiles);
         hipError_t e = exclusive_scan<uint32_t>(hist, hist, (int64_t)ntiles * 256, nullptr, sscr, st);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kSortThreads), 0, st, ki, ii, n, 8 * p, hist, ntiles,
-                           ko, io);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kSortThreads), 0, st, ki, ii, n, n_dev, 8 * p, hist,
+                           ntiles, ko, io);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
