@@ -62,6 +62,51 @@ def cpu_baseline(coll, gpu_pairs):
                       f"oracle/fgss_oracle.c closed form, {threads} threads"}, exact
 
 
+def batched_leg(fg, n_jobs: int, reps: int):
+    """512 independent self-joins (titles, sizes log-uniform in 500..50k, per-job seed
+    4000+j, t in {0.50, 0.55, ..., 0.95}): one batched pipeline run (autotuned, replayed
+    HIP graph) vs the same joins as sequential plain launches."""
+    import numpy as np
+    rng = np.random.default_rng(4)
+    sizes = np.exp(rng.uniform(np.log(500), np.log(50000), n_jobs)).astype(int)
+    ts = [float(np.float32(x)) for x in rng.choice(np.round(np.arange(0.5, 0.951, 0.05), 2), n_jobs)]
+    t0 = time.perf_counter()
+    colls = [fg.Collection.generate(fg.GEN_TITLES, int(n), 4000 + j) for j, n in enumerate(sizes)]
+    log(f"batched: generated {n_jobs} collections, {int(sizes.sum())} records in {time.perf_counter() - t0:.1f} s")
+    for c in colls:
+        c.upload()
+    # plain: one launch sequence per join, results in host memory
+    for c, t in zip(colls, ts):   # warm: every collection's workspace is allocated before timing
+        c.selfjoin(t)
+    fg.device_synchronize()
+    t0 = time.perf_counter()
+    plain = [c.selfjoin(t) for c, t in zip(colls, ts)]
+    plain_s = time.perf_counter() - t0
+    log(f"batched: {n_jobs} sequential plain joins in {plain_s * 1e3:.1f} ms")
+    b = fg.Batch(colls, ts)
+    tune = b.autotune(reps=2)
+    b.run(graph=True, table_slots=tune["table_slots"], load_pct=tune["load_pct"])
+    times = []
+    exact = True
+    for r in range(reps):
+        t0 = time.perf_counter()
+        b.run(graph=True, table_slots=tune["table_slots"], load_pct=tune["load_pct"])
+        got = [b.pairs(j) for j in range(n_jobs)]
+        times.append(time.perf_counter() - t0)
+        if r == 0:
+            for g, p in zip(got, plain):
+                exact &= (np.array_equal(g.q, p.q) and np.array_equal(g.s, p.s)
+                          and np.array_equal(g.overlap, p.overlap))
+    times.sort()
+    batch_s = times[len(times) // 2]
+    log(f"batched: one graph-replayed batch run in {batch_s * 1e3:.1f} ms (median of {reps}), tune {tune}")
+    b.close()
+    return {"jobs": n_jobs, "records": int(sizes.sum()),
+            "plain_sequential_ms": plain_s * 1e3, "batched_ms": batch_s * 1e3,
+            "plain_joins_per_s": n_jobs / plain_s, "batched_joins_per_s": n_jobs / batch_s,
+            "speedup": plain_s / batch_s, "autotune": tune, "bit_exact_vs_plain": bool(exact)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +115,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--table-slots", type=int, default=0)
     ap.add_argument("--load-pct", type=int, default=0)
+    ap.add_argument("--batch-jobs", type=int, default=512, help="jobs of the batched leg (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the join does not shard across GPUs "
@@ -137,6 +183,8 @@ def main():
                      "kernel": "k_join", "kernel_ms_per_step": kms / args.steps,
                      "bytes_per_step": kbytes / args.steps},
     }
+    if args.batch_jobs > 0:
+        out["batched"] = batched_leg(fg, args.batch_jobs, reps=3)
     if not args.no_cpu:
         log("cpu baseline: running the oracle on the same step ...")
         cb, exact = cpu_baseline(coll, last)
