@@ -70,6 +70,11 @@ int fgss_collection_from_bin_buffer(const int32_t *ints, int64_t n_ints, fgss_co
  * among non-empty lines; lines shorter than q chars yield no record. */
 int fgss_collection_from_text(const char *path, int q, fgss_collection **out);
 int fgss_collection_from_text_buffer(const char *text, int64_t nbytes, int q, fgss_collection **out);
+/* Tokenize text into the token-id space of dict (a collection built from text):
+ * q-grams dict knows take its ids, others get new ids from dict's n_terms on (in
+ * order of first occurrence).  Used for probe records of R-S joins. */
+int fgss_collection_from_text_dict(const char *text, int64_t nbytes, const fgss_collection *dict,
+                                   fgss_collection **out);
 
 /* Write the collection as a reference .bin (records by size ascending, i.e.
  * internal order reversed) and, when loc_path is non-null and the collection
@@ -179,6 +184,52 @@ int fgss_batch_pairs(fgss_batch *b, int32_t job, fgss_pairs **out);
  * the fastest options (graph replays, median of reps). */
 int fgss_batch_autotune(fgss_batch *b, int32_t reps, fgss_join_opts *best, double *best_ms);
 void fgss_batch_free(fgss_batch *b);
+
+/* ------------------------------------------------------------------ */
+/* Serving: R-S joins against a resident index, continuously batched   */
+/* (no reference counterpart; its "dualjoin" mode is only a usage      */
+/* string, src/preprocess_cli.cpp:26)                                  */
+/* ------------------------------------------------------------------ */
+typedef struct fgss_server fgss_server;
+
+typedef struct {
+    double ms_queue;          /* submit -> launch of the run that carried it     */
+    double ms_latency;        /* submit -> its results were on the host          */
+    int32_t batch_requests;   /* requests that shared that run (itself included) */
+    int32_t reserved;
+} fgss_request_stats;
+
+/* Index `indexed` (a plain collection in internal order, lengths non-increasing)
+ * once for every threshold >= t_min: each record's first max_prefix(|s|, t_min)
+ * tokens.  The server keeps a private device copy; the caller keeps `indexed`
+ * alive while results are read (record ids come from it).  `slots` runs may be in
+ * flight at once, each on its own HIP stream; `budget_bytes` (<= 0: 1 GiB) caps
+ * the candidate-memory estimate of the requests in flight. */
+int fgss_server_create(const fgss_collection *indexed, float t_min, int32_t slots, int64_t budget_bytes,
+                       const fgss_join_opts *opts, fgss_server **out);
+/* Queue an R-S join of `probes` (a plain collection in the index's token space,
+ * e.g. from fgss_collection_from_text_dict; any length order) at threshold
+ * t in [t_min, 1).  Pending requests are admitted, first come first served, into
+ * idle slots, several per run, while their estimates fit the budget.  The caller
+ * keeps `probes` alive until fgss_server_result. */
+int fgss_server_submit(fgss_server *srv, const fgss_collection *probes, float t, int64_t *request_id);
+/* Retire completed runs and admit pending requests; with block set, first wait for
+ * the oldest run in flight.  Reports the requests still queued and running. */
+int fgss_server_poll(fgss_server *srv, int32_t block, int32_t *pending, int32_t *running);
+int fgss_server_done(fgss_server *srv, int64_t request_id, int32_t *done);
+/* Drive the server until the request is done (an error if it is queued while
+ * admission is held and nothing runs). */
+int fgss_server_wait(fgss_server *srv, int64_t request_id);
+/* Hold (1) or resume (0) admission: held, submitted requests only queue, so a
+ * caller can hand over a burst that is then batched as one. */
+int fgss_server_hold(fgss_server *srv, int32_t hold);
+/* Take a done request's pairs: q indexes `probes`, s the indexed collection
+ * (both internal order), sorted by (q, s); a pair (r, s) is a result iff the size
+ * filter holds, the max-prefixes of both records share a token, the positional
+ * filter passes, |R n S| >= floor(min_overlap) and the float32 Jaccard is >= t --
+ * the same whatever else ran in the same batch.  The request is then forgotten. */
+int fgss_server_result(fgss_server *srv, int64_t request_id, fgss_pairs **out, fgss_request_stats *stats);
+void fgss_server_free(fgss_server *srv);
 
 /* ------------------------------------------------------------------ */
 /* Synthetic workloads (no reference counterpart; used by the bench)  */

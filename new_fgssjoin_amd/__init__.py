@@ -12,7 +12,9 @@ Python mirror of the reference's host interface over the C ABI in
 
 Errors raise FgssError carrying the C status code; nothing here falls back to
 a CPU implementation: every join runs the HIP kernels of libfgssjoin.so, and
-importing the package fails when the library is missing.
+the first call into the package raises ImportError when the library is missing.
+Served R-S joins (``Server``) and batches (``Batch``) have no reference
+counterpart.
 """
 from __future__ import annotations
 
@@ -33,13 +35,15 @@ GEN_TITLES, GEN_LONG = 0, 1
 EXPORTS = (
     "fgss_last_error", "fgss_version",
     "fgss_collection_from_bin", "fgss_collection_from_bin_buffer",
-    "fgss_collection_from_text", "fgss_collection_from_text_buffer",
+    "fgss_collection_from_text", "fgss_collection_from_text_buffer", "fgss_collection_from_text_dict",
     "fgss_collection_write_bin", "fgss_collection_get_info", "fgss_collection_arrays",
     "fgss_collection_upload", "fgss_collection_free", "fgss_device_synchronize",
     "fgss_selfjoin", "fgss_pairs_free", "fgss_pairs_print",
     "fgss_generate_text", "fgss_free",
     "fgss_batch_create", "fgss_batch_set_thresholds", "fgss_batch_run", "fgss_batch_pairs",
     "fgss_batch_autotune", "fgss_batch_free",
+    "fgss_server_create", "fgss_server_submit", "fgss_server_poll", "fgss_server_done", "fgss_server_wait",
+    "fgss_server_result", "fgss_server_hold", "fgss_server_free",
 )
 
 
@@ -83,6 +87,14 @@ class BatchStats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class RequestStats(C.Structure):
+    _fields_ = [("ms_queue", C.c_double), ("ms_latency", C.c_double), ("batch_requests", C.c_int32),
+                ("reserved", C.c_int32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
+
+
 class _Pairs(C.Structure):
     _fields_ = [("n_pairs", C.c_int64), ("q", C.POINTER(C.c_int32)), ("s", C.POINTER(C.c_int32)),
                 ("rid", C.POINTER(C.c_int32)), ("sid", C.POINTER(C.c_int32)),
@@ -110,6 +122,7 @@ def lib():
         "fgss_collection_from_bin_buffer": [i32p, C.c_int64, C.POINTER(vp)],
         "fgss_collection_from_text": [C.c_char_p, C.c_int, C.POINTER(vp)],
         "fgss_collection_from_text_buffer": [C.c_char_p, C.c_int64, C.c_int, C.POINTER(vp)],
+        "fgss_collection_from_text_dict": [C.c_char_p, C.c_int64, vp, C.POINTER(vp)],
         "fgss_collection_write_bin": [vp, C.c_char_p, C.c_char_p],
         "fgss_collection_get_info": [vp, C.POINTER(CollectionInfo)],
         "fgss_collection_arrays": [vp, C.POINTER(i32p), C.POINTER(i32p), C.POINTER(i64p), C.POINTER(i32p)],
@@ -123,6 +136,13 @@ def lib():
         "fgss_batch_run": [vp, C.POINTER(JoinOpts), C.c_int32, C.POINTER(BatchStats)],
         "fgss_batch_pairs": [vp, C.c_int32, C.POINTER(C.POINTER(_Pairs))],
         "fgss_batch_autotune": [vp, C.c_int32, C.POINTER(JoinOpts), C.POINTER(C.c_double)],
+        "fgss_server_create": [vp, C.c_float, C.c_int32, C.c_int64, C.POINTER(JoinOpts), C.POINTER(vp)],
+        "fgss_server_submit": [vp, vp, C.c_float, i64p],
+        "fgss_server_poll": [vp, C.c_int32, i32p, i32p],
+        "fgss_server_done": [vp, C.c_int64, i32p],
+        "fgss_server_wait": [vp, C.c_int64],
+        "fgss_server_hold": [vp, C.c_int32],
+        "fgss_server_result": [vp, C.c_int64, C.POINTER(C.POINTER(_Pairs)), C.POINTER(RequestStats)],
     }.items():
         f = getattr(L, name)
         f.argtypes = args
@@ -135,6 +155,8 @@ def lib():
     L.fgss_free.restype = None
     L.fgss_batch_free.argtypes = [vp]
     L.fgss_batch_free.restype = None
+    L.fgss_server_free.argtypes = [vp]
+    L.fgss_server_free.restype = None
     _lib = L
     return L
 
@@ -217,6 +239,14 @@ class Collection:
     def from_text_bytes(cls, text: bytes, q: int) -> "Collection":
         h = C.c_void_p()
         _check(lib().fgss_collection_from_text_buffer(text, len(text), q, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_text_dict(cls, text: bytes, dictionary: "Collection") -> "Collection":
+        """Tokenize text into the token space of `dictionary` (a collection built from
+        text): probe records for Server.submit."""
+        h = C.c_void_p()
+        _check(lib().fgss_collection_from_text_dict(text, len(text), dictionary._h, C.byref(h)))
         return cls(h.value)
 
     @classmethod
@@ -343,4 +373,66 @@ class Batch:
             pass
 
 
-lib()
+class Server:
+    """R-S joins against a resident index with continuous batching.
+
+    `indexed` and every submitted probe collection must stay alive until their
+    results are taken (record ids come from them)."""
+
+    def __init__(self, indexed: Collection, t_min: float, slots: int = 2, budget_bytes: int = 0,
+                 table_slots: int = 0, load_pct: int = 0):
+        self.indexed = indexed
+        self._probes = {}
+        opts = JoinOpts(load_pct, table_slots, 0, 0)
+        h = C.c_void_p()
+        _check(lib().fgss_server_create(indexed._h, C.c_float(t_min), slots, budget_bytes, C.byref(opts),
+                                         C.byref(h)))
+        self._h = h
+
+    def submit(self, probes: Collection, t: float) -> int:
+        rid = C.c_int64()
+        _check(lib().fgss_server_submit(self._h, probes._h, C.c_float(t), C.byref(rid)))
+        self._probes[rid.value] = probes
+        return rid.value
+
+    def poll(self, block: bool = False):
+        pend, run = C.c_int32(), C.c_int32()
+        _check(lib().fgss_server_poll(self._h, 1 if block else 0, C.byref(pend), C.byref(run)))
+        return pend.value, run.value
+
+    def done(self, request_id: int) -> bool:
+        d = C.c_int32()
+        _check(lib().fgss_server_done(self._h, request_id, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self, request_id: int):
+        _check(lib().fgss_server_wait(self._h, request_id))
+
+    def hold(self, on: bool = True):
+        """Pause (or resume) admission: held requests queue and are then batched together."""
+        _check(lib().fgss_server_hold(self._h, 1 if on else 0))
+
+    def result(self, request_id: int, wait: bool = True):
+        """(Pairs, request stats) of a request; it is forgotten afterwards."""
+        if wait:
+            self.wait(request_id)
+        out = C.POINTER(_Pairs)()
+        st = RequestStats()
+        try:
+            _check(lib().fgss_server_result(self._h, request_id, C.byref(out), C.byref(st)))
+        finally:
+            self._probes.pop(request_id, None)
+        return _take_pairs(out), st.as_dict()
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().fgss_server_free(self._h)
+        self._h = None
+        self._probes.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

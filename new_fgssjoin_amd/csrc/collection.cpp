@@ -236,7 +236,12 @@ static uint64_t gram_hash(const unsigned char *g, int q, int occ)
 
 struct HashRef { uint64_t h; uint32_t idx; uint32_t pad; };
 
-extern "C" int fgss_collection_from_text_buffer(const char *text, int64_t nbytes, int q, fgss_collection **out)
+// Tokenize text.  Without dict, token ids are ranks by (document frequency, first
+// occurrence) and the collection keeps its dictionary (hash -> id).  With dict, the
+// q-grams take dict's ids; q-grams dict has never seen get new ids from dict's
+// n_terms on, in order of first occurrence (they can match no record of dict).
+static int tokenize_text(const char *text, int64_t nbytes, int q, const fgss_collection *dict,
+                         fgss_collection **out)
 {
     if (!out || (!text && nbytes > 0) || nbytes < 0) return fgss_fail(FGSS_ERR_ARG, "null argument");
     if (q < 1 || q > 1024) return fgss_fail(FGSS_ERR_ARG, "q must be in 1..1024");   // preprocess_cli.cpp:86
@@ -322,10 +327,39 @@ extern "C" int fgss_collection_from_text_buffer(const char *text, int64_t nbytes
         return hr[(size_t)gstart[a]].idx < hr[(size_t)gstart[b]].idx;
     });
     std::vector<int32_t> tid((size_t)total);
-    parallel_for(nd, 4096, [&](int64_t k) {
-        int64_t g = gorder[(size_t)k];
-        for (int64_t i = gstart[g]; i < gstart[g + 1]; i++) tid[hr[(size_t)i].idx] = (int32_t)k;
-    });
+    std::vector<uint64_t> dict_hash;
+    std::vector<int32_t> dict_id;
+    int64_t n_terms = nd;
+    if (!dict) {
+        parallel_for(nd, 4096, [&](int64_t k) {
+            int64_t g = gorder[(size_t)k];
+            for (int64_t i = gstart[g]; i < gstart[g + 1]; i++) tid[hr[(size_t)i].idx] = (int32_t)k;
+        });
+        dict_hash.resize((size_t)nd);
+        dict_id.resize((size_t)nd);
+        for (int64_t k = 0; k < nd; k++) {   // groups are in hash order
+            dict_hash[(size_t)k] = hr[(size_t)gstart[k]].h;
+            dict_id[(size_t)k] = 0;
+        }
+        for (int64_t k = 0; k < nd; k++) dict_id[(size_t)gorder[(size_t)k]] = (int32_t)k;
+    } else {
+        // unseen q-grams by first occurrence
+        std::vector<std::pair<uint32_t, int64_t>> unseen;
+        std::vector<int32_t> gid((size_t)nd, -1);
+        for (int64_t g = 0; g < nd; g++) {
+            const uint64_t h = hr[(size_t)gstart[g]].h;
+            auto it = std::lower_bound(dict->dict_hash.begin(), dict->dict_hash.end(), h);
+            if (it != dict->dict_hash.end() && *it == h) gid[(size_t)g] = dict->dict_id[(size_t)(it - dict->dict_hash.begin())];
+            else unseen.push_back({hr[(size_t)gstart[g]].idx, g});
+        }
+        std::sort(unseen.begin(), unseen.end());
+        for (size_t k = 0; k < unseen.size(); k++) gid[(size_t)unseen[k].second] = dict->n_terms + (int32_t)k;
+        n_terms = (int64_t)dict->n_terms + (int64_t)unseen.size();
+        if (n_terms >= INT32_MAX) return fgss_fail(FGSS_ERR_UNSUPPORTED, "too many distinct q-grams");
+        parallel_for(nd, 4096, [&](int64_t g) {
+            for (int64_t i = gstart[g]; i < gstart[g + 1]; i++) tid[hr[(size_t)i].idx] = gid[(size_t)g];
+        });
+    }
     std::vector<HashRef>().swap(hr);
 
     // records by size ascending (stable in line order); internal order = reversed
@@ -355,9 +389,26 @@ extern "C" int fgss_collection_from_text_buffer(const char *text, int64_t nbytes
             c->loc[(size_t)(c->pos[k] + i)] = v[(size_t)i].second;
         }
     });
-    c->n_terms = (int32_t)nd;
+    c->n_terms = (int32_t)n_terms;
+    c->q = q;
+    c->dict_hash.swap(dict_hash);
+    c->dict_id.swap(dict_id);
     *out = c;
     return FGSS_OK;
+}
+
+extern "C" int fgss_collection_from_text_buffer(const char *text, int64_t nbytes, int q, fgss_collection **out)
+{
+    return tokenize_text(text, nbytes, q, nullptr, out);
+}
+
+extern "C" int fgss_collection_from_text_dict(const char *text, int64_t nbytes, const fgss_collection *dict,
+                                              fgss_collection **out)
+{
+    if (!dict || !out) return fgss_fail(FGSS_ERR_ARG, "null argument");
+    if (dict->q < 1 || dict->dict_hash.empty())
+        return fgss_fail(FGSS_ERR_ARG, "the dictionary collection was not built from text");
+    return tokenize_text(text, nbytes, dict->q, dict, out);
 }
 
 extern "C" int fgss_collection_from_text(const char *path, int q, fgss_collection **out)

@@ -23,6 +23,9 @@ struct fgss_collection {
     std::vector<int32_t> loc;       // q-gram offsets (text front end only), parallel to tok
     std::vector<uint32_t> job;      // batches only: job id per record
     std::vector<uint32_t> jend;     // batches only: per job, one past its last record
+    std::vector<uint64_t> dict_hash;  // text front end: q-gram hashes, ascending
+    std::vector<int32_t> dict_id;     // ... and their token ids
+    int q = 0;                        // text front end: q-gram length
     int32_t n_terms = 0;
     int32_t max_len = 0;
     bool sorted_desc = true;

@@ -30,6 +30,10 @@
 
 #include <algorithm>
 #include <chrono>
+#include <deque>
+#include <functional>
+#include <map>
+#include <memory>
 #include <vector>
 
 #include "fgss_bounds.h"
@@ -262,14 +266,15 @@ static int validate_collection(const fgss_collection *c)
     return FGSS_OK;
 }
 
-extern "C" int fgss_collection_upload(fgss_collection *c)
+// Upload a collection into a new device record set (validated first).
+static int upload_records(fgss_device_sets **out, const fgss_collection *c)
 {
-    if (!c) return fgss_fail(FGSS_ERR_ARG, "null collection");
-    if (c->dev) return FGSS_OK;
+    *out = nullptr;
     int rc = validate_collection(c);
     if (rc) return rc;
     auto *d = new fgss_device_sets();
-    c->dev = d;
+    fgss_collection holder;   // releases d on failure
+    holder.dev = d;
     HIPCK(hipGetDevice(&d->device));
     HIPCK(hipStreamCreateWithFlags(&d->st, hipStreamNonBlocking));
     for (auto &e : d->ev) HIPCK(hipEventCreate(&e));
@@ -288,7 +293,6 @@ extern "C" int fgss_collection_upload(fgss_collection *c)
         hipMalloc(&d->sig64, nb * kSig64Words) != hipSuccess || hipMalloc(&d->sigbad, nb) != hipSuccess ||
         hipMalloc(&d->job, nb) != hipSuccess || hipMalloc(&d->jend, jb) != hipSuccess ||
         hipMalloc(&d->tj, jb) != hipSuccess) {
-        fgss_device_release(c);
         return fgss_fail(FGSS_ERR_NOMEM, "device allocation for the collection failed");
     }
     if (c->jend.empty()) {
@@ -312,7 +316,16 @@ extern "C" int fgss_collection_upload(fgss_collection *c)
         HIPCK(hipGetLastError());
     }
     HIPCK(hipStreamSynchronize(d->st));
+    holder.dev = nullptr;
+    *out = d;
     return FGSS_OK;
+}
+
+extern "C" int fgss_collection_upload(fgss_collection *c)
+{
+    if (!c) return fgss_fail(FGSS_ERR_ARG, "null collection");
+    if (c->dev) return FGSS_OK;
+    return upload_records(&c->dev, c);
 }
 
 // ---------------------------------------------------------------------------
@@ -320,15 +333,17 @@ extern "C" int fgss_collection_upload(fgss_collection *c)
 // ---------------------------------------------------------------------------
 // per record: mid-prefix length (indexed tokens) and probe-prefix length, both
 // clamped to the record length so that an empty record contributes nothing
+// With rs set the indexed prefix is the probe prefix max_prefix (an R-S index built
+// at the smallest threshold it will serve) instead of the self-join mid-prefix.
 __global__ void k_midlen(const uint32_t *__restrict__ len, const uint32_t *__restrict__ job,
                          const float *__restrict__ tj, uint32_t *__restrict__ midlen,
-                         uint32_t *__restrict__ probelen, int64_t n)
+                         uint32_t *__restrict__ probelen, int64_t n, int rs)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         const float t = tj[job[i]];
         uint32_t l = len[i];
-        uint32_t mp = fgss_mid_prefix(l, t);
+        uint32_t mp = rs ? fgss_max_prefix(l, t) : fgss_mid_prefix(l, t);
         uint32_t qp = fgss_max_prefix(l, t);
         midlen[i] = mp < l ? mp : l;
         probelen[i] = qp < l ? qp : l;
@@ -493,7 +508,8 @@ __device__ __forceinline__ uint32_t part_start(uint64_t a0, uint64_t W, uint32_t
 // (boundary, token) pairs, one independent binary search each.
 __global__ __launch_bounds__(256) void k_bounds(int64_t n, const uint32_t *__restrict__ probelen,
                                                 const uint32_t *__restrict__ poff, const uint2 *__restrict__ rng,
-                                                const uint32_t *__restrict__ hi, const uint32_t *__restrict__ nparts,
+                                                const uint32_t *__restrict__ wlo, const uint32_t *__restrict__ hi,
+                                                const uint32_t *__restrict__ nparts,
                                                 const uint32_t *__restrict__ bofs, const uint32_t *__restrict__ psid,
                                                 const uint32_t *__restrict__ fence, uint32_t *__restrict__ bounds,
                                                 uint32_t cap_bounds)
@@ -503,7 +519,7 @@ __global__ __launch_bounds__(256) void k_bounds(int64_t n, const uint32_t *__res
         const uint32_t np = nparts[q];
         if (np < 2) continue;
         const uint32_t qp = probelen[q];
-        const uint64_t a0 = (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
+        const uint64_t a0 = wlo ? (uint64_t)wlo[q] : (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
         uint32_t *B = bounds + bofs[q];
         const uint2 *R = rng + poff[q];
         const uint32_t total = (np - 1) * qp;
@@ -516,7 +532,9 @@ __global__ __launch_bounds__(256) void k_bounds(int64_t n, const uint32_t *__res
     }
 }
 
-__global__ void k_items(const uint32_t *__restrict__ hi, const uint32_t *__restrict__ nparts,
+// wlo: window start per query (null: q + 1, the self-join window)
+__global__ void k_items(const uint32_t *__restrict__ wlo, const uint32_t *__restrict__ hi,
+                        const uint32_t *__restrict__ nparts,
                         const uint32_t *__restrict__ eq, const uint32_t *__restrict__ partoff,
                         const uint32_t *__restrict__ bofs, const uint32_t *__restrict__ job,
                         const float *__restrict__ tj, const uint32_t *__restrict__ len,
@@ -527,7 +545,7 @@ __global__ void k_items(const uint32_t *__restrict__ hi, const uint32_t *__restr
     if (q >= n) return;
     uint32_t np = nparts[q];
     if (np == 0) return;
-    uint64_t a0 = (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
+    uint64_t a0 = wlo ? (uint64_t)wlo[q] : (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
     uint32_t o = partoff[q];
     if ((uint64_t)o + np > cap_items) return;   // too small: the host resizes and reruns
     const uint32_t per = (eq[q] + np - 1) / np;   // expected postings per part: sizes its LDS table
@@ -573,7 +591,10 @@ __global__ void k_resplit(const Item *__restrict__ in, const uint32_t *__restric
 // the join kernel: one wavefront per work item, items taken from a queue
 // ---------------------------------------------------------------------------
 struct JoinArgs {
-    const uint32_t *tok, *pos, *len;
+    const uint32_t *tok, *pos, *len;    // source records (and the queries of a self-join)
+    const uint32_t *qtok;               // query tokens (R-S: the probe records)
+    const uint32_t *qsig, *qsig64;      // query signatures
+    const uint8_t *qsigbad;
     const uint32_t *loff;
     const uint2 *post;          // postings (record id, token position | length << 16), grouped by term
     const uint32_t *psid;       // record ids of the postings alone
@@ -649,7 +670,9 @@ constexpr uint32_t join_wave_words(int cap)
     return (2u * kAccSlots + 4u * cap + 2u * kTokBatch + 1u + kQStage + 1u) & ~1u;
 }
 
-template <int CAP>
+// RS: an R-S join against an index holding max_prefix(|S|, t_min) tokens per source;
+// postings past max_prefix(|S|, t) of the item's threshold are skipped.
+template <int CAP, bool RS>
 __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
 {
     constexpr int LOG2CAP = CAP == 256 ? 8 : CAP == 512 ? 9 : CAP == 1024 ? 10 : CAP == 2048 ? 11 : 12;
@@ -788,7 +811,9 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                 }
 #pragma unroll
                 for (int k = 0; k < kUnroll; k++) {
-                    const bool valid = jv[k] != kEmpty;
+                    bool valid = jv[k] != kEmpty;
+                    if (RS && valid)
+                        valid = (pe[k].y & 0xFFFFu) < fgss_max_prefix(pe[k].y >> 16, t);
                     bool ok = true, isnew = false;
                     uint32_t h = 0;
                     if (valid) {
@@ -866,9 +891,9 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         const uint32_t nsig = ncand;
         uint32_t nver = 0;
         {
-            const uint4 *qs4 = (const uint4 *)(A.sig + (size_t)q * kSigWords);
-            const uint4 *qh4 = (const uint4 *)(A.sig64 + (size_t)q * kSig64Words);
-            const bool qbad = A.sigbad[q] != 0;
+            const uint4 *qs4 = (const uint4 *)(A.qsig + (size_t)q * kSigWords);
+            const uint4 *qh4 = (const uint4 *)(A.qsig64 + (size_t)q * kSig64Words);
+            const bool qbad = A.qsigbad[q] != 0;
             for (uint32_t base = 0; base < nsig; base += kWave) {
                 const uint32_t c = base + lane;
                 uint32_t ent = 0, h = 0, s = 0, ls = 0;
@@ -927,10 +952,10 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         uint32_t inter = 0, vtok = 0, nres = 0;
         const bool staged = lq <= (uint32_t)kQStage;
         if (nver > 0 && staged) {
-            for (uint32_t i = lane; i < lq; i += kWave) qstage[i] = A.tok[qpos + i];
+            for (uint32_t i = lane; i < lq; i += kWave) qstage[i] = A.qtok[qpos + i];
             wave_sync();
         }
-        const uint32_t *Qv = staged ? qstage : A.tok + qpos;
+        const uint32_t *Qv = staged ? qstage : A.qtok + qpos;
         for (uint32_t c = 0; c < nver; c++) {
             const uint32_t ent = clist[c];
             const uint32_t h = ent & 0xFFFFu, ls = ent >> 16;
@@ -1014,33 +1039,43 @@ static size_t join_smem_bytes()
 }
 
 // persistent grid: as many blocks as fit on the device, each wave loops over items
-template <int CAP>
+template <int CAP, bool RS>
 static int launch_join(fgss_device_sets *d, const JoinArgs &A, bool dry = false)
 {
     static thread_local int cached[2] = {-1, -1};   // blocks, device
     const size_t smem = join_smem_bytes<CAP>();
     if (cached[0] < 0 || cached[1] != d->device) {   // runtime queries: never inside a graph capture
-        HIPCK(hipFuncSetAttribute((const void *)k_join<CAP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+        HIPCK(hipFuncSetAttribute((const void *)k_join<CAP, RS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
         int per_cu = 0, cus = 0;
-        HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_join<CAP>, join_waves(CAP) * kWave, smem));
+        HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_join<CAP, RS>, join_waves(CAP) * kWave, smem));
         HIPCK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device));
         cached[0] = std::max(per_cu, 1) * cus;
         cached[1] = d->device;
     }
     if (dry) return FGSS_OK;
-    hipLaunchKernelGGL(k_join<CAP>, dim3(cached[0]), dim3(join_waves(CAP) * kWave), smem, d->st, A);
+    hipLaunchKernelGGL((k_join<CAP, RS>), dim3(cached[0]), dim3(join_waves(CAP) * kWave), smem, d->st, A);
     HIPCK(hipGetLastError());
     return FGSS_OK;
 }
 
-static int launch_join_cap(fgss_device_sets *d, const JoinArgs &A, int cap, bool dry = false)
+static int launch_join_cap(fgss_device_sets *d, const JoinArgs &A, int cap, bool dry = false, bool rs = false)
 {
+    if (rs) {
+        switch (cap) {
+        case 256: return launch_join<256, true>(d, A, dry);
+        case 512: return launch_join<512, true>(d, A, dry);
+        case 1024: return launch_join<1024, true>(d, A, dry);
+        case 2048: return launch_join<2048, true>(d, A, dry);
+        case 4096: return launch_join<4096, true>(d, A, dry);
+        default: break;
+        }
+    }
     switch (cap) {
-    case 256: return launch_join<256>(d, A, dry);
-    case 512: return launch_join<512>(d, A, dry);
-    case 1024: return launch_join<1024>(d, A, dry);
-    case 2048: return launch_join<2048>(d, A, dry);
-    case 4096: return launch_join<4096>(d, A, dry);
+    case 256: return launch_join<256, false>(d, A, dry);
+    case 512: return launch_join<512, false>(d, A, dry);
+    case 1024: return launch_join<1024, false>(d, A, dry);
+    case 2048: return launch_join<2048, false>(d, A, dry);
+    case 4096: return launch_join<4096, false>(d, A, dry);
     default: return fgss_fail(FGSS_ERR_ARG, "table_slots must be 256, 512, 1024, 2048 or 4096");
     }
 }
@@ -1127,6 +1162,7 @@ static JoinArgs join_args(fgss_device_sets *d)
     A.loff = P<uint32_t>(d->b_loff); A.post = P<uint2>(d->b_post);
     A.psid = P<uint32_t>(d->b_psid); A.fence = P<uint32_t>(d->b_fence);
     A.sig = d->sig; A.sig64 = d->sig64; A.sigbad = d->sigbad;
+    A.qtok = d->tok; A.qsig = d->sig; A.qsig64 = d->sig64; A.qsigbad = d->sigbad;
     A.rng = P<uint2>(d->b_rng); A.poff = P<uint32_t>(d->b_poff); A.bounds = P<uint32_t>(d->b_bounds);
     A.items = P<Item>(d->b_items);
     A.n_items_dev = P<uint32_t>(d->b_misc) + M_NITEMS; A.n_items_cap = d->cap_items;
@@ -1137,10 +1173,11 @@ static JoinArgs join_args(fgss_device_sets *d)
     return A;
 }
 
-// Enqueue the whole join on d->st with no host synchronization (so it can be
-// captured into a HIP graph): index build, planning, the join kernel, and the
-// device->host copies of all counters into pinned memory.
-static int enqueue_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool timing)
+// Index build (steps 1-5) on d->st: per record the indexed prefix (self-join:
+// mid_prefix at the record's job threshold; rs: max_prefix at tj[0], the smallest
+// threshold an R-S index will serve) and the probe prefix, the stable sort of the
+// entries by term, packed postings, the id-only copy, the fence and list offsets.
+static int enqueue_index(fgss_device_sets *d, int rs)
 {
     hipStream_t st = d->st;
     const int64_t n = d->n, E = std::max<int64_t>(d->ntok, 1);
@@ -1151,16 +1188,7 @@ static int enqueue_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool timing)
              *ib = P<uint32_t>(d->b_ib), *sid_of = P<uint32_t>(d->b_sid_of), *loff = P<uint32_t>(d->b_loff);
     uint32_t *psid = P<uint32_t>(d->b_psid), *fence = P<uint32_t>(d->b_fence);
     uint2 *post = P<uint2>(d->b_post);
-    uint32_t *hi = P<uint32_t>(d->b_hi), *eq = P<uint32_t>(d->b_eq), *nparts = P<uint32_t>(d->b_nparts),
-             *partoff = P<uint32_t>(d->b_partoff), *bcnt = P<uint32_t>(d->b_bcnt), *bofs = P<uint32_t>(d->b_bofs);
-    unsigned long long *ctr = P<unsigned long long>(d->b_ctr), *jctr = P<unsigned long long>(d->b_jctr);
-
-    if (timing) HIPCK(hipEventRecord(d->ev[0], st));
-    HIPCK(fill<unsigned long long>(ctr, C_COUNT, 0ull, st));
-    HIPCK(fill<unsigned long long>(jctr, (int64_t)d->njobs * kJobCtr, 0ull, st));
-    HIPCK(fill<uint32_t>(misc, M_COUNT, 0u, st));
-    // 1-5: mid-prefix entries, stable sort by term, postings and list offsets
-    hipLaunchKernelGGL(k_midlen, dim3(grid_for(n, 256)), dim3(256), 0, st, d->len, d->job, d->tj, midlen, probelen, n);
+    hipLaunchKernelGGL(k_midlen, dim3(grid_for(n, 256)), dim3(256), 0, st, d->len, d->job, d->tj, midlen, probelen, n, rs);
     HIPCK(hipGetLastError());
     HIPCK(exclusive_scan<uint32_t>(midlen, midoff, n, misc + M_NE, scr, st));
     HIPCK(exclusive_scan<uint32_t>(probelen, poff, n, misc + M_NP, scr, st));
@@ -1177,6 +1205,29 @@ static int enqueue_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool timing)
     HIPCK(hipGetLastError());
     hipLaunchKernelGGL(k_list_offsets, dim3(grid_for(E + 1, 256)), dim3(256), 0, st, skeys, misc + M_NE, d->n_terms, loff);
     HIPCK(hipGetLastError());
+    return FGSS_OK;
+}
+
+// Enqueue the whole join on d->st with no host synchronization (so it can be
+// captured into a HIP graph): index build, planning, the join kernel, and the
+// device->host copies of all counters into pinned memory.
+static int enqueue_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool timing)
+{
+    hipStream_t st = d->st;
+    const int64_t n = d->n;
+    uint32_t *misc = P<uint32_t>(d->b_misc), *scr = P<uint32_t>(d->b_scanscr);
+    uint32_t *probelen = P<uint32_t>(d->b_probelen), *poff = P<uint32_t>(d->b_poff);
+    uint32_t *loff = P<uint32_t>(d->b_loff), *psid = P<uint32_t>(d->b_psid), *fence = P<uint32_t>(d->b_fence);
+    uint32_t *hi = P<uint32_t>(d->b_hi), *eq = P<uint32_t>(d->b_eq), *nparts = P<uint32_t>(d->b_nparts),
+             *partoff = P<uint32_t>(d->b_partoff), *bcnt = P<uint32_t>(d->b_bcnt), *bofs = P<uint32_t>(d->b_bofs);
+    unsigned long long *ctr = P<unsigned long long>(d->b_ctr), *jctr = P<unsigned long long>(d->b_jctr);
+
+    if (timing) HIPCK(hipEventRecord(d->ev[0], st));
+    HIPCK(fill<unsigned long long>(ctr, C_COUNT, 0ull, st));
+    HIPCK(fill<unsigned long long>(jctr, (int64_t)d->njobs * kJobCtr, 0ull, st));
+    HIPCK(fill<uint32_t>(misc, M_COUNT, 0u, st));
+    int rc = enqueue_index(d, 0);
+    if (rc) return rc;
     if (timing) HIPCK(hipEventRecord(d->ev[1], st));
     // 6-8: plan work items
     hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 4096)), dim3(256), 0, st,
@@ -1186,15 +1237,16 @@ static int enqueue_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool timing)
     HIPCK(exclusive_scan<uint32_t>(nparts, partoff, n, misc + M_NITEMS, scr, st));
     HIPCK(exclusive_scan<uint32_t>(bcnt, bofs, n, misc + M_NB, scr, st));
     hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 8192)), dim3(256), 0, st, n,
-                       probelen, poff, P<uint2>(d->b_rng), hi, nparts, bofs, psid, fence, P<uint32_t>(d->b_bounds),
-                       d->cap_bounds);
+                       probelen, poff, P<uint2>(d->b_rng), (const uint32_t *)nullptr, hi, nparts, bofs, psid, fence,
+                       P<uint32_t>(d->b_bounds), d->cap_bounds);
     HIPCK(hipGetLastError());
-    hipLaunchKernelGGL(k_items, dim3(grid_for(n, 256)), dim3(256), 0, st, hi, nparts, eq, partoff, bofs, d->job, d->tj,
+    hipLaunchKernelGGL(k_items, dim3(grid_for(n, 256)), dim3(256), 0, st, (const uint32_t *)nullptr, hi, nparts, eq,
+                       partoff, bofs, d->job, d->tj,
                        d->len, d->pos, poff, n, P<Item>(d->b_items), d->cap_items);
     HIPCK(hipGetLastError());
     // 9: the join
     if (timing) HIPCK(hipEventRecord(d->ev[4], st));
-    int rc = launch_join_cap(d, join_args(d), cfg.cap);
+    rc = launch_join_cap(d, join_args(d), cfg.cap);
     if (rc) return rc;
     if (timing) HIPCK(hipEventRecord(d->ev[5], st));
     HIPCK(hipMemcpyAsync(d->h_ctr, ctr, C_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
@@ -1242,57 +1294,82 @@ static int capture_graph(fgss_device_sets *d, const RunCfg &cfg)
     return FGSS_OK;
 }
 
-// Run the join for the thresholds already in d->tj: the pipeline (eager, or as a
-// replayed graph), grown and rerun until every capacity holds, then host-driven
-// rounds for work items whose LDS table overflowed (split four ways each time).
-// Appends (q, s, overlap) triples in device emission order.
-static int run_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool use_graph, std::vector<int32_t> &tri, RunTotals &T)
+// How a record set runs its pipeline: a self-join (its own records are queries and
+// sources) or an R-S join of a slot's probe records against a resident index.
+struct PipeOps {
+    std::function<int()> prepare;            // size every buffer (may allocate)
+    std::function<int(bool)> enqueue;        // enqueue everything, no host syncs (arg: record timing events)
+    std::function<JoinArgs()> args;          // join-kernel arguments (for re-split rounds)
+    bool rs = false;
+    bool graph_ok = false;
+};
+
+static PipeOps self_ops(fgss_device_sets *d, const RunCfg &cfg)
+{
+    PipeOps o;
+    o.prepare = [d] { return prepare_buffers(d); };
+    o.enqueue = [d, cfg](bool timing) { return enqueue_pipeline(d, cfg, timing); };
+    o.args = [d] { return join_args(d); };
+    o.graph_ok = true;
+    return o;
+}
+
+// Enqueue one run (eager, or a replay of the captured graph).
+static int launch_once(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops, bool use_graph)
+{
+    int rc = ops.prepare();
+    if (rc) return rc;
+    if (use_graph && ops.graph_ok) {
+        rc = launch_join_cap(d, ops.args(), cfg.cap, true, ops.rs);   // launch configuration, outside the capture
+        if (rc) return rc;
+        if (!d->graph_exec || d->graph_key[0] != cfg.cap || d->graph_key[1] != (int)cfg.target ||
+            d->graph_key[2] != d->cap_gen) {
+            rc = capture_graph(d, cfg);
+            if (rc) return rc;
+        }
+        HIPCK(hipGraphLaunch(d->graph_exec, d->st));
+        return FGSS_OK;
+    }
+    return ops.enqueue(true);
+}
+
+// After a launched run has completed (stream synchronized or its event done): rerun
+// with grown buffers while any capacity was exceeded, then collect the counters and
+// the (q, s, overlap) triples, then run host-driven rounds for the work items whose
+// LDS table overflowed (each split four ways).
+static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops, bool use_graph,
+                      std::vector<int32_t> &tri, RunTotals &T)
 {
     hipStream_t st = d->st;
     tri.clear();
     T = RunTotals();
     T.jctr.assign((size_t)d->njobs * kJobCtr, 0);
-    if (d->n == 0) return FGSS_OK;
-    trace_mark("begin");
     for (int attempt = 0;; attempt++) {
-        if (attempt > 8) return fgss_fail(FGSS_ERR_DEVICE, "internal: buffer sizing did not converge");
-        int rc = prepare_buffers(d);
-        if (rc) return rc;
-        if (use_graph) {
-            rc = launch_join_cap(d, join_args(d), cfg.cap, true);   // launch configuration, outside the capture
-            if (rc) return rc;
-            if (!d->graph_exec || d->graph_key[0] != cfg.cap || d->graph_key[1] != (int)cfg.target ||
-                d->graph_key[2] != d->cap_gen) {
-                rc = capture_graph(d, cfg);
-                if (rc) return rc;
-            }
-            HIPCK(hipGraphLaunch(d->graph_exec, st));
-        } else {
-            rc = enqueue_pipeline(d, cfg, true);
-            if (rc) return rc;
-        }
-        HIPCK(hipStreamSynchronize(st));
-        trace_mark("pipeline");
         const uint32_t n_items = d->h_misc[M_NITEMS], nB = d->h_misc[M_NB];
         const uint64_t nres = d->h_ctr[C_RESULTS];
         bool again = false;
         if (n_items > d->cap_items) { d->cap_items = n_items + n_items / 4 + 1024; again = true; }
         if (nB > d->cap_bounds) { d->cap_bounds = nB + nB / 4 + 1024; again = true; }
-        if (nres > d->cap_out) { d->cap_out = (uint32_t)std::min<uint64_t>(nres + nres / 2 + 1024, UINT32_MAX / 16); again = true; }
+        if (nres > d->cap_out) {
+            d->cap_out = (uint32_t)std::min<uint64_t>(nres + nres / 2 + 1024, UINT32_MAX / 16);
+            again = true;
+        }
         if (!again) break;
+        if (attempt > 8) return fgss_fail(FGSS_ERR_DEVICE, "internal: buffer sizing did not converge");
+        int rc = launch_once(d, cfg, ops, use_graph);
+        if (rc) return rc;
+        HIPCK(hipStreamSynchronize(st));
     }
-    T.graph = use_graph;
+    trace_mark("pipeline");
+    T.graph = use_graph && ops.graph_ok;
     T.launches = 1;
     T.index_entries = d->h_misc[M_NE];
     T.probe_tokens = d->h_misc[M_NP];
-    if (!use_graph) {
+    if (!T.graph) {
         float ms = 0;
-        HIPCK(hipEventElapsedTime(&ms, d->ev[0], d->ev[1]));
-        T.ms_index = ms;
-        HIPCK(hipEventElapsedTime(&ms, d->ev[1], d->ev[5]));
-        T.ms_join = ms;
-        HIPCK(hipEventElapsedTime(&ms, d->ev[4], d->ev[5]));
-        T.ms_kernel = ms;
+        if (hipEventElapsedTime(&ms, d->ev[0], d->ev[1]) == hipSuccess) T.ms_index = ms;
+        if (hipEventElapsedTime(&ms, d->ev[1], d->ev[5]) == hipSuccess) T.ms_join = ms;
+        if (hipEventElapsedTime(&ms, d->ev[4], d->ev[5]) == hipSuccess) T.ms_kernel = ms;
     }
     for (int k = C_ESCAN; k <= C_CYC_RANGES; k++) T.acc[k] += d->h_ctr[k];
     uint64_t nres = d->h_ctr[C_RESULTS], novf = d->h_ctr[C_OVERFLOW];
@@ -1320,13 +1397,16 @@ static int run_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool use_graph, 
             uint32_t *misc = P<uint32_t>(d->b_misc);
             HIPCK(fill<unsigned long long>(ctr, C_COUNT, 0ull, st));
             HIPCK(fill<uint32_t>(misc + M_NRESPLIT, 1, cnt, st));
-            JoinArgs A = join_args(d);
+            JoinArgs A = ops.args();
             A.items = cur;
             A.n_items_dev = misc + M_NRESPLIT;
             A.n_items_cap = cnt;
             A.nb_cap = UINT32_MAX;   // re-split items never read the boundary table
+            A.out = P<int32_t>(d->b_out);
+            A.out_cap = d->cap_out;
+            A.ovf = P<uint32_t>(d->b_ovf);
             HIPCK(hipEventRecord(d->ev[4], st));
-            rc = launch_join_cap(d, A, cfg.cap);
+            rc = launch_join_cap(d, A, cfg.cap, false, ops.rs);
             if (rc) return rc;
             HIPCK(hipEventRecord(d->ev[5], st));
             HIPCK(hipMemcpyAsync(d->h_ctr, ctr, C_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
@@ -1353,6 +1433,21 @@ static int run_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool use_graph, 
     for (size_t k = 0; k < T.jctr.size(); k++) T.jctr[k] = d->h_jctr[k];
     trace_mark("collect");
     return FGSS_OK;
+}
+
+// Run the join for the thresholds already in d->tj, synchronously.
+static int run_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool use_graph, std::vector<int32_t> &tri, RunTotals &T)
+{
+    tri.clear();
+    T = RunTotals();
+    T.jctr.assign((size_t)d->njobs * kJobCtr, 0);
+    if (d->n == 0) return FGSS_OK;
+    trace_mark("begin");
+    const PipeOps ops = self_ops(d, cfg);
+    int rc = launch_once(d, cfg, ops, use_graph);
+    if (rc) return rc;
+    HIPCK(hipStreamSynchronize(d->st));
+    return finish_run(d, cfg, ops, use_graph, tri, T);
 }
 
 // Order of the (q, s, overlap) triples by (q, s): an LSD radix sort (11-bit digits)
@@ -1673,4 +1768,642 @@ extern "C" int fgss_batch_autotune(fgss_batch *b, int32_t reps, fgss_join_opts *
     *best = bo;
     if (best_ms) *best_ms = bms;
     return FGSS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// serving: R-S joins of probe records against a resident indexed collection,
+// with continuous batching.  No reference counterpart (the reference's
+// "dualjoin" is only a usage string, src/preprocess_cli.cpp:26).
+//
+// R-S semantics, by analogy with the self-join: probe r and indexed record s form a
+// result at threshold t iff the 4-way size filter holds, m = |R[0,qp) n S[0,sp)| > 0
+// with qp = max_prefix(|R|, t) and sp = max_prefix(|S|, t), the positional filter
+// passes at the last prefix match, |R n S| >= floor(min_overlap) and the float32
+// Jaccard is >= t.  The index holds max_prefix(|S|, t_min) tokens per record, which
+// covers every t >= t_min; postings past max_prefix(|S|, t) are skipped.
+// ---------------------------------------------------------------------------
+
+// The largest partner length ls >= l with size_ok(l, ls) (at most max_len).
+__device__ __forceinline__ uint32_t partner_lmax(uint32_t l, float t, uint32_t max_len)
+{
+    uint32_t top = fgss_floor_max_size(l, t);   // the filter needs ls <= floor(l / t)
+    if (top > max_len) top = max_len;
+    if (top < l) return l;
+    uint32_t a = l + 1, b = top + 1;            // first failing length in [l + 1, top + 1)
+    while (a < b) {
+        uint32_t mid = (a + b) >> 1;
+        if (fgss_size_ok(l, mid, t)) a = mid + 1;
+        else b = mid;
+    }
+    return a - 1;
+}
+
+// Per probe (a wave each): the source window -- indexed records whose length is in
+// the size-filter band, a contiguous id range because lengths are non-increasing --
+// and each probe token's posting range inside it.
+__global__ __launch_bounds__(256) void k_plan_rs(const uint32_t *__restrict__ ptok, const uint32_t *__restrict__ ppos,
+                                                 const uint32_t *__restrict__ plen, int64_t np,
+                                                 const uint32_t *__restrict__ pjob, const float *__restrict__ tj,
+                                                 const uint32_t *__restrict__ probelen,
+                                                 const uint32_t *__restrict__ poff, const uint32_t *__restrict__ slen,
+                                                 uint32_t ns, uint32_t smax_len, const uint32_t *__restrict__ loff,
+                                                 const uint32_t *__restrict__ psid, const uint32_t *__restrict__ fence,
+                                                 uint32_t n_terms, uint32_t target, uint2 *__restrict__ rng,
+                                                 uint32_t *__restrict__ wlo_out, uint32_t *__restrict__ hi_out,
+                                                 uint32_t *__restrict__ nparts, uint32_t *__restrict__ eq_out,
+                                                 uint32_t *__restrict__ bcnt)
+{
+    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; r < np; r += nwaves) {
+        const uint32_t lr = plen[r], qp = probelen[r];
+        const float t = tj[pjob[r]];
+        const uint32_t lmin = partner_lmin(lr, t), lmax = partner_lmax(lr, t, smax_len);
+        uint32_t a = 0, b = ns;   // first source with length <= lmax
+        while (a < b) {
+            uint32_t mid = (a + b) >> 1;
+            if (slen[mid] > lmax) a = mid + 1;
+            else b = mid;
+        }
+        const uint32_t wlo = a;
+        b = ns;                   // first source with length < lmin
+        while (a < b) {
+            uint32_t mid = (a + b) >> 1;
+            if (slen[mid] >= lmin) a = mid + 1;
+            else b = mid;
+        }
+        const uint32_t whi = a;
+        const uint32_t ro = poff[r];
+        uint32_t e = 0;
+        for (uint32_t j = lane_id(); j < qp; j += kWave) {
+            const uint32_t term = ptok[ppos[r] + j];
+            uint32_t x = 0, y = 0;
+            if (term < n_terms) {   // a q-gram the index has never seen has no postings
+                const uint32_t L0 = loff[term], L1 = loff[term + 1];
+                x = lower_bound_sid(psid, fence, L0, L1, wlo);
+                y = lower_bound_sid(psid, fence, x, L1, whi);
+            }
+            rng[ro + j] = make_uint2(x, y);
+            e += y - x;
+        }
+        e = wave_sum(e);
+        if (lane_id() == 0) {
+            wlo_out[r] = wlo;
+            hi_out[r] = whi;
+            eq_out[r] = e;
+            const uint32_t npt = e == 0 ? 0u : (e + target - 1) / target;
+            nparts[r] = npt;
+            bcnt[r] = npt > 1 ? (npt - 1) * qp : 0u;
+        }
+    }
+}
+
+// host-side request and server state
+struct ServeRequest {
+    int64_t id = 0;
+    const fgss_collection *probes = nullptr;
+    float t = 0.f;
+    int64_t est = 0;                    // candidate-memory estimate (bytes)
+    bool done = false;
+    int status = FGSS_OK;
+    std::vector<int32_t> tri;           // (probe, source, overlap), probe indices local to the request
+    uint64_t cnt[kJobCtr] = {0, 0, 0, 0};
+    int32_t launches = 0;               // join-kernel launches of its run
+    int32_t batch_requests = 0;         // requests that shared its run
+    std::chrono::steady_clock::time_point t_submit, t_start, t_done;
+};
+
+struct ServeSlot {
+    fgss_device_sets *d = nullptr;      // probe records + workspace (own stream)
+    DevBuf b_tok, b_len, b_pos, b_sig, b_sig64, b_sigbad, b_job, b_jend, b_tj, b_wlo;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    std::vector<ServeRequest *> reqs;
+    std::vector<int64_t> first;         // first probe of each request in the merged set
+    fgss_collection merged;             // host copy of the merged probes (alive while in flight)
+    std::vector<uint32_t> pos32;        // their token offsets
+    std::vector<float> tjs;
+    size_t jctr_cap = 0;                // jobs the pinned counter mirror holds
+    int64_t est = 0;
+    uint64_t seq = 0;                   // launch order
+};
+
+struct fgss_server {
+    fgss_device_sets *idx = nullptr;    // private device copy of the indexed records + the R-S index
+    const fgss_collection *coll = nullptr;
+    float t_min = 0.f;
+    int64_t budget = 0, inflight = 0;
+    double mean_list = 1.0;             // index entries per term (for the estimate)
+    RunCfg cfg;
+    std::vector<ServeSlot> slots;
+    std::deque<ServeRequest *> pending;
+    std::map<int64_t, ServeRequest *> reqs;
+    int64_t next_id = 1;
+    int64_t max_probes = 1 << 16;       // probes per slot run
+    uint64_t launch_seq = 0;
+    uint32_t index_entries = 0;
+    bool hold = false;                  // admission paused
+    bool empty = false;                 // no indexed tokens: every request has no pairs
+};
+
+static int slot_prepare(ServeSlot &sl)
+{
+    fgss_device_sets *d = sl.d;
+    const int64_t n = std::max<int64_t>(d->n, 1), E = std::max<int64_t>(d->ntok, 1);
+    if (d->cap_items == 0) d->cap_items = (uint32_t)(2 * n + 4096);
+    if (d->cap_bounds == 0) d->cap_bounds = (uint32_t)(E + 4096);
+    if (d->cap_out == 0) d->cap_out = 65536;
+    GROW(d->b_midlen, (size_t)n * 4);
+    GROW(d->b_probelen, (size_t)n * 4);
+    GROW(d->b_poff, (size_t)(n + 1) * 4);
+    GROW(d->b_scanscr, (size_t)scan_scratch_elems(n + 1) * 4);
+    GROW(d->b_rng, (size_t)E * sizeof(uint2));
+    GROW(sl.b_wlo, (size_t)n * 4);
+    GROW(d->b_hi, (size_t)n * 4);
+    GROW(d->b_eq, (size_t)n * 4);
+    GROW(d->b_nparts, (size_t)n * 4);
+    GROW(d->b_partoff, (size_t)(n + 1) * 4);
+    GROW(d->b_bcnt, (size_t)n * 4);
+    GROW(d->b_bofs, (size_t)(n + 1) * 4);
+    GROW(d->b_bounds, (size_t)d->cap_bounds * 4);
+    GROW(d->b_items, (size_t)d->cap_items * sizeof(Item));
+    GROW(d->b_ovf, (size_t)d->cap_items * 4);
+    GROW(d->b_out, (size_t)d->cap_out * 12);
+    GROW(d->b_ctr, C_COUNT * sizeof(uint64_t));
+    GROW(d->b_jctr, (size_t)std::max<uint32_t>(d->njobs, 1) * kJobCtr * sizeof(uint64_t));
+    GROW(d->b_misc, M_COUNT * 4);
+    return FGSS_OK;
+}
+
+static JoinArgs rs_args(fgss_server *srv, ServeSlot &sl)
+{
+    fgss_device_sets *d = sl.d, *x = srv->idx;
+    JoinArgs A = join_args(d);
+    A.tok = x->tok; A.pos = x->pos; A.len = x->len;
+    A.sig = x->sig; A.sig64 = x->sig64; A.sigbad = x->sigbad;
+    A.loff = P<uint32_t>(x->b_loff); A.post = P<uint2>(x->b_post);
+    A.psid = P<uint32_t>(x->b_psid); A.fence = P<uint32_t>(x->b_fence);
+    A.qtok = d->tok; A.qsig = d->sig; A.qsig64 = d->sig64; A.qsigbad = d->sigbad;
+    return A;
+}
+
+// Enqueue an R-S run of the slot's probes on the slot's stream (no host syncs).
+static int enqueue_rs(fgss_server *srv, ServeSlot &sl, bool timing)
+{
+    fgss_device_sets *d = sl.d, *x = srv->idx;
+    hipStream_t st = d->st;
+    const int64_t n = d->n;
+    const RunCfg &cfg = srv->cfg;
+    uint32_t *misc = P<uint32_t>(d->b_misc), *scr = P<uint32_t>(d->b_scanscr);
+    uint32_t *probelen = P<uint32_t>(d->b_probelen), *poff = P<uint32_t>(d->b_poff);
+    uint32_t *hi = P<uint32_t>(d->b_hi), *eq = P<uint32_t>(d->b_eq), *nparts = P<uint32_t>(d->b_nparts),
+             *partoff = P<uint32_t>(d->b_partoff), *bcnt = P<uint32_t>(d->b_bcnt), *bofs = P<uint32_t>(d->b_bofs),
+             *wlo = P<uint32_t>(sl.b_wlo);
+    unsigned long long *ctr = P<unsigned long long>(d->b_ctr), *jctr = P<unsigned long long>(d->b_jctr);
+    if (timing) HIPCK(hipEventRecord(d->ev[0], st));
+    HIPCK(fill<unsigned long long>(ctr, C_COUNT, 0ull, st));
+    HIPCK(fill<unsigned long long>(jctr, (int64_t)d->njobs * kJobCtr, 0ull, st));
+    HIPCK(fill<uint32_t>(misc, M_COUNT, 0u, st));
+    hipLaunchKernelGGL(k_midlen, dim3(grid_for(n, 256)), dim3(256), 0, st, d->len, d->job, d->tj,
+                       P<uint32_t>(d->b_midlen), probelen, n, 1);
+    HIPCK(hipGetLastError());
+    HIPCK(exclusive_scan<uint32_t>(probelen, poff, n, misc + M_NP, scr, st));
+    if (timing) HIPCK(hipEventRecord(d->ev[1], st));
+    hipLaunchKernelGGL(k_plan_rs, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 4096)), dim3(256), 0, st,
+                       d->tok, d->pos, d->len, n, d->job, d->tj, probelen, poff, x->len, (uint32_t)x->n,
+                       (uint32_t)x->max_len, P<uint32_t>(x->b_loff), P<uint32_t>(x->b_psid), P<uint32_t>(x->b_fence),
+                       x->n_terms, cfg.target, P<uint2>(d->b_rng), wlo, hi, nparts, eq, bcnt);
+    HIPCK(hipGetLastError());
+    HIPCK(exclusive_scan<uint32_t>(nparts, partoff, n, misc + M_NITEMS, scr, st));
+    HIPCK(exclusive_scan<uint32_t>(bcnt, bofs, n, misc + M_NB, scr, st));
+    hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 8192)), dim3(256), 0, st, n,
+                       probelen, poff, P<uint2>(d->b_rng), wlo, hi, nparts, bofs, P<uint32_t>(x->b_psid),
+                       P<uint32_t>(x->b_fence), P<uint32_t>(d->b_bounds), d->cap_bounds);
+    HIPCK(hipGetLastError());
+    hipLaunchKernelGGL(k_items, dim3(grid_for(n, 256)), dim3(256), 0, st, wlo, hi, nparts, eq, partoff, bofs, d->job,
+                       d->tj, d->len, d->pos, poff, n, P<Item>(d->b_items), d->cap_items);
+    HIPCK(hipGetLastError());
+    if (timing) HIPCK(hipEventRecord(d->ev[4], st));
+    int rc = launch_join_cap(d, rs_args(srv, sl), cfg.cap, false, true);
+    if (rc) return rc;
+    if (timing) HIPCK(hipEventRecord(d->ev[5], st));
+    HIPCK(hipMemcpyAsync(d->h_ctr, ctr, C_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    HIPCK(hipMemcpyAsync(d->h_misc, misc, M_COUNT * 4, hipMemcpyDeviceToHost, st));
+    HIPCK(hipEventRecord(sl.done, st));
+    return FGSS_OK;
+}
+
+static PipeOps rs_ops(fgss_server *srv, ServeSlot &sl)
+{
+    PipeOps o;
+    o.prepare = [&sl] { return slot_prepare(sl); };
+    o.enqueue = [srv, &sl](bool timing) { return enqueue_rs(srv, sl, timing); };
+    o.args = [srv, &sl] { return rs_args(srv, sl); };
+    o.rs = true;
+    o.graph_ok = false;
+    return o;
+}
+
+// Copy the merged probes of a slot to the device (growing its arrays) and compute
+// their signatures, asynchronously on the slot's stream.
+static int slot_load(ServeSlot &sl)
+{
+    fgss_device_sets *d = sl.d;
+    const fgss_collection &m = sl.merged;
+    const int64_t n = (int64_t)m.len.size(), ntok = (int64_t)m.tok.size();
+    const uint32_t nj = (uint32_t)m.jend.size();
+    GROW(sl.b_tok, (size_t)std::max<int64_t>(ntok, 1) * 4);
+    GROW(sl.b_len, (size_t)std::max<int64_t>(n, 1) * 4);
+    GROW(sl.b_pos, (size_t)std::max<int64_t>(n, 1) * 4);
+    GROW(sl.b_sig, (size_t)std::max<int64_t>(n, 1) * 4 * kSigWords);
+    GROW(sl.b_sig64, (size_t)std::max<int64_t>(n, 1) * 4 * kSig64Words);
+    GROW(sl.b_sigbad, (size_t)std::max<int64_t>(n, 1));
+    GROW(sl.b_job, (size_t)std::max<int64_t>(n, 1) * 4);
+    GROW(sl.b_jend, (size_t)std::max<uint32_t>(nj, 1) * 4);
+    GROW(sl.b_tj, (size_t)std::max<uint32_t>(nj, 1) * 4);
+    d->tok = P<uint32_t>(sl.b_tok); d->len = P<uint32_t>(sl.b_len); d->pos = P<uint32_t>(sl.b_pos);
+    d->sig = P<uint32_t>(sl.b_sig); d->sig64 = P<uint32_t>(sl.b_sig64); d->sigbad = P<uint8_t>(sl.b_sigbad);
+    d->job = P<uint32_t>(sl.b_job); d->jend = P<uint32_t>(sl.b_jend); d->tj = P<float>(sl.b_tj);
+    d->n = n; d->ntok = ntok; d->njobs = nj; d->max_len = m.max_len; d->n_terms = (uint32_t)m.n_terms;
+    // pinned per-job counter mirror, grown to this batch's jobs
+    if (sl.jctr_cap < (size_t)std::max<uint32_t>(nj, 1)) {
+        if (d->h_jctr) HIPCK(hipHostFree(d->h_jctr));
+        d->h_jctr = nullptr;
+        sl.jctr_cap = std::max<size_t>((size_t)nj, 64);
+        HIPCK(hipHostMalloc((void **)&d->h_jctr, sl.jctr_cap * kJobCtr * 8, hipHostMallocDefault));
+    }
+    hipStream_t st = d->st;
+    if (ntok) HIPCK(hipMemcpyAsync(d->tok, m.tok.data(), (size_t)ntok * 4, hipMemcpyHostToDevice, st));
+    if (n) {
+        HIPCK(hipMemcpyAsync(d->len, m.len.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
+        HIPCK(hipMemcpyAsync(d->pos, sl.pos32.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
+        HIPCK(hipMemcpyAsync(d->job, m.job.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
+    }
+    HIPCK(hipMemcpyAsync(d->jend, m.jend.data(), (size_t)nj * 4, hipMemcpyHostToDevice, st));
+    HIPCK(hipMemcpyAsync(d->tj, sl.tjs.data(), (size_t)nj * 4, hipMemcpyHostToDevice, st));
+    if (n) {
+        const int64_t blocks = std::min<int64_t>((n + 3) / 4, 65536);
+        hipLaunchKernelGGL(k_signature, dim3((unsigned)blocks), dim3(256), 0, st, d->tok, d->pos, d->len, n, d->sig,
+                           d->sig64, d->sigbad);
+        HIPCK(hipGetLastError());
+    }
+    return FGSS_OK;
+}
+
+// Merge the slot's requests into one probe set (request k = job k, its own
+// threshold) and enqueue its run on the slot's stream.
+static int slot_launch(fgss_server *srv, ServeSlot &sl)
+{
+    fgss_collection &m = sl.merged;
+    m.len.clear(); m.tok.clear(); m.job.clear(); m.jend.clear();
+    sl.first.clear();
+    sl.tjs.clear();
+    const auto now = std::chrono::steady_clock::now();
+    for (size_t k = 0; k < sl.reqs.size(); k++) {
+        const fgss_collection *c = sl.reqs[k]->probes;
+        sl.first.push_back((int64_t)m.len.size());
+        m.len.insert(m.len.end(), c->len.begin(), c->len.end());
+        m.tok.insert(m.tok.end(), c->tok.begin(), c->tok.end());
+        m.job.insert(m.job.end(), c->len.size(), (uint32_t)k);
+        m.jend.push_back((uint32_t)m.len.size());
+        sl.tjs.push_back(sl.reqs[k]->t);
+        sl.reqs[k]->t_start = now;
+        sl.reqs[k]->batch_requests = (int32_t)sl.reqs.size();
+    }
+    sl.first.push_back((int64_t)m.len.size());
+    sl.pos32.resize(m.len.size());
+    uint32_t p = 0;
+    int32_t mx = 0;
+    for (size_t i = 0; i < m.len.size(); i++) {
+        sl.pos32[i] = p;
+        p += (uint32_t)m.len[i];
+        mx = std::max(mx, m.len[i]);
+    }
+    m.max_len = mx;
+    m.n_terms = (int32_t)srv->idx->n_terms;
+    int rc = slot_load(sl);
+    if (rc) return rc;
+    if (sl.d->n > 0 && !srv->empty) {
+        rc = slot_prepare(sl);
+        if (rc) return rc;
+        rc = enqueue_rs(srv, sl, true);
+        if (rc) return rc;
+    } else {
+        HIPCK(hipEventRecord(sl.done, sl.d->st));   // nothing to join
+    }
+    sl.busy = true;
+    return FGSS_OK;
+}
+
+// A slot's run has completed on the device: finish it on the host (reruns with
+// grown capacities, rounds for overflowed items), then hand every request its
+// (probe, source, overlap) triples and counters.
+static int slot_retire(fgss_server *srv, ServeSlot &sl)
+{
+    std::vector<int32_t> tri;
+    RunTotals T;
+    int rc = FGSS_OK;
+    if (sl.d->n > 0 && !srv->empty) {
+        const PipeOps ops = rs_ops(srv, sl);
+        rc = finish_run(sl.d, srv->cfg, ops, false, tri, T);
+    } else {
+        T.jctr.assign(sl.reqs.size() * kJobCtr, 0);
+    }
+    const auto now = std::chrono::steady_clock::now();
+    std::vector<std::vector<int32_t>> per(sl.reqs.size());
+    if (rc == FGSS_OK) {
+        for (size_t i = 0; i < tri.size() / 3; i++) {
+            const int32_t q = tri[3 * i];
+            const size_t k =
+                (size_t)(std::upper_bound(sl.first.begin(), sl.first.end(), (int64_t)q) - sl.first.begin()) - 1;
+            per[k].push_back(q - (int32_t)sl.first[k]);
+            per[k].push_back(tri[3 * i + 1]);
+            per[k].push_back(tri[3 * i + 2]);
+        }
+    }
+    for (size_t k = 0; k < sl.reqs.size(); k++) {
+        ServeRequest *r = sl.reqs[k];
+        r->status = rc;
+        r->tri.swap(per[k]);
+        for (int c = 0; c < kJobCtr; c++) r->cnt[c] = rc == FGSS_OK ? T.jctr[k * kJobCtr + c] : 0;
+        r->launches = T.launches;
+        r->done = true;
+        r->t_done = now;
+    }
+    srv->inflight -= sl.est;
+    sl.est = 0;
+    sl.reqs.clear();
+    sl.busy = false;
+    return rc;
+}
+
+// Retire completed slots, then admit pending requests into idle slots, first come
+// first served, as many per slot as the candidate-memory budget and the per-slot
+// probe limit allow (at least one when nothing is in flight, so every request is
+// eventually admitted).  With block set, waits for the oldest busy slot first.
+static int server_progress(fgss_server *srv, bool block)
+{
+    if (block) {
+        ServeSlot *oldest = nullptr;
+        for (ServeSlot &sl : srv->slots)
+            if (sl.busy && (!oldest || sl.seq < oldest->seq)) oldest = &sl;
+        if (oldest) HIPCK(hipEventSynchronize(oldest->done));
+    }
+    for (ServeSlot &sl : srv->slots) {
+        if (!sl.busy) continue;
+        const hipError_t q = hipEventQuery(sl.done);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) return fgss_fail(FGSS_ERR_DEVICE, "slot event: %s", hipGetErrorString(q));
+        slot_retire(srv, sl);   // a failed run is reported through its requests
+    }
+    for (ServeSlot &sl : srv->slots) {
+        if (srv->hold || srv->pending.empty()) break;
+        if (sl.busy) continue;
+        int64_t probes = 0;
+        while (!srv->pending.empty()) {
+            ServeRequest *r = srv->pending.front();
+            const int64_t np = (int64_t)r->probes->len.size();
+            const bool fits = srv->inflight + r->est <= srv->budget && probes + np <= srv->max_probes;
+            const bool alone = sl.reqs.empty() && srv->inflight == 0;
+            if (!fits && !alone) break;
+            srv->pending.pop_front();
+            sl.reqs.push_back(r);
+            sl.est += r->est;
+            srv->inflight += r->est;
+            probes += np;
+            if (!fits) break;
+        }
+        if (sl.reqs.empty()) break;
+        sl.seq = srv->launch_seq++;
+        int rc = slot_launch(srv, sl);
+        if (rc) {
+            for (ServeRequest *r : sl.reqs) { r->status = rc; r->done = true; r->t_done = std::chrono::steady_clock::now(); }
+            srv->inflight -= sl.est;
+            sl.est = 0;
+            sl.reqs.clear();
+            (void)hipStreamSynchronize(sl.d->st);
+            return rc;
+        }
+    }
+    return FGSS_OK;
+}
+
+extern "C" int fgss_server_create(const fgss_collection *indexed, float t_min, int32_t slots, int64_t budget_bytes,
+                                  const fgss_join_opts *opts, fgss_server **out)
+{
+    if (!indexed || !out) return fgss_fail(FGSS_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (!(t_min > 0.0f) || !(t_min < 1.0f)) return fgss_fail(FGSS_ERR_ARG, "t_min must be a float in (0.0..1.0)");
+    if (slots < 1 || slots > 16) return fgss_fail(FGSS_ERR_ARG, "slots must be in 1..16");
+    if (opts && opts->table_slots && opts->table_slots != 256 && opts->table_slots != 512 &&
+        opts->table_slots != 1024 && opts->table_slots != 2048 && opts->table_slots != 4096)
+        return fgss_fail(FGSS_ERR_ARG, "table_slots must be 256, 512, 1024, 2048 or 4096");
+    if (!indexed->jend.empty()) return fgss_fail(FGSS_ERR_ARG, "the indexed collection must be a plain collection");
+    auto *srv = new fgss_server();
+    srv->coll = indexed;
+    srv->t_min = t_min;
+    srv->budget = budget_bytes > 0 ? budget_bytes : (int64_t)1 << 30;
+    srv->cfg = make_cfg(opts);
+    int rc = upload_records(&srv->idx, indexed);
+    if (rc) { delete srv; return rc; }
+    fgss_device_sets *x = srv->idx;
+    srv->empty = x->n == 0 || x->ntok == 0;
+    // the R-S index: the first max_prefix(|s|, t_min) tokens of every record
+    rc = set_thresholds(x, &t_min, 1);
+    if (!rc) rc = prepare_buffers(x);
+    if (!rc && srv->empty) x->h_misc[M_NE] = 0;
+    if (!rc && !srv->empty) {
+        uint32_t *misc = P<uint32_t>(x->b_misc);
+        if (fill<uint32_t>(misc, M_COUNT, 0u, x->st) != hipSuccess) rc = fgss_fail(FGSS_ERR_DEVICE, "fill failed");
+        if (!rc) rc = enqueue_index(x, 1);
+        if (!rc && hipMemcpyAsync(x->h_misc, misc, M_COUNT * 4, hipMemcpyDeviceToHost, x->st) != hipSuccess)
+            rc = fgss_fail(FGSS_ERR_DEVICE, "copy of the index counters failed");
+        if (!rc && hipStreamSynchronize(x->st) != hipSuccess) rc = fgss_fail(FGSS_ERR_DEVICE, "index build failed");
+    }
+    if (rc) { fgss_server_free(srv); return rc; }
+    srv->index_entries = x->h_misc[M_NE];
+    srv->mean_list = x->n_terms ? std::max(1.0, (double)x->h_misc[M_NE] / (double)x->n_terms) : 1.0;
+    // the self-join workspace of the index is no longer needed: keep postings,
+    // list offsets, ids and fences only
+    DevBuf *spare[] = {&x->b_ka, &x->b_kb, &x->b_ia, &x->b_ib, &x->b_sid_of, &x->b_sortscr, &x->b_rng,
+                       &x->b_items, &x->b_ovf, &x->b_bounds, &x->b_out};
+    for (DevBuf *b : spare)
+        if (b->p) { (void)hipFree(b->p); b->p = nullptr; b->cap = 0; }
+    srv->slots.resize((size_t)slots);
+    for (ServeSlot &sl : srv->slots) {
+        sl.d = new fgss_device_sets();
+        sl.d->device = x->device;
+        bool ok = hipStreamCreateWithFlags(&sl.d->st, hipStreamNonBlocking) == hipSuccess &&
+                  hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess &&
+                  hipHostMalloc((void **)&sl.d->h_ctr, C_COUNT * sizeof(uint64_t), hipHostMallocDefault) == hipSuccess &&
+                  hipHostMalloc((void **)&sl.d->h_misc, 16 * sizeof(uint32_t), hipHostMallocDefault) == hipSuccess;
+        for (auto &e : sl.d->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+        if (!ok) {
+            fgss_server_free(srv);
+            return fgss_fail(FGSS_ERR_DEVICE, "server slot set-up failed");
+        }
+    }
+    *out = srv;
+    return FGSS_OK;
+}
+
+extern "C" int fgss_server_submit(fgss_server *srv, const fgss_collection *probes, float t, int64_t *request_id)
+{
+    if (!srv || !probes || !request_id) return fgss_fail(FGSS_ERR_ARG, "null argument");
+    if (!(t >= srv->t_min) || !(t < 1.0f))
+        return fgss_fail(FGSS_ERR_ARG, "threshold %g outside [t_min, 1.0) of the server", (double)t);
+    if (!probes->jend.empty()) return fgss_fail(FGSS_ERR_ARG, "probes must be a plain collection");
+    if ((int64_t)probes->len.size() > srv->max_probes)
+        return fgss_fail(FGSS_ERR_UNSUPPORTED, "more than %lld probes in one request", (long long)srv->max_probes);
+    for (size_t i = 0; i < probes->len.size(); i++) {
+        if (probes->len[i] >= 65536) return fgss_fail(FGSS_ERR_UNSUPPORTED, "probe %zu longer than 65535 tokens", i);
+        const int32_t *r = probes->tok.data() + probes->pos[i];
+        for (int32_t k = 0; k < probes->len[i]; k++)
+            if (r[k] < 0 || (k > 0 && r[k] <= r[k - 1]))
+                return fgss_fail(FGSS_ERR_UNSUPPORTED, "probe %zu: tokens must be non-negative and strictly ascending", i);
+    }
+    auto *r = new ServeRequest();
+    r->id = srv->next_id++;
+    r->probes = probes;
+    r->t = t;
+    // candidate-memory estimate: the postings the probe prefixes may scan, at 16 B each
+    double ptok = 0;
+    for (int32_t l : probes->len) ptok += (double)std::min<uint32_t>(fgss_max_prefix((uint32_t)l, t), (uint32_t)l);
+    r->est = (int64_t)(16.0 * ptok * srv->mean_list) + 4096;
+    r->t_submit = std::chrono::steady_clock::now();
+    srv->reqs[r->id] = r;
+    srv->pending.push_back(r);
+    *request_id = r->id;
+    return server_progress(srv, false);
+}
+
+extern "C" int fgss_server_poll(fgss_server *srv, int32_t block, int32_t *pending, int32_t *running)
+{
+    if (!srv) return fgss_fail(FGSS_ERR_ARG, "null server");
+    int rc = server_progress(srv, block != 0);
+    if (pending) *pending = (int32_t)srv->pending.size();
+    if (running) {
+        int32_t k = 0;
+        for (ServeSlot &sl : srv->slots) k += sl.busy ? (int32_t)sl.reqs.size() : 0;
+        *running = k;
+    }
+    return rc;
+}
+
+static ServeRequest *find_request(fgss_server *srv, int64_t id)
+{
+    auto it = srv->reqs.find(id);
+    return it == srv->reqs.end() ? nullptr : it->second;
+}
+
+extern "C" int fgss_server_done(fgss_server *srv, int64_t request_id, int32_t *done)
+{
+    if (!srv || !done) return fgss_fail(FGSS_ERR_ARG, "null argument");
+    ServeRequest *r = find_request(srv, request_id);
+    if (!r) return fgss_fail(FGSS_ERR_ARG, "unknown request %lld", (long long)request_id);
+    *done = r->done ? 1 : 0;
+    return FGSS_OK;
+}
+
+extern "C" int fgss_server_wait(fgss_server *srv, int64_t request_id)
+{
+    if (!srv) return fgss_fail(FGSS_ERR_ARG, "null server");
+    ServeRequest *r = find_request(srv, request_id);
+    if (!r) return fgss_fail(FGSS_ERR_ARG, "unknown request %lld", (long long)request_id);
+    while (!r->done) {
+        bool busy = false;
+        for (ServeSlot &sl : srv->slots) busy |= sl.busy;
+        if (!busy && srv->hold)
+            return fgss_fail(FGSS_ERR_ARG, "request %lld is queued while admission is held", (long long)request_id);
+        int rc = server_progress(srv, true);
+        if (rc && !r->done) return rc;
+    }
+    return FGSS_OK;
+}
+
+extern "C" int fgss_server_hold(fgss_server *srv, int32_t hold)
+{
+    if (!srv) return fgss_fail(FGSS_ERR_ARG, "null server");
+    srv->hold = hold != 0;
+    return hold ? FGSS_OK : server_progress(srv, false);
+}
+
+extern "C" int fgss_server_result(fgss_server *srv, int64_t request_id, fgss_pairs **out, fgss_request_stats *stats)
+{
+    if (!srv || !out) return fgss_fail(FGSS_ERR_ARG, "null argument");
+    *out = nullptr;
+    ServeRequest *r = find_request(srv, request_id);
+    if (!r) return fgss_fail(FGSS_ERR_ARG, "unknown request %lld", (long long)request_id);
+    if (!r->done) return fgss_fail(FGSS_ERR_ARG, "request %lld is still running", (long long)request_id);
+    srv->reqs.erase(request_id);
+    std::unique_ptr<ServeRequest> own(r);
+    if (r->status != FGSS_OK) return fgss_fail(r->status, "request %lld failed on the device", (long long)request_id);
+    std::vector<uint64_t> order;
+    sort_pairs_by_key(r->tri, order);
+    const size_t R = order.size(), nb = std::max<size_t>(R, 1);
+    auto *p = (fgss_pairs *)calloc(1, sizeof(fgss_pairs));
+    if (p) {
+        p->q = (int32_t *)malloc(nb * 4); p->s = (int32_t *)malloc(nb * 4); p->rid = (int32_t *)malloc(nb * 4);
+        p->sid = (int32_t *)malloc(nb * 4); p->overlap = (int32_t *)malloc(nb * 4);
+        p->similarity = (float *)malloc(nb * 4);
+    }
+    if (!p || !p->q || !p->s || !p->rid || !p->sid || !p->overlap || !p->similarity) {
+        fgss_pairs_free(p);
+        return fgss_fail(FGSS_ERR_NOMEM, "out of host memory");
+    }
+    const fgss_collection *Rc = r->probes, *Sc = srv->coll;
+    for (size_t k = 0; k < R; k++) {
+        const size_t i = (size_t)(order[k] & 0xFFFFFFFFull);
+        const int32_t q = r->tri[3 * i], s = r->tri[3 * i + 1], ov = r->tri[3 * i + 2];
+        p->q[k] = q;
+        p->s[k] = s;
+        p->rid[k] = Rc->id[(size_t)q];
+        p->sid[k] = Sc->id[(size_t)s];
+        p->overlap[k] = ov;
+        p->similarity[k] = fgss_similarity((uint32_t)Rc->len[(size_t)q], (uint32_t)Sc->len[(size_t)s], (uint32_t)ov);
+    }
+    p->n_pairs = (int64_t)R;
+    for (int32_t l : Rc->len)
+        p->stats.probe_tokens += (int64_t)std::min<uint32_t>(fgss_max_prefix((uint32_t)l, r->t), (uint32_t)l);
+    p->stats.index_entries = (int64_t)srv->index_entries;
+    p->stats.postings_scanned = (int64_t)r->cnt[0];
+    p->stats.candidates = (int64_t)r->cnt[1];
+    p->stats.intersections = (int64_t)r->cnt[2];
+    p->stats.results = (int64_t)r->cnt[3];
+    p->stats.kernel_launches = r->launches;
+    if (stats) {
+        stats->ms_queue = std::chrono::duration<double, std::milli>(r->t_start - r->t_submit).count();
+        stats->ms_latency = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+        stats->batch_requests = r->batch_requests;
+        stats->reserved = 0;
+    }
+    *out = p;
+    return FGSS_OK;
+}
+
+extern "C" void fgss_server_free(fgss_server *srv)
+{
+    if (!srv) return;
+    for (ServeSlot &sl : srv->slots) {
+        if (!sl.d) continue;
+        if (sl.d->st) (void)hipStreamSynchronize(sl.d->st);
+        DevBuf *bufs[] = {&sl.b_tok, &sl.b_len, &sl.b_pos, &sl.b_sig, &sl.b_sig64, &sl.b_sigbad,
+                          &sl.b_job, &sl.b_jend, &sl.b_tj, &sl.b_wlo};
+        for (DevBuf *b : bufs)
+            if (b->p) (void)hipFree(b->p);
+        // the record arrays belong to the slot buffers freed above
+        sl.d->tok = sl.d->len = sl.d->pos = sl.d->sig = sl.d->sig64 = sl.d->job = sl.d->jend = nullptr;
+        sl.d->sigbad = nullptr;
+        sl.d->tj = nullptr;
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        fgss_collection holder;
+        holder.dev = sl.d;
+        fgss_device_release(&holder);
+    }
+    for (auto &kv : srv->reqs) delete kv.second;
+    if (srv->idx) {
+        fgss_collection holder;
+        holder.dev = srv->idx;
+        fgss_device_release(&holder);
+    }
+    delete srv;
 }

@@ -66,6 +66,12 @@ def lib():
                                            C.POINTER(i32p), C.POINTER(Counters)]
         L.orc_selfjoin_brute.restype = C.c_int64
         L.orc_selfjoin_brute.argtypes = [C.c_int64, i32p, i64p, i32p, f, C.c_int, C.POINTER(i32p)]
+        L.orc_rsjoin.restype = C.c_int64
+        L.orc_rsjoin.argtypes = [C.c_int64, i32p, i64p, i32p, C.c_int64, i32p, i64p, i32p, C.c_int32, f, f,
+                                 C.c_int, C.POINTER(i32p), C.POINTER(Counters)]
+        L.orc_rsjoin_brute.restype = C.c_int64
+        L.orc_rsjoin_brute.argtypes = [C.c_int64, i32p, i64p, i32p, C.c_int64, i32p, i64p, i32p, f,
+                                       C.POINTER(i32p)]
         _lib = L
     return _lib
 
@@ -157,6 +163,28 @@ def selfjoin(sets: Sets, t: float, nthreads: int = 0, mode: str = "closed", bloc
         raise ValueError("oracle join failed")
     tri = _take(out, 3 * n).reshape(-1, 3).astype(np.int32)
     return tri, ctr.as_dict()
+
+
+def rsjoin(probes: Sets, sets: Sets, t: float, t_index: float | None = None, nthreads: int = 0,
+           mode: str = "closed"):
+    """R-S join -> (triples int32 [R,3] of (probe, source, overlap) sorted, counters dict)"""
+    L = lib()
+    if nthreads <= 0:
+        nthreads = os.cpu_count() or 1
+    out = C.POINTER(C.c_int32)()
+    ctr = Counters()
+    args = (probes.n, _p32(probes.len), _p64(probes.pos), _p32(probes.tok),
+            sets.n, _p32(sets.len), _p64(sets.pos), _p32(sets.tok))
+    if mode == "closed":
+        n = L.orc_rsjoin(*args, sets.n_terms, t, t if t_index is None else t_index, nthreads, C.byref(out),
+                         C.byref(ctr))
+    elif mode == "brute":
+        n = L.orc_rsjoin_brute(*args, t, C.byref(out))
+    else:
+        raise ValueError(mode)
+    if n < 0:
+        raise ValueError("oracle R-S join failed")
+    return _take(out, 3 * n).reshape(-1, 3).astype(np.int32), ctr.as_dict()
 
 
 def similarity(a: int, b: int, inter: int) -> float:

@@ -300,14 +300,19 @@ static uint32_t prefix_clamp(uint32_t p, uint32_t len) { return p < len ? p : le
  * kernel :276-296).  Lists hold (set, pos) grouped by term, sets ascending. */
 typedef struct { int64_t *off; int32_t *sid; int32_t *spos; } orc_index;
 
-static void build_index(int64_t n, const int32_t *len, const int64_t *pos,
-                        const int32_t *tok, int32_t n_terms, float t, orc_index *ix,
-                        int64_t *e_idx)
+static uint32_t index_prefix(uint32_t l, float t, int use_max)
+{
+    return prefix_clamp(use_max ? orc_max_prefix(l, t) : orc_mid_prefix(l, t), l);
+}
+
+static void build_index_sel(int64_t n, const int32_t *len, const int64_t *pos,
+                            const int32_t *tok, int32_t n_terms, float t, int use_max,
+                            orc_index *ix, int64_t *e_idx)
 {
     ix->off = (int64_t *)calloc((size_t)n_terms + 1, sizeof(int64_t));
     int64_t E = 0;
     for (int64_t s = 0; s < n; s++) {
-        uint32_t mp = prefix_clamp(orc_mid_prefix((uint32_t)len[s], t), (uint32_t)len[s]);   /* data.cu:241-257 */
+        uint32_t mp = index_prefix((uint32_t)len[s], t, use_max);   /* data.cu:241-257 */
         for (uint32_t j = 0; j < mp; j++) ix->off[tok[pos[s] + j] + 1]++;
         E += mp;
     }
@@ -317,7 +322,7 @@ static void build_index(int64_t n, const int32_t *len, const int64_t *pos,
     int64_t *fill = (int64_t *)malloc((size_t)(n_terms > 0 ? n_terms : 1) * sizeof(int64_t));
     memcpy(fill, ix->off, (size_t)n_terms * sizeof(int64_t));
     for (int64_t s = 0; s < n; s++) {
-        uint32_t mp = prefix_clamp(orc_mid_prefix((uint32_t)len[s], t), (uint32_t)len[s]);
+        uint32_t mp = index_prefix((uint32_t)len[s], t, use_max);
         for (uint32_t j = 0; j < mp; j++) {
             int64_t at = fill[tok[pos[s] + j]]++;
             ix->sid[at] = (int32_t)s; ix->spos[at] = (int32_t)j;
@@ -325,6 +330,13 @@ static void build_index(int64_t n, const int32_t *len, const int64_t *pos,
     }
     free(fill);
     *e_idx = E;
+}
+
+static void build_index(int64_t n, const int32_t *len, const int64_t *pos,
+                        const int32_t *tok, int32_t n_terms, float t, orc_index *ix,
+                        int64_t *e_idx)
+{
+    build_index_sel(n, len, pos, tok, n_terms, t, 0, ix, e_idx);
 }
 
 static void free_index(orc_index *ix) { free(ix->off); free(ix->sid); free(ix->spos); }
@@ -663,4 +675,142 @@ int64_t orc_selfjoin_brute(int64_t n_sets, const int32_t *len, const int64_t *po
     free(ws); free(th);
     *out = res;
     return R;
+}
+
+/* ------------------------------------------------------------------------ */
+/* R-S join (probe records against an indexed collection)                    */
+/* ------------------------------------------------------------------------ */
+/* The reference is a self-join only; its R-S mode exists as a usage string
+ * (src/preprocess_cli.cpp:26).  The R-S rules carry the self-join's closed form
+ * over with both prefixes the probe prefix max_prefix (the classic R-S prefix
+ * filter): (r, s) is a result iff size_ok(|R|, |S|), m = |R[0,qp) n S[0,sp)| > 0
+ * with qp = max_prefix(|R|, t) and sp = max_prefix(|S|, t), the positional filter
+ * of filtering.cu:127-133 passes at the last prefix match, |R n S| >=
+ * floor(min_overlap) and the float32 Jaccard is >= t.  The index holds
+ * max_prefix(|S|, t_index) tokens (t_index <= t); postings past
+ * max_prefix(|S|, t) are skipped, after being counted in e_scan. */
+typedef struct {
+    int64_t np; const int32_t *plen; const int64_t *ppos; const int32_t *ptok;
+    int64_t n; const int32_t *len; const int64_t *pos; const int32_t *tok; int32_t n_terms;
+    float t;
+    const orc_index *ix;
+    atomic_llong next;
+} rs_shared;
+
+typedef struct { rs_shared *sh; tri_buf out; orc_counters c; } rs_worker;
+
+static void *rs_run(void *arg)
+{
+    rs_worker *w = (rs_worker *)arg;
+    rs_shared *sh = w->sh;
+    const float t = sh->t;
+    const int64_t n = sh->n;
+    int32_t *cnt = (int32_t *)calloc((size_t)(n > 0 ? n : 1), sizeof(int32_t));
+    int32_t *lastj = (int32_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    int32_t *lasts = (int32_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    int32_t *touched = (int32_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    for (;;) {
+        int64_t r = atomic_fetch_add(&sh->next, 1);
+        if (r >= sh->np) break;
+        const uint32_t lr = (uint32_t)sh->plen[r];
+        const int32_t *R = sh->ptok + sh->ppos[r];
+        const uint32_t qp = prefix_clamp(orc_max_prefix(lr, t), lr);
+        int64_t nt = 0;
+        w->c.p_q += qp;
+        for (uint32_t j = 0; j < qp; j++) {
+            int32_t tk = R[j];
+            if (tk >= sh->n_terms) continue;            /* a token the index never saw */
+            for (int64_t e = sh->ix->off[tk]; e < sh->ix->off[tk + 1]; e++) {
+                int32_t s = sh->ix->sid[e];
+                uint32_t ls = (uint32_t)sh->len[s];
+                if (!size_ok(lr, ls, t)) continue;
+                w->c.e_scan++;
+                if ((uint32_t)sh->ix->spos[e] >= prefix_clamp(orc_max_prefix(ls, t), ls)) continue;
+                if (cnt[s] == 0) touched[nt++] = s;
+                cnt[s]++; lastj[s] = (int32_t)j; lasts[s] = sh->ix->spos[e];
+            }
+        }
+        for (int64_t k = 0; k < nt; k++) {
+            int32_t s = touched[k];
+            uint32_t m = (uint32_t)cnt[s];
+            cnt[s] = 0;
+            uint32_t ls = (uint32_t)sh->len[s];
+            float bound = orc_min_overlap(lr, ls, t);
+            uint32_t rq = lr - (uint32_t)lastj[s], rs = ls - (uint32_t)lasts[s];
+            uint32_t minrem = rq < rs ? rq : rs;
+            if ((float)(m + minrem) < bound) continue;
+            w->c.candidates++;
+            uint32_t ov = overlap_full(R, lr, sh->tok + sh->pos[s], ls);
+            if (ov > 0 && (int)ov >= (int)floorf(bound)) {
+                w->c.intersections++;
+                if (orc_similarity(lr, ls, ov) >= t) {
+                    tri_push(&w->out, (int32_t)r, s, (int32_t)ov);
+                    w->c.results++;
+                }
+            }
+        }
+    }
+    free(cnt); free(lastj); free(lasts); free(touched);
+    return NULL;
+}
+
+static int64_t gather(tri_buf *bufs, size_t stride, int nthreads, int32_t **out)
+{
+    int64_t R = 0;
+    for (int i = 0; i < nthreads; i++) R += ((tri_buf *)((char *)bufs + (size_t)i * stride))->n;
+    int32_t *res = (int32_t *)malloc((size_t)(R > 0 ? R : 1) * 3 * sizeof(int32_t));
+    int64_t o = 0;
+    for (int i = 0; i < nthreads; i++) {
+        tri_buf *b = (tri_buf *)((char *)bufs + (size_t)i * stride);
+        if (b->n) memcpy(res + 3 * o, b->v, (size_t)b->n * 3 * sizeof(int32_t));
+        o += b->n;
+        free(b->v);
+    }
+    qsort(res, (size_t)R, 3 * sizeof(int32_t), cmp_tri);
+    *out = res;
+    return R;
+}
+
+int64_t orc_rsjoin(int64_t n_probes, const int32_t *plen, const int64_t *ppos, const int32_t *ptok,
+                   int64_t n_sets, const int32_t *len, const int64_t *pos, const int32_t *tok,
+                   int32_t n_terms, float t, float t_index, int nthreads, int32_t **out,
+                   orc_counters *ctr)
+{
+    if (nthreads < 1) nthreads = 1;
+    if (!(t_index <= t)) return -1;
+    orc_index ix;
+    orc_counters total;
+    memset(&total, 0, sizeof(total));
+    build_index_sel(n_sets, len, pos, tok, n_terms, t_index, 1, &ix, &total.e_idx);
+    rs_shared sh;
+    sh.np = n_probes; sh.plen = plen; sh.ppos = ppos; sh.ptok = ptok;
+    sh.n = n_sets; sh.len = len; sh.pos = pos; sh.tok = tok; sh.n_terms = n_terms; sh.t = t; sh.ix = &ix;
+    atomic_init(&sh.next, 0);
+    rs_worker *ws = (rs_worker *)calloc((size_t)nthreads, sizeof(rs_worker));
+    pthread_t *th = (pthread_t *)malloc((size_t)nthreads * sizeof(pthread_t));
+    for (int i = 0; i < nthreads; i++) { ws[i].sh = &sh; pthread_create(&th[i], NULL, rs_run, &ws[i]); }
+    for (int i = 0; i < nthreads; i++) pthread_join(th[i], NULL);
+    for (int i = 0; i < nthreads; i++) {
+        total.p_q += ws[i].c.p_q; total.e_scan += ws[i].c.e_scan; total.candidates += ws[i].c.candidates;
+        total.intersections += ws[i].c.intersections; total.results += ws[i].c.results;
+    }
+    int64_t R = gather(&ws[0].out, sizeof(rs_worker), nthreads, out);
+    free(ws); free(th); free_index(&ix);
+    if (ctr) *ctr = total;
+    return R;
+}
+
+/* every (r, s) with float32 Jaccard >= t and a common token (a superset of orc_rsjoin) */
+int64_t orc_rsjoin_brute(int64_t n_probes, const int32_t *plen, const int64_t *ppos, const int32_t *ptok,
+                         int64_t n_sets, const int32_t *len, const int64_t *pos, const int32_t *tok,
+                         float t, int32_t **out)
+{
+    tri_buf b = {0};
+    for (int64_t r = 0; r < n_probes; r++)
+        for (int64_t s = 0; s < n_sets; s++) {
+            uint32_t lr = (uint32_t)plen[r], ls = (uint32_t)len[s];
+            uint32_t ov = overlap_full(ptok + ppos[r], lr, tok + pos[s], ls);
+            if (ov > 0 && orc_similarity(lr, ls, ov) >= t) tri_push(&b, (int32_t)r, (int32_t)s, (int32_t)ov);
+        }
+    return gather(&b, sizeof(tri_buf), 1, out);
 }

@@ -30,6 +30,12 @@
  *                          tile loop), subset of orc_selfjoin_brute with the
  *                          difference confined to float-boundary pairs, and the
  *                          known answers on the fixture collection at 8 thresholds
+ *   orc_rsjoin             subset of orc_rsjoin_brute with the difference confined
+ *                          to float-boundary pairs; R-S of a collection with itself
+ *                          is symmetric, holds every non-empty self pair, and its
+ *                          q<s half equals orc_selfjoin (and so the fixture's known
+ *                          answers) away from float-boundary pairs; t_index <= t
+ *                          changes no result
  * No function is left "parity unpinned".
  */
 #ifndef FGSS_ORACLE_H
@@ -93,6 +99,21 @@ int64_t orc_selfjoin_literal(int64_t n_sets, const int32_t *len, const int64_t *
                              int64_t block_size, int32_t **out, orc_counters *ctr);
 int64_t orc_selfjoin_brute(int64_t n_sets, const int32_t *len, const int64_t *pos,
                            const int32_t *tok, float t, int nthreads, int32_t **out);
+
+/* R-S join of probe records (any order) against an indexed collection (internal
+ * order), both in one token space; tokens >= n_terms occur only in probes.
+ * Triples (r, s, overlap) sorted by (r, s).  t_index <= t is the threshold the
+ * index prefixes were cut at (a server's t_min); it changes only e_scan.
+ *   orc_rsjoin        closed form (the self-join's rules with max-prefixes on both
+ *                     sides), nthreads workers
+ *   orc_rsjoin_brute  every pair with float32 Jaccard >= t (superset), one thread */
+int64_t orc_rsjoin(int64_t n_probes, const int32_t *plen, const int64_t *ppos, const int32_t *ptok,
+                   int64_t n_sets, const int32_t *len, const int64_t *pos, const int32_t *tok,
+                   int32_t n_terms, float t, float t_index, int nthreads, int32_t **out,
+                   orc_counters *ctr);
+int64_t orc_rsjoin_brute(int64_t n_probes, const int32_t *plen, const int64_t *ppos, const int32_t *ptok,
+                         int64_t n_sets, const int32_t *len, const int64_t *pos, const int32_t *tok,
+                         float t, int32_t **out);
 
 #ifdef __cplusplus
 }

@@ -44,3 +44,14 @@ def assert_join_equal(oracle, sets, pairs, t, counters=None):
     assert st["intersections"] == ctr["intersections"]
     assert st["results"] == ctr["results"]
     return tri, ctr
+
+
+def records_to_bin(records, ids=None) -> np.ndarray:
+    """A .bin int32 stream whose internal order (file order reversed) is `records`
+    in list order; record k gets id ids[k] (default k + 1)."""
+    out = []
+    n = len(records)
+    for k in range(n - 1, -1, -1):
+        r = np.asarray(records[k], np.int32)
+        out += [int(ids[k]) if ids is not None else k + 1, int(r.size)] + r.tolist()
+    return np.array(out, np.int32)
