@@ -107,6 +107,130 @@ def batched_leg(fg, n_jobs: int, reps: int):
             "speedup": plain_s / batch_s, "autotune": tune, "bit_exact_vs_plain": bool(exact)}
 
 
+def _edit(line: bytes, rng, n_edits: int) -> bytes:
+    s = bytearray(line)
+    for _ in range(n_edits):
+        if not s:
+            break
+        i = int(rng.integers(len(s)))
+        op = int(rng.integers(3))
+        if op == 0:
+            s[i] = int(rng.integers(97, 123))
+        elif op == 1:
+            del s[i]
+        else:
+            s.insert(i, int(rng.integers(97, 123)))
+    return bytes(s)
+
+
+def _percentile(xs, p):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs, np.float64), p)) if len(xs) else 0.0
+
+
+def served_leg(fg, n_index: int, n_requests: int, slots: int):
+    """R-S serving against a resident index of n_index titles (seed 5, q=3, t_min=0.7):
+    requests of 1..4096 probes (log-uniform; 20% edited copies of indexed titles, the
+    rest fresh titles), t ~ U[0.7, 0.95].  Plain = one request at a time (submit, wait,
+    take the pairs); served = continuous batching of a Poisson stream, at a rate far
+    above capacity (throughput) and at half the measured capacity (latency)."""
+    import numpy as np
+    t_min = float(np.float32(0.7))
+    t0 = time.perf_counter()
+    text = fg.generate_text(fg.GEN_TITLES, n_index, 5)
+    idx = fg.Collection.from_text_bytes(text, 3)
+    log(f"served: index of {idx.info.n_sets} titles ({idx.info.n_tokens} tokens) in {time.perf_counter() - t0:.1f} s")
+    lines = [ln for ln in text.split(b"\n") if ln]
+    del text
+    rng = np.random.default_rng(55)
+    sizes = np.minimum(np.exp(rng.uniform(0.0, np.log(4096.0), n_requests)).astype(int), 4096)
+    sizes = np.maximum(sizes, 1)
+    ts = [float(np.float32(x)) for x in rng.uniform(0.7, 0.95, n_requests)]
+    fresh = [ln for ln in fg.generate_text(fg.GEN_TITLES, int(sizes.sum()), 50).split(b"\n") if ln]
+    reqs, f = [], 0
+    for n in sizes:
+        out = []
+        for _ in range(int(n)):
+            if rng.random() < 0.2:
+                out.append(_edit(lines[int(rng.integers(len(lines)))], rng, int(rng.integers(1, 4))))
+            else:
+                out.append(fresh[f % len(fresh)])
+                f += 1
+        reqs.append(fg.Collection.from_text_dict(b"\n".join(out) + b"\n", idx))
+    del lines, fresh
+    log(f"served: {n_requests} requests, {int(sizes.sum())} probes tokenized in {time.perf_counter() - t0:.1f} s")
+
+    # plain: one request at a time
+    srv = fg.Server(idx, t_min, slots=1)
+    for k in range(min(8, n_requests)):   # warm the slot's workspace
+        srv.result(srv.submit(reqs[k], ts[k]))
+    plain, plain_lat = [], []
+    t1 = time.perf_counter()
+    for r, t in zip(reqs, ts):
+        a = time.perf_counter()
+        plain.append(srv.result(srv.submit(r, t))[0])
+        plain_lat.append(time.perf_counter() - a)
+    plain_s = time.perf_counter() - t1
+    srv.close()
+    log(f"served: plain sequential {n_requests} requests in {plain_s * 1e3:.1f} ms")
+
+    def drive(rate):
+        """Open-loop Poisson arrivals at `rate` req/s; returns (makespan s, latencies s, pairs)."""
+        srv = fg.Server(idx, t_min, slots=slots)
+        for k in range(min(8, n_requests)):
+            srv.result(srv.submit(reqs[k], ts[k]))
+        gaps = np.random.default_rng(7).exponential(1.0 / rate, n_requests)
+        arrive = np.cumsum(gaps) - gaps[0]
+        out, lat = [None] * n_requests, [0.0] * n_requests
+        live = {}
+        i = 0
+        shared = []
+        start = time.perf_counter()
+        while i < n_requests or live:
+            now = time.perf_counter() - start
+            while i < n_requests and arrive[i] <= now:
+                live[srv.submit(reqs[i], ts[i])] = i
+                i += 1
+            srv.poll()
+            finished = [rid for rid in live if srv.done(rid)]
+            for rid in finished:
+                k = live.pop(rid)
+                out[k], st = srv.result(rid, wait=False)
+                lat[k] = time.perf_counter() - start - arrive[k]
+                shared.append(st["batch_requests"])
+            if not finished and i < n_requests and not live:
+                time.sleep(max(0.0, min(arrive[i] - (time.perf_counter() - start), 0.0005)))
+        makespan = time.perf_counter() - start
+        srv.close()
+        return makespan, lat, out, shared
+
+    sat_s, sat_lat, sat_out, sat_shared = drive(rate=1e7)
+    cap = n_requests / sat_s
+    log(f"served: saturated stream {n_requests} requests in {sat_s * 1e3:.1f} ms ({cap:.0f} req/s)")
+    lat_s, lat, lat_out, lat_shared = drive(rate=0.5 * cap)
+    log(f"served: Poisson at {0.5 * cap:.0f} req/s, p50 {_percentile(lat, 50) * 1e3:.2f} ms, "
+        f"p99 {_percentile(lat, 99) * 1e3:.2f} ms")
+    exact = True
+    for outs in (sat_out, lat_out):
+        for g, p in zip(outs, plain):
+            exact &= (np.array_equal(g.q, p.q) and np.array_equal(g.s, p.s) and np.array_equal(g.overlap, p.overlap)
+                      and g.stats["candidates"] == p.stats["candidates"])
+    idx.close()
+    ms = lambda xs, q: round(_percentile(xs, q) * 1e3, 3)
+    return {"index_records": n_index, "requests": n_requests, "probes": int(sizes.sum()), "slots": slots,
+            "t_min": t_min, "pairs": int(sum(len(p) for p in plain)),
+            "candidates": int(sum(p.stats["candidates"] for p in plain)),
+            "plain_req_per_s": n_requests / plain_s, "plain_p50_ms": ms(plain_lat, 50),
+            "plain_p99_ms": ms(plain_lat, 99),
+            "served_req_per_s": cap, "speedup": cap / (n_requests / plain_s),
+            "saturated_p50_ms": ms(sat_lat, 50), "saturated_p99_ms": ms(sat_lat, 99),
+            "saturated_mean_batch": float(np.mean(sat_shared)) if sat_shared else 0.0,
+            "poisson_rate": 0.5 * cap, "poisson_req_per_s": n_requests / lat_s,
+            "poisson_p50_ms": ms(lat, 50), "poisson_p99_ms": ms(lat, 99),
+            "poisson_mean_batch": float(np.mean(lat_shared)) if lat_shared else 0.0,
+            "bit_exact_vs_plain": bool(exact)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +240,9 @@ def main():
     ap.add_argument("--table-slots", type=int, default=0)
     ap.add_argument("--load-pct", type=int, default=0)
     ap.add_argument("--batch-jobs", type=int, default=512, help="jobs of the batched leg (0 skips it)")
+    ap.add_argument("--serve-index", type=int, default=2_000_000, help="indexed titles of the served leg")
+    ap.add_argument("--serve-requests", type=int, default=1000, help="requests of the served leg (0 skips it)")
+    ap.add_argument("--serve-slots", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the join does not shard across GPUs "
@@ -185,6 +312,8 @@ def main():
     }
     if args.batch_jobs > 0:
         out["batched"] = batched_leg(fg, args.batch_jobs, reps=3)
+    if args.serve_requests > 0:
+        out["served"] = served_leg(fg, args.serve_index, args.serve_requests, args.serve_slots)
     if not args.no_cpu:
         log("cpu baseline: running the oracle on the same step ...")
         cb, exact = cpu_baseline(coll, last)

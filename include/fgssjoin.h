@@ -224,10 +224,13 @@ int fgss_server_wait(fgss_server *srv, int64_t request_id);
  * caller can hand over a burst that is then batched as one. */
 int fgss_server_hold(fgss_server *srv, int32_t hold);
 /* Take a done request's pairs: q indexes `probes`, s the indexed collection
- * (both internal order), sorted by (q, s); a pair (r, s) is a result iff the size
- * filter holds, the max-prefixes of both records share a token, the positional
- * filter passes, |R n S| >= floor(min_overlap) and the float32 Jaccard is >= t --
- * the same whatever else ran in the same batch.  The request is then forgotten. */
+ * (both internal order), sorted by (q, s).  A pair (r, s) is a result iff it is one
+ * under the self-join's rules with the longer record as the query (the probe on
+ * ties): size filter, a common prefix token (max prefix of the longer record, mid
+ * prefix of the shorter), positional filter, |R n S| >= floor(min_overlap) and
+ * float32 Jaccard >= t -- the same whatever else ran in the same batch.  Serving a
+ * collection against itself thus returns the self-join's pairs for q < s.  The
+ * request is then forgotten. */
 int fgss_server_result(fgss_server *srv, int64_t request_id, fgss_pairs **out, fgss_request_stats *stats);
 void fgss_server_free(fgss_server *srv);
 

@@ -95,8 +95,11 @@ struct Item {
     uint32_t bofs;          // planned items: first boundary position of the query in the bounds table
     uint32_t job;           // the query's job
     float t;                // its threshold
-    uint32_t lq, qpos, qpoff;   // the query's length, token offset and probe-range offset
+    uint32_t lq, qpos, qpoff;   // the query's length (| kMidProbe), token offset and probe-range offset
 };
+// R-S items over the sources longer than the probe: the probe contributes its mid
+// prefix and the sources their max prefix (the longer record is the query)
+constexpr uint32_t kMidProbe = 0x10000u;
 constexpr uint32_t kWholeWindow = 0x80000000u;   // the item covers the query's whole window
 constexpr uint32_t kPlanned = 0x40000000u;       // part of a planned split: ranges come from the bounds table
 constexpr uint32_t kPadMask = 0x3FFFFFFFu;
@@ -532,8 +535,9 @@ __global__ __launch_bounds__(256) void k_bounds(int64_t n, const uint32_t *__res
     }
 }
 
-// wlo: window start per query (null: q + 1, the self-join window)
-__global__ void k_items(const uint32_t *__restrict__ wlo, const uint32_t *__restrict__ hi,
+// wlo: window start per query (null: q + 1, the self-join window).  rs: the queries
+// are R-S half-probes, 2r (sources longer than probe r) and 2r + 1 (the others).
+__global__ void k_items(int rs, const uint32_t *__restrict__ wlo, const uint32_t *__restrict__ hi,
                         const uint32_t *__restrict__ nparts,
                         const uint32_t *__restrict__ eq, const uint32_t *__restrict__ partoff,
                         const uint32_t *__restrict__ bofs, const uint32_t *__restrict__ job,
@@ -549,19 +553,21 @@ __global__ void k_items(const uint32_t *__restrict__ wlo, const uint32_t *__rest
     uint32_t o = partoff[q];
     if ((uint64_t)o + np > cap_items) return;   // too small: the host resizes and reruns
     const uint32_t per = (eq[q] + np - 1) / np;   // expected postings per part: sizes its LDS table
+    const int64_t r = rs ? q >> 1 : q;           // the record whose tokens probe
+    const uint32_t flag = rs && (q & 1) == 0 ? kMidProbe : 0u;
     for (uint32_t p = 0; p < np; p++) {
         Item it;
-        it.q = (uint32_t)q;
+        it.q = (uint32_t)r;
         it.a = part_start(a0, W, p, np);
         it.b = part_start(a0, W, p + 1, np);
         it.pad = per | (np == 1 ? kWholeWindow : kPlanned);
         it.part = p;
         it.nparts = np;
         it.bofs = bofs[q];
-        it.job = job[q];
+        it.job = job[r];
         it.t = tj[it.job];
-        it.lq = len[q];
-        it.qpos = pos[q];
+        it.lq = len[r] | flag;
+        it.qpos = pos[r];
         it.qpoff = poff[q];
         items[o + p] = it;
     }
@@ -726,9 +732,10 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             }
             cur_job = jq;
         }
-        const uint32_t lq = item.lq;
+        const uint32_t lq = item.lq & 0xFFFFu;
+        const bool midprobe = RS && (item.lq & kMidProbe) != 0;
         const uint32_t qpos = item.qpos;
-        uint32_t qp = fgss_max_prefix(lq, t);
+        uint32_t qp = midprobe ? fgss_mid_prefix(lq, t) : fgss_max_prefix(lq, t);
         if (qp > lq) qp = lq;
         // table sized for twice the expected postings (64 .. CAP slots)
         int tlog = ceil_log2(2 * (item.pad & kPadMask));
@@ -812,8 +819,10 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
 #pragma unroll
                 for (int k = 0; k < kUnroll; k++) {
                     bool valid = jv[k] != kEmpty;
-                    if (RS && valid)
-                        valid = (pe[k].y & 0xFFFFu) < fgss_max_prefix(pe[k].y >> 16, t);
+                    if (RS && valid) {   // the source's own prefix at t
+                        const uint32_t ls = pe[k].y >> 16;
+                        valid = (pe[k].y & 0xFFFFu) < (midprobe ? fgss_max_prefix(ls, t) : fgss_mid_prefix(ls, t));
+                    }
                     bool ok = true, isnew = false;
                     uint32_t h = 0;
                     if (valid) {
@@ -1240,7 +1249,7 @@ static int enqueue_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool timing)
                        probelen, poff, P<uint2>(d->b_rng), (const uint32_t *)nullptr, hi, nparts, bofs, psid, fence,
                        P<uint32_t>(d->b_bounds), d->cap_bounds);
     HIPCK(hipGetLastError());
-    hipLaunchKernelGGL(k_items, dim3(grid_for(n, 256)), dim3(256), 0, st, (const uint32_t *)nullptr, hi, nparts, eq,
+    hipLaunchKernelGGL(k_items, dim3(grid_for(n, 256)), dim3(256), 0, st, 0, (const uint32_t *)nullptr, hi, nparts, eq,
                        partoff, bofs, d->job, d->tj,
                        d->len, d->pos, poff, n, P<Item>(d->b_items), d->cap_items);
     HIPCK(hipGetLastError());
@@ -1775,12 +1784,16 @@ extern "C" int fgss_batch_autotune(fgss_batch *b, int32_t reps, fgss_join_opts *
 // with continuous batching.  No reference counterpart (the reference's
 // "dualjoin" is only a usage string, src/preprocess_cli.cpp:26).
 //
-// R-S semantics, by analogy with the self-join: probe r and indexed record s form a
-// result at threshold t iff the 4-way size filter holds, m = |R[0,qp) n S[0,sp)| > 0
-// with qp = max_prefix(|R|, t) and sp = max_prefix(|S|, t), the positional filter
-// passes at the last prefix match, |R n S| >= floor(min_overlap) and the float32
-// Jaccard is >= t.  The index holds max_prefix(|S|, t_min) tokens per record, which
-// covers every t >= t_min; postings past max_prefix(|S|, t) are skipped.
+// R-S semantics: the self-join's closed form with the longer record of each pair as
+// the query (the probe on ties).  Probe r and indexed record s form a result at
+// threshold t iff the 4-way size filter holds, m = |R[0,qp) n S[0,sp)| > 0 with
+// (qp, sp) = (max_prefix(|R|), mid_prefix(|S|)) when |S| <= |R| and
+// (mid_prefix(|R|), max_prefix(|S|)) when |S| > |R|, the positional filter passes at
+// the last prefix match, |R n S| >= floor(min_overlap) and the float32 Jaccard is
+// >= t.  So an R-S join of a collection with itself has the self-join as its q < s
+// half.  Each probe runs as two half-probes over the two parts of its size band.
+// The index holds max_prefix(|S|, t_min) tokens per record, which covers every
+// t >= t_min; postings past the source's own prefix at t are skipped.
 // ---------------------------------------------------------------------------
 
 // The largest partner length ls >= l with size_ok(l, ls) (at most max_len).
@@ -1798,9 +1811,24 @@ __device__ __forceinline__ uint32_t partner_lmax(uint32_t l, float t, uint32_t m
     return a - 1;
 }
 
-// Per probe (a wave each): the source window -- indexed records whose length is in
-// the size-filter band, a contiguous id range because lengths are non-increasing --
-// and each probe token's posting range inside it.
+// Probe-prefix lengths of the two half-probes of every probe r: 2r probes the
+// sources longer than r with r's mid prefix, 2r + 1 the others with its max prefix.
+__global__ void k_probelen_rs(const uint32_t *__restrict__ len, const uint32_t *__restrict__ job,
+                              const float *__restrict__ tj, uint32_t *__restrict__ probelen, int64_t np)
+{
+    int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= 2 * np) return;
+    const int64_t r = v >> 1;
+    const uint32_t l = len[r];
+    const float t = tj[job[r]];
+    const uint32_t p = (v & 1) ? fgss_max_prefix(l, t) : fgss_mid_prefix(l, t);
+    probelen[v] = p < l ? p : l;
+}
+
+// Per half-probe (a wave each): its source window -- indexed records in the
+// size-filter band of the probe, longer than it (2r) or not (2r + 1); a contiguous id
+// range because lengths are non-increasing -- and each probe token's posting range
+// inside it.
 __global__ __launch_bounds__(256) void k_plan_rs(const uint32_t *__restrict__ ptok, const uint32_t *__restrict__ ppos,
                                                  const uint32_t *__restrict__ plen, int64_t np,
                                                  const uint32_t *__restrict__ pjob, const float *__restrict__ tj,
@@ -1814,30 +1842,34 @@ __global__ __launch_bounds__(256) void k_plan_rs(const uint32_t *__restrict__ pt
                                                  uint32_t *__restrict__ bcnt)
 {
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
-    for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; r < np; r += nwaves) {
-        const uint32_t lr = plen[r], qp = probelen[r];
+    for (int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; v < 2 * np; v += nwaves) {
+        const int64_t r = v >> 1;
+        const bool longer = (v & 1) == 0;
+        const uint32_t lr = plen[r], qp = probelen[v];
         const float t = tj[pjob[r]];
-        const uint32_t lmin = partner_lmin(lr, t), lmax = partner_lmax(lr, t, smax_len);
-        uint32_t a = 0, b = ns;   // first source with length <= lmax
+        // band [lmin, lmax] split at lr: (lr, lmax] for 2r, [lmin, lr] for 2r + 1
+        const uint32_t top = longer ? partner_lmax(lr, t, smax_len) : lr;
+        const uint32_t bot = longer ? lr + 1 : partner_lmin(lr, t);
+        uint32_t a = 0, b = ns;   // first source with length <= top
         while (a < b) {
             uint32_t mid = (a + b) >> 1;
-            if (slen[mid] > lmax) a = mid + 1;
+            if (slen[mid] > top) a = mid + 1;
             else b = mid;
         }
         const uint32_t wlo = a;
-        b = ns;                   // first source with length < lmin
+        b = ns;                   // first source with length < bot
         while (a < b) {
             uint32_t mid = (a + b) >> 1;
-            if (slen[mid] >= lmin) a = mid + 1;
+            if (slen[mid] >= bot) a = mid + 1;
             else b = mid;
         }
         const uint32_t whi = a;
-        const uint32_t ro = poff[r];
+        const uint32_t ro = poff[v];
         uint32_t e = 0;
         for (uint32_t j = lane_id(); j < qp; j += kWave) {
             const uint32_t term = ptok[ppos[r] + j];
             uint32_t x = 0, y = 0;
-            if (term < n_terms) {   // a q-gram the index has never seen has no postings
+            if (term < n_terms && wlo < whi) {   // a q-gram the index has never seen has no postings
                 const uint32_t L0 = loff[term], L1 = loff[term + 1];
                 x = lower_bound_sid(psid, fence, L0, L1, wlo);
                 y = lower_bound_sid(psid, fence, x, L1, whi);
@@ -1847,12 +1879,12 @@ __global__ __launch_bounds__(256) void k_plan_rs(const uint32_t *__restrict__ pt
         }
         e = wave_sum(e);
         if (lane_id() == 0) {
-            wlo_out[r] = wlo;
-            hi_out[r] = whi;
-            eq_out[r] = e;
+            wlo_out[v] = wlo;
+            hi_out[v] = whi;
+            eq_out[v] = e;
             const uint32_t npt = e == 0 ? 0u : (e + target - 1) / target;
-            nparts[r] = npt;
-            bcnt[r] = npt > 1 ? (npt - 1) * qp : 0u;
+            nparts[v] = npt;
+            bcnt[v] = npt > 1 ? (npt - 1) * qp : 0u;
         }
     }
 }
@@ -1908,22 +1940,22 @@ struct fgss_server {
 static int slot_prepare(ServeSlot &sl)
 {
     fgss_device_sets *d = sl.d;
-    const int64_t n = std::max<int64_t>(d->n, 1), E = std::max<int64_t>(d->ntok, 1);
-    if (d->cap_items == 0) d->cap_items = (uint32_t)(2 * n + 4096);
+    // two half-probes per probe; their prefixes hold at most 2 x the probe tokens
+    const int64_t v = 2 * std::max<int64_t>(d->n, 1), E = 2 * std::max<int64_t>(d->ntok, 1);
+    if (d->cap_items == 0) d->cap_items = (uint32_t)(v + 4096);
     if (d->cap_bounds == 0) d->cap_bounds = (uint32_t)(E + 4096);
     if (d->cap_out == 0) d->cap_out = 65536;
-    GROW(d->b_midlen, (size_t)n * 4);
-    GROW(d->b_probelen, (size_t)n * 4);
-    GROW(d->b_poff, (size_t)(n + 1) * 4);
-    GROW(d->b_scanscr, (size_t)scan_scratch_elems(n + 1) * 4);
+    GROW(d->b_probelen, (size_t)v * 4);
+    GROW(d->b_poff, (size_t)(v + 1) * 4);
+    GROW(d->b_scanscr, (size_t)scan_scratch_elems(v + 1) * 4);
     GROW(d->b_rng, (size_t)E * sizeof(uint2));
-    GROW(sl.b_wlo, (size_t)n * 4);
-    GROW(d->b_hi, (size_t)n * 4);
-    GROW(d->b_eq, (size_t)n * 4);
-    GROW(d->b_nparts, (size_t)n * 4);
-    GROW(d->b_partoff, (size_t)(n + 1) * 4);
-    GROW(d->b_bcnt, (size_t)n * 4);
-    GROW(d->b_bofs, (size_t)(n + 1) * 4);
+    GROW(sl.b_wlo, (size_t)v * 4);
+    GROW(d->b_hi, (size_t)v * 4);
+    GROW(d->b_eq, (size_t)v * 4);
+    GROW(d->b_nparts, (size_t)v * 4);
+    GROW(d->b_partoff, (size_t)(v + 1) * 4);
+    GROW(d->b_bcnt, (size_t)v * 4);
+    GROW(d->b_bofs, (size_t)(v + 1) * 4);
     GROW(d->b_bounds, (size_t)d->cap_bounds * 4);
     GROW(d->b_items, (size_t)d->cap_items * sizeof(Item));
     GROW(d->b_ovf, (size_t)d->cap_items * 4);
@@ -1963,24 +1995,24 @@ static int enqueue_rs(fgss_server *srv, ServeSlot &sl, bool timing)
     HIPCK(fill<unsigned long long>(ctr, C_COUNT, 0ull, st));
     HIPCK(fill<unsigned long long>(jctr, (int64_t)d->njobs * kJobCtr, 0ull, st));
     HIPCK(fill<uint32_t>(misc, M_COUNT, 0u, st));
-    hipLaunchKernelGGL(k_midlen, dim3(grid_for(n, 256)), dim3(256), 0, st, d->len, d->job, d->tj,
-                       P<uint32_t>(d->b_midlen), probelen, n, 1);
+    const int64_t v = 2 * n;   // half-probes
+    hipLaunchKernelGGL(k_probelen_rs, dim3(grid_for(v, 256)), dim3(256), 0, st, d->len, d->job, d->tj, probelen, n);
     HIPCK(hipGetLastError());
-    HIPCK(exclusive_scan<uint32_t>(probelen, poff, n, misc + M_NP, scr, st));
+    HIPCK(exclusive_scan<uint32_t>(probelen, poff, v, misc + M_NP, scr, st));
     if (timing) HIPCK(hipEventRecord(d->ev[1], st));
-    hipLaunchKernelGGL(k_plan_rs, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 4096)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_plan_rs, dim3((unsigned)std::min<int64_t>(grid_for(v * kWave, 256), 4096)), dim3(256), 0, st,
                        d->tok, d->pos, d->len, n, d->job, d->tj, probelen, poff, x->len, (uint32_t)x->n,
                        (uint32_t)x->max_len, P<uint32_t>(x->b_loff), P<uint32_t>(x->b_psid), P<uint32_t>(x->b_fence),
                        x->n_terms, cfg.target, P<uint2>(d->b_rng), wlo, hi, nparts, eq, bcnt);
     HIPCK(hipGetLastError());
-    HIPCK(exclusive_scan<uint32_t>(nparts, partoff, n, misc + M_NITEMS, scr, st));
-    HIPCK(exclusive_scan<uint32_t>(bcnt, bofs, n, misc + M_NB, scr, st));
-    hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 8192)), dim3(256), 0, st, n,
+    HIPCK(exclusive_scan<uint32_t>(nparts, partoff, v, misc + M_NITEMS, scr, st));
+    HIPCK(exclusive_scan<uint32_t>(bcnt, bofs, v, misc + M_NB, scr, st));
+    hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(grid_for(v * kWave, 256), 8192)), dim3(256), 0, st, v,
                        probelen, poff, P<uint2>(d->b_rng), wlo, hi, nparts, bofs, P<uint32_t>(x->b_psid),
                        P<uint32_t>(x->b_fence), P<uint32_t>(d->b_bounds), d->cap_bounds);
     HIPCK(hipGetLastError());
-    hipLaunchKernelGGL(k_items, dim3(grid_for(n, 256)), dim3(256), 0, st, wlo, hi, nparts, eq, partoff, bofs, d->job,
-                       d->tj, d->len, d->pos, poff, n, P<Item>(d->b_items), d->cap_items);
+    hipLaunchKernelGGL(k_items, dim3(grid_for(v, 256)), dim3(256), 0, st, 1, wlo, hi, nparts, eq, partoff, bofs, d->job,
+                       d->tj, d->len, d->pos, poff, v, P<Item>(d->b_items), d->cap_items);
     HIPCK(hipGetLastError());
     if (timing) HIPCK(hipEventRecord(d->ev[4], st));
     int rc = launch_join_cap(d, rs_args(srv, sl), cfg.cap, false, true);

@@ -681,14 +681,16 @@ int64_t orc_selfjoin_brute(int64_t n_sets, const int32_t *len, const int64_t *po
 /* R-S join (probe records against an indexed collection)                    */
 /* ------------------------------------------------------------------------ */
 /* The reference is a self-join only; its R-S mode exists as a usage string
- * (src/preprocess_cli.cpp:26).  The R-S rules carry the self-join's closed form
- * over with both prefixes the probe prefix max_prefix (the classic R-S prefix
- * filter): (r, s) is a result iff size_ok(|R|, |S|), m = |R[0,qp) n S[0,sp)| > 0
- * with qp = max_prefix(|R|, t) and sp = max_prefix(|S|, t), the positional filter
- * of filtering.cu:127-133 passes at the last prefix match, |R n S| >=
- * floor(min_overlap) and the float32 Jaccard is >= t.  The index holds
- * max_prefix(|S|, t_index) tokens (t_index <= t); postings past
- * max_prefix(|S|, t) are skipped, after being counted in e_scan. */
+ * (src/preprocess_cli.cpp:26).  The R-S rules are the self-join's closed form
+ * (filtering.cu:76-141, checking.cu:93-175, block.cu:183-197) with the longer record
+ * of each pair as the query, the probe on ties: a probe r and an indexed s with
+ * |S| <= |R| use qp = max_prefix(|R|, t), sp = mid_prefix(|S|, t); with |S| > |R|,
+ * qp = mid_prefix(|R|, t), sp = max_prefix(|S|, t).  (r, s) is a result iff
+ * size_ok(|R|, |S|), m = |R[0,qp) n S[0,sp)| > 0, the positional filter passes at
+ * the last prefix match, |R n S| >= floor(min_overlap) and the float32 Jaccard is
+ * >= t.  The index holds max_prefix(|S|, t_index) tokens (t_index <= t); e_scan
+ * counts the postings of the probe's prefix tokens (qp of each half) inside the
+ * size band before postings past sp are skipped. */
 typedef struct {
     int64_t np; const int32_t *plen; const int64_t *ppos; const int32_t *ptok;
     int64_t n; const int32_t *len; const int64_t *pos; const int32_t *tok; int32_t n_terms;
@@ -714,18 +716,22 @@ static void *rs_run(void *arg)
         if (r >= sh->np) break;
         const uint32_t lr = (uint32_t)sh->plen[r];
         const int32_t *R = sh->ptok + sh->ppos[r];
-        const uint32_t qp = prefix_clamp(orc_max_prefix(lr, t), lr);
+        const uint32_t qmax = prefix_clamp(orc_max_prefix(lr, t), lr);
+        const uint32_t qmid = prefix_clamp(orc_mid_prefix(lr, t), lr);
         int64_t nt = 0;
-        w->c.p_q += qp;
-        for (uint32_t j = 0; j < qp; j++) {
+        w->c.p_q += qmax;
+        for (uint32_t j = 0; j < qmax; j++) {
             int32_t tk = R[j];
             if (tk >= sh->n_terms) continue;            /* a token the index never saw */
             for (int64_t e = sh->ix->off[tk]; e < sh->ix->off[tk + 1]; e++) {
                 int32_t s = sh->ix->sid[e];
                 uint32_t ls = (uint32_t)sh->len[s];
                 if (!size_ok(lr, ls, t)) continue;
+                const int longer = ls > lr;              /* s is the query of the pair */
+                if (longer && j >= qmid) continue;
                 w->c.e_scan++;
-                if ((uint32_t)sh->ix->spos[e] >= prefix_clamp(orc_max_prefix(ls, t), ls)) continue;
+                uint32_t sp = prefix_clamp(longer ? orc_max_prefix(ls, t) : orc_mid_prefix(ls, t), ls);
+                if ((uint32_t)sh->ix->spos[e] >= sp) continue;
                 if (cnt[s] == 0) touched[nt++] = s;
                 cnt[s]++; lastj[s] = (int32_t)j; lasts[s] = sh->ix->spos[e];
             }

@@ -32,10 +32,10 @@
  *                          known answers on the fixture collection at 8 thresholds
  *   orc_rsjoin             subset of orc_rsjoin_brute with the difference confined
  *                          to float-boundary pairs; R-S of a collection with itself
- *                          is symmetric, holds every non-empty self pair, and its
- *                          q<s half equals orc_selfjoin (and so the fixture's known
- *                          answers) away from float-boundary pairs; t_index <= t
- *                          changes no result
+ *                          has a q<s half equal to orc_selfjoin (and so reproduces
+ *                          the fixture's known answers), holds every non-empty self
+ *                          pair and is symmetric away from float-boundary pairs;
+ *                          t_index <= t changes no result
  * No function is left "parity unpinned".
  */
 #ifndef FGSS_ORACLE_H
@@ -104,8 +104,8 @@ int64_t orc_selfjoin_brute(int64_t n_sets, const int32_t *len, const int64_t *po
  * order), both in one token space; tokens >= n_terms occur only in probes.
  * Triples (r, s, overlap) sorted by (r, s).  t_index <= t is the threshold the
  * index prefixes were cut at (a server's t_min); it changes only e_scan.
- *   orc_rsjoin        closed form (the self-join's rules with max-prefixes on both
- *                     sides), nthreads workers
+ *   orc_rsjoin        closed form: the self-join's rules with the longer record of
+ *                     each pair as the query (the probe on ties), nthreads workers
  *   orc_rsjoin_brute  every pair with float32 Jaccard >= t (superset), one thread */
 int64_t orc_rsjoin(int64_t n_probes, const int32_t *plen, const int64_t *ppos, const int32_t *ptok,
                    int64_t n_sets, const int32_t *len, const int64_t *pos, const int32_t *tok,

@@ -188,3 +188,20 @@ def test_empty_index_and_empty_records(fg, oracle_mod, world):
     got = np.stack([pairs.q, pairs.s, pairs.overlap], 1) if len(pairs) else np.zeros((0, 3), np.int32)
     assert np.array_equal(got, tri) and len(tri) > 0
     srv.close()
+
+
+@pytest.mark.parametrize("t", [0.6, 0.8, 0.9])
+def test_served_against_itself_is_the_selfjoin(fg, t):
+    # the longer record of a pair is its query, so the q < s half is the self-join
+    coll = fg.Collection.generate(fg.GEN_TITLES, 20000, 12)
+    t = float(np.float32(t))
+    srv = fg.Server(coll, 0.6, slots=2)
+    pairs, _ = srv.result(srv.submit(coll, t))
+    srv.close()
+    sj = coll.selfjoin(t)
+    up = pairs.q < pairs.s
+    assert np.array_equal(pairs.q[up], sj.q) and np.array_equal(pairs.s[up], sj.s)
+    assert np.array_equal(pairs.overlap[up], sj.overlap)
+    assert np.array_equal(pairs.similarity[up].view(np.uint32), sj.similarity.view(np.uint32))
+    diag = pairs.q == pairs.s
+    assert diag.sum() == int((coll.arrays()[1] > 0).sum())
