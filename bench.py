@@ -31,12 +31,42 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(st):
-    """Bytes the join kernel's algorithm must move for one join: 4 B per probe token,
-    8 B per useful posting (record id, position|length), 64 B per 64-bucket signature
-    test, 128 B per 128-bucket test, 4 B per verified source token, 12 B per result."""
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_join_titles100k_pmc.txt")
+
+
+def alg_bytes_join_kernel(c):
+    """Algorithmic bytes of one join's candidate generation + verification (the
+    k_join launch), counted by the oracle from the reference algorithm: 4 B per probe
+    token, 8 B per useful posting (sid, pos), 4 B per token read of the reference's
+    merge loop (checking.cu:126-160), 8 B per result pair."""
+    return 4 * c["p_q"] + 8 * c["e_scan"] + 4 * c["m_reads"] + 8 * c["results"]
+
+
+def alg_bytes_join(c):
+    """... plus the index build: 12 B per index entry (read a token, write sid, pos)."""
+    return 12 * c["e_idx"] + alg_bytes_join_kernel(c)
+
+
+def design_bytes(st):
+    """Bytes this build's k_join reads for one join (no oracle needed): 4 B per probe
+    token, 8 B per useful posting, 64 B per 64-bucket signature test, 128 B per
+    128-bucket test, 4 B per verified source token, 12 B per result."""
     return (4 * st["probe_tokens"] + 8 * st["postings_scanned"] + 64 * st["signature_checks"]
             + 128 * st["signature_checks_128"] + 4 * st["verify_tokens"] + 12 * st["results"])
+
+
+def pmc_summary():
+    """key -> value from the committed rocprofv3 PMC summary of this workload"""
+    out = {}
+    if os.path.exists(PMC_SUMMARY):
+        for ln in open(PMC_SUMMARY):
+            parts = ln.split()
+            if len(parts) == 2:
+                try:
+                    out[parts[0]] = float(parts[1])
+                except ValueError:
+                    pass
+    return out
 
 
 def cpu_baseline(coll, gpu_pairs):
@@ -50,16 +80,18 @@ def cpu_baseline(coll, gpu_pairs):
     threads = max(1, min(16, os.cpu_count() or 1))
     cand = 0
     exact = True
+    counters = []
     t0 = time.perf_counter()
     for t, p in zip(THRESHOLDS, gpu_pairs):
         tri, ctr = oracle.selfjoin(sets, t, nthreads=threads)
+        counters.append(ctr)
         cand += ctr["candidates"]
         got = np.stack([p.q, p.s, p.overlap], axis=1) if len(p) else np.zeros((0, 3), np.int32)
         exact &= got.shape == tri.shape and bool(np.array_equal(got, tri)) and p.stats["candidates"] == ctr["candidates"]
     dt = time.perf_counter() - t0
     return {"value": cand / dt, "unit": "candidates/s", "cores": threads, "kind": "port",
             "sample": f"full step: {N_RECORDS} titles x t={list(THRESHOLDS)}, {dt:.1f} s, "
-                      f"oracle/fgss_oracle.c closed form, {threads} threads"}, exact
+                      f"oracle/fgss_oracle.c closed form, {threads} threads"}, exact, counters
 
 
 def batched_leg(fg, n_jobs: int, reps: int):
@@ -271,26 +303,26 @@ def main():
     sync()
     t0 = time.perf_counter()
     last = None
-    kms = 0.0
-    kbytes = 0
+    kms = []
     for i in range(args.steps):
         last = step()
-        kms += sum(p.stats["ms_kernel"] for p in last)
-        kbytes += sum(kernel_bytes(p.stats) for p in last)
+        kms.append([p.stats["ms_kernel"] for p in last])
     sync()
     dt = time.perf_counter() - t0
     log(f"timed {args.steps} steps in {dt:.3f} s")
     cand = sum(p.stats["candidates"] for p in last)
     ms_per_step = dt * 1e3 / args.steps
-    per_t = {str(t): {"join_ms": round(p.stats["ms_total"], 3), "kernel_ms": round(p.stats["ms_kernel"], 3),
+    # k_join launch duration per threshold: mean over the timed steps (HIP events on
+    # the join's own stream around the launch)
+    kms_t = [sum(k[i] for k in kms) / len(kms) for i in range(len(THRESHOLDS))]
+    per_t = {str(t): {"join_ms": round(p.stats["ms_total"], 3), "kernel_ms": round(km, 3),
                       "index_ms": round(p.stats["ms_index"], 3), "pairs": len(p),
                       "candidates": p.stats["candidates"], "postings": p.stats["postings_scanned"],
                       "verify_tokens": p.stats["verify_tokens"], "sig_checks": p.stats["signature_checks"],
                       "verified": p.stats["verified"], "phase_cycles": p.stats["phase_cycles"]}
-             for t, p in zip(THRESHOLDS, last)}
-    achieved = kbytes / (kms / 1e3) / 1e9 if kms > 0 else 0.0
+             for t, p, km in zip(THRESHOLDS, last, kms_t)}
     out = {
-        "metric": "join ms + candidates verified/s (bit-exact)",
+        "metric": "join ms + candidates verified/s (bit-exact); batched & served p99 vs plain",
         "value": cand / (ms_per_step / 1e3),
         "unit": "candidates/s",
         "n_gpus": 1,
@@ -305,22 +337,39 @@ def main():
         "config": {"workload": f"plain self-join, {N_RECORDS} synthetic titles, q=3, t={list(THRESHOLDS)}",
                    "records": info.n_sets, "tokens": info.n_tokens, "terms": info.n_terms,
                    "per_threshold": per_t},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_join", "kernel_ms_per_step": kms / args.steps,
-                     "bytes_per_step": kbytes / args.steps},
     }
-    if args.batch_jobs > 0:
-        out["batched"] = batched_leg(fg, args.batch_jobs, reps=3)
-    if args.serve_requests > 0:
-        out["served"] = served_leg(fg, args.serve_index, args.serve_requests, args.serve_slots)
+    counters = None
     if not args.no_cpu:
         log("cpu baseline: running the oracle on the same step ...")
-        cb, exact = cpu_baseline(coll, last)
+        cb, exact, counters = cpu_baseline(coll, last)
         out["cpu_baseline"] = cb
         out["bit_exact_vs_oracle"] = exact
     else:
         out["cpu_baseline"] = None
+    kernel_s = sum(kms_t) / 1e3
+    n_launch = len(THRESHOLDS)
+    if counters is not None:
+        kbytes = sum(alg_bytes_join_kernel(c) for c in counters)
+        basis = "algorithmic bytes of the reference algorithm, counted by the oracle"
+        join_s = sum(p.stats["ms_total"] for p in last) / 1e3
+        join_frac = sum(alg_bytes_join(c) for c in counters) / join_s / (HBM_PEAK_GBS * 1e9)
+    else:
+        kbytes = sum(design_bytes(p.stats) for p in last)
+        basis = "bytes this build's kernel reads (oracle skipped)"
+        join_frac = None
+    achieved = kbytes / kernel_s / 1e9 if kernel_s > 0 else 0.0
+    prof = pmc_summary()
+    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": prof.get("traffic_bytes_per_launch"),
+                       "kernel": "k_join", "launches_per_step": n_launch,
+                       "bytes_per_launch": kbytes / n_launch, "kernel_ms_per_launch": kernel_s * 1e3 / n_launch,
+                       "basis": basis, "join_frac": join_frac,
+                       "lds_bank_conflict_ratio": prof.get("lds_bank_conflict_ratio"),
+                       "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) if prof else None}
+    if args.batch_jobs > 0:
+        out["batched"] = batched_leg(fg, args.batch_jobs, reps=3)
+    if args.serve_requests > 0:
+        out["served"] = served_leg(fg, args.serve_index, args.serve_requests, args.serve_slots)
     print(json.dumps(out), flush=True)
 
 
