@@ -76,6 +76,12 @@ int fgss_collection_from_text_buffer(const char *text, int64_t nbytes, int q, fg
 int fgss_collection_from_text_dict(const char *text, int64_t nbytes, const fgss_collection *dict,
                                    fgss_collection **out);
 
+/* The same tokenization on the current HIP device (newline scan, q-gram keys,
+ * occurrence numbers, sdbm hashes, frequency ranks and record order from stable
+ * radix sorts): results identical to fgss_collection_from_text_buffer, for
+ * q <= 8 and texts under 4 GiB (FGSS_ERR_UNSUPPORTED otherwise). */
+int fgss_collection_from_text_device(const char *text, int64_t nbytes, int q, fgss_collection **out);
+
 /* Write the collection as a reference .bin (records by size ascending, i.e.
  * internal order reversed) and, when loc_path is non-null and the collection
  * came from text, the .loc stream (q-gram start offset per stored token). */

@@ -36,6 +36,7 @@ EXPORTS = (
     "fgss_last_error", "fgss_version",
     "fgss_collection_from_bin", "fgss_collection_from_bin_buffer",
     "fgss_collection_from_text", "fgss_collection_from_text_buffer", "fgss_collection_from_text_dict",
+    "fgss_collection_from_text_device",
     "fgss_collection_write_bin", "fgss_collection_get_info", "fgss_collection_arrays",
     "fgss_collection_upload", "fgss_collection_free", "fgss_device_synchronize",
     "fgss_selfjoin", "fgss_pairs_free", "fgss_pairs_print",
@@ -123,6 +124,7 @@ def lib():
         "fgss_collection_from_text": [C.c_char_p, C.c_int, C.POINTER(vp)],
         "fgss_collection_from_text_buffer": [C.c_char_p, C.c_int64, C.c_int, C.POINTER(vp)],
         "fgss_collection_from_text_dict": [C.c_char_p, C.c_int64, vp, C.POINTER(vp)],
+        "fgss_collection_from_text_device": [C.c_char_p, C.c_int64, C.c_int, C.POINTER(vp)],
         "fgss_collection_write_bin": [vp, C.c_char_p, C.c_char_p],
         "fgss_collection_get_info": [vp, C.POINTER(CollectionInfo)],
         "fgss_collection_arrays": [vp, C.POINTER(i32p), C.POINTER(i32p), C.POINTER(i64p), C.POINTER(i32p)],
@@ -236,9 +238,12 @@ class Collection:
         return cls(h.value)
 
     @classmethod
-    def from_text_bytes(cls, text: bytes, q: int) -> "Collection":
+    def from_text_bytes(cls, text: bytes, q: int, device: bool = False) -> "Collection":
+        """Tokenize text (one record per non-empty line); device=True runs the
+        tokenizer on the GPU (q <= 8), with the same result."""
         h = C.c_void_p()
-        _check(lib().fgss_collection_from_text_buffer(text, len(text), q, C.byref(h)))
+        fn = lib().fgss_collection_from_text_device if device else lib().fgss_collection_from_text_buffer
+        _check(fn(text, len(text), q, C.byref(h)))
         return cls(h.value)
 
     @classmethod
@@ -250,8 +255,8 @@ class Collection:
         return cls(h.value)
 
     @classmethod
-    def generate(cls, kind: int, n_records: int, seed: int, q: int = 3) -> "Collection":
-        return cls.from_text_bytes(generate_text(kind, n_records, seed), q)
+    def generate(cls, kind: int, n_records: int, seed: int, q: int = 3, device: bool = False) -> "Collection":
+        return cls.from_text_bytes(generate_text(kind, n_records, seed), q, device=device)
 
     def close(self):
         if self._h is not None and self._h.value:

@@ -40,6 +40,7 @@ DEVICE = ["-x", "hip", f"--offload-arch={ARCH}",
 SOURCES = [
     ("collection.cpp", False),
     ("join.hip", True),
+    ("tokenize.hip", True),
 ]
 
 

@@ -178,7 +178,7 @@ __device__ __forceinline__ int64_t dev_count(int64_t n, const uint32_t *n_dev)
     return n_dev ? min(n, (int64_t)*n_dev) : n;
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t *__restrict__ keys, int64_t n,
+static __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t *__restrict__ keys, int64_t n,
                                                              const uint32_t *__restrict__ n_dev, int shift,
                                                              uint32_t *__restrict__ hist, int ntiles)
 {
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t *__r
 
 // Stable scatter.  offs[d * ntiles + tile] = exclusive-scanned histogram.
 // If idx_in is null the index of element i is i itself.
-__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t *__restrict__ keys_in,
+static __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t *__restrict__ keys_in,
                                                                 const uint32_t *__restrict__ idx_in, int64_t n,
                                                                 const uint32_t *__restrict__ n_dev, int shift,
                                                                 const uint32_t *__restrict__ offs, int ntiles,
@@ -273,8 +273,12 @@ inline int64_t radix_scratch_elems(int64_t n)
 // Result in (keys_b, idx_b) after an odd number of passes, else (keys_a, idx_a); the
 // function returns which: 0 -> a, 1 -> b.  keys_a holds the input keys.  With n_dev the
 // element count is read on the device (n is then the capacity the launch is sized for).
-inline hipError_t radix_sort_pairs(uint32_t *keys_a, uint32_t *idx_a, uint32_t *keys_b, uint32_t *idx_b, int64_t n,
-                                   const uint32_t *n_dev, int bits, uint32_t *scratch, hipStream_t st, int *which)
+// Stable sort of (key, value) pairs by key bits [0, bits).  Values come from vals_a
+// when vals_given, else the first pass takes the identity (value of element i = i).
+// The result is in (keys_b, vals_b) when *which is 1, else in (keys_a, vals_a).
+inline hipError_t radix_sort_kv(uint32_t *keys_a, uint32_t *vals_a, uint32_t *keys_b, uint32_t *vals_b, int64_t n,
+                                const uint32_t *n_dev, int bits, bool vals_given, uint32_t *scratch, hipStream_t st,
+                                int *which)
 {
     *which = 0;
     if (n <= 0) return hipSuccess;
@@ -283,14 +287,13 @@ inline hipError_t radix_sort_pairs(uint32_t *keys_a, uint32_t *idx_a, uint32_t *
     uint32_t *sscr = scratch + (int64_t)ntiles * 256;
     int passes = (bits + 7) / 8;
     if (passes < 1) passes = 1;
-    // even passes read a and write b, odd passes read b and write a; the first
-    // pass takes the implicit identity index
+    // even passes read a and write b, odd passes read b and write a
     for (int p = 0; p < passes; p++) {
         const bool even = (p & 1) == 0;
         const uint32_t *ki = even ? keys_a : keys_b;
-        const uint32_t *ii = p == 0 ? nullptr : (even ? idx_a : idx_b);
+        const uint32_t *ii = (p == 0 && !vals_given) ? nullptr : (even ? vals_a : vals_b);
         uint32_t *ko = even ? keys_b : keys_a;
-        uint32_t *io = even ? idx_b : idx_a;
+        uint32_t *io = even ? vals_b : vals_a;
         hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kSortThreads), 0, st, ki, n, n_dev, 8 * p, hist, ntiles);
         hipError_t e = exclusive_scan<uint32_t>(hist, hist, (int64_t)ntiles * 256, nullptr, sscr, st);
         if (e != hipSuccess) return e;
@@ -301,6 +304,13 @@ inline hipError_t radix_sort_pairs(uint32_t *keys_a, uint32_t *idx_a, uint32_t *
     }
     *which = (passes & 1) ? 1 : 0;
     return hipSuccess;
+}
+
+// Stable sort of keys with their original indices (identity values).
+inline hipError_t radix_sort_pairs(uint32_t *keys_a, uint32_t *idx_a, uint32_t *keys_b, uint32_t *idx_b, int64_t n,
+                                   const uint32_t *n_dev, int bits, uint32_t *scratch, hipStream_t st, int *which)
+{
+    return radix_sort_kv(keys_a, idx_a, keys_b, idx_b, n, n_dev, bits, false, scratch, st, which);
 }
 
 }  // namespace fgss
