@@ -103,7 +103,7 @@ def batched_leg(fg, n_jobs: int, reps: int):
     sizes = np.exp(rng.uniform(np.log(500), np.log(50000), n_jobs)).astype(int)
     ts = [float(np.float32(x)) for x in rng.choice(np.round(np.arange(0.5, 0.951, 0.05), 2), n_jobs)]
     t0 = time.perf_counter()
-    colls = [fg.Collection.generate(fg.GEN_TITLES, int(n), 4000 + j) for j, n in enumerate(sizes)]
+    colls = [fg.Collection.generate(fg.GEN_TITLES, int(n), 4000 + j, device=True) for j, n in enumerate(sizes)]
     log(f"batched: generated {n_jobs} collections, {int(sizes.sum())} records in {time.perf_counter() - t0:.1f} s")
     for c in colls:
         c.upload()
@@ -160,7 +160,7 @@ def _percentile(xs, p):
     return float(np.percentile(np.asarray(xs, np.float64), p)) if len(xs) else 0.0
 
 
-def served_leg(fg, n_index: int, n_requests: int, slots: int):
+def served_leg(fg, n_index: int, n_requests: int, slots: int, check_tokenizer: bool = True):
     """R-S serving against a resident index of n_index titles (seed 5, q=3, t_min=0.7):
     requests of 1..4096 probes (log-uniform; 20% edited copies of indexed titles, the
     rest fresh titles), t ~ U[0.7, 0.95].  Plain = one request at a time (submit, wait,
@@ -170,8 +170,23 @@ def served_leg(fg, n_index: int, n_requests: int, slots: int):
     t_min = float(np.float32(0.7))
     t0 = time.perf_counter()
     text = fg.generate_text(fg.GEN_TITLES, n_index, 5)
-    idx = fg.Collection.from_text_bytes(text, 3)
-    log(f"served: index of {idx.info.n_sets} titles ({idx.info.n_tokens} tokens) in {time.perf_counter() - t0:.1f} s")
+    t1 = time.perf_counter()
+    idx = fg.Collection.from_text_bytes(text, 3, device=True)
+    dev_s = time.perf_counter() - t1
+    log(f"served: index of {idx.info.n_sets} titles ({idx.info.n_tokens} tokens) tokenized on the GPU in "
+        f"{dev_s:.2f} s")
+    tok = {"records": int(idx.info.n_sets), "bytes": len(text), "qgrams": int(idx.info.n_tokens),
+           "terms": int(idx.info.n_terms), "device_s": dev_s}
+    if check_tokenizer:
+        t1 = time.perf_counter()
+        host = fg.Collection.from_text_bytes(text, 3)
+        host_s = time.perf_counter() - t1
+        exact = all(np.array_equal(a, b) for a, b in zip(host.arrays(), idx.arrays()))
+        exact &= host.info.n_terms == idx.info.n_terms
+        host.close()
+        tok.update({"host_s": host_s, "host_threads": min(16, os.cpu_count() or 1), "speedup": host_s / dev_s,
+                    "bit_exact": bool(exact)})
+        log(f"served: host tokenizer {host_s:.2f} s, bit-exact {exact}")
     lines = [ln for ln in text.split(b"\n") if ln]
     del text
     rng = np.random.default_rng(55)
@@ -260,7 +275,7 @@ def served_leg(fg, n_index: int, n_requests: int, slots: int):
             "poisson_rate": 0.5 * cap, "poisson_req_per_s": n_requests / lat_s,
             "poisson_p50_ms": ms(lat, 50), "poisson_p99_ms": ms(lat, 99),
             "poisson_mean_batch": float(np.mean(lat_shared)) if lat_shared else 0.0,
-            "bit_exact_vs_plain": bool(exact)}
+            "bit_exact_vs_plain": bool(exact), "tokenize_index": tok}
 
 
 def main():
@@ -288,7 +303,7 @@ def main():
     sync = fg.device_synchronize
 
     t0 = time.perf_counter()
-    coll = fg.Collection.generate(fg.GEN_TITLES, N_RECORDS, SEED)
+    coll = fg.Collection.generate(fg.GEN_TITLES, N_RECORDS, SEED, device=True)
     info = coll.info
     log(f"generated+tokenized {info.n_sets} records, {info.n_tokens} tokens, {info.n_terms} terms "
         f"in {time.perf_counter() - t0:.1f} s")

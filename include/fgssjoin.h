@@ -129,10 +129,11 @@ typedef struct {
 } fgss_join_stats;
 
 typedef struct {
-    int32_t load_pct;          /* postings per work item as % of its table slots, 0 = default (50) */
+    int32_t load_pct;          /* postings per work item as % of its table slots, 0 = default (65) */
     int32_t table_slots;       /* LDS candidate table slots per work item: 256..4096, 0 = default (512) */
     int32_t verify_only;       /* reserved, must be 0                                 */
-    int32_t reserved;
+    int32_t bounds_cap;        /* precomputed split boundaries at most, 0 = default (2^28); the
+                                  parts of queries beyond it locate their posting ranges by search */
 } fgss_join_opts;
 
 typedef struct {
@@ -153,6 +154,13 @@ typedef struct {
  * returned.  opts may be null. */
 int fgss_selfjoin(fgss_collection *c, float threshold, const fgss_join_opts *opts, fgss_pairs **out);
 void fgss_pairs_free(fgss_pairs *p);
+
+/* Print the self-join's inverted index at threshold t as the reference's debug
+ * mode does (print_inverted_index, src/fgssjoin/index.cu:218-243): per term,
+ * "term: [t], size: n" and the ids of the records whose mid prefix holds it.  Lists
+ * are in internal order (the reference's are in atomic, arbitrary order) and a term
+ * with no entries prints its own id (the reference prints a neighbour's). */
+int fgss_collection_print_index(fgss_collection *c, float threshold, void *file /* FILE* */);
 
 /* Print pairs as the reference does: "%d %d %f\n" = id[q] id[s] similarity
  * (src/fgssjoin/block.cu:193). */

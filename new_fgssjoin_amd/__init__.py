@@ -39,7 +39,7 @@ EXPORTS = (
     "fgss_collection_from_text_device",
     "fgss_collection_write_bin", "fgss_collection_get_info", "fgss_collection_arrays",
     "fgss_collection_upload", "fgss_collection_free", "fgss_device_synchronize",
-    "fgss_selfjoin", "fgss_pairs_free", "fgss_pairs_print",
+    "fgss_selfjoin", "fgss_pairs_free", "fgss_pairs_print", "fgss_collection_print_index",
     "fgss_generate_text", "fgss_free",
     "fgss_batch_create", "fgss_batch_set_thresholds", "fgss_batch_run", "fgss_batch_pairs",
     "fgss_batch_autotune", "fgss_batch_free",
@@ -77,7 +77,7 @@ class JoinStats(C.Structure):
 
 class JoinOpts(C.Structure):
     _fields_ = [("load_pct", C.c_int32), ("table_slots", C.c_int32), ("verify_only", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("bounds_cap", C.c_int32)]
 
 
 class BatchStats(C.Structure):
@@ -132,6 +132,7 @@ def lib():
         "fgss_device_synchronize": [],
         "fgss_selfjoin": [vp, C.c_float, C.POINTER(JoinOpts), C.POINTER(C.POINTER(_Pairs))],
         "fgss_pairs_print": [C.POINTER(_Pairs), vp],
+        "fgss_collection_print_index": [vp, C.c_float, vp],
         "fgss_generate_text": [C.c_int, C.c_int64, C.c_uint64, C.POINTER(C.c_void_p), i64p],
         "fgss_batch_create": [C.POINTER(vp), C.POINTER(C.c_float), C.c_int32, C.POINTER(vp)],
         "fgss_batch_set_thresholds": [vp, C.POINTER(C.c_float)],
@@ -309,9 +310,9 @@ class Collection:
     def upload(self):
         _check(lib().fgss_collection_upload(self._h))
 
-    def selfjoin(self, threshold: float, table_slots: int = 0, load_pct: int = 0) -> Pairs:
+    def selfjoin(self, threshold: float, table_slots: int = 0, load_pct: int = 0, bounds_cap: int = 0) -> Pairs:
         L = lib()
-        opts = JoinOpts(load_pct, table_slots, 0, 0)
+        opts = JoinOpts(load_pct, table_slots, 0, bounds_cap)
         out = C.POINTER(_Pairs)()
         _check(L.fgss_selfjoin(self._h, C.c_float(threshold), C.byref(opts), C.byref(out)))
         return _take_pairs(out)
@@ -385,10 +386,10 @@ class Server:
     results are taken (record ids come from them)."""
 
     def __init__(self, indexed: Collection, t_min: float, slots: int = 2, budget_bytes: int = 0,
-                 table_slots: int = 0, load_pct: int = 0):
+                 table_slots: int = 0, load_pct: int = 0, bounds_cap: int = 0):
         self.indexed = indexed
         self._probes = {}
-        opts = JoinOpts(load_pct, table_slots, 0, 0)
+        opts = JoinOpts(load_pct, table_slots, 0, bounds_cap)
         h = C.c_void_p()
         _check(lib().fgss_server_create(indexed._h, C.c_float(t_min), slots, budget_bytes, C.byref(opts),
                                          C.byref(h)))

@@ -7,6 +7,8 @@
 // stdout: one "id id similarity" line per result pair (src/fgssjoin/block.cu:193);
 // stderr: the reference's counters and timings (fgssjoin_cli.cu:55-108, block.cu:209-212).
 // With -o the tokenized collection is also written as <file.bin> and <file.bin>.loc.
+// Text is tokenized on the GPU for q <= 8 (identical result), on the host otherwise or
+// with --host-tokenizer.  debug also dumps the inverted index (fgssjoin_cli.cu:93-94).
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -14,6 +16,7 @@
 #include <unistd.h>
 
 #include <string>
+#include <vector>
 
 #include "fgssjoin.h"
 
@@ -24,10 +27,26 @@ static double now_ms()
     return (double)tv.tv_sec * 1e3 + (double)tv.tv_usec / 1e3;
 }
 
+// whole file into buf; FGSS_ERR_IO when it cannot be read
+static int read_all(const char *path, std::vector<char> &buf)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) {
+        fprintf(stderr, "File error\n");
+        return FGSS_ERR_IO;
+    }
+    char chunk[1 << 16];
+    size_t k;
+    while ((k = fread(chunk, 1, sizeof(chunk), f)) > 0) buf.insert(buf.end(), chunk, chunk + k);
+    fclose(f);
+    return FGSS_OK;
+}
+
 static int usage(const char *argv0)
 {
     fprintf(stdout, "Usage:\n./%s <ppjoin bin file> <threshold> [verbose]\n", argv0);
-    fprintf(stdout, "       %s -f <text file> -q <q> -t <threshold> [verbose|debug] [-o <out.bin>]\n", argv0);
+    fprintf(stdout, "       %s -f <text file> -q <q> -t <threshold> [verbose|debug] [-o <out.bin>] [--host-tokenizer]\n",
+            argv0);
     return 1;
 }
 
@@ -35,7 +54,7 @@ int main(int argc, char **argv)
 {
     const char *path = nullptr, *tstr = nullptr, *outbin = nullptr;
     int q = 0;
-    bool text = false, verbose = false, debug = false;
+    bool text = false, verbose = false, debug = false, host_tok = false;
     if (argc < 3) return usage(argv[0]);
     if (argv[1][0] == '-' && argv[1][1] != '\0') {
         for (int i = 1; i < argc; i++) {
@@ -44,6 +63,7 @@ int main(int argc, char **argv)
             else if (a == "-q" && i + 1 < argc) q = atoi(argv[++i]);
             else if (a == "-t" && i + 1 < argc) tstr = argv[++i];
             else if (a == "-o" && i + 1 < argc) outbin = argv[++i];
+            else if (a == "--host-tokenizer") host_tok = true;
             else if (a == "verbose") verbose = true;
             else if (a == "debug") debug = true;
             else return usage(argv[0]);
@@ -72,7 +92,14 @@ int main(int argc, char **argv)
     fprintf(stderr, "Threshold: Jaccard %g\n", threshold);
     fprintf(stderr, "... LOADING DATASET ...\n");
     fgss_collection *c = nullptr;
-    int rc = text ? fgss_collection_from_text(path, q, &c) : fgss_collection_from_bin(path, &c);
+    int rc;
+    if (text && q <= 8 && !host_tok) {   // the device tokenizer packs q-grams of up to 8 bytes
+        std::vector<char> buf;
+        rc = read_all(path, buf);
+        if (rc == FGSS_OK) rc = fgss_collection_from_text_device(buf.data(), (int64_t)buf.size(), q, &c);
+    } else {
+        rc = text ? fgss_collection_from_text(path, q, &c) : fgss_collection_from_bin(path, &c);
+    }
     if (rc != FGSS_OK) {
         const char *msg = fgss_last_error();
         if (rc == FGSS_ERR_FORMAT) fprintf(stdout, "%s\n", msg);   // data.cu:67,83 print to stdout
@@ -115,6 +142,10 @@ int main(int argc, char **argv)
         return 1;
     }
     fprintf(stderr, "  - Time spent: %g ms\n", t01 - t00);
+    if (debug && fgss_collection_print_index(c, threshold, stderr) != FGSS_OK) {   // fgssjoin_cli.cu:93-94
+        fprintf(stderr, "%s\n", fgss_last_error());
+        return 1;
+    }
     fgss_pairs *p = nullptr;
     rc = fgss_selfjoin(c, threshold, nullptr, &p);
     if (rc != FGSS_OK) {

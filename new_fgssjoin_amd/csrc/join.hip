@@ -132,7 +132,7 @@ struct fgss_device_sets {
     uint64_t *h_jctr = nullptr;   // pinned mirror of the per-job counters
     hipGraph_t graph = nullptr;   // captured pipeline (batch runs)
     hipGraphExec_t graph_exec = nullptr;
-    int graph_key[3] = {0, 0, 0}; // table slots, load, capacity generation it was captured with
+    int graph_key[4] = {0, 0, 0, 0};   // table slots, load, capacity generation, boundary budget it was captured with
     int cap_gen = 0;              // bumped whenever a buffer the pipeline uses is reallocated
 };
 
@@ -461,7 +461,7 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
                                               const uint32_t *__restrict__ fence, uint32_t target,
                                               uint2 *__restrict__ rng, uint32_t *__restrict__ hi_out,
                                               uint32_t *__restrict__ nparts, uint32_t *__restrict__ eq_out,
-                                              uint32_t *__restrict__ bcnt)
+                                              uint64_t *__restrict__ bcnt)
 {
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < n; q += nwaves) {
@@ -494,7 +494,7 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
             eq_out[q] = e;
             const uint32_t np = e == 0 ? 0u : (e + target - 1) / target;
             nparts[q] = np;
-            bcnt[q] = np > 1 ? (np - 1) * qp : 0u;   // internal boundaries x probe tokens
+            bcnt[q] = np > 1 ? (uint64_t)(np - 1) * qp : 0ull;   // internal boundaries x probe tokens
         }
     }
 }
@@ -508,12 +508,14 @@ __device__ __forceinline__ uint32_t part_start(uint64_t a0, uint64_t W, uint32_t
 // For every query split into np > 1 parts: the position of each internal part
 // boundary in each probe token's posting range, computed once here so that k_join
 // reads its ranges instead of searching.  A wave per query; its lanes take the
-// (boundary, token) pairs, one independent binary search each.
+// (boundary, token) pairs, one independent binary search each.  Offsets are 64-bit:
+// a query whose rows do not fit the table's capacity is not planned (k_items applies
+// the same test and its items search their ranges instead).
 __global__ __launch_bounds__(256) void k_bounds(int64_t n, const uint32_t *__restrict__ probelen,
                                                 const uint32_t *__restrict__ poff, const uint2 *__restrict__ rng,
                                                 const uint32_t *__restrict__ wlo, const uint32_t *__restrict__ hi,
                                                 const uint32_t *__restrict__ nparts,
-                                                const uint32_t *__restrict__ bofs, const uint32_t *__restrict__ psid,
+                                                const uint64_t *__restrict__ bofs, const uint32_t *__restrict__ psid,
                                                 const uint32_t *__restrict__ fence, uint32_t *__restrict__ bounds,
                                                 uint32_t cap_bounds)
 {
@@ -523,11 +525,11 @@ __global__ __launch_bounds__(256) void k_bounds(int64_t n, const uint32_t *__res
         if (np < 2) continue;
         const uint32_t qp = probelen[q];
         const uint64_t a0 = wlo ? (uint64_t)wlo[q] : (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
+        const uint64_t total = (uint64_t)(np - 1) * qp;
+        if (bofs[q] + total > cap_bounds) continue;   // not planned
         uint32_t *B = bounds + bofs[q];
         const uint2 *R = rng + poff[q];
-        const uint32_t total = (np - 1) * qp;
-        if ((uint64_t)bofs[q] + total > cap_bounds) continue;   // too small: the host resizes and reruns
-        for (uint32_t k = lane_id(); k < total; k += kWave) {
+        for (uint32_t k = lane_id(); k < (uint32_t)total; k += kWave) {
             const uint32_t p = k / qp + 1, j = k - (p - 1) * qp;
             const uint2 r = R[j];
             B[k] = lower_bound_sid(psid, fence, r.x, r.y, part_start(a0, W, p, np));
@@ -540,7 +542,8 @@ __global__ __launch_bounds__(256) void k_bounds(int64_t n, const uint32_t *__res
 __global__ void k_items(int rs, const uint32_t *__restrict__ wlo, const uint32_t *__restrict__ hi,
                         const uint32_t *__restrict__ nparts,
                         const uint32_t *__restrict__ eq, const uint32_t *__restrict__ partoff,
-                        const uint32_t *__restrict__ bofs, const uint32_t *__restrict__ job,
+                        const uint64_t *__restrict__ bofs, const uint64_t *__restrict__ bcnt,
+                        uint32_t cap_bounds, const uint32_t *__restrict__ job,
                         const float *__restrict__ tj, const uint32_t *__restrict__ len,
                         const uint32_t *__restrict__ pos, const uint32_t *__restrict__ poff, int64_t n,
                         Item *__restrict__ items, uint32_t cap_items)
@@ -555,15 +558,18 @@ __global__ void k_items(int rs, const uint32_t *__restrict__ wlo, const uint32_t
     const uint32_t per = (eq[q] + np - 1) / np;   // expected postings per part: sizes its LDS table
     const int64_t r = rs ? q >> 1 : q;           // the record whose tokens probe
     const uint32_t flag = rs && (q & 1) == 0 ? kMidProbe : 0u;
+    // split queries read their ranges from the boundary table when k_bounds wrote
+    // their rows, else each part searches its ranges
+    const uint32_t split = bofs[q] + bcnt[q] <= cap_bounds ? kPlanned : 0u;
     for (uint32_t p = 0; p < np; p++) {
         Item it;
         it.q = (uint32_t)r;
         it.a = part_start(a0, W, p, np);
         it.b = part_start(a0, W, p + 1, np);
-        it.pad = per | (np == 1 ? kWholeWindow : kPlanned);
+        it.pad = per | (np == 1 ? kWholeWindow : split);
         it.part = p;
         it.nparts = np;
-        it.bofs = bofs[q];
+        it.bofs = split ? (uint32_t)bofs[q] : 0u;
         it.job = job[r];
         it.t = tj[it.job];
         it.lq = len[r] | flag;
@@ -614,8 +620,6 @@ struct JoinArgs {
     const Item *items;
     const uint32_t *n_items_dev;  // number of items (device); the launch is void if above n_items_cap
     uint32_t n_items_cap;
-    const uint32_t *nb_dev;       // entries of the boundary table (device); void if above nb_cap
-    uint32_t nb_cap;
     const uint32_t *job;        // per record: job id
     const float *tj;            // per job: Jaccard threshold
     unsigned long long *ctr_job;  // per job: kJobCtr counters
@@ -696,7 +700,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
     uint32_t *qstage = rpre + kTokBatch + 1;
     // a launch whose item list or boundary table did not fit does nothing: the host
     // sees the totals, grows the buffers and runs the pipeline again
-    if (*A.n_items_dev > A.n_items_cap || *A.nb_dev > A.nb_cap) return;
+    if (*A.n_items_dev > A.n_items_cap) return;
     const uint32_t n_items = *A.n_items_dev;
 
     for (uint32_t i = lane; i < CAP; i += kWave) { tkey[i] = kEmpty; tcnt[i] = 0; tlast[i] = 0; }
@@ -1092,11 +1096,19 @@ static int launch_join_cap(fgss_device_sets *d, const JoinArgs &A, int cap, bool
 static inline unsigned grid_for(int64_t n, int threads) { return (unsigned)std::max<int64_t>((n + threads - 1) / threads, 1); }
 
 // device totals written by the scans (indices into b_misc)
-enum Misc { M_NE = 0, M_NITEMS, M_NP, M_NB, M_NRESPLIT, M_COUNT = 16 };
+// device-side counts; M_NB64 (two words, 8-byte aligned) = boundary-table rows wanted
+enum Misc { M_NE = 0, M_NITEMS, M_NP, M_NRESPLIT, M_NB64 = 6, M_COUNT = 16 };
+constexpr uint32_t kBoundsBudget = 1u << 28;   // boundary-table entries at most (1 GiB)
+
+static uint64_t misc_nb(const fgss_device_sets *d)
+{
+    return (uint64_t)d->h_misc[M_NB64] | ((uint64_t)d->h_misc[M_NB64 + 1] << 32);
+}
 
 struct RunCfg {
     int cap = 512;          // LDS table slots per wave
     uint32_t target = 332;  // postings per work item
+    uint32_t bounds = 1u << 28;   // boundary-table entries at most
 };
 
 static RunCfg make_cfg(const fgss_join_opts *opts)
@@ -1105,6 +1117,7 @@ static RunCfg make_cfg(const fgss_join_opts *opts)
     c.cap = opts && opts->table_slots ? opts->table_slots : 512;
     const int load = opts && opts->load_pct > 0 ? std::min(opts->load_pct, 100) : 65;
     c.target = std::max<uint32_t>(1u, (uint32_t)(c.cap * load / 100));
+    if (opts && opts->bounds_cap > 0) c.bounds = std::min<uint32_t>((uint32_t)opts->bounds_cap, c.bounds);
     return c;
 }
 
@@ -1130,13 +1143,13 @@ static int prepare_buffers(fgss_device_sets *d)
 {
     const int64_t n = std::max<int64_t>(d->n, 1), E = std::max<int64_t>(d->ntok, 1);
     if (d->cap_items == 0) d->cap_items = (uint32_t)std::min<int64_t>(2 * n + E / 64 + 4096, UINT32_MAX / 2);
-    if (d->cap_bounds == 0) d->cap_bounds = (uint32_t)std::min<int64_t>(E + 4096, UINT32_MAX / 2);
+    if (d->cap_bounds == 0) d->cap_bounds = (uint32_t)std::min<int64_t>(E + 4096, kBoundsBudget);
     if (d->cap_out == 0) d->cap_out = 65536;
     GROW(d->b_midlen, (size_t)n * 4);
     GROW(d->b_midoff, (size_t)(n + 1) * 4);
     GROW(d->b_probelen, (size_t)n * 4);
     GROW(d->b_poff, (size_t)(n + 1) * 4);
-    GROW(d->b_scanscr, (size_t)scan_scratch_elems(n + 1) * 4);
+    GROW(d->b_scanscr, (size_t)scan_scratch_elems(n + 1) * 8);   // also 64-bit scans
     GROW(d->b_ka, (size_t)E * 4);
     GROW(d->b_kb, (size_t)E * 4);
     GROW(d->b_ia, (size_t)E * 4);
@@ -1152,8 +1165,8 @@ static int prepare_buffers(fgss_device_sets *d)
     GROW(d->b_eq, (size_t)n * 4);
     GROW(d->b_nparts, (size_t)n * 4);
     GROW(d->b_partoff, (size_t)(n + 1) * 4);
-    GROW(d->b_bcnt, (size_t)n * 4);
-    GROW(d->b_bofs, (size_t)(n + 1) * 4);
+    GROW(d->b_bcnt, (size_t)n * 8);
+    GROW(d->b_bofs, (size_t)(n + 1) * 8);
     GROW(d->b_bounds, (size_t)d->cap_bounds * 4);
     GROW(d->b_items, (size_t)d->cap_items * sizeof(Item));
     GROW(d->b_ovf, (size_t)d->cap_items * 4);
@@ -1175,7 +1188,6 @@ static JoinArgs join_args(fgss_device_sets *d)
     A.rng = P<uint2>(d->b_rng); A.poff = P<uint32_t>(d->b_poff); A.bounds = P<uint32_t>(d->b_bounds);
     A.items = P<Item>(d->b_items);
     A.n_items_dev = P<uint32_t>(d->b_misc) + M_NITEMS; A.n_items_cap = d->cap_items;
-    A.nb_dev = P<uint32_t>(d->b_misc) + M_NB; A.nb_cap = d->cap_bounds;
     A.job = d->job; A.tj = d->tj; A.ctr_job = P<unsigned long long>(d->b_jctr);
     A.out = P<int32_t>(d->b_out); A.out_cap = d->cap_out; A.ovf = P<uint32_t>(d->b_ovf);
     A.ctr = P<unsigned long long>(d->b_ctr);
@@ -1228,7 +1240,8 @@ static int enqueue_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool timing)
     uint32_t *probelen = P<uint32_t>(d->b_probelen), *poff = P<uint32_t>(d->b_poff);
     uint32_t *loff = P<uint32_t>(d->b_loff), *psid = P<uint32_t>(d->b_psid), *fence = P<uint32_t>(d->b_fence);
     uint32_t *hi = P<uint32_t>(d->b_hi), *eq = P<uint32_t>(d->b_eq), *nparts = P<uint32_t>(d->b_nparts),
-             *partoff = P<uint32_t>(d->b_partoff), *bcnt = P<uint32_t>(d->b_bcnt), *bofs = P<uint32_t>(d->b_bofs);
+             *partoff = P<uint32_t>(d->b_partoff);
+    uint64_t *bcnt = P<uint64_t>(d->b_bcnt), *bofs = P<uint64_t>(d->b_bofs), *scr64 = P<uint64_t>(d->b_scanscr);
     unsigned long long *ctr = P<unsigned long long>(d->b_ctr), *jctr = P<unsigned long long>(d->b_jctr);
 
     if (timing) HIPCK(hipEventRecord(d->ev[0], st));
@@ -1244,13 +1257,13 @@ static int enqueue_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool timing)
                        P<uint2>(d->b_rng), hi, nparts, eq, bcnt);
     HIPCK(hipGetLastError());
     HIPCK(exclusive_scan<uint32_t>(nparts, partoff, n, misc + M_NITEMS, scr, st));
-    HIPCK(exclusive_scan<uint32_t>(bcnt, bofs, n, misc + M_NB, scr, st));
+    HIPCK(exclusive_scan<uint64_t>(bcnt, bofs, n, (uint64_t *)(misc + M_NB64), scr64, st));
     hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 8192)), dim3(256), 0, st, n,
                        probelen, poff, P<uint2>(d->b_rng), (const uint32_t *)nullptr, hi, nparts, bofs, psid, fence,
-                       P<uint32_t>(d->b_bounds), d->cap_bounds);
+                       P<uint32_t>(d->b_bounds), std::min(d->cap_bounds, cfg.bounds));
     HIPCK(hipGetLastError());
     hipLaunchKernelGGL(k_items, dim3(grid_for(n, 256)), dim3(256), 0, st, 0, (const uint32_t *)nullptr, hi, nparts, eq,
-                       partoff, bofs, d->job, d->tj,
+                       partoff, bofs, bcnt, std::min(d->cap_bounds, cfg.bounds), d->job, d->tj,
                        d->len, d->pos, poff, n, P<Item>(d->b_items), d->cap_items);
     HIPCK(hipGetLastError());
     // 9: the join
@@ -1300,6 +1313,7 @@ static int capture_graph(fgss_device_sets *d, const RunCfg &cfg)
     d->graph_key[0] = cfg.cap;
     d->graph_key[1] = (int)cfg.target;
     d->graph_key[2] = d->cap_gen;
+    d->graph_key[3] = (int)cfg.bounds;
     return FGSS_OK;
 }
 
@@ -1332,7 +1346,7 @@ static int launch_once(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &op
         rc = launch_join_cap(d, ops.args(), cfg.cap, true, ops.rs);   // launch configuration, outside the capture
         if (rc) return rc;
         if (!d->graph_exec || d->graph_key[0] != cfg.cap || d->graph_key[1] != (int)cfg.target ||
-            d->graph_key[2] != d->cap_gen) {
+            d->graph_key[2] != d->cap_gen || d->graph_key[3] != (int)cfg.bounds) {
             rc = capture_graph(d, cfg);
             if (rc) return rc;
         }
@@ -1354,11 +1368,14 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
     T = RunTotals();
     T.jctr.assign((size_t)d->njobs * kJobCtr, 0);
     for (int attempt = 0;; attempt++) {
-        const uint32_t n_items = d->h_misc[M_NITEMS], nB = d->h_misc[M_NB];
+        const uint32_t n_items = d->h_misc[M_NITEMS];
+        const uint64_t nB = misc_nb(d);
         const uint64_t nres = d->h_ctr[C_RESULTS];
         bool again = false;
         if (n_items > d->cap_items) { d->cap_items = n_items + n_items / 4 + 1024; again = true; }
-        if (nB > d->cap_bounds) { d->cap_bounds = nB + nB / 4 + 1024; again = true; }
+        // a larger boundary table only saves searches: grow it within the budget
+        const uint64_t want = std::min<uint64_t>(nB + nB / 4 + 1024, cfg.bounds);
+        if (nB > d->cap_bounds && want > d->cap_bounds) { d->cap_bounds = (uint32_t)want; again = true; }
         if (nres > d->cap_out) {
             d->cap_out = (uint32_t)std::min<uint64_t>(nres + nres / 2 + 1024, UINT32_MAX / 16);
             again = true;
@@ -1410,7 +1427,6 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
             A.items = cur;
             A.n_items_dev = misc + M_NRESPLIT;
             A.n_items_cap = cnt;
-            A.nb_cap = UINT32_MAX;   // re-split items never read the boundary table
             A.out = P<int32_t>(d->b_out);
             A.out_cap = d->cap_out;
             A.ovf = P<uint32_t>(d->b_ovf);
@@ -1611,6 +1627,45 @@ extern "C" int fgss_pairs_print(const fgss_pairs *p, void *file)
 extern "C" int fgss_device_synchronize(void)
 {
     HIPCK(hipDeviceSynchronize());
+    return FGSS_OK;
+}
+
+// The self-join index at t, one line pair per term (print_inverted_index,
+// src/fgssjoin/index.cu:218-243): "term: [t], size: n" then the ids of the records
+// whose mid prefix holds the term, in internal order.
+extern "C" int fgss_collection_print_index(fgss_collection *c, float threshold, void *file)
+{
+    if (!c || !file) return fgss_fail(FGSS_ERR_ARG, "null argument");
+    if (!(threshold > 0.0f) || !(threshold < 1.0f))
+        return fgss_fail(FGSS_ERR_ARG, "Threshold must be a float in the range (0.0..1.0).");
+    if (!c->jend.empty()) return fgss_fail(FGSS_ERR_ARG, "a batch collection has one index per job");
+    int rc = fgss_collection_upload(c);
+    if (rc) return rc;
+    fgss_device_sets *d = c->dev;
+    rc = set_thresholds(d, &threshold, 1);
+    if (!rc) rc = prepare_buffers(d);
+    if (rc) return rc;
+    uint32_t *misc = P<uint32_t>(d->b_misc);
+    HIPCK(fill<uint32_t>(misc, M_COUNT, 0u, d->st));
+    if (d->n > 0) {
+        rc = enqueue_index(d, 0);
+        if (rc) return rc;
+    }
+    HIPCK(hipMemcpyAsync(d->h_misc, misc, M_COUNT * 4, hipMemcpyDeviceToHost, d->st));
+    HIPCK(hipStreamSynchronize(d->st));
+    const uint32_t E = d->n > 0 ? d->h_misc[M_NE] : 0u, T = d->n_terms;
+    std::vector<uint32_t> loff((size_t)T + 1, 0), psid((size_t)E);
+    if (d->n > 0) {
+        HIPCK(hipMemcpy(loff.data(), d->b_loff.p, ((size_t)T + 1) * 4, hipMemcpyDeviceToHost));
+        if (E) HIPCK(hipMemcpy(psid.data(), d->b_psid.p, (size_t)E * 4, hipMemcpyDeviceToHost));
+    }
+    FILE *f = (FILE *)file;
+    for (uint32_t t = 0; t < T; t++) {
+        const uint32_t a = loff[t], b = loff[t + 1];
+        fprintf(f, "term: [%u], size: %u\n", t, b - a);
+        for (uint32_t k = a; k < b; k++) fprintf(f, "(%u) ", (unsigned)c->id[psid[k]]);
+        fprintf(f, "\n\n");
+    }
     return FGSS_OK;
 }
 
@@ -1839,7 +1894,7 @@ __global__ __launch_bounds__(256) void k_plan_rs(const uint32_t *__restrict__ pt
                                                  uint32_t n_terms, uint32_t target, uint2 *__restrict__ rng,
                                                  uint32_t *__restrict__ wlo_out, uint32_t *__restrict__ hi_out,
                                                  uint32_t *__restrict__ nparts, uint32_t *__restrict__ eq_out,
-                                                 uint32_t *__restrict__ bcnt)
+                                                 uint64_t *__restrict__ bcnt)
 {
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; v < 2 * np; v += nwaves) {
@@ -1884,7 +1939,7 @@ __global__ __launch_bounds__(256) void k_plan_rs(const uint32_t *__restrict__ pt
             eq_out[v] = e;
             const uint32_t npt = e == 0 ? 0u : (e + target - 1) / target;
             nparts[v] = npt;
-            bcnt[v] = npt > 1 ? (npt - 1) * qp : 0u;
+            bcnt[v] = npt > 1 ? (uint64_t)(npt - 1) * qp : 0ull;
         }
     }
 }
@@ -1943,19 +1998,19 @@ static int slot_prepare(ServeSlot &sl)
     // two half-probes per probe; their prefixes hold at most 2 x the probe tokens
     const int64_t v = 2 * std::max<int64_t>(d->n, 1), E = 2 * std::max<int64_t>(d->ntok, 1);
     if (d->cap_items == 0) d->cap_items = (uint32_t)(v + 4096);
-    if (d->cap_bounds == 0) d->cap_bounds = (uint32_t)(E + 4096);
+    if (d->cap_bounds == 0) d->cap_bounds = (uint32_t)std::min<int64_t>(E + 4096, kBoundsBudget);
     if (d->cap_out == 0) d->cap_out = 65536;
     GROW(d->b_probelen, (size_t)v * 4);
     GROW(d->b_poff, (size_t)(v + 1) * 4);
-    GROW(d->b_scanscr, (size_t)scan_scratch_elems(v + 1) * 4);
+    GROW(d->b_scanscr, (size_t)scan_scratch_elems(v + 1) * 8);   // also 64-bit scans
     GROW(d->b_rng, (size_t)E * sizeof(uint2));
     GROW(sl.b_wlo, (size_t)v * 4);
     GROW(d->b_hi, (size_t)v * 4);
     GROW(d->b_eq, (size_t)v * 4);
     GROW(d->b_nparts, (size_t)v * 4);
     GROW(d->b_partoff, (size_t)(v + 1) * 4);
-    GROW(d->b_bcnt, (size_t)v * 4);
-    GROW(d->b_bofs, (size_t)(v + 1) * 4);
+    GROW(d->b_bcnt, (size_t)v * 8);
+    GROW(d->b_bofs, (size_t)(v + 1) * 8);
     GROW(d->b_bounds, (size_t)d->cap_bounds * 4);
     GROW(d->b_items, (size_t)d->cap_items * sizeof(Item));
     GROW(d->b_ovf, (size_t)d->cap_items * 4);
@@ -1988,8 +2043,8 @@ static int enqueue_rs(fgss_server *srv, ServeSlot &sl, bool timing)
     uint32_t *misc = P<uint32_t>(d->b_misc), *scr = P<uint32_t>(d->b_scanscr);
     uint32_t *probelen = P<uint32_t>(d->b_probelen), *poff = P<uint32_t>(d->b_poff);
     uint32_t *hi = P<uint32_t>(d->b_hi), *eq = P<uint32_t>(d->b_eq), *nparts = P<uint32_t>(d->b_nparts),
-             *partoff = P<uint32_t>(d->b_partoff), *bcnt = P<uint32_t>(d->b_bcnt), *bofs = P<uint32_t>(d->b_bofs),
-             *wlo = P<uint32_t>(sl.b_wlo);
+             *partoff = P<uint32_t>(d->b_partoff), *wlo = P<uint32_t>(sl.b_wlo);
+    uint64_t *bcnt = P<uint64_t>(d->b_bcnt), *bofs = P<uint64_t>(d->b_bofs), *scr64 = P<uint64_t>(d->b_scanscr);
     unsigned long long *ctr = P<unsigned long long>(d->b_ctr), *jctr = P<unsigned long long>(d->b_jctr);
     if (timing) HIPCK(hipEventRecord(d->ev[0], st));
     HIPCK(fill<unsigned long long>(ctr, C_COUNT, 0ull, st));
@@ -2006,13 +2061,14 @@ static int enqueue_rs(fgss_server *srv, ServeSlot &sl, bool timing)
                        x->n_terms, cfg.target, P<uint2>(d->b_rng), wlo, hi, nparts, eq, bcnt);
     HIPCK(hipGetLastError());
     HIPCK(exclusive_scan<uint32_t>(nparts, partoff, v, misc + M_NITEMS, scr, st));
-    HIPCK(exclusive_scan<uint32_t>(bcnt, bofs, v, misc + M_NB, scr, st));
+    HIPCK(exclusive_scan<uint64_t>(bcnt, bofs, v, (uint64_t *)(misc + M_NB64), scr64, st));
     hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(grid_for(v * kWave, 256), 8192)), dim3(256), 0, st, v,
                        probelen, poff, P<uint2>(d->b_rng), wlo, hi, nparts, bofs, P<uint32_t>(x->b_psid),
-                       P<uint32_t>(x->b_fence), P<uint32_t>(d->b_bounds), d->cap_bounds);
+                       P<uint32_t>(x->b_fence), P<uint32_t>(d->b_bounds), std::min(d->cap_bounds, cfg.bounds));
     HIPCK(hipGetLastError());
-    hipLaunchKernelGGL(k_items, dim3(grid_for(v, 256)), dim3(256), 0, st, 1, wlo, hi, nparts, eq, partoff, bofs, d->job,
-                       d->tj, d->len, d->pos, poff, v, P<Item>(d->b_items), d->cap_items);
+    hipLaunchKernelGGL(k_items, dim3(grid_for(v, 256)), dim3(256), 0, st, 1, wlo, hi, nparts, eq, partoff, bofs, bcnt,
+                       std::min(d->cap_bounds, cfg.bounds), d->job, d->tj, d->len, d->pos, poff, v, P<Item>(d->b_items),
+                       d->cap_items);
     HIPCK(hipGetLastError());
     if (timing) HIPCK(hipEventRecord(d->ev[4], st));
     int rc = launch_join_cap(d, rs_args(srv, sl), cfg.cap, false, true);

@@ -136,3 +136,75 @@ def test_cli_outputs(fg, oracle_mod, tmp_path):
     want = sorted("%d %d %f" % (sets.id[q], sets.id[s], oracle_mod.similarity(int(sets.len[q]), int(sets.len[s]), int(o)))
                   for q, s, o in tri)
     assert sorted(r.stdout.splitlines()) == want
+
+
+def _expected_index_dump(oracle_mod, sets, t):
+    """Per term, the ids of the records whose (clamped) mid prefix holds it, in
+    internal order -- built from the oracle's bound expression."""
+    lists = [[] for _ in range(sets.n_terms)]
+    for s in range(sets.n):
+        mp = min(oracle_mod.bounds("mid_prefix", int(sets.len[s]), t), int(sets.len[s]))
+        for tok in sets.record(s)[:mp]:
+            lists[int(tok)].append(int(sets.id[s]))
+    out = []
+    for term, ids in enumerate(lists):
+        out.append(f"term: [{term}], size: {len(ids)}")
+        out.append("".join(f"({i}) " for i in ids))
+        out.append("")
+    return out
+
+
+def test_cli_debug_index_dump_and_host_tokenizer(fg, oracle_mod, tmp_path):
+    import subprocess
+    # the fixture through the debug form: sets on stdout, the index on stderr
+    t = oracle_mod.parse_threshold("0.5")
+    sets = oracle_mod.Sets(np.fromfile(golden_path("small.txt.3gram.bin"), "<i4"))
+    r = subprocess.run([fg.CLI_PATH, golden_path("small.txt.3gram.bin"), "0.5", "debug"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    want = _expected_index_dump(oracle_mod, sets, t)
+    lines = r.stderr.splitlines()
+    k = lines.index(want[0])
+    assert lines[k:k + len(want)] == want
+    assert "id: [4]" in r.stdout and "size: [24]" in r.stdout
+    # a bigger collection, and the text form with either tokenizer
+    text = dblp_text()[:300000]
+    src = tmp_path / "d.txt"
+    src.write_bytes(text)
+    binp = tmp_path / "d.bin"
+    fg.Collection.from_text_bytes(text, 3, device=True).write_bin(str(binp))
+    s2 = oracle_mod.Sets(np.fromfile(str(binp), "<i4"))
+    r = subprocess.run([fg.CLI_PATH, str(binp), "0.8", "debug"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    want = _expected_index_dump(oracle_mod, s2, oracle_mod.parse_threshold("0.8"))
+    lines = r.stderr.splitlines()
+    k = lines.index(want[0])
+    assert lines[k:k + len(want)] == want
+    outs = []
+    for extra in ([], ["--host-tokenizer"]):
+        r = subprocess.run([fg.CLI_PATH, "-f", str(src), "-q", "3", "-t", "0.8"] + extra, capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs.append(sorted(r.stdout.splitlines()))
+    assert outs[0] == outs[1] and len(outs[0]) > 0
+
+
+@pytest.mark.parametrize("bounds_cap", [1, 300, 20000])
+def test_split_queries_beyond_the_boundary_budget(fg, oracle_mod, bounds_cap):
+    """Split queries whose boundary rows exceed the budget search their posting ranges
+    per part instead; results must not change."""
+    _say(f"titles n=4096 seed=7 t=0.5 slots=256 bounds_cap={bounds_cap}")
+    coll = fg.Collection.generate(fg.GEN_TITLES, 4096, 7)
+    sets = oracle_sets(oracle_mod, coll)
+    assert_join_equal(oracle_mod, sets, coll.selfjoin(0.5, table_slots=256, bounds_cap=bounds_cap), 0.5)
+    _say(f"long n=1500 seed=3 t=0.5 bounds_cap={bounds_cap}")
+    coll = fg.Collection.generate(fg.GEN_LONG, 1500, 3)
+    sets = oracle_sets(oracle_mod, coll)
+    assert_join_equal(oracle_mod, sets, coll.selfjoin(0.5, bounds_cap=bounds_cap), 0.5)
+
+
+def test_titles_100k_small_boundary_budget(fg, oracle_mod):
+    _say("titles n=100000 seed=2 t=0.6 bounds_cap=5000")
+    coll = fg.Collection.generate(fg.GEN_TITLES, 100000, 2, device=True)
+    sets = oracle_sets(oracle_mod, coll)
+    assert_join_equal(oracle_mod, sets, coll.selfjoin(0.6, bounds_cap=5000), 0.6)
