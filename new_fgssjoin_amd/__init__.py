@@ -76,7 +76,7 @@ class JoinStats(C.Structure):
 
 
 class JoinOpts(C.Structure):
-    _fields_ = [("load_pct", C.c_int32), ("table_slots", C.c_int32), ("verify_only", C.c_int32),
+    _fields_ = [("load_pct", C.c_int32), ("table_slots", C.c_int32), ("stream_parts", C.c_int32),
                 ("bounds_cap", C.c_int32)]
 
 
@@ -310,9 +310,10 @@ class Collection:
     def upload(self):
         _check(lib().fgss_collection_upload(self._h))
 
-    def selfjoin(self, threshold: float, table_slots: int = 0, load_pct: int = 0, bounds_cap: int = 0) -> Pairs:
+    def selfjoin(self, threshold: float, table_slots: int = 0, load_pct: int = 0, bounds_cap: int = 0,
+                 stream_parts: int = 0) -> Pairs:
         L = lib()
-        opts = JoinOpts(load_pct, table_slots, 0, bounds_cap)
+        opts = JoinOpts(load_pct, table_slots, stream_parts, bounds_cap)
         out = C.POINTER(_Pairs)()
         _check(L.fgss_selfjoin(self._h, C.c_float(threshold), C.byref(opts), C.byref(out)))
         return _take_pairs(out)
@@ -386,10 +387,10 @@ class Server:
     results are taken (record ids come from them)."""
 
     def __init__(self, indexed: Collection, t_min: float, slots: int = 2, budget_bytes: int = 0,
-                 table_slots: int = 0, load_pct: int = 0, bounds_cap: int = 0):
+                 table_slots: int = 0, load_pct: int = 0, bounds_cap: int = 0, stream_parts: int = 0):
         self.indexed = indexed
         self._probes = {}
-        opts = JoinOpts(load_pct, table_slots, 0, bounds_cap)
+        opts = JoinOpts(load_pct, table_slots, stream_parts, bounds_cap)
         h = C.c_void_p()
         _check(lib().fgss_server_create(indexed._h, C.c_float(t_min), slots, budget_bytes, C.byref(opts),
                                          C.byref(h)))

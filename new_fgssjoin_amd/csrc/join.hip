@@ -90,7 +90,7 @@ enum Ctr {
 
 struct Item {
     uint32_t q, a, b;   // query and source sub-window [a, b)
-    uint32_t pad;       // expected useful postings (sizes the LDS table) | kWholeWindow | kPlanned
+    uint32_t pad;       // expected useful postings (sizes the LDS table) | kWholeWindow | kPlanned | kStreamed
     uint32_t part, nparts;  // planned items: part index and part count of the query
     uint32_t bofs;          // planned items: first boundary position of the query in the bounds table
     uint32_t job;           // the query's job
@@ -102,7 +102,21 @@ struct Item {
 constexpr uint32_t kMidProbe = 0x10000u;
 constexpr uint32_t kWholeWindow = 0x80000000u;   // the item covers the query's whole window
 constexpr uint32_t kPlanned = 0x40000000u;       // part of a planned split: ranges come from the bounds table
-constexpr uint32_t kPadMask = 0x3FFFFFFFu;
+constexpr uint32_t kStreamed = 0x20000000u;      // the whole window, walked part by part by one wavefront
+constexpr uint32_t kPadMask = 0x1FFFFFFFu;
+// streamed items keep one posting cursor per probe token in a per-wave global slot
+constexpr uint32_t kStreamQp = 256;
+constexpr uint32_t kStreamParts = 16;            // default: queries split into more parts are streamed
+
+// How the items of a query with np parts read their posting ranges.
+enum ItemMode : uint32_t { IM_NONE = 0, IM_WHOLE, IM_PLANNED, IM_STREAMED, IM_SEARCHED };
+__device__ __forceinline__ uint32_t item_mode(uint32_t np, uint32_t qp, uint64_t bcnt, uint64_t bofs, uint32_t cap_bounds)
+{
+    if (np == 0) return IM_NONE;
+    if (np == 1) return IM_WHOLE;
+    if (bcnt > 0 && bofs + bcnt <= cap_bounds) return IM_PLANNED;
+    return qp <= kStreamQp ? IM_STREAMED : IM_SEARCHED;
+}
 
 }  // namespace
 
@@ -124,15 +138,17 @@ struct fgss_device_sets {
     float *tj = nullptr;        // per job: threshold of the current run
     DevBuf b_psid, b_fence, b_bcnt, b_bofs, b_bounds, b_probelen, b_poff, b_rng, b_eq, b_midlen, b_midoff, b_ka,
         b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_post, b_loff, b_hi, b_nparts, b_partoff, b_scanscr, b_items,
-        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc;
+        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc, b_nitems, b_ovfp, b_cursor;
     uint32_t cap_items = 0, cap_bounds = 0, cap_out = 0;   // current capacities (elements)
+    uint32_t cursor_slots = 0;    // wavefront slots of b_cursor (streamed items)
+    uint32_t cap_ovf = 0;         // overflow records (a streamed item can record one per part)
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     uint64_t *h_ctr = nullptr;    // pinned mirror of the counters
     uint32_t *h_misc = nullptr;   // pinned mirror of the device totals
     uint64_t *h_jctr = nullptr;   // pinned mirror of the per-job counters
     hipGraph_t graph = nullptr;   // captured pipeline (batch runs)
     hipGraphExec_t graph_exec = nullptr;
-    int graph_key[4] = {0, 0, 0, 0};   // table slots, load, capacity generation, boundary budget it was captured with
+    int graph_key[5] = {0, 0, 0, 0, 0};   // table slots, load, capacity generation, boundary budget, stream parts
     int cap_gen = 0;              // bumped whenever a buffer the pipeline uses is reallocated
 };
 
@@ -173,6 +189,7 @@ void fgss_device_release(fgss_collection *c)
                       &d->b_rng, &d->b_eq, &d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib,
                       &d->b_sortscr, &d->b_sid_of, &d->b_post, &d->b_loff, &d->b_hi, &d->b_nparts, &d->b_partoff,
                       &d->b_scanscr, &d->b_items, &d->b_items2, &d->b_ovf, &d->b_out, &d->b_ctr, &d->b_jctr,
+                      &d->b_nitems, &d->b_ovfp, &d->b_cursor,
                       &d->b_misc};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -437,6 +454,29 @@ __device__ __forceinline__ uint32_t lower_bound_sid(const uint32_t *__restrict__
     return a;
 }
 
+// First index in [lo, hi) whose record id is >= v when it is expected close to lo:
+// doubling steps from lo, then a binary search inside the last step (a streamed
+// item moves each cursor by a few postings per part).
+__device__ __forceinline__ uint32_t gallop_sid(const uint32_t *__restrict__ psid, uint32_t lo, uint32_t hi, uint32_t v)
+{
+    if (lo >= hi || psid[lo] >= v) return lo;
+    uint32_t prev = lo, step = 1;   // psid[prev] < v
+    for (;;) {
+        const uint32_t nxt = prev + step;
+        if (nxt >= hi) break;
+        if (psid[nxt] >= v) { hi = nxt; break; }
+        prev = nxt;
+        step <<= 1;
+    }
+    uint32_t a = prev + 1;
+    while (a < hi) {
+        const uint32_t mid = (a + hi) >> 1;
+        if (psid[mid] < v) a = mid + 1;
+        else hi = mid;
+    }
+    return a;
+}
+
 // first index in [lo, hi) with a[index] >= v (a ascending)
 __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *__restrict__ a, uint32_t lo, uint32_t hi, uint32_t v)
 {
@@ -461,7 +501,7 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
                                               const uint32_t *__restrict__ fence, uint32_t target,
                                               uint2 *__restrict__ rng, uint32_t *__restrict__ hi_out,
                                               uint32_t *__restrict__ nparts, uint32_t *__restrict__ eq_out,
-                                              uint64_t *__restrict__ bcnt)
+                                              uint64_t *__restrict__ bcnt, uint32_t stream_parts)
 {
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < n; q += nwaves) {
@@ -494,7 +534,9 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
             eq_out[q] = e;
             const uint32_t np = e == 0 ? 0u : (e + target - 1) / target;
             nparts[q] = np;
-            bcnt[q] = np > 1 ? (uint64_t)(np - 1) * qp : 0ull;   // internal boundaries x probe tokens
+            // internal boundaries x probe tokens, for queries whose parts read a planned
+            // boundary table (heavier ones are streamed when their probe fits a cursor slot)
+            bcnt[q] = np > 1 && (np <= stream_parts || qp > kStreamQp) ? (uint64_t)(np - 1) * qp : 0ull;
         }
     }
 }
@@ -515,7 +557,8 @@ __global__ __launch_bounds__(256) void k_bounds(int64_t n, const uint32_t *__res
                                                 const uint32_t *__restrict__ poff, const uint2 *__restrict__ rng,
                                                 const uint32_t *__restrict__ wlo, const uint32_t *__restrict__ hi,
                                                 const uint32_t *__restrict__ nparts,
-                                                const uint64_t *__restrict__ bofs, const uint32_t *__restrict__ psid,
+                                                const uint64_t *__restrict__ bofs, const uint64_t *__restrict__ bcnt,
+                                                const uint32_t *__restrict__ psid,
                                                 const uint32_t *__restrict__ fence, uint32_t *__restrict__ bounds,
                                                 uint32_t cap_bounds)
 {
@@ -525,8 +568,8 @@ __global__ __launch_bounds__(256) void k_bounds(int64_t n, const uint32_t *__res
         if (np < 2) continue;
         const uint32_t qp = probelen[q];
         const uint64_t a0 = wlo ? (uint64_t)wlo[q] : (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
-        const uint64_t total = (uint64_t)(np - 1) * qp;
-        if (bofs[q] + total > cap_bounds) continue;   // not planned
+        const uint64_t total = bcnt[q];
+        if (item_mode(np, qp, total, bofs[q], cap_bounds) != IM_PLANNED) continue;
         uint32_t *B = bounds + bofs[q];
         const uint2 *R = rng + poff[q];
         for (uint32_t k = lane_id(); k < (uint32_t)total; k += kWave) {
@@ -537,10 +580,23 @@ __global__ __launch_bounds__(256) void k_bounds(int64_t n, const uint32_t *__res
     }
 }
 
+// Items per query (before the scan of their offsets): one streamed item, np planned
+// or searched parts, one whole-window item, or none.
+__global__ void k_nitems(const uint32_t *__restrict__ nparts, const uint32_t *__restrict__ probelen,
+                         const uint64_t *__restrict__ bofs, const uint64_t *__restrict__ bcnt, uint32_t cap_bounds,
+                         int64_t n, uint32_t *__restrict__ nitems)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t np = nparts[q];
+    const uint32_t m = item_mode(np, probelen[q], bcnt[q], bofs[q], cap_bounds);
+    nitems[q] = m == IM_NONE ? 0u : (m == IM_WHOLE || m == IM_STREAMED) ? 1u : np;
+}
+
 // wlo: window start per query (null: q + 1, the self-join window).  rs: the queries
 // are R-S half-probes, 2r (sources longer than probe r) and 2r + 1 (the others).
 __global__ void k_items(int rs, const uint32_t *__restrict__ wlo, const uint32_t *__restrict__ hi,
-                        const uint32_t *__restrict__ nparts,
+                        const uint32_t *__restrict__ nparts, const uint32_t *__restrict__ probelen,
                         const uint32_t *__restrict__ eq, const uint32_t *__restrict__ partoff,
                         const uint64_t *__restrict__ bofs, const uint64_t *__restrict__ bcnt,
                         uint32_t cap_bounds, const uint32_t *__restrict__ job,
@@ -550,26 +606,27 @@ __global__ void k_items(int rs, const uint32_t *__restrict__ wlo, const uint32_t
 {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
-    uint32_t np = nparts[q];
-    if (np == 0) return;
+    const uint32_t np = nparts[q];
+    const uint32_t mode = item_mode(np, probelen[q], bcnt[q], bofs[q], cap_bounds);
+    if (mode == IM_NONE) return;
+    const uint32_t nit = (mode == IM_WHOLE || mode == IM_STREAMED) ? 1u : np;
     uint64_t a0 = wlo ? (uint64_t)wlo[q] : (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
     uint32_t o = partoff[q];
-    if ((uint64_t)o + np > cap_items) return;   // too small: the host resizes and reruns
+    if ((uint64_t)o + nit > cap_items) return;   // too small: the host resizes and reruns
     const uint32_t per = (eq[q] + np - 1) / np;   // expected postings per part: sizes its LDS table
     const int64_t r = rs ? q >> 1 : q;           // the record whose tokens probe
     const uint32_t flag = rs && (q & 1) == 0 ? kMidProbe : 0u;
-    // split queries read their ranges from the boundary table when k_bounds wrote
-    // their rows, else each part searches its ranges
-    const uint32_t split = bofs[q] + bcnt[q] <= cap_bounds ? kPlanned : 0u;
-    for (uint32_t p = 0; p < np; p++) {
+    const uint32_t fl = mode == IM_WHOLE ? kWholeWindow : mode == IM_PLANNED ? kPlanned
+                      : mode == IM_STREAMED ? kStreamed : 0u;
+    for (uint32_t p = 0; p < nit; p++) {
         Item it;
         it.q = (uint32_t)r;
-        it.a = part_start(a0, W, p, np);
-        it.b = part_start(a0, W, p + 1, np);
-        it.pad = per | (np == 1 ? kWholeWindow : split);
+        it.a = mode == IM_STREAMED ? (uint32_t)a0 : part_start(a0, W, p, np);
+        it.b = mode == IM_STREAMED ? (uint32_t)(a0 + W) : part_start(a0, W, p + 1, np);
+        it.pad = per | fl;
         it.part = p;
         it.nparts = np;
-        it.bofs = split ? (uint32_t)bofs[q] : 0u;
+        it.bofs = mode == IM_PLANNED ? (uint32_t)bofs[q] : 0u;
         it.job = job[r];
         it.t = tj[it.job];
         it.lq = len[r] | flag;
@@ -579,13 +636,20 @@ __global__ void k_items(int rs, const uint32_t *__restrict__ wlo, const uint32_t
     }
 }
 
-// split every overflowed item into `ways` equal source sub-windows
-__global__ void k_resplit(const Item *__restrict__ in, const uint32_t *__restrict__ ovf, uint32_t novf, uint32_t ways,
-                          Item *__restrict__ out)
+// split every overflowed item -- or, for a streamed item, its overflowed part
+// (ovfp) -- into `ways` equal source sub-windows whose parts search their ranges
+__global__ void k_resplit(const Item *__restrict__ in, const uint32_t *__restrict__ ovf,
+                          const uint32_t *__restrict__ ovfp, uint32_t novf, uint32_t ways, Item *__restrict__ out)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= novf) return;
     Item it = in[ovf[i]];
+    if (ovfp[i] != kEmpty) {
+        const uint64_t W0 = (uint64_t)it.b - it.a;
+        const uint32_t a = part_start(it.a, W0, ovfp[i], it.nparts), b = part_start(it.a, W0, ovfp[i] + 1, it.nparts);
+        it.a = a;
+        it.b = b;
+    }
     uint64_t W = (uint64_t)it.b - it.a;
     for (uint32_t p = 0; p < ways; p++) {
         Item o = it;
@@ -626,6 +690,10 @@ struct JoinArgs {
     int32_t *out;               // (q, s, overlap) triples
     uint32_t out_cap;
     uint32_t *ovf;              // indices of overflowed items
+    uint32_t *ovfp;             // ... and, for streamed items, the overflowed part (else kEmpty)
+    uint32_t ovf_cap;           // records beyond it are counted only: the host grows the list and reruns
+    uint32_t *cursor;           // streamed items: kStreamQp posting cursors per wavefront slot
+    uint32_t cursor_slots;
     unsigned long long *ctr;
 };
 
@@ -690,6 +758,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
     extern __shared__ uint32_t smem[];
     const int w = (int)threadIdx.x / kWave;
     const int lane = lane_id();
+    const uint32_t wslot = blockIdx.x * (uint32_t)join_waves(CAP) + (uint32_t)w;   // < A.cursor_slots (launch_join)
     unsigned long long *wacc = (unsigned long long *)(smem + (size_t)w * join_wave_words(CAP));
     uint32_t *tkey = (uint32_t *)(wacc + kAccSlots);
     uint32_t *tcnt = tkey + CAP;
@@ -726,7 +795,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         if (batch_next >= n_items) break;
         const uint32_t it = batch_next++;
         const Item item = A.items[it];
-        const uint32_t q = item.q, wa = item.a, wb = item.b;
+        const uint32_t q = item.q;
         const uint32_t jq = item.job;
         const float t = item.t;
         if (jq != cur_job) {   // per-job counters: flush when the job changes
@@ -748,22 +817,41 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
 
         const bool whole = (item.pad & kWholeWindow) != 0;
         const bool planned = (item.pad & kPlanned) != 0;
+        const bool streamed = (item.pad & kStreamed) != 0;
         const uint2 *qrng = A.rng + item.qpoff;
         const uint32_t *lo_row = planned && item.part > 0 ? A.bounds + item.bofs + (size_t)(item.part - 1) * qp : nullptr;
         const uint32_t *hi_row = planned && item.part + 1 < item.nparts ? A.bounds + item.bofs + (size_t)item.part * qp
                                                                         : nullptr;
+        // a streamed item walks its nparts windows in order; lane (j mod 64) owns the
+        // cursor of probe token j, which starts at the token's range in the whole window
+        uint32_t *cur = A.cursor + (size_t)wslot * kStreamQp;
+        if (streamed)
+            for (uint32_t j = lane; j < qp; j += kWave) cur[j] = qrng[j].x;
+        const uint32_t nwin = streamed ? item.nparts : 1u;
         c1 = clock64(); WACC(W_CYC0 + 0, c1 - c0); c0 = c1;
+
+        for (uint32_t win = 0; win < nwin; win++) {
+        uint32_t wa = item.a, wb = item.b;
+        if (streamed) {
+            const uint64_t W = (uint64_t)item.b - item.a;
+            wa = part_start(item.a, W, win, nwin);
+            wb = part_start(item.a, W, win + 1, nwin);
+        }
 
         // ---- A: candidate generation (filtering.cu:94-139) ----
         bool overflow = false;
-        uint32_t escan = 0, nlist = 0;
+        uint32_t escan = 0, nlist = 0, jdone = 0;
         for (uint32_t jb = 0; jb < qp && !overflow; jb += kTokBatch) {
             const uint32_t nb = min((uint32_t)kTokBatch, qp - jb);
             const unsigned long long r0 = clock64();
             for (uint32_t jj = lane; jj < nb; jj += kWave) {
                 const uint2 r = qrng[jb + jj];   // the token's postings inside the whole window
                 uint32_t lo = r.x, hi = r.y;
-                if (planned) {                   // boundaries precomputed by k_bounds
+                if (streamed) {                  // from the cursor to the end of this window
+                    lo = cur[jb + jj];
+                    hi = gallop_sid(A.psid, lo, r.y, wb);
+                    cur[jb + jj] = hi;
+                } else if (planned) {            // boundaries precomputed by k_bounds
                     if (lo_row) lo = lo_row[jb + jj];
                     if (hi_row) hi = hi_row[jb + jj];
                 } else if (!whole) {             // a re-split item: narrow by search
@@ -773,6 +861,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                 rlo[jj] = lo;
                 rpre[jj] = hi - lo;
             }
+            jdone = jb + nb;
             wave_sync();
             // exclusive scan of the segment lengths, kTokBatch/64 entries per lane
             constexpr int PER = kTokBatch / kWave;
@@ -855,9 +944,15 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             wave_sync();
         }
         if (overflow) {
+            // a streamed item still moves the cursors of the tokens it did not reach
+            if (streamed)
+                for (uint32_t j = jdone + lane; j < qp; j += kWave) cur[j] = gallop_sid(A.psid, cur[j], qrng[j].y, wb);
             if (lane == 0) {
                 uint32_t k = (uint32_t)atomicAdd(&A.ctr[C_OVERFLOW], 1ull);
-                A.ovf[k] = it;
+                if (k < A.ovf_cap) {
+                    A.ovf[k] = it;
+                    A.ovfp[k] = streamed ? win : kEmpty;
+                }
             }
             for (uint32_t k = lane; k < nlist; k += kWave) {
                 const uint32_t h = clist[k] & 0xFFFFu;
@@ -1031,6 +1126,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         WACC(W_VER, nver);
         WACC(W_INTER, inter);
         WACC(W_VTOK, vtok);
+        }   // windows
     }
     if (lane == 0) {
         atomicAdd(&A.ctr[C_ESCAN], wacc[W_ESCAN]);
@@ -1066,7 +1162,11 @@ static int launch_join(fgss_device_sets *d, const JoinArgs &A, bool dry = false)
         cached[1] = d->device;
     }
     if (dry) return FGSS_OK;
-    hipLaunchKernelGGL((k_join<CAP, RS>), dim3(cached[0]), dim3(join_waves(CAP) * kWave), smem, d->st, A);
+    // every wavefront needs its own cursor slot; the grid is persistent, so fewer
+    // blocks only means fewer wavefronts sharing the queue
+    const int blocks = std::min<int>(cached[0], (int)(A.cursor_slots / (uint32_t)join_waves(CAP)));
+    if (blocks < 1) return fgss_fail(FGSS_ERR_DEVICE, "internal: no cursor slots for the join kernel");
+    hipLaunchKernelGGL((k_join<CAP, RS>), dim3(blocks), dim3(join_waves(CAP) * kWave), smem, d->st, A);
     HIPCK(hipGetLastError());
     return FGSS_OK;
 }
@@ -1109,6 +1209,7 @@ struct RunCfg {
     int cap = 512;          // LDS table slots per wave
     uint32_t target = 332;  // postings per work item
     uint32_t bounds = 1u << 28;   // boundary-table entries at most
+    uint32_t stream_parts = kStreamParts;   // queries split into more parts are streamed
 };
 
 static RunCfg make_cfg(const fgss_join_opts *opts)
@@ -1118,6 +1219,7 @@ static RunCfg make_cfg(const fgss_join_opts *opts)
     const int load = opts && opts->load_pct > 0 ? std::min(opts->load_pct, 100) : 65;
     c.target = std::max<uint32_t>(1u, (uint32_t)(c.cap * load / 100));
     if (opts && opts->bounds_cap > 0) c.bounds = std::min<uint32_t>((uint32_t)opts->bounds_cap, c.bounds);
+    if (opts && opts->stream_parts > 0) c.stream_parts = (uint32_t)opts->stream_parts;
     return c;
 }
 
@@ -1136,6 +1238,18 @@ static int grow(fgss_device_sets *d, DevBuf &b, size_t bytes)
         int rc_ = grow(d, (buf), (bytes));    \
         if (rc_) return rc_;                  \
     } while (0)
+
+// Cursor slots for streamed items: kStreamQp words for every wavefront the device
+// can hold (32 per CU), so that any persistent grid of the join kernel fits.
+static int cursor_buffer(fgss_device_sets *d)
+{
+    if (d->cursor_slots == 0) {
+        int cus = 0;
+        HIPCK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device));
+        d->cursor_slots = (uint32_t)std::max(cus, 1) * 32u;
+    }
+    return grow(d, d->b_cursor, (size_t)d->cursor_slots * kStreamQp * 4);
+}
 
 // Every buffer the pipeline touches, sized for upper bounds where one exists (index
 // entries and probe tokens <= tokens) and for the current capacities otherwise.
@@ -1167,9 +1281,16 @@ static int prepare_buffers(fgss_device_sets *d)
     GROW(d->b_partoff, (size_t)(n + 1) * 4);
     GROW(d->b_bcnt, (size_t)n * 8);
     GROW(d->b_bofs, (size_t)(n + 1) * 8);
+    GROW(d->b_nitems, (size_t)n * 4);
     GROW(d->b_bounds, (size_t)d->cap_bounds * 4);
     GROW(d->b_items, (size_t)d->cap_items * sizeof(Item));
-    GROW(d->b_ovf, (size_t)d->cap_items * 4);
+    d->cap_ovf = std::max(d->cap_ovf, d->cap_items);
+    GROW(d->b_ovf, (size_t)d->cap_ovf * 4);
+    GROW(d->b_ovfp, (size_t)d->cap_ovf * 4);
+    {
+        int rc_ = cursor_buffer(d);
+        if (rc_) return rc_;
+    }
     GROW(d->b_out, (size_t)d->cap_out * 12);
     GROW(d->b_ctr, C_COUNT * sizeof(uint64_t));
     GROW(d->b_jctr, (size_t)d->njobs * kJobCtr * sizeof(uint64_t));
@@ -1190,6 +1311,8 @@ static JoinArgs join_args(fgss_device_sets *d)
     A.n_items_dev = P<uint32_t>(d->b_misc) + M_NITEMS; A.n_items_cap = d->cap_items;
     A.job = d->job; A.tj = d->tj; A.ctr_job = P<unsigned long long>(d->b_jctr);
     A.out = P<int32_t>(d->b_out); A.out_cap = d->cap_out; A.ovf = P<uint32_t>(d->b_ovf);
+    A.ovfp = P<uint32_t>(d->b_ovfp); A.ovf_cap = d->cap_ovf;
+    A.cursor = P<uint32_t>(d->b_cursor); A.cursor_slots = d->cursor_slots;
     A.ctr = P<unsigned long long>(d->b_ctr);
     return A;
 }
@@ -1254,16 +1377,20 @@ static int enqueue_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool timing)
     // 6-8: plan work items
     hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 4096)), dim3(256), 0, st,
                        d->tok, d->pos, d->len, n, d->job, d->tj, d->jend, probelen, poff, loff, psid, fence, cfg.target,
-                       P<uint2>(d->b_rng), hi, nparts, eq, bcnt);
+                       P<uint2>(d->b_rng), hi, nparts, eq, bcnt, cfg.stream_parts);
     HIPCK(hipGetLastError());
-    HIPCK(exclusive_scan<uint32_t>(nparts, partoff, n, misc + M_NITEMS, scr, st));
+    const uint32_t blim = std::min(d->cap_bounds, cfg.bounds);
+    uint32_t *nitems = P<uint32_t>(d->b_nitems);
     HIPCK(exclusive_scan<uint64_t>(bcnt, bofs, n, (uint64_t *)(misc + M_NB64), scr64, st));
-    hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 8192)), dim3(256), 0, st, n,
-                       probelen, poff, P<uint2>(d->b_rng), (const uint32_t *)nullptr, hi, nparts, bofs, psid, fence,
-                       P<uint32_t>(d->b_bounds), std::min(d->cap_bounds, cfg.bounds));
+    hipLaunchKernelGGL(k_nitems, dim3(grid_for(n, 256)), dim3(256), 0, st, nparts, probelen, bofs, bcnt, blim, n, nitems);
     HIPCK(hipGetLastError());
-    hipLaunchKernelGGL(k_items, dim3(grid_for(n, 256)), dim3(256), 0, st, 0, (const uint32_t *)nullptr, hi, nparts, eq,
-                       partoff, bofs, bcnt, std::min(d->cap_bounds, cfg.bounds), d->job, d->tj,
+    HIPCK(exclusive_scan<uint32_t>(nitems, partoff, n, misc + M_NITEMS, scr, st));
+    hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 8192)), dim3(256), 0, st, n,
+                       probelen, poff, P<uint2>(d->b_rng), (const uint32_t *)nullptr, hi, nparts, bofs, bcnt, psid,
+                       fence, P<uint32_t>(d->b_bounds), blim);
+    HIPCK(hipGetLastError());
+    hipLaunchKernelGGL(k_items, dim3(grid_for(n, 256)), dim3(256), 0, st, 0, (const uint32_t *)nullptr, hi, nparts,
+                       probelen, eq, partoff, bofs, bcnt, blim, d->job, d->tj,
                        d->len, d->pos, poff, n, P<Item>(d->b_items), d->cap_items);
     HIPCK(hipGetLastError());
     // 9: the join
@@ -1314,6 +1441,7 @@ static int capture_graph(fgss_device_sets *d, const RunCfg &cfg)
     d->graph_key[1] = (int)cfg.target;
     d->graph_key[2] = d->cap_gen;
     d->graph_key[3] = (int)cfg.bounds;
+    d->graph_key[4] = (int)cfg.stream_parts;
     return FGSS_OK;
 }
 
@@ -1346,7 +1474,8 @@ static int launch_once(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &op
         rc = launch_join_cap(d, ops.args(), cfg.cap, true, ops.rs);   // launch configuration, outside the capture
         if (rc) return rc;
         if (!d->graph_exec || d->graph_key[0] != cfg.cap || d->graph_key[1] != (int)cfg.target ||
-            d->graph_key[2] != d->cap_gen || d->graph_key[3] != (int)cfg.bounds) {
+            d->graph_key[2] != d->cap_gen || d->graph_key[3] != (int)cfg.bounds ||
+            d->graph_key[4] != (int)cfg.stream_parts) {
             rc = capture_graph(d, cfg);
             if (rc) return rc;
         }
@@ -1376,6 +1505,10 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
         // a larger boundary table only saves searches: grow it within the budget
         const uint64_t want = std::min<uint64_t>(nB + nB / 4 + 1024, cfg.bounds);
         if (nB > d->cap_bounds && want > d->cap_bounds) { d->cap_bounds = (uint32_t)want; again = true; }
+        if (d->h_ctr[C_OVERFLOW] > d->cap_ovf) {
+            d->cap_ovf = (uint32_t)std::min<uint64_t>(d->h_ctr[C_OVERFLOW] * 5 / 4 + 1024, UINT32_MAX / 8);
+            again = true;
+        }
         if (nres > d->cap_out) {
             d->cap_out = (uint32_t)std::min<uint64_t>(nres + nres / 2 + 1024, UINT32_MAX / 16);
             again = true;
@@ -1411,12 +1544,15 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
         int rc = ensure(d->b_items2, (size_t)cnt * sizeof(Item));
         if (rc) return rc;
         hipLaunchKernelGGL(k_resplit, dim3(grid_for((int64_t)novf, 256)), dim3(256), 0, st, cur, P<uint32_t>(d->b_ovf),
+                           P<uint32_t>(d->b_ovfp),
                            (uint32_t)novf, ways, P<Item>(d->b_items2));
         HIPCK(hipGetLastError());
         std::swap(d->b_items, d->b_items2);   // the parent list stays alive in b_items2
         d->cap_gen++;
         cur = P<Item>(d->b_items);
         rc = ensure(d->b_ovf, (size_t)cnt * 4);
+        if (rc) return rc;
+        rc = ensure(d->b_ovfp, (size_t)cnt * 4);
         if (rc) return rc;
         for (;;) {
             unsigned long long *ctr = P<unsigned long long>(d->b_ctr);
@@ -1430,6 +1566,8 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
             A.out = P<int32_t>(d->b_out);
             A.out_cap = d->cap_out;
             A.ovf = P<uint32_t>(d->b_ovf);
+            A.ovfp = P<uint32_t>(d->b_ovfp);
+            A.ovf_cap = cnt;   // a re-split item overflows at most once
             HIPCK(hipEventRecord(d->ev[4], st));
             rc = launch_join_cap(d, A, cfg.cap, false, ops.rs);
             if (rc) return rc;
@@ -1894,7 +2032,7 @@ __global__ __launch_bounds__(256) void k_plan_rs(const uint32_t *__restrict__ pt
                                                  uint32_t n_terms, uint32_t target, uint2 *__restrict__ rng,
                                                  uint32_t *__restrict__ wlo_out, uint32_t *__restrict__ hi_out,
                                                  uint32_t *__restrict__ nparts, uint32_t *__restrict__ eq_out,
-                                                 uint64_t *__restrict__ bcnt)
+                                                 uint64_t *__restrict__ bcnt, uint32_t stream_parts)
 {
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; v < 2 * np; v += nwaves) {
@@ -1939,7 +2077,7 @@ __global__ __launch_bounds__(256) void k_plan_rs(const uint32_t *__restrict__ pt
             eq_out[v] = e;
             const uint32_t npt = e == 0 ? 0u : (e + target - 1) / target;
             nparts[v] = npt;
-            bcnt[v] = npt > 1 ? (uint64_t)(npt - 1) * qp : 0ull;
+            bcnt[v] = npt > 1 && (npt <= stream_parts || qp > kStreamQp) ? (uint64_t)(npt - 1) * qp : 0ull;
         }
     }
 }
@@ -2011,9 +2149,16 @@ static int slot_prepare(ServeSlot &sl)
     GROW(d->b_partoff, (size_t)(v + 1) * 4);
     GROW(d->b_bcnt, (size_t)v * 8);
     GROW(d->b_bofs, (size_t)(v + 1) * 8);
+    GROW(d->b_nitems, (size_t)v * 4);
+    d->cap_ovf = std::max(d->cap_ovf, d->cap_items);
+    GROW(d->b_ovfp, (size_t)d->cap_ovf * 4);
+    {
+        int rc_ = cursor_buffer(d);
+        if (rc_) return rc_;
+    }
     GROW(d->b_bounds, (size_t)d->cap_bounds * 4);
     GROW(d->b_items, (size_t)d->cap_items * sizeof(Item));
-    GROW(d->b_ovf, (size_t)d->cap_items * 4);
+    GROW(d->b_ovf, (size_t)d->cap_ovf * 4);
     GROW(d->b_out, (size_t)d->cap_out * 12);
     GROW(d->b_ctr, C_COUNT * sizeof(uint64_t));
     GROW(d->b_jctr, (size_t)std::max<uint32_t>(d->njobs, 1) * kJobCtr * sizeof(uint64_t));
@@ -2058,17 +2203,20 @@ static int enqueue_rs(fgss_server *srv, ServeSlot &sl, bool timing)
     hipLaunchKernelGGL(k_plan_rs, dim3((unsigned)std::min<int64_t>(grid_for(v * kWave, 256), 4096)), dim3(256), 0, st,
                        d->tok, d->pos, d->len, n, d->job, d->tj, probelen, poff, x->len, (uint32_t)x->n,
                        (uint32_t)x->max_len, P<uint32_t>(x->b_loff), P<uint32_t>(x->b_psid), P<uint32_t>(x->b_fence),
-                       x->n_terms, cfg.target, P<uint2>(d->b_rng), wlo, hi, nparts, eq, bcnt);
+                       x->n_terms, cfg.target, P<uint2>(d->b_rng), wlo, hi, nparts, eq, bcnt, cfg.stream_parts);
     HIPCK(hipGetLastError());
-    HIPCK(exclusive_scan<uint32_t>(nparts, partoff, v, misc + M_NITEMS, scr, st));
+    const uint32_t blim = std::min(d->cap_bounds, cfg.bounds);
+    uint32_t *nitems = P<uint32_t>(d->b_nitems);
     HIPCK(exclusive_scan<uint64_t>(bcnt, bofs, v, (uint64_t *)(misc + M_NB64), scr64, st));
-    hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(grid_for(v * kWave, 256), 8192)), dim3(256), 0, st, v,
-                       probelen, poff, P<uint2>(d->b_rng), wlo, hi, nparts, bofs, P<uint32_t>(x->b_psid),
-                       P<uint32_t>(x->b_fence), P<uint32_t>(d->b_bounds), std::min(d->cap_bounds, cfg.bounds));
+    hipLaunchKernelGGL(k_nitems, dim3(grid_for(v, 256)), dim3(256), 0, st, nparts, probelen, bofs, bcnt, blim, v, nitems);
     HIPCK(hipGetLastError());
-    hipLaunchKernelGGL(k_items, dim3(grid_for(v, 256)), dim3(256), 0, st, 1, wlo, hi, nparts, eq, partoff, bofs, bcnt,
-                       std::min(d->cap_bounds, cfg.bounds), d->job, d->tj, d->len, d->pos, poff, v, P<Item>(d->b_items),
-                       d->cap_items);
+    HIPCK(exclusive_scan<uint32_t>(nitems, partoff, v, misc + M_NITEMS, scr, st));
+    hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(grid_for(v * kWave, 256), 8192)), dim3(256), 0, st, v,
+                       probelen, poff, P<uint2>(d->b_rng), wlo, hi, nparts, bofs, bcnt, P<uint32_t>(x->b_psid),
+                       P<uint32_t>(x->b_fence), P<uint32_t>(d->b_bounds), blim);
+    HIPCK(hipGetLastError());
+    hipLaunchKernelGGL(k_items, dim3(grid_for(v, 256)), dim3(256), 0, st, 1, wlo, hi, nparts, probelen, eq, partoff,
+                       bofs, bcnt, blim, d->job, d->tj, d->len, d->pos, poff, v, P<Item>(d->b_items), d->cap_items);
     HIPCK(hipGetLastError());
     if (timing) HIPCK(hipEventRecord(d->ev[4], st));
     int rc = launch_join_cap(d, rs_args(srv, sl), cfg.cap, false, true);

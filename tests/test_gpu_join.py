@@ -208,3 +208,24 @@ def test_titles_100k_small_boundary_budget(fg, oracle_mod):
     coll = fg.Collection.generate(fg.GEN_TITLES, 100000, 2, device=True)
     sets = oracle_sets(oracle_mod, coll)
     assert_join_equal(oracle_mod, sets, coll.selfjoin(0.6, bounds_cap=5000), 0.6)
+
+
+@pytest.mark.parametrize("slots,load", [(512, 65), (256, 100)])
+def test_streamed_queries(fg, oracle_mod, slots, load):
+    """stream_parts=1 walks every split query part by part in one wavefront (posting
+    cursors, galloping search); with 256-slot tables at full load some parts overflow
+    and are re-split on their own.  Results must not change."""
+    for gen, n, seed, ts in ((fg.GEN_TITLES, 4096, 7, (0.5, 0.7)), (fg.GEN_LONG, 1500, 3, (0.5, 0.8))):
+        coll = fg.Collection.generate(gen, n, seed)
+        sets = oracle_sets(oracle_mod, coll)
+        for t in ts:
+            _say(f"streamed gen={gen} n={n} t={t} slots={slots} load={load}")
+            p = coll.selfjoin(t, table_slots=slots, load_pct=load, stream_parts=1)
+            assert_join_equal(oracle_mod, sets, p, t)
+
+
+def test_titles_100k_streamed(fg, oracle_mod):
+    _say("titles n=100000 seed=2 t=0.6 stream_parts=2")
+    coll = fg.Collection.generate(fg.GEN_TITLES, 100000, 2, device=True)
+    sets = oracle_sets(oracle_mod, coll)
+    assert_join_equal(oracle_mod, sets, coll.selfjoin(0.6, stream_parts=2), 0.6)

@@ -125,11 +125,12 @@ def test_batched_equals_alone(fg, world, slots, budget):
     srv.close()
 
 
-@pytest.mark.parametrize("bounds_cap", [0, 1])
-def test_overflow_resplit_and_small_tables(fg, oracle_mod, world, bounds_cap):
+@pytest.mark.parametrize("bounds_cap,stream_parts", [(0, 0), (1, 0), (0, 1)])
+def test_overflow_resplit_and_small_tables(fg, oracle_mod, world, bounds_cap, stream_parts):
     # 256-slot tables at full load force the re-split rounds of overflowing items; a
-    # one-entry boundary budget makes every split part search its ranges
-    srv = fg.Server(world["idx"], T_MIN, slots=2, table_slots=256, load_pct=100, bounds_cap=bounds_cap)
+    # one-entry boundary budget or stream_parts=1 makes split queries stream
+    srv = fg.Server(world["idx"], T_MIN, slots=2, table_slots=256, load_pct=100, bounds_cap=bounds_cap,
+                    stream_parts=stream_parts)
     probes, _ = world["reqs"][0]
     ids = [(srv.submit(p, t), p, t) for p, t in world["reqs"][:8]]
     for rid, p, t in ids:
