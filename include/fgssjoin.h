@@ -132,7 +132,7 @@ typedef struct {
     int32_t load_pct;          /* postings per work item as % of its table slots, 0 = default (65) */
     int32_t table_slots;       /* LDS candidate table slots per work item: 256..4096, 0 = default (512) */
     int32_t stream_parts;      /* a query split into more parts than this is walked by one wavefront,
-                                  part after part, 0 = default (16)                  */
+                                  part after part, 0 = default (64)                  */
     int32_t bounds_cap;        /* precomputed split boundaries at most, 0 = default (2^28); the
                                   parts of queries beyond it locate their posting ranges by search */
 } fgss_join_opts;

@@ -104,9 +104,10 @@ constexpr uint32_t kWholeWindow = 0x80000000u;   // the item covers the query's 
 constexpr uint32_t kPlanned = 0x40000000u;       // part of a planned split: ranges come from the bounds table
 constexpr uint32_t kStreamed = 0x20000000u;      // the whole window, walked part by part by one wavefront
 constexpr uint32_t kPadMask = 0x1FFFFFFFu;
-// streamed items keep one posting cursor per probe token in a per-wave global slot
+// streamed items keep one posting cursor per probe token in the wavefront's LDS
+// (the range arrays of the other items)
 constexpr uint32_t kStreamQp = 256;
-constexpr uint32_t kStreamParts = 16;            // default: queries split into more parts are streamed
+constexpr uint32_t kStreamParts = 64;            // default: queries split into more parts are streamed
 
 // How the items of a query with np parts read their posting ranges.
 enum ItemMode : uint32_t { IM_NONE = 0, IM_WHOLE, IM_PLANNED, IM_STREAMED, IM_SEARCHED };
@@ -138,9 +139,8 @@ struct fgss_device_sets {
     float *tj = nullptr;        // per job: threshold of the current run
     DevBuf b_psid, b_fence, b_bcnt, b_bofs, b_bounds, b_probelen, b_poff, b_rng, b_eq, b_midlen, b_midoff, b_ka,
         b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_post, b_loff, b_hi, b_nparts, b_partoff, b_scanscr, b_items,
-        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc, b_nitems, b_ovfp, b_cursor;
+        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc, b_nitems, b_ovfp;
     uint32_t cap_items = 0, cap_bounds = 0, cap_out = 0;   // current capacities (elements)
-    uint32_t cursor_slots = 0;    // wavefront slots of b_cursor (streamed items)
     uint32_t cap_ovf = 0;         // overflow records (a streamed item can record one per part)
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     uint64_t *h_ctr = nullptr;    // pinned mirror of the counters
@@ -189,7 +189,7 @@ void fgss_device_release(fgss_collection *c)
                       &d->b_rng, &d->b_eq, &d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib,
                       &d->b_sortscr, &d->b_sid_of, &d->b_post, &d->b_loff, &d->b_hi, &d->b_nparts, &d->b_partoff,
                       &d->b_scanscr, &d->b_items, &d->b_items2, &d->b_ovf, &d->b_out, &d->b_ctr, &d->b_jctr,
-                      &d->b_nitems, &d->b_ovfp, &d->b_cursor,
+                      &d->b_nitems, &d->b_ovfp,
                       &d->b_misc};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -456,22 +456,23 @@ __device__ __forceinline__ uint32_t lower_bound_sid(const uint32_t *__restrict__
 
 // First index in [lo, hi) whose record id is >= v when it is expected close to lo:
 // doubling steps from lo, then a binary search inside the last step (a streamed
-// item moves each cursor by a few postings per part).
-__device__ __forceinline__ uint32_t gallop_sid(const uint32_t *__restrict__ psid, uint32_t lo, uint32_t hi, uint32_t v)
+// item moves each cursor by a few postings per part).  It reads the postings
+// themselves, whose lines the previous part has just brought in.
+__device__ __forceinline__ uint32_t gallop_post(const uint2 *__restrict__ post, uint32_t lo, uint32_t hi, uint32_t v)
 {
-    if (lo >= hi || psid[lo] >= v) return lo;
-    uint32_t prev = lo, step = 1;   // psid[prev] < v
+    if (lo >= hi || post[lo].x >= v) return lo;
+    uint32_t prev = lo, step = 1;   // post[prev].x < v
     for (;;) {
         const uint32_t nxt = prev + step;
         if (nxt >= hi) break;
-        if (psid[nxt] >= v) { hi = nxt; break; }
+        if (post[nxt].x >= v) { hi = nxt; break; }
         prev = nxt;
         step <<= 1;
     }
     uint32_t a = prev + 1;
     while (a < hi) {
         const uint32_t mid = (a + hi) >> 1;
-        if (psid[mid] < v) a = mid + 1;
+        if (post[mid].x < v) a = mid + 1;
         else hi = mid;
     }
     return a;
@@ -692,8 +693,6 @@ struct JoinArgs {
     uint32_t *ovf;              // indices of overflowed items
     uint32_t *ovfp;             // ... and, for streamed items, the overflowed part (else kEmpty)
     uint32_t ovf_cap;           // records beyond it are counted only: the host grows the list and reruns
-    uint32_t *cursor;           // streamed items: kStreamQp posting cursors per wavefront slot
-    uint32_t cursor_slots;
     unsigned long long *ctr;
 };
 
@@ -758,7 +757,6 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
     extern __shared__ uint32_t smem[];
     const int w = (int)threadIdx.x / kWave;
     const int lane = lane_id();
-    const uint32_t wslot = blockIdx.x * (uint32_t)join_waves(CAP) + (uint32_t)w;   // < A.cursor_slots (launch_join)
     unsigned long long *wacc = (unsigned long long *)(smem + (size_t)w * join_wave_words(CAP));
     uint32_t *tkey = (uint32_t *)(wacc + kAccSlots);
     uint32_t *tcnt = tkey + CAP;
@@ -822,11 +820,14 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         const uint32_t *lo_row = planned && item.part > 0 ? A.bounds + item.bofs + (size_t)(item.part - 1) * qp : nullptr;
         const uint32_t *hi_row = planned && item.part + 1 < item.nparts ? A.bounds + item.bofs + (size_t)item.part * qp
                                                                         : nullptr;
-        // a streamed item walks its nparts windows in order; lane (j mod 64) owns the
-        // cursor of probe token j, which starts at the token's range in the whole window
-        uint32_t *cur = A.cursor + (size_t)wslot * kStreamQp;
-        if (streamed)
+        // a streamed item walks its nparts windows in order with one cursor per probe
+        // token (LDS), starting at the token's range in the whole window
+        uint32_t *cur = rlo;
+        static_assert(2 * kTokBatch + 1 >= (int)kStreamQp, "cursors live in rlo and rpre");
+        if (streamed) {
             for (uint32_t j = lane; j < qp; j += kWave) cur[j] = qrng[j].x;
+            wave_sync();
+        }
         const uint32_t nwin = streamed ? item.nparts : 1u;
         c1 = clock64(); WACC(W_CYC0 + 0, c1 - c0); c0 = c1;
 
@@ -841,17 +842,13 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         // ---- A: candidate generation (filtering.cu:94-139) ----
         bool overflow = false;
         uint32_t escan = 0, nlist = 0, jdone = 0;
-        for (uint32_t jb = 0; jb < qp && !overflow; jb += kTokBatch) {
+        for (uint32_t jb = 0; !streamed && jb < qp && !overflow; jb += kTokBatch) {
             const uint32_t nb = min((uint32_t)kTokBatch, qp - jb);
             const unsigned long long r0 = clock64();
             for (uint32_t jj = lane; jj < nb; jj += kWave) {
                 const uint2 r = qrng[jb + jj];   // the token's postings inside the whole window
                 uint32_t lo = r.x, hi = r.y;
-                if (streamed) {                  // from the cursor to the end of this window
-                    lo = cur[jb + jj];
-                    hi = gallop_sid(A.psid, lo, r.y, wb);
-                    cur[jb + jj] = hi;
-                } else if (planned) {            // boundaries precomputed by k_bounds
+                if (planned) {                   // boundaries precomputed by k_bounds
                     if (lo_row) lo = lo_row[jb + jj];
                     if (hi_row) hi = hi_row[jb + jj];
                 } else if (!whole) {             // a re-split item: narrow by search
@@ -943,10 +940,109 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             }
             wave_sync();
         }
+        if (streamed) {
+            // A streamed part [wa, wb): every probe token with postings left takes the
+            // next K of them from its cursor (independent loads, no search).  Those
+            // before wb are this part's; the group of K lanes of a token moves its
+            // cursor past them, and a token whose K were all before wb takes K more
+            // (K doubling up to 64) in the next round.
+            uint8_t *toks = (uint8_t *)qstage;          // active token ids (bytes)
+            uint32_t *satm = qstage + kStreamQp / 4;    // kStreamQp bits: tokens with postings left
+            static_assert(kStreamQp / 4 + kStreamQp / 32 <= kQStage, "token list fits the query stage");
+            uint32_t nact = 0;
+            for (uint32_t j0 = 0; j0 < qp; j0 += kWave) {
+                const uint32_t j = j0 + lane;
+                const bool act = j < qp && cur[j] < qrng[j].y;
+                const uint64_t m = __ballot(act);
+                if (act) toks[nact + (uint32_t)__popcll(m & lanemask_lt())] = (uint8_t)j;
+                nact += (uint32_t)__popcll(m);
+            }
+            wave_sync();
+            uint32_t logK = 2;
+            while (nact > 0 && !overflow) {
+                if (lane < (int)(kStreamQp / 32)) satm[lane] = 0;
+                wave_sync();
+                const uint32_t K = 1u << logK, total = nact << logK;
+                for (uint32_t e0 = 0; e0 < total && !overflow; e0 += kWave * kUnroll) {
+                    uint2 pe[kUnroll];
+                    uint32_t jv[kUnroll];
+                    bool below[kUnroll];
+#pragma unroll
+                    for (int k = 0; k < kUnroll; k++) {
+                        const uint32_t e = e0 + (uint32_t)(k * kWave) + lane;
+                        below[k] = false;
+                        jv[k] = kEmpty;
+                        if (e < total) {
+                            const uint32_t j = toks[e >> logK];
+                            const uint32_t idx = cur[j] + (e & (K - 1));
+                            jv[k] = j;
+                            if (idx < qrng[j].y) {
+                                pe[k] = A.post[idx];
+                                below[k] = pe[k].x < wb;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < kUnroll; k++) {
+                        // below is a prefix of each token's K lanes (postings ascend by
+                        // record): its length is the count, taken by the group's first lane
+                        const uint64_t bm = __ballot(below[k]);
+                        escan += (uint32_t)__popcll(bm);
+                        const uint32_t e = e0 + (uint32_t)(k * kWave) + lane;
+                        const uint32_t gs = (uint32_t)lane & ~(K - 1);
+                        const uint64_t gmask = K >= 64 ? ~0ull : ((1ull << K) - 1ull);
+                        const uint32_t cnt = (uint32_t)__popcll((bm >> gs) & gmask);
+                        if (e < total && ((uint32_t)lane & (K - 1)) == 0 && cnt > 0) {
+                            cur[jv[k]] += cnt;
+                            if (cnt == K) atomicOr(&satm[jv[k] >> 5], 1u << (jv[k] & 31));
+                        }
+                        bool valid = below[k];
+                        if (RS && valid) {   // the source's own prefix at t
+                            const uint32_t ls = pe[k].y >> 16;
+                            valid = (pe[k].y & 0xFFFFu) < (midprobe ? fgss_max_prefix(ls, t) : fgss_mid_prefix(ls, t));
+                        }
+                        bool ok = true, isnew = false;
+                        uint32_t h = 0;
+                        if (valid) {
+                            const uint32_t s = pe[k].x;
+                            h = slot_hash(s, tlog);
+                            uint32_t probes = 0;
+                            for (;;) {
+                                const uint32_t prev = atomicCAS(&tkey[h], kEmpty, s);
+                                if (prev == kEmpty) { isnew = true; break; }
+                                if (prev == s) break;
+                                h = (h + 1) & (tsize - 1);
+                                if (++probes >= tsize) { ok = false; break; }
+                            }
+                            if (ok) {
+                                atomicAdd(&tcnt[h], 1u);
+                                atomicMax(&tlast[h], (jv[k] << 16) | (pe[k].y & 0xFFFFu));
+                            }
+                        }
+                        const uint64_t nm = __ballot(isnew);
+                        if (isnew) clist[nlist + (uint32_t)__popcll(nm & lanemask_lt())] = h | (pe[k].y & 0xFFFF0000u);
+                        nlist += (uint32_t)__popcll(nm);
+                        if (__ballot(!ok)) overflow = true;
+                    }
+                }
+                wave_sync();
+                nact = 0;
+                for (uint32_t j0 = 0; j0 < qp; j0 += kWave) {
+                    const uint32_t j = j0 + lane;
+                    const bool act = j < qp && ((satm[j >> 5] >> (j & 31)) & 1u);
+                    const uint64_t m = __ballot(act);
+                    wave_sync();
+                    if (act) toks[nact + (uint32_t)__popcll(m & lanemask_lt())] = (uint8_t)j;
+                    nact += (uint32_t)__popcll(m);
+                }
+                wave_sync();
+                if (logK < 6) logK++;
+            }
+        }
         if (overflow) {
             // a streamed item still moves the cursors of the tokens it did not reach
             if (streamed)
-                for (uint32_t j = jdone + lane; j < qp; j += kWave) cur[j] = gallop_sid(A.psid, cur[j], qrng[j].y, wb);
+                for (uint32_t j = jdone + lane; j < qp; j += kWave) cur[j] = gallop_post(A.post, cur[j], qrng[j].y, wb);
             if (lane == 0) {
                 uint32_t k = (uint32_t)atomicAdd(&A.ctr[C_OVERFLOW], 1ull);
                 if (k < A.ovf_cap) {
@@ -1162,11 +1258,7 @@ static int launch_join(fgss_device_sets *d, const JoinArgs &A, bool dry = false)
         cached[1] = d->device;
     }
     if (dry) return FGSS_OK;
-    // every wavefront needs its own cursor slot; the grid is persistent, so fewer
-    // blocks only means fewer wavefronts sharing the queue
-    const int blocks = std::min<int>(cached[0], (int)(A.cursor_slots / (uint32_t)join_waves(CAP)));
-    if (blocks < 1) return fgss_fail(FGSS_ERR_DEVICE, "internal: no cursor slots for the join kernel");
-    hipLaunchKernelGGL((k_join<CAP, RS>), dim3(blocks), dim3(join_waves(CAP) * kWave), smem, d->st, A);
+    hipLaunchKernelGGL((k_join<CAP, RS>), dim3(cached[0]), dim3(join_waves(CAP) * kWave), smem, d->st, A);
     HIPCK(hipGetLastError());
     return FGSS_OK;
 }
@@ -1239,18 +1331,6 @@ static int grow(fgss_device_sets *d, DevBuf &b, size_t bytes)
         if (rc_) return rc_;                  \
     } while (0)
 
-// Cursor slots for streamed items: kStreamQp words for every wavefront the device
-// can hold (32 per CU), so that any persistent grid of the join kernel fits.
-static int cursor_buffer(fgss_device_sets *d)
-{
-    if (d->cursor_slots == 0) {
-        int cus = 0;
-        HIPCK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device));
-        d->cursor_slots = (uint32_t)std::max(cus, 1) * 32u;
-    }
-    return grow(d, d->b_cursor, (size_t)d->cursor_slots * kStreamQp * 4);
-}
-
 // Every buffer the pipeline touches, sized for upper bounds where one exists (index
 // entries and probe tokens <= tokens) and for the current capacities otherwise.
 static int prepare_buffers(fgss_device_sets *d)
@@ -1287,10 +1367,6 @@ static int prepare_buffers(fgss_device_sets *d)
     d->cap_ovf = std::max(d->cap_ovf, d->cap_items);
     GROW(d->b_ovf, (size_t)d->cap_ovf * 4);
     GROW(d->b_ovfp, (size_t)d->cap_ovf * 4);
-    {
-        int rc_ = cursor_buffer(d);
-        if (rc_) return rc_;
-    }
     GROW(d->b_out, (size_t)d->cap_out * 12);
     GROW(d->b_ctr, C_COUNT * sizeof(uint64_t));
     GROW(d->b_jctr, (size_t)d->njobs * kJobCtr * sizeof(uint64_t));
@@ -1312,7 +1388,6 @@ static JoinArgs join_args(fgss_device_sets *d)
     A.job = d->job; A.tj = d->tj; A.ctr_job = P<unsigned long long>(d->b_jctr);
     A.out = P<int32_t>(d->b_out); A.out_cap = d->cap_out; A.ovf = P<uint32_t>(d->b_ovf);
     A.ovfp = P<uint32_t>(d->b_ovfp); A.ovf_cap = d->cap_ovf;
-    A.cursor = P<uint32_t>(d->b_cursor); A.cursor_slots = d->cursor_slots;
     A.ctr = P<unsigned long long>(d->b_ctr);
     return A;
 }
@@ -2152,10 +2227,6 @@ static int slot_prepare(ServeSlot &sl)
     GROW(d->b_nitems, (size_t)v * 4);
     d->cap_ovf = std::max(d->cap_ovf, d->cap_items);
     GROW(d->b_ovfp, (size_t)d->cap_ovf * 4);
-    {
-        int rc_ = cursor_buffer(d);
-        if (rc_) return rc_;
-    }
     GROW(d->b_bounds, (size_t)d->cap_bounds * 4);
     GROW(d->b_items, (size_t)d->cap_items * sizeof(Item));
     GROW(d->b_ovf, (size_t)d->cap_ovf * 4);
