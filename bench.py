@@ -139,6 +139,86 @@ def batched_leg(fg, n_jobs: int, reps: int):
             "speedup": plain_s / batch_s, "autotune": tune, "bit_exact_vs_plain": bool(exact)}
 
 
+def large_cpu_baseline(coll, pairs, t, sample: int, seed: int = 11):
+    """The oracle on a uniform sample of queries of the large join (each against the
+    whole collection): the CPU rate, the GPU's pairs for those queries bit-exact, and
+    the sample's merge-read count for the algorithmic bytes."""
+    import numpy as np
+
+    import oracle
+    oracle.build()
+    ids, lens, pos, tok = coll.arrays()
+    sets = oracle.Sets.from_arrays(ids, lens, pos, tok, coll.info.n_terms)
+    qs = np.sort(np.random.default_rng(seed).choice(sets.n, size=min(sample, sets.n), replace=False))
+    threads = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    tri, ctr = oracle.selfjoin(sets, t, nthreads=threads, queries=qs)
+    dt = time.perf_counter() - t0
+    got = np.stack([pairs.q, pairs.s, pairs.overlap], axis=1) if len(pairs) else np.zeros((0, 3), np.int32)
+    got = got[np.isin(got[:, 0], qs)] if len(got) else got
+    exact = got.shape == tri.shape and bool(np.array_equal(got, tri))
+    return ({"value": ctr["candidates"] / dt, "unit": "candidates/s", "cores": threads, "kind": "port",
+             "sample": f"{qs.size} random queries (seed {seed}) of the {sets.n}-record join against the whole "
+                       f"collection, {dt:.1f} s, oracle/fgss_oracle.c closed form, {threads} threads",
+             "sample_s": dt, "sample_queries": int(qs.size)}, exact, ctr)
+
+
+def large_leg(fg, n: int, sample: int, steps: int = 2, t: float = 0.8):
+    """Plain large self-join: n long records (200-1200 chars of Zipf words, 5%
+    near-duplicates with 1-10 edits, seed 3), q=3, t=0.8, tokenized on the GPU."""
+    import numpy as np
+    t = float(np.float32(t))
+    t0 = time.perf_counter()
+    text = fg.generate_text(fg.GEN_LONG, n, 3)
+    t1 = time.perf_counter()
+    coll = fg.Collection.from_text_bytes(text, 3, device=True)
+    tok_s = time.perf_counter() - t1
+    del text
+    info = coll.info
+    log(f"large: {info.n_sets} long records, {info.n_tokens} tokens (GPU tokenizer {tok_s:.2f} s, "
+        f"set-up {time.perf_counter() - t0:.1f} s)")
+    coll.upload()
+    coll.selfjoin(t)   # sizes the workspace
+    runs = []
+    for _ in range(steps):
+        a = time.perf_counter()
+        p = coll.selfjoin(t)
+        runs.append((time.perf_counter() - a, p))
+    runs.sort(key=lambda r: r[0])
+    wall_s, p = runs[len(runs) // 2]
+    st = p.stats
+    log(f"large: join {st['ms_total']:.0f} ms (kernel {st['ms_kernel']:.0f} ms), {len(p)} pairs, "
+        f"{st['candidates']} candidates")
+    out = {"workload": f"plain self-join, {info.n_sets} synthetic long records (200-1200 chars), q=3, t=0.8",
+           "records": int(info.n_sets), "tokens": int(info.n_tokens), "terms": int(info.n_terms),
+           "tokenize_s": tok_s, "join_ms": st["ms_total"], "kernel_ms": st["ms_kernel"], "wall_ms": wall_s * 1e3,
+           "pairs": len(p), "candidates": st["candidates"], "candidates_per_s": st["candidates"] / (st["ms_total"] / 1e3),
+           "postings": st["postings_scanned"], "probe_tokens": st["probe_tokens"],
+           "index_entries": st["index_entries"], "signature_checks": st["signature_checks"],
+           "verified": st["verified"], "kernel_launches": st["kernel_launches"]}
+    if sample > 0:
+        log(f"large: cpu baseline on {sample} sampled queries ...")
+        cb, exact, c = large_cpu_baseline(coll, p, t, sample)
+        out["cpu_baseline"] = cb
+        out["bit_exact_vs_oracle_on_sample"] = exact
+        # merge reads of the reference scale with the candidates: estimate the whole
+        # join's from the sample, the other terms are the GPU's exact counts
+        m_est = c["m_reads"] * st["candidates"] / max(c["candidates"], 1)
+        kbytes = 4 * st["probe_tokens"] + 8 * st["postings_scanned"] + 4 * m_est + 8 * st["results"]
+        achieved = kbytes / (st["ms_kernel"] / 1e3) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": achieved / HBM_PEAK_GBS, "kernel": "k_join", "bytes_per_launch": kbytes,
+                           "basis": "algorithmic bytes of the reference algorithm; merge reads estimated from the "
+                                    "oracle's sample (scaled by candidates), the rest counted exactly"}
+        cpu_full_s = cb["sample_s"] * info.n_sets / max(cb["sample_queries"], 1)
+        out["cpu_full_estimate_s"] = cpu_full_s
+        out["speedup_vs_cpu_estimate"] = cpu_full_s / (st["ms_total"] / 1e3)
+        log(f"large: sample bit-exact {exact}; CPU estimate {cpu_full_s:.0f} s for the whole join "
+            f"({out['speedup_vs_cpu_estimate']:.0f}x); roofline {achieved:.0f} GB/s")
+    coll.close()
+    return out
+
+
 def _edit(line: bytes, rng, n_edits: int) -> bytes:
     s = bytearray(line)
     for _ in range(n_edits):
@@ -290,6 +370,8 @@ def main():
     ap.add_argument("--serve-index", type=int, default=2_000_000, help="indexed titles of the served leg")
     ap.add_argument("--serve-requests", type=int, default=1000, help="requests of the served leg (0 skips it)")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--large-records", type=int, default=1_000_000, help="records of the large leg (0 skips it)")
+    ap.add_argument("--large-sample", type=int, default=1000, help="oracle-checked queries of the large leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the join does not shard across GPUs "
@@ -385,6 +467,8 @@ def main():
         out["batched"] = batched_leg(fg, args.batch_jobs, reps=3)
     if args.serve_requests > 0:
         out["served"] = served_leg(fg, args.serve_index, args.serve_requests, args.serve_slots)
+    if args.large_records > 0:
+        out["large"] = large_leg(fg, args.large_records, 0 if args.no_cpu else args.large_sample)
     print(json.dumps(out), flush=True)
 
 

@@ -61,6 +61,9 @@ def lib():
         L.orc_selfjoin.restype = C.c_int64
         L.orc_selfjoin.argtypes = [C.c_int64, i32p, i64p, i32p, C.c_int32, f, C.c_int,
                                    C.POINTER(i32p), C.POINTER(Counters)]
+        L.orc_selfjoin_queries.restype = C.c_int64
+        L.orc_selfjoin_queries.argtypes = [C.c_int64, i32p, i64p, i32p, C.c_int32, f, i64p, C.c_int64, C.c_int,
+                                           C.POINTER(i32p), C.POINTER(Counters)]
         L.orc_selfjoin_literal.restype = C.c_int64
         L.orc_selfjoin_literal.argtypes = [C.c_int64, i32p, i64p, i32p, C.c_int32, f, C.c_int64,
                                            C.POINTER(i32p), C.POINTER(Counters)]
@@ -141,14 +144,19 @@ class Sets:
         return self.tok[self.pos[i]:self.pos[i] + self.len[i]]
 
 
-def selfjoin(sets: Sets, t: float, nthreads: int = 0, mode: str = "closed", block_size: int = 0):
-    """-> (triples int32 [R,3] of (q, s, overlap) internal indices sorted, counters dict)"""
+def selfjoin(sets: Sets, t: float, nthreads: int = 0, mode: str = "closed", block_size: int = 0, queries=None):
+    """-> (triples int32 [R,3] of (q, s, overlap) internal indices sorted, counters dict).
+    queries: restrict the closed form to these query indices."""
     L = lib()
     if nthreads <= 0:
         nthreads = os.cpu_count() or 1
     out = C.POINTER(C.c_int32)()
     ctr = Counters()
-    if mode == "closed":
+    if queries is not None:
+        qs = np.ascontiguousarray(queries, np.int64)
+        n = L.orc_selfjoin_queries(sets.n, _p32(sets.len), _p64(sets.pos), _p32(sets.tok), sets.n_terms, t,
+                                   _p64(qs), qs.size, nthreads, C.byref(out), C.byref(ctr))
+    elif mode == "closed":
         n = L.orc_selfjoin(sets.n, _p32(sets.len), _p64(sets.pos), _p32(sets.tok), sets.n_terms,
                            t, nthreads, C.byref(out), C.byref(ctr))
     elif mode == "literal":

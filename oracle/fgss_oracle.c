@@ -428,6 +428,7 @@ static int cmp_tri(const void *a, const void *b)
 typedef struct {
     int64_t n; const int32_t *len; const int64_t *pos; const int32_t *tok; float t;
     const orc_index *ix;
+    const int64_t *queries; int64_t nq;   /* a query subset (NULL: every record) */
     atomic_llong next;
 } cf_shared;
 
@@ -448,11 +449,13 @@ static void *cf_run(void *arg)
     int32_t *lasts = (int32_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
     int32_t *touched = (int32_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
     const int64_t CH = 32;
+    const int64_t nq = sh->queries ? sh->nq : n;
     for (;;) {
         int64_t q0 = atomic_fetch_add(&sh->next, CH);
-        if (q0 >= n) break;
-        int64_t q1 = q0 + CH < n ? q0 + CH : n;
-        for (int64_t q = q0; q < q1; q++) {
+        if (q0 >= nq) break;
+        int64_t q1 = q0 + CH < nq ? q0 + CH : nq;
+        for (int64_t qk = q0; qk < q1; qk++) {
+            const int64_t q = sh->queries ? sh->queries[qk] : qk;
             const uint32_t lq = (uint32_t)sh->len[q];
             const int32_t *Q = sh->tok + sh->pos[q];
             const uint32_t qp = prefix_clamp(orc_max_prefix(lq, t), lq);
@@ -508,9 +511,9 @@ static void *cf_run(void *arg)
     return NULL;
 }
 
-int64_t orc_selfjoin(int64_t n_sets, const int32_t *len, const int64_t *pos,
-                     const int32_t *tok, int32_t n_terms, float t, int nthreads,
-                     int32_t **out, orc_counters *ctr)
+static int64_t selfjoin_closed(int64_t n_sets, const int32_t *len, const int64_t *pos,
+                               const int32_t *tok, int32_t n_terms, float t, int nthreads,
+                               const int64_t *queries, int64_t nq, int32_t **out, orc_counters *ctr)
 {
     if (nthreads < 1) nthreads = 1;
     orc_index ix;
@@ -519,6 +522,7 @@ int64_t orc_selfjoin(int64_t n_sets, const int32_t *len, const int64_t *pos,
     build_index(n_sets, len, pos, tok, n_terms, t, &ix, &total.e_idx);
     cf_shared sh;
     sh.n = n_sets; sh.len = len; sh.pos = pos; sh.tok = tok; sh.t = t; sh.ix = &ix;
+    sh.queries = queries; sh.nq = nq;
     atomic_init(&sh.next, 0);
     cf_worker *ws = (cf_worker *)calloc((size_t)nthreads, sizeof(cf_worker));
     pthread_t *th = (pthread_t *)malloc((size_t)nthreads * sizeof(pthread_t));
@@ -540,6 +544,22 @@ int64_t orc_selfjoin(int64_t n_sets, const int32_t *len, const int64_t *pos,
     *out = res;
     if (ctr) *ctr = total;
     return R;
+}
+
+int64_t orc_selfjoin(int64_t n_sets, const int32_t *len, const int64_t *pos,
+                     const int32_t *tok, int32_t n_terms, float t, int nthreads,
+                     int32_t **out, orc_counters *ctr)
+{
+    return selfjoin_closed(n_sets, len, pos, tok, n_terms, t, nthreads, NULL, 0, out, ctr);
+}
+
+int64_t orc_selfjoin_queries(int64_t n_sets, const int32_t *len, const int64_t *pos,
+                             const int32_t *tok, int32_t n_terms, float t, const int64_t *queries,
+                             int64_t n_queries, int nthreads, int32_t **out, orc_counters *ctr)
+{
+    for (int64_t k = 0; k < n_queries; k++)
+        if (queries[k] < 0 || queries[k] >= n_sets) return -1;
+    return selfjoin_closed(n_sets, len, pos, tok, n_terms, t, nthreads, queries, n_queries, out, ctr);
 }
 
 /* ------------------------------------------------------------------------ */

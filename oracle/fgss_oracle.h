@@ -25,6 +25,8 @@
  *   orc_bin_load           fixture decode (ids, lengths, reversal) + round trip
  *   bounds (orc_*size ...) hand-derived exact values + an independent float64
  *                          restatement of the correctly rounded float32 formulas
+ *   orc_selfjoin_queries   == orc_selfjoin restricted to the query subset (pairs and
+ *                          counters summed over the subset)
  *   orc_selfjoin           == orc_selfjoin_literal for block sizes {1, 7, N}
  *                          (two independent restatements: closed form vs literal
  *                          tile loop), subset of orc_selfjoin_brute with the
@@ -94,6 +96,11 @@ typedef struct {
 int64_t orc_selfjoin(int64_t n_sets, const int32_t *len, const int64_t *pos,
                      const int32_t *tok, int32_t n_terms, float t, int nthreads,
                      int32_t **out, orc_counters *ctr);
+/* orc_selfjoin restricted to the given queries (internal indices, each against
+ * every s > q): their triples and counters (e_idx is the whole index's). */
+int64_t orc_selfjoin_queries(int64_t n_sets, const int32_t *len, const int64_t *pos,
+                             const int32_t *tok, int32_t n_terms, float t, const int64_t *queries,
+                             int64_t n_queries, int nthreads, int32_t **out, orc_counters *ctr);
 int64_t orc_selfjoin_literal(int64_t n_sets, const int32_t *len, const int64_t *pos,
                              const int32_t *tok, int32_t n_terms, float t,
                              int64_t block_size, int32_t **out, orc_counters *ctr);

@@ -198,3 +198,26 @@ def test_empty_records_never_join(oracle_mod):
     s = oracle_mod.Sets(ints)
     tri, _ = oracle_mod.selfjoin(s, np.float32(0.5))
     assert tri.tolist() == [[0, 1, 2]]
+
+
+@pytest.mark.parametrize("t", [0.5, 0.8])
+def test_query_subset_equals_full_join_restricted(oracle_mod, t):
+    b, _ = oracle_mod.tokenize(_titles(800, 77), 3)
+    s = oracle_mod.Sets(b)
+    tri, ctr = oracle_mod.selfjoin(s, t)
+    qs = np.random.default_rng(5).choice(s.n, 120, replace=False)
+    sub, sc = oracle_mod.selfjoin(s, t, queries=qs)
+    keep = np.isin(tri[:, 0], qs)
+    assert np.array_equal(sub, tri[keep])
+    # counters: per-query sums, from single-query runs
+    tot = {k: 0 for k in ("p_q", "e_scan", "candidates", "intersections", "results", "m_reads")}
+    for q in qs[:20]:
+        _, c1 = oracle_mod.selfjoin(s, t, queries=[q])
+        for k in tot:
+            tot[k] += c1[k]
+    _, c20 = oracle_mod.selfjoin(s, t, queries=qs[:20])
+    for k in tot:
+        assert c20[k] == tot[k], k
+    assert sc["e_idx"] == ctr["e_idx"]
+    with pytest.raises(ValueError):
+        oracle_mod.selfjoin(s, t, queries=[s.n])
