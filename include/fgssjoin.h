@@ -131,8 +131,10 @@ typedef struct {
 typedef struct {
     int32_t load_pct;          /* postings per work item as % of its table slots, 0 = default (65) */
     int32_t table_slots;       /* LDS candidate table slots per work item: 256..4096, 0 = default (512) */
-    int32_t stream_parts;      /* a query split into more parts than this is walked by one wavefront,
-                                  part after part, 0 = default (64)                  */
+    int32_t stream_parts;      /* a query split into more parts than this is walked part after part
+                                  by wavefronts holding one posting cursor per probe token (32 parts
+                                  each); 0 = default: only queries whose boundary rows do not fit
+                                  the boundary table                                   */
     int32_t bounds_cap;        /* precomputed split boundaries at most, 0 = default (2^28); the
                                   parts of queries beyond it locate their posting ranges by search */
 } fgss_join_opts;

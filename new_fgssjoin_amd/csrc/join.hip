@@ -107,7 +107,10 @@ constexpr uint32_t kPadMask = 0x1FFFFFFFu;
 // streamed items keep one posting cursor per probe token in the wavefront's LDS
 // (the range arrays of the other items)
 constexpr uint32_t kStreamQp = 256;
-constexpr uint32_t kStreamParts = 64;            // default: queries split into more parts are streamed
+// default: only queries whose boundary rows do not fit the table are streamed (for
+// short probes the planned parts spread better over the wavefronts)
+constexpr uint32_t kStreamParts = 0xFFFFFFFFu;
+constexpr uint32_t kStreamChunk = 32;            // parts per streamed item (a heavy query gets several)
 
 // How the items of a query with np parts read their posting ranges.
 enum ItemMode : uint32_t { IM_NONE = 0, IM_WHOLE, IM_PLANNED, IM_STREAMED, IM_SEARCHED };
@@ -591,7 +594,7 @@ __global__ void k_nitems(const uint32_t *__restrict__ nparts, const uint32_t *__
     if (q >= n) return;
     const uint32_t np = nparts[q];
     const uint32_t m = item_mode(np, probelen[q], bcnt[q], bofs[q], cap_bounds);
-    nitems[q] = m == IM_NONE ? 0u : (m == IM_WHOLE || m == IM_STREAMED) ? 1u : np;
+    nitems[q] = m == IM_NONE ? 0u : m == IM_WHOLE ? 1u : m == IM_STREAMED ? (np + kStreamChunk - 1) / kStreamChunk : np;
 }
 
 // wlo: window start per query (null: q + 1, the self-join window).  rs: the queries
@@ -610,7 +613,7 @@ __global__ void k_items(int rs, const uint32_t *__restrict__ wlo, const uint32_t
     const uint32_t np = nparts[q];
     const uint32_t mode = item_mode(np, probelen[q], bcnt[q], bofs[q], cap_bounds);
     if (mode == IM_NONE) return;
-    const uint32_t nit = (mode == IM_WHOLE || mode == IM_STREAMED) ? 1u : np;
+    const uint32_t nit = mode == IM_WHOLE ? 1u : mode == IM_STREAMED ? (np + kStreamChunk - 1) / kStreamChunk : np;
     uint64_t a0 = wlo ? (uint64_t)wlo[q] : (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
     uint32_t o = partoff[q];
     if ((uint64_t)o + nit > cap_items) return;   // too small: the host resizes and reruns
@@ -622,11 +625,18 @@ __global__ void k_items(int rs, const uint32_t *__restrict__ wlo, const uint32_t
     for (uint32_t p = 0; p < nit; p++) {
         Item it;
         it.q = (uint32_t)r;
-        it.a = mode == IM_STREAMED ? (uint32_t)a0 : part_start(a0, W, p, np);
-        it.b = mode == IM_STREAMED ? (uint32_t)(a0 + W) : part_start(a0, W, p + 1, np);
+        if (mode == IM_STREAMED) {   // chunk p: parts [p * kStreamChunk, ...) of the query's np
+            const uint32_t p0 = p * kStreamChunk, p1 = min(p0 + kStreamChunk, np);
+            it.a = part_start(a0, W, p0, np);
+            it.b = part_start(a0, W, p1, np);
+            it.nparts = p1 - p0;
+        } else {
+            it.a = part_start(a0, W, p, np);
+            it.b = part_start(a0, W, p + 1, np);
+            it.nparts = np;
+        }
         it.pad = per | fl;
         it.part = p;
-        it.nparts = np;
         it.bofs = mode == IM_PLANNED ? (uint32_t)bofs[q] : 0u;
         it.job = job[r];
         it.t = tj[it.job];
@@ -821,11 +831,14 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         const uint32_t *hi_row = planned && item.part + 1 < item.nparts ? A.bounds + item.bofs + (size_t)item.part * qp
                                                                         : nullptr;
         // a streamed item walks its nparts windows in order with one cursor per probe
-        // token (LDS), starting at the token's range in the whole window
+        // token (LDS), starting at the token's first posting inside the item's window
         uint32_t *cur = rlo;
         static_assert(2 * kTokBatch + 1 >= (int)kStreamQp, "cursors live in rlo and rpre");
         if (streamed) {
-            for (uint32_t j = lane; j < qp; j += kWave) cur[j] = qrng[j].x;
+            for (uint32_t j = lane; j < qp; j += kWave) {
+                const uint2 r = qrng[j];
+                cur[j] = item.part == 0 ? r.x : lower_bound_sid(A.psid, A.fence, r.x, r.y, item.a);
+            }
             wave_sync();
         }
         const uint32_t nwin = streamed ? item.nparts : 1u;

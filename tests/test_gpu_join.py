@@ -229,3 +229,14 @@ def test_titles_100k_streamed(fg, oracle_mod):
     coll = fg.Collection.generate(fg.GEN_TITLES, 100000, 2, device=True)
     sets = oracle_sets(oracle_mod, coll)
     assert_join_equal(oracle_mod, sets, coll.selfjoin(0.6, stream_parts=2), 0.6)
+
+
+@pytest.mark.parametrize("load", [3, 20])
+def test_streamed_chunks(fg, oracle_mod, load):
+    """A few postings per part: heavy queries get many parts, streamed 16 parts per
+    item, each item positioning its cursors at its own window start."""
+    coll = fg.Collection.generate(fg.GEN_TITLES, 4096, 7)
+    sets = oracle_sets(oracle_mod, coll)
+    for t in (0.5, 0.8):
+        _say(f"streamed chunks t={t} load={load}")
+        assert_join_equal(oracle_mod, sets, coll.selfjoin(t, table_slots=256, load_pct=load, stream_parts=1), t)
