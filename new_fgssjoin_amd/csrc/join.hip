@@ -142,7 +142,7 @@ struct fgss_device_sets {
     float *tj = nullptr;        // per job: threshold of the current run
     DevBuf b_psid, b_fence, b_bcnt, b_bofs, b_bounds, b_probelen, b_poff, b_rng, b_eq, b_midlen, b_midoff, b_ka,
         b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_post, b_loff, b_hi, b_nparts, b_partoff, b_scanscr, b_items,
-        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc, b_nitems, b_ovfp;
+        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc, b_nitems, b_ovfp, b_wctr;
     uint32_t cap_items = 0, cap_bounds = 0, cap_out = 0;   // current capacities (elements)
     uint32_t cap_ovf = 0;         // overflow records (a streamed item can record one per part)
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -192,7 +192,7 @@ void fgss_device_release(fgss_collection *c)
                       &d->b_rng, &d->b_eq, &d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib,
                       &d->b_sortscr, &d->b_sid_of, &d->b_post, &d->b_loff, &d->b_hi, &d->b_nparts, &d->b_partoff,
                       &d->b_scanscr, &d->b_items, &d->b_items2, &d->b_ovf, &d->b_out, &d->b_ctr, &d->b_jctr,
-                      &d->b_nitems, &d->b_ovfp,
+                      &d->b_nitems, &d->b_ovfp, &d->b_wctr,
                       &d->b_misc};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -700,6 +700,7 @@ struct JoinArgs {
     unsigned long long *ctr_job;  // per job: kJobCtr counters
     int32_t *out;               // (q, s, overlap) triples
     uint32_t out_cap;
+    unsigned long long *wctr;   // kWaveSlot words per wavefront of the grid
     uint32_t *ovf;              // indices of overflowed items
     uint32_t *ovfp;             // ... and, for streamed items, the overflowed part (else kEmpty)
     uint32_t ovf_cap;           // records beyond it are counted only: the host grows the list and reruns
@@ -745,11 +746,23 @@ __device__ __forceinline__ int ceil_log2(uint32_t x)
 //     ballot + popcount.
 // Each phase compacts the list in place and resets the table slots it drops, so the
 // table is empty again when the item ends: no per-item clear.
-constexpr int kUnroll = 4;
+// postings each lane loads per step of phase A, and LDS table slots per expected
+// posting (tuning knobs, overridable at build time)
+#ifndef FGSS_UNROLL
+#define FGSS_UNROLL 4
+#endif
+#ifndef FGSS_TABLE_FACTOR
+#define FGSS_TABLE_FACTOR 2
+#endif
+constexpr int kUnroll = FGSS_UNROLL;
 constexpr int kQStage = 160;   // query tokens staged in LDS for verification (longer queries stay in HBM)
 // per-wave counters kept in LDS (not registers): totals, shader-clock phases, current job
 enum WaveAcc { W_ESCAN = 0, W_CAND, W_SIG, W_VER, W_SIG2, W_INTER, W_VTOK, W_CYC0, W_JOB0 = W_CYC0 + 6,
                kAccSlots = W_JOB0 + kJobCtr };
+// per-wavefront slot in global memory at the end of a launch: the kAccSlots
+// counters and the wavefront's last job
+constexpr int kWaveSlot = 32;
+static_assert(kAccSlots + 1 <= kWaveSlot && kAccSlots + 1 <= kWave, "slot layout");
 // LDS words per wavefront: counters (u64), table (key, count, last match), list,
 // token ranges, query stage; even so that the next wave's counters stay 8-byte aligned
 constexpr uint32_t join_wave_words(int cap)
@@ -818,8 +831,8 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         const uint32_t qpos = item.qpos;
         uint32_t qp = midprobe ? fgss_mid_prefix(lq, t) : fgss_max_prefix(lq, t);
         if (qp > lq) qp = lq;
-        // table sized for twice the expected postings (64 .. CAP slots)
-        int tlog = ceil_log2(2 * (item.pad & kPadMask));
+        // table sized for FGSS_TABLE_FACTOR x the expected postings (64 .. CAP slots)
+        int tlog = ceil_log2(FGSS_TABLE_FACTOR * (item.pad & kPadMask));
         tlog = tlog < 6 ? 6 : (tlog > LOG2CAP ? LOG2CAP : tlog);
         const uint32_t tsize = 1u << tlog;
 
@@ -896,56 +909,66 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             WACC(W_CYC0 + 5, clock64() - r0);
             escan += total;
             const int steps = ceil_log2(nb);   // wave-uniform search depth
+            // The posting loop runs without branches on per-lane conditions (on this
+            // chip every divergent branch costs scalar issue slots, which bound the
+            // kernel): lanes without a posting load posting 0 and aim their table
+            // updates at a word of their own in the idle query stage.
+            uint32_t *scr = qstage;
             for (uint32_t e0 = 0; e0 < total; e0 += kWave * kUnroll) {
                 uint2 pe[kUnroll];
-                uint32_t jv[kUnroll], seg[kUnroll], e[kUnroll];
-                // segment of each flattened posting: the largest jj with rpre[jj] <= e,
-                // found by a fixed-depth branchless search so the kUnroll searches interleave
+                uint32_t seg[kUnroll], e[kUnroll];
 #pragma unroll
                 for (int k = 0; k < kUnroll; k++) { e[k] = e0 + (uint32_t)(k * kWave) + lane; seg[k] = 0; }
+                // segment of each flattened posting: the largest jj with rpre[jj] <= e, by a
+                // fixed-depth search (rpre[nb] = total keeps it below nb for e < total)
                 for (int st = steps - 1; st >= 0; st--) {
 #pragma unroll
                     for (int k = 0; k < kUnroll; k++) {
-                        const uint32_t mid = seg[k] + (1u << st);
-                        if (mid < nb && rpre[mid] <= e[k]) seg[k] = mid;
-                    }
-                }
-                // load kUnroll postings per lane before touching the table
-#pragma unroll
-                for (int k = 0; k < kUnroll; k++) {
-                    jv[k] = kEmpty;
-                    if (e[k] < total) {
-                        jv[k] = jb + seg[k];
-                        pe[k] = A.post[rlo[seg[k]] + (e[k] - rpre[seg[k]])];
+                        const uint32_t mid = min(seg[k] + (1u << st), nb);
+                        seg[k] = rpre[mid] <= e[k] ? mid : seg[k];
                     }
                 }
 #pragma unroll
                 for (int k = 0; k < kUnroll; k++) {
-                    bool valid = jv[k] != kEmpty;
-                    if (RS && valid) {   // the source's own prefix at t
+                    const uint32_t idx = e[k] < total ? rlo[seg[k]] + (e[k] - rpre[seg[k]]) : 0u;
+                    pe[k] = A.post[idx];
+                }
+#pragma unroll
+                for (int k = 0; k < kUnroll; k++) {
+                    bool valid = e[k] < total;
+                    if (RS) {   // the source's own prefix at t
                         const uint32_t ls = pe[k].y >> 16;
-                        valid = (pe[k].y & 0xFFFFu) < (midprobe ? fgss_max_prefix(ls, t) : fgss_mid_prefix(ls, t));
+                        valid = valid &&
+                                (pe[k].y & 0xFFFFu) < (midprobe ? fgss_max_prefix(ls, t) : fgss_mid_prefix(ls, t));
                     }
-                    bool ok = true, isnew = false;
-                    uint32_t h = 0;
-                    if (valid) {
-                        const uint32_t s = pe[k].x;
-                        h = slot_hash(s, tlog);
+                    const uint32_t s = pe[k].x;
+                    uint32_t h = slot_hash(s, tlog);
+                    uint32_t prev = atomicCAS(valid ? &tkey[h] : &scr[lane], kEmpty, s);
+                    bool isnew = valid && prev == kEmpty;
+                    bool pend = valid && prev != kEmpty && prev != s;
+                    bool ok = true;
+                    if (__ballot(pend)) {   // collisions: linear probing, tsize attempts at most
                         uint32_t probes = 0;
-                        for (;;) {
-                            const uint32_t prev = atomicCAS(&tkey[h], kEmpty, s);
-                            if (prev == kEmpty) { isnew = true; break; }
-                            if (prev == s) break;
-                            h = (h + 1) & (tsize - 1);
-                            if (++probes >= tsize) { ok = false; break; }
-                        }
-                        if (ok) {
-                            atomicAdd(&tcnt[h], 1u);
-                            atomicMax(&tlast[h], (jv[k] << 16) | (pe[k].y & 0xFFFFu));
+                        while (__ballot(pend)) {
+                            if (pend) {
+                                h = (h + 1) & (tsize - 1);
+                                if (++probes >= tsize) {
+                                    ok = false;
+                                    pend = false;
+                                } else {
+                                    prev = atomicCAS(&tkey[h], kEmpty, s);
+                                    if (prev == kEmpty) { isnew = true; pend = false; }
+                                    else if (prev == s) pend = false;
+                                }
+                            }
                         }
                     }
+                    const bool hit = valid && ok;
+                    atomicAdd(hit ? &tcnt[h] : &scr[lane], 1u);
+                    atomicMax(hit ? &tlast[h] : &scr[lane], ((jb + seg[k]) << 16) | (pe[k].y & 0xFFFFu));
                     const uint64_t nm = __ballot(isnew);
-                    if (isnew) clist[nlist + (uint32_t)__popcll(nm & lanemask_lt())] = h | (pe[k].y & 0xFFFF0000u);
+                    *(isnew ? &clist[nlist + (uint32_t)__popcll(nm & lanemask_lt())] : &scr[lane]) =
+                        h | (pe[k].y & 0xFFFF0000u);
                     nlist += (uint32_t)__popcll(nm);
                     if (__ballot(!ok)) overflow = true;
                 }
@@ -1237,16 +1260,53 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         WACC(W_VTOK, vtok);
         }   // windows
     }
-    if (lane == 0) {
-        atomicAdd(&A.ctr[C_ESCAN], wacc[W_ESCAN]);
-        atomicAdd(&A.ctr[C_CAND], wacc[W_CAND]);
-        atomicAdd(&A.ctr[C_SIGCHK], wacc[W_SIG]);
-        atomicAdd(&A.ctr[C_VERIFIED], wacc[W_VER]);
-        atomicAdd(&A.ctr[C_SIG2], wacc[W_SIG2]);
-        atomicAdd(&A.ctr[C_INTER], wacc[W_INTER]);
-        atomicAdd(&A.ctr[C_VTOK], wacc[W_VTOK]);
-        for (int k = 0; k < 6; k++) atomicAdd(&A.ctr[C_CYC_SETUP + k], wacc[W_CYC0 + k]);
-        if (cur_job != kEmpty) flush_job(A.ctr_job, cur_job, wacc + W_JOB0);
+    // this wavefront's totals and its last job's pending counters to its own slot:
+    // one coalesced store, no atomics (thousands of wavefronts adding to the same
+    // counters at the end of the launch serialize); k_wave_reduce sums the slots
+    wave_sync();
+    unsigned long long *slot = A.wctr + ((size_t)blockIdx.x * join_waves(CAP) + (size_t)w) * kWaveSlot;
+    if (lane < kAccSlots) slot[lane] = wacc[lane];
+    if (lane == kAccSlots) slot[lane] = cur_job;
+}
+
+// Sum the per-wavefront slots of a k_join launch into the global counters and the
+// per-job counters: one block; each thread adds up a contiguous run of slots and
+// issues one atomic per job change of its run.
+__global__ __launch_bounds__(256) void k_wave_reduce(const unsigned long long *__restrict__ wctr, uint32_t nwaves,
+                                                     unsigned long long *__restrict__ ctr,
+                                                     unsigned long long *__restrict__ ctr_job)
+{
+    constexpr int kMain = W_JOB0;   // global counters per slot
+    const int dst[kMain] = {C_ESCAN, C_CAND, C_SIGCHK, C_VERIFIED, C_SIG2, C_INTER, C_VTOK,
+                            C_CYC_SETUP, C_CYC_SETUP + 1, C_CYC_SETUP + 2, C_CYC_SETUP + 3, C_CYC_SETUP + 4,
+                            C_CYC_RANGES};
+    __shared__ unsigned long long part[16][16];
+    const int t = (int)threadIdx.x, k = t & 15, g = t >> 4;
+    unsigned long long sum = 0;
+    if (k < kMain)
+        for (uint32_t wv = (uint32_t)g; wv < nwaves; wv += 16) sum += wctr[(size_t)wv * kWaveSlot + k];
+    part[g][k] = sum;
+    __syncthreads();
+    if (t < kMain) {
+        unsigned long long tot = 0;
+        for (int i = 0; i < 16; i++) tot += part[i][t];
+        if (tot) atomicAdd(&ctr[dst[t]], tot);
+    }
+    // per-job counters of each slot's last job
+    const uint32_t w0 = (uint32_t)((uint64_t)nwaves * t / 256), w1 = (uint32_t)((uint64_t)nwaves * (t + 1) / 256);
+    uint32_t job = kEmpty;
+    unsigned long long acc[kJobCtr] = {0, 0, 0, 0};
+    for (uint32_t wv = w0; wv <= w1; wv++) {
+        const uint32_t j = wv < w1 ? (uint32_t)wctr[(size_t)wv * kWaveSlot + kAccSlots] : kEmpty;
+        if (j != job) {
+            if (job != kEmpty)
+                for (int c = 0; c < kJobCtr; c++)
+                    if (acc[c]) atomicAdd(&ctr_job[(size_t)job * kJobCtr + c], acc[c]);
+            job = j;
+            for (int c = 0; c < kJobCtr; c++) acc[c] = 0;
+        }
+        if (j != kEmpty)
+            for (int c = 0; c < kJobCtr; c++) acc[c] += wctr[(size_t)wv * kWaveSlot + W_JOB0 + c];
     }
 }
 
@@ -1271,7 +1331,12 @@ static int launch_join(fgss_device_sets *d, const JoinArgs &A, bool dry = false)
         cached[1] = d->device;
     }
     if (dry) return FGSS_OK;
+    if ((uint64_t)cached[0] * join_waves(CAP) * kWaveSlot * 8 > d->b_wctr.cap)
+        return fgss_fail(FGSS_ERR_DEVICE, "internal: wavefront counter slots too small");
     hipLaunchKernelGGL((k_join<CAP, RS>), dim3(cached[0]), dim3(join_waves(CAP) * kWave), smem, d->st, A);
+    HIPCK(hipGetLastError());
+    hipLaunchKernelGGL(k_wave_reduce, dim3(1), dim3(256), 0, d->st, (const unsigned long long *)A.wctr,
+                       (uint32_t)(cached[0] * join_waves(CAP)), A.ctr, A.ctr_job);
     HIPCK(hipGetLastError());
     return FGSS_OK;
 }
@@ -1344,6 +1409,15 @@ static int grow(fgss_device_sets *d, DevBuf &b, size_t bytes)
         if (rc_) return rc_;                  \
     } while (0)
 
+// Slots for the per-wavefront counters of a k_join launch: kWaveSlot words for
+// every wavefront the device can hold (32 per CU), so any persistent grid fits.
+static int wave_slots(fgss_device_sets *d)
+{
+    int cus = 0;
+    HIPCK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device));
+    return grow(d, d->b_wctr, (size_t)std::max(cus, 1) * 32 * kWaveSlot * 8);
+}
+
 // Every buffer the pipeline touches, sized for upper bounds where one exists (index
 // entries and probe tokens <= tokens) and for the current capacities otherwise.
 static int prepare_buffers(fgss_device_sets *d)
@@ -1380,6 +1454,10 @@ static int prepare_buffers(fgss_device_sets *d)
     d->cap_ovf = std::max(d->cap_ovf, d->cap_items);
     GROW(d->b_ovf, (size_t)d->cap_ovf * 4);
     GROW(d->b_ovfp, (size_t)d->cap_ovf * 4);
+    {
+        int rc_ = wave_slots(d);
+        if (rc_) return rc_;
+    }
     GROW(d->b_out, (size_t)d->cap_out * 12);
     GROW(d->b_ctr, C_COUNT * sizeof(uint64_t));
     GROW(d->b_jctr, (size_t)d->njobs * kJobCtr * sizeof(uint64_t));
@@ -1401,6 +1479,7 @@ static JoinArgs join_args(fgss_device_sets *d)
     A.job = d->job; A.tj = d->tj; A.ctr_job = P<unsigned long long>(d->b_jctr);
     A.out = P<int32_t>(d->b_out); A.out_cap = d->cap_out; A.ovf = P<uint32_t>(d->b_ovf);
     A.ovfp = P<uint32_t>(d->b_ovfp); A.ovf_cap = d->cap_ovf;
+    A.wctr = P<unsigned long long>(d->b_wctr);
     A.ctr = P<unsigned long long>(d->b_ctr);
     return A;
 }
@@ -2240,6 +2319,10 @@ static int slot_prepare(ServeSlot &sl)
     GROW(d->b_nitems, (size_t)v * 4);
     d->cap_ovf = std::max(d->cap_ovf, d->cap_items);
     GROW(d->b_ovfp, (size_t)d->cap_ovf * 4);
+    {
+        int rc_ = wave_slots(d);
+        if (rc_) return rc_;
+    }
     GROW(d->b_bounds, (size_t)d->cap_bounds * 4);
     GROW(d->b_items, (size_t)d->cap_items * sizeof(Item));
     GROW(d->b_ovf, (size_t)d->cap_ovf * 4);
