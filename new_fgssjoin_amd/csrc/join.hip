@@ -805,17 +805,37 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
     // items are claimed kQueueBatch at a time: one global atomic per batch
     uint32_t batch_next = 0, batch_end = 0;
     uint32_t cur_job = kEmpty;
+    // the next item of the claimed batch is loaded while the current one runs
+    Item nxt;
+    bool have_nxt = false;
     for (;;) {
         if (batch_next >= batch_end) {
             uint32_t b0 = 0;
             if (lane == 0) b0 = atomicAdd((unsigned int *)&A.ctr[C_QUEUE], (unsigned)kQueueBatch);
-            b0 = __shfl(b0, 0, kWave);
+            b0 = __builtin_amdgcn_readfirstlane(b0);
             batch_next = b0;
             batch_end = min(b0 + (uint32_t)kQueueBatch, n_items);
+            have_nxt = false;
         }
         if (batch_next >= n_items) break;
         const uint32_t it = batch_next++;
-        const Item item = A.items[it];
+        Item ld = have_nxt ? nxt : A.items[it];
+        have_nxt = batch_next < batch_end;
+        if (have_nxt) nxt = A.items[batch_next];
+        // the item is the same for every lane: its fields live in scalar registers
+        Item item;
+        item.q = __builtin_amdgcn_readfirstlane(ld.q);
+        item.a = __builtin_amdgcn_readfirstlane(ld.a);
+        item.b = __builtin_amdgcn_readfirstlane(ld.b);
+        item.pad = __builtin_amdgcn_readfirstlane(ld.pad);
+        item.part = __builtin_amdgcn_readfirstlane(ld.part);
+        item.nparts = __builtin_amdgcn_readfirstlane(ld.nparts);
+        item.bofs = __builtin_amdgcn_readfirstlane(ld.bofs);
+        item.job = __builtin_amdgcn_readfirstlane(ld.job);
+        item.t = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(ld.t)));
+        item.lq = __builtin_amdgcn_readfirstlane(ld.lq);
+        item.qpos = __builtin_amdgcn_readfirstlane(ld.qpos);
+        item.qpoff = __builtin_amdgcn_readfirstlane(ld.qpoff);
         const uint32_t q = item.q;
         const uint32_t jq = item.job;
         const float t = item.t;
