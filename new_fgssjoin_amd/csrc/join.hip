@@ -1290,8 +1290,9 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
 }
 
 // Sum the per-wavefront slots of a k_join launch into the global counters and the
-// per-job counters: one block; each thread adds up a contiguous run of slots and
-// issues one atomic per job change of its run.
+// per-job counters: a thread per slot.  Global counters: a wavefront sum, then one
+// atomic per wavefront.  Per-job counters: consecutive slots usually share a job,
+// so a segmented sum over equal job ids leaves one atomic per run of slots.
 __global__ __launch_bounds__(256) void k_wave_reduce(const unsigned long long *__restrict__ wctr, uint32_t nwaves,
                                                      unsigned long long *__restrict__ ctr,
                                                      unsigned long long *__restrict__ ctr_job)
@@ -1300,33 +1301,32 @@ __global__ __launch_bounds__(256) void k_wave_reduce(const unsigned long long *_
     const int dst[kMain] = {C_ESCAN, C_CAND, C_SIGCHK, C_VERIFIED, C_SIG2, C_INTER, C_VTOK,
                             C_CYC_SETUP, C_CYC_SETUP + 1, C_CYC_SETUP + 2, C_CYC_SETUP + 3, C_CYC_SETUP + 4,
                             C_CYC_RANGES};
-    __shared__ unsigned long long part[16][16];
-    const int t = (int)threadIdx.x, k = t & 15, g = t >> 4;
-    unsigned long long sum = 0;
-    if (k < kMain)
-        for (uint32_t wv = (uint32_t)g; wv < nwaves; wv += 16) sum += wctr[(size_t)wv * kWaveSlot + k];
-    part[g][k] = sum;
-    __syncthreads();
-    if (t < kMain) {
-        unsigned long long tot = 0;
-        for (int i = 0; i < 16; i++) tot += part[i][t];
-        if (tot) atomicAdd(&ctr[dst[t]], tot);
+    const uint32_t wv = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = wv < nwaves;
+    const unsigned long long *slot = wctr + (size_t)(in ? wv : 0) * kWaveSlot;
+    const int lane = lane_id();
+#pragma unroll
+    for (int k = 0; k < kMain; k++) {
+        const unsigned long long tot = wave_sum(in ? slot[k] : 0ull);
+        if (lane == 0 && tot) atomicAdd(&ctr[dst[k]], tot);
     }
-    // per-job counters of each slot's last job
-    const uint32_t w0 = (uint32_t)((uint64_t)nwaves * t / 256), w1 = (uint32_t)((uint64_t)nwaves * (t + 1) / 256);
-    uint32_t job = kEmpty;
-    unsigned long long acc[kJobCtr] = {0, 0, 0, 0};
-    for (uint32_t wv = w0; wv <= w1; wv++) {
-        const uint32_t j = wv < w1 ? (uint32_t)wctr[(size_t)wv * kWaveSlot + kAccSlots] : kEmpty;
-        if (j != job) {
-            if (job != kEmpty)
-                for (int c = 0; c < kJobCtr; c++)
-                    if (acc[c]) atomicAdd(&ctr_job[(size_t)job * kJobCtr + c], acc[c]);
-            job = j;
-            for (int c = 0; c < kJobCtr; c++) acc[c] = 0;
-        }
-        if (j != kEmpty)
-            for (int c = 0; c < kJobCtr; c++) acc[c] += wctr[(size_t)wv * kWaveSlot + W_JOB0 + c];
+    const uint32_t job = in ? (uint32_t)slot[kAccSlots] : kEmpty;
+    const uint32_t prev = __shfl_up(job, 1, kWave), next = __shfl_down(job, 1, kWave);
+    const bool head = lane == 0 || prev != job, tail = lane == kWave - 1 || next != job;
+    // each lane's segment start: a max-scan of head positions
+    uint32_t start = head ? (uint32_t)lane : 0u;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t o = __shfl_up(start, d, kWave);
+        if (lane >= d) start = max(start, o);
+    }
+#pragma unroll
+    for (int c = 0; c < kJobCtr; c++) {
+        const unsigned long long v = (in && job != kEmpty) ? slot[W_JOB0 + c] : 0ull;
+        const unsigned long long incl = wave_inclusive_scan(v);
+        const unsigned long long before = __shfl(incl, (int)(start > 0 ? start - 1 : 0), kWave);
+        const unsigned long long seg = incl - (start > 0 ? before : 0ull);
+        if (tail && job != kEmpty && seg) atomicAdd(&ctr_job[(size_t)job * kJobCtr + c], seg);
     }
 }
 
@@ -1355,8 +1355,9 @@ static int launch_join(fgss_device_sets *d, const JoinArgs &A, bool dry = false)
         return fgss_fail(FGSS_ERR_DEVICE, "internal: wavefront counter slots too small");
     hipLaunchKernelGGL((k_join<CAP, RS>), dim3(cached[0]), dim3(join_waves(CAP) * kWave), smem, d->st, A);
     HIPCK(hipGetLastError());
-    hipLaunchKernelGGL(k_wave_reduce, dim3(1), dim3(256), 0, d->st, (const unsigned long long *)A.wctr,
-                       (uint32_t)(cached[0] * join_waves(CAP)), A.ctr, A.ctr_job);
+    const uint32_t nw = (uint32_t)(cached[0] * join_waves(CAP));
+    hipLaunchKernelGGL(k_wave_reduce, dim3((nw + 255) / 256), dim3(256), 0, d->st, (const unsigned long long *)A.wctr,
+                       nw, A.ctr, A.ctr_job);
     HIPCK(hipGetLastError());
     return FGSS_OK;
 }
