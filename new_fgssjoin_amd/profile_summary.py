@@ -2,6 +2,7 @@
 short text files kept beside the benchmark.
 
     python3 new_fgssjoin_amd/profile_summary.py stats <kernel_stats.csv>
+    python3 new_fgssjoin_amd/profile_summary.py trace <kernel_trace.csv> --kernel k_join --last 2
     python3 new_fgssjoin_amd/profile_summary.py pmc --kernel k_join --last 3 \\
         --copy-fetch <copy FETCH_SIZE csv> --copy-write <copy WRITE_SIZE csv> \\
         --fetch <FETCH_SIZE csv> --write <WRITE_SIZE csv> [--lds <SQ_LDS_* csv>]
@@ -24,7 +25,7 @@ COPY_BYTES = 1 << 30
 
 
 def short(name: str) -> str:
-    name = re.sub(r"^void ", "", name)
+    name = re.sub(r"^void ", "", name).replace("(anonymous namespace)::", "")
     return re.sub(r"\(.*$", "", name).replace("fgss::", "")
 
 
@@ -36,6 +37,22 @@ def stats(path: str, top: int = 16):
     for r in rows[:top]:
         print(f"| {short(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
               f"{float(r['TotalDurationNs']) / 1e6:.3f} | {float(r['Percentage']):.1f} |")
+
+
+def trace(path: str, kernel: str, last: int):
+    """durations of the last `last` dispatches of one kernel, from a kernel_trace.csv"""
+    rows = [r for r in csv.DictReader(open(path)) if kernel in r["Kernel_Name"]]
+    rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+    rows = rows[-last:]
+    print(f"| {kernel} dispatch | duration ms | grid threads |")
+    print("|---:|---:|---:|")
+    tot = 0.0
+    for r in rows:
+        ms = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
+        tot += ms
+        print(f"| {r['Dispatch_Id']} | {ms:.3f} | {r['Grid_Size_X']} |")
+    print()
+    print(f"kernel_ms_per_launch {tot / max(len(rows), 1):.6g}")
 
 
 def _per_dispatch(path: str, kernel: str):
@@ -92,6 +109,10 @@ def main():
     s = sub.add_parser("stats")
     s.add_argument("csv")
     s.add_argument("--top", type=int, default=16)
+    t = sub.add_parser("trace")
+    t.add_argument("csv")
+    t.add_argument("--kernel", default="k_join")
+    t.add_argument("--last", type=int, default=3)
     p = sub.add_parser("pmc")
     p.add_argument("--kernel", default="k_join")
     p.add_argument("--last", type=int, default=3)
@@ -103,6 +124,8 @@ def main():
     a = ap.parse_args()
     if a.cmd == "stats":
         stats(a.csv, a.top)
+    elif a.cmd == "trace":
+        trace(a.csv, a.kernel, a.last)
     else:
         pmc(a)
 
