@@ -923,7 +923,8 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                 if (jj < nb) rpre[jj] = run;
                 run += v[k];
             }
-            const uint32_t total = __shfl(inc, kWave - 1, kWave);
+            // wave-uniform: the posting loop below is a scalar loop
+            const uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(inc, kWave - 1, kWave));
             if (lane == 0) rpre[nb] = total;
             wave_sync();
             WACC(W_CYC0 + 5, clock64() - r0);
@@ -931,67 +932,94 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             const int steps = ceil_log2(nb);   // wave-uniform search depth
             // The posting loop runs without branches on per-lane conditions (on this
             // chip every divergent branch costs scalar issue slots, which bound the
-            // kernel): lanes without a posting load posting 0 and aim their table
-            // updates at a word of their own in the idle query stage.
+            // kernel): lanes without a posting load posting 0 and aim their LDS updates
+            // at a word of their own in the idle query stage.  The kUnroll table probes
+            // of a step are issued together and wait once; collisions are resolved for
+            // all of them in one loop (linear probing stays exact whatever the order
+            // in which keys are placed, as no slot is ever emptied during the phase).
             uint32_t *scr = qstage;
             for (uint32_t e0 = 0; e0 < total; e0 += kWave * kUnroll) {
                 uint2 pe[kUnroll];
                 uint32_t seg[kUnroll], e[kUnroll];
+                // lanes past the end work on the last posting (a real one: every load
+                // and LDS read below is unconditional) and take no part in the updates
 #pragma unroll
-                for (int k = 0; k < kUnroll; k++) { e[k] = e0 + (uint32_t)(k * kWave) + lane; seg[k] = 0; }
+                for (int k = 0; k < kUnroll; k++) { e[k] = min(e0 + (uint32_t)(k * kWave) + lane, total - 1u); seg[k] = 0; }
                 // segment of each flattened posting: the largest jj with rpre[jj] <= e, by a
-                // fixed-depth search (rpre[nb] = total keeps it below nb for e < total)
-                for (int st = steps - 1; st >= 0; st--) {
+                // search of wave-uniform depth (rpre[nb] = total > e keeps it below nb)
 #pragma unroll
-                    for (int k = 0; k < kUnroll; k++) {
-                        const uint32_t mid = min(seg[k] + (1u << st), nb);
-                        seg[k] = rpre[mid] <= e[k] ? mid : seg[k];
-                    }
-                }
+                for (int st = 6; st >= 0; st--) {
+                    static_assert(kTokBatch <= 128, "search depth");
+                    if (st < steps) {
 #pragma unroll
-                for (int k = 0; k < kUnroll; k++) {
-                    const uint32_t idx = e[k] < total ? rlo[seg[k]] + (e[k] - rpre[seg[k]]) : 0u;
-                    pe[k] = A.post[idx];
-                }
-#pragma unroll
-                for (int k = 0; k < kUnroll; k++) {
-                    bool valid = e[k] < total;
-                    if (RS) {   // the source's own prefix at t
-                        const uint32_t ls = pe[k].y >> 16;
-                        valid = valid &&
-                                (pe[k].y & 0xFFFFu) < (midprobe ? fgss_max_prefix(ls, t) : fgss_mid_prefix(ls, t));
-                    }
-                    const uint32_t s = pe[k].x;
-                    uint32_t h = slot_hash(s, tlog);
-                    uint32_t prev = atomicCAS(valid ? &tkey[h] : &scr[lane], kEmpty, s);
-                    bool isnew = valid && prev == kEmpty;
-                    bool pend = valid && prev != kEmpty && prev != s;
-                    bool ok = true;
-                    if (__ballot(pend)) {   // collisions: linear probing, tsize attempts at most
-                        uint32_t probes = 0;
-                        while (__ballot(pend)) {
-                            if (pend) {
-                                h = (h + 1) & (tsize - 1);
-                                if (++probes >= tsize) {
-                                    ok = false;
-                                    pend = false;
-                                } else {
-                                    prev = atomicCAS(&tkey[h], kEmpty, s);
-                                    if (prev == kEmpty) { isnew = true; pend = false; }
-                                    else if (prev == s) pend = false;
-                                }
-                            }
+                        for (int k = 0; k < kUnroll; k++) {
+                            const uint32_t mid = min(seg[k] + (1u << st), nb);
+                            seg[k] = rpre[mid] <= e[k] ? mid : seg[k];
                         }
                     }
-                    const bool hit = valid && ok;
-                    atomicAdd(hit ? &tcnt[h] : &scr[lane], 1u);
-                    atomicMax(hit ? &tlast[h] : &scr[lane], ((jb + seg[k]) << 16) | (pe[k].y & 0xFFFFu));
-                    const uint64_t nm = __ballot(isnew);
-                    *(isnew ? &clist[nlist + (uint32_t)__popcll(nm & lanemask_lt())] : &scr[lane]) =
-                        h | (pe[k].y & 0xFFFF0000u);
-                    nlist += (uint32_t)__popcll(nm);
-                    if (__ballot(!ok)) overflow = true;
                 }
+#pragma unroll
+                for (int k = 0; k < kUnroll; k++) pe[k] = A.post[rlo[seg[k]] + (e[k] - rpre[seg[k]])];
+                // per lane, bit k of each mask: posting k is in range and, for R-S items,
+                // inside the source's prefix (valm); placed a new key (newm); probing on
+                // (pendm); found no room (failm)
+                uint32_t valm = 0, newm = 0, pendm = 0, failm = 0;
+                uint32_t h[kUnroll], prev[kUnroll];
+#pragma unroll
+                for (int k = 0; k < kUnroll; k++) {
+                    uint32_t v = (e0 + (uint32_t)(k * kWave) + (uint32_t)lane) < total;
+                    if (RS) {   // the source's own prefix at t
+                        const uint32_t ls = pe[k].y >> 16;
+                        v &= (uint32_t)((pe[k].y & 0xFFFFu) < (midprobe ? fgss_max_prefix(ls, t) : fgss_mid_prefix(ls, t)));
+                    }
+                    valm |= v << k;
+                    h[k] = slot_hash(pe[k].x, tlog);
+                }
+#pragma unroll
+                for (int k = 0; k < kUnroll; k++)
+                    prev[k] = atomicCAS(((valm >> k) & 1u) ? &tkey[h[k]] : &scr[lane], kEmpty, pe[k].x);
+#pragma unroll
+                for (int k = 0; k < kUnroll; k++) {
+                    const uint32_t vk = (valm >> k) & 1u;
+                    newm |= (vk & (uint32_t)(prev[k] == kEmpty)) << k;
+                    pendm |= (vk & (uint32_t)(prev[k] != kEmpty) & (uint32_t)(prev[k] != pe[k].x)) << k;
+                }
+                // collisions: linear probing.  Every pending probe started in the pass
+                // above and advances one slot per round, so one wave-uniform round
+                // counter bounds them all by tsize slots.
+                for (uint32_t round = 1; __ballot(pendm != 0); round++) {
+                    if (round >= tsize) { failm |= pendm; break; }
+#pragma unroll
+                    for (int k = 0; k < kUnroll; k++) {
+                        const uint32_t pk = (pendm >> k) & 1u;
+                        h[k] = (h[k] + pk) & (tsize - 1u);
+                        prev[k] = atomicCAS(pk ? &tkey[h[k]] : &scr[lane], kEmpty, pe[k].x);
+                    }
+#pragma unroll
+                    for (int k = 0; k < kUnroll; k++) {
+                        const uint32_t pk = (pendm >> k) & 1u;
+                        const uint32_t placed = pk & (uint32_t)(prev[k] == kEmpty);
+                        newm |= placed << k;
+                        pendm &= ~((placed | (pk & (uint32_t)(prev[k] == pe[k].x))) << k);
+                    }
+                }
+                const uint32_t hitm = valm & ~failm;
+#pragma unroll
+                for (int k = 0; k < kUnroll; k++) {
+                    const bool hit = (hitm >> k) & 1u;
+                    atomicAdd(hit ? &tcnt[h[k]] : &scr[lane], 1u);
+                    atomicMax(hit ? &tlast[h[k]] : &scr[lane], ((jb + seg[k]) << 16) | (pe[k].y & 0xFFFFu));
+                }
+#pragma unroll
+                for (int k = 0; k < kUnroll; k++) {
+                    const bool isnew = (newm >> k) & 1u;
+                    const uint64_t nm = __ballot(isnew);
+                    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(nm >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)nm, 0u));
+                    *(isnew ? &clist[nlist + below] : &scr[lane]) = h[k] | (pe[k].y & 0xFFFF0000u);
+                    nlist += (uint32_t)__popcll(nm);
+                }
+                if (__ballot(failm != 0)) overflow = true;
                 if (overflow) break;
             }
             wave_sync();
