@@ -731,6 +731,65 @@ __device__ __forceinline__ int ceil_log2(uint32_t x)
     return x <= 1 ? 0 : 32 - __clz(x - 1);
 }
 
+// Phase A's table update for the kUnroll postings a lane holds (pe[k], query position
+// jpos[k]; bit k of valm set when posting k takes part).  The probes of all kUnroll
+// postings are issued together and wait once; collisions are resolved for all of them
+// in one loop (linear probing stays exact whatever the order in which keys are placed,
+// as no slot is emptied during the phase, and every pending probe advances one slot per
+// round, so one wave-uniform round counter bounds them by tsize slots).  New keys are
+// appended to clist.  Lanes without work aim their LDS updates at scr[lane].  Returns
+// true on any lane when some posting found no free slot.
+template <int N>
+__device__ __forceinline__ bool table_insert(const uint2 (&pe)[N], uint32_t valm, const uint32_t (&jpos)[N],
+                                             uint32_t *tkey, uint32_t *tcnt, uint32_t *tlast, uint32_t *clist,
+                                             uint32_t *scr, int lane, int tlog, uint32_t tsize, uint32_t &nlist)
+{
+    uint32_t newm = 0, pendm = 0, failm = 0;
+    uint32_t h[N], prev[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) h[k] = slot_hash(pe[k].x, tlog);
+#pragma unroll
+    for (int k = 0; k < N; k++) prev[k] = atomicCAS(((valm >> k) & 1u) ? &tkey[h[k]] : &scr[lane], kEmpty, pe[k].x);
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const uint32_t vk = (valm >> k) & 1u;
+        newm |= (vk & (uint32_t)(prev[k] == kEmpty)) << k;
+        pendm |= (vk & (uint32_t)(prev[k] != kEmpty) & (uint32_t)(prev[k] != pe[k].x)) << k;
+    }
+    for (uint32_t round = 1; __ballot(pendm != 0); round++) {
+        if (round >= tsize) { failm |= pendm; break; }
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            const uint32_t pk = (pendm >> k) & 1u;
+            h[k] = (h[k] + pk) & (tsize - 1u);
+            prev[k] = atomicCAS(pk ? &tkey[h[k]] : &scr[lane], kEmpty, pe[k].x);
+        }
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            const uint32_t pk = (pendm >> k) & 1u;
+            const uint32_t placed = pk & (uint32_t)(prev[k] == kEmpty);
+            newm |= placed << k;
+            pendm &= ~((placed | (pk & (uint32_t)(prev[k] == pe[k].x))) << k);
+        }
+    }
+    const uint32_t hitm = valm & ~failm;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const bool hit = (hitm >> k) & 1u;
+        atomicAdd(hit ? &tcnt[h[k]] : &scr[lane], 1u);
+        atomicMax(hit ? &tlast[h[k]] : &scr[lane], (jpos[k] << 16) | (pe[k].y & 0xFFFFu));
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const bool isnew = (newm >> k) & 1u;
+        const uint64_t nm = __ballot(isnew);
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(nm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)nm, 0u));
+        *(isnew ? &clist[nlist + below] : &scr[lane]) = h[k] | (pe[k].y & 0xFFFF0000u);
+        nlist += (uint32_t)__popcll(nm);
+    }
+    return __ballot(failm != 0) != 0;
+}
+
 // Work item = (query q, source window [a, b)).  Phases, all inside one wavefront:
 //  A  candidate generation: the postings of the probe-prefix tokens whose record id
 //     lies in the window, flattened over all tokens, kUnroll per lane in flight; per
@@ -932,11 +991,8 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             const int steps = ceil_log2(nb);   // wave-uniform search depth
             // The posting loop runs without branches on per-lane conditions (on this
             // chip every divergent branch costs scalar issue slots, which bound the
-            // kernel): lanes without a posting load posting 0 and aim their LDS updates
-            // at a word of their own in the idle query stage.  The kUnroll table probes
-            // of a step are issued together and wait once; collisions are resolved for
-            // all of them in one loop (linear probing stays exact whatever the order
-            // in which keys are placed, as no slot is ever emptied during the phase).
+            // kernel): lanes without a posting work on the last one and aim their LDS
+            // updates at a word of their own in the idle query stage.
             uint32_t *scr = qstage;
             for (uint32_t e0 = 0; e0 < total; e0 += kWave * kUnroll) {
                 uint2 pe[kUnroll];
@@ -960,11 +1016,8 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                 }
 #pragma unroll
                 for (int k = 0; k < kUnroll; k++) pe[k] = A.post[rlo[seg[k]] + (e[k] - rpre[seg[k]])];
-                // per lane, bit k of each mask: posting k is in range and, for R-S items,
-                // inside the source's prefix (valm); placed a new key (newm); probing on
-                // (pendm); found no room (failm)
-                uint32_t valm = 0, newm = 0, pendm = 0, failm = 0;
-                uint32_t h[kUnroll], prev[kUnroll];
+                // bit k: posting k is in range and, for R-S items, inside the source's prefix
+                uint32_t valm = 0, jpos[kUnroll];
 #pragma unroll
                 for (int k = 0; k < kUnroll; k++) {
                     uint32_t v = (e0 + (uint32_t)(k * kWave) + (uint32_t)lane) < total;
@@ -973,54 +1026,12 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                         v &= (uint32_t)((pe[k].y & 0xFFFFu) < (midprobe ? fgss_max_prefix(ls, t) : fgss_mid_prefix(ls, t)));
                     }
                     valm |= v << k;
-                    h[k] = slot_hash(pe[k].x, tlog);
+                    jpos[k] = jb + seg[k];
                 }
-#pragma unroll
-                for (int k = 0; k < kUnroll; k++)
-                    prev[k] = atomicCAS(((valm >> k) & 1u) ? &tkey[h[k]] : &scr[lane], kEmpty, pe[k].x);
-#pragma unroll
-                for (int k = 0; k < kUnroll; k++) {
-                    const uint32_t vk = (valm >> k) & 1u;
-                    newm |= (vk & (uint32_t)(prev[k] == kEmpty)) << k;
-                    pendm |= (vk & (uint32_t)(prev[k] != kEmpty) & (uint32_t)(prev[k] != pe[k].x)) << k;
+                if (table_insert(pe, valm, jpos, tkey, tcnt, tlast, clist, scr, lane, tlog, tsize, nlist)) {
+                    overflow = true;
+                    break;
                 }
-                // collisions: linear probing.  Every pending probe started in the pass
-                // above and advances one slot per round, so one wave-uniform round
-                // counter bounds them all by tsize slots.
-                for (uint32_t round = 1; __ballot(pendm != 0); round++) {
-                    if (round >= tsize) { failm |= pendm; break; }
-#pragma unroll
-                    for (int k = 0; k < kUnroll; k++) {
-                        const uint32_t pk = (pendm >> k) & 1u;
-                        h[k] = (h[k] + pk) & (tsize - 1u);
-                        prev[k] = atomicCAS(pk ? &tkey[h[k]] : &scr[lane], kEmpty, pe[k].x);
-                    }
-#pragma unroll
-                    for (int k = 0; k < kUnroll; k++) {
-                        const uint32_t pk = (pendm >> k) & 1u;
-                        const uint32_t placed = pk & (uint32_t)(prev[k] == kEmpty);
-                        newm |= placed << k;
-                        pendm &= ~((placed | (pk & (uint32_t)(prev[k] == pe[k].x))) << k);
-                    }
-                }
-                const uint32_t hitm = valm & ~failm;
-#pragma unroll
-                for (int k = 0; k < kUnroll; k++) {
-                    const bool hit = (hitm >> k) & 1u;
-                    atomicAdd(hit ? &tcnt[h[k]] : &scr[lane], 1u);
-                    atomicMax(hit ? &tlast[h[k]] : &scr[lane], ((jb + seg[k]) << 16) | (pe[k].y & 0xFFFFu));
-                }
-#pragma unroll
-                for (int k = 0; k < kUnroll; k++) {
-                    const bool isnew = (newm >> k) & 1u;
-                    const uint64_t nm = __ballot(isnew);
-                    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(nm >> 32),
-                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)nm, 0u));
-                    *(isnew ? &clist[nlist + below] : &scr[lane]) = h[k] | (pe[k].y & 0xFFFF0000u);
-                    nlist += (uint32_t)__popcll(nm);
-                }
-                if (__ballot(failm != 0)) overflow = true;
-                if (overflow) break;
             }
             wave_sync();
         }
@@ -1032,7 +1043,8 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             // (K doubling up to 64) in the next round.
             uint8_t *toks = (uint8_t *)qstage;          // active token ids (bytes)
             uint32_t *satm = qstage + kStreamQp / 4;    // kStreamQp bits: tokens with postings left
-            static_assert(kStreamQp / 4 + kStreamQp / 32 <= kQStage, "token list fits the query stage");
+            uint32_t *sscr = satm + kStreamQp / 32;     // a word per lane for updates without work
+            static_assert(kStreamQp / 4 + kStreamQp / 32 + kWave <= kQStage, "token list fits the query stage");
             uint32_t nact = 0;
             for (uint32_t j0 = 0; j0 < qp; j0 += kWave) {
                 const uint32_t j = j0 + lane;
@@ -1047,66 +1059,48 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                 if (lane < (int)(kStreamQp / 32)) satm[lane] = 0;
                 wave_sync();
                 const uint32_t K = 1u << logK, total = nact << logK;
-                for (uint32_t e0 = 0; e0 < total && !overflow; e0 += kWave * kUnroll) {
+                for (uint32_t e0 = 0; e0 < total; e0 += kWave * kUnroll) {
+                    // every lane reads a posting (lanes past the end the last group's;
+                    // reads past a token's list land in the next list or the spare
+                    // postings) together with its token's list end, and then tests it
                     uint2 pe[kUnroll];
-                    uint32_t jv[kUnroll];
-                    bool below[kUnroll];
+                    uint32_t jv[kUnroll], curj[kUnroll], idx[kUnroll], end[kUnroll];
+#pragma unroll
+                    for (int k = 0; k < kUnroll; k++) {
+                        const uint32_t ec = min(e0 + (uint32_t)(k * kWave) + lane, total - 1u);
+                        jv[k] = toks[ec >> logK];
+                        curj[k] = cur[jv[k]];
+                        idx[k] = curj[k] + (ec & (K - 1));
+                        pe[k] = A.post[idx[k]];
+                        end[k] = qrng[jv[k]].y;
+                    }
+                    uint32_t valm = 0;
 #pragma unroll
                     for (int k = 0; k < kUnroll; k++) {
                         const uint32_t e = e0 + (uint32_t)(k * kWave) + lane;
-                        below[k] = false;
-                        jv[k] = kEmpty;
-                        if (e < total) {
-                            const uint32_t j = toks[e >> logK];
-                            const uint32_t idx = cur[j] + (e & (K - 1));
-                            jv[k] = j;
-                            if (idx < qrng[j].y) {
-                                pe[k] = A.post[idx];
-                                below[k] = pe[k].x < wb;
-                            }
-                        }
-                    }
-#pragma unroll
-                    for (int k = 0; k < kUnroll; k++) {
+                        const bool below = e < total && idx[k] < end[k] && pe[k].x < wb;
                         // below is a prefix of each token's K lanes (postings ascend by
                         // record): its length is the count, taken by the group's first lane
-                        const uint64_t bm = __ballot(below[k]);
+                        const uint64_t bm = __ballot(below);
                         escan += (uint32_t)__popcll(bm);
-                        const uint32_t e = e0 + (uint32_t)(k * kWave) + lane;
                         const uint32_t gs = (uint32_t)lane & ~(K - 1);
                         const uint64_t gmask = K >= 64 ? ~0ull : ((1ull << K) - 1ull);
                         const uint32_t cnt = (uint32_t)__popcll((bm >> gs) & gmask);
-                        if (e < total && ((uint32_t)lane & (K - 1)) == 0 && cnt > 0) {
-                            cur[jv[k]] += cnt;
-                            if (cnt == K) atomicOr(&satm[jv[k] >> 5], 1u << (jv[k] & 31));
-                        }
-                        bool valid = below[k];
-                        if (RS && valid) {   // the source's own prefix at t
+                        // each token has one group per round, in one row k: the groups
+                        // of a row move different cursors
+                        const bool lead = e < total && ((uint32_t)lane & (K - 1)) == 0 && cnt > 0;
+                        *(lead ? &cur[jv[k]] : &sscr[lane]) = curj[k] + cnt;
+                        atomicOr(lead && cnt == K ? &satm[jv[k] >> 5] : &sscr[lane], 1u << (jv[k] & 31));
+                        uint32_t v = below;
+                        if (RS) {   // the source's own prefix at t
                             const uint32_t ls = pe[k].y >> 16;
-                            valid = (pe[k].y & 0xFFFFu) < (midprobe ? fgss_max_prefix(ls, t) : fgss_mid_prefix(ls, t));
+                            v &= (uint32_t)((pe[k].y & 0xFFFFu) < (midprobe ? fgss_max_prefix(ls, t) : fgss_mid_prefix(ls, t)));
                         }
-                        bool ok = true, isnew = false;
-                        uint32_t h = 0;
-                        if (valid) {
-                            const uint32_t s = pe[k].x;
-                            h = slot_hash(s, tlog);
-                            uint32_t probes = 0;
-                            for (;;) {
-                                const uint32_t prev = atomicCAS(&tkey[h], kEmpty, s);
-                                if (prev == kEmpty) { isnew = true; break; }
-                                if (prev == s) break;
-                                h = (h + 1) & (tsize - 1);
-                                if (++probes >= tsize) { ok = false; break; }
-                            }
-                            if (ok) {
-                                atomicAdd(&tcnt[h], 1u);
-                                atomicMax(&tlast[h], (jv[k] << 16) | (pe[k].y & 0xFFFFu));
-                            }
-                        }
-                        const uint64_t nm = __ballot(isnew);
-                        if (isnew) clist[nlist + (uint32_t)__popcll(nm & lanemask_lt())] = h | (pe[k].y & 0xFFFF0000u);
-                        nlist += (uint32_t)__popcll(nm);
-                        if (__ballot(!ok)) overflow = true;
+                        valm |= v << k;
+                    }
+                    if (table_insert(pe, valm, jv, tkey, tcnt, tlast, clist, sscr, lane, tlog, tsize, nlist)) {
+                        overflow = true;
+                        break;
                     }
                 }
                 wave_sync();
@@ -1144,31 +1138,33 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         c1 = clock64(); WACC(W_CYC0 + 1, c1 - c0); c0 = c1;
 
         // ---- B: positional filter over the distinct sources ----
+        // (branch-free like phase A: lanes past the list work on its last entry and
+        // aim their stores at a word of their own in the idle query stage)
         uint32_t ncand = 0, cand = 0;
         for (uint32_t base = 0; base < nlist; base += kWave) {
             const uint32_t k = base + lane;
-            bool keep = false;
-            uint32_t ent = 0, h = 0;
-            if (k < nlist) {
-                ent = clist[k];
-                h = ent & 0xFFFFu;
-                const uint32_t ls = ent >> 16;
-                const uint32_t m = tcnt[h];
-                const uint32_t last = tlast[h];
-                const float bound = fgss_min_overlap(lq, ls, t);
-                const uint32_t js = last >> 16, ss = last & 0xFFFFu;
-                const uint32_t rq = lq - js, rs = ls - ss;
-                const uint32_t minrem = rq < rs ? rq : rs;
-                if (!((float)(m + minrem) < bound)) {          // filtering.cu:127-133 at the last match
-                    cand++;
-                    // overlap can still reach floor(bound): m + min(|Q|-js-1, |S|-ss-1)
-                    keep = (int)(m + minrem - 1) >= (int)floorf(bound);
-                }
-                if (!keep) { tkey[h] = kEmpty; tcnt[h] = 0; tlast[h] = 0; }
-            }
+            const bool in = k < nlist;
+            const uint32_t ent = clist[min(k, nlist - 1u)];
+            const uint32_t h = ent & 0xFFFFu;
+            const uint32_t ls = ent >> 16;
+            const uint32_t m = tcnt[h];
+            const uint32_t last = tlast[h];
+            const float bound = fgss_min_overlap(lq, ls, t);
+            const uint32_t js = last >> 16, ss = last & 0xFFFFu;
+            const uint32_t rq = lq - js, rs = ls - ss;
+            const uint32_t minrem = rq < rs ? rq : rs;
+            const bool pass = in && !((float)(m + minrem) < bound);   // filtering.cu:127-133 at the last match
+            cand += pass;
+            // overlap can still reach floor(bound): m + min(|Q|-js-1, |S|-ss-1)
+            const bool keep = pass && (int)(m + minrem - 1) >= (int)floorf(bound);
+            const bool drop = in && !keep;
+            *(drop ? &tkey[h] : &qstage[lane]) = kEmpty;
+            *(drop ? &tcnt[h] : &qstage[lane]) = 0;
+            *(drop ? &tlast[h] : &qstage[lane]) = 0;
             const uint64_t bal = __ballot(keep);
             wave_sync();
-            if (keep) clist[ncand + (uint32_t)__popcll(bal & lanemask_lt())] = ent;
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            *(keep ? &clist[ncand + below] : &qstage[lane]) = ent;
             ncand += (uint32_t)__popcll(bal);
         }
         cand = wave_sum(cand);
@@ -1176,6 +1172,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         c1 = clock64(); WACC(W_CYC0 + 2, c1 - c0); c0 = c1;
 
         // ---- C: signature bound, one lane per candidate: 64 buckets, then 128 for the rest ----
+        // (the source's bad-signature flag and its 64-bucket line are read together)
         const uint32_t nsig = ncand;
         uint32_t nver = 0;
         {
@@ -1184,52 +1181,54 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             const bool qbad = A.qsigbad[q] != 0;
             for (uint32_t base = 0; base < nsig; base += kWave) {
                 const uint32_t c = base + lane;
-                uint32_t ent = 0, h = 0, s = 0, ls = 0;
-                int floorb = 0;
-                bool keep = false, exact = false;
-                if (c < nsig) {
-                    ent = clist[c];
-                    h = ent & 0xFFFFu;
-                    ls = ent >> 16;
-                    s = tkey[h];
-                    if (qbad || A.sigbad[s]) {
-                        keep = true;
-                    } else {
-                        floorb = (int)floorf(fgss_min_overlap(lq, ls, t));
-                        const uint4 *sh4 = (const uint4 *)(A.sig64 + (size_t)s * kSig64Words);
-                        uint32_t sad = 0;
+                const bool in = c < nsig;
+                const uint32_t ent = clist[min(c, nsig - 1u)];
+                const uint32_t h = ent & 0xFFFFu;
+                const uint32_t ls = ent >> 16;
+                const uint32_t s = tkey[h];
+                const uint8_t sbad = A.sigbad[s];
+                const uint4 *sh4 = (const uint4 *)(A.sig64 + (size_t)s * kSig64Words);
+                uint4 b4[kSig64Words / 4];
 #pragma unroll
-                        for (int k = 0; k < kSig64Words / 4; k++) {
-                            const uint4 a4 = qh4[k], b4 = sh4[k];
-                            sad = __builtin_amdgcn_sad_u8(a4.x, b4.x, sad);
-                            sad = __builtin_amdgcn_sad_u8(a4.y, b4.y, sad);
-                            sad = __builtin_amdgcn_sad_u8(a4.z, b4.z, sad);
-                            sad = __builtin_amdgcn_sad_u8(a4.w, b4.w, sad);
-                        }
-                        // sum over buckets of min(cQ, cS) = (|Q| + |S| - SAD) / 2 >= |Q n S|
-                        exact = ((int)(lq + ls) - (int)sad) / 2 >= floorb;
-                    }
+                for (int k = 0; k < kSig64Words / 4; k++) b4[k] = sh4[k];
+                const int floorb = (int)floorf(fgss_min_overlap(lq, ls, t));
+                uint32_t sad = 0;
+#pragma unroll
+                for (int k = 0; k < kSig64Words / 4; k++) {
+                    const uint4 a4 = qh4[k];
+                    sad = __builtin_amdgcn_sad_u8(a4.x, b4[k].x, sad);
+                    sad = __builtin_amdgcn_sad_u8(a4.y, b4[k].y, sad);
+                    sad = __builtin_amdgcn_sad_u8(a4.z, b4[k].z, sad);
+                    sad = __builtin_amdgcn_sad_u8(a4.w, b4[k].w, sad);
                 }
+                const bool bad = qbad || sbad != 0;
+                // sum over buckets of min(cQ, cS) = (|Q| + |S| - SAD) / 2 >= |Q n S|
+                const bool exact = in && !bad && ((int)(lq + ls) - (int)sad) / 2 >= floorb;
+                bool keep = in && bad;
                 if (__ballot(exact)) {
                     WACC(W_SIG2, __popcll(__ballot(exact)));
                     if (exact) {
                         const uint4 *ss4 = (const uint4 *)(A.sig + (size_t)s * kSigWords);
-                        uint32_t sad = 0;
+                        uint32_t sad2 = 0;
 #pragma unroll
                         for (int k = 0; k < kSigWords / 4; k++) {
-                            const uint4 a4 = qs4[k], b4 = ss4[k];
-                            sad = __builtin_amdgcn_sad_u8(a4.x, b4.x, sad);
-                            sad = __builtin_amdgcn_sad_u8(a4.y, b4.y, sad);
-                            sad = __builtin_amdgcn_sad_u8(a4.z, b4.z, sad);
-                            sad = __builtin_amdgcn_sad_u8(a4.w, b4.w, sad);
+                            const uint4 a4 = qs4[k], c4 = ss4[k];
+                            sad2 = __builtin_amdgcn_sad_u8(a4.x, c4.x, sad2);
+                            sad2 = __builtin_amdgcn_sad_u8(a4.y, c4.y, sad2);
+                            sad2 = __builtin_amdgcn_sad_u8(a4.z, c4.z, sad2);
+                            sad2 = __builtin_amdgcn_sad_u8(a4.w, c4.w, sad2);
                         }
-                        keep = ((int)(lq + ls) - (int)sad) / 2 >= floorb;
+                        keep = ((int)(lq + ls) - (int)sad2) / 2 >= floorb;
                     }
                 }
-                if (c < nsig && !keep) { tkey[h] = kEmpty; tcnt[h] = 0; tlast[h] = 0; }
+                const bool drop = in && !keep;
+                *(drop ? &tkey[h] : &qstage[lane]) = kEmpty;
+                *(drop ? &tcnt[h] : &qstage[lane]) = 0;
+                *(drop ? &tlast[h] : &qstage[lane]) = 0;
                 const uint64_t bal = __ballot(keep);
                 wave_sync();
-                if (keep) clist[nver + (uint32_t)__popcll(bal & lanemask_lt())] = ent;
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                *(keep ? &clist[nver + below] : &qstage[lane]) = ent;
                 nver += (uint32_t)__popcll(bal);
             }
             wave_sync();
@@ -1485,7 +1484,9 @@ static int prepare_buffers(fgss_device_sets *d)
     GROW(d->b_ia, (size_t)E * 4);
     GROW(d->b_ib, (size_t)E * 4);
     GROW(d->b_sid_of, (size_t)E * 4);
-    GROW(d->b_post, (size_t)(E + 1) * 8);
+    // kWave spare postings: a streamed round reads up to 63 postings past the end of a
+    // token's list before it tests them against the end
+    GROW(d->b_post, (size_t)(E + kWave) * 8);
     GROW(d->b_psid, (size_t)(E + 1) * 4);
     GROW(d->b_fence, (size_t)(E / kFence + 2) * 4);
     GROW(d->b_loff, ((size_t)d->n_terms + 2) * 4);
