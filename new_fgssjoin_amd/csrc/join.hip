@@ -111,6 +111,10 @@ constexpr uint32_t kStreamQp = 256;
 // short probes the planned parts spread better over the wavefronts)
 constexpr uint32_t kStreamParts = 0xFFFFFFFFu;
 constexpr uint32_t kStreamChunk = 32;            // parts per streamed item (a heavy query gets several)
+#ifndef FGSS_MAX_SPAN
+#define FGSS_MAX_SPAN 8
+#endif
+constexpr uint32_t kMaxSpan = FGSS_MAX_SPAN;     // parts per window of a streamed item at most
 
 // How the items of a query with np parts read their posting ranges.
 enum ItemMode : uint32_t { IM_NONE = 0, IM_WHOLE, IM_PLANNED, IM_STREAMED, IM_SEARCHED };
@@ -936,12 +940,18 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         const uint32_t nwin = streamed ? item.nparts : 1u;
         c1 = clock64(); WACC(W_CYC0 + 0, c1 - c0); c0 = c1;
 
-        for (uint32_t win = 0; win < nwin; win++) {
+        // a streamed item joins its parts a window of `span` consecutive parts at a
+        // time: the span doubles while the windows leave the table at most a quarter
+        // full and halves when one fills it more than half (fewer, fuller windows:
+        // fewer rounds and signature passes, posting lines read once)
+        uint32_t span = 1;
+        for (uint32_t p0 = 0, p1 = 0; p0 < nwin; p0 = p1) {
+        p1 = min(p0 + span, nwin);
         uint32_t wa = item.a, wb = item.b;
         if (streamed) {
             const uint64_t W = (uint64_t)item.b - item.a;
-            wa = part_start(item.a, W, win, nwin);
-            wb = part_start(item.a, W, win + 1, nwin);
+            wa = part_start(item.a, W, p0, nwin);
+            wb = part_start(item.a, W, p1, nwin);
         }
 
         // ---- A: candidate generation (filtering.cu:94-139) ----
@@ -1121,11 +1131,15 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             // a streamed item still moves the cursors of the tokens it did not reach
             if (streamed)
                 for (uint32_t j = jdone + lane; j < qp; j += kWave) cur[j] = gallop_post(A.post, cur[j], qrng[j].y, wb);
-            if (lane == 0) {
-                uint32_t k = (uint32_t)atomicAdd(&A.ctr[C_OVERFLOW], 1ull);
+            // one record per part of the window (a whole item: one record)
+            uint32_t k0 = 0;
+            if (lane == 0) k0 = (uint32_t)atomicAdd(&A.ctr[C_OVERFLOW], (unsigned long long)(p1 - p0));
+            k0 = __builtin_amdgcn_readfirstlane(k0);
+            for (uint32_t p = p0 + lane; p < p1; p += kWave) {
+                const uint32_t k = k0 + (p - p0);
                 if (k < A.ovf_cap) {
                     A.ovf[k] = it;
-                    A.ovfp[k] = streamed ? win : kEmpty;
+                    A.ovfp[k] = streamed ? p : kEmpty;
                 }
             }
             for (uint32_t k = lane; k < nlist; k += kWave) {
@@ -1133,6 +1147,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                 tkey[h] = kEmpty; tcnt[h] = 0; tlast[h] = 0;
             }
             wave_sync();
+            span = 1;
             continue;
         }
         c1 = clock64(); WACC(W_CYC0 + 1, c1 - c0); c0 = c1;
@@ -1305,6 +1320,8 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         WACC(W_VER, nver);
         WACC(W_INTER, inter);
         WACC(W_VTOK, vtok);
+        if (nlist * 4 <= tsize) span = min(span * 2, kMaxSpan);
+        else if (nlist * 2 > tsize) span = max(span / 2, 1u);
         }   // windows
     }
     // this wavefront's totals and its last job's pending counters to its own slot:
@@ -1423,6 +1440,9 @@ static uint64_t misc_nb(const fgss_device_sets *d)
     return (uint64_t)d->h_misc[M_NB64] | ((uint64_t)d->h_misc[M_NB64 + 1] << 32);
 }
 
+#ifndef FGSS_LOAD_MAX
+#define FGSS_LOAD_MAX 100
+#endif
 struct RunCfg {
     int cap = 512;          // LDS table slots per wave
     uint32_t target = 332;  // postings per work item
@@ -1434,7 +1454,7 @@ static RunCfg make_cfg(const fgss_join_opts *opts)
 {
     RunCfg c;
     c.cap = opts && opts->table_slots ? opts->table_slots : 512;
-    const int load = opts && opts->load_pct > 0 ? std::min(opts->load_pct, 100) : 65;
+    const int load = opts && opts->load_pct > 0 ? std::min(opts->load_pct, FGSS_LOAD_MAX) : 65;
     c.target = std::max<uint32_t>(1u, (uint32_t)(c.cap * load / 100));
     if (opts && opts->bounds_cap > 0) c.bounds = std::min<uint32_t>((uint32_t)opts->bounds_cap, c.bounds);
     if (opts && opts->stream_parts > 0) c.stream_parts = (uint32_t)opts->stream_parts;
