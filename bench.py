@@ -13,6 +13,7 @@ join kernel and the CPU oracle baseline on this host.
 from __future__ import annotations
 
 import argparse
+import itertools
 import json
 import os
 import sys
@@ -240,14 +241,22 @@ def _percentile(xs, p):
     return float(np.percentile(np.asarray(xs, np.float64), p)) if len(xs) else 0.0
 
 
-def served_leg(fg, n_index: int, n_requests: int, slots: int, check_tokenizer: bool = True):
+def served_leg(fg, n_index: int, n_requests: int, slots: int, check_tokenizer: bool = True, server_kw=None):
     """R-S serving against a resident index of n_index titles (seed 5, q=3, t_min=0.7):
     requests of 1..4096 probes (log-uniform; 20% edited copies of indexed titles, the
     rest fresh titles), t ~ U[0.7, 0.95].  Plain = one request at a time (submit, wait,
     take the pairs); served = continuous batching of a Poisson stream, at a rate far
     above capacity (throughput) and at half the measured capacity (latency)."""
+    idx, reqs, ts, tok = served_workload(fg, n_index, n_requests, check_tokenizer)
+    out = served_measure(fg, idx, reqs, ts, slots, server_kw or {})
+    out["tokenize_index"] = tok
+    idx.close()
+    return out
+
+
+def served_workload(fg, n_index: int, n_requests: int, check_tokenizer: bool = True):
+    """The resident index and the tokenized requests of the served leg."""
     import numpy as np
-    t_min = float(np.float32(0.7))
     t0 = time.perf_counter()
     text = fg.generate_text(fg.GEN_TITLES, n_index, 5)
     t1 = time.perf_counter()
@@ -286,9 +295,17 @@ def served_leg(fg, n_index: int, n_requests: int, slots: int, check_tokenizer: b
         reqs.append(fg.Collection.from_text_dict(b"\n".join(out) + b"\n", idx))
     del lines, fresh
     log(f"served: {n_requests} requests, {int(sizes.sum())} probes tokenized in {time.perf_counter() - t0:.1f} s")
+    return idx, reqs, ts, tok
 
-    # plain: one request at a time
-    srv = fg.Server(idx, t_min, slots=1)
+
+def served_measure(fg, idx, reqs, ts, slots: int, server_kw: dict):
+    """Plain (one request at a time) against served (continuous batching) on the same
+    requests; every served request's pairs must equal its plain run's."""
+    import numpy as np
+    t_min = float(np.float32(0.7))
+    n_requests = len(reqs)
+    sizes = np.array([r.info.n_sets for r in reqs])
+    srv = fg.Server(idx, t_min, slots=1, **server_kw)
     for k in range(min(8, n_requests)):   # warm the slot's workspace
         srv.result(srv.submit(reqs[k], ts[k]))
     plain, plain_lat = [], []
@@ -303,7 +320,7 @@ def served_leg(fg, n_index: int, n_requests: int, slots: int, check_tokenizer: b
 
     def drive(rate):
         """Open-loop Poisson arrivals at `rate` req/s; returns (makespan s, latencies s, pairs)."""
-        srv = fg.Server(idx, t_min, slots=slots)
+        srv = fg.Server(idx, t_min, slots=slots, **server_kw)
         for k in range(min(8, n_requests)):
             srv.result(srv.submit(reqs[k], ts[k]))
         gaps = np.random.default_rng(7).exponential(1.0 / rate, n_requests)
@@ -318,8 +335,11 @@ def served_leg(fg, n_index: int, n_requests: int, slots: int, check_tokenizer: b
             while i < n_requests and arrive[i] <= now:
                 live[srv.submit(reqs[i], ts[i])] = i
                 i += 1
-            srv.poll()
-            finished = [rid for rid in live if srv.done(rid)]
+            # with every request submitted, wait for the oldest run instead of spinning
+            srv.poll(block=i >= n_requests)
+            # runs are admitted first come first served, so requests finish about in
+            # arrival order: look at the oldest live ones only
+            finished = [rid for rid in itertools.islice(live, 64) if srv.done(rid)]
             for rid in finished:
                 k = live.pop(rid)
                 out[k], st = srv.result(rid, wait=False)
@@ -342,9 +362,8 @@ def served_leg(fg, n_index: int, n_requests: int, slots: int, check_tokenizer: b
         for g, p in zip(outs, plain):
             exact &= (np.array_equal(g.q, p.q) and np.array_equal(g.s, p.s) and np.array_equal(g.overlap, p.overlap)
                       and g.stats["candidates"] == p.stats["candidates"])
-    idx.close()
     ms = lambda xs, q: round(_percentile(xs, q) * 1e3, 3)
-    return {"index_records": n_index, "requests": n_requests, "probes": int(sizes.sum()), "slots": slots,
+    return {"index_records": int(idx.info.n_sets), "requests": n_requests, "probes": int(sizes.sum()), "slots": slots,
             "t_min": t_min, "pairs": int(sum(len(p) for p in plain)),
             "candidates": int(sum(p.stats["candidates"] for p in plain)),
             "plain_req_per_s": n_requests / plain_s, "plain_p50_ms": ms(plain_lat, 50),
@@ -355,7 +374,7 @@ def served_leg(fg, n_index: int, n_requests: int, slots: int, check_tokenizer: b
             "poisson_rate": 0.5 * cap, "poisson_req_per_s": n_requests / lat_s,
             "poisson_p50_ms": ms(lat, 50), "poisson_p99_ms": ms(lat, 99),
             "poisson_mean_batch": float(np.mean(lat_shared)) if lat_shared else 0.0,
-            "bit_exact_vs_plain": bool(exact), "tokenize_index": tok}
+            "bit_exact_vs_plain": bool(exact)}
 
 
 def main():
@@ -369,7 +388,7 @@ def main():
     ap.add_argument("--batch-jobs", type=int, default=512, help="jobs of the batched leg (0 skips it)")
     ap.add_argument("--serve-index", type=int, default=2_000_000, help="indexed titles of the served leg")
     ap.add_argument("--serve-requests", type=int, default=1000, help="requests of the served leg (0 skips it)")
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--large-records", type=int, default=1_000_000, help="records of the large leg (0 skips it)")
     ap.add_argument("--large-sample", type=int, default=1000, help="oracle-checked queries of the large leg")
     args = ap.parse_args()

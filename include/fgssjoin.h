@@ -126,6 +126,9 @@ typedef struct {
                                   candidate generation, positional filter, signature bound,
                                   token verification, and the part of candidate generation
                                   spent locating posting ranges                        */
+    int64_t window_retries;    /* streamed windows of several parts whose table overflowed and that
+                                  ran again part by part (streamed items join a growing number of
+                                  consecutive parts at a time while the table stays sparse)  */
 } fgss_join_stats;
 
 typedef struct {
@@ -220,8 +223,10 @@ typedef struct {
  * once for every threshold >= t_min: each record's first max_prefix(|s|, t_min)
  * tokens.  The server keeps a private device copy; the caller keeps `indexed`
  * alive while results are read (record ids come from it).  `slots` runs may be in
- * flight at once, each on its own HIP stream; `budget_bytes` (<= 0: 1 GiB) caps
- * the candidate-memory estimate of the requests in flight. */
+ * flight at once, each on its own HIP stream; `budget_bytes` (<= 0: 16 GiB) caps
+ * the work admitted in flight, estimated at 16 B per posting a request's probe
+ * prefixes may reach (prefix tokens x the index's mean list length).  Larger
+ * budgets batch more requests per run: more throughput, longer queueing. */
 int fgss_server_create(const fgss_collection *indexed, float t_min, int32_t slots, int64_t budget_bytes,
                        const fgss_join_opts *opts, fgss_server **out);
 /* Queue an R-S join of `probes` (a plain collection in the index's token space,

@@ -85,6 +85,7 @@ enum Ctr {
     C_CYC_C,
     C_CYC_D,
     C_CYC_RANGES,    // part of C_CYC_A spent locating posting ranges
+    C_RETRY,         // streamed windows of several parts that overflowed and ran again part by part
     C_COUNT
 };
 
@@ -110,11 +111,22 @@ constexpr uint32_t kStreamQp = 256;
 // default: only queries whose boundary rows do not fit the table are streamed (for
 // short probes the planned parts spread better over the wavefronts)
 constexpr uint32_t kStreamParts = 0xFFFFFFFFu;
-constexpr uint32_t kStreamChunk = 32;            // parts per streamed item (a heavy query gets several)
+#ifndef FGSS_STREAM_CHUNK
+#define FGSS_STREAM_CHUNK 64
+#endif
+constexpr uint32_t kStreamChunk = FGSS_STREAM_CHUNK;   // parts per streamed item (a heavy query gets several)
 #ifndef FGSS_MAX_SPAN
-#define FGSS_MAX_SPAN 8
+#define FGSS_MAX_SPAN 16
 #endif
 constexpr uint32_t kMaxSpan = FGSS_MAX_SPAN;     // parts per window of a streamed item at most
+#ifndef FGSS_SPAN_GROW_PCT
+#define FGSS_SPAN_GROW_PCT 40
+#endif
+#ifndef FGSS_SPAN_SHRINK_PCT
+#define FGSS_SPAN_SHRINK_PCT 70
+#endif
+// table fill (% of its slots) after which a streamed item's window span doubles / halves
+constexpr uint32_t kSpanGrowPct = FGSS_SPAN_GROW_PCT, kSpanShrinkPct = FGSS_SPAN_SHRINK_PCT;
 
 // How the items of a query with np parts read their posting ranges.
 enum ItemMode : uint32_t { IM_NONE = 0, IM_WHOLE, IM_PLANNED, IM_STREAMED, IM_SEARCHED };
@@ -820,7 +832,7 @@ __device__ __forceinline__ bool table_insert(const uint2 (&pe)[N], uint32_t valm
 constexpr int kUnroll = FGSS_UNROLL;
 constexpr int kQStage = 160;   // query tokens staged in LDS for verification (longer queries stay in HBM)
 // per-wave counters kept in LDS (not registers): totals, shader-clock phases, current job
-enum WaveAcc { W_ESCAN = 0, W_CAND, W_SIG, W_VER, W_SIG2, W_INTER, W_VTOK, W_CYC0, W_JOB0 = W_CYC0 + 6,
+enum WaveAcc { W_ESCAN = 0, W_CAND, W_SIG, W_VER, W_SIG2, W_INTER, W_VTOK, W_RETRY, W_CYC0, W_JOB0 = W_CYC0 + 6,
                kAccSlots = W_JOB0 + kJobCtr };
 // per-wavefront slot in global memory at the end of a launch: the kAccSlots
 // counters and the wavefront's last job
@@ -941,9 +953,10 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         c1 = clock64(); WACC(W_CYC0 + 0, c1 - c0); c0 = c1;
 
         // a streamed item joins its parts a window of `span` consecutive parts at a
-        // time: the span doubles while the windows leave the table at most a quarter
-        // full and halves when one fills it more than half (fewer, fuller windows:
-        // fewer rounds and signature passes, posting lines read once)
+        // time: the span doubles while the windows leave the table at most 40% full
+        // and halves when one fills it more than 70% (fewer, fuller windows: fewer
+        // rounds and signature passes, posting lines read once); a window of several
+        // parts that overflows runs again part by part
         uint32_t span = 1;
         for (uint32_t p0 = 0, p1 = 0; p0 < nwin; p0 = p1) {
         p1 = min(p0 + span, nwin);
@@ -1128,26 +1141,36 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             }
         }
         if (overflow) {
-            // a streamed item still moves the cursors of the tokens it did not reach
-            if (streamed)
-                for (uint32_t j = jdone + lane; j < qp; j += kWave) cur[j] = gallop_post(A.post, cur[j], qrng[j].y, wb);
-            // one record per part of the window (a whole item: one record)
-            uint32_t k0 = 0;
-            if (lane == 0) k0 = (uint32_t)atomicAdd(&A.ctr[C_OVERFLOW], (unsigned long long)(p1 - p0));
-            k0 = __builtin_amdgcn_readfirstlane(k0);
-            for (uint32_t p = p0 + lane; p < p1; p += kWave) {
-                const uint32_t k = k0 + (p - p0);
-                if (k < A.ovf_cap) {
-                    A.ovf[k] = it;
-                    A.ovfp[k] = streamed ? p : kEmpty;
-                }
-            }
             for (uint32_t k = lane; k < nlist; k += kWave) {
                 const uint32_t h = clist[k] & 0xFFFFu;
                 tkey[h] = kEmpty; tcnt[h] = 0; tlast[h] = 0;
             }
             wave_sync();
             span = 1;
+            if (streamed && p1 - p0 > 1) {
+                // a window of several parts runs again part by part: cursors back to
+                // the first posting of each token at or after the window's start
+                WACC(W_RETRY, 1);
+                for (uint32_t j = lane; j < qp; j += kWave) {
+                    const uint2 r = qrng[j];
+                    cur[j] = lower_bound_sid(A.psid, A.fence, r.x, r.y, wa);
+                }
+                wave_sync();
+                p1 = p0;
+                continue;
+            }
+            // a streamed item still moves the cursors of the tokens it did not reach
+            if (streamed) {
+                for (uint32_t j = jdone + lane; j < qp; j += kWave) cur[j] = gallop_post(A.post, cur[j], qrng[j].y, wb);
+                wave_sync();
+            }
+            if (lane == 0) {
+                uint32_t k = (uint32_t)atomicAdd(&A.ctr[C_OVERFLOW], 1ull);
+                if (k < A.ovf_cap) {
+                    A.ovf[k] = it;
+                    A.ovfp[k] = streamed ? p0 : kEmpty;
+                }
+            }
             continue;
         }
         c1 = clock64(); WACC(W_CYC0 + 1, c1 - c0); c0 = c1;
@@ -1320,8 +1343,8 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         WACC(W_VER, nver);
         WACC(W_INTER, inter);
         WACC(W_VTOK, vtok);
-        if (nlist * 4 <= tsize) span = min(span * 2, kMaxSpan);
-        else if (nlist * 2 > tsize) span = max(span / 2, 1u);
+        if (nlist * 100 <= tsize * kSpanGrowPct) span = min(span * 2, kMaxSpan);
+        else if (nlist * 100 > tsize * kSpanShrinkPct) span = max(span / 2, 1u);
         }   // windows
     }
     // this wavefront's totals and its last job's pending counters to its own slot:
@@ -1342,7 +1365,7 @@ __global__ __launch_bounds__(256) void k_wave_reduce(const unsigned long long *_
                                                      unsigned long long *__restrict__ ctr_job)
 {
     constexpr int kMain = W_JOB0;   // global counters per slot
-    const int dst[kMain] = {C_ESCAN, C_CAND, C_SIGCHK, C_VERIFIED, C_SIG2, C_INTER, C_VTOK,
+    const int dst[kMain] = {C_ESCAN, C_CAND, C_SIGCHK, C_VERIFIED, C_SIG2, C_INTER, C_VTOK, C_RETRY,
                             C_CYC_SETUP, C_CYC_SETUP + 1, C_CYC_SETUP + 2, C_CYC_SETUP + 3, C_CYC_SETUP + 4,
                             C_CYC_RANGES};
     const uint32_t wv = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1767,7 +1790,7 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
         if (hipEventElapsedTime(&ms, d->ev[1], d->ev[5]) == hipSuccess) T.ms_join = ms;
         if (hipEventElapsedTime(&ms, d->ev[4], d->ev[5]) == hipSuccess) T.ms_kernel = ms;
     }
-    for (int k = C_ESCAN; k <= C_CYC_RANGES; k++) T.acc[k] += d->h_ctr[k];
+    for (int k = C_ESCAN; k <= C_RETRY; k++) T.acc[k] += d->h_ctr[k];
     uint64_t nres = d->h_ctr[C_RESULTS], novf = d->h_ctr[C_OVERFLOW];
     tri.resize(3 * (size_t)nres);
     if (nres) HIPCK(hipMemcpy(tri.data(), d->b_out.p, (size_t)nres * 12, hipMemcpyDeviceToHost));
@@ -1821,7 +1844,7 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
             if (rc) return rc;
             d->cap_gen++;
         }
-        for (int k = C_ESCAN; k <= C_CYC_RANGES; k++) T.acc[k] += d->h_ctr[k];
+        for (int k = C_ESCAN; k <= C_RETRY; k++) T.acc[k] += d->h_ctr[k];
         nres = d->h_ctr[C_RESULTS];
         novf = d->h_ctr[C_OVERFLOW];
         const size_t old = tri.size();
@@ -1927,6 +1950,7 @@ static void fill_stats(const RunTotals &T, size_t job, fgss_join_stats &st)
     st.signature_checks_128 = (int64_t)T.acc[C_SIG2];
     st.verified = (int64_t)T.acc[C_VERIFIED];
     for (int k = 0; k < 6; k++) st.phase_cycles[k] = (int64_t)T.acc[C_CYC_SETUP + k];
+    st.window_retries = (int64_t)T.acc[C_RETRY];
     st.ms_index = T.ms_index;
     st.ms_join = T.ms_join;
     st.ms_kernel = T.ms_kernel;
@@ -2324,7 +2348,7 @@ struct ServeRequest {
     int64_t id = 0;
     const fgss_collection *probes = nullptr;
     float t = 0.f;
-    int64_t est = 0;                    // candidate-memory estimate (bytes)
+    int64_t est = 0;                    // work estimate (16 B per posting the probe prefixes may reach)
     bool done = false;
     int status = FGSS_OK;
     std::vector<int32_t> tri;           // (probe, source, overlap), probe indices local to the request
@@ -2610,7 +2634,7 @@ static int slot_retire(fgss_server *srv, ServeSlot &sl)
 }
 
 // Retire completed slots, then admit pending requests into idle slots, first come
-// first served, as many per slot as the candidate-memory budget and the per-slot
+// first served, as many per slot as the work budget and the per-slot
 // probe limit allow (at least one when nothing is in flight, so every request is
 // eventually admitted).  With block set, waits for the oldest busy slot first.
 static int server_progress(fgss_server *srv, bool block)
@@ -2674,7 +2698,7 @@ extern "C" int fgss_server_create(const fgss_collection *indexed, float t_min, i
     auto *srv = new fgss_server();
     srv->coll = indexed;
     srv->t_min = t_min;
-    srv->budget = budget_bytes > 0 ? budget_bytes : (int64_t)1 << 30;
+    srv->budget = budget_bytes > 0 ? budget_bytes : (int64_t)16 << 30;
     srv->cfg = make_cfg(opts);
     int rc = upload_records(&srv->idx, indexed);
     if (rc) { delete srv; return rc; }
@@ -2738,7 +2762,8 @@ extern "C" int fgss_server_submit(fgss_server *srv, const fgss_collection *probe
     r->id = srv->next_id++;
     r->probes = probes;
     r->t = t;
-    // candidate-memory estimate: the postings the probe prefixes may scan, at 16 B each
+    // work estimate: the postings the probe prefixes may reach (prefix tokens x the
+    // index's mean list length, before the size filter), at 16 B each
     double ptok = 0;
     for (int32_t l : probes->len) ptok += (double)std::min<uint32_t>(fgss_max_prefix((uint32_t)l, t), (uint32_t)l);
     r->est = (int64_t)(16.0 * ptok * srv->mean_list) + 4096;

@@ -231,9 +231,21 @@ def test_titles_100k_streamed(fg, oracle_mod):
     assert_join_equal(oracle_mod, sets, coll.selfjoin(0.6, stream_parts=2), 0.6)
 
 
+def test_titles_100k_streamed_window_retries(fg, oracle_mod):
+    """Streamed items join several consecutive parts per window while the table stays
+    sparse; with 256-slot tables some multi-part windows overflow and run again part
+    by part (counted in window_retries).  Results must not change."""
+    _say("titles n=100000 seed=2 t=0.6 slots=256 stream_parts=1")
+    coll = fg.Collection.generate(fg.GEN_TITLES, 100000, 2, device=True)
+    sets = oracle_sets(oracle_mod, coll)
+    p = coll.selfjoin(0.6, table_slots=256, stream_parts=1)
+    assert p.stats["window_retries"] > 0
+    assert_join_equal(oracle_mod, sets, p, 0.6)
+
+
 @pytest.mark.parametrize("load", [3, 20])
 def test_streamed_chunks(fg, oracle_mod, load):
-    """A few postings per part: heavy queries get many parts, streamed 16 parts per
+    """A few postings per part: heavy queries get many parts, streamed 64 parts per
     item, each item positioning its cursors at its own window start."""
     coll = fg.Collection.generate(fg.GEN_TITLES, 4096, 7)
     sets = oracle_sets(oracle_mod, coll)

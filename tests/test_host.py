@@ -22,6 +22,31 @@ def test_library_exports_every_declared_symbol(fg):
     assert declared == set(fg.EXPORTS)
 
 
+def test_ctypes_structs_match_the_header(fg, tmp_path):
+    """The Python mirror's structures have the C header's sizes and field offsets
+    (compiled with gcc against include/fgssjoin.h)."""
+    import ctypes as C
+    pairs = [("fgss_collection_info", fg.CollectionInfo), ("fgss_join_stats", fg.JoinStats),
+             ("fgss_join_opts", fg.JoinOpts), ("fgss_batch_stats", fg.BatchStats),
+             ("fgss_request_stats", fg.RequestStats), ("fgss_pairs", fg._Pairs)]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fgssjoin.h"', "int main(void) {"]
+    for cname, py in pairs:
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["return 0; }"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(ln.split() for ln in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                   check=True).stdout.splitlines())
+    for cname, py in pairs:
+        assert int(got[cname]) == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, f"{cname}.{f}"
+
+
 def test_text_front_end_matches_fixture(fg, tmp_path):
     coll = fg.Collection.from_text(golden_path("small.txt"), 3)
     out, loc = tmp_path / "s.bin", tmp_path / "s.bin.loc"
