@@ -318,9 +318,9 @@ def served_measure(fg, idx, reqs, ts, slots: int, server_kw: dict):
     srv.close()
     log(f"served: plain sequential {n_requests} requests in {plain_s * 1e3:.1f} ms")
 
-    def drive(rate):
+    def drive(rate, n_slots=slots, budget=0):
         """Open-loop Poisson arrivals at `rate` req/s; returns (makespan s, latencies s, pairs)."""
-        srv = fg.Server(idx, t_min, slots=slots, **server_kw)
+        srv = fg.Server(idx, t_min, slots=n_slots, budget_bytes=budget, **server_kw)
         for k in range(min(8, n_requests)):
             srv.result(srv.submit(reqs[k], ts[k]))
         gaps = np.random.default_rng(7).exponential(1.0 / rate, n_requests)
@@ -357,8 +357,17 @@ def served_measure(fg, idx, reqs, ts, slots: int, server_kw: dict):
     lat_s, lat, lat_out, lat_shared = drive(rate=0.5 * cap)
     log(f"served: Poisson at {0.5 * cap:.0f} req/s, p50 {_percentile(lat, 50) * 1e3:.2f} ms, "
         f"p99 {_percentile(lat, 99) * 1e3:.2f} ms")
+    # equal load: plain (one request per run, first come first served: a work budget
+    # of one byte admits a request only when nothing is in flight) and served at the
+    # same Poisson rate, half the plain capacity
+    eq_rate = 0.5 * n_requests / plain_s
+    _, eq_plain_lat, eq_plain_out, _ = drive(rate=eq_rate, n_slots=1, budget=1)
+    _, eq_lat, eq_out, _ = drive(rate=eq_rate)
+    log(f"served: equal load {eq_rate:.0f} req/s: plain p50/p99 {_percentile(eq_plain_lat, 50) * 1e3:.2f}/"
+        f"{_percentile(eq_plain_lat, 99) * 1e3:.2f} ms, served {_percentile(eq_lat, 50) * 1e3:.2f}/"
+        f"{_percentile(eq_lat, 99) * 1e3:.2f} ms")
     exact = True
-    for outs in (sat_out, lat_out):
+    for outs in (sat_out, lat_out, eq_plain_out, eq_out):
         for g, p in zip(outs, plain):
             exact &= (np.array_equal(g.q, p.q) and np.array_equal(g.s, p.s) and np.array_equal(g.overlap, p.overlap)
                       and g.stats["candidates"] == p.stats["candidates"])
@@ -374,6 +383,9 @@ def served_measure(fg, idx, reqs, ts, slots: int, server_kw: dict):
             "poisson_rate": 0.5 * cap, "poisson_req_per_s": n_requests / lat_s,
             "poisson_p50_ms": ms(lat, 50), "poisson_p99_ms": ms(lat, 99),
             "poisson_mean_batch": float(np.mean(lat_shared)) if lat_shared else 0.0,
+            "equal_load_rate": eq_rate, "equal_load_plain_p50_ms": ms(eq_plain_lat, 50),
+            "equal_load_plain_p99_ms": ms(eq_plain_lat, 99), "equal_load_served_p50_ms": ms(eq_lat, 50),
+            "equal_load_served_p99_ms": ms(eq_lat, 99),
             "bit_exact_vs_plain": bool(exact)}
 
 

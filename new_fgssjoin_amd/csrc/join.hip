@@ -175,10 +175,12 @@ static int ensure(DevBuf &b, size_t bytes)
 {
     if (bytes == 0) bytes = 16;
     if (b.cap >= bytes) return FGSS_OK;
+    // a buffer that grows again grows by half at least: hipFree waits for the whole
+    // device, so a workspace that creeps upward (a server's slots) must reallocate rarely
+    size_t want = std::max(bytes + bytes / 8, b.cap + b.cap / 2);
     if (b.p) HIPCK(hipFree(b.p));
     b.p = nullptr;
     b.cap = 0;
-    size_t want = bytes + bytes / 8;
     hipError_t e = hipMalloc(&b.p, want);
     if (e != hipSuccess) return fgss_fail(FGSS_ERR_NOMEM, "hipMalloc(%zu): %s", want, hipGetErrorString(e));
     b.cap = want;
@@ -2344,6 +2346,8 @@ __global__ __launch_bounds__(256) void k_plan_rs(const uint32_t *__restrict__ pt
 }
 
 // host-side request and server state
+constexpr uint32_t kReserveJobs = 4096;   // requests per run a slot is sized for at creation
+
 struct ServeRequest {
     int64_t id = 0;
     const fgss_collection *probes = nullptr;
@@ -2396,8 +2400,10 @@ static int slot_prepare(ServeSlot &sl)
     fgss_device_sets *d = sl.d;
     // two half-probes per probe; their prefixes hold at most 2 x the probe tokens
     const int64_t v = 2 * std::max<int64_t>(d->n, 1), E = 2 * std::max<int64_t>(d->ntok, 1);
-    if (d->cap_items == 0) d->cap_items = (uint32_t)(v + 4096);
-    if (d->cap_bounds == 0) d->cap_bounds = (uint32_t)std::min<int64_t>(E + 4096, kBoundsBudget);
+    // capacities follow the run's size (items >= half-probes); data-dependent growth
+    // beyond that is found after the run, which then runs again
+    d->cap_items = std::max<uint32_t>(d->cap_items, (uint32_t)std::min<int64_t>(v + 4096, UINT32_MAX / 2));
+    d->cap_bounds = std::max<uint32_t>(d->cap_bounds, (uint32_t)std::min<int64_t>(E + 4096, kBoundsBudget));
     if (d->cap_out == 0) d->cap_out = 65536;
     GROW(d->b_probelen, (size_t)v * 4);
     GROW(d->b_poff, (size_t)(v + 1) * 4);
@@ -2525,7 +2531,7 @@ static int slot_load(ServeSlot &sl)
     if (sl.jctr_cap < (size_t)std::max<uint32_t>(nj, 1)) {
         if (d->h_jctr) HIPCK(hipHostFree(d->h_jctr));
         d->h_jctr = nullptr;
-        sl.jctr_cap = std::max<size_t>((size_t)nj, 64);
+        sl.jctr_cap = std::max<size_t>({(size_t)nj, 64, sl.jctr_cap * 2});
         HIPCK(hipHostMalloc((void **)&d->h_jctr, sl.jctr_cap * kJobCtr * 8, hipHostMallocDefault));
     }
     hipStream_t st = d->st;
@@ -2544,6 +2550,37 @@ static int slot_load(ServeSlot &sl)
         HIPCK(hipGetLastError());
     }
     return FGSS_OK;
+}
+
+// Size a slot, when the server is created, for runs of up to n probes holding ntok
+// tokens and `jobs` requests, so that such runs never reallocate: a reallocation
+// waits for the whole device (stalling the runs of the other slots) and the run that
+// found a buffer too small is enqueued again.
+static int slot_reserve(ServeSlot &sl, int64_t n, int64_t ntok, uint32_t jobs)
+{
+    fgss_device_sets *d = sl.d;
+    GROW(sl.b_tok, (size_t)ntok * 4);
+    GROW(sl.b_len, (size_t)n * 4);
+    GROW(sl.b_pos, (size_t)n * 4);
+    GROW(sl.b_sig, (size_t)n * 4 * kSigWords);
+    GROW(sl.b_sig64, (size_t)n * 4 * kSig64Words);
+    GROW(sl.b_sigbad, (size_t)n);
+    GROW(sl.b_job, (size_t)n * 4);
+    GROW(sl.b_jend, (size_t)jobs * 4);
+    GROW(sl.b_tj, (size_t)jobs * 4);
+    if (sl.jctr_cap < jobs) {
+        if (d->h_jctr) HIPCK(hipHostFree(d->h_jctr));
+        d->h_jctr = nullptr;
+        sl.jctr_cap = jobs;
+        HIPCK(hipHostMalloc((void **)&d->h_jctr, sl.jctr_cap * kJobCtr * 8, hipHostMallocDefault));
+    }
+    const int64_t n0 = d->n, ntok0 = d->ntok;
+    const uint32_t nj0 = d->njobs;
+    d->n = n; d->ntok = ntok; d->njobs = jobs;
+    d->cap_out = std::max<uint32_t>(d->cap_out, 1u << 20);
+    const int rc = slot_prepare(sl);
+    d->n = n0; d->ntok = ntok0; d->njobs = nj0;
+    return rc;
 }
 
 // Merge the slot's requests into one probe set (request k = job k, its own
@@ -2737,6 +2774,13 @@ extern "C" int fgss_server_create(const fgss_collection *indexed, float t_min, i
         if (!ok) {
             fgss_server_free(srv);
             return fgss_fail(FGSS_ERR_DEVICE, "server slot set-up failed");
+        }
+        // workspace for runs of up to max_probes probes of the index's mean length
+        // (at most 2^24 tokens) and kReserveJobs requests
+        if (!srv->empty) {
+            const int64_t mean_len = (x->ntok + x->n - 1) / x->n;
+            rc = slot_reserve(sl, srv->max_probes, std::min<int64_t>(srv->max_probes * mean_len, 1ll << 24), kReserveJobs);
+            if (rc) { fgss_server_free(srv); return rc; }
         }
     }
     *out = srv;
