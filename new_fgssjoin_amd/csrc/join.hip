@@ -510,9 +510,13 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *__restrict__
     return lo;
 }
 
-// Waves loop over queries: the source window [q+1, hi) and, for each probe-prefix
-// token, the range of its posting list inside the window (stored for k_join), whose
-// total E(q) sets the number of work items.
+// lanes per query in k_plan: a wavefront plans kPlanGroup queries at once (probe
+// prefixes are often shorter than a wavefront)
+constexpr int kPlanGroup = 16;
+
+// Groups of kPlanGroup lanes loop over queries: the source window [q+1, hi) and, for
+// each probe-prefix token, the range of its posting list inside the window (stored
+// for k_join), whose total E(q) sets the number of work items.
 __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, const uint32_t *__restrict__ pos,
                                               const uint32_t *__restrict__ len, int64_t n,
                                               const uint32_t *__restrict__ job, const float *__restrict__ tj,
@@ -525,10 +529,16 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
                                               uint32_t *__restrict__ nparts, uint32_t *__restrict__ eq_out,
                                               uint64_t *__restrict__ bcnt, uint32_t stream_parts)
 {
-    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
-    for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < n; q += nwaves) {
+    const int64_t ngroups = (int64_t)gridDim.x * (blockDim.x / kPlanGroup);
+    const uint32_t gl = (uint32_t)lane_id() & (kPlanGroup - 1);   // lane within the group
+    // every group of a wavefront runs the same number of iterations (the loop bound is
+    // rounded up to whole wavefronts), so the group sum below sees all its lanes
+    const int64_t nround = (n + kWave / kPlanGroup - 1) / (kWave / kPlanGroup) * (kWave / kPlanGroup);
+    for (int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kPlanGroup; g < nround; g += ngroups) {
+        const bool live = g < n;
+        const int64_t q = live ? g : n - 1;
         const uint32_t lq = len[q];
-        const uint32_t qp = probelen[q];
+        const uint32_t qp = live ? probelen[q] : 0u;
         const uint32_t jb = job[q];
         // window: the s > q of the same job with len[s] >= the smallest partner length;
         // lengths are non-increasing within a job
@@ -542,7 +552,7 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
         const uint32_t wend = lo;   // window is [q+1, wend)
         const uint32_t ro = poff[q];
         uint32_t e = 0;
-        for (uint32_t j = lane_id(); j < qp; j += kWave) {
+        for (uint32_t j = gl; j < qp; j += kPlanGroup) {
             uint32_t term = tok[pos[q] + j];
             uint32_t L0 = loff[term], L1 = loff[term + 1];
             uint32_t a = lower_bound_sid(psid, fence, L0, L1, (uint32_t)q + 1);
@@ -550,8 +560,9 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
             rng[ro + j] = make_uint2(a, b);
             e += b - a;
         }
-        e = wave_sum(e);
-        if (lane_id() == 0) {
+#pragma unroll
+        for (int d = kPlanGroup / 2; d > 0; d >>= 1) e += __shfl_xor(e, d, kPlanGroup);
+        if (live && gl == 0) {
             hi_out[q] = wend;
             eq_out[q] = e;
             const uint32_t np = e == 0 ? 0u : (e + target - 1) / target;
@@ -1219,57 +1230,81 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             const uint4 *qs4 = (const uint4 *)(A.qsig + (size_t)q * kSigWords);
             const uint4 *qh4 = (const uint4 *)(A.qsig64 + (size_t)q * kSig64Words);
             const bool qbad = A.qsigbad[q] != 0;
-            for (uint32_t base = 0; base < nsig; base += kWave) {
-                const uint32_t c = base + lane;
-                const bool in = c < nsig;
-                const uint32_t ent = clist[min(c, nsig - 1u)];
-                const uint32_t h = ent & 0xFFFFu;
-                const uint32_t ls = ent >> 16;
-                const uint32_t s = tkey[h];
-                const uint8_t sbad = A.sigbad[s];
-                const uint4 *sh4 = (const uint4 *)(A.sig64 + (size_t)s * kSig64Words);
-                uint4 b4[kSig64Words / 4];
+            // kSigPer candidates per lane and step: their signature lines are gathered
+            // together, so a step waits for memory once
+            constexpr int kSigPer = 2;
+            for (uint32_t base = 0; base < nsig; base += kWave * kSigPer) {
+                uint32_t ent[kSigPer], s[kSigPer];
+                uint8_t sbad[kSigPer];
+                uint4 b4[kSigPer][kSig64Words / 4];
 #pragma unroll
-                for (int k = 0; k < kSig64Words / 4; k++) b4[k] = sh4[k];
-                const int floorb = (int)floorf(fgss_min_overlap(lq, ls, t));
-                uint32_t sad = 0;
-#pragma unroll
-                for (int k = 0; k < kSig64Words / 4; k++) {
-                    const uint4 a4 = qh4[k];
-                    sad = __builtin_amdgcn_sad_u8(a4.x, b4[k].x, sad);
-                    sad = __builtin_amdgcn_sad_u8(a4.y, b4[k].y, sad);
-                    sad = __builtin_amdgcn_sad_u8(a4.z, b4[k].z, sad);
-                    sad = __builtin_amdgcn_sad_u8(a4.w, b4[k].w, sad);
+                for (int u = 0; u < kSigPer; u++) {
+                    const uint32_t c = base + (uint32_t)(u * kWave) + lane;
+                    ent[u] = clist[min(c, nsig - 1u)];
+                    s[u] = tkey[ent[u] & 0xFFFFu];
                 }
-                const bool bad = qbad || sbad != 0;
-                // sum over buckets of min(cQ, cS) = (|Q| + |S| - SAD) / 2 >= |Q n S|
-                const bool exact = in && !bad && ((int)(lq + ls) - (int)sad) / 2 >= floorb;
-                bool keep = in && bad;
-                if (__ballot(exact)) {
-                    WACC(W_SIG2, __popcll(__ballot(exact)));
-                    if (exact) {
-                        const uint4 *ss4 = (const uint4 *)(A.sig + (size_t)s * kSigWords);
-                        uint32_t sad2 = 0;
 #pragma unroll
-                        for (int k = 0; k < kSigWords / 4; k++) {
-                            const uint4 a4 = qs4[k], c4 = ss4[k];
-                            sad2 = __builtin_amdgcn_sad_u8(a4.x, c4.x, sad2);
-                            sad2 = __builtin_amdgcn_sad_u8(a4.y, c4.y, sad2);
-                            sad2 = __builtin_amdgcn_sad_u8(a4.z, c4.z, sad2);
-                            sad2 = __builtin_amdgcn_sad_u8(a4.w, c4.w, sad2);
+                for (int u = 0; u < kSigPer; u++) {
+                    sbad[u] = A.sigbad[s[u]];
+                    const uint4 *sh4 = (const uint4 *)(A.sig64 + (size_t)s[u] * kSig64Words);
+#pragma unroll
+                    for (int k = 0; k < kSig64Words / 4; k++) b4[u][k] = sh4[k];
+                }
+                bool keep[kSigPer];
+#pragma unroll
+                for (int u = 0; u < kSigPer; u++) {
+                    const bool in = base + (uint32_t)(u * kWave) + lane < nsig;
+                    const uint32_t ls = ent[u] >> 16;
+                    const int floorb = (int)floorf(fgss_min_overlap(lq, ls, t));
+                    uint32_t sad = 0;
+#pragma unroll
+                    for (int k = 0; k < kSig64Words / 4; k++) {
+                        const uint4 a4 = qh4[k];
+                        sad = __builtin_amdgcn_sad_u8(a4.x, b4[u][k].x, sad);
+                        sad = __builtin_amdgcn_sad_u8(a4.y, b4[u][k].y, sad);
+                        sad = __builtin_amdgcn_sad_u8(a4.z, b4[u][k].z, sad);
+                        sad = __builtin_amdgcn_sad_u8(a4.w, b4[u][k].w, sad);
+                    }
+                    const bool bad = qbad || sbad[u] != 0;
+                    // sum over buckets of min(cQ, cS) = (|Q| + |S| - SAD) / 2 >= |Q n S|
+                    const bool exact = in && !bad && ((int)(lq + ls) - (int)sad) / 2 >= floorb;
+                    keep[u] = in && bad;
+                    if (__ballot(exact)) {
+                        WACC(W_SIG2, __popcll(__ballot(exact)));
+                        if (exact) {
+                            const uint4 *ss4 = (const uint4 *)(A.sig + (size_t)s[u] * kSigWords);
+                            uint32_t sad2 = 0;
+#pragma unroll
+                            for (int k = 0; k < kSigWords / 4; k++) {
+                                const uint4 a4 = qs4[k], c4 = ss4[k];
+                                sad2 = __builtin_amdgcn_sad_u8(a4.x, c4.x, sad2);
+                                sad2 = __builtin_amdgcn_sad_u8(a4.y, c4.y, sad2);
+                                sad2 = __builtin_amdgcn_sad_u8(a4.z, c4.z, sad2);
+                                sad2 = __builtin_amdgcn_sad_u8(a4.w, c4.w, sad2);
+                            }
+                            keep[u] = ((int)(lq + ls) - (int)sad2) / 2 >= floorb;
                         }
-                        keep = ((int)(lq + ls) - (int)sad2) / 2 >= floorb;
                     }
                 }
-                const bool drop = in && !keep;
-                *(drop ? &tkey[h] : &qstage[lane]) = kEmpty;
-                *(drop ? &tcnt[h] : &qstage[lane]) = 0;
-                *(drop ? &tlast[h] : &qstage[lane]) = 0;
-                const uint64_t bal = __ballot(keep);
+                // every entry of this step is in registers: compact in place (the
+                // writes land at or below the entries just read) and clear the dropped
+#pragma unroll
+                for (int u = 0; u < kSigPer; u++) {
+                    const bool in = base + (uint32_t)(u * kWave) + lane < nsig;
+                    const uint32_t h = ent[u] & 0xFFFFu;
+                    const bool drop = in && !keep[u];
+                    *(drop ? &tkey[h] : &qstage[lane]) = kEmpty;
+                    *(drop ? &tcnt[h] : &qstage[lane]) = 0;
+                    *(drop ? &tlast[h] : &qstage[lane]) = 0;
+                }
                 wave_sync();
-                const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                *(keep ? &clist[nver + below] : &qstage[lane]) = ent;
-                nver += (uint32_t)__popcll(bal);
+#pragma unroll
+                for (int u = 0; u < kSigPer; u++) {
+                    const uint64_t bal = __ballot(keep[u]);
+                    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                    *(keep[u] ? &clist[nver + below] : &qstage[lane]) = ent[u];
+                    nver += (uint32_t)__popcll(bal);
+                }
             }
             wave_sync();
         }
@@ -1637,7 +1672,7 @@ static int enqueue_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool timing)
     if (rc) return rc;
     if (timing) HIPCK(hipEventRecord(d->ev[1], st));
     // 6-8: plan work items
-    hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 4096)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(grid_for(n * kPlanGroup, 256), 4096)), dim3(256), 0, st,
                        d->tok, d->pos, d->len, n, d->job, d->tj, d->jend, probelen, poff, loff, psid, fence, cfg.target,
                        P<uint2>(d->b_rng), hi, nparts, eq, bcnt, cfg.stream_parts);
     HIPCK(hipGetLastError());
@@ -2281,7 +2316,7 @@ __global__ void k_probelen_rs(const uint32_t *__restrict__ len, const uint32_t *
     probelen[v] = p < l ? p : l;
 }
 
-// Per half-probe (a wave each): its source window -- indexed records in the
+// Per half-probe (kPlanGroup lanes each): its source window -- indexed records in the
 // size-filter band of the probe, longer than it (2r) or not (2r + 1); a contiguous id
 // range because lengths are non-increasing -- and each probe token's posting range
 // inside it.
@@ -2297,11 +2332,16 @@ __global__ __launch_bounds__(256) void k_plan_rs(const uint32_t *__restrict__ pt
                                                  uint32_t *__restrict__ nparts, uint32_t *__restrict__ eq_out,
                                                  uint64_t *__restrict__ bcnt, uint32_t stream_parts)
 {
-    const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
-    for (int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; v < 2 * np; v += nwaves) {
+    const int64_t ngroups = (int64_t)gridDim.x * (blockDim.x / kPlanGroup);
+    const uint32_t gl = (uint32_t)lane_id() & (kPlanGroup - 1);   // lane within the group
+    // whole wavefronts iterate together (see k_plan)
+    const int64_t nv = 2 * np, nround = (nv + kWave / kPlanGroup - 1) / (kWave / kPlanGroup) * (kWave / kPlanGroup);
+    for (int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kPlanGroup; g < nround; g += ngroups) {
+        const bool live = g < nv;
+        const int64_t v = live ? g : nv - 1;
         const int64_t r = v >> 1;
         const bool longer = (v & 1) == 0;
-        const uint32_t lr = plen[r], qp = probelen[v];
+        const uint32_t lr = plen[r], qp = live ? probelen[v] : 0u;
         const float t = tj[pjob[r]];
         // band [lmin, lmax] split at lr: (lr, lmax] for 2r, [lmin, lr] for 2r + 1
         const uint32_t top = longer ? partner_lmax(lr, t, smax_len) : lr;
@@ -2322,7 +2362,7 @@ __global__ __launch_bounds__(256) void k_plan_rs(const uint32_t *__restrict__ pt
         const uint32_t whi = a;
         const uint32_t ro = poff[v];
         uint32_t e = 0;
-        for (uint32_t j = lane_id(); j < qp; j += kWave) {
+        for (uint32_t j = gl; j < qp; j += kPlanGroup) {
             const uint32_t term = ptok[ppos[r] + j];
             uint32_t x = 0, y = 0;
             if (term < n_terms && wlo < whi) {   // a q-gram the index has never seen has no postings
@@ -2333,8 +2373,9 @@ __global__ __launch_bounds__(256) void k_plan_rs(const uint32_t *__restrict__ pt
             rng[ro + j] = make_uint2(x, y);
             e += y - x;
         }
-        e = wave_sum(e);
-        if (lane_id() == 0) {
+#pragma unroll
+        for (int d = kPlanGroup / 2; d > 0; d >>= 1) e += __shfl_xor(e, d, kPlanGroup);
+        if (live && gl == 0) {
             wlo_out[v] = wlo;
             hi_out[v] = whi;
             eq_out[v] = e;
@@ -2467,7 +2508,7 @@ static int enqueue_rs(fgss_server *srv, ServeSlot &sl, bool timing)
     HIPCK(hipGetLastError());
     HIPCK(exclusive_scan<uint32_t>(probelen, poff, v, misc + M_NP, scr, st));
     if (timing) HIPCK(hipEventRecord(d->ev[1], st));
-    hipLaunchKernelGGL(k_plan_rs, dim3((unsigned)std::min<int64_t>(grid_for(v * kWave, 256), 4096)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_plan_rs, dim3((unsigned)std::min<int64_t>(grid_for(v * kPlanGroup, 256), 4096)), dim3(256), 0, st,
                        d->tok, d->pos, d->len, n, d->job, d->tj, probelen, poff, x->len, (uint32_t)x->n,
                        (uint32_t)x->max_len, P<uint32_t>(x->b_loff), P<uint32_t>(x->b_psid), P<uint32_t>(x->b_fence),
                        x->n_terms, cfg.target, P<uint2>(d->b_rng), wlo, hi, nparts, eq, bcnt, cfg.stream_parts);
