@@ -34,6 +34,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "fgss_bounds.h"
@@ -98,6 +99,19 @@ struct Item {
     float t;                // its threshold
     uint32_t lq, qpos, qpoff;   // the query's length (| kMidProbe), token offset and probe-range offset
 };
+constexpr int kItemWords = (int)(sizeof(Item) / 4);
+static_assert(sizeof(Item) == 48 && offsetof(Item, q) == 0 && offsetof(Item, a) == 4 && offsetof(Item, b) == 8 &&
+                  offsetof(Item, pad) == 12 && offsetof(Item, part) == 16 && offsetof(Item, nparts) == 20 &&
+                  offsetof(Item, bofs) == 24 && offsetof(Item, job) == 28 && offsetof(Item, t) == 32 &&
+                  offsetof(Item, lq) == 36 && offsetof(Item, qpos) == 40 && offsetof(Item, qpoff) == 44,
+              "k_join reads item fields by word index");
+
+// word `lane` of item i for lanes below kItemWords (the others read word 0)
+__device__ __forceinline__ uint32_t item_word(const Item *__restrict__ items, uint32_t i, int lane)
+{
+    return ((const uint32_t *)(items + i))[lane < kItemWords ? lane : 0];
+}
+
 // R-S items over the sources longer than the probe: the probe contributes its mid
 // prefix and the sources their max prefix (the longer record is the query)
 constexpr uint32_t kMidProbe = 0x10000u;
@@ -894,7 +908,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
     uint32_t batch_next = 0, batch_end = 0;
     uint32_t cur_job = kEmpty;
     // the next item of the claimed batch is loaded while the current one runs
-    Item nxt;
+    uint32_t nxtw = 0;
     bool have_nxt = false;
     for (;;) {
         if (batch_next >= batch_end) {
@@ -907,23 +921,25 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         }
         if (batch_next >= n_items) break;
         const uint32_t it = batch_next++;
-        Item ld = have_nxt ? nxt : A.items[it];
+        // an item is kItemWords words, lane k holding word k (one register for the
+        // prefetched item instead of kItemWords)
+        const uint32_t ldw = have_nxt ? nxtw : item_word(A.items, it, lane);
         have_nxt = batch_next < batch_end;
-        if (have_nxt) nxt = A.items[batch_next];
+        if (have_nxt) nxtw = item_word(A.items, batch_next, lane);
         // the item is the same for every lane: its fields live in scalar registers
         Item item;
-        item.q = __builtin_amdgcn_readfirstlane(ld.q);
-        item.a = __builtin_amdgcn_readfirstlane(ld.a);
-        item.b = __builtin_amdgcn_readfirstlane(ld.b);
-        item.pad = __builtin_amdgcn_readfirstlane(ld.pad);
-        item.part = __builtin_amdgcn_readfirstlane(ld.part);
-        item.nparts = __builtin_amdgcn_readfirstlane(ld.nparts);
-        item.bofs = __builtin_amdgcn_readfirstlane(ld.bofs);
-        item.job = __builtin_amdgcn_readfirstlane(ld.job);
-        item.t = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(ld.t)));
-        item.lq = __builtin_amdgcn_readfirstlane(ld.lq);
-        item.qpos = __builtin_amdgcn_readfirstlane(ld.qpos);
-        item.qpoff = __builtin_amdgcn_readfirstlane(ld.qpoff);
+        item.q = __builtin_amdgcn_readlane(ldw, 0);
+        item.a = __builtin_amdgcn_readlane(ldw, 1);
+        item.b = __builtin_amdgcn_readlane(ldw, 2);
+        item.pad = __builtin_amdgcn_readlane(ldw, 3);
+        item.part = __builtin_amdgcn_readlane(ldw, 4);
+        item.nparts = __builtin_amdgcn_readlane(ldw, 5);
+        item.bofs = __builtin_amdgcn_readlane(ldw, 6);
+        item.job = __builtin_amdgcn_readlane(ldw, 7);
+        item.t = __int_as_float(__builtin_amdgcn_readlane(ldw, 8));
+        item.lq = __builtin_amdgcn_readlane(ldw, 9);
+        item.qpos = __builtin_amdgcn_readlane(ldw, 10);
+        item.qpoff = __builtin_amdgcn_readlane(ldw, 11);
         const uint32_t q = item.q;
         const uint32_t jq = item.job;
         const float t = item.t;
@@ -1030,13 +1046,17 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             // kernel): lanes without a posting work on the last one and aim their LDS
             // updates at a word of their own in the idle query stage.
             uint32_t *scr = qstage;
-            for (uint32_t e0 = 0; e0 < total; e0 += kWave * kUnroll) {
-                uint2 pe[kUnroll];
-                uint32_t seg[kUnroll], e[kUnroll];
+            // A step takes N postings per lane (rows of kWave): full steps of kUnroll
+            // rows, then a tail of 2 or 1 rows, so that a short range (most items hold
+            // a few hundred postings or less) does not pay for empty rows.
+            auto fstep = [&](auto rows, uint32_t e0) -> bool {
+                constexpr int N = decltype(rows)::value;
+                uint2 pe[N];
+                uint32_t seg[N], e[N];
                 // lanes past the end work on the last posting (a real one: every load
                 // and LDS read below is unconditional) and take no part in the updates
 #pragma unroll
-                for (int k = 0; k < kUnroll; k++) { e[k] = min(e0 + (uint32_t)(k * kWave) + lane, total - 1u); seg[k] = 0; }
+                for (int k = 0; k < N; k++) { e[k] = min(e0 + (uint32_t)(k * kWave) + lane, total - 1u); seg[k] = 0; }
                 // segment of each flattened posting: the largest jj with rpre[jj] <= e, by a
                 // search of wave-uniform depth (rpre[nb] = total > e keeps it below nb)
 #pragma unroll
@@ -1044,18 +1064,18 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                     static_assert(kTokBatch <= 128, "search depth");
                     if (st < steps) {
 #pragma unroll
-                        for (int k = 0; k < kUnroll; k++) {
+                        for (int k = 0; k < N; k++) {
                             const uint32_t mid = min(seg[k] + (1u << st), nb);
                             seg[k] = rpre[mid] <= e[k] ? mid : seg[k];
                         }
                     }
                 }
 #pragma unroll
-                for (int k = 0; k < kUnroll; k++) pe[k] = A.post[rlo[seg[k]] + (e[k] - rpre[seg[k]])];
+                for (int k = 0; k < N; k++) pe[k] = A.post[rlo[seg[k]] + (e[k] - rpre[seg[k]])];
                 // bit k: posting k is in range and, for R-S items, inside the source's prefix
-                uint32_t valm = 0, jpos[kUnroll];
+                uint32_t valm = 0, jpos[N];
 #pragma unroll
-                for (int k = 0; k < kUnroll; k++) {
+                for (int k = 0; k < N; k++) {
                     uint32_t v = (e0 + (uint32_t)(k * kWave) + (uint32_t)lane) < total;
                     if (RS) {   // the source's own prefix at t
                         const uint32_t ls = pe[k].y >> 16;
@@ -1064,7 +1084,15 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                     valm |= v << k;
                     jpos[k] = jb + seg[k];
                 }
-                if (table_insert(pe, valm, jpos, tkey, tcnt, tlast, clist, scr, lane, tlog, tsize, nlist)) {
+                return table_insert(pe, valm, jpos, tkey, tcnt, tlast, clist, scr, lane, tlog, tsize, nlist);
+            };
+            for (uint32_t e0 = 0; e0 < total;) {
+                const uint32_t rem = total - e0;   // wave-uniform
+                bool full;
+                if (rem > 3u * kWave) { full = fstep(std::integral_constant<int, kUnroll>(), e0); e0 += kWave * kUnroll; }
+                else if (rem > (uint32_t)kWave) { full = fstep(std::integral_constant<int, 2>(), e0); e0 += 2 * kWave; }
+                else { full = fstep(std::integral_constant<int, 1>(), e0); e0 += kWave; }
+                if (full) {
                     overflow = true;
                     break;
                 }
@@ -1095,14 +1123,16 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                 if (lane < (int)(kStreamQp / 32)) satm[lane] = 0;
                 wave_sync();
                 const uint32_t K = 1u << logK, total = nact << logK;
-                for (uint32_t e0 = 0; e0 < total; e0 += kWave * kUnroll) {
+                // rows per step as in the flattened loop (a round's total is nact x K)
+                auto sstep = [&](auto rows, uint32_t e0) -> bool {
+                    constexpr int N = decltype(rows)::value;
                     // every lane reads a posting (lanes past the end the last group's;
                     // reads past a token's list land in the next list or the spare
                     // postings) together with its token's list end, and then tests it
-                    uint2 pe[kUnroll];
-                    uint32_t jv[kUnroll], curj[kUnroll], idx[kUnroll], end[kUnroll];
+                    uint2 pe[N];
+                    uint32_t jv[N], curj[N], idx[N], end[N];
 #pragma unroll
-                    for (int k = 0; k < kUnroll; k++) {
+                    for (int k = 0; k < N; k++) {
                         const uint32_t ec = min(e0 + (uint32_t)(k * kWave) + lane, total - 1u);
                         jv[k] = toks[ec >> logK];
                         curj[k] = cur[jv[k]];
@@ -1112,7 +1142,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                     }
                     uint32_t valm = 0;
 #pragma unroll
-                    for (int k = 0; k < kUnroll; k++) {
+                    for (int k = 0; k < N; k++) {
                         const uint32_t e = e0 + (uint32_t)(k * kWave) + lane;
                         const bool below = e < total && idx[k] < end[k] && pe[k].x < wb;
                         // below is a prefix of each token's K lanes (postings ascend by
@@ -1134,7 +1164,15 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                         }
                         valm |= v << k;
                     }
-                    if (table_insert(pe, valm, jv, tkey, tcnt, tlast, clist, sscr, lane, tlog, tsize, nlist)) {
+                    return table_insert(pe, valm, jv, tkey, tcnt, tlast, clist, sscr, lane, tlog, tsize, nlist);
+                };
+                for (uint32_t e0 = 0; e0 < total;) {
+                    const uint32_t rem = total - e0;   // wave-uniform
+                    bool full;
+                    if (rem > 3u * kWave) { full = sstep(std::integral_constant<int, kUnroll>(), e0); e0 += kWave * kUnroll; }
+                    else if (rem > (uint32_t)kWave) { full = sstep(std::integral_constant<int, 2>(), e0); e0 += 2 * kWave; }
+                    else { full = sstep(std::integral_constant<int, 1>(), e0); e0 += kWave; }
+                    if (full) {
                         overflow = true;
                         break;
                     }
