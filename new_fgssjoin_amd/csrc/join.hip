@@ -596,33 +596,54 @@ __device__ __forceinline__ uint32_t part_start(uint64_t a0, uint64_t W, uint32_t
 
 // For every query split into np > 1 parts: the position of each internal part
 // boundary in each probe token's posting range, computed once here so that k_join
-// reads its ranges instead of searching.  A wave per query; its lanes take the
-// (boundary, token) pairs, one independent binary search each.  Offsets are 64-bit:
+// reads its ranges instead of searching.  The table is cut into chunks of
+// kBoundsChunk entries, a wavefront each, whatever queries they belong to (a long
+// query's thousands of entries would otherwise keep one wavefront busy while the
+// others idle); each lane finds its entry's query among the chunk's and does one
+// independent binary search.  Offsets are 64-bit:
 // a query whose rows do not fit the table's capacity is not planned (k_items applies
 // the same test and its items search their ranges instead).
+// entries of the boundary table a wavefront of k_bounds takes at a time
+constexpr uint32_t kBoundsChunk = 256;
+
+// last q in [lo, hi] with ofs[q] <= g (ofs non-decreasing, ofs[lo] <= g)
+__device__ __forceinline__ int64_t last_le(const uint64_t *__restrict__ ofs, int64_t lo, int64_t hi, uint64_t g)
+{
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (ofs[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 __global__ __launch_bounds__(256) void k_bounds(int64_t n, const uint32_t *__restrict__ probelen,
                                                 const uint32_t *__restrict__ poff, const uint2 *__restrict__ rng,
                                                 const uint32_t *__restrict__ wlo, const uint32_t *__restrict__ hi,
                                                 const uint32_t *__restrict__ nparts,
                                                 const uint64_t *__restrict__ bofs, const uint64_t *__restrict__ bcnt,
+                                                const uint64_t *__restrict__ nb_total,
                                                 const uint32_t *__restrict__ psid,
                                                 const uint32_t *__restrict__ fence, uint32_t *__restrict__ bounds,
                                                 uint32_t cap_bounds)
 {
+    // entries at or beyond cap_bounds belong to queries that are not planned
+    const uint64_t NB = min(*nb_total, (uint64_t)cap_bounds);
+    const uint64_t nchunks = (NB + kBoundsChunk - 1) / kBoundsChunk;
     const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
-    for (int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; q < n; q += nwaves) {
-        const uint32_t np = nparts[q];
-        if (np < 2) continue;
-        const uint32_t qp = probelen[q];
-        const uint64_t a0 = wlo ? (uint64_t)wlo[q] : (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
-        const uint64_t total = bcnt[q];
-        if (item_mode(np, qp, total, bofs[q], cap_bounds) != IM_PLANNED) continue;
-        uint32_t *B = bounds + bofs[q];
-        const uint2 *R = rng + poff[q];
-        for (uint32_t k = lane_id(); k < (uint32_t)total; k += kWave) {
+    for (uint64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; c < nchunks; c += nwaves) {
+        const uint64_t g0 = c * kBoundsChunk, g1 = min(g0 + kBoundsChunk, NB);
+        // the queries holding the chunk's first and last entries (wave-uniform)
+        const int64_t qa = last_le(bofs, 0, n - 1, g0), qb = last_le(bofs, qa, n - 1, g1 - 1);
+        for (uint64_t g = g0 + lane_id(); g < g1; g += kWave) {
+            const int64_t q = last_le(bofs, qa, qb, g);   // its entries [bofs[q], bofs[q] + bcnt[q]) hold g
+            const uint32_t np = nparts[q], qp = probelen[q];
+            if (item_mode(np, qp, bcnt[q], bofs[q], cap_bounds) != IM_PLANNED) continue;
+            const uint64_t a0 = wlo ? (uint64_t)wlo[q] : (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
+            const uint32_t k = (uint32_t)(g - bofs[q]);
             const uint32_t p = k / qp + 1, j = k - (p - 1) * qp;
-            const uint2 r = R[j];
-            B[k] = lower_bound_sid(psid, fence, r.x, r.y, part_start(a0, W, p, np));
+            const uint2 r = rng[poff[q] + j];
+            bounds[g] = lower_bound_sid(psid, fence, r.x, r.y, part_start(a0, W, p, np));
         }
     }
 }
@@ -1532,6 +1553,7 @@ static inline unsigned grid_for(int64_t n, int threads) { return (unsigned)std::
 // device-side counts; M_NB64 (two words, 8-byte aligned) = boundary-table rows wanted
 enum Misc { M_NE = 0, M_NITEMS, M_NP, M_NRESPLIT, M_NB64 = 6, M_COUNT = 16 };
 constexpr uint32_t kBoundsBudget = 1u << 28;   // boundary-table entries at most (1 GiB)
+constexpr unsigned kBoundsGrid = 2048;          // k_bounds blocks (the table size is known on the device only)
 
 static uint64_t misc_nb(const fgss_device_sets *d)
 {
@@ -1720,8 +1742,8 @@ static int enqueue_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool timing)
     hipLaunchKernelGGL(k_nitems, dim3(grid_for(n, 256)), dim3(256), 0, st, nparts, probelen, bofs, bcnt, blim, n, nitems);
     HIPCK(hipGetLastError());
     HIPCK(exclusive_scan<uint32_t>(nitems, partoff, n, misc + M_NITEMS, scr, st));
-    hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(grid_for(n * kWave, 256), 8192)), dim3(256), 0, st, n,
-                       probelen, poff, P<uint2>(d->b_rng), (const uint32_t *)nullptr, hi, nparts, bofs, bcnt, psid,
+    hipLaunchKernelGGL(k_bounds, dim3(kBoundsGrid), dim3(256), 0, st, n, probelen, poff, P<uint2>(d->b_rng),
+                       (const uint32_t *)nullptr, hi, nparts, bofs, bcnt, (const uint64_t *)(misc + M_NB64), psid,
                        fence, P<uint32_t>(d->b_bounds), blim);
     HIPCK(hipGetLastError());
     hipLaunchKernelGGL(k_items, dim3(grid_for(n, 256)), dim3(256), 0, st, 0, (const uint32_t *)nullptr, hi, nparts,
@@ -2557,8 +2579,8 @@ static int enqueue_rs(fgss_server *srv, ServeSlot &sl, bool timing)
     hipLaunchKernelGGL(k_nitems, dim3(grid_for(v, 256)), dim3(256), 0, st, nparts, probelen, bofs, bcnt, blim, v, nitems);
     HIPCK(hipGetLastError());
     HIPCK(exclusive_scan<uint32_t>(nitems, partoff, v, misc + M_NITEMS, scr, st));
-    hipLaunchKernelGGL(k_bounds, dim3((unsigned)std::min<int64_t>(grid_for(v * kWave, 256), 8192)), dim3(256), 0, st, v,
-                       probelen, poff, P<uint2>(d->b_rng), wlo, hi, nparts, bofs, bcnt, P<uint32_t>(x->b_psid),
+    hipLaunchKernelGGL(k_bounds, dim3(kBoundsGrid), dim3(256), 0, st, v, probelen, poff, P<uint2>(d->b_rng), wlo, hi,
+                       nparts, bofs, bcnt, (const uint64_t *)(misc + M_NB64), P<uint32_t>(x->b_psid),
                        P<uint32_t>(x->b_fence), P<uint32_t>(d->b_bounds), blim);
     HIPCK(hipGetLastError());
     hipLaunchKernelGGL(k_items, dim3(grid_for(v, 256)), dim3(256), 0, st, 1, wlo, hi, nparts, probelen, eq, partoff,
