@@ -493,6 +493,7 @@ def main():
                        "bytes_per_launch": kbytes / n_launch, "kernel_ms_per_launch": kernel_s * 1e3 / n_launch,
                        "basis": basis, "join_frac": join_frac,
                        "lds_bank_conflict_ratio": prof.get("lds_bank_conflict_ratio"),
+                       "valu_busy": prof.get("valu_busy_max"),
                        "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) if prof else None}
     if args.batch_jobs > 0:
         out["batched"] = batched_leg(fg, args.batch_jobs, reps=3)

@@ -183,6 +183,7 @@ struct fgss_device_sets {
     hipGraphExec_t graph_exec = nullptr;
     int graph_key[5] = {0, 0, 0, 0, 0};   // table slots, load, capacity generation, boundary budget, stream parts
     int cap_gen = 0;              // bumped whenever a buffer the pipeline uses is reallocated
+    bool sized = false;           // a run has completed (first-run buffer sizing is over)
 };
 
 static int ensure(DevBuf &b, size_t bytes)
@@ -1859,9 +1860,15 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
         const uint64_t nres = d->h_ctr[C_RESULTS];
         bool again = false;
         if (n_items > d->cap_items) { d->cap_items = n_items + n_items / 4 + 1024; again = true; }
-        // a larger boundary table only saves searches: grow it within the budget
+        // A larger boundary table only saves searches (the queries it could not plan
+        // streamed or searched their ranges, so the run is complete): grow it within
+        // the budget for the next run, and rerun now only for the first run of a
+        // collection, whose other buffers are being sized anyway.
         const uint64_t want = std::min<uint64_t>(nB + nB / 4 + 1024, cfg.bounds);
-        if (nB > d->cap_bounds && want > d->cap_bounds) { d->cap_bounds = (uint32_t)want; again = true; }
+        if (nB > d->cap_bounds && want > d->cap_bounds) {
+            d->cap_bounds = (uint32_t)want;
+            if (!d->sized) again = true;
+        }
         if (d->h_ctr[C_OVERFLOW] > d->cap_ovf) {
             d->cap_ovf = (uint32_t)std::min<uint64_t>(d->h_ctr[C_OVERFLOW] * 5 / 4 + 1024, UINT32_MAX / 8);
             again = true;
@@ -1876,6 +1883,7 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
         if (rc) return rc;
         HIPCK(hipStreamSynchronize(st));
     }
+    d->sized = true;
     trace_mark("pipeline");
     T.graph = use_graph && ops.graph_ok;
     T.launches = 1;
@@ -1966,7 +1974,9 @@ static int run_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool use_graph, 
     const PipeOps ops = self_ops(d, cfg);
     int rc = launch_once(d, cfg, ops, use_graph);
     if (rc) return rc;
+    trace_mark("enqueued");
     HIPCK(hipStreamSynchronize(d->st));
+    trace_mark("synced");
     return finish_run(d, cfg, ops, use_graph, tri, T);
 }
 
@@ -2679,6 +2689,10 @@ static int slot_reserve(ServeSlot &sl, int64_t n, int64_t ntok, uint32_t jobs)
     const uint32_t nj0 = d->njobs;
     d->n = n; d->ntok = ntok; d->njobs = jobs;
     d->cap_out = std::max<uint32_t>(d->cap_out, 1u << 20);
+    // boundary rows grow with the probes' posting reach, not their tokens: 2^26 entries
+    // (256 MB) plan the heaviest runs of a 2M-title index without growing
+    d->cap_bounds = std::max<uint32_t>(d->cap_bounds, std::min<uint32_t>(1u << 26, kBoundsBudget));
+    d->sized = true;   // runs that outgrow a reserved capacity still complete
     const int rc = slot_prepare(sl);
     d->n = n0; d->ntok = ntok0; d->njobs = nj0;
     return rc;

@@ -5,13 +5,14 @@ short text files kept beside the benchmark.
     python3 new_fgssjoin_amd/profile_summary.py trace <kernel_trace.csv> --kernel k_join --last 2
     python3 new_fgssjoin_amd/profile_summary.py pmc --kernel k_join --last 3 \\
         --copy-fetch <copy FETCH_SIZE csv> --copy-write <copy WRITE_SIZE csv> \\
-        --fetch <FETCH_SIZE csv> --write <WRITE_SIZE csv> [--lds <SQ_LDS_* csv>]
+        --fetch <FETCH_SIZE csv> --write <WRITE_SIZE csv> [--lds <SQ_LDS_* csv>] [--sq <SQ_INSTS_* csv>]
 
 `pmc` scales FETCH_SIZE and WRITE_SIZE (KiB per dispatch) by what the same counter
 reads for the 1 GiB copy of copy_calibration.py, takes the last `--last` dispatches
 of the kernel (the timed step of a run whose warm-up absorbed buffer sizing), and
 prints per-dispatch HBM bytes plus the LDS bank-conflict ratio
-SQ_LDS_BANK_CONFLICT / (SQ_LDS_IDX_ACTIVE - SQ_LDS_BANK_CONFLICT) when given.
+SQ_LDS_BANK_CONFLICT / (SQ_LDS_IDX_ACTIVE - SQ_LDS_BANK_CONFLICT) when given, and
+the VALU issue utilisation from SQ_INSTS_VALU when given.
 The final lines are `key value` pairs that bench.py reads.
 """
 from __future__ import annotations
@@ -98,9 +99,37 @@ def pmc(a):
               f"cycles over the last {a.last} dispatches")
         lines.append(f"lds_bank_conflict_ratio {ratio:.6g}")
         lines.append(f"lds_bank_conflict_cycles_per_inst {bank / insts if insts else float('nan'):.6g}")
+    if a.sq:
+        # VALU issue: a wave64 VALU instruction occupies one of the chip's SIMD16 units
+        # for 4 cycles; busy = instructions x 4 / (SIMDs x dispatch cycles)
+        c = _per_dispatch(a.sq, a.kernel)
+        dur = _durations(a.sq, a.kernel)[-a.last:]
+        valu = c["SQ_INSTS_VALU"][-a.last:]
+        print()
+        print(f"| {a.kernel} dispatch | us | VALU insts | SALU insts | LDS insts | VALU busy |")
+        print("|---:|---:|---:|---:|---:|---:|")
+        busy = []
+        for k, (d, v, sa, ld) in enumerate(zip(dur, valu, c["SQ_INSTS_SALU"][-a.last:], c["SQ_INSTS_LDS"][-a.last:])):
+            b = v * 4 / (SIMDS * d * 1e-9 * CLOCK_HZ) if d else float("nan")
+            busy.append(b)
+            print(f"| {k} | {d / 1e3:.1f} | {v:.4g} | {sa:.4g} | {ld:.4g} | {b:.2f} |")
+        lines.append(f"valu_busy_max {max(busy):.4g}")
     print()
     for ln in lines:
         print(ln)
+
+
+SIMDS = 256 * 4           # MI355X: 256 CUs of 4 SIMD16 units
+CLOCK_HZ = 2.4e9          # peak engine clock
+
+
+def _durations(path: str, kernel: str):
+    """dispatch durations (ns) of one kernel, in dispatch order"""
+    seen = {}
+    for r in csv.DictReader(open(path)):
+        if kernel in r["Kernel_Name"]:
+            seen[int(r["Dispatch_Id"])] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    return [seen[k] for k in sorted(seen)]
 
 
 def main():
@@ -121,6 +150,7 @@ def main():
     p.add_argument("--fetch", required=True)
     p.add_argument("--write", required=True)
     p.add_argument("--lds")
+    p.add_argument("--sq", help="SQ_INSTS_VALU / SQ_INSTS_SALU / SQ_INSTS_LDS csv")
     a = ap.parse_args()
     if a.cmd == "stats":
         stats(a.csv, a.top)
