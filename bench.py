@@ -124,7 +124,7 @@ def batched_leg(fg, n_jobs: int, reps: int):
     for r in range(reps):
         t0 = time.perf_counter()
         b.run(graph=True, table_slots=tune["table_slots"], load_pct=tune["load_pct"])
-        got = [b.pairs(j) for j in range(n_jobs)]
+        got = b.all_pairs()
         times.append(time.perf_counter() - t0)
         if r == 0:
             for g, p in zip(got, plain):

@@ -200,6 +200,12 @@ int fgss_batch_run(fgss_batch *b, const fgss_join_opts *opts, int32_t use_graph,
 /* Pairs of one job from the last run, exactly as fgss_selfjoin would return them
  * (sorted, indices and ids of that job's collection). */
 int fgss_batch_pairs(fgss_batch *b, int32_t job, fgss_pairs **out);
+/* Pairs of every job from the last run in one result: job j's pairs, exactly as
+ * fgss_batch_pairs(b, j) returns them, are entries [job_first[j], job_first[j+1]);
+ * job_first holds njobs + 1 entries.  job_stats (njobs entries, or null) receives
+ * each job's counters; out->stats is not filled.  No reference counterpart (the
+ * reference joins one collection per run). */
+int fgss_batch_pairs_all(fgss_batch *b, fgss_pairs **out, int64_t *job_first, fgss_join_stats *job_stats);
 /* Autotuner: time the batch for each LDS table size x work-item load and return
  * the fastest options (graph replays, median of reps). */
 int fgss_batch_autotune(fgss_batch *b, int32_t reps, fgss_join_opts *best, double *best_ms);

@@ -42,7 +42,7 @@ EXPORTS = (
     "fgss_selfjoin", "fgss_pairs_free", "fgss_pairs_print", "fgss_collection_print_index",
     "fgss_generate_text", "fgss_free",
     "fgss_batch_create", "fgss_batch_set_thresholds", "fgss_batch_run", "fgss_batch_pairs",
-    "fgss_batch_autotune", "fgss_batch_free",
+    "fgss_batch_pairs_all", "fgss_batch_autotune", "fgss_batch_free",
     "fgss_server_create", "fgss_server_submit", "fgss_server_poll", "fgss_server_done", "fgss_server_wait",
     "fgss_server_result", "fgss_server_hold", "fgss_server_free",
 )
@@ -138,6 +138,7 @@ def lib():
         "fgss_batch_set_thresholds": [vp, C.POINTER(C.c_float)],
         "fgss_batch_run": [vp, C.POINTER(JoinOpts), C.c_int32, C.POINTER(BatchStats)],
         "fgss_batch_pairs": [vp, C.c_int32, C.POINTER(C.POINTER(_Pairs))],
+        "fgss_batch_pairs_all": [vp, C.POINTER(C.POINTER(_Pairs)), C.POINTER(C.c_int64), C.POINTER(JoinStats)],
         "fgss_batch_autotune": [vp, C.c_int32, C.POINTER(JoinOpts), C.POINTER(C.c_double)],
         "fgss_server_create": [vp, C.c_float, C.c_int32, C.c_int64, C.POINTER(JoinOpts), C.POINTER(vp)],
         "fgss_server_submit": [vp, vp, C.c_float, i64p],
@@ -361,6 +362,26 @@ class Batch:
         out = C.POINTER(_Pairs)()
         _check(lib().fgss_batch_pairs(self._h, job, C.byref(out)))
         return _take_pairs(out)
+
+    def all_pairs(self) -> list:
+        """Every job's Pairs from the last run (as pairs(j) for each j), in one call."""
+        out = C.POINTER(_Pairs)()
+        first = (C.c_int64 * (self.njobs + 1))()
+        stats = (JoinStats * self.njobs)()
+        _check(lib().fgss_batch_pairs_all(self._h, C.byref(out), first, stats))
+        try:
+            p = out.contents
+            n = p.n_pairs
+
+            def cp(ptr, dt):
+                return np.ctypeslib.as_array(ptr, shape=(n,)).copy() if n > 0 else np.zeros(0, dt)
+            q, s, rid, sid = cp(p.q, np.int32), cp(p.s, np.int32), cp(p.rid, np.int32), cp(p.sid, np.int32)
+            ov, sim = cp(p.overlap, np.int32), cp(p.similarity, np.float32)
+        finally:
+            lib().fgss_pairs_free(out)
+        f = np.ctypeslib.as_array(first)
+        return [Pairs(q[a:b], s[a:b], rid[a:b], sid[a:b], ov[a:b], sim[a:b], stats[j].as_dict())
+                for j, (a, b) in enumerate(zip(f[:-1], f[1:]))]
 
     def autotune(self, reps: int = 3):
         best = JoinOpts()

@@ -2307,6 +2307,48 @@ extern "C" int fgss_batch_pairs(fgss_batch *b, int32_t job, fgss_pairs **out)
     return FGSS_OK;
 }
 
+extern "C" int fgss_batch_pairs_all(fgss_batch *b, fgss_pairs **out, int64_t *job_first, fgss_join_stats *job_stats)
+{
+    if (!b || !out || !job_first) return fgss_fail(FGSS_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (!b->has_run) return fgss_fail(FGSS_ERR_ARG, "the batch has not been run");
+    const size_t nj = b->colls.size(), R = b->job_first[nj];
+    auto *p = (fgss_pairs *)calloc(1, sizeof(fgss_pairs));
+    if (!p) return fgss_fail(FGSS_ERR_NOMEM, "out of host memory");
+    const size_t nb = std::max<size_t>(R, 1);
+    p->q = (int32_t *)malloc(nb * 4); p->s = (int32_t *)malloc(nb * 4); p->rid = (int32_t *)malloc(nb * 4);
+    p->sid = (int32_t *)malloc(nb * 4); p->overlap = (int32_t *)malloc(nb * 4);
+    p->similarity = (float *)malloc(nb * 4);
+    if (!p->q || !p->s || !p->rid || !p->sid || !p->overlap || !p->similarity) {
+        fgss_pairs_free(p);
+        return fgss_fail(FGSS_ERR_NOMEM, "out of host memory");
+    }
+    for (size_t j = 0; j < nj; j++) {
+        const fgss_collection *c = b->colls[j];
+        const int64_t q0 = b->rec_begin[j];
+        job_first[j] = (int64_t)b->job_first[j];
+        for (size_t k = b->job_first[j]; k < b->job_first[j + 1]; k++) {
+            const size_t i = (size_t)(b->order[k] & 0xFFFFFFFFull);
+            const int32_t q = (int32_t)(b->tri[3 * i] - q0), sv = (int32_t)(b->tri[3 * i + 1] - q0);
+            const int32_t ov = b->tri[3 * i + 2];
+            p->q[k] = q;
+            p->s[k] = sv;
+            p->rid[k] = c->id[(size_t)q];
+            p->sid[k] = c->id[(size_t)sv];
+            p->overlap[k] = ov;
+            p->similarity[k] = fgss_similarity((uint32_t)c->len[(size_t)q], (uint32_t)c->len[(size_t)sv], (uint32_t)ov);
+        }
+        if (job_stats) {
+            job_stats[j] = fgss_join_stats{};
+            fill_stats(b->totals, j, job_stats[j]);
+        }
+    }
+    job_first[nj] = (int64_t)R;
+    p->n_pairs = (int64_t)R;
+    *out = p;
+    return FGSS_OK;
+}
+
 // Autotuner: time the batch under each (table slots, item load) choice, replaying
 // captured graphs, and return the fastest.  reps runs per choice, the median counts.
 extern "C" int fgss_batch_autotune(fgss_batch *b, int32_t reps, fgss_join_opts *best, double *best_ms)

@@ -45,11 +45,15 @@ def test_batch_equals_plain_runs(fg, oracle_mod):
         assert st["graph"] == (1 if graph else 0)
         assert st["n_pairs"] == sum(len(p) for p in plain)
         assert st["candidates"] == sum(p.stats["candidates"] for p in plain)
+        every = b.all_pairs()   # all jobs in one call: the same pairs and counters
+        assert len(every) == len(plain)
         for j, p in enumerate(plain):
             got = b.pairs(j)
             assert _same(got, p), f"job {j}"
+            assert _same(every[j], p), f"job {j} (all_pairs)"
             for k in ("candidates", "intersections", "results", "postings_scanned"):
                 assert got.stats[k] == p.stats[k], (j, k)
+                assert every[j].stats[k] == p.stats[k], (j, k)
     # spot-check three jobs against the oracle directly
     for j in (0, 5, len(colls) - 2):
         tri, ctr = oracle_mod.selfjoin(oracle_sets(oracle_mod, colls[j]), ts[j])
