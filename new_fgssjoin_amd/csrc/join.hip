@@ -641,8 +641,11 @@ __global__ __launch_bounds__(256) void k_bounds(int64_t n, const uint32_t *__res
             const uint32_t np = nparts[q], qp = probelen[q];
             if (item_mode(np, qp, bcnt[q], bofs[q], cap_bounds) != IM_PLANNED) continue;
             const uint64_t a0 = wlo ? (uint64_t)wlo[q] : (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
+            // token-major rows: entry k = j * (np - 1) + (p - 1), so the lanes of a
+            // wavefront search one token's list for consecutive boundaries and share
+            // the fence and posting lines they read
             const uint32_t k = (uint32_t)(g - bofs[q]);
-            const uint32_t p = k / qp + 1, j = k - (p - 1) * qp;
+            const uint32_t j = k / (np - 1), p = k - j * (np - 1) + 1;
             const uint2 r = rng[poff[q] + j];
             bounds[g] = lower_bound_sid(psid, fence, r.x, r.y, part_start(a0, W, p, np));
         }
@@ -986,9 +989,10 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         const bool planned = (item.pad & kPlanned) != 0;
         const bool streamed = (item.pad & kStreamed) != 0;
         const uint2 *qrng = A.rng + item.qpoff;
-        const uint32_t *lo_row = planned && item.part > 0 ? A.bounds + item.bofs + (size_t)(item.part - 1) * qp : nullptr;
-        const uint32_t *hi_row = planned && item.part + 1 < item.nparts ? A.bounds + item.bofs + (size_t)item.part * qp
-                                                                        : nullptr;
+        // planned: token j's boundaries are bounds[bofs + j * (nparts - 1) + p - 1], p = 1 .. nparts - 1
+        const uint32_t *brow = planned ? A.bounds + item.bofs : nullptr;
+        const uint32_t bstride = planned ? item.nparts - 1 : 0u;
+        const bool has_lo = planned && item.part > 0, has_hi = planned && item.part + 1 < item.nparts;
         // a streamed item walks its nparts windows in order with one cursor per probe
         // token (LDS), starting at the token's first posting inside the item's window
         uint32_t *cur = rlo;
@@ -1028,8 +1032,9 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                 const uint2 r = qrng[jb + jj];   // the token's postings inside the whole window
                 uint32_t lo = r.x, hi = r.y;
                 if (planned) {                   // boundaries precomputed by k_bounds
-                    if (lo_row) lo = lo_row[jb + jj];
-                    if (hi_row) hi = hi_row[jb + jj];
+                    const uint32_t *bj = brow + (size_t)(jb + jj) * bstride + item.part;
+                    if (has_lo) lo = bj[-1];
+                    if (has_hi) hi = bj[0];
                 } else if (!whole) {             // a re-split item: narrow by search
                     lo = lower_bound_sid(A.psid, A.fence, r.x, r.y, wa);
                     hi = lower_bound_sid(A.psid, A.fence, lo, r.y, wb);
