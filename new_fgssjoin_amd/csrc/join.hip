@@ -179,8 +179,15 @@ struct fgss_device_sets {
     uint64_t *h_ctr = nullptr;    // pinned mirror of the counters
     uint32_t *h_misc = nullptr;   // pinned mirror of the device totals
     uint64_t *h_jctr = nullptr;   // pinned mirror of the per-job counters
-    hipGraph_t graph = nullptr;   // captured pipeline (batch runs)
-    hipGraphExec_t graph_exec = nullptr;
+    bool jctr_async = false;      // the run's tail copies the per-job counters into h_jctr
+    int32_t *h_out = nullptr;     // pinned mirror of the first out_copied result triples
+    size_t h_out_cap = 0;         // triples h_out holds
+    uint32_t out_hint = 0;        // triples the run's tail copies: the most a run has found, plus a margin
+    uint32_t out_copied = 0;      // triples the last run's tail copied
+    // captured pipeline stages (index build, planning, join kernel), replayed with the
+    // timing events and the counter copies enqueued between and after them
+    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};
     int graph_key[5] = {0, 0, 0, 0, 0};   // table slots, load, capacity generation, boundary budget, stream parts
     int cap_gen = 0;              // bumped whenever a buffer the pipeline uses is reallocated
     bool sized = false;           // a run has completed (first-run buffer sizing is over)
@@ -219,8 +226,10 @@ void fgss_device_release(fgss_collection *c)
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(d->device);
     if (d->st) (void)hipStreamSynchronize(d->st);
-    if (d->graph_exec) (void)hipGraphExecDestroy(d->graph_exec);
-    if (d->graph) (void)hipGraphDestroy(d->graph);
+    for (int k = 0; k < 3; k++) {
+        if (d->graph_exec[k]) (void)hipGraphExecDestroy(d->graph_exec[k]);
+        if (d->graph[k]) (void)hipGraphDestroy(d->graph[k]);
+    }
     DevBuf *bufs[] = {&d->b_psid, &d->b_fence, &d->b_bcnt, &d->b_bofs, &d->b_bounds, &d->b_probelen, &d->b_poff,
                       &d->b_rng, &d->b_eq, &d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib,
                       &d->b_sortscr, &d->b_sid_of, &d->b_post, &d->b_loff, &d->b_hi, &d->b_nparts, &d->b_partoff,
@@ -237,6 +246,7 @@ void fgss_device_release(fgss_collection *c)
     if (d->h_ctr) (void)hipHostFree(d->h_ctr);
     if (d->h_misc) (void)hipHostFree(d->h_misc);
     if (d->h_jctr) (void)hipHostFree(d->h_jctr);
+    if (d->h_out) (void)hipHostFree(d->h_out);
     if (d->st) (void)hipStreamDestroy(d->st);
     (void)hipSetDevice(cur);
     delete d;
@@ -878,6 +888,9 @@ __device__ __forceinline__ bool table_insert(const uint2 (&pe)[N], uint32_t valm
 #ifndef FGSS_UNROLL
 #define FGSS_UNROLL 4
 #endif
+#ifndef FGSS_PHASE_CLOCKS
+#define FGSS_PHASE_CLOCKS 1
+#endif
 #ifndef FGSS_TABLE_FACTOR
 #define FGSS_TABLE_FACTOR 2
 #endif
@@ -923,12 +936,30 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
     for (uint32_t i = lane; i < CAP; i += kWave) { tkey[i] = kEmpty; tcnt[i] = 0; tlast[i] = 0; }
     if (lane < kAccSlots) wacc[lane] = 0;
     wave_sync();
+// shader-clock time per phase, summed per wavefront (FGSS_PHASE_CLOCKS=0 builds
+// without the clock reads and their LDS updates)
+#if FGSS_PHASE_CLOCKS
+#define PHASE_MARK(k)                                  \
+    do {                                               \
+        c1 = clock64();                                \
+        WACC(W_CYC0 + (k), c1 - c0);                   \
+        c0 = c1;                                       \
+    } while (0)
+#define PHASE_START(r) const unsigned long long r = clock64()
+#define PHASE_ADD(k, r) WACC(W_CYC0 + (k), clock64() - (r))
+#else
+#define PHASE_MARK(k) do { } while (0)
+#define PHASE_START(r) do { } while (0)
+#define PHASE_ADD(k, r) do { } while (0)
+#endif
 #define WACC(k, v)                                     \
     do {                                               \
         if (lane == 0) wacc[(k)] += (unsigned long long)(v); \
     } while (0)
 
-    unsigned long long c0 = clock64(), c1;
+    unsigned long long c0 = clock64(), c1 = 0;
+    (void)c0;
+    (void)c1;
     // items are claimed kQueueBatch at a time: one global atomic per batch
     uint32_t batch_next = 0, batch_end = 0;
     uint32_t cur_job = kEmpty;
@@ -1005,7 +1036,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             wave_sync();
         }
         const uint32_t nwin = streamed ? item.nparts : 1u;
-        c1 = clock64(); WACC(W_CYC0 + 0, c1 - c0); c0 = c1;
+        PHASE_MARK(0);
 
         // a streamed item joins its parts a window of `span` consecutive parts at a
         // time: the span doubles while the windows leave the table at most 40% full
@@ -1027,7 +1058,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         uint32_t escan = 0, nlist = 0, jdone = 0;
         for (uint32_t jb = 0; !streamed && jb < qp && !overflow; jb += kTokBatch) {
             const uint32_t nb = min((uint32_t)kTokBatch, qp - jb);
-            const unsigned long long r0 = clock64();
+            PHASE_START(r0);
             for (uint32_t jj = lane; jj < nb; jj += kWave) {
                 const uint2 r = qrng[jb + jj];   // the token's postings inside the whole window
                 uint32_t lo = r.x, hi = r.y;
@@ -1065,7 +1096,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             const uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(inc, kWave - 1, kWave));
             if (lane == 0) rpre[nb] = total;
             wave_sync();
-            WACC(W_CYC0 + 5, clock64() - r0);
+            PHASE_ADD(5, r0);
             escan += total;
             const int steps = ceil_log2(nb);   // wave-uniform search depth
             // The posting loop runs without branches on per-lane conditions (on this
@@ -1251,7 +1282,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             }
             continue;
         }
-        c1 = clock64(); WACC(W_CYC0 + 1, c1 - c0); c0 = c1;
+        PHASE_MARK(1);
 
         // ---- B: positional filter over the distinct sources ----
         // (branch-free like phase A: lanes past the list work on its last entry and
@@ -1285,7 +1316,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         }
         cand = wave_sum(cand);
         wave_sync();
-        c1 = clock64(); WACC(W_CYC0 + 2, c1 - c0); c0 = c1;
+        PHASE_MARK(2);
 
         // ---- C: signature bound, one lane per candidate: 64 buckets, then 128 for the rest ----
         // (the source's bad-signature flag and its 64-bucket line are read together)
@@ -1373,7 +1404,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             }
             wave_sync();
         }
-        c1 = clock64(); WACC(W_CYC0 + 3, c1 - c0); c0 = c1;
+        PHASE_MARK(3);
 
         // ---- D: verification (checking.cu:93-175): overlap of S after the last prefix match ----
         uint32_t inter = 0, vtok = 0, nres = 0;
@@ -1434,7 +1465,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             tkey[h] = kEmpty; tcnt[h] = 0; tlast[h] = 0;
         }
         wave_sync();
-        c1 = clock64(); WACC(W_CYC0 + 4, c1 - c0); c0 = c1;
+        PHASE_MARK(4);
         WACC(W_JOB0 + 0, escan);
         WACC(W_JOB0 + 1, cand);
         WACC(W_JOB0 + 2, inter);
@@ -1715,10 +1746,21 @@ static int enqueue_index(fgss_device_sets *d, int rs)
     return FGSS_OK;
 }
 
-// Enqueue the whole join on d->st with no host synchronization (so it can be
-// captured into a HIP graph): index build, planning, the join kernel, and the
-// device->host copies of all counters into pinned memory.
-static int enqueue_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool timing)
+// The self-join pipeline on d->st in three stages with no host synchronization (so
+// that each can be captured into a HIP graph): the index build (counters cleared
+// first), the planning of work items, and the join kernel; then the device->host
+// copies of all counters into pinned memory.
+static int enqueue_stage_index(fgss_device_sets *d)
+{
+    hipStream_t st = d->st;
+    unsigned long long *ctr = P<unsigned long long>(d->b_ctr), *jctr = P<unsigned long long>(d->b_jctr);
+    HIPCK(fill<unsigned long long>(ctr, C_COUNT, 0ull, st));
+    HIPCK(fill<unsigned long long>(jctr, (int64_t)d->njobs * kJobCtr, 0ull, st));
+    HIPCK(fill<uint32_t>(P<uint32_t>(d->b_misc), M_COUNT, 0u, st));
+    return enqueue_index(d, 0);
+}
+
+static int enqueue_stage_plan(fgss_device_sets *d, const RunCfg &cfg)
 {
     hipStream_t st = d->st;
     const int64_t n = d->n;
@@ -1728,15 +1770,6 @@ static int enqueue_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool timing)
     uint32_t *hi = P<uint32_t>(d->b_hi), *eq = P<uint32_t>(d->b_eq), *nparts = P<uint32_t>(d->b_nparts),
              *partoff = P<uint32_t>(d->b_partoff);
     uint64_t *bcnt = P<uint64_t>(d->b_bcnt), *bofs = P<uint64_t>(d->b_bofs), *scr64 = P<uint64_t>(d->b_scanscr);
-    unsigned long long *ctr = P<unsigned long long>(d->b_ctr), *jctr = P<unsigned long long>(d->b_jctr);
-
-    if (timing) HIPCK(hipEventRecord(d->ev[0], st));
-    HIPCK(fill<unsigned long long>(ctr, C_COUNT, 0ull, st));
-    HIPCK(fill<unsigned long long>(jctr, (int64_t)d->njobs * kJobCtr, 0ull, st));
-    HIPCK(fill<uint32_t>(misc, M_COUNT, 0u, st));
-    int rc = enqueue_index(d, 0);
-    if (rc) return rc;
-    if (timing) HIPCK(hipEventRecord(d->ev[1], st));
     // 6-8: plan work items
     hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(grid_for(n * kPlanGroup, 256), 4096)), dim3(256), 0, st,
                        d->tok, d->pos, d->len, n, d->job, d->tj, d->jend, probelen, poff, loff, psid, fence, cfg.target,
@@ -1756,15 +1789,61 @@ static int enqueue_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool timing)
                        probelen, eq, partoff, bofs, bcnt, blim, d->job, d->tj,
                        d->len, d->pos, poff, n, P<Item>(d->b_items), d->cap_items);
     HIPCK(hipGetLastError());
-    // 9: the join
+    return FGSS_OK;
+}
+
+static int enqueue_stage_join(fgss_device_sets *d, const RunCfg &cfg)
+{
+    return launch_join_cap(d, join_args(d), cfg.cap);   // 9: the join
+}
+
+// The first out_hint result triples into pinned memory, behind the join on its stream
+// (the count is known on the device only; a run that finds more copies the rest
+// after it completes).  Never inside a graph capture: it may allocate.
+static int enqueue_out_copy(fgss_device_sets *d)
+{
+    const uint32_t want = std::min(d->cap_out, d->out_hint);
+    d->out_copied = 0;
+    if (want == 0) return FGSS_OK;
+    if (d->h_out_cap < want) {   // pinned allocations are slow: grow by doubling at least
+        const size_t cap = std::max<size_t>({(size_t)want, 2 * d->h_out_cap, 16384});
+        if (d->h_out) HIPCK(hipHostFree(d->h_out));
+        d->h_out = nullptr;
+        d->h_out_cap = 0;
+        HIPCK(hipHostMalloc((void **)&d->h_out, cap * 12, hipHostMallocDefault));
+        d->h_out_cap = cap;
+    }
+    HIPCK(hipMemcpyAsync(d->h_out, d->b_out.p, (size_t)want * 12, hipMemcpyDeviceToHost, d->st));
+    d->out_copied = want;
+    return FGSS_OK;
+}
+
+static int enqueue_copies(fgss_device_sets *d)
+{
+    hipStream_t st = d->st;
+    HIPCK(hipMemcpyAsync(d->h_ctr, d->b_ctr.p, C_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    HIPCK(hipMemcpyAsync(d->h_misc, d->b_misc.p, M_COUNT * 4, hipMemcpyDeviceToHost, st));
+    HIPCK(hipMemcpyAsync(d->h_jctr, d->b_jctr.p, (size_t)d->njobs * kJobCtr * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                         st));
+    d->jctr_async = true;
+    return enqueue_out_copy(d);
+}
+
+// The whole join, eager; timing events around the stages (ev[0], ev[1], ev[4], ev[5]).
+static int enqueue_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool timing)
+{
+    hipStream_t st = d->st;
+    if (timing) HIPCK(hipEventRecord(d->ev[0], st));
+    int rc = enqueue_stage_index(d);
+    if (rc) return rc;
+    if (timing) HIPCK(hipEventRecord(d->ev[1], st));
+    rc = enqueue_stage_plan(d, cfg);
+    if (rc) return rc;
     if (timing) HIPCK(hipEventRecord(d->ev[4], st));
-    rc = launch_join_cap(d, join_args(d), cfg.cap);
+    rc = enqueue_stage_join(d, cfg);
     if (rc) return rc;
     if (timing) HIPCK(hipEventRecord(d->ev[5], st));
-    HIPCK(hipMemcpyAsync(d->h_ctr, ctr, C_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    HIPCK(hipMemcpyAsync(d->h_misc, misc, M_COUNT * 4, hipMemcpyDeviceToHost, st));
-    HIPCK(hipMemcpyAsync(d->h_jctr, jctr, (size_t)d->njobs * kJobCtr * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    return FGSS_OK;
+    return enqueue_copies(d);
 }
 
 // FGSS_TRACE=1 prints a host-side timeline of each join to stderr
@@ -1790,16 +1869,20 @@ struct RunTotals {
 
 static int capture_graph(fgss_device_sets *d, const RunCfg &cfg)
 {
-    if (d->graph_exec) { HIPCK(hipGraphExecDestroy(d->graph_exec)); d->graph_exec = nullptr; }
-    if (d->graph) { HIPCK(hipGraphDestroy(d->graph)); d->graph = nullptr; }
-    HIPCK(hipStreamBeginCapture(d->st, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_pipeline(d, cfg, false);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(d->st, &g);
-    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-    if (e != hipSuccess) return fgss_fail(FGSS_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
-    d->graph = g;
-    HIPCK(hipGraphInstantiate(&d->graph_exec, g, nullptr, nullptr, 0));
+    for (int k = 0; k < 3; k++) {
+        if (d->graph_exec[k]) { HIPCK(hipGraphExecDestroy(d->graph_exec[k])); d->graph_exec[k] = nullptr; }
+        if (d->graph[k]) { HIPCK(hipGraphDestroy(d->graph[k])); d->graph[k] = nullptr; }
+    }
+    for (int k = 0; k < 3; k++) {
+        HIPCK(hipStreamBeginCapture(d->st, hipStreamCaptureModeThreadLocal));
+        int rc = k == 0 ? enqueue_stage_index(d) : k == 1 ? enqueue_stage_plan(d, cfg) : enqueue_stage_join(d, cfg);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(d->st, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return fgss_fail(FGSS_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        d->graph[k] = g;
+        HIPCK(hipGraphInstantiate(&d->graph_exec[k], g, nullptr, nullptr, 0));
+    }
     d->graph_key[0] = cfg.cap;
     d->graph_key[1] = (int)cfg.target;
     d->graph_key[2] = d->cap_gen;
@@ -1836,14 +1919,20 @@ static int launch_once(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &op
     if (use_graph && ops.graph_ok) {
         rc = launch_join_cap(d, ops.args(), cfg.cap, true, ops.rs);   // launch configuration, outside the capture
         if (rc) return rc;
-        if (!d->graph_exec || d->graph_key[0] != cfg.cap || d->graph_key[1] != (int)cfg.target ||
+        if (!d->graph_exec[0] || d->graph_key[0] != cfg.cap || d->graph_key[1] != (int)cfg.target ||
             d->graph_key[2] != d->cap_gen || d->graph_key[3] != (int)cfg.bounds ||
             d->graph_key[4] != (int)cfg.stream_parts) {
             rc = capture_graph(d, cfg);
             if (rc) return rc;
         }
-        HIPCK(hipGraphLaunch(d->graph_exec, d->st));
-        return FGSS_OK;
+        // the stages' graphs with the timing events between them, then the copies
+        const int evk[3] = {0, 1, 4};
+        for (int k = 0; k < 3; k++) {
+            HIPCK(hipEventRecord(d->ev[evk[k]], d->st));
+            HIPCK(hipGraphLaunch(d->graph_exec[k], d->st));
+        }
+        HIPCK(hipEventRecord(d->ev[5], d->st));
+        return enqueue_copies(d);
     }
     return ops.enqueue(true);
 }
@@ -1894,7 +1983,7 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
     T.launches = 1;
     T.index_entries = d->h_misc[M_NE];
     T.probe_tokens = d->h_misc[M_NP];
-    if (!T.graph) {
+    {   // events recorded around the stages in both modes
         float ms = 0;
         if (hipEventElapsedTime(&ms, d->ev[0], d->ev[1]) == hipSuccess) T.ms_index = ms;
         if (hipEventElapsedTime(&ms, d->ev[1], d->ev[5]) == hipSuccess) T.ms_join = ms;
@@ -1903,7 +1992,9 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
     for (int k = C_ESCAN; k <= C_RETRY; k++) T.acc[k] += d->h_ctr[k];
     uint64_t nres = d->h_ctr[C_RESULTS], novf = d->h_ctr[C_OVERFLOW];
     tri.resize(3 * (size_t)nres);
-    if (nres) HIPCK(hipMemcpy(tri.data(), d->b_out.p, (size_t)nres * 12, hipMemcpyDeviceToHost));
+    if (nres && nres <= d->out_copied) memcpy(tri.data(), d->h_out, (size_t)nres * 12);
+    else if (nres) HIPCK(hipMemcpy(tri.data(), d->b_out.p, (size_t)nres * 12, hipMemcpyDeviceToHost));
+    d->out_hint = std::max<uint32_t>(d->out_hint, (uint32_t)std::min<uint64_t>(nres + nres / 4 + 1024, UINT32_MAX / 16));
 
     // overflowed work items: split each four ways and run them again
     const Item *cur = P<Item>(d->b_items);
@@ -1961,8 +2052,10 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
         tri.resize(old + 3 * (size_t)nres);
         if (nres) HIPCK(hipMemcpy(tri.data() + old, d->b_out.p, (size_t)nres * 12, hipMemcpyDeviceToHost));
     }
-    // per-job counters (overflowed items never flushed theirs)
-    HIPCK(hipMemcpy(d->h_jctr, d->b_jctr.p, (size_t)d->njobs * kJobCtr * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    // per-job counters: the run's tail copied them unless rounds ran after it (the
+    // items that overflowed had not flushed theirs)
+    if (round > 0 || !d->jctr_async)
+        HIPCK(hipMemcpy(d->h_jctr, d->b_jctr.p, (size_t)d->njobs * kJobCtr * sizeof(uint64_t), hipMemcpyDeviceToHost));
     for (size_t k = 0; k < T.jctr.size(); k++) T.jctr[k] = d->h_jctr[k];
     trace_mark("collect");
     return FGSS_OK;
@@ -2095,6 +2188,9 @@ extern "C" int fgss_selfjoin(fgss_collection *c, float threshold, const fgss_joi
     if (rc) return rc;
     std::vector<int32_t> tri;
     RunTotals T;
+    // eager launches: replaying the stages as captured HIP graphs measured the same
+    // (the dependent launches of a join cost a few microseconds each), and a one-off
+    // join would pay for the capture
     rc = run_pipeline(d, cfg, false, tri, T);
     if (rc) return rc;
     auto t2 = std::chrono::steady_clock::now();
@@ -2647,8 +2743,8 @@ static int enqueue_rs(fgss_server *srv, ServeSlot &sl, bool timing)
     int rc = launch_join_cap(d, rs_args(srv, sl), cfg.cap, false, true);
     if (rc) return rc;
     if (timing) HIPCK(hipEventRecord(d->ev[5], st));
-    HIPCK(hipMemcpyAsync(d->h_ctr, ctr, C_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    HIPCK(hipMemcpyAsync(d->h_misc, misc, M_COUNT * 4, hipMemcpyDeviceToHost, st));
+    rc = enqueue_copies(d);
+    if (rc) return rc;
     HIPCK(hipEventRecord(sl.done, st));
     return FGSS_OK;
 }
@@ -2740,6 +2836,13 @@ static int slot_reserve(ServeSlot &sl, int64_t n, int64_t ntok, uint32_t jobs)
     // (256 MB) plan the heaviest runs of a 2M-title index without growing
     d->cap_bounds = std::max<uint32_t>(d->cap_bounds, std::min<uint32_t>(1u << 26, kBoundsBudget));
     d->sized = true;   // runs that outgrow a reserved capacity still complete
+    if (d->h_out_cap < (1u << 18)) {   // pinned mirror of a run's first 2^18 result triples (3 MB)
+        if (d->h_out) HIPCK(hipHostFree(d->h_out));
+        d->h_out = nullptr;
+        d->h_out_cap = 0;
+        HIPCK(hipHostMalloc((void **)&d->h_out, (size_t)(1u << 18) * 12, hipHostMallocDefault));
+        d->h_out_cap = 1u << 18;
+    }
     const int rc = slot_prepare(sl);
     d->n = n0; d->ntok = ntok0; d->njobs = nj0;
     return rc;

@@ -42,6 +42,8 @@ def test_batch_equals_plain_runs(fg, oracle_mod):
     for graph in (False, True, True, True):   # eager, capture + replay, two more replays
         _say(f"batch run graph={graph}")
         st = b.run(graph=graph)
+        colls[1].selfjoin(ts[1])   # eager device work and a host sync between replays
+        fg.device_synchronize()
         assert st["graph"] == (1 if graph else 0)
         assert st["n_pairs"] == sum(len(p) for p in plain)
         assert st["candidates"] == sum(p.stats["candidates"] for p in plain)

@@ -103,6 +103,21 @@ def test_repeatable(fg):
     assert a.stats["candidates"] == b.stats["candidates"]
 
 
+def test_repeated_joins_equal_the_oracle(fg, oracle_mod):
+    """Joins of one collection that reuse its device workspace, across thresholds and
+    with another collection's joins and host syncs in between, each equal the oracle."""
+    coll = fg.Collection.generate(fg.GEN_TITLES, 6000, 21)
+    other = fg.Collection.generate(fg.GEN_TITLES, 3000, 22)
+    sets = oracle_sets(oracle_mod, coll)
+    want = {t: oracle_mod.selfjoin(sets, t) for t in (0.6, 0.8)}
+    for rep in range(3):
+        for t in (0.6, 0.8, 0.6):
+            _say(f"graph replay rep={rep} t={t}")
+            assert_join_equal(oracle_mod, sets, coll.selfjoin(t), t, counters=want[t])
+            other.selfjoin(0.7)
+            fg.device_synchronize()
+
+
 def test_cli_outputs(fg, oracle_mod, tmp_path):
     """The CLI's stdout lines and stderr counters, both input forms."""
     import subprocess
