@@ -785,12 +785,6 @@ struct JoinArgs {
     unsigned long long *ctr;
 };
 
-__device__ __forceinline__ void flush_job(unsigned long long *ctr_job, uint32_t j, const volatile unsigned long long *acc)
-{
-    for (int k = 0; k < kJobCtr; k++)
-        if (acc[k]) atomicAdd(&ctr_job[(size_t)j * kJobCtr + k], acc[k]);
-}
-
 __device__ __forceinline__ uint32_t slot_hash(uint32_t s, int log2cap)
 {
     return (s * 0x9E3779B1u) >> (32 - log2cap);
@@ -934,7 +928,6 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
     const uint32_t n_items = *A.n_items_dev;
 
     for (uint32_t i = lane; i < CAP; i += kWave) { tkey[i] = kEmpty; tcnt[i] = 0; tlast[i] = 0; }
-    if (lane < kAccSlots) wacc[lane] = 0;
     wave_sync();
 // shader-clock time per phase, summed per wavefront (FGSS_PHASE_CLOCKS=0 builds
 // without the clock reads and their LDS updates)
@@ -952,9 +945,13 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
 #define PHASE_START(r) do { } while (0)
 #define PHASE_ADD(k, r) do { } while (0)
 #endif
-#define WACC(k, v)                                     \
-    do {                                               \
-        if (lane == 0) wacc[(k)] += (unsigned long long)(v); \
+    // The wavefront's counters live in one 64-bit register, counter k in lane k: an
+    // update (of a wave-uniform value) is a compare and a select, no branch or LDS
+    // round trip.
+    unsigned long long wv = 0;
+#define WACC(k, v)                                                             \
+    do {                                                                       \
+        wv += (uint32_t)lane == (uint32_t)(k) ? (unsigned long long)(v) : 0ull; \
     } while (0)
 
     unsigned long long c0 = clock64(), c1 = 0;
@@ -1000,10 +997,15 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         const uint32_t jq = item.job;
         const float t = item.t;
         if (jq != cur_job) {   // per-job counters: flush when the job changes
-            if (lane == 0) {
-                if (cur_job != kEmpty) flush_job(A.ctr_job, cur_job, wacc + W_JOB0);
-                for (int k = 0; k < kJobCtr; k++) wacc[W_JOB0 + k] = 0;
+            if (cur_job != kEmpty) {
+#pragma unroll
+                for (int k = 0; k < kJobCtr; k++) {
+                    const unsigned long long v = (unsigned long long)__builtin_amdgcn_readlane((uint32_t)wv, W_JOB0 + k) |
+                                                 ((unsigned long long)__builtin_amdgcn_readlane((uint32_t)(wv >> 32), W_JOB0 + k) << 32);
+                    if (lane == 0 && v) atomicAdd(&A.ctr_job[(size_t)cur_job * kJobCtr + k], v);
+                }
             }
+            wv = lane >= W_JOB0 && lane < W_JOB0 + kJobCtr ? 0ull : wv;
             cur_job = jq;
         }
         const uint32_t lq = item.lq & 0xFFFFu;
@@ -1301,7 +1303,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             const uint32_t rq = lq - js, rs = ls - ss;
             const uint32_t minrem = rq < rs ? rq : rs;
             const bool pass = in && !((float)(m + minrem) < bound);   // filtering.cu:127-133 at the last match
-            cand += pass;
+            cand += (uint32_t)__popcll(__ballot(pass));
             // overlap can still reach floor(bound): m + min(|Q|-js-1, |S|-ss-1)
             const bool keep = pass && (int)(m + minrem - 1) >= (int)floorf(bound);
             const bool drop = in && !keep;
@@ -1314,7 +1316,6 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             *(keep ? &clist[ncand + below] : &qstage[lane]) = ent;
             ncand += (uint32_t)__popcll(bal);
         }
-        cand = wave_sum(cand);
         wave_sync();
         PHASE_MARK(2);
 
@@ -1485,7 +1486,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
     // counters at the end of the launch serialize); k_wave_reduce sums the slots
     wave_sync();
     unsigned long long *slot = A.wctr + ((size_t)blockIdx.x * join_waves(CAP) + (size_t)w) * kWaveSlot;
-    if (lane < kAccSlots) slot[lane] = wacc[lane];
+    if (lane < kAccSlots) slot[lane] = wv;
     if (lane == kAccSlots) slot[lane] = cur_job;
 }
 
