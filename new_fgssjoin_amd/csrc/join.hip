@@ -171,7 +171,7 @@ struct fgss_device_sets {
     uint32_t *jend = nullptr;   // per job: one past its last record
     float *tj = nullptr;        // per job: threshold of the current run
     DevBuf b_psid, b_fence, b_bcnt, b_bofs, b_bounds, b_probelen, b_poff, b_rng, b_eq, b_midlen, b_midoff, b_ka,
-        b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_post, b_loff, b_hi, b_nparts, b_partoff, b_scanscr, b_items,
+        b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_pl_of, b_post, b_loff, b_hi, b_nparts, b_partoff, b_scanscr, b_items,
         b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc, b_nitems, b_ovfp, b_wctr;
     uint32_t cap_items = 0, cap_bounds = 0, cap_out = 0;   // current capacities (elements)
     uint32_t cap_ovf = 0;         // overflow records (a streamed item can record one per part)
@@ -232,7 +232,7 @@ void fgss_device_release(fgss_collection *c)
     }
     DevBuf *bufs[] = {&d->b_psid, &d->b_fence, &d->b_bcnt, &d->b_bofs, &d->b_bounds, &d->b_probelen, &d->b_poff,
                       &d->b_rng, &d->b_eq, &d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib,
-                      &d->b_sortscr, &d->b_sid_of, &d->b_post, &d->b_loff, &d->b_hi, &d->b_nparts, &d->b_partoff,
+                      &d->b_sortscr, &d->b_sid_of, &d->b_pl_of, &d->b_post, &d->b_loff, &d->b_hi, &d->b_nparts, &d->b_partoff,
                       &d->b_scanscr, &d->b_items, &d->b_items2, &d->b_ovf, &d->b_out, &d->b_ctr, &d->b_jctr,
                       &d->b_nitems, &d->b_ovfp, &d->b_wctr,
                       &d->b_misc};
@@ -416,32 +416,50 @@ __global__ void k_midlen(const uint32_t *__restrict__ len, const uint32_t *__res
     }
 }
 
-// thread per record: keys[e] = token, sid_of[e] = record, for e in its mid-prefix range
-__global__ void k_entries(const uint32_t *__restrict__ tok, const uint32_t *__restrict__ pos,
-                          const uint32_t *__restrict__ midlen, const uint32_t *__restrict__ midoff, int64_t n,
-                          uint32_t *__restrict__ keys, uint32_t *__restrict__ sid_of)
+// Index entries in record order: for e in record r's indexed prefix, keys[e] = the
+// token, sid_of[e] = r and pl_of[e] = its position | |r| << 16.  A block takes 256
+// consecutive records, whose entries [midoff[r0], midoff[r0 + 256]) are contiguous,
+// one entry per thread (coalesced stores); a thread finds its record among the
+// block's by a search over their offsets in LDS.
+__global__ __launch_bounds__(256) void k_entries(const uint32_t *__restrict__ tok, const uint32_t *__restrict__ pos,
+                                                 const uint32_t *__restrict__ len,
+                                                 const uint32_t *__restrict__ midoff,
+                                                 const uint32_t *__restrict__ nE_dev, int64_t n,
+                                                 uint32_t *__restrict__ keys, uint32_t *__restrict__ sid_of,
+                                                 uint32_t *__restrict__ pl_of)
 {
-    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    uint32_t m = midlen[s], o = midoff[s], p = pos[s];
-    for (uint32_t j = 0; j < m; j++) {
-        keys[o + j] = tok[p + j];
-        sid_of[o + j] = (uint32_t)s;
+    __shared__ uint32_t soff[257], spos[256], slen[256];
+    const int64_t r0 = (int64_t)blockIdx.x * 256;
+    if (r0 >= n) return;
+    const int nr = (int)min<int64_t>(256, n - r0);
+    for (int i = threadIdx.x; i <= nr; i += 256) soff[i] = r0 + i < n ? midoff[r0 + i] : *nE_dev;
+    for (int i = threadIdx.x; i < nr; i += 256) { spos[i] = pos[r0 + i]; slen[i] = len[r0 + i]; }
+    __syncthreads();
+    for (uint32_t e = soff[0] + threadIdx.x; e < soff[nr]; e += 256) {
+        int lo = 0, hi = nr - 1;   // the last record of the block whose entries start at or before e
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (soff[mid] <= e) lo = mid;
+            else hi = mid - 1;
+        }
+        const uint32_t j = e - soff[lo];
+        keys[e] = tok[spos[lo] + j];
+        sid_of[e] = (uint32_t)(r0 + lo);
+        pl_of[e] = j | (slen[lo] << 16);
     }
 }
 
 // postings in term order, packed: x = record id, y = token position | record length << 16
 __global__ void k_postings(const uint32_t *__restrict__ sidx, const uint32_t *__restrict__ sid_of,
-                           const uint32_t *__restrict__ midoff, const uint32_t *__restrict__ len,
-                           const uint32_t *__restrict__ nE_dev, uint2 *__restrict__ post, uint32_t *__restrict__ psid,
-                           uint32_t *__restrict__ fence)
+                           const uint32_t *__restrict__ pl_of, const uint32_t *__restrict__ nE_dev,
+                           uint2 *__restrict__ post, uint32_t *__restrict__ psid, uint32_t *__restrict__ fence)
 {
     const uint32_t nE = *nE_dev;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nE) return;
-    uint32_t e = sidx[i];
-    uint32_t s = sid_of[e];
-    post[i] = make_uint2(s, (e - midoff[s]) | (len[s] << 16));
+    const uint32_t e = sidx[i];
+    const uint32_t s = sid_of[e];
+    post[i] = make_uint2(s, pl_of[e]);
     psid[i] = s;
     if ((i & (kFence - 1)) == 0) fence[i / kFence] = s;
 }
@@ -1662,6 +1680,7 @@ static int prepare_buffers(fgss_device_sets *d)
     GROW(d->b_ia, (size_t)E * 4);
     GROW(d->b_ib, (size_t)E * 4);
     GROW(d->b_sid_of, (size_t)E * 4);
+    GROW(d->b_pl_of, (size_t)E * 4);
     // kWave spare postings: a streamed round reads up to 63 postings past the end of a
     // token's list before it tests them against the end
     GROW(d->b_post, (size_t)(E + kWave) * 8);
@@ -1731,7 +1750,8 @@ static int enqueue_index(fgss_device_sets *d, int rs)
     HIPCK(hipGetLastError());
     HIPCK(exclusive_scan<uint32_t>(midlen, midoff, n, misc + M_NE, scr, st));
     HIPCK(exclusive_scan<uint32_t>(probelen, poff, n, misc + M_NP, scr, st));
-    hipLaunchKernelGGL(k_entries, dim3(grid_for(n, 256)), dim3(256), 0, st, d->tok, d->pos, midlen, midoff, n, ka, sid_of);
+    hipLaunchKernelGGL(k_entries, dim3(grid_for(n, 256)), dim3(256), 0, st, d->tok, d->pos, d->len, midoff, misc + M_NE,
+                       n, ka, sid_of, P<uint32_t>(d->b_pl_of));
     HIPCK(hipGetLastError());
     int bits = 1;
     while (bits < 32 && (1ll << bits) < (long long)d->n_terms) bits++;
@@ -1739,8 +1759,8 @@ static int enqueue_index(fgss_device_sets *d, int rs)
     HIPCK(radix_sort_pairs(ka, ia, kb, ib, E, misc + M_NE, bits, P<uint32_t>(d->b_sortscr), st, &which));
     const uint32_t *skeys = which ? kb : ka;
     const uint32_t *sidx = which ? ib : ia;
-    hipLaunchKernelGGL(k_postings, dim3(grid_for(E, 256)), dim3(256), 0, st, sidx, sid_of, midoff, d->len, misc + M_NE,
-                       post, psid, fence);
+    hipLaunchKernelGGL(k_postings, dim3(grid_for(E, 256)), dim3(256), 0, st, sidx, sid_of, P<uint32_t>(d->b_pl_of),
+                       misc + M_NE, post, psid, fence);
     HIPCK(hipGetLastError());
     hipLaunchKernelGGL(k_list_offsets, dim3(grid_for(E + 1, 256)), dim3(256), 0, st, skeys, misc + M_NE, d->n_terms, loff);
     HIPCK(hipGetLastError());
@@ -3024,7 +3044,7 @@ extern "C" int fgss_server_create(const fgss_collection *indexed, float t_min, i
     srv->mean_list = x->n_terms ? std::max(1.0, (double)x->h_misc[M_NE] / (double)x->n_terms) : 1.0;
     // the self-join workspace of the index is no longer needed: keep postings,
     // list offsets, ids and fences only
-    DevBuf *spare[] = {&x->b_ka, &x->b_kb, &x->b_ia, &x->b_ib, &x->b_sid_of, &x->b_sortscr, &x->b_rng,
+    DevBuf *spare[] = {&x->b_ka, &x->b_kb, &x->b_ia, &x->b_ib, &x->b_sid_of, &x->b_pl_of, &x->b_sortscr, &x->b_rng,
                        &x->b_items, &x->b_ovf, &x->b_bounds, &x->b_out};
     for (DevBuf *b : spare)
         if (b->p) { (void)hipFree(b->p); b->p = nullptr; b->cap = 0; }
