@@ -1196,7 +1196,11 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                 nact += (uint32_t)__popcll(m);
             }
             wave_sync();
-            uint32_t logK = 2;
+            // first round: K = 8 lanes per token.  A round waits for the cursor moves of
+            // the one before it, so fewer rounds beat fewer idle lanes up to a point
+            // (1M long records at t = 0.8, about 2.4 postings per token and part:
+            // K = 4 2.11 s, 8 2.02 s, 16 2.51 s; starting at 1 2.59 s)
+            uint32_t logK = 3;
             while (nact > 0 && !overflow) {
                 if (lane < (int)(kStreamQp / 32)) satm[lane] = 0;
                 wave_sync();
