@@ -232,7 +232,9 @@ typedef struct {
  * flight at once, each on its own HIP stream; `budget_bytes` (<= 0: 16 GiB) caps
  * the work admitted in flight, estimated at 16 B per posting a request's probe
  * prefixes may reach (prefix tokens x the index's mean list length).  Larger
- * budgets batch more requests per run: more throughput, longer queueing. */
+ * budgets batch more requests per run: more throughput, longer queueing.  Each slot's
+ * workspace is allocated here, for runs of up to 65,536 probes of the index's mean
+ * length, so that serving does not allocate; larger runs grow it. */
 int fgss_server_create(const fgss_collection *indexed, float t_min, int32_t slots, int64_t budget_bytes,
                        const fgss_join_opts *opts, fgss_server **out);
 /* Queue an R-S join of `probes` (a plain collection in the index's token space,
