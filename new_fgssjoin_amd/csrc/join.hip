@@ -518,6 +518,36 @@ __device__ __forceinline__ uint32_t lower_bound_sid(const uint32_t *__restrict__
     return a;
 }
 
+// lower_bound_sid for two values v1 <= v2 in the same range, the two searches
+// advanced together so that their dependent loads overlap (results r1 <= r2)
+__device__ __forceinline__ void lower_bound_sid2(const uint32_t *__restrict__ psid, const uint32_t *__restrict__ fence,
+                                                 uint32_t lo, uint32_t hi, uint32_t v1, uint32_t v2, uint32_t &r1,
+                                                 uint32_t &r2)
+{
+    if (lo >= hi) { r1 = r2 = lo; return; }
+    const uint32_t f0 = (lo + kFence - 1) / kFence, f1 = (hi - 1) / kFence + 1;
+    uint32_t a1 = f0, b1 = f1, a2 = f0, b2 = f1;   // first fence entry with id >= v
+    while (a1 < b1 || a2 < b2) {
+        const uint32_t m1 = (a1 + b1) >> 1, m2 = (a2 + b2) >> 1;
+        const uint32_t x1 = a1 < b1 ? fence[m1] : 0u, x2 = a2 < b2 ? fence[m2] : 0u;
+        if (a1 < b1) { if (x1 < v1) a1 = m1 + 1; else b1 = m1; }
+        if (a2 < b2) { if (x2 < v2) a2 = m2 + 1; else b2 = m2; }
+    }
+    // each answer lies in [max(lo, (a-1)*kFence), min(hi, a*kFence)]
+    uint32_t p1 = a1 > 0 ? (a1 - 1) * kFence : 0, q1 = a1 * kFence;
+    uint32_t p2 = a2 > 0 ? (a2 - 1) * kFence : 0, q2 = a2 * kFence;
+    p1 = max(p1, lo); q1 = min(q1, hi);
+    p2 = max(p2, lo); q2 = min(q2, hi);
+    while (p1 < q1 || p2 < q2) {
+        const uint32_t m1 = (p1 + q1) >> 1, m2 = (p2 + q2) >> 1;
+        const uint32_t x1 = p1 < q1 ? psid[m1] : 0u, x2 = p2 < q2 ? psid[m2] : 0u;
+        if (p1 < q1) { if (x1 < v1) p1 = m1 + 1; else q1 = m1; }
+        if (p2 < q2) { if (x2 < v2) p2 = m2 + 1; else q2 = m2; }
+    }
+    r1 = p1;
+    r2 = p2;
+}
+
 // First index in [lo, hi) whose record id is >= v when it is expected close to lo:
 // doubling steps from lo, then a binary search inside the last step (a streamed
 // item moves each cursor by a few postings per part).  It reads the postings
@@ -598,8 +628,8 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
         for (uint32_t j = gl; j < qp; j += kPlanGroup) {
             uint32_t term = tok[pos[q] + j];
             uint32_t L0 = loff[term], L1 = loff[term + 1];
-            uint32_t a = lower_bound_sid(psid, fence, L0, L1, (uint32_t)q + 1);
-            uint32_t b = lower_bound_sid(psid, fence, a, L1, wend);
+            uint32_t a, b;   // window [q + 1, wend), wend >= q + 1
+            lower_bound_sid2(psid, fence, L0, L1, (uint32_t)q + 1, wend, a, b);
             rng[ro + j] = make_uint2(a, b);
             e += b - a;
         }
@@ -908,18 +938,18 @@ __device__ __forceinline__ bool table_insert(const uint2 (&pe)[N], uint32_t valm
 #endif
 constexpr int kUnroll = FGSS_UNROLL;
 constexpr int kQStage = 160;   // query tokens staged in LDS for verification (longer queries stay in HBM)
-// per-wave counters kept in LDS (not registers): totals, shader-clock phases, current job
+// per-wave counters (lane k of one 64-bit register holds counter k): totals,
+// shader-clock phases, the current job's counters
 enum WaveAcc { W_ESCAN = 0, W_CAND, W_SIG, W_VER, W_SIG2, W_INTER, W_VTOK, W_RETRY, W_CYC0, W_JOB0 = W_CYC0 + 6,
                kAccSlots = W_JOB0 + kJobCtr };
 // per-wavefront slot in global memory at the end of a launch: the kAccSlots
 // counters and the wavefront's last job
 constexpr int kWaveSlot = 32;
 static_assert(kAccSlots + 1 <= kWaveSlot && kAccSlots + 1 <= kWave, "slot layout");
-// LDS words per wavefront: counters (u64), table (key, count, last match), list,
-// token ranges, query stage; even so that the next wave's counters stay 8-byte aligned
+// LDS words per wavefront: table (key, count, last match), list, token ranges, query stage
 constexpr uint32_t join_wave_words(int cap)
 {
-    return (2u * kAccSlots + 4u * cap + 2u * kTokBatch + 1u + kQStage + 1u) & ~1u;
+    return 4u * cap + 2u * kTokBatch + 1u + kQStage;
 }
 
 // RS: an R-S join against an index holding max_prefix(|S|, t_min) tokens per source;
@@ -932,8 +962,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
     extern __shared__ uint32_t smem[];
     const int w = (int)threadIdx.x / kWave;
     const int lane = lane_id();
-    unsigned long long *wacc = (unsigned long long *)(smem + (size_t)w * join_wave_words(CAP));
-    uint32_t *tkey = (uint32_t *)(wacc + kAccSlots);
+    uint32_t *tkey = smem + (size_t)w * join_wave_words(CAP);
     uint32_t *tcnt = tkey + CAP;
     uint32_t *tlast = tcnt + CAP;
     uint32_t *clist = tlast + CAP;        // slot | |S| << 16, one per distinct source
@@ -2605,8 +2634,7 @@ __global__ __launch_bounds__(256) void k_plan_rs(const uint32_t *__restrict__ pt
             uint32_t x = 0, y = 0;
             if (term < n_terms && wlo < whi) {   // a q-gram the index has never seen has no postings
                 const uint32_t L0 = loff[term], L1 = loff[term + 1];
-                x = lower_bound_sid(psid, fence, L0, L1, wlo);
-                y = lower_bound_sid(psid, fence, x, L1, whi);
+                lower_bound_sid2(psid, fence, L0, L1, wlo, whi, x, y);   // wlo < whi here
             }
             rng[ro + j] = make_uint2(x, y);
             e += y - x;
