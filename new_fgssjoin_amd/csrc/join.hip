@@ -518,15 +518,15 @@ __device__ __forceinline__ uint32_t lower_bound_sid(const uint32_t *__restrict__
     return a;
 }
 
-// lower_bound_sid for two values v1 <= v2 in the same range, the two searches
-// advanced together so that their dependent loads overlap (results r1 <= r2)
+// lower_bound_sid for two (range, value) pairs at once: the two searches advance
+// together, so that their dependent loads overlap
 __device__ __forceinline__ void lower_bound_sid2(const uint32_t *__restrict__ psid, const uint32_t *__restrict__ fence,
-                                                 uint32_t lo, uint32_t hi, uint32_t v1, uint32_t v2, uint32_t &r1,
-                                                 uint32_t &r2)
+                                                 uint32_t lo1, uint32_t hi1, uint32_t v1, uint32_t lo2, uint32_t hi2,
+                                                 uint32_t v2, uint32_t &r1, uint32_t &r2)
 {
-    if (lo >= hi) { r1 = r2 = lo; return; }
-    const uint32_t f0 = (lo + kFence - 1) / kFence, f1 = (hi - 1) / kFence + 1;
-    uint32_t a1 = f0, b1 = f1, a2 = f0, b2 = f1;   // first fence entry with id >= v
+    // first fence entry with id >= v inside each range (an empty range stays empty)
+    uint32_t a1 = (lo1 + kFence - 1) / kFence, b1 = lo1 < hi1 ? (hi1 - 1) / kFence + 1 : a1;
+    uint32_t a2 = (lo2 + kFence - 1) / kFence, b2 = lo2 < hi2 ? (hi2 - 1) / kFence + 1 : a2;
     while (a1 < b1 || a2 < b2) {
         const uint32_t m1 = (a1 + b1) >> 1, m2 = (a2 + b2) >> 1;
         const uint32_t x1 = a1 < b1 ? fence[m1] : 0u, x2 = a2 < b2 ? fence[m2] : 0u;
@@ -534,10 +534,10 @@ __device__ __forceinline__ void lower_bound_sid2(const uint32_t *__restrict__ ps
         if (a2 < b2) { if (x2 < v2) a2 = m2 + 1; else b2 = m2; }
     }
     // each answer lies in [max(lo, (a-1)*kFence), min(hi, a*kFence)]
-    uint32_t p1 = a1 > 0 ? (a1 - 1) * kFence : 0, q1 = a1 * kFence;
-    uint32_t p2 = a2 > 0 ? (a2 - 1) * kFence : 0, q2 = a2 * kFence;
-    p1 = max(p1, lo); q1 = min(q1, hi);
-    p2 = max(p2, lo); q2 = min(q2, hi);
+    uint32_t p1 = max(a1 > 0 ? (a1 - 1) * kFence : 0u, lo1), q1 = min(a1 * kFence, hi1);
+    uint32_t p2 = max(a2 > 0 ? (a2 - 1) * kFence : 0u, lo2), q2 = min(a2 * kFence, hi2);
+    if (lo1 >= hi1) p1 = q1 = lo1;
+    if (lo2 >= hi2) p2 = q2 = lo2;
     while (p1 < q1 || p2 < q2) {
         const uint32_t m1 = (p1 + q1) >> 1, m2 = (p2 + q2) >> 1;
         const uint32_t x1 = p1 < q1 ? psid[m1] : 0u, x2 = p2 < q2 ? psid[m2] : 0u;
@@ -629,7 +629,7 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
             uint32_t term = tok[pos[q] + j];
             uint32_t L0 = loff[term], L1 = loff[term + 1];
             uint32_t a, b;   // window [q + 1, wend), wend >= q + 1
-            lower_bound_sid2(psid, fence, L0, L1, (uint32_t)q + 1, wend, a, b);
+            lower_bound_sid2(psid, fence, L0, L1, (uint32_t)q + 1, L0, L1, wend, a, b);
             rng[ro + j] = make_uint2(a, b);
             e += b - a;
         }
@@ -2634,7 +2634,7 @@ __global__ __launch_bounds__(256) void k_plan_rs(const uint32_t *__restrict__ pt
             uint32_t x = 0, y = 0;
             if (term < n_terms && wlo < whi) {   // a q-gram the index has never seen has no postings
                 const uint32_t L0 = loff[term], L1 = loff[term + 1];
-                lower_bound_sid2(psid, fence, L0, L1, wlo, whi, x, y);   // wlo < whi here
+                lower_bound_sid2(psid, fence, L0, L1, wlo, L0, L1, whi, x, y);   // wlo < whi here
             }
             rng[ro + j] = make_uint2(x, y);
             e += y - x;
