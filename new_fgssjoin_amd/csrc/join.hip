@@ -2881,14 +2881,6 @@ static int slot_reserve(ServeSlot &sl, int64_t n, int64_t ntok, uint32_t jobs)
         sl.jctr_cap = jobs;
         HIPCK(hipHostMalloc((void **)&d->h_jctr, sl.jctr_cap * kJobCtr * 8, hipHostMallocDefault));
     }
-    const int64_t n0 = d->n, ntok0 = d->ntok;
-    const uint32_t nj0 = d->njobs;
-    d->n = n; d->ntok = ntok; d->njobs = jobs;
-    d->cap_out = std::max<uint32_t>(d->cap_out, 1u << 20);
-    // boundary rows grow with the probes' posting reach, not their tokens: 2^26 entries
-    // (256 MB) plan the heaviest runs of a 2M-title index without growing
-    d->cap_bounds = std::max<uint32_t>(d->cap_bounds, std::min<uint32_t>(1u << 26, kBoundsBudget));
-    d->sized = true;   // runs that outgrow a reserved capacity still complete
     if (d->h_out_cap < (1u << 18)) {   // pinned mirror of a run's first 2^18 result triples (3 MB)
         if (d->h_out) HIPCK(hipHostFree(d->h_out));
         d->h_out = nullptr;
@@ -2896,6 +2888,15 @@ static int slot_reserve(ServeSlot &sl, int64_t n, int64_t ntok, uint32_t jobs)
         HIPCK(hipHostMalloc((void **)&d->h_out, (size_t)(1u << 18) * 12, hipHostMallocDefault));
         d->h_out_cap = 1u << 18;
     }
+    d->cap_out = std::max<uint32_t>(d->cap_out, 1u << 20);
+    // boundary rows grow with the probes' posting reach, not their tokens: 2^26 entries
+    // (256 MB) plan the heaviest runs of a 2M-title index without growing
+    d->cap_bounds = std::max<uint32_t>(d->cap_bounds, std::min<uint32_t>(1u << 26, kBoundsBudget));
+    d->sized = true;   // runs that outgrow a reserved capacity still complete
+    // slot_prepare sizes the run buffers for d's record counts: the reserved ones for now
+    const int64_t n0 = d->n, ntok0 = d->ntok;
+    const uint32_t nj0 = d->njobs;
+    d->n = n; d->ntok = ntok; d->njobs = jobs;
     const int rc = slot_prepare(sl);
     d->n = n0; d->ntok = ntok0; d->njobs = nj0;
     return rc;
