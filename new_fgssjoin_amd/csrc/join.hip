@@ -846,6 +846,20 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
+// inclusive max-scan over the wavefront (values >= 0): shifts by 1, 2, 4 and 8 lanes
+// inside each row of 16 (lanes shifted in from outside the row read 0), then lane 15
+// into rows 1 and 3 and lane 31 into rows 2 and 3
+__device__ __forceinline__ uint32_t wave_max_scan(uint32_t v)
+{
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));   // row_shr:1
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));   // row_shr:2
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));   // row_shr:4
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));   // row_shr:8
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));   // row_bcast:15
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));   // row_bcast:31
+    return v;
+}
+
 __device__ __forceinline__ int ceil_log2(uint32_t x)
 {
     return x <= 1 ? 0 : 32 - __clz(x - 1);
@@ -1124,35 +1138,43 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             }
             jdone = jb + nb;
             wave_sync();
-            // exclusive scan of the segment lengths, kTokBatch/64 entries per lane
+            // exclusive scan of the segment lengths, kTokBatch/64 segments per lane
+            // (segments lane * PER + k), kept in registers: slen their lengths, sbeg
+            // their flattened starts; rlo[jj] becomes the posting index of flattened
+            // position 0 in segment jj's numbering, so that posting e of segment jj is
+            // post[rlo[jj] + e]
             constexpr int PER = kTokBatch / kWave;
-            uint32_t v[PER], sum = 0;
+            uint32_t slen[PER], sbeg[PER], sum = 0;
 #pragma unroll
             for (int k = 0; k < PER; k++) {
                 uint32_t jj = (uint32_t)lane * PER + k;
-                v[k] = jj < nb ? rpre[jj] : 0u;
-                sum += v[k];
+                slen[k] = jj < nb ? rpre[jj] : 0u;
+                sum += slen[k];
             }
             uint32_t inc = wave_inclusive_scan(sum);
             uint32_t run = inc - sum;
 #pragma unroll
             for (int k = 0; k < PER; k++) {
                 uint32_t jj = (uint32_t)lane * PER + k;
-                if (jj < nb) rpre[jj] = run;
-                run += v[k];
+                sbeg[k] = run;
+                if (jj < nb) rlo[jj] -= run;
+                run += slen[k];
             }
             // wave-uniform: the posting loop below is a scalar loop
             const uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(inc, kWave - 1, kWave));
-            if (lane == 0) rpre[nb] = total;
             wave_sync();
             PHASE_ADD(5, r0);
             escan += total;
-            const int steps = ceil_log2(nb);   // wave-uniform search depth
             // The posting loop runs without branches on per-lane conditions (on this
             // chip every divergent branch costs scalar issue slots, which bound the
             // kernel): lanes without a posting work on the last one and aim their LDS
             // updates at a word of their own in the idle query stage.
             uint32_t *scr = qstage;
+            // segment marks of a step, one byte per flattened position, after the
+            // per-lane words in the query stage
+            uint8_t *smark = (uint8_t *)(qstage + kWave);
+            static_assert(kUnroll * kWave <= 4 * (kQStage - kWave), "a step's marks fit the query stage");
+            uint32_t segc = 0;   // segment of the last posting of the previous step
             // A step takes N postings per lane (rows of kWave): full steps of kUnroll
             // rows, then a tail of 2 or 1 rows, so that a short range (most items hold
             // a few hundred postings or less) does not pay for empty rows.
@@ -1163,22 +1185,31 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                 // lanes past the end work on the last posting (a real one: every load
                 // and LDS read below is unconditional) and take no part in the updates
 #pragma unroll
-                for (int k = 0; k < N; k++) { e[k] = min(e0 + (uint32_t)(k * kWave) + lane, total - 1u); seg[k] = 0; }
-                // segment of each flattened posting: the largest jj with rpre[jj] <= e, by a
-                // search of wave-uniform depth (rpre[nb] = total > e keeps it below nb)
+                for (int k = 0; k < N; k++) e[k] = min(e0 + (uint32_t)(k * kWave) + lane, total - 1u);
+                // segment of each flattened posting: the last non-empty segment starting
+                // at or before it.  Non-empty segments start at distinct positions; each
+                // one starting inside this step marks its position with its index, and a
+                // max-scan over the step's rows (carrying segc in from the step before)
+                // hands every posting its segment.
+                static_assert(kTokBatch <= 256, "segment indices fit a byte");
+                ((uint32_t *)smark)[lane] = 0u;
 #pragma unroll
-                for (int st = 6; st >= 0; st--) {
-                    static_assert(kTokBatch <= 128, "search depth");
-                    if (st < steps) {
-#pragma unroll
-                        for (int k = 0; k < N; k++) {
-                            const uint32_t mid = min(seg[k] + (1u << st), nb);
-                            seg[k] = rpre[mid] <= e[k] ? mid : seg[k];
-                        }
-                    }
+                for (int k = 0; k < PER; k++) {
+                    const uint32_t at = sbeg[k] - e0;   // wraps above the range for starts before e0
+                    const bool mk = slen[k] != 0 && at < (uint32_t)(N * kWave);
+                    *(mk ? smark + at : (uint8_t *)(scr + lane)) = (uint8_t)(lane * PER + k);
                 }
+                wave_sync();
+                uint32_t c = segc;
 #pragma unroll
-                for (int k = 0; k < N; k++) pe[k] = A.post[rlo[seg[k]] + (e[k] - rpre[seg[k]])];
+                for (int k = 0; k < N; k++) {
+                    const uint32_t m = wave_max_scan((uint32_t)smark[k * kWave + lane]);
+                    seg[k] = max(m, c);
+                    c = max(c, (uint32_t)__builtin_amdgcn_readlane((int)m, kWave - 1));
+                }
+                segc = c;
+#pragma unroll
+                for (int k = 0; k < N; k++) pe[k] = A.post[rlo[seg[k]] + e[k]];
                 // bit k: posting k is in range and, for R-S items, inside the source's prefix
                 uint32_t valm = 0, jpos[N];
 #pragma unroll
