@@ -846,20 +846,6 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
-// inclusive max-scan over the wavefront (values >= 0): shifts by 1, 2, 4 and 8 lanes
-// inside each row of 16 (lanes shifted in from outside the row read 0), then lane 15
-// into rows 1 and 3 and lane 31 into rows 2 and 3
-__device__ __forceinline__ uint32_t wave_max_scan(uint32_t v)
-{
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));   // row_shr:1
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));   // row_shr:2
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));   // row_shr:4
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));   // row_shr:8
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));   // row_bcast:15
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));   // row_bcast:31
-    return v;
-}
-
 __device__ __forceinline__ int ceil_log2(uint32_t x)
 {
     return x <= 1 ? 0 : 32 - __clz(x - 1);
@@ -1151,7 +1137,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                 slen[k] = jj < nb ? rpre[jj] : 0u;
                 sum += slen[k];
             }
-            uint32_t inc = wave_inclusive_scan(sum);
+            uint32_t inc = wave_inclusive_scan_dpp(sum);
             uint32_t run = inc - sum;
 #pragma unroll
             for (int k = 0; k < PER; k++) {
@@ -1161,7 +1147,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                 run += slen[k];
             }
             // wave-uniform: the posting loop below is a scalar loop
-            const uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(inc, kWave - 1, kWave));
+            const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
             wave_sync();
             PHASE_ADD(5, r0);
             escan += total;

@@ -28,6 +28,34 @@ __device__ __forceinline__ T wave_inclusive_scan(T v)
     return v;
 }
 
+// inclusive wave scan of 32-bit values in six DPP adds (no LDS round trips): shifts
+// by 1, 2, 4 and 8 lanes inside each row of 16 (lanes shifted in from outside the
+// row add 0), then lane 15 into rows 1 and 3 and lane 31 into rows 2 and 3
+__device__ __forceinline__ uint32_t wave_inclusive_scan_dpp(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);   // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);   // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);   // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);   // row_bcast:31
+    return v;
+}
+
+// inclusive max-scan over the wavefront (values >= 0): shifts by 1, 2, 4 and 8 lanes
+// inside each row of 16 (lanes shifted in from outside the row read 0), then lane 15
+// into rows 1 and 3 and lane 31 into rows 2 and 3
+__device__ __forceinline__ uint32_t wave_max_scan(uint32_t v)
+{
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));   // row_shr:1
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));   // row_shr:2
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));   // row_shr:4
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));   // row_shr:8
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));   // row_bcast:15
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));   // row_bcast:31
+    return v;
+}
+
 template <class T>
 __device__ __forceinline__ T wave_sum(T v)
 {
