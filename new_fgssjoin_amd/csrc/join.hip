@@ -125,6 +125,7 @@ constexpr uint32_t kStreamQp = 256;
 // default: only queries whose boundary rows do not fit the table are streamed (for
 // short probes the planned parts spread better over the wavefronts)
 constexpr uint32_t kStreamParts = 0xFFFFFFFFu;
+constexpr uint32_t kHeavyParts = 64;   // queries split into more parts use the heavy target
 #ifndef FGSS_STREAM_CHUNK
 #define FGSS_STREAM_CHUNK 64
 #endif
@@ -188,7 +189,7 @@ struct fgss_device_sets {
     // timing events and the counter copies enqueued between and after them
     hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
     hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};
-    int graph_key[5] = {0, 0, 0, 0, 0};   // table slots, load, capacity generation, boundary budget, stream parts
+    int graph_key[6] = {0, 0, 0, 0, 0, 0};   // table slots, load, capacity generation, boundary budget, stream parts, heavy load
     int cap_gen = 0;              // bumped whenever a buffer the pipeline uses is reallocated
     bool sized = false;           // a run has completed (first-run buffer sizing is over)
 };
@@ -598,6 +599,7 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
                                               const uint32_t *__restrict__ poff,
                                               const uint32_t *__restrict__ loff, const uint32_t *__restrict__ psid,
                                               const uint32_t *__restrict__ fence, uint32_t target,
+                                              uint32_t heavy_target,
                                               uint2 *__restrict__ rng, uint32_t *__restrict__ hi_out,
                                               uint32_t *__restrict__ nparts, uint32_t *__restrict__ eq_out,
                                               uint64_t *__restrict__ bcnt, uint32_t stream_parts)
@@ -638,7 +640,10 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
         if (live && gl == 0) {
             hi_out[q] = wend;
             eq_out[q] = e;
-            const uint32_t np = e == 0 ? 0u : (e + target - 1) / target;
+            uint32_t np = e == 0 ? 0u : (e + target - 1) / target;
+            // a query of more than kHeavyParts parts fills its parts fuller: fewer parts
+            // (boundary rows, streamed rounds) outweigh the longer probe chains there
+            if (np > kHeavyParts) np = (e + heavy_target - 1) / heavy_target;
             nparts[q] = np;
             // internal boundaries x probe tokens, for queries whose parts read a planned
             // boundary table (heavier ones are streamed when their probe fits a cursor slot)
@@ -1672,6 +1677,7 @@ static uint64_t misc_nb(const fgss_device_sets *d)
 struct RunCfg {
     int cap = 512;          // LDS table slots per wave
     uint32_t target = 332;  // postings per work item
+    uint32_t heavy_target = 409;   // postings per work item of a query of more than kHeavyParts parts
     uint32_t bounds = 1u << 28;   // boundary-table entries at most
     uint32_t stream_parts = kStreamParts;   // queries split into more parts are streamed
 };
@@ -1682,6 +1688,10 @@ static RunCfg make_cfg(const fgss_join_opts *opts)
     c.cap = opts && opts->table_slots ? opts->table_slots : 512;
     const int load = opts && opts->load_pct > 0 ? std::min(opts->load_pct, FGSS_LOAD_MAX) : 65;
     c.target = std::max<uint32_t>(1u, (uint32_t)(c.cap * load / 100));
+    // heavy queries at 80% unless a load is asked for (1M long records at t = 0.8:
+    // k_join 2.01 s with every part at 65%, 1.93 s with heavy queries at 80%; 100k
+    // titles, whose items are small, are slower with every part at 80%)
+    c.heavy_target = opts && opts->load_pct > 0 ? c.target : std::max<uint32_t>(1u, (uint32_t)(c.cap * 80 / 100));
     if (opts && opts->bounds_cap > 0) c.bounds = std::min<uint32_t>((uint32_t)opts->bounds_cap, c.bounds);
     if (opts && opts->stream_parts > 0) c.stream_parts = (uint32_t)opts->stream_parts;
     return c;
@@ -1844,7 +1854,7 @@ static int enqueue_stage_plan(fgss_device_sets *d, const RunCfg &cfg)
     // 6-8: plan work items
     hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(grid_for(n * kPlanGroup, 256), 4096)), dim3(256), 0, st,
                        d->tok, d->pos, d->len, n, d->job, d->tj, d->jend, probelen, poff, loff, psid, fence, cfg.target,
-                       P<uint2>(d->b_rng), hi, nparts, eq, bcnt, cfg.stream_parts);
+                       cfg.heavy_target, P<uint2>(d->b_rng), hi, nparts, eq, bcnt, cfg.stream_parts);
     HIPCK(hipGetLastError());
     const uint32_t blim = std::min(d->cap_bounds, cfg.bounds);
     uint32_t *nitems = P<uint32_t>(d->b_nitems);
@@ -1959,6 +1969,7 @@ static int capture_graph(fgss_device_sets *d, const RunCfg &cfg)
     d->graph_key[2] = d->cap_gen;
     d->graph_key[3] = (int)cfg.bounds;
     d->graph_key[4] = (int)cfg.stream_parts;
+    d->graph_key[5] = (int)cfg.heavy_target;
     return FGSS_OK;
 }
 
@@ -1992,7 +2003,7 @@ static int launch_once(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &op
         if (rc) return rc;
         if (!d->graph_exec[0] || d->graph_key[0] != cfg.cap || d->graph_key[1] != (int)cfg.target ||
             d->graph_key[2] != d->cap_gen || d->graph_key[3] != (int)cfg.bounds ||
-            d->graph_key[4] != (int)cfg.stream_parts) {
+            d->graph_key[4] != (int)cfg.stream_parts || d->graph_key[5] != (int)cfg.heavy_target) {
             rc = capture_graph(d, cfg);
             if (rc) return rc;
         }
