@@ -730,30 +730,41 @@ __global__ void k_nitems(const uint32_t *__restrict__ nparts, const uint32_t *__
 
 // wlo: window start per query (null: q + 1, the self-join window).  rs: the queries
 // are R-S half-probes, 2r (sources longer than probe r) and 2r + 1 (the others).
-__global__ void k_items(int rs, const uint32_t *__restrict__ wlo, const uint32_t *__restrict__ hi,
-                        const uint32_t *__restrict__ nparts, const uint32_t *__restrict__ probelen,
-                        const uint32_t *__restrict__ eq, const uint32_t *__restrict__ partoff,
-                        const uint64_t *__restrict__ bofs, const uint64_t *__restrict__ bcnt,
-                        uint32_t cap_bounds, const uint32_t *__restrict__ job,
-                        const float *__restrict__ tj, const uint32_t *__restrict__ len,
-                        const uint32_t *__restrict__ pos, const uint32_t *__restrict__ poff, int64_t n,
-                        Item *__restrict__ items, uint32_t cap_items)
+// A thread per item: item o belongs to the last query q with partoff[q] <= o (the
+// queries without items share the offset of the next one) and is its part
+// o - partoff[q], so that a query of thousands of parts is written by thousands of
+// threads.
+constexpr unsigned kItemsGrid = 1024;   // k_items blocks (the item count is known on the device only)
+
+__global__ __launch_bounds__(256) void k_items(int rs, const uint32_t *__restrict__ wlo, const uint32_t *__restrict__ hi,
+                                               const uint32_t *__restrict__ nparts, const uint32_t *__restrict__ probelen,
+                                               const uint32_t *__restrict__ eq, const uint32_t *__restrict__ partoff,
+                                               const uint64_t *__restrict__ bofs, const uint64_t *__restrict__ bcnt,
+                                               uint32_t cap_bounds, const uint32_t *__restrict__ job,
+                                               const float *__restrict__ tj, const uint32_t *__restrict__ len,
+                                               const uint32_t *__restrict__ pos, const uint32_t *__restrict__ poff,
+                                               int64_t n, const uint32_t *__restrict__ n_items_dev,
+                                               Item *__restrict__ items, uint32_t cap_items)
 {
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    const uint32_t np = nparts[q];
-    const uint32_t mode = item_mode(np, probelen[q], bcnt[q], bofs[q], cap_bounds);
-    if (mode == IM_NONE) return;
-    const uint32_t nit = mode == IM_WHOLE ? 1u : mode == IM_STREAMED ? (np + kStreamChunk - 1) / kStreamChunk : np;
-    uint64_t a0 = wlo ? (uint64_t)wlo[q] : (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
-    uint32_t o = partoff[q];
-    if ((uint64_t)o + nit > cap_items) return;   // too small: the host resizes and reruns
-    const uint32_t per = (eq[q] + np - 1) / np;   // expected postings per part: sizes its LDS table
-    const int64_t r = rs ? q >> 1 : q;           // the record whose tokens probe
-    const uint32_t flag = rs && (q & 1) == 0 ? kMidProbe : 0u;
-    const uint32_t fl = mode == IM_WHOLE ? kWholeWindow : mode == IM_PLANNED ? kPlanned
-                      : mode == IM_STREAMED ? kStreamed : 0u;
-    for (uint32_t p = 0; p < nit; p++) {
+    const uint32_t total = *n_items_dev;
+    if (total > cap_items) return;   // too small: the host resizes and reruns
+    for (uint32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < total; o += gridDim.x * blockDim.x) {
+        int64_t lo = 0, top = n - 1;   // partoff[0] = 0 <= o
+        while (lo < top) {
+            const int64_t mid = (lo + top + 1) >> 1;
+            if (partoff[mid] <= o) lo = mid;
+            else top = mid - 1;
+        }
+        const int64_t q = lo;
+        const uint32_t p = o - partoff[q];
+        const uint32_t np = nparts[q];
+        const uint32_t mode = item_mode(np, probelen[q], bcnt[q], bofs[q], cap_bounds);
+        const uint64_t a0 = wlo ? (uint64_t)wlo[q] : (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
+        const uint32_t per = (eq[q] + np - 1) / np;   // expected postings per part: sizes its LDS table
+        const int64_t r = rs ? q >> 1 : q;           // the record whose tokens probe
+        const uint32_t flag = rs && (q & 1) == 0 ? kMidProbe : 0u;
+        const uint32_t fl = mode == IM_WHOLE ? kWholeWindow : mode == IM_PLANNED ? kPlanned
+                          : mode == IM_STREAMED ? kStreamed : 0u;
         Item it;
         it.q = (uint32_t)r;
         if (mode == IM_STREAMED) {   // chunk p: parts [p * kStreamChunk, ...) of the query's np
@@ -774,7 +785,7 @@ __global__ void k_items(int rs, const uint32_t *__restrict__ wlo, const uint32_t
         it.lq = len[r] | flag;
         it.qpos = pos[r];
         it.qpoff = poff[q];
-        items[o + p] = it;
+        items[o] = it;
     }
 }
 
@@ -1009,18 +1020,27 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
     unsigned long long c0 = clock64(), c1 = 0;
     (void)c0;
     (void)c1;
-    // items are claimed kQueueBatch at a time: one global atomic per batch
-    uint32_t batch_next = 0, batch_end = 0;
+    // Each wavefront's first g0 items are its own (consecutive, by its index in the
+    // grid), g0 = ceil(items / wavefronts) up to kQueueBatch, so that a launch of few
+    // items (a small served batch) spreads over every wavefront without atomics; the
+    // items past the static chunks are claimed kQueueBatch at a time, one global
+    // atomic per claim (claims of fewer items contend on the counter).
+    const uint32_t nwaves_grid = gridDim.x * (blockDim.x / kWave);
+    const uint32_t g0 = min((n_items + nwaves_grid - 1) / nwaves_grid, (uint32_t)kQueueBatch);
+    const uint32_t qbase = nwaves_grid * g0;   // first item of the dynamic claims
+    const uint32_t gw = blockIdx.x * (blockDim.x / kWave) + (uint32_t)w;
+    uint32_t batch_next = min(gw * g0, n_items), batch_end = min(gw * g0 + g0, n_items);
     uint32_t cur_job = kEmpty;
     // the next item of the claimed batch is loaded while the current one runs
     uint32_t nxtw = 0;
     bool have_nxt = false;
     for (;;) {
         if (batch_next >= batch_end) {
+            if (qbase >= n_items) break;   // every item was in a static chunk
             uint32_t b0 = 0;
             if (lane == 0) b0 = atomicAdd((unsigned int *)&A.ctr[C_QUEUE], (unsigned)kQueueBatch);
-            b0 = __builtin_amdgcn_readfirstlane(b0);
-            batch_next = b0;
+            b0 = qbase + __builtin_amdgcn_readfirstlane(b0);
+            batch_next = min(b0, n_items);
             batch_end = min(b0 + (uint32_t)kQueueBatch, n_items);
             have_nxt = false;
         }
@@ -1866,9 +1886,9 @@ static int enqueue_stage_plan(fgss_device_sets *d, const RunCfg &cfg)
                        (const uint32_t *)nullptr, hi, nparts, bofs, bcnt, (const uint64_t *)(misc + M_NB64), psid,
                        fence, P<uint32_t>(d->b_bounds), blim);
     HIPCK(hipGetLastError());
-    hipLaunchKernelGGL(k_items, dim3(grid_for(n, 256)), dim3(256), 0, st, 0, (const uint32_t *)nullptr, hi, nparts,
+    hipLaunchKernelGGL(k_items, dim3(kItemsGrid), dim3(256), 0, st, 0, (const uint32_t *)nullptr, hi, nparts,
                        probelen, eq, partoff, bofs, bcnt, blim, d->job, d->tj,
-                       d->len, d->pos, poff, n, P<Item>(d->b_items), d->cap_items);
+                       d->len, d->pos, poff, n, misc + M_NITEMS, P<Item>(d->b_items), d->cap_items);
     HIPCK(hipGetLastError());
     return FGSS_OK;
 }
@@ -2817,8 +2837,9 @@ static int enqueue_rs(fgss_server *srv, ServeSlot &sl, bool timing)
                        nparts, bofs, bcnt, (const uint64_t *)(misc + M_NB64), P<uint32_t>(x->b_psid),
                        P<uint32_t>(x->b_fence), P<uint32_t>(d->b_bounds), blim);
     HIPCK(hipGetLastError());
-    hipLaunchKernelGGL(k_items, dim3(grid_for(v, 256)), dim3(256), 0, st, 1, wlo, hi, nparts, probelen, eq, partoff,
-                       bofs, bcnt, blim, d->job, d->tj, d->len, d->pos, poff, v, P<Item>(d->b_items), d->cap_items);
+    hipLaunchKernelGGL(k_items, dim3(kItemsGrid), dim3(256), 0, st, 1, wlo, hi, nparts, probelen, eq, partoff,
+                       bofs, bcnt, blim, d->job, d->tj, d->len, d->pos, poff, v, misc + M_NITEMS, P<Item>(d->b_items),
+                       d->cap_items);
     HIPCK(hipGetLastError());
     if (timing) HIPCK(hipEventRecord(d->ev[4], st));
     int rc = launch_join_cap(d, rs_args(srv, sl), cfg.cap, false, true);
@@ -2934,6 +2955,7 @@ static int slot_reserve(ServeSlot &sl, int64_t n, int64_t ntok, uint32_t jobs)
 // threshold) and enqueue its run on the slot's stream.
 static int slot_launch(fgss_server *srv, ServeSlot &sl)
 {
+    trace_mark("begin");
     fgss_collection &m = sl.merged;
     m.len.clear(); m.tok.clear(); m.job.clear(); m.jend.clear();
     sl.first.clear();
@@ -2961,13 +2983,16 @@ static int slot_launch(fgss_server *srv, ServeSlot &sl)
     }
     m.max_len = mx;
     m.n_terms = (int32_t)srv->idx->n_terms;
+    trace_mark("srv_merge");
     int rc = slot_load(sl);
     if (rc) return rc;
+    trace_mark("srv_load");
     if (sl.d->n > 0 && !srv->empty) {
         rc = slot_prepare(sl);
         if (rc) return rc;
         rc = enqueue_rs(srv, sl, true);
         if (rc) return rc;
+        trace_mark("srv_enqueue");
     } else {
         HIPCK(hipEventRecord(sl.done, sl.d->st));   // nothing to join
     }
@@ -2983,9 +3008,11 @@ static int slot_retire(fgss_server *srv, ServeSlot &sl)
     std::vector<int32_t> tri;
     RunTotals T;
     int rc = FGSS_OK;
+    trace_mark("srv_ready");
     if (sl.d->n > 0 && !srv->empty) {
         const PipeOps ops = rs_ops(srv, sl);
         rc = finish_run(sl.d, srv->cfg, ops, false, tri, T);
+        trace_mark("srv_finish");
     } else {
         T.jctr.assign(sl.reqs.size() * kJobCtr, 0);
     }
@@ -3014,6 +3041,7 @@ static int slot_retire(fgss_server *srv, ServeSlot &sl)
     sl.est = 0;
     sl.reqs.clear();
     sl.busy = false;
+    trace_mark("srv_split");
     return rc;
 }
 
