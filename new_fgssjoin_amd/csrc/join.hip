@@ -2719,14 +2719,14 @@ struct ServeRequest {
 
 struct ServeSlot {
     fgss_device_sets *d = nullptr;      // probe records + workspace (own stream)
-    DevBuf b_tok, b_len, b_pos, b_sig, b_sig64, b_sigbad, b_job, b_jend, b_tj, b_wlo;
+    DevBuf b_in;                        // the run's probes: StageLayout arrays
+    DevBuf b_sig, b_sig64, b_sigbad, b_wlo;
     hipEvent_t done = nullptr;
     bool busy = false;
     std::vector<ServeRequest *> reqs;
     std::vector<int64_t> first;         // first probe of each request in the merged set
-    fgss_collection merged;             // host copy of the merged probes (alive while in flight)
-    std::vector<uint32_t> pos32;        // their token offsets
-    std::vector<float> tjs;
+    uint32_t *h_in = nullptr;           // pinned staging of the merged probes (StageLayout)
+    size_t h_in_cap = 0;                // its words
     size_t jctr_cap = 0;                // jobs the pinned counter mirror holds
     int64_t est = 0;
     uint64_t seq = 0;                   // launch order
@@ -2862,27 +2862,54 @@ static PipeOps rs_ops(fgss_server *srv, ServeSlot &sl)
     return o;
 }
 
-// Copy the merged probes of a slot to the device (growing its arrays) and compute
+// The merged probes of a slot run as one block of 32-bit words, staged in pinned host
+// memory and copied to the device in one asynchronous copy: the tokens, then per
+// probe its length, token offset and request, then per request its end and
+// threshold, each array starting on a 16-byte boundary.
+struct StageLayout {
+    size_t tok = 0, len = 0, pos = 0, job = 0, jend = 0, tj = 0, words = 0;
+    StageLayout(int64_t n, int64_t ntok, uint32_t jobs)
+    {
+        auto up = [](size_t x) { return (x + 3) & ~(size_t)3; };
+        len = up((size_t)ntok);
+        pos = len + up((size_t)n);
+        job = pos + up((size_t)n);
+        jend = job + up((size_t)n);
+        tj = jend + up(jobs);
+        words = std::max<size_t>(tj + up(jobs), 4);
+    }
+};
+
+// Grow a slot's pinned staging (doubling; sized at server creation, so that runs
+// within the reservation never free pinned memory, which waits for the device).  The
+// slot reuses it only after the run that copied from it has retired.
+static int stage_reserve(ServeSlot &sl, size_t words)
+{
+    if (sl.h_in_cap >= words) return FGSS_OK;
+    if (sl.h_in) HIPCK(hipHostFree(sl.h_in));
+    sl.h_in = nullptr;
+    const size_t cap = std::max<size_t>({words, 2 * sl.h_in_cap, (size_t)1 << 16});
+    sl.h_in_cap = 0;
+    HIPCK(hipHostMalloc((void **)&sl.h_in, cap * 4, hipHostMallocDefault));
+    sl.h_in_cap = cap;
+    return FGSS_OK;
+}
+
+// Copy the staged probes of a slot to the device (growing its arrays) and compute
 // their signatures, asynchronously on the slot's stream.
-static int slot_load(ServeSlot &sl)
+static int slot_load(ServeSlot &sl, int64_t n, int64_t ntok, uint32_t nj, int32_t max_len, uint32_t n_terms)
 {
     fgss_device_sets *d = sl.d;
-    const fgss_collection &m = sl.merged;
-    const int64_t n = (int64_t)m.len.size(), ntok = (int64_t)m.tok.size();
-    const uint32_t nj = (uint32_t)m.jend.size();
-    GROW(sl.b_tok, (size_t)std::max<int64_t>(ntok, 1) * 4);
-    GROW(sl.b_len, (size_t)std::max<int64_t>(n, 1) * 4);
-    GROW(sl.b_pos, (size_t)std::max<int64_t>(n, 1) * 4);
+    const StageLayout L(n, ntok, nj);
+    GROW(sl.b_in, L.words * 4);
     GROW(sl.b_sig, (size_t)std::max<int64_t>(n, 1) * 4 * kSigWords);
     GROW(sl.b_sig64, (size_t)std::max<int64_t>(n, 1) * 4 * kSig64Words);
     GROW(sl.b_sigbad, (size_t)std::max<int64_t>(n, 1));
-    GROW(sl.b_job, (size_t)std::max<int64_t>(n, 1) * 4);
-    GROW(sl.b_jend, (size_t)std::max<uint32_t>(nj, 1) * 4);
-    GROW(sl.b_tj, (size_t)std::max<uint32_t>(nj, 1) * 4);
-    d->tok = P<uint32_t>(sl.b_tok); d->len = P<uint32_t>(sl.b_len); d->pos = P<uint32_t>(sl.b_pos);
+    uint32_t *in = P<uint32_t>(sl.b_in);
+    d->tok = in + L.tok; d->len = in + L.len; d->pos = in + L.pos; d->job = in + L.job; d->jend = in + L.jend;
+    d->tj = (float *)(in + L.tj);
     d->sig = P<uint32_t>(sl.b_sig); d->sig64 = P<uint32_t>(sl.b_sig64); d->sigbad = P<uint8_t>(sl.b_sigbad);
-    d->job = P<uint32_t>(sl.b_job); d->jend = P<uint32_t>(sl.b_jend); d->tj = P<float>(sl.b_tj);
-    d->n = n; d->ntok = ntok; d->njobs = nj; d->max_len = m.max_len; d->n_terms = (uint32_t)m.n_terms;
+    d->n = n; d->ntok = ntok; d->njobs = nj; d->max_len = max_len; d->n_terms = n_terms;
     // pinned per-job counter mirror, grown to this batch's jobs
     if (sl.jctr_cap < (size_t)std::max<uint32_t>(nj, 1)) {
         if (d->h_jctr) HIPCK(hipHostFree(d->h_jctr));
@@ -2891,14 +2918,7 @@ static int slot_load(ServeSlot &sl)
         HIPCK(hipHostMalloc((void **)&d->h_jctr, sl.jctr_cap * kJobCtr * 8, hipHostMallocDefault));
     }
     hipStream_t st = d->st;
-    if (ntok) HIPCK(hipMemcpyAsync(d->tok, m.tok.data(), (size_t)ntok * 4, hipMemcpyHostToDevice, st));
-    if (n) {
-        HIPCK(hipMemcpyAsync(d->len, m.len.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
-        HIPCK(hipMemcpyAsync(d->pos, sl.pos32.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
-        HIPCK(hipMemcpyAsync(d->job, m.job.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
-    }
-    HIPCK(hipMemcpyAsync(d->jend, m.jend.data(), (size_t)nj * 4, hipMemcpyHostToDevice, st));
-    HIPCK(hipMemcpyAsync(d->tj, sl.tjs.data(), (size_t)nj * 4, hipMemcpyHostToDevice, st));
+    HIPCK(hipMemcpyAsync(in, sl.h_in, L.words * 4, hipMemcpyHostToDevice, st));
     if (n) {
         const int64_t blocks = std::min<int64_t>((n + 3) / 4, 65536);
         hipLaunchKernelGGL(k_signature, dim3((unsigned)blocks), dim3(256), 0, st, d->tok, d->pos, d->len, n, d->sig,
@@ -2915,15 +2935,15 @@ static int slot_load(ServeSlot &sl)
 static int slot_reserve(ServeSlot &sl, int64_t n, int64_t ntok, uint32_t jobs)
 {
     fgss_device_sets *d = sl.d;
-    GROW(sl.b_tok, (size_t)ntok * 4);
-    GROW(sl.b_len, (size_t)n * 4);
-    GROW(sl.b_pos, (size_t)n * 4);
+    const StageLayout L(n, ntok, jobs);
+    GROW(sl.b_in, L.words * 4);
+    {
+        const int rc_ = stage_reserve(sl, L.words);
+        if (rc_) return rc_;
+    }
     GROW(sl.b_sig, (size_t)n * 4 * kSigWords);
     GROW(sl.b_sig64, (size_t)n * 4 * kSig64Words);
     GROW(sl.b_sigbad, (size_t)n);
-    GROW(sl.b_job, (size_t)n * 4);
-    GROW(sl.b_jend, (size_t)jobs * 4);
-    GROW(sl.b_tj, (size_t)jobs * 4);
     if (sl.jctr_cap < jobs) {
         if (d->h_jctr) HIPCK(hipHostFree(d->h_jctr));
         d->h_jctr = nullptr;
@@ -2956,35 +2976,45 @@ static int slot_reserve(ServeSlot &sl, int64_t n, int64_t ntok, uint32_t jobs)
 static int slot_launch(fgss_server *srv, ServeSlot &sl)
 {
     trace_mark("begin");
-    fgss_collection &m = sl.merged;
-    m.len.clear(); m.tok.clear(); m.job.clear(); m.jend.clear();
+    int64_t n = 0, ntok = 0;
+    for (const ServeRequest *r : sl.reqs) {
+        n += (int64_t)r->probes->len.size();
+        ntok += (int64_t)r->probes->tok.size();
+    }
+    const uint32_t nj = (uint32_t)sl.reqs.size();
+    const StageLayout L(n, ntok, nj);
+    {
+        const int rc_ = stage_reserve(sl, L.words);
+        if (rc_) return rc_;
+    }
+    uint32_t *tok = sl.h_in + L.tok, *len = sl.h_in + L.len, *pos = sl.h_in + L.pos, *job = sl.h_in + L.job,
+             *jend = sl.h_in + L.jend;
+    float *tj = (float *)(sl.h_in + L.tj);
     sl.first.clear();
-    sl.tjs.clear();
     const auto now = std::chrono::steady_clock::now();
-    for (size_t k = 0; k < sl.reqs.size(); k++) {
-        const fgss_collection *c = sl.reqs[k]->probes;
-        sl.first.push_back((int64_t)m.len.size());
-        m.len.insert(m.len.end(), c->len.begin(), c->len.end());
-        m.tok.insert(m.tok.end(), c->tok.begin(), c->tok.end());
-        m.job.insert(m.job.end(), c->len.size(), (uint32_t)k);
-        m.jend.push_back((uint32_t)m.len.size());
-        sl.tjs.push_back(sl.reqs[k]->t);
-        sl.reqs[k]->t_start = now;
-        sl.reqs[k]->batch_requests = (int32_t)sl.reqs.size();
-    }
-    sl.first.push_back((int64_t)m.len.size());
-    sl.pos32.resize(m.len.size());
-    uint32_t p = 0;
+    int64_t i = 0;
+    uint32_t p = 0;   // token offset of probe i (records are concatenated in order)
     int32_t mx = 0;
-    for (size_t i = 0; i < m.len.size(); i++) {
-        sl.pos32[i] = p;
-        p += (uint32_t)m.len[i];
-        mx = std::max(mx, m.len[i]);
+    for (uint32_t k = 0; k < nj; k++) {
+        const fgss_collection *c = sl.reqs[k]->probes;
+        sl.first.push_back(i);
+        if (!c->tok.empty()) memcpy(tok + p, c->tok.data(), c->tok.size() * 4);
+        for (const int32_t l : c->len) {
+            len[i] = (uint32_t)l;
+            pos[i] = p;
+            job[i] = k;
+            p += (uint32_t)l;
+            mx = std::max(mx, l);
+            i++;
+        }
+        jend[k] = (uint32_t)i;
+        tj[k] = sl.reqs[k]->t;
+        sl.reqs[k]->t_start = now;
+        sl.reqs[k]->batch_requests = (int32_t)nj;
     }
-    m.max_len = mx;
-    m.n_terms = (int32_t)srv->idx->n_terms;
+    sl.first.push_back(i);
     trace_mark("srv_merge");
-    int rc = slot_load(sl);
+    int rc = slot_load(sl, n, ntok, nj, mx, (uint32_t)srv->idx->n_terms);
     if (rc) return rc;
     trace_mark("srv_load");
     if (sl.d->n > 0 && !srv->empty) {
@@ -3303,10 +3333,11 @@ extern "C" void fgss_server_free(fgss_server *srv)
     for (ServeSlot &sl : srv->slots) {
         if (!sl.d) continue;
         if (sl.d->st) (void)hipStreamSynchronize(sl.d->st);
-        DevBuf *bufs[] = {&sl.b_tok, &sl.b_len, &sl.b_pos, &sl.b_sig, &sl.b_sig64, &sl.b_sigbad,
-                          &sl.b_job, &sl.b_jend, &sl.b_tj, &sl.b_wlo};
+        DevBuf *bufs[] = {&sl.b_in, &sl.b_sig, &sl.b_sig64, &sl.b_sigbad, &sl.b_wlo};
         for (DevBuf *b : bufs)
             if (b->p) (void)hipFree(b->p);
+        if (sl.h_in) (void)hipHostFree(sl.h_in);
+        sl.h_in = nullptr;
         // the record arrays belong to the slot buffers freed above
         sl.d->tok = sl.d->len = sl.d->pos = sl.d->sig = sl.d->sig64 = sl.d->job = sl.d->jend = nullptr;
         sl.d->sigbad = nullptr;
