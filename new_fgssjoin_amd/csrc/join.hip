@@ -1847,17 +1847,35 @@ static int enqueue_index(fgss_device_sets *d, int rs)
     return FGSS_OK;
 }
 
+// A run's counters (ctr), per-job counters (jctr) and small counters (misc) cleared
+// in one launch
+__global__ void k_clear_run(unsigned long long *__restrict__ ctr, int nctr, unsigned long long *__restrict__ jctr,
+                            int64_t njctr, uint32_t *__restrict__ misc, int nmisc)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < njctr; i += (int64_t)gridDim.x * blockDim.x)
+        jctr[i] = 0ull;
+    if (blockIdx.x == 0) {
+        for (int i = (int)threadIdx.x; i < nctr; i += (int)blockDim.x) ctr[i] = 0ull;
+        for (int i = (int)threadIdx.x; i < nmisc; i += (int)blockDim.x) misc[i] = 0u;
+    }
+}
+
+static hipError_t clear_run(fgss_device_sets *d, hipStream_t st)
+{
+    const int64_t nj = (int64_t)d->njobs * kJobCtr;
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nj + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_clear_run, dim3(blocks), dim3(256), 0, st, P<unsigned long long>(d->b_ctr), (int)C_COUNT,
+                       P<unsigned long long>(d->b_jctr), nj, P<uint32_t>(d->b_misc), (int)M_COUNT);
+    return hipGetLastError();
+}
+
 // The self-join pipeline on d->st in three stages with no host synchronization (so
 // that each can be captured into a HIP graph): the index build (counters cleared
 // first), the planning of work items, and the join kernel; then the device->host
 // copies of all counters into pinned memory.
 static int enqueue_stage_index(fgss_device_sets *d)
 {
-    hipStream_t st = d->st;
-    unsigned long long *ctr = P<unsigned long long>(d->b_ctr), *jctr = P<unsigned long long>(d->b_jctr);
-    HIPCK(fill<unsigned long long>(ctr, C_COUNT, 0ull, st));
-    HIPCK(fill<unsigned long long>(jctr, (int64_t)d->njobs * kJobCtr, 0ull, st));
-    HIPCK(fill<uint32_t>(P<uint32_t>(d->b_misc), M_COUNT, 0u, st));
+    HIPCK(clear_run(d, d->st));
     return enqueue_index(d, 0);
 }
 
@@ -2812,11 +2830,8 @@ static int enqueue_rs(fgss_server *srv, ServeSlot &sl, bool timing)
     uint32_t *hi = P<uint32_t>(d->b_hi), *eq = P<uint32_t>(d->b_eq), *nparts = P<uint32_t>(d->b_nparts),
              *partoff = P<uint32_t>(d->b_partoff), *wlo = P<uint32_t>(sl.b_wlo);
     uint64_t *bcnt = P<uint64_t>(d->b_bcnt), *bofs = P<uint64_t>(d->b_bofs), *scr64 = P<uint64_t>(d->b_scanscr);
-    unsigned long long *ctr = P<unsigned long long>(d->b_ctr), *jctr = P<unsigned long long>(d->b_jctr);
     if (timing) HIPCK(hipEventRecord(d->ev[0], st));
-    HIPCK(fill<unsigned long long>(ctr, C_COUNT, 0ull, st));
-    HIPCK(fill<unsigned long long>(jctr, (int64_t)d->njobs * kJobCtr, 0ull, st));
-    HIPCK(fill<uint32_t>(misc, M_COUNT, 0u, st));
+    HIPCK(clear_run(d, st));
     const int64_t v = 2 * n;   // half-probes
     hipLaunchKernelGGL(k_probelen_rs, dim3(grid_for(v, 256)), dim3(256), 0, st, d->len, d->job, d->tj, probelen, n);
     HIPCK(hipGetLastError());

@@ -159,6 +159,45 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tile(const T *__restrict_
     if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total_out = off + tot;
 }
 
+// k_scan_tile whose block offset is the sum of sums[0 .. blockIdx.x), added up by the
+// block itself (used when the block sums fit one tile, so that the scan of the sums
+// is not a launch of its own)
+template <class T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_tile_pre(const T *__restrict__ in, T *__restrict__ out, int64_t n,
+                                                                const T *__restrict__ sums, T *__restrict__ total_out)
+{
+    __shared__ T tile[kScanTile];
+    T part = 0;
+    for (int k = (int)threadIdx.x; k < (int)blockIdx.x; k += kScanThreads) part += sums[k];
+    T off;
+    (void)block_exclusive_scan(part, &off);
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        int idx = k * kScanThreads + (int)threadIdx.x;
+        int64_t i = base + idx;
+        tile[idx] = i < n ? in[i] : (T)0;
+    }
+    __syncthreads();
+    T v[kScanItems];
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) { v[k] = tile[threadIdx.x * kScanItems + k]; s += v[k]; }
+    T tot;
+    T ex = block_exclusive_scan(s, &tot);
+    T run = off + ex;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) { tile[threadIdx.x * kScanItems + k] = run; run += v[k]; }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        int idx = k * kScanThreads + (int)threadIdx.x;
+        int64_t i = base + idx;
+        if (i < n) out[i] = tile[idx];
+    }
+    if (total_out && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total_out = off + tot;
+}
+
 // Scratch needed by exclusive_scan for n elements (in elements of T).
 inline int64_t scan_scratch_elems(int64_t n)
 {
@@ -184,6 +223,11 @@ hipError_t exclusive_scan(const T *in, T *out, int64_t n, T *total, T *scratch, 
     T *sums = scratch;
     T *rest = scratch + nb + 1;
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, sums, n);
+    if (nb <= kScanTile) {   // each tile block adds up the sums before it
+        hipLaunchKernelGGL(k_scan_tile_pre<T>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, out, n,
+                           (const T *)sums, total);
+        return hipGetLastError();
+    }
     hipError_t e = exclusive_scan<T>(sums, sums, nb, (T *)nullptr, rest, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_scan_tile<T>, dim3((unsigned)nb), dim3(kScanThreads), 0, st, in, out, n, (const T *)sums, total);
