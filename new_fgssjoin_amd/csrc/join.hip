@@ -29,6 +29,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <deque>
 #include <functional>
@@ -189,7 +190,7 @@ struct fgss_device_sets {
     // timing events and the counter copies enqueued between and after them
     hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
     hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};
-    int graph_key[6] = {0, 0, 0, 0, 0, 0};   // table slots, load, capacity generation, boundary budget, stream parts, heavy load
+    std::array<int64_t, 12> graph_key{};   // what the captured stages were built for (graph_key_of)
     int cap_gen = 0;              // bumped whenever a buffer the pipeline uses is reallocated
     bool sized = false;           // a run has completed (first-run buffer sizing is over)
 };
@@ -1986,6 +1987,16 @@ struct RunTotals {
     bool graph = false;
 };
 
+// Everything the captured stages bake into their launches besides device pointers
+// (a reallocation of any buffer bumps cap_gen): the configuration, the capacities
+// the kernels test against and the record counts.  A replay is valid while it is
+// unchanged; a capacity can grow inside a buffer's slack without a reallocation.
+static std::array<int64_t, 12> graph_key_of(const fgss_device_sets *d, const RunCfg &cfg)
+{
+    return {cfg.cap, cfg.target, cfg.heavy_target, cfg.bounds, cfg.stream_parts, d->cap_gen,
+            d->cap_items, d->cap_out, d->cap_ovf, d->cap_bounds, d->n, d->njobs};
+}
+
 static int capture_graph(fgss_device_sets *d, const RunCfg &cfg)
 {
     for (int k = 0; k < 3; k++) {
@@ -2002,12 +2013,7 @@ static int capture_graph(fgss_device_sets *d, const RunCfg &cfg)
         d->graph[k] = g;
         HIPCK(hipGraphInstantiate(&d->graph_exec[k], g, nullptr, nullptr, 0));
     }
-    d->graph_key[0] = cfg.cap;
-    d->graph_key[1] = (int)cfg.target;
-    d->graph_key[2] = d->cap_gen;
-    d->graph_key[3] = (int)cfg.bounds;
-    d->graph_key[4] = (int)cfg.stream_parts;
-    d->graph_key[5] = (int)cfg.heavy_target;
+    d->graph_key = graph_key_of(d, cfg);
     return FGSS_OK;
 }
 
@@ -2039,9 +2045,7 @@ static int launch_once(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &op
     if (use_graph && ops.graph_ok) {
         rc = launch_join_cap(d, ops.args(), cfg.cap, true, ops.rs);   // launch configuration, outside the capture
         if (rc) return rc;
-        if (!d->graph_exec[0] || d->graph_key[0] != cfg.cap || d->graph_key[1] != (int)cfg.target ||
-            d->graph_key[2] != d->cap_gen || d->graph_key[3] != (int)cfg.bounds ||
-            d->graph_key[4] != (int)cfg.stream_parts || d->graph_key[5] != (int)cfg.heavy_target) {
+        if (!d->graph_exec[0] || d->graph_key != graph_key_of(d, cfg)) {
             rc = capture_graph(d, cfg);
             if (rc) return rc;
         }
@@ -2304,8 +2308,8 @@ extern "C" int fgss_selfjoin(fgss_collection *c, float threshold, const fgss_joi
     if (rc) return rc;
     auto t1 = std::chrono::steady_clock::now();
     fgss_device_sets *d = c->dev;
-    rc = set_thresholds(d, &threshold, 1);
-    if (rc) return rc;
+    // the one threshold travels as a kernel argument: no host copy, no synchronization
+    HIPCK(fill<float>(d->tj, 1, threshold, d->st));
     std::vector<int32_t> tri;
     RunTotals T;
     // eager launches: replaying the stages as captured HIP graphs measured the same

@@ -76,6 +76,20 @@ def test_batch_new_thresholds_and_table_sizes(fg):
             assert _same(b.pairs(j), c.selfjoin(ts2[j])), f"job {j}"
 
 
+def test_graph_replays_follow_growing_capacities(fg):
+    """Lower thresholds run after run: item, result and overflow counts grow, and a
+    capacity that grows inside a buffer's slack must still reach the replayed launches."""
+    colls, ts = _jobs(fg)
+    b = fg.Batch(colls, ts)
+    for k in range(6):
+        tk = [float(np.float32(max(0.3, t - 0.05 * k))) for t in ts]
+        b.set_thresholds(tk)
+        _say(f"batch graph replay, thresholds lowered {k} times")
+        b.run(graph=True)
+        for j, c in enumerate(colls):
+            assert _same(b.pairs(j), c.selfjoin(tk[j])), (k, j)
+
+
 def test_autotune_returns_a_valid_choice(fg):
     colls, ts = _jobs(fg)
     b = fg.Batch(colls[:6], ts[:6])
