@@ -275,7 +275,12 @@ static __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32
 }
 
 // Stable scatter.  offs[d * ntiles + tile] = exclusive-scanned histogram.
-// If idx_in is null the index of element i is i itself.
+// If idx_in is null the index of element i is i itself.  The tile is first ranked
+// into LDS in digit order (stable: waves in order, lanes ranked by a ballot match on
+// the digit), then written out position by position, so that each digit's run of
+// the tile goes to consecutive addresses (coalesced) instead of one scattered store
+// per element.
+static_assert(kSortThreads == 256, "one thread per digit");
 static __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t *__restrict__ keys_in,
                                                                 const uint32_t *__restrict__ idx_in, int64_t n,
                                                                 const uint32_t *__restrict__ n_dev, int shift,
@@ -284,11 +289,15 @@ static __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uin
                                                                 uint32_t *__restrict__ idx_out)
 {
     n = dev_count(n, n_dev);
-    __shared__ uint32_t h[kSortWaves][256];
+    const int64_t tile0 = (int64_t)blockIdx.x * kSortTile;
+    if (tile0 >= n) return;   // the whole block: past the device-side count
+    __shared__ uint32_t h[kSortWaves][256];   // per-wave digit counts, then per-wave local starts
+    __shared__ uint32_t dbase[256];           // global start of (digit, tile) minus its local start
+    __shared__ uint32_t skey[kSortTile], sval[kSortTile];
     const int w = (int)threadIdx.x / kWave;
     for (int i = threadIdx.x; i < kSortWaves * 256; i += kSortThreads) (&h[0][0])[i] = 0;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kSortTile + (int64_t)w * kSortWaveSpan;
+    const int64_t base = tile0 + (int64_t)w * kSortWaveSpan;
     // per-wave digit counts
 #pragma unroll 4
     for (int k = 0; k < kSortItems; k++) {
@@ -296,18 +305,20 @@ static __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uin
         if (i < n) atomicAdd(&h[w][(keys_in[i] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    // per-wave starting offsets: global digit offset + counts of earlier waves
-    for (int d = threadIdx.x; d < 256; d += kSortThreads) {
-        uint32_t run = offs[(int64_t)d * ntiles + blockIdx.x];
+    {   // digit d (this thread): its run in the tile starts at the count of smaller digits
+        const int d = (int)threadIdx.x;
+        uint32_t c[kSortWaves], tot = 0;
 #pragma unroll
-        for (int k = 0; k < kSortWaves; k++) {
-            uint32_t c = h[k][d];
-            h[k][d] = run;
-            run += c;
-        }
+        for (int k = 0; k < kSortWaves; k++) { c[k] = h[k][d]; tot += c[k]; }
+        uint32_t all;
+        const uint32_t loc = block_exclusive_scan<uint32_t>(tot, &all);
+        uint32_t run = loc;
+#pragma unroll
+        for (int k = 0; k < kSortWaves; k++) { h[k][d] = run; run += c[k]; }
+        dbase[d] = offs[(int64_t)d * ntiles + blockIdx.x] - loc;
     }
     __syncthreads();
-    // in-order ranking within each 64-element chunk (ballot match on 8 digit bits)
+    // in-order ranking within each 64-element chunk (ballot match on 8 digit bits) into LDS
     for (int k = 0; k < kSortItems; k++) {
         int64_t i = base + (int64_t)k * kWave + lane_id();
         const bool valid = i < n;
@@ -325,10 +336,18 @@ static __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uin
         // all lanes have read their base; the highest peer advances it
         if (valid && rank == cnt - 1) h[w][d] = start + cnt;
         if (valid) {
-            uint32_t dst = start + rank;
-            keys_out[dst] = key;
-            idx_out[dst] = idx_in ? idx_in[i] : (uint32_t)i;
+            skey[start + rank] = key;
+            sval[start + rank] = idx_in ? idx_in[i] : (uint32_t)i;
         }
+    }
+    __syncthreads();
+    // local position j of digit d goes to dbase[d] + j: consecutive within a run
+    const int cnt_tile = (int)min((int64_t)kSortTile, n - tile0);
+    for (int j = (int)threadIdx.x; j < cnt_tile; j += kSortThreads) {
+        const uint32_t key = skey[j];
+        const uint32_t dst = dbase[(key >> shift) & 255u] + (uint32_t)j;
+        keys_out[dst] = key;
+        idx_out[dst] = sval[j];
     }
 }
 
