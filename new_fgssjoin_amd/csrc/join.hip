@@ -30,11 +30,13 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <deque>
 #include <functional>
 #include <map>
 #include <memory>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -2202,34 +2204,80 @@ static int run_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool use_graph, 
     return finish_run(d, cfg, ops, use_graph, tri, T);
 }
 
+// Host threads for result post-processing: at most 16, the host share of one GPU
+static unsigned host_workers()
+{
+    const unsigned n = std::thread::hardware_concurrency();
+    return n == 0 ? 1u : std::min(n, 16u);
+}
+
 // Order of the (q, s, overlap) triples by (q, s): an LSD radix sort (11-bit digits)
 // of 64-bit keys (q, s) carried with the triple index; passes whose digit is
 // constant are skipped.  out[k] low 32 bits = index of the k-th triple.
 static void sort_pairs_by_key(const std::vector<int32_t> &tri, std::vector<uint64_t> &out)
 {
     const size_t R = tri.size() / 3;
+    // large result sets (batches) are sorted by up to 16 host threads: each counts and
+    // scatters its own chunk, chunk after chunk per digit, so the sort stays stable
+    const unsigned T = R >= ((size_t)1 << 17) ? host_workers() : 1u;
     std::vector<uint64_t> key(R), idx(R), k2(R), i2(R);
-    for (size_t i = 0; i < R; i++) {
-        key[i] = ((uint64_t)(uint32_t)tri[3 * i] << 31) | (uint32_t)tri[3 * i + 1];
-        idx[i] = i;
-    }
-    for (int shift = 0; shift < 62; shift += 11) {
-        size_t cnt[2049] = {0};
-        for (size_t i = 0; i < R; i++) cnt[((key[i] >> shift) & 2047) + 1]++;
-        bool constant = false;
-        for (int dd = 1; dd <= 2048; dd++)
-            if (cnt[dd] == R) constant = true;
-        if (constant) continue;
-        for (int dd = 0; dd < 2048; dd++) cnt[dd + 1] += cnt[dd];
-        for (size_t i = 0; i < R; i++) {
-            size_t at = cnt[(key[i] >> shift) & 2047]++;
-            k2[at] = key[i];
-            i2[at] = idx[i];
+    uint64_t *kb[2] = {key.data(), k2.data()}, *ib[2] = {idx.data(), i2.data()};
+    std::vector<std::array<size_t, 2048>> cnt(T);
+    bool skip = false;   // written by thread 0 between barriers
+    std::atomic<unsigned> arrived{0}, phase{0};
+    auto barrier = [&](unsigned &mine) {
+        const unsigned target = ++mine;
+        if (arrived.fetch_add(1) + 1 == T * target) phase.store(target);
+        while (phase.load() < target) std::this_thread::yield();
+    };
+    int final_buf = 0;
+    auto worker = [&](unsigned t) {
+        unsigned mine = 0;
+        const size_t a = R * t / T, b = R * (t + 1) / T;
+        for (size_t i = a; i < b; i++) {
+            key[i] = ((uint64_t)(uint32_t)tri[3 * i] << 31) | (uint32_t)tri[3 * i + 1];
+            idx[i] = i;
         }
-        key.swap(k2);
-        idx.swap(i2);
-    }
-    out.swap(idx);
+        int cur = 0;
+        for (int shift = 0; shift < 62; shift += 11) {
+            const uint64_t *ks = kb[cur], *is = ib[cur];
+            std::array<size_t, 2048> &c = cnt[t];
+            c.fill(0);
+            for (size_t i = a; i < b; i++) c[(ks[i] >> shift) & 2047]++;
+            barrier(mine);
+            if (t == 0) {   // starting position of (digit, chunk), digits in order, chunks in order
+                size_t run = 0;
+                skip = false;
+                for (int dd = 0; dd < 2048; dd++) {
+                    size_t tot = 0;
+                    for (unsigned u = 0; u < T; u++) tot += cnt[u][dd];
+                    if (tot == R) skip = true;   // one digit value only: the pass changes nothing
+                    for (unsigned u = 0; u < T; u++) {
+                        const size_t n_u = cnt[u][dd];
+                        cnt[u][dd] = run;
+                        run += n_u;
+                    }
+                }
+            }
+            barrier(mine);
+            if (skip) continue;
+            uint64_t *kd = kb[cur ^ 1], *id = ib[cur ^ 1];
+            for (size_t i = a; i < b; i++) {
+                const size_t at = c[(ks[i] >> shift) & 2047]++;
+                kd[at] = ks[i];
+                id[at] = is[i];
+            }
+            barrier(mine);
+            cur ^= 1;
+        }
+        if (t == 0) final_buf = cur;
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < T; t++) th.emplace_back(worker, t);
+    worker(0);
+    for (auto &x : th) x.join();
+    if (final_buf == 0) out.swap(idx);
+    else out.swap(i2);
 }
 
 // Build a result object from triples (all with query index in [q0, q1) of the
@@ -2548,26 +2596,36 @@ extern "C" int fgss_batch_pairs_all(fgss_batch *b, fgss_pairs **out, int64_t *jo
         fgss_pairs_free(p);
         return fgss_fail(FGSS_ERR_NOMEM, "out of host memory");
     }
-    for (size_t j = 0; j < nj; j++) {
-        const fgss_collection *c = b->colls[j];
-        const int64_t q0 = b->rec_begin[j];
-        job_first[j] = (int64_t)b->job_first[j];
-        for (size_t k = b->job_first[j]; k < b->job_first[j + 1]; k++) {
-            const size_t i = (size_t)(b->order[k] & 0xFFFFFFFFull);
-            const int32_t q = (int32_t)(b->tri[3 * i] - q0), sv = (int32_t)(b->tri[3 * i + 1] - q0);
-            const int32_t ov = b->tri[3 * i + 2];
-            p->q[k] = q;
-            p->s[k] = sv;
-            p->rid[k] = c->id[(size_t)q];
-            p->sid[k] = c->id[(size_t)sv];
-            p->overlap[k] = ov;
-            p->similarity[k] = fgss_similarity((uint32_t)c->len[(size_t)q], (uint32_t)c->len[(size_t)sv], (uint32_t)ov);
+    // jobs fill disjoint ranges: up to 16 host threads take them one at a time
+    std::atomic<size_t> next{0};
+    auto fill_jobs = [&] {
+        for (size_t j; (j = next.fetch_add(1)) < nj;) {
+            const fgss_collection *c = b->colls[j];
+            const int64_t q0 = b->rec_begin[j];
+            job_first[j] = (int64_t)b->job_first[j];
+            for (size_t k = b->job_first[j]; k < b->job_first[j + 1]; k++) {
+                const size_t i = (size_t)(b->order[k] & 0xFFFFFFFFull);
+                const int32_t q = (int32_t)(b->tri[3 * i] - q0), sv = (int32_t)(b->tri[3 * i + 1] - q0);
+                const int32_t ov = b->tri[3 * i + 2];
+                p->q[k] = q;
+                p->s[k] = sv;
+                p->rid[k] = c->id[(size_t)q];
+                p->sid[k] = c->id[(size_t)sv];
+                p->overlap[k] = ov;
+                p->similarity[k] =
+                    fgss_similarity((uint32_t)c->len[(size_t)q], (uint32_t)c->len[(size_t)sv], (uint32_t)ov);
+            }
+            if (job_stats) {
+                job_stats[j] = fgss_join_stats{};
+                fill_stats(b->totals, j, job_stats[j]);
+            }
         }
-        if (job_stats) {
-            job_stats[j] = fgss_join_stats{};
-            fill_stats(b->totals, j, job_stats[j]);
-        }
-    }
+    };
+    const unsigned T = R >= ((size_t)1 << 17) ? std::min<unsigned>(host_workers(), (unsigned)nj) : 1u;
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < T; t++) th.emplace_back(fill_jobs);
+    fill_jobs();
+    for (auto &x : th) x.join();
     job_first[nj] = (int64_t)R;
     p->n_pairs = (int64_t)R;
     *out = p;
