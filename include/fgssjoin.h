@@ -118,7 +118,7 @@ typedef struct {
     double ms_copy;            /* device -> host copy of the result pairs             */
     double ms_total;           /* whole call                                          */
     double ms_kernel;          /* device time of the join kernel launches (HIP events) */
-    int64_t kernel_launches;   /* join kernel launches (1 unless work items were re-split) */
+    int64_t kernel_launches;   /* join kernel launches (1 unless buffers grew or work items were re-split) */
     int64_t signature_checks;  /* candidates tested against the 64-bucket overlap bound */
     int64_t signature_checks_128; /* of those, tested again with 128 buckets           */
     int64_t verified;          /* candidates whose tokens were merged                  */
@@ -184,7 +184,7 @@ typedef struct {
     int64_t n_pairs;          /* result pairs over all jobs                          */
     int64_t candidates;       /* "# Candidates" summed over all jobs                 */
     int32_t graph;            /* 1 when the pipeline ran as a replayed HIP graph     */
-    int32_t kernel_launches;  /* join kernel launches (1 unless items were re-split) */
+    int32_t kernel_launches;  /* join kernel launches (1 unless buffers grew or items were re-split) */
 } fgss_batch_stats;
 
 /* Create a batch of njobs self-joins: job j joins colls[j] at thresholds[j].

@@ -2074,6 +2074,7 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
     tri.clear();
     T = RunTotals();
     T.jctr.assign((size_t)d->njobs * kJobCtr, 0);
+    int reruns = 0;   // whole runs repeated with grown buffers
     for (int attempt = 0;; attempt++) {
         const uint32_t n_items = d->h_misc[M_NITEMS];
         const uint64_t nB = misc_nb(d);
@@ -2099,6 +2100,7 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
         }
         if (!again) break;
         if (attempt > 8) return fgss_fail(FGSS_ERR_DEVICE, "internal: buffer sizing did not converge");
+        reruns++;
         int rc = launch_once(d, cfg, ops, use_graph);
         if (rc) return rc;
         HIPCK(hipStreamSynchronize(st));
@@ -2106,7 +2108,7 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
     d->sized = true;
     trace_mark("pipeline");
     T.graph = use_graph && ops.graph_ok;
-    T.launches = 1;
+    T.launches = 1 + reruns;
     T.index_entries = d->h_misc[M_NE];
     T.probe_tokens = d->h_misc[M_NP];
     {   // events recorded around the stages in both modes
@@ -3009,9 +3011,10 @@ static int slot_load(ServeSlot &sl, int64_t n, int64_t ntok, uint32_t nj, int32_
 // tokens and `jobs` requests, so that such runs never reallocate: a reallocation
 // waits for the whole device (stalling the runs of the other slots) and the run that
 // found a buffer too small is enqueued again.
-static int slot_reserve(ServeSlot &sl, int64_t n, int64_t ntok, uint32_t jobs)
+static int slot_reserve(ServeSlot &sl, int64_t n, int64_t ntok, uint32_t jobs, int64_t items)
 {
     fgss_device_sets *d = sl.d;
+    d->cap_items = std::max<uint32_t>(d->cap_items, (uint32_t)std::min<int64_t>(items, UINT32_MAX / 2));
     const StageLayout L(n, ntok, jobs);
     GROW(sl.b_in, L.words * 4);
     {
@@ -3261,7 +3264,11 @@ extern "C" int fgss_server_create(const fgss_collection *indexed, float t_min, i
         // (at most 2^24 tokens) and kReserveJobs requests
         if (!srv->empty) {
             const int64_t mean_len = (x->ntok + x->n - 1) / x->n;
-            rc = slot_reserve(sl, srv->max_probes, std::min<int64_t>(srv->max_probes * mean_len, 1ll << 24), kReserveJobs);
+            // work items: the budget's postings (16 B each) in parts of cfg.target, plus
+            // the two half-probes of every probe (about 3.4M items, 180 MB, at 16 GiB)
+            const int64_t items = srv->budget / 16 / std::max<uint32_t>(srv->cfg.target, 1u) + 2 * srv->max_probes + 4096;
+            rc = slot_reserve(sl, srv->max_probes, std::min<int64_t>(srv->max_probes * mean_len, 1ll << 24), kReserveJobs,
+                              std::min<int64_t>(items, 1ll << 23));
             if (rc) { fgss_server_free(srv); return rc; }
         }
     }
