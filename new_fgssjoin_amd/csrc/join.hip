@@ -2222,6 +2222,17 @@ static void sort_pairs_by_key(const std::vector<int32_t> &tri, std::vector<uint6
     // large result sets (batches) are sorted by up to 16 host threads: each counts and
     // scatters its own chunk, chunk after chunk per digit, so the sort stays stable
     const unsigned T = R >= ((size_t)1 << 17) ? host_workers() : 1u;
+    // key = q above the bits of the largest s, so that the passes stop at the highest
+    // bit any key holds
+    uint32_t smax = 0, qmax = 0;
+    for (size_t i = 0; i < R; i++) {
+        qmax = std::max(qmax, (uint32_t)tri[3 * i]);
+        smax = std::max(smax, (uint32_t)tri[3 * i + 1]);
+    }
+    int sbits = 0, kbits = 0;
+    while (sbits < 32 && (smax >> sbits) != 0) sbits++;
+    while (kbits < 32 && (qmax >> kbits) != 0) kbits++;
+    kbits += sbits;
     std::vector<uint64_t> key(R), idx(R), k2(R), i2(R);
     uint64_t *kb[2] = {key.data(), k2.data()}, *ib[2] = {idx.data(), i2.data()};
     std::vector<std::array<size_t, 2048>> cnt(T);
@@ -2237,11 +2248,11 @@ static void sort_pairs_by_key(const std::vector<int32_t> &tri, std::vector<uint6
         unsigned mine = 0;
         const size_t a = R * t / T, b = R * (t + 1) / T;
         for (size_t i = a; i < b; i++) {
-            key[i] = ((uint64_t)(uint32_t)tri[3 * i] << 31) | (uint32_t)tri[3 * i + 1];
+            key[i] = ((uint64_t)(uint32_t)tri[3 * i] << sbits) | (uint32_t)tri[3 * i + 1];
             idx[i] = i;
         }
         int cur = 0;
-        for (int shift = 0; shift < 62; shift += 11) {
+        for (int shift = 0; shift < kbits; shift += 11) {
             const uint64_t *ks = kb[cur], *is = ib[cur];
             std::array<size_t, 2048> &c = cnt[t];
             c.fill(0);
