@@ -13,6 +13,7 @@ join kernel and the CPU oracle baseline on this host.
 from __future__ import annotations
 
 import argparse
+import gc
 import itertools
 import json
 import os
@@ -329,6 +330,9 @@ def served_measure(fg, idx, reqs, ts, slots: int, server_kw: dict):
         live = {}
         i = 0
         shared = []
+        # the driver's own garbage-collector pauses are not the server's latency
+        gc.collect()
+        gc.disable()
         start = time.perf_counter()
         while i < n_requests or live:
             now = time.perf_counter() - start
@@ -348,6 +352,7 @@ def served_measure(fg, idx, reqs, ts, slots: int, server_kw: dict):
             if not finished and i < n_requests and not live:
                 time.sleep(max(0.0, min(arrive[i] - (time.perf_counter() - start), 0.0005)))
         makespan = time.perf_counter() - start
+        gc.enable()
         srv.close()
         return makespan, lat, out, shared
 
