@@ -13,6 +13,7 @@ join kernel and the CPU oracle baseline on this host.
 from __future__ import annotations
 
 import argparse
+import contextlib
 import gc
 import itertools
 import json
@@ -27,6 +28,19 @@ THRESHOLDS = (0.6, 0.75, 0.9)
 N_RECORDS = 100_000
 SEED = 2
 HBM_PEAK_GBS = 8000.0
+
+
+@contextlib.contextmanager
+def gc_paused():
+    """Timed regions run with the Python garbage collector paused (as `timeit` does):
+    its full passes over the objects that earlier legs leave alive would otherwise land
+    inside them (a served stream's p99 rose from ~4 ms to 14-20 ms that way)."""
+    gc.collect()
+    gc.disable()
+    try:
+        yield
+    finally:
+        gc.enable()
 
 
 def log(msg):
@@ -113,9 +127,10 @@ def batched_leg(fg, n_jobs: int, reps: int):
     for c, t in zip(colls, ts):   # warm: every collection's workspace is allocated before timing
         c.selfjoin(t)
     fg.device_synchronize()
-    t0 = time.perf_counter()
-    plain = [c.selfjoin(t) for c, t in zip(colls, ts)]
-    plain_s = time.perf_counter() - t0
+    with gc_paused():
+        t0 = time.perf_counter()
+        plain = [c.selfjoin(t) for c, t in zip(colls, ts)]
+        plain_s = time.perf_counter() - t0
     log(f"batched: {n_jobs} sequential plain joins in {plain_s * 1e3:.1f} ms")
     b = fg.Batch(colls, ts)
     tune = b.autotune(reps=2)
@@ -123,10 +138,11 @@ def batched_leg(fg, n_jobs: int, reps: int):
     times = []
     exact = True
     for r in range(reps):
-        t0 = time.perf_counter()
-        b.run(graph=True, table_slots=tune["table_slots"], load_pct=tune["load_pct"])
-        got = b.all_pairs()
-        times.append(time.perf_counter() - t0)
+        with gc_paused():
+            t0 = time.perf_counter()
+            b.run(graph=True, table_slots=tune["table_slots"], load_pct=tune["load_pct"])
+            got = b.all_pairs()
+            times.append(time.perf_counter() - t0)
         if r == 0:
             for g, p in zip(got, plain):
                 exact &= (np.array_equal(g.q, p.q) and np.array_equal(g.s, p.s)
@@ -183,9 +199,10 @@ def large_leg(fg, n: int, sample: int, steps: int = 2, t: float = 0.8):
     coll.selfjoin(t)   # sizes the workspace
     runs = []
     for _ in range(steps):
-        a = time.perf_counter()
-        p = coll.selfjoin(t)
-        runs.append((time.perf_counter() - a, p))
+        with gc_paused():
+            a = time.perf_counter()
+            p = coll.selfjoin(t)
+            runs.append((time.perf_counter() - a, p))
     runs.sort(key=lambda r: r[0])
     wall_s, p = runs[len(runs) // 2]
     st = p.stats
@@ -310,12 +327,13 @@ def served_measure(fg, idx, reqs, ts, slots: int, server_kw: dict):
     for k in range(min(8, n_requests)):   # warm the slot's workspace
         srv.result(srv.submit(reqs[k], ts[k]))
     plain, plain_lat = [], []
-    t1 = time.perf_counter()
-    for r, t in zip(reqs, ts):
-        a = time.perf_counter()
-        plain.append(srv.result(srv.submit(r, t))[0])
-        plain_lat.append(time.perf_counter() - a)
-    plain_s = time.perf_counter() - t1
+    with gc_paused():
+        t1 = time.perf_counter()
+        for r, t in zip(reqs, ts):
+            a = time.perf_counter()
+            plain.append(srv.result(srv.submit(r, t))[0])
+            plain_lat.append(time.perf_counter() - a)
+        plain_s = time.perf_counter() - t1
     srv.close()
     log(f"served: plain sequential {n_requests} requests in {plain_s * 1e3:.1f} ms")
 
@@ -330,29 +348,26 @@ def served_measure(fg, idx, reqs, ts, slots: int, server_kw: dict):
         live = {}
         i = 0
         shared = []
-        # the driver's own garbage-collector pauses are not the server's latency
-        gc.collect()
-        gc.disable()
-        start = time.perf_counter()
-        while i < n_requests or live:
-            now = time.perf_counter() - start
-            while i < n_requests and arrive[i] <= now:
-                live[srv.submit(reqs[i], ts[i])] = i
-                i += 1
-            # with every request submitted, wait for the oldest run instead of spinning
-            srv.poll(block=i >= n_requests)
-            # runs are admitted first come first served, so requests finish about in
-            # arrival order: look at the oldest live ones only
-            finished = [rid for rid in itertools.islice(live, 64) if srv.done(rid)]
-            for rid in finished:
-                k = live.pop(rid)
-                out[k], st = srv.result(rid, wait=False)
-                lat[k] = time.perf_counter() - start - arrive[k]
-                shared.append(st["batch_requests"])
-            if not finished and i < n_requests and not live:
-                time.sleep(max(0.0, min(arrive[i] - (time.perf_counter() - start), 0.0005)))
-        makespan = time.perf_counter() - start
-        gc.enable()
+        with gc_paused():
+            start = time.perf_counter()
+            while i < n_requests or live:
+                now = time.perf_counter() - start
+                while i < n_requests and arrive[i] <= now:
+                    live[srv.submit(reqs[i], ts[i])] = i
+                    i += 1
+                # with every request submitted, wait for the oldest run instead of spinning
+                srv.poll(block=i >= n_requests)
+                # runs are admitted first come first served, so requests finish about in
+                # arrival order: look at the oldest live ones only
+                finished = [rid for rid in itertools.islice(live, 64) if srv.done(rid)]
+                for rid in finished:
+                    k = live.pop(rid)
+                    out[k], st = srv.result(rid, wait=False)
+                    lat[k] = time.perf_counter() - start - arrive[k]
+                    shared.append(st["batch_requests"])
+                if not finished and i < n_requests and not live:
+                    time.sleep(max(0.0, min(arrive[i] - (time.perf_counter() - start), 0.0005)))
+            makespan = time.perf_counter() - start
         srv.close()
         return makespan, lat, out, shared
 
@@ -437,14 +452,15 @@ def main():
         step()
         log(f"warmup {i + 1}/{args.warmup} done")
     sync()
-    t0 = time.perf_counter()
     last = None
     kms = []
-    for i in range(args.steps):
-        last = step()
-        kms.append([p.stats["ms_kernel"] for p in last])
-    sync()
-    dt = time.perf_counter() - t0
+    with gc_paused():
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            last = step()
+            kms.append([p.stats["ms_kernel"] for p in last])
+        sync()
+        dt = time.perf_counter() - t0
     log(f"timed {args.steps} steps in {dt:.3f} s")
     cand = sum(p.stats["candidates"] for p in last)
     ms_per_step = dt * 1e3 / args.steps
