@@ -73,12 +73,12 @@ static void parallel_for(int64_t n, int64_t chunk, F &&f)
 // ---------------------------------------------------------------------------
 // collection helpers
 // ---------------------------------------------------------------------------
-void fgss_finish_collection(fgss_collection *c)
+int fgss_finish_collection(fgss_collection *c)
 {
     const int64_t n = (int64_t)c->len.size();
     c->pos.resize(n);
     int64_t p = 0;
-    int32_t terms = 0, mx = 0;
+    int32_t mx = 0;
     bool desc = true;
     for (int64_t i = 0; i < n; i++) {
         c->pos[i] = p;
@@ -86,10 +86,18 @@ void fgss_finish_collection(fgss_collection *c)
         mx = std::max(mx, c->len[i]);
         if (i > 0 && c->len[i] > c->len[i - 1]) desc = false;
     }
-    for (int32_t v : c->tok) terms = std::max(terms, v + 1);
-    c->n_terms = terms;
+    // num_terms = max token + 1 (data.cu:77-78), in 64 bits: a token id of 2^31-1
+    // would make the int32 count wrap
+    int64_t terms = 0;
+    for (int32_t v : c->tok) terms = std::max<int64_t>(terms, (int64_t)v + 1);
     c->max_len = mx;
     c->sorted_desc = desc;
+    if (terms > (int64_t)INT32_MAX) {
+        c->n_terms = INT32_MAX;
+        return fgss_fail(FGSS_ERR_UNSUPPORTED, "token id %lld: ids must be below 2^31-1", (long long)(terms - 1));
+    }
+    c->n_terms = (int32_t)terms;
+    return FGSS_OK;
 }
 
 extern "C" void fgss_collection_free(fgss_collection *c)
@@ -152,7 +160,11 @@ extern "C" int fgss_collection_from_bin_buffer(const int32_t *ints, int64_t n_in
         memcpy(c->tok.data() + p, ints + s + 2, (size_t)ints[s + 1] * sizeof(int32_t));
         p += ints[s + 1];
     }
-    fgss_finish_collection(c);
+    const int rc = fgss_finish_collection(c);
+    if (rc) {
+        delete c;
+        return rc;
+    }
     *out = c;
     return FGSS_OK;
 }
@@ -376,7 +388,7 @@ static int tokenize_text(const char *text, int64_t nbytes, int q, const fgss_col
         c->id[k] = (int32_t)(r + 1);
         c->len[k] = (int32_t)(roff[r + 1] - roff[r]);
     }
-    fgss_finish_collection(c);
+    (void)fgss_finish_collection(c);   // offsets and lengths (no tokens yet; n_terms is set below)
     c->tok.resize((size_t)total);
     c->loc.resize((size_t)total);
     parallel_for(n, 256, [&](int64_t k) {

@@ -33,7 +33,8 @@ struct fgss_collection {
 };
 
 // Fill len/pos/n_terms/max_len/sorted_desc from id/len/tok already in internal order.
-void fgss_finish_collection(fgss_collection *c);
+// FGSS_ERR_UNSUPPORTED (n_terms clamped) when a token id is 2^31-1 (n_terms would not fit int32).
+int fgss_finish_collection(fgss_collection *c);
 
 // Device-side teardown (join.hip).
 void fgss_device_release(fgss_collection *c);

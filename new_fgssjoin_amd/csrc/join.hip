@@ -176,7 +176,7 @@ struct fgss_device_sets {
     float *tj = nullptr;        // per job: threshold of the current run
     DevBuf b_psid, b_fence, b_bcnt, b_bofs, b_bounds, b_probelen, b_poff, b_rng, b_eq, b_midlen, b_midoff, b_ka,
         b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_pl_of, b_post, b_loff, b_hi, b_nparts, b_partoff, b_scanscr, b_items,
-        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc, b_nitems, b_ovfp, b_wctr;
+        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc, b_nitems, b_ovfp, b_wctr, b_jsave;
     uint32_t cap_items = 0, cap_bounds = 0, cap_out = 0;   // current capacities (elements)
     uint32_t cap_ovf = 0;         // overflow records (a streamed item can record one per part)
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -238,7 +238,7 @@ void fgss_device_release(fgss_collection *c)
                       &d->b_rng, &d->b_eq, &d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib,
                       &d->b_sortscr, &d->b_sid_of, &d->b_pl_of, &d->b_post, &d->b_loff, &d->b_hi, &d->b_nparts, &d->b_partoff,
                       &d->b_scanscr, &d->b_items, &d->b_items2, &d->b_ovf, &d->b_out, &d->b_ctr, &d->b_jctr,
-                      &d->b_nitems, &d->b_ovfp, &d->b_wctr,
+                      &d->b_nitems, &d->b_ovfp, &d->b_wctr, &d->b_jsave,
                       &d->b_misc};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -325,10 +325,16 @@ static int validate_collection(const fgss_collection *c)
             start = end;
         }
     }
+    if (c->n_terms < 0 || c->n_terms >= INT32_MAX)
+        return fgss_fail(FGSS_ERR_UNSUPPORTED, "token ids must be below 2^31-1");
     for (int64_t i = 0; i < n; i++) {
         const int32_t *r = c->tok.data() + c->pos[i];
         for (int32_t k = 0; k < c->len[i]; k++) {
             if (r[k] < 0) return fgss_fail(FGSS_ERR_UNSUPPORTED, "negative token id in record %lld", (long long)i);
+            // the index's list offsets have n_terms + 1 entries: a larger id would index past them
+            if (r[k] >= c->n_terms)
+                return fgss_fail(FGSS_ERR_UNSUPPORTED, "record %lld: token id %d not below n_terms %d", (long long)i,
+                                 r[k], c->n_terms);
             if (k > 0 && r[k] <= r[k - 1])
                 return fgss_fail(FGSS_ERR_UNSUPPORTED, "record %lld: tokens must be strictly ascending", (long long)i);
         }
@@ -2143,9 +2149,17 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
         if (rc) return rc;
         rc = ensure(d->b_ovfp, (size_t)cnt * 4);
         if (rc) return rc;
-        for (;;) {
+        // the round's launch adds to the per-job counters as its items complete: kept
+        // before it, so that a launch repeated for a larger result buffer counts once
+        const size_t jbytes = (size_t)d->njobs * kJobCtr * sizeof(uint64_t);
+        rc = ensure(d->b_jsave, jbytes);
+        if (rc) return rc;
+        HIPCK(hipMemcpyAsync(d->b_jsave.p, d->b_jctr.p, jbytes, hipMemcpyDeviceToDevice, st));
+        for (int tries = 0;; tries++) {
+            if (tries > 8) return fgss_fail(FGSS_ERR_DEVICE, "internal: result buffer sizing did not converge");
             unsigned long long *ctr = P<unsigned long long>(d->b_ctr);
             uint32_t *misc = P<uint32_t>(d->b_misc);
+            if (tries > 0) HIPCK(hipMemcpyAsync(d->b_jctr.p, d->b_jsave.p, jbytes, hipMemcpyDeviceToDevice, st));
             HIPCK(fill<unsigned long long>(ctr, C_COUNT, 0ull, st));
             HIPCK(fill<uint32_t>(misc + M_NRESPLIT, 1, cnt, st));
             JoinArgs A = ops.args();
@@ -2504,6 +2518,14 @@ extern "C" int fgss_batch_create(fgss_collection *const *colls, const float *thr
     m.tok.reserve((size_t)ntok);
     m.job.reserve((size_t)nrec);
     int64_t term_off = 0;
+    for (int32_t j = 0; j < njobs; j++) {   // every job's term range must fit int32 once offset
+        term_off += colls[j]->n_terms;
+        if (term_off >= (int64_t)INT32_MAX) {
+            delete b;
+            return fgss_fail(FGSS_ERR_UNSUPPORTED, "batch exceeds 2^31-1 distinct terms");
+        }
+    }
+    term_off = 0;
     for (int32_t j = 0; j < njobs; j++) {
         const fgss_collection *c = colls[j];
         b->colls.push_back(c);
@@ -2516,12 +2538,8 @@ extern "C" int fgss_batch_create(fgss_collection *const *colls, const float *thr
         m.jend.push_back((uint32_t)m.len.size());
         term_off += c->n_terms;
     }
-    if (term_off >= (int64_t)INT32_MAX) {
-        delete b;
-        return fgss_fail(FGSS_ERR_UNSUPPORTED, "batch exceeds 2^31 distinct terms");
-    }
     b->rec_begin.push_back((int64_t)m.len.size());
-    fgss_finish_collection(&m);
+    (void)fgss_finish_collection(&m);   // the offset ids stay below 2^31-1 (checked above)
     m.n_terms = (int32_t)term_off;
     int rc = fgss_collection_upload(&m);
     if (rc) { fgss_device_release(&m); delete b; return rc; }
@@ -3450,3 +3468,4 @@ extern "C" void fgss_server_free(fgss_server *srv)
     }
     delete srv;
 }
+

@@ -533,7 +533,7 @@ extern "C" int fgss_collection_from_text_device(const char *text, int64_t nbytes
         TKCK(hipMemcpyAsync(c->len.data(), dlen, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     }
     TKCK(hipStreamSynchronize(st));
-    fgss_finish_collection(c);   // offsets and lengths (no tokens yet)
+    (void)fgss_finish_collection(c);   // offsets and lengths (no tokens yet; n_terms is set below)
     c->tok.resize((size_t)G);
     c->loc.resize((size_t)G);
     c->dict_hash.resize((size_t)nd);

@@ -64,6 +64,9 @@ def lib():
         L.orc_selfjoin_queries.restype = C.c_int64
         L.orc_selfjoin_queries.argtypes = [C.c_int64, i32p, i64p, i32p, C.c_int32, f, i64p, C.c_int64, C.c_int,
                                            C.POINTER(i32p), C.POINTER(Counters)]
+        L.orc_selfjoin_ex.restype = C.c_int64
+        L.orc_selfjoin_ex.argtypes = [C.c_int64, i32p, i64p, i32p, C.c_int32, f, C.c_int, i64p, C.c_int64, C.c_int,
+                                      C.POINTER(i32p), C.POINTER(Counters)]
         L.orc_selfjoin_literal.restype = C.c_int64
         L.orc_selfjoin_literal.argtypes = [C.c_int64, i32p, i64p, i32p, C.c_int32, f, C.c_int64,
                                            C.POINTER(i32p), C.POINTER(Counters)]
@@ -144,15 +147,22 @@ class Sets:
         return self.tok[self.pos[i]:self.pos[i] + self.len[i]]
 
 
-def selfjoin(sets: Sets, t: float, nthreads: int = 0, mode: str = "closed", block_size: int = 0, queries=None):
+def selfjoin(sets: Sets, t: float, nthreads: int = 0, mode: str = "closed", block_size: int = 0, queries=None,
+             checks: bool = True):
     """-> (triples int32 [R,3] of (q, s, overlap) internal indices sorted, counters dict).
-    queries: restrict the closed form to these query indices."""
+    queries: restrict the closed form to these query indices.  checks=False verifies
+    each candidate with the reference's merge alone (no closed-form cross-check; the
+    CPU baseline's timed pass)."""
     L = lib()
     if nthreads <= 0:
         nthreads = os.cpu_count() or 1
     out = C.POINTER(C.c_int32)()
     ctr = Counters()
-    if queries is not None:
+    if not checks and mode == "closed":
+        qs = np.ascontiguousarray(queries if queries is not None else np.zeros(0), np.int64)
+        n = L.orc_selfjoin_ex(sets.n, _p32(sets.len), _p64(sets.pos), _p32(sets.tok), sets.n_terms, t, nthreads,
+                              _p64(qs) if queries is not None else None, qs.size, 0, C.byref(out), C.byref(ctr))
+    elif queries is not None:
         qs = np.ascontiguousarray(queries, np.int64)
         n = L.orc_selfjoin_queries(sets.n, _p32(sets.len), _p64(sets.pos), _p32(sets.tok), sets.n_terms, t,
                                    _p64(qs), qs.size, nthreads, C.byref(out), C.byref(ctr))

@@ -429,6 +429,7 @@ typedef struct {
     int64_t n; const int32_t *len; const int64_t *pos; const int32_t *tok; float t;
     const orc_index *ix;
     const int64_t *queries; int64_t nq;   /* a query subset (NULL: every record) */
+    int checks;                           /* 0: the reference's merge only, once per candidate */
     atomic_llong next;
 } cf_shared;
 
@@ -489,6 +490,17 @@ static void *cf_run(void *arg)
                 uint32_t minrem = rq < rs ? rq : rs;
                 if ((float)(m + minrem) < bound) continue;        /* positional filter */
                 w->c.candidates++;
+                if (!sh->checks) {   /* checking.cu:93-175 alone: its score is the overlap or 0 */
+                    uint32_t sc = rq_verify(Q, lq, S, ls, t, m, &w->c.m_reads);
+                    if (sc > 0) {
+                        w->c.intersections++;
+                        if (orc_similarity(lq, ls, sc) >= t) {             /* block.cu:189-192 */
+                            tri_push(&w->out, (int32_t)q, s, (int32_t)sc);
+                            w->c.results++;
+                        }
+                    }
+                    continue;
+                }
                 uint32_t ov = overlap_full(Q, lq, S, ls);
                 uint32_t rqs = rq_verify(Q, lq, S, ls, t, m, &w->c.m_reads);
                 w->c.v_reads += suffix_reads(Q, lq, S, ls, qp, orc_mid_prefix(ls, t));
@@ -513,7 +525,7 @@ static void *cf_run(void *arg)
 
 static int64_t selfjoin_closed(int64_t n_sets, const int32_t *len, const int64_t *pos,
                                const int32_t *tok, int32_t n_terms, float t, int nthreads,
-                               const int64_t *queries, int64_t nq, int32_t **out, orc_counters *ctr)
+                               const int64_t *queries, int64_t nq, int checks, int32_t **out, orc_counters *ctr)
 {
     if (nthreads < 1) nthreads = 1;
     orc_index ix;
@@ -522,7 +534,7 @@ static int64_t selfjoin_closed(int64_t n_sets, const int32_t *len, const int64_t
     build_index(n_sets, len, pos, tok, n_terms, t, &ix, &total.e_idx);
     cf_shared sh;
     sh.n = n_sets; sh.len = len; sh.pos = pos; sh.tok = tok; sh.t = t; sh.ix = &ix;
-    sh.queries = queries; sh.nq = nq;
+    sh.queries = queries; sh.nq = nq; sh.checks = checks;
     atomic_init(&sh.next, 0);
     cf_worker *ws = (cf_worker *)calloc((size_t)nthreads, sizeof(cf_worker));
     pthread_t *th = (pthread_t *)malloc((size_t)nthreads * sizeof(pthread_t));
@@ -550,7 +562,17 @@ int64_t orc_selfjoin(int64_t n_sets, const int32_t *len, const int64_t *pos,
                      const int32_t *tok, int32_t n_terms, float t, int nthreads,
                      int32_t **out, orc_counters *ctr)
 {
-    return selfjoin_closed(n_sets, len, pos, tok, n_terms, t, nthreads, NULL, 0, out, ctr);
+    return selfjoin_closed(n_sets, len, pos, tok, n_terms, t, nthreads, NULL, 0, 1, out, ctr);
+}
+
+int64_t orc_selfjoin_ex(int64_t n_sets, const int32_t *len, const int64_t *pos, const int32_t *tok,
+                        int32_t n_terms, float t, int nthreads, const int64_t *queries, int64_t n_queries,
+                        int checks, int32_t **out, orc_counters *ctr)
+{
+    for (int64_t k = 0; queries && k < n_queries; k++)
+        if (queries[k] < 0 || queries[k] >= n_sets) return -1;
+    return selfjoin_closed(n_sets, len, pos, tok, n_terms, t, nthreads, queries, queries ? n_queries : 0, checks,
+                           out, ctr);
 }
 
 int64_t orc_selfjoin_queries(int64_t n_sets, const int32_t *len, const int64_t *pos,
@@ -559,7 +581,7 @@ int64_t orc_selfjoin_queries(int64_t n_sets, const int32_t *len, const int64_t *
 {
     for (int64_t k = 0; k < n_queries; k++)
         if (queries[k] < 0 || queries[k] >= n_sets) return -1;
-    return selfjoin_closed(n_sets, len, pos, tok, n_terms, t, nthreads, queries, n_queries, out, ctr);
+    return selfjoin_closed(n_sets, len, pos, tok, n_terms, t, nthreads, queries, n_queries, 1, out, ctr);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -27,6 +27,9 @@
  *                          restatement of the correctly rounded float32 formulas
  *   orc_selfjoin_queries   == orc_selfjoin restricted to the query subset (pairs and
  *                          counters summed over the subset)
+ *   orc_selfjoin_ex        checks = 1: orc_selfjoin / _queries; checks = 0: the same
+ *                          triples and counters (v_reads aside), its verification
+ *                          being the one the checks=1 run compares with the closed form
  *   orc_selfjoin           == orc_selfjoin_literal for block sizes {1, 7, N}
  *                          (two independent restatements: closed form vs literal
  *                          tile loop), subset of orc_selfjoin_brute with the
@@ -101,6 +104,13 @@ int64_t orc_selfjoin(int64_t n_sets, const int32_t *len, const int64_t *pos,
 int64_t orc_selfjoin_queries(int64_t n_sets, const int32_t *len, const int64_t *pos,
                              const int32_t *tok, int32_t n_terms, float t, const int64_t *queries,
                              int64_t n_queries, int nthreads, int32_t **out, orc_counters *ctr);
+/* orc_selfjoin (queries null) or orc_selfjoin_queries, with checks = 0: each
+ * candidate is verified by the reference's merge alone (checking.cu:93-175, the
+ * score it stores), without the closed-form overlap it is checked against and the
+ * suffix read count (v_reads stays 0).  The CPU baseline's timed pass. */
+int64_t orc_selfjoin_ex(int64_t n_sets, const int32_t *len, const int64_t *pos, const int32_t *tok,
+                        int32_t n_terms, float t, int nthreads, const int64_t *queries, int64_t n_queries,
+                        int checks, int32_t **out, orc_counters *ctr);
 int64_t orc_selfjoin_literal(int64_t n_sets, const int32_t *len, const int64_t *pos,
                              const int32_t *tok, int32_t n_terms, float t,
                              int64_t block_size, int32_t **out, orc_counters *ctr);

@@ -267,3 +267,23 @@ def test_streamed_chunks(fg, oracle_mod, load):
     for t in (0.5, 0.8):
         _say(f"streamed chunks t={t} load={load}")
         assert_join_equal(oracle_mod, sets, coll.selfjoin(t, table_slots=256, load_pct=load, stream_parts=1), t)
+
+
+def test_resplit_round_larger_than_the_result_buffer(fg, oracle_mod):
+    """700 identical records among 20,000 unrelated ones of the same length: the
+    planned parts of the first 440 queries hold more distinct sources than a 256-slot
+    table, so they are re-split, and the re-split round finds about 212k pairs, more
+    than a fresh collection's 65,536-triple result buffer.  That round is launched
+    again with a larger buffer; the counters of its first launch must not count twice."""
+    from _util import records_to_bin
+    recs = [list(range(20))] * 700 + [list(range(100 + 20 * i, 120 + 20 * i)) for i in range(20000)]
+    ints = records_to_bin(recs)
+    sets = oracle_mod.Sets(ints)
+    want = oracle_mod.selfjoin(sets, 0.5)
+    coll = fg.Collection.from_bin_ints(ints)
+    _say("resplit retry: 700 duplicates + 20000 singles, t=0.5, table_slots=256")
+    p = coll.selfjoin(0.5, table_slots=256)
+    assert p.stats["kernel_launches"] >= 3, p.stats["kernel_launches"]   # main, re-split, its repeat
+    assert_join_equal(oracle_mod, sets, p, 0.5, counters=want)
+    _say("resplit retry: the same collection again (buffers now sized)")
+    assert_join_equal(oracle_mod, sets, coll.selfjoin(0.5, table_slots=256), 0.5, counters=want)

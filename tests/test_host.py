@@ -157,3 +157,22 @@ def test_cli_argument_errors(fg, tmp_path):
         assert r.returncode == 1 and "Threshold must be" in r.stderr, t
     r = _cli(fg, "-f", golden_path("small.txt"), "-q", "0", "-t", "0.5")
     assert r.returncode == 1
+
+
+def test_token_id_range_rejected_before_device_work(fg):
+    """A token id of 2^31-1 would make n_terms = max id + 1 wrap in int32 (the index's
+    list offsets then hold fewer entries than a record's ids reach): the loader
+    refuses it; ids up to 2^31-2 still load with n_terms = max id + 1."""
+    with pytest.raises(fg.FgssError) as e:
+        fg.Collection.from_bin_ints(np.array([7, 1, 2147483647, 8, 2, 3, 5], np.int32))
+    assert e.value.status == fg.FGSS_ERR_UNSUPPORTED
+    c = fg.Collection.from_bin_ints(np.array([7, 1, 2147483646, 8, 2, 3, 5], np.int32))
+    assert c.info.n_terms == 2147483647
+    with pytest.raises(fg.FgssError) as e:
+        c.upload()   # n_terms + 1 list offsets would not fit the index's 32-bit term space
+    assert e.value.status == fg.FGSS_ERR_UNSUPPORTED
+    neg = fg.Collection.from_bin_ints(np.array([7, 2, -4, 1], np.int32))
+    with pytest.raises(fg.FgssError) as e:
+        neg.upload()   # validated on the host before the device is touched
+    assert e.value.status == fg.FGSS_ERR_UNSUPPORTED
+

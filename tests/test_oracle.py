@@ -221,3 +221,26 @@ def test_query_subset_equals_full_join_restricted(oracle_mod, t):
     assert sc["e_idx"] == ctr["e_idx"]
     with pytest.raises(ValueError):
         oracle_mod.selfjoin(s, t, queries=[s.n])
+
+
+@pytest.mark.parametrize("t", [0.5, 0.8])
+def test_single_merge_mode_equals_checked_join(oracle_mod, t):
+    """checks=False (each candidate verified by the reference's merge alone, the CPU
+    baseline's timed pass) gives the checked join's triples and counters, on the whole
+    collection, on a query subset and on the literal tile loop's answer."""
+    b, _ = oracle_mod.tokenize(_titles(900, 41), 3)
+    s = oracle_mod.Sets(b)
+    tri, ctr = oracle_mod.selfjoin(s, t)
+    fast, fc = oracle_mod.selfjoin(s, t, checks=False)
+    assert np.array_equal(fast, tri)
+    lit, _ = oracle_mod.selfjoin(s, t, mode="literal", block_size=s.n)
+    assert np.array_equal(fast, lit)
+    assert fc["v_reads"] == 0
+    for k in ("e_idx", "p_q", "e_scan", "m_reads", "candidates", "intersections", "results"):
+        assert fc[k] == ctr[k], k
+    qs = np.random.default_rng(9).choice(s.n, 150, replace=False)
+    sub, sc = oracle_mod.selfjoin(s, t, queries=qs)
+    fsub, fsc = oracle_mod.selfjoin(s, t, queries=qs, checks=False)
+    assert np.array_equal(fsub, sub)
+    for k in ("p_q", "e_scan", "m_reads", "candidates", "intersections", "results"):
+        assert fsc[k] == sc[k], k
