@@ -161,6 +161,28 @@ typedef struct {
 int fgss_selfjoin(fgss_collection *c, float threshold, const fgss_join_opts *opts, fgss_pairs **out);
 void fgss_pairs_free(fgss_pairs *p);
 
+/* Self-join without allocating.  No counterpart in the reference: for callers that
+ * cannot allocate device or host memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  The records are already in
+ * device memory in the layout fgss_collection_arrays describes (internal order,
+ * lengths non-increasing, tokens strictly ascending in [0, n_terms), offsets the
+ * exclusive scan of the lengths; int32 each); every workspace buffer lies in one
+ * static device array of FGSS_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends, so collections above the limits below fail with FGSS_ERR_UNSUPPORTED.
+ * Result triples (q, s, overlap), the same set fgss_selfjoin returns, in no fixed
+ * order, go to the device buffer out (out_cap triples); *n_out receives their
+ * number (with FGSS_ERR_ARG and the count found so far when out_cap is too small).
+ * stats (may be null) receives the counters, without timing events.  All work runs
+ * on `stream` (a hipStream_t; null is the null stream) and the call returns after
+ * it has drained.  Calls that overlap on one GPU share its array and race. */
+#define FGSS_NOALLOC_MAX_SETS 65536
+#define FGSS_NOALLOC_MAX_TOKENS 524288
+#define FGSS_NOALLOC_MAX_TERMS 524288
+#define FGSS_NOALLOC_SCRATCH_BYTES (72u << 20)
+int fgss_selfjoin_noalloc(const int32_t *tokens, const int32_t *lengths, const int32_t *offsets, int64_t n_sets,
+                          int64_t n_tokens, int32_t n_terms, float threshold, const fgss_join_opts *opts,
+                          int32_t *out, int64_t out_cap, int64_t *n_out, fgss_join_stats *stats, void *stream);
+
 /* Print the self-join's inverted index at threshold t as the reference's debug
  * mode does (print_inverted_index, src/fgssjoin/index.cu:218-243): per term,
  * "term: [t], size: n" and the ids of the records whose mid prefix holds it.  Lists

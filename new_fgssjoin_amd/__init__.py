@@ -39,7 +39,7 @@ EXPORTS = (
     "fgss_collection_from_text_device",
     "fgss_collection_write_bin", "fgss_collection_get_info", "fgss_collection_arrays",
     "fgss_collection_upload", "fgss_collection_free", "fgss_device_synchronize",
-    "fgss_selfjoin", "fgss_pairs_free", "fgss_pairs_print", "fgss_collection_print_index",
+    "fgss_selfjoin", "fgss_selfjoin_noalloc", "fgss_pairs_free", "fgss_pairs_print", "fgss_collection_print_index",
     "fgss_generate_text", "fgss_free",
     "fgss_batch_create", "fgss_batch_set_thresholds", "fgss_batch_run", "fgss_batch_pairs",
     "fgss_batch_pairs_all", "fgss_batch_autotune", "fgss_batch_free",
@@ -131,6 +131,8 @@ def lib():
         "fgss_collection_upload": [vp],
         "fgss_device_synchronize": [],
         "fgss_selfjoin": [vp, C.c_float, C.POINTER(JoinOpts), C.POINTER(C.POINTER(_Pairs))],
+        "fgss_selfjoin_noalloc": [vp, vp, vp, C.c_int64, C.c_int64, C.c_int32, C.c_float, C.POINTER(JoinOpts), vp,
+                                  C.c_int64, i64p, C.POINTER(JoinStats), vp],
         "fgss_pairs_print": [C.POINTER(_Pairs), vp],
         "fgss_collection_print_index": [vp, C.c_float, vp],
         "fgss_generate_text": [C.c_int, C.c_int64, C.c_uint64, C.POINTER(C.c_void_p), i64p],
@@ -318,6 +320,25 @@ class Collection:
         out = C.POINTER(_Pairs)()
         _check(L.fgss_selfjoin(self._h, C.c_float(threshold), C.byref(opts), C.byref(out)))
         return _take_pairs(out)
+
+
+NOALLOC_MAX_SETS, NOALLOC_MAX_TOKENS, NOALLOC_MAX_TERMS = 65536, 524288, 524288
+
+
+def selfjoin_noalloc(tokens: int, lengths: int, offsets: int, n_sets: int, n_tokens: int, n_terms: int,
+                     threshold: float, out: int, out_cap: int, stream: int = 0, table_slots: int = 0,
+                     load_pct: int = 0):
+    """fgss_selfjoin_noalloc on device addresses (int): the records in device memory
+    (int32 tokens, lengths, offsets in internal order) and a device output of out_cap
+    (q, s, overlap) int32 triples.  Returns (number of triples written, stats dict);
+    the triples are in no fixed order.  Nothing is allocated by the call."""
+    n = C.c_int64()
+    st = JoinStats()
+    opts = JoinOpts(load_pct, table_slots, 0, 0)
+    _check(lib().fgss_selfjoin_noalloc(C.c_void_p(tokens), C.c_void_p(lengths), C.c_void_p(offsets), n_sets,
+                                       n_tokens, n_terms, C.c_float(threshold), C.byref(opts), C.c_void_p(out),
+                                       out_cap, C.byref(n), C.byref(st), C.c_void_p(stream)))
+    return n.value, st.as_dict()
 
 
 def _take_pairs(out) -> Pairs:

@@ -36,6 +36,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <type_traits>
 #include <vector>
@@ -3469,3 +3470,354 @@ extern "C" void fgss_server_free(fgss_server *srv)
     delete srv;
 }
 
+// ---------------------------------------------------------------------------
+// fgss_selfjoin_noalloc: the self-join of records the caller already holds in
+// device memory, with every workspace buffer carved out of one static device
+// array.  The call makes no device or host allocation (no hipMalloc, hipFree,
+// pinned memory, events or streams): for callers that cannot allocate.  It runs
+// the same kernels as fgss_selfjoin; capacities are fixed by the array, so a
+// collection or a run that does not fit is rejected instead of growing buffers.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocMaxSets = FGSS_NOALLOC_MAX_SETS;
+constexpr int64_t kNoAllocMaxTokens = FGSS_NOALLOC_MAX_TOKENS;
+constexpr int64_t kNoAllocMaxTerms = FGSS_NOALLOC_MAX_TERMS;
+constexpr uint32_t kNoAllocItems = 1u << 18;    // work items of the main launch
+constexpr uint32_t kNoAllocBounds = 1u << 19;   // boundary-table entries
+constexpr uint32_t kNoAllocSplit = 1u << 16;    // re-split items per launch; overflow records per launch
+constexpr uint32_t kNoAllocPending = 1u << 15;  // overflowed items waiting for the next re-split round
+constexpr int kNoAllocMaxCUs = 512;             // wavefront counter slots for up to this many CUs
+constexpr size_t kNoAllocBytes = (size_t)FGSS_NOALLOC_SCRATCH_BYTES;
+
+// The workspace of every no-allocation call on this GPU (calls that overlap on one
+// GPU race on it).  Host code never uses this name as an address: the device
+// address comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+static std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices];
+
+// Device address of `scratch` on the current device (looked up on its first use there).
+static int noalloc_scratch(int dev, uint8_t **out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return fgss_fail(FGSS_ERR_DEVICE, "device %d out of range", dev);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fgss_fail(FGSS_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = (uint8_t *)g_scratch_ptr[dev];
+    return FGSS_OK;
+}
+
+// Carves the workspace of a no-allocation run out of `base` (null: only sizes it),
+// the buffers and sizes of prepare_buffers at their upper bounds.  Returns the bytes.
+struct NoAllocBufs {
+    uint32_t *err = nullptr;     // validation flags
+    Item *split = nullptr;       // children of a re-split launch (kNoAllocSplit)
+    Item *pend[2] = {nullptr, nullptr};   // overflowed items between rounds (kNoAllocPending each)
+};
+
+static size_t noalloc_carve(fgss_device_sets *d, uint8_t *base, int64_t n, int64_t ntok, uint32_t n_terms, int cus,
+                            NoAllocBufs *nb)
+{
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> void * {
+        bytes = std::max<size_t>(bytes, 16);
+        void *p = base ? (void *)(base + off) : nullptr;
+        off = (off + bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    auto buf = [&](DevBuf &b, size_t bytes) {
+        b.p = take(bytes);
+        b.cap = bytes;
+    };
+    const int64_t N = std::max<int64_t>(n, 1), E = std::max<int64_t>(ntok, 1);
+    d->cap_items = kNoAllocItems;
+    d->cap_bounds = kNoAllocBounds;
+    d->cap_ovf = kNoAllocSplit;
+    buf(d->b_midlen, (size_t)N * 4);
+    buf(d->b_midoff, (size_t)(N + 1) * 4);
+    buf(d->b_probelen, (size_t)N * 4);
+    buf(d->b_poff, (size_t)(N + 1) * 4);
+    buf(d->b_scanscr, (size_t)scan_scratch_elems(N + 1) * 8);
+    buf(d->b_ka, (size_t)E * 4);
+    buf(d->b_kb, (size_t)E * 4);
+    buf(d->b_ia, (size_t)E * 4);
+    buf(d->b_ib, (size_t)E * 4);
+    buf(d->b_sid_of, (size_t)E * 4);
+    buf(d->b_pl_of, (size_t)E * 4);
+    buf(d->b_post, (size_t)(E + kWave) * 8);
+    buf(d->b_psid, (size_t)(E + 1) * 4);
+    buf(d->b_fence, (size_t)(E / kFence + 2) * 4);
+    buf(d->b_loff, ((size_t)n_terms + 2) * 4);
+    buf(d->b_sortscr, (size_t)radix_scratch_elems(E) * 4);
+    buf(d->b_rng, (size_t)E * sizeof(uint2));
+    buf(d->b_hi, (size_t)N * 4);
+    buf(d->b_eq, (size_t)N * 4);
+    buf(d->b_nparts, (size_t)N * 4);
+    buf(d->b_partoff, (size_t)(N + 1) * 4);
+    buf(d->b_bcnt, (size_t)N * 8);
+    buf(d->b_bofs, (size_t)(N + 1) * 8);
+    buf(d->b_nitems, (size_t)N * 4);
+    buf(d->b_bounds, (size_t)d->cap_bounds * 4);
+    buf(d->b_items, (size_t)d->cap_items * sizeof(Item));
+    buf(d->b_ovf, (size_t)d->cap_ovf * 4);
+    buf(d->b_ovfp, (size_t)d->cap_ovf * 4);
+    buf(d->b_wctr, (size_t)std::max(cus, 1) * 32 * kWaveSlot * 8);
+    buf(d->b_ctr, C_COUNT * sizeof(uint64_t));
+    buf(d->b_jctr, kJobCtr * sizeof(uint64_t));
+    buf(d->b_misc, M_COUNT * 4);
+    d->sig = (uint32_t *)take((size_t)N * 4 * kSigWords);
+    d->sig64 = (uint32_t *)take((size_t)N * 4 * kSig64Words);
+    d->sigbad = (uint8_t *)take((size_t)N);
+    d->job = (uint32_t *)take((size_t)N * 4);
+    d->jend = (uint32_t *)take(4);
+    d->tj = (float *)take(4);
+    nb->err = (uint32_t *)take(4);
+    nb->split = (Item *)take((size_t)kNoAllocSplit * sizeof(Item));
+    nb->pend[0] = (Item *)take((size_t)kNoAllocPending * sizeof(Item));
+    nb->pend[1] = (Item *)take((size_t)kNoAllocPending * sizeof(Item));
+    return off;
+}
+
+// Checks of caller-given records before any kernel indexes with them: lengths below
+// 2^16 and non-increasing, offsets the exclusive scan of the lengths, tokens strictly
+// ascending and in [0, n_terms).  A thread per record; flags or-ed into *err.
+__global__ void k_validate_records(const uint32_t *__restrict__ tok, const uint32_t *__restrict__ len,
+                                   const uint32_t *__restrict__ pos, int64_t n, int64_t ntok, uint32_t n_terms,
+                                   uint32_t *__restrict__ err)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t l = len[i], p = pos[i];
+    uint32_t e = 0;
+    if (l >= 65536u) e |= 1u;
+    if (i > 0 && l > len[i - 1]) e |= 2u;
+    if ((i == 0 && p != 0u) || (i > 0 && (uint64_t)p != (uint64_t)pos[i - 1] + len[i - 1])) e |= 4u;
+    if ((uint64_t)p + l > (uint64_t)ntok || (i == n - 1 && (uint64_t)p + l != (uint64_t)ntok)) e |= 4u;
+    if ((e & 4u) == 0) {   // the record's range lies inside the token array
+        uint32_t prev = 0;
+        for (uint32_t k = 0; k < l; k++) {
+            const uint32_t v = tok[p + k];
+            if (v >= n_terms) e |= 8u;   // also a negative int32 id
+            if (k > 0 && v <= prev) e |= 16u;
+            prev = v;
+        }
+    }
+    if (e) atomicOr(err, e);
+}
+
+// overflowed item ovf[i] of `in` (narrowed to its overflowed part for a streamed item)
+// as a searched item over its source sub-window
+__global__ void k_take_ovf(const Item *__restrict__ in, const uint32_t *__restrict__ ovf,
+                           const uint32_t *__restrict__ ovfp, uint32_t novf, Item *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= novf) return;
+    Item it = in[ovf[i]];
+    if (ovfp[i] != kEmpty) {
+        const uint64_t W0 = (uint64_t)it.b - it.a;
+        const uint32_t a = part_start(it.a, W0, ovfp[i], it.nparts), b = part_start(it.a, W0, ovfp[i] + 1, it.nparts);
+        it.a = a;
+        it.b = b;
+    }
+    it.pad &= kPadMask;
+    it.part = 0;
+    it.nparts = 0;
+    it.bofs = 0;
+    out[i] = it;
+}
+
+// every item of `in` split into `ways` equal source sub-windows
+__global__ void k_split_items(const Item *__restrict__ in, uint32_t n, uint32_t ways, Item *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Item it = in[i];
+    const uint64_t W = (uint64_t)it.b - it.a;
+    for (uint32_t p = 0; p < ways; p++) {
+        Item o = it;
+        o.a = (uint32_t)(it.a + W * p / ways);
+        o.b = (uint32_t)(it.a + W * (p + 1) / ways);
+        out[i * ways + p] = o;
+    }
+}
+
+extern "C" int fgss_selfjoin_noalloc(const int32_t *tokens, const int32_t *lengths, const int32_t *offsets,
+                                     int64_t n_sets, int64_t n_tokens, int32_t n_terms, float threshold,
+                                     const fgss_join_opts *opts, int32_t *out, int64_t out_cap, int64_t *n_out,
+                                     fgss_join_stats *stats, void *stream)
+{
+    // argument and size checks first: no device call before them
+    if (!n_out) return fgss_fail(FGSS_ERR_ARG, "null n_out");
+    *n_out = 0;
+    if (stats) memset(stats, 0, sizeof(*stats));
+    if (n_sets < 0 || n_tokens < 0 || n_terms < 0 || out_cap < 0) return fgss_fail(FGSS_ERR_ARG, "negative size");
+    if (n_sets > 0 && (!lengths || !offsets || (n_tokens > 0 && !tokens)))
+        return fgss_fail(FGSS_ERR_ARG, "null record array");
+    if (out_cap > 0 && !out) return fgss_fail(FGSS_ERR_ARG, "null output buffer");
+    if (!(threshold > 0.0f) || !(threshold < 1.0f))
+        return fgss_fail(FGSS_ERR_ARG, "Threshold must be a float in the range (0.0..1.0).");
+    if (n_sets > kNoAllocMaxSets || n_tokens > kNoAllocMaxTokens || n_terms > kNoAllocMaxTerms)
+        return fgss_fail(FGSS_ERR_UNSUPPORTED,
+                         "%lld records, %lld tokens, %d terms: the no-allocation scratch holds at most %lld records, "
+                         "%lld tokens and %lld terms",
+                         (long long)n_sets, (long long)n_tokens, n_terms, (long long)kNoAllocMaxSets,
+                         (long long)kNoAllocMaxTokens, (long long)kNoAllocMaxTerms);
+    const int ts = opts && opts->table_slots ? opts->table_slots : 512;
+    if (ts != 256 && ts != 512 && ts != 1024 && ts != 2048 && ts != 4096)
+        return fgss_fail(FGSS_ERR_ARG, "table_slots must be 256, 512, 1024, 2048 or 4096");
+    if (n_sets == 0) return FGSS_OK;
+    auto t0 = std::chrono::steady_clock::now();
+    const RunCfg cfg = make_cfg(opts);
+    const hipStream_t st = (hipStream_t)stream;
+    int dev = 0, cus = 0;
+    HIPCK(hipGetDevice(&dev));
+    HIPCK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (cus > kNoAllocMaxCUs) return fgss_fail(FGSS_ERR_UNSUPPORTED, "more than %d compute units", kNoAllocMaxCUs);
+    uint8_t *base = nullptr;
+    int rc = noalloc_scratch(dev, &base);
+    if (rc) return rc;
+    fgss_device_sets d;   // on the stack: its buffers point into the scratch
+    NoAllocBufs nb;
+    if (noalloc_carve(&d, nullptr, n_sets, n_tokens, (uint32_t)n_terms, cus, &nb) > kNoAllocBytes)
+        return fgss_fail(FGSS_ERR_UNSUPPORTED, "workspace exceeds the %zu-byte no-allocation scratch", kNoAllocBytes);
+    noalloc_carve(&d, base, n_sets, n_tokens, (uint32_t)n_terms, cus, &nb);
+    d.device = dev;
+    d.st = st;
+    d.n = n_sets;
+    d.ntok = n_tokens;
+    d.n_terms = (uint32_t)n_terms;
+    d.njobs = 1;
+    d.max_len = 65535;
+    d.tok = (uint32_t *)(n_tokens > 0 ? tokens : lengths);   // any readable address when there are no tokens
+    d.len = (uint32_t *)lengths;
+    d.pos = (uint32_t *)offsets;
+    d.b_out.p = out;
+    d.b_out.cap = (size_t)out_cap * 12;
+    d.cap_out = (uint32_t)std::min<int64_t>(out_cap, UINT32_MAX / 16);
+    uint64_t h_ctr[C_COUNT] = {0}, h_jctr[kJobCtr] = {0};
+    uint32_t h_misc[M_COUNT] = {0}, h_err = 0;
+    d.h_ctr = h_ctr;
+    d.h_misc = h_misc;
+    d.h_jctr = h_jctr;
+    d.out_hint = 0;   // no pinned result mirror
+
+    // 1. validate the caller's records
+    HIPCK(fill<uint32_t>(nb.err, 1, 0u, st));
+    hipLaunchKernelGGL(k_validate_records, dim3(grid_for(n_sets, 256)), dim3(256), 0, st, d.tok, d.len, d.pos, n_sets,
+                       n_tokens, (uint32_t)n_terms, nb.err);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(&h_err, nb.err, 4, hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    if (h_err & 1u) return fgss_fail(FGSS_ERR_UNSUPPORTED, "records longer than 65535 tokens");
+    if (h_err & 2u)
+        return fgss_fail(FGSS_ERR_UNSUPPORTED, "record lengths must be non-increasing (a size-ascending .bin)");
+    if (h_err & 4u) return fgss_fail(FGSS_ERR_ARG, "offsets are not the exclusive scan of the lengths");
+    if (h_err & 8u) return fgss_fail(FGSS_ERR_UNSUPPORTED, "token ids must lie in [0, n_terms)");
+    if (h_err & 16u) return fgss_fail(FGSS_ERR_UNSUPPORTED, "tokens must be strictly ascending in each record");
+
+    // 2. the record set's job table, threshold and signatures, then the pipeline
+    HIPCK(fill<uint32_t>(d.job, n_sets, 0u, st));
+    HIPCK(fill<uint32_t>(d.jend, 1, (uint32_t)n_sets, st));
+    HIPCK(fill<float>(d.tj, 1, threshold, st));
+    hipLaunchKernelGGL(k_signature, dim3((unsigned)std::min<int64_t>((n_sets + 3) / 4, 65536)), dim3(256), 0, st, d.tok,
+                       d.pos, d.len, n_sets, d.sig, d.sig64, d.sigbad);
+    HIPCK(hipGetLastError());
+    rc = enqueue_pipeline(&d, cfg, false);
+    if (rc) return rc;
+    HIPCK(hipStreamSynchronize(st));
+    if (h_misc[M_NITEMS] > d.cap_items)
+        return fgss_fail(FGSS_ERR_UNSUPPORTED, "%u work items exceed the no-allocation scratch's %u", h_misc[M_NITEMS],
+                         d.cap_items);
+    if (h_ctr[C_OVERFLOW] > kNoAllocPending)
+        return fgss_fail(FGSS_ERR_UNSUPPORTED, "%llu overflowed work items exceed the no-allocation scratch's %u",
+                         (unsigned long long)h_ctr[C_OVERFLOW], kNoAllocPending);
+    uint64_t written = h_ctr[C_RESULTS];
+    if (written > (uint64_t)out_cap) {
+        *n_out = (int64_t)written;
+        return fgss_fail(FGSS_ERR_ARG, "the output holds %lld triples, the join found at least %llu", (long long)out_cap,
+                         (unsigned long long)written);
+    }
+    uint64_t acc[C_COUNT] = {0};
+    for (int k = C_ESCAN; k <= C_RETRY; k++) acc[k] += h_ctr[k];
+
+    // 3. overflowed work items: split four ways and run again, in launches of at most
+    //    kNoAllocSplit items; the items that overflow again wait in the other buffer
+    uint32_t npend = (uint32_t)h_ctr[C_OVERFLOW];
+    int cur = 0, launches = 1;
+    if (npend) {
+        hipLaunchKernelGGL(k_take_ovf, dim3(grid_for(npend, 256)), dim3(256), 0, st, P<Item>(d.b_items),
+                           P<uint32_t>(d.b_ovf), P<uint32_t>(d.b_ovfp), npend, nb.pend[0]);
+        HIPCK(hipGetLastError());
+    }
+    constexpr uint32_t ways = 4, chunk = kNoAllocSplit / ways;
+    for (int round = 0; npend > 0; round++) {
+        if (round >= 48) return fgss_fail(FGSS_ERR_DEVICE, "internal: work-item splitting did not converge");
+        uint32_t nnext = 0;
+        for (uint32_t c0 = 0; c0 < npend; c0 += chunk) {
+            const uint32_t nc = std::min(chunk, npend - c0), cnt = nc * ways;
+            hipLaunchKernelGGL(k_split_items, dim3(grid_for(nc, 256)), dim3(256), 0, st, nb.pend[cur] + c0, nc, ways,
+                               nb.split);
+            HIPCK(hipGetLastError());
+            uint32_t *misc = P<uint32_t>(d.b_misc);
+            HIPCK(fill<unsigned long long>(P<unsigned long long>(d.b_ctr), C_COUNT, 0ull, st));
+            HIPCK(fill<uint32_t>(misc + M_NRESPLIT, 1, cnt, st));
+            JoinArgs A = join_args(&d);
+            A.items = nb.split;
+            A.n_items_dev = misc + M_NRESPLIT;
+            A.n_items_cap = cnt;
+            A.out = out + 3 * written;
+            A.out_cap = (uint32_t)std::min<uint64_t>((uint64_t)out_cap - written, UINT32_MAX / 16);
+            A.ovf_cap = cnt;
+            rc = launch_join_cap(&d, A, cfg.cap, false, false);
+            if (rc) return rc;
+            launches++;
+            HIPCK(hipMemcpyAsync(h_ctr, d.b_ctr.p, C_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+            HIPCK(hipStreamSynchronize(st));
+            if (written + h_ctr[C_RESULTS] > (uint64_t)out_cap) {
+                *n_out = (int64_t)(written + h_ctr[C_RESULTS]);
+                return fgss_fail(FGSS_ERR_ARG, "the output holds %lld triples, the join found at least %lld",
+                                 (long long)out_cap, (long long)*n_out);
+            }
+            written += h_ctr[C_RESULTS];
+            for (int k = C_ESCAN; k <= C_RETRY; k++) acc[k] += h_ctr[k];
+            const uint32_t o = (uint32_t)h_ctr[C_OVERFLOW];
+            if ((uint64_t)nnext + o > kNoAllocPending)
+                return fgss_fail(FGSS_ERR_UNSUPPORTED, "overflowed work items exceed the no-allocation scratch's %u",
+                                 kNoAllocPending);
+            if (o) {
+                hipLaunchKernelGGL(k_take_ovf, dim3(grid_for(o, 256)), dim3(256), 0, st, nb.split, P<uint32_t>(d.b_ovf),
+                                   P<uint32_t>(d.b_ovfp), o, nb.pend[cur ^ 1] + nnext);
+                HIPCK(hipGetLastError());
+            }
+            nnext += o;
+        }
+        cur ^= 1;
+        npend = nnext;
+    }
+    HIPCK(hipMemcpyAsync(h_jctr, d.b_jctr.p, kJobCtr * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    *n_out = (int64_t)written;
+    if (stats) {   // the counters fgss_selfjoin reports (no timing events here)
+        stats->index_entries = h_misc[M_NE];
+        stats->probe_tokens = h_misc[M_NP];
+        stats->postings_scanned = (int64_t)h_jctr[0];
+        stats->candidates = (int64_t)h_jctr[1];
+        stats->intersections = (int64_t)h_jctr[2];
+        stats->results = (int64_t)h_jctr[3];
+        stats->verify_tokens = (int64_t)acc[C_VTOK];
+        stats->signature_checks = (int64_t)acc[C_SIGCHK];
+        stats->signature_checks_128 = (int64_t)acc[C_SIG2];
+        stats->verified = (int64_t)acc[C_VERIFIED];
+        for (int k = 0; k < 6; k++) stats->phase_cycles[k] = (int64_t)acc[C_CYC_SETUP + k];
+        stats->window_retries = (int64_t)acc[C_RETRY];
+        stats->kernel_launches = launches;
+        stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return FGSS_OK;
+}

@@ -176,3 +176,25 @@ def test_token_id_range_rejected_before_device_work(fg):
         neg.upload()   # validated on the host before the device is touched
     assert e.value.status == fg.FGSS_ERR_UNSUPPORTED
 
+
+def test_noalloc_argument_and_size_checks(fg):
+    """fgss_selfjoin_noalloc refuses bad arguments and collections beyond its static
+    scratch before any device call (so these run without a GPU)."""
+    def err(**kw):
+        a = dict(tokens=0, lengths=0, offsets=0, n_sets=10, n_tokens=10, n_terms=10, threshold=0.5, out=0,
+                 out_cap=0)
+        a.update(kw)
+        with pytest.raises(fg.FgssError) as e:
+            fg.selfjoin_noalloc(**a)
+        return e.value.status
+    assert err() == fg.FGSS_ERR_ARG                      # null record arrays
+    assert err(tokens=8, lengths=8, offsets=8, threshold=1.0) == fg.FGSS_ERR_ARG
+    assert err(tokens=8, lengths=8, offsets=8, out_cap=5) == fg.FGSS_ERR_ARG   # null output
+    assert err(tokens=8, lengths=8, offsets=8, n_sets=fg.NOALLOC_MAX_SETS + 1) == fg.FGSS_ERR_UNSUPPORTED
+    assert err(tokens=8, lengths=8, offsets=8, n_tokens=fg.NOALLOC_MAX_TOKENS + 1) == fg.FGSS_ERR_UNSUPPORTED
+    assert err(tokens=8, lengths=8, offsets=8, n_terms=fg.NOALLOC_MAX_TERMS + 1) == fg.FGSS_ERR_UNSUPPORTED
+    assert err(tokens=8, lengths=8, offsets=8, table_slots=300) == fg.FGSS_ERR_ARG
+    assert fg.selfjoin_noalloc(0, 0, 0, 0, 0, 0, 0.5, 0, 0) == (0, fg.JoinStats().as_dict())
+    header = open(os.path.join(ROOT, "include", "fgssjoin.h")).read()
+    for name, v in (("SETS", fg.NOALLOC_MAX_SETS), ("TOKENS", fg.NOALLOC_MAX_TOKENS), ("TERMS", fg.NOALLOC_MAX_TERMS)):
+        assert re.search(rf"#define FGSS_NOALLOC_MAX_{name} {v}\b", header)
