@@ -16,7 +16,9 @@
  * "id[q] id[s] similarity" (src/fgssjoin/block.cu:193).
  *
  * Threading.  Handles are thread-compatible: one thread per handle at a time.
- * All device work runs on the current HIP device.
+ * All device work runs on the current HIP device, on a stream the handle owns
+ * or on the one fgss_join_opts.stream names; no call waits for the whole device
+ * (buffers a handle outgrows are freed with the handle, since hipFree would).
  */
 #ifndef FGSSJOIN_H
 #define FGSSJOIN_H
@@ -140,6 +142,16 @@ typedef struct {
                                   the boundary table                                   */
     int32_t bounds_cap;        /* precomputed split boundaries at most, 0 = default (2^28); the
                                   parts of queries beyond it locate their posting ranges by search */
+    int32_t reserved;
+    int64_t query_begin;       /* self-joins: only the queries q in [query_begin, query_end) (internal
+                                  indices), each against every s > q; 0, 0 = all (another empty range:
+                                  none).  One GPU's share of
+                                  a join whose queries are split over several (the reference's query
+                                  tiles, src/fgssjoin/block.cu:109, taken as a contiguous range) */
+    int64_t query_end;
+    void *stream;              /* hipStream_t carrying the call's device work (the reference's
+                                  per-thread default stream, include/util.cuh:4); null = the handle's
+                                  own stream.  The call still returns after that stream has drained. */
 } fgss_join_opts;
 
 typedef struct {

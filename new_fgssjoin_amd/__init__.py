@@ -77,7 +77,8 @@ class JoinStats(C.Structure):
 
 class JoinOpts(C.Structure):
     _fields_ = [("load_pct", C.c_int32), ("table_slots", C.c_int32), ("stream_parts", C.c_int32),
-                ("bounds_cap", C.c_int32)]
+                ("bounds_cap", C.c_int32), ("reserved", C.c_int32), ("query_begin", C.c_int64),
+                ("query_end", C.c_int64), ("stream", C.c_void_p)]
 
 
 class BatchStats(C.Structure):
@@ -314,9 +315,12 @@ class Collection:
         _check(lib().fgss_collection_upload(self._h))
 
     def selfjoin(self, threshold: float, table_slots: int = 0, load_pct: int = 0, bounds_cap: int = 0,
-                 stream_parts: int = 0) -> Pairs:
+                 stream_parts: int = 0, queries: tuple | None = None, stream: int = 0) -> Pairs:
+        """queries=(begin, end): only those queries (internal indices), each against every
+        later record; stream: a hipStream_t (as int) for the call's device work."""
         L = lib()
-        opts = JoinOpts(load_pct, table_slots, stream_parts, bounds_cap)
+        qb, qe = queries if queries is not None else (0, 0)
+        opts = JoinOpts(load_pct, table_slots, stream_parts, bounds_cap, 0, qb, qe, stream or None)
         out = C.POINTER(_Pairs)()
         _check(L.fgss_selfjoin(self._h, C.c_float(threshold), C.byref(opts), C.byref(out)))
         return _take_pairs(out)
@@ -373,9 +377,9 @@ class Batch:
         ts = (C.c_float * self.njobs)(*[float(t) for t in thresholds])
         _check(lib().fgss_batch_set_thresholds(self._h, ts))
 
-    def run(self, graph: bool = True, table_slots: int = 0, load_pct: int = 0) -> dict:
+    def run(self, graph: bool = True, table_slots: int = 0, load_pct: int = 0, stream: int = 0) -> dict:
         st = BatchStats()
-        opts = JoinOpts(load_pct, table_slots, 0, 0)
+        opts = JoinOpts(load_pct, table_slots, 0, 0, 0, 0, 0, stream or None)
         _check(lib().fgss_batch_run(self._h, C.byref(opts), 1 if graph else 0, C.byref(st)))
         return st.as_dict()
 

@@ -193,19 +193,24 @@ struct fgss_device_sets {
     // timing events and the counter copies enqueued between and after them
     hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
     hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};
-    std::array<int64_t, 12> graph_key{};   // what the captured stages were built for (graph_key_of)
+    std::array<int64_t, 14> graph_key{};   // what the captured stages were built for (graph_key_of)
     int cap_gen = 0;              // bumped whenever a buffer the pipeline uses is reallocated
     bool sized = false;           // a run has completed (first-run buffer sizing is over)
+    int32_t q_begin = 0, q_end = 0;   // the run's query range (internal indices; 0, 0: every record)
+    // Buffers replaced by larger ones (device, pinned host): freed when the record set is
+    // released.  hipFree and hipHostFree wait for the whole device, which would stall the
+    // runs of other record sets (a server's other slots, another thread's collection).
+    std::vector<void *> dead_dev, dead_host;
 };
 
-static int ensure(DevBuf &b, size_t bytes)
+static int ensure(fgss_device_sets *d, DevBuf &b, size_t bytes)
 {
     if (bytes == 0) bytes = 16;
     if (b.cap >= bytes) return FGSS_OK;
-    // a buffer that grows again grows by half at least: hipFree waits for the whole
-    // device, so a workspace that creeps upward (a server's slots) must reallocate rarely
+    // a buffer that grows again grows by half at least, so that a workspace that creeps
+    // upward (a server's slots) reallocates rarely and its retired buffers stay few
     size_t want = std::max(bytes + bytes / 8, b.cap + b.cap / 2);
-    if (b.p) HIPCK(hipFree(b.p));
+    if (b.p) d->dead_dev.push_back(b.p);   // work already enqueued on the stream may still read it
     b.p = nullptr;
     b.cap = 0;
     hipError_t e = hipMalloc(&b.p, want);
@@ -214,14 +219,15 @@ static int ensure(DevBuf &b, size_t bytes)
     return FGSS_OK;
 }
 
+// A pinned host buffer replaced by a larger one: freed with the record set.
+static void retire_host(fgss_device_sets *d, void *p)
+{
+    if (p) d->dead_host.push_back(p);
+}
+
 template <class T>
 static T *P(DevBuf &b) { return (T *)b.p; }
 
-#define ENSURE(buf, bytes)                \
-    do {                                  \
-        int rc_ = ensure((buf), (bytes)); \
-        if (rc_) return rc_;              \
-    } while (0)
 
 void fgss_device_release(fgss_collection *c)
 {
@@ -246,6 +252,8 @@ void fgss_device_release(fgss_collection *c)
     void *arrs[] = {d->tok, d->len, d->pos, d->sig, d->sig64, d->sigbad, d->job, d->jend, d->tj};
     for (void *a : arrs)
         if (a) (void)hipFree(a);
+    for (void *a : d->dead_dev) (void)hipFree(a);
+    for (void *a : d->dead_host) (void)hipHostFree(a);
     for (auto &e : d->ev)
         if (e) (void)hipEventDestroy(e);
     if (d->h_ctr) (void)hipHostFree(d->h_ctr);
@@ -412,9 +420,11 @@ extern "C" int fgss_collection_upload(fgss_collection *c)
 // clamped to the record length so that an empty record contributes nothing
 // With rs set the indexed prefix is the probe prefix max_prefix (an R-S index built
 // at the smallest threshold it will serve) instead of the self-join mid-prefix.
+// Records outside the query range [qb, qe) (unless qb = qe = 0: every record) probe with
+// no tokens: they stay sources of the queries inside it, which are joined as usual.
 __global__ void k_midlen(const uint32_t *__restrict__ len, const uint32_t *__restrict__ job,
                          const float *__restrict__ tj, uint32_t *__restrict__ midlen,
-                         uint32_t *__restrict__ probelen, int64_t n, int rs)
+                         uint32_t *__restrict__ probelen, int64_t n, int rs, int64_t qb, int64_t qe)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
@@ -423,7 +433,7 @@ __global__ void k_midlen(const uint32_t *__restrict__ len, const uint32_t *__res
         uint32_t mp = rs ? fgss_max_prefix(l, t) : fgss_mid_prefix(l, t);
         uint32_t qp = fgss_max_prefix(l, t);
         midlen[i] = mp < l ? mp : l;
-        probelen[i] = qp < l ? qp : l;
+        probelen[i] = (qb != 0 || qe != 0) && (i < qb || i >= qe) ? 0u : (qp < l ? qp : l);
     }
 }
 
@@ -1731,7 +1741,7 @@ static RunCfg make_cfg(const fgss_join_opts *opts)
 static int grow(fgss_device_sets *d, DevBuf &b, size_t bytes)
 {
     const void *before = b.p;
-    int rc = ensure(b, bytes);
+    int rc = ensure(d, b, bytes);
     if (rc) return rc;
     if (b.p != before) d->cap_gen++;
     return FGSS_OK;
@@ -1836,7 +1846,8 @@ static int enqueue_index(fgss_device_sets *d, int rs)
              *ib = P<uint32_t>(d->b_ib), *sid_of = P<uint32_t>(d->b_sid_of), *loff = P<uint32_t>(d->b_loff);
     uint32_t *psid = P<uint32_t>(d->b_psid), *fence = P<uint32_t>(d->b_fence);
     uint2 *post = P<uint2>(d->b_post);
-    hipLaunchKernelGGL(k_midlen, dim3(grid_for(n, 256)), dim3(256), 0, st, d->len, d->job, d->tj, midlen, probelen, n, rs);
+    hipLaunchKernelGGL(k_midlen, dim3(grid_for(n, 256)), dim3(256), 0, st, d->len, d->job, d->tj, midlen, probelen, n, rs,
+                       (int64_t)d->q_begin, (int64_t)d->q_end);
     HIPCK(hipGetLastError());
     HIPCK(exclusive_scan<uint32_t>(midlen, midoff, n, misc + M_NE, scr, st));
     HIPCK(exclusive_scan<uint32_t>(probelen, poff, n, misc + M_NP, scr, st));
@@ -1936,7 +1947,7 @@ static int enqueue_out_copy(fgss_device_sets *d)
     if (want == 0) return FGSS_OK;
     if (d->h_out_cap < want) {   // pinned allocations are slow: grow by doubling at least
         const size_t cap = std::max<size_t>({(size_t)want, 2 * d->h_out_cap, 16384});
-        if (d->h_out) HIPCK(hipHostFree(d->h_out));
+        retire_host(d, d->h_out);
         d->h_out = nullptr;
         d->h_out_cap = 0;
         HIPCK(hipHostMalloc((void **)&d->h_out, cap * 12, hipHostMallocDefault));
@@ -2000,10 +2011,41 @@ struct RunTotals {
 // (a reallocation of any buffer bumps cap_gen): the configuration, the capacities
 // the kernels test against and the record counts.  A replay is valid while it is
 // unchanged; a capacity can grow inside a buffer's slack without a reallocation.
-static std::array<int64_t, 12> graph_key_of(const fgss_device_sets *d, const RunCfg &cfg)
+static std::array<int64_t, 14> graph_key_of(const fgss_device_sets *d, const RunCfg &cfg)
 {
     return {cfg.cap, cfg.target, cfg.heavy_target, cfg.bounds, cfg.stream_parts, d->cap_gen,
-            d->cap_items, d->cap_out, d->cap_ovf, d->cap_bounds, d->n, d->njobs};
+            d->cap_items, d->cap_out, d->cap_ovf, d->cap_bounds, d->n, d->njobs, d->q_begin, d->q_end};
+}
+
+// For the length of one call: the record set's work on the caller's stream (opts->stream)
+// and, for a self-join, its query range; both restored when the call returns.
+struct CallScope {
+    fgss_device_sets *d;
+    hipStream_t own;
+    int32_t qb, qe;
+    CallScope(fgss_device_sets *dd, const fgss_join_opts *opts, bool ranged) : d(dd), own(dd->st), qb(dd->q_begin), qe(dd->q_end)
+    {
+        if (opts && opts->stream) d->st = (hipStream_t)opts->stream;
+        d->q_begin = ranged && opts ? (int32_t)opts->query_begin : 0;
+        d->q_end = ranged && opts ? (int32_t)opts->query_end : 0;
+    }
+    ~CallScope()
+    {
+        d->st = own;
+        d->q_begin = qb;
+        d->q_end = qe;
+    }
+};
+
+// A query range in opts must lie inside the n records (0, 0 = all).
+static int check_range(const fgss_join_opts *opts, int64_t n, bool allowed)
+{
+    if (!opts || (opts->query_begin == 0 && opts->query_end == 0)) return FGSS_OK;
+    if (!allowed) return fgss_fail(FGSS_ERR_ARG, "a query range applies to plain self-joins only");
+    if (opts->query_begin < 0 || opts->query_end < opts->query_begin || opts->query_end > n)
+        return fgss_fail(FGSS_ERR_ARG, "query range [%lld, %lld) outside the %lld records",
+                         (long long)opts->query_begin, (long long)opts->query_end, (long long)n);
+    return FGSS_OK;
 }
 
 static int capture_graph(fgss_device_sets *d, const RunCfg &cfg)
@@ -2128,7 +2170,10 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
     uint64_t nres = d->h_ctr[C_RESULTS], novf = d->h_ctr[C_OVERFLOW];
     tri.resize(3 * (size_t)nres);
     if (nres && nres <= d->out_copied) memcpy(tri.data(), d->h_out, (size_t)nres * 12);
-    else if (nres) HIPCK(hipMemcpy(tri.data(), d->b_out.p, (size_t)nres * 12, hipMemcpyDeviceToHost));
+    else if (nres) {
+        HIPCK(hipMemcpyAsync(tri.data(), d->b_out.p, (size_t)nres * 12, hipMemcpyDeviceToHost, st));
+        HIPCK(hipStreamSynchronize(st));
+    }
     d->out_hint = std::max<uint32_t>(d->out_hint, (uint32_t)std::min<uint64_t>(nres + nres / 4 + 1024, UINT32_MAX / 16));
 
     // overflowed work items: split each four ways and run them again
@@ -2137,7 +2182,7 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
     while (novf > 0) {
         if (++round > 48) return fgss_fail(FGSS_ERR_DEVICE, "internal: work-item splitting did not converge");
         const uint32_t ways = 4, cnt = (uint32_t)(novf * ways);
-        int rc = ensure(d->b_items2, (size_t)cnt * sizeof(Item));
+        int rc = ensure(d, d->b_items2, (size_t)cnt * sizeof(Item));
         if (rc) return rc;
         hipLaunchKernelGGL(k_resplit, dim3(grid_for((int64_t)novf, 256)), dim3(256), 0, st, cur, P<uint32_t>(d->b_ovf),
                            P<uint32_t>(d->b_ovfp),
@@ -2146,14 +2191,14 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
         std::swap(d->b_items, d->b_items2);   // the parent list stays alive in b_items2
         d->cap_gen++;
         cur = P<Item>(d->b_items);
-        rc = ensure(d->b_ovf, (size_t)cnt * 4);
+        rc = ensure(d, d->b_ovf, (size_t)cnt * 4);
         if (rc) return rc;
-        rc = ensure(d->b_ovfp, (size_t)cnt * 4);
+        rc = ensure(d, d->b_ovfp, (size_t)cnt * 4);
         if (rc) return rc;
         // the round's launch adds to the per-job counters as its items complete: kept
         // before it, so that a launch repeated for a larger result buffer counts once
         const size_t jbytes = (size_t)d->njobs * kJobCtr * sizeof(uint64_t);
-        rc = ensure(d->b_jsave, jbytes);
+        rc = ensure(d, d->b_jsave, jbytes);
         if (rc) return rc;
         HIPCK(hipMemcpyAsync(d->b_jsave.p, d->b_jctr.p, jbytes, hipMemcpyDeviceToDevice, st));
         for (int tries = 0;; tries++) {
@@ -2184,7 +2229,7 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
             T.ms_kernel += ms;
             if (d->h_ctr[C_RESULTS] <= d->cap_out) break;
             d->cap_out = (uint32_t)std::min<uint64_t>(d->h_ctr[C_RESULTS] * 2 + 1024, UINT32_MAX / 16);
-            rc = ensure(d->b_out, (size_t)d->cap_out * 12);
+            rc = ensure(d, d->b_out, (size_t)d->cap_out * 12);
             if (rc) return rc;
             d->cap_gen++;
         }
@@ -2193,12 +2238,18 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
         novf = d->h_ctr[C_OVERFLOW];
         const size_t old = tri.size();
         tri.resize(old + 3 * (size_t)nres);
-        if (nres) HIPCK(hipMemcpy(tri.data() + old, d->b_out.p, (size_t)nres * 12, hipMemcpyDeviceToHost));
+        if (nres) {
+            HIPCK(hipMemcpyAsync(tri.data() + old, d->b_out.p, (size_t)nres * 12, hipMemcpyDeviceToHost, st));
+            HIPCK(hipStreamSynchronize(st));
+        }
     }
     // per-job counters: the run's tail copied them unless rounds ran after it (the
     // items that overflowed had not flushed theirs)
-    if (round > 0 || !d->jctr_async)
-        HIPCK(hipMemcpy(d->h_jctr, d->b_jctr.p, (size_t)d->njobs * kJobCtr * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (round > 0 || !d->jctr_async) {
+        HIPCK(hipMemcpyAsync(d->h_jctr, d->b_jctr.p, (size_t)d->njobs * kJobCtr * sizeof(uint64_t),
+                             hipMemcpyDeviceToHost, st));
+        HIPCK(hipStreamSynchronize(st));
+    }
     for (size_t k = 0; k < T.jctr.size(); k++) T.jctr[k] = d->h_jctr[k];
     trace_mark("collect");
     return FGSS_OK;
@@ -2380,10 +2431,13 @@ extern "C" int fgss_selfjoin(fgss_collection *c, float threshold, const fgss_joi
         return fgss_fail(FGSS_ERR_ARG, "Threshold must be a float in the range (0.0..1.0).");
     if (!c->jend.empty()) return fgss_fail(FGSS_ERR_ARG, "a batch collection is run with fgss_batch_run");
     const RunCfg cfg = make_cfg(opts);
-    int rc = fgss_collection_upload(c);
+    int rc = check_range(opts, (int64_t)c->len.size(), true);
+    if (rc) return rc;
+    rc = fgss_collection_upload(c);
     if (rc) return rc;
     auto t1 = std::chrono::steady_clock::now();
     fgss_device_sets *d = c->dev;
+    CallScope scope(d, opts, true);
     // the one threshold travels as a kernel argument: no host copy, no synchronization
     HIPCK(fill<float>(d->tj, 1, threshold, d->st));
     std::vector<int32_t> tri;
@@ -2463,8 +2517,9 @@ extern "C" int fgss_collection_print_index(fgss_collection *c, float threshold, 
     const uint32_t E = d->n > 0 ? d->h_misc[M_NE] : 0u, T = d->n_terms;
     std::vector<uint32_t> loff((size_t)T + 1, 0), psid((size_t)E);
     if (d->n > 0) {
-        HIPCK(hipMemcpy(loff.data(), d->b_loff.p, ((size_t)T + 1) * 4, hipMemcpyDeviceToHost));
-        if (E) HIPCK(hipMemcpy(psid.data(), d->b_psid.p, (size_t)E * 4, hipMemcpyDeviceToHost));
+        HIPCK(hipMemcpyAsync(loff.data(), d->b_loff.p, ((size_t)T + 1) * 4, hipMemcpyDeviceToHost, d->st));
+        if (E) HIPCK(hipMemcpyAsync(psid.data(), d->b_psid.p, (size_t)E * 4, hipMemcpyDeviceToHost, d->st));
+        HIPCK(hipStreamSynchronize(d->st));
     }
     FILE *f = (FILE *)file;
     for (uint32_t t = 0; t < T; t++) {
@@ -2489,9 +2544,31 @@ struct fgss_batch {
     std::vector<int32_t> tri;         // last run: triples sorted by (q, s), merged indices
     std::vector<uint64_t> order;
     std::vector<size_t> job_first;    // last run: first sorted triple of each job
+    // per job: its distinct record lengths and how many records have each, for the
+    // job's own index-entry and probe-token counts (a run counts them for the batch)
+    std::vector<std::vector<std::pair<uint32_t, int64_t>>> len_hist;
     RunTotals totals;
     bool has_run = false;
 };
+
+// Job j's "# Inverted Index Entries" and probe tokens at its threshold: the sums
+// k_midlen's per-record prefix lengths make over the job's records.
+static void batch_job_prefix_counts(const fgss_batch *b, size_t j, int64_t &entries, int64_t &probes)
+{
+    entries = probes = 0;
+    const float t = b->t[j];
+    for (const auto &lc : b->len_hist[j]) {
+        const uint32_t l = lc.first;
+        entries += (int64_t)std::min(fgss_mid_prefix(l, t), l) * lc.second;
+        probes += (int64_t)std::min(fgss_max_prefix(l, t), l) * lc.second;
+    }
+}
+
+static void batch_job_stats(const fgss_batch *b, size_t j, fgss_join_stats &st)
+{
+    fill_stats(b->totals, j, st);
+    batch_job_prefix_counts(b, j, st.index_entries, st.probe_tokens);
+}
 
 extern "C" int fgss_batch_create(fgss_collection *const *colls, const float *thresholds, int32_t njobs,
                                  fgss_batch **out)
@@ -2537,6 +2614,11 @@ extern "C" int fgss_batch_create(fgss_collection *const *colls, const float *thr
         for (int32_t v : c->tok) m.tok.push_back((int32_t)(v + term_off));   // private term range per job
         m.job.insert(m.job.end(), c->len.size(), (uint32_t)j);
         m.jend.push_back((uint32_t)m.len.size());
+        std::vector<int64_t> cnt((size_t)c->max_len + 1, 0);
+        for (const int32_t l : c->len) cnt[(size_t)l]++;
+        b->len_hist.emplace_back();
+        for (size_t l = 0; l < cnt.size(); l++)
+            if (cnt[l]) b->len_hist.back().emplace_back((uint32_t)l, cnt[l]);
         term_off += c->n_terms;
     }
     b->rec_begin.push_back((int64_t)m.len.size());
@@ -2572,7 +2654,10 @@ extern "C" int fgss_batch_run(fgss_batch *b, const fgss_join_opts *opts, int32_t
     auto t0 = std::chrono::steady_clock::now();
     fgss_device_sets *d = b->merged.dev;
     const RunCfg cfg = make_cfg(opts);
-    int rc = set_thresholds(d, b->t.data(), (uint32_t)b->t.size());
+    int rc = check_range(opts, d->n, false);
+    if (rc) return rc;
+    CallScope scope(d, opts, false);
+    rc = set_thresholds(d, b->t.data(), (uint32_t)b->t.size());
     if (rc) return rc;
     rc = run_pipeline(d, cfg, use_graph != 0, b->tri, b->totals);
     if (rc) return rc;
@@ -2608,7 +2693,7 @@ extern "C" int fgss_batch_pairs(fgss_batch *b, int32_t job, fgss_pairs **out)
     const size_t first = b->job_first[(size_t)job], last = b->job_first[(size_t)job + 1];
     int rc = make_pairs(b->colls[(size_t)job], b->tri, b->order, first, last - first, b->rec_begin[(size_t)job], out);
     if (rc) return rc;
-    fill_stats(b->totals, (size_t)job, (*out)->stats);
+    batch_job_stats(b, (size_t)job, (*out)->stats);
     return FGSS_OK;
 }
 
@@ -2649,7 +2734,7 @@ extern "C" int fgss_batch_pairs_all(fgss_batch *b, fgss_pairs **out, int64_t *jo
             }
             if (job_stats) {
                 job_stats[j] = fgss_join_stats{};
-                fill_stats(b->totals, j, job_stats[j]);
+                batch_job_stats(b, j, job_stats[j]);
             }
         }
     };
@@ -2995,7 +3080,7 @@ struct StageLayout {
 static int stage_reserve(ServeSlot &sl, size_t words)
 {
     if (sl.h_in_cap >= words) return FGSS_OK;
-    if (sl.h_in) HIPCK(hipHostFree(sl.h_in));
+    retire_host(sl.d, sl.h_in);
     sl.h_in = nullptr;
     const size_t cap = std::max<size_t>({words, 2 * sl.h_in_cap, (size_t)1 << 16});
     sl.h_in_cap = 0;
@@ -3021,7 +3106,7 @@ static int slot_load(ServeSlot &sl, int64_t n, int64_t ntok, uint32_t nj, int32_
     d->n = n; d->ntok = ntok; d->njobs = nj; d->max_len = max_len; d->n_terms = n_terms;
     // pinned per-job counter mirror, grown to this batch's jobs
     if (sl.jctr_cap < (size_t)std::max<uint32_t>(nj, 1)) {
-        if (d->h_jctr) HIPCK(hipHostFree(d->h_jctr));
+        retire_host(d, d->h_jctr);
         d->h_jctr = nullptr;
         sl.jctr_cap = std::max<size_t>({(size_t)nj, 64, sl.jctr_cap * 2});
         HIPCK(hipHostMalloc((void **)&d->h_jctr, sl.jctr_cap * kJobCtr * 8, hipHostMallocDefault));
@@ -3055,13 +3140,13 @@ static int slot_reserve(ServeSlot &sl, int64_t n, int64_t ntok, uint32_t jobs, i
     GROW(sl.b_sig64, (size_t)n * 4 * kSig64Words);
     GROW(sl.b_sigbad, (size_t)n);
     if (sl.jctr_cap < jobs) {
-        if (d->h_jctr) HIPCK(hipHostFree(d->h_jctr));
+        retire_host(d, d->h_jctr);
         d->h_jctr = nullptr;
         sl.jctr_cap = jobs;
         HIPCK(hipHostMalloc((void **)&d->h_jctr, sl.jctr_cap * kJobCtr * 8, hipHostMallocDefault));
     }
     if (d->h_out_cap < (1u << 18)) {   // pinned mirror of a run's first 2^18 result triples (3 MB)
-        if (d->h_out) HIPCK(hipHostFree(d->h_out));
+        retire_host(d, d->h_out);
         d->h_out = nullptr;
         d->h_out_cap = 0;
         HIPCK(hipHostMalloc((void **)&d->h_out, (size_t)(1u << 18) * 12, hipHostMallocDefault));
@@ -3247,6 +3332,8 @@ extern "C" int fgss_server_create(const fgss_collection *indexed, float t_min, i
         opts->table_slots != 1024 && opts->table_slots != 2048 && opts->table_slots != 4096)
         return fgss_fail(FGSS_ERR_ARG, "table_slots must be 256, 512, 1024, 2048 or 4096");
     if (!indexed->jend.empty()) return fgss_fail(FGSS_ERR_ARG, "the indexed collection must be a plain collection");
+    int rc0 = check_range(opts, (int64_t)indexed->len.size(), false);
+    if (rc0) return rc0;
     auto *srv = new fgss_server();
     srv->coll = indexed;
     srv->t_min = t_min;
@@ -3256,17 +3343,22 @@ extern "C" int fgss_server_create(const fgss_collection *indexed, float t_min, i
     if (rc) { delete srv; return rc; }
     fgss_device_sets *x = srv->idx;
     srv->empty = x->n == 0 || x->ntok == 0;
-    // the R-S index: the first max_prefix(|s|, t_min) tokens of every record
-    rc = set_thresholds(x, &t_min, 1);
-    if (!rc) rc = prepare_buffers(x);
-    if (!rc && srv->empty) x->h_misc[M_NE] = 0;
-    if (!rc && !srv->empty) {
-        uint32_t *misc = P<uint32_t>(x->b_misc);
-        if (fill<uint32_t>(misc, M_COUNT, 0u, x->st) != hipSuccess) rc = fgss_fail(FGSS_ERR_DEVICE, "fill failed");
-        if (!rc) rc = enqueue_index(x, 1);
-        if (!rc && hipMemcpyAsync(x->h_misc, misc, M_COUNT * 4, hipMemcpyDeviceToHost, x->st) != hipSuccess)
-            rc = fgss_fail(FGSS_ERR_DEVICE, "copy of the index counters failed");
-        if (!rc && hipStreamSynchronize(x->st) != hipSuccess) rc = fgss_fail(FGSS_ERR_DEVICE, "index build failed");
+    // the R-S index: the first max_prefix(|s|, t_min) tokens of every record (built on
+    // the caller's stream when opts names one; the slots' runs use streams of their own,
+    // so that they overlap)
+    {
+        CallScope scope(x, opts, false);   // restored before x can be released below
+        rc = set_thresholds(x, &t_min, 1);
+        if (!rc) rc = prepare_buffers(x);
+        if (!rc && srv->empty) x->h_misc[M_NE] = 0;
+        if (!rc && !srv->empty) {
+            uint32_t *misc = P<uint32_t>(x->b_misc);
+            if (fill<uint32_t>(misc, M_COUNT, 0u, x->st) != hipSuccess) rc = fgss_fail(FGSS_ERR_DEVICE, "fill failed");
+            if (!rc) rc = enqueue_index(x, 1);
+            if (!rc && hipMemcpyAsync(x->h_misc, misc, M_COUNT * 4, hipMemcpyDeviceToHost, x->st) != hipSuccess)
+                rc = fgss_fail(FGSS_ERR_DEVICE, "copy of the index counters failed");
+            if (!rc && hipStreamSynchronize(x->st) != hipSuccess) rc = fgss_fail(FGSS_ERR_DEVICE, "index build failed");
+        }
     }
     if (rc) { fgss_server_free(srv); return rc; }
     srv->index_entries = x->h_misc[M_NE];
@@ -3687,6 +3779,12 @@ extern "C" int fgss_selfjoin_noalloc(const int32_t *tokens, const int32_t *lengt
     if (noalloc_carve(&d, nullptr, n_sets, n_tokens, (uint32_t)n_terms, cus, &nb) > kNoAllocBytes)
         return fgss_fail(FGSS_ERR_UNSUPPORTED, "workspace exceeds the %zu-byte no-allocation scratch", kNoAllocBytes);
     noalloc_carve(&d, base, n_sets, n_tokens, (uint32_t)n_terms, cus, &nb);
+    rc = check_range(opts, n_sets, true);
+    if (rc) return rc;
+    if (opts) {
+        d.q_begin = (int32_t)opts->query_begin;
+        d.q_end = (int32_t)opts->query_end;
+    }
     d.device = dev;
     d.st = st;
     d.n = n_sets;

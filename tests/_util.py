@@ -55,3 +55,19 @@ def records_to_bin(records, ids=None) -> np.ndarray:
         r = np.asarray(records[k], np.int32)
         out += [int(ids[k]) if ids is not None else k + 1, int(r.size)] + r.tolist()
     return np.array(out, np.int32)
+
+
+def hip_runtime():
+    """The HIP runtime that libfgssjoin.so links (already mapped in this process), for
+    tests that create their own streams or device buffers."""
+    import ctypes as C
+
+    import new_fgssjoin_amd as fg
+    fg.lib()
+    path = next((ln.split()[-1] for ln in open("/proc/self/maps") if "libamdhip64.so" in ln), None)
+    assert path, "libamdhip64 not mapped"
+    h = C.CDLL(path)
+    h.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+    h.hipStreamDestroy.argtypes = [C.c_void_p]
+    h.hipStreamSynchronize.argtypes = [C.c_void_p]
+    return h
