@@ -1,14 +1,16 @@
-"""Benchmark: plain self-join of 100k synthetic titles (q=3) at t = 0.6, 0.75, 0.9.
+"""Benchmark: plain self-join of 1M synthetic long records (q=3, t=0.8) on one MI355X.
 
-One step = the three self-joins in sequence, each from the HBM-resident
-collection to the sorted result pairs in host memory (index build, candidate
-generation, verification, compaction, device->host copy).  Data: seeded
-synthetic titles (4-15 Zipf words over a 50k-word vocabulary, 20-120 chars,
-10% near-duplicates), tokenized into 3-gram sets before timing.
+One step = one self-join of the HBM-resident collection, from the records on the
+device to the sorted result pairs in host memory (index build, planning, candidate
+generation, verification, compaction, device->host copy).  Data: 1M seeded synthetic
+records of 200-1200 chars (~60 Zipf words, 5% near-duplicates with 1-10 edits),
+tokenized into 3-gram sets on the GPU before timing.
 
 Prints one JSON line: candidates verified per second (the reference's
-"# Candidates", summed over the step) with the join time, the roofline of the
-join kernel and the CPU oracle baseline on this host.
+"# Candidates" / join time) with the join time, the roofline of the join kernel and
+the CPU oracle baseline on this host.  Secondary keys: 100k titles at
+t = 0.6 / 0.75 / 0.9 ("titles"), 512 batched joins ("batched"), R-S serving against
+a 2M-title index ("served").
 """
 from __future__ import annotations
 
@@ -24,10 +26,11 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-THRESHOLDS = (0.6, 0.75, 0.9)
-N_RECORDS = 100_000
-SEED = 2
 HBM_PEAK_GBS = 8000.0
+LONG_RECORDS, LONG_SEED, LONG_T = 1_000_000, 3, 0.8
+TITLES_RECORDS, TITLES_SEED, TITLES_T = 100_000, 2, (0.6, 0.75, 0.9)
+PMC = {"long": os.path.join(ROOT, "profiles", "k_join_long1m_pmc.txt"),
+       "titles": os.path.join(ROOT, "profiles", "k_join_titles100k_pmc.txt")}
 
 
 @contextlib.contextmanager
@@ -47,35 +50,22 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_join_titles100k_pmc.txt")
-
-
-def alg_bytes_join_kernel(c):
-    """Algorithmic bytes of one join's candidate generation + verification (the
-    k_join launch), counted by the oracle from the reference algorithm: 4 B per probe
-    token, 8 B per useful posting (sid, pos), 4 B per token read of the reference's
-    merge loop (checking.cu:126-160), 8 B per result pair."""
-    return 4 * c["p_q"] + 8 * c["e_scan"] + 4 * c["m_reads"] + 8 * c["results"]
-
-
-def alg_bytes_join(c):
-    """... plus the index build: 12 B per index entry (read a token, write sid, pos)."""
-    return 12 * c["e_idx"] + alg_bytes_join_kernel(c)
-
-
 def design_bytes(st):
-    """Bytes this build's k_join reads for one join (no oracle needed): 4 B per probe
-    token, 8 B per useful posting, 64 B per 64-bucket signature test, 128 B per
-    128-bucket test, 4 B per verified source token, 12 B per result."""
+    """Bytes one join's k_join launch must read and write as designed: 4 B per probe
+    token, 8 B per useful posting (record id, position | length), one 64-byte signature
+    line per candidate tested and a 128-byte one per candidate tested again, 4 B per
+    verified source token, 12 B per result triple."""
     return (4 * st["probe_tokens"] + 8 * st["postings_scanned"] + 64 * st["signature_checks"]
             + 128 * st["signature_checks_128"] + 4 * st["verify_tokens"] + 12 * st["results"])
 
 
-def pmc_summary():
-    """key -> value from the committed rocprofv3 PMC summary of this workload"""
+def pmc_summary(workload):
+    """key -> value pairs of the committed rocprofv3 PMC summary of this workload
+    (its last lines: `key value`)."""
     out = {}
-    if os.path.exists(PMC_SUMMARY):
-        for ln in open(PMC_SUMMARY):
+    path = PMC[workload]
+    if os.path.exists(path):
+        for ln in open(path):
             parts = ln.split()
             if len(parts) == 2:
                 try:
@@ -85,29 +75,216 @@ def pmc_summary():
     return out
 
 
-def cpu_baseline(coll, gpu_pairs):
-    """The CPU oracle on this host over the same step (all three thresholds)."""
+def roofline(workload, st_list, kernel_ms, extra=None):
+    """Roofline of k_join over launches with stats st_list and mean duration kernel_ms
+    (HIP events around the launch on its stream)."""
+    kbytes = sum(design_bytes(st) for st in st_list) / len(st_list)
+    achieved = kbytes / (kernel_ms / 1e3) / 1e9 if kernel_ms > 0 else 0.0
+    prof = pmc_summary(workload)
+    fetch, write = prof.get("fetch_bytes_per_launch"), prof.get("write_bytes_per_launch")
+    traffic = fetch + write if fetch is not None and write is not None else None
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_join",
+           "bytes_per_launch": kbytes, "kernel_ms_per_launch": kernel_ms,
+           "basis": "bytes this kernel must move as designed (probe tokens, useful postings, signature lines, "
+                    "verified tokens, results) / HIP-event launch time",
+           "traffic_basis": "unscaled FETCH_SIZE + WRITE_SIZE per k_join launch from the same-build PMC summary",
+           "traffic_frac": (traffic / (kernel_ms / 1e3) / 1e9 / HBM_PEAK_GBS) if traffic and kernel_ms > 0 else None,
+           "traffic_over_design": (traffic / kbytes) if traffic and kbytes else None,
+           "traffic_source": os.path.relpath(PMC[workload], ROOT) if prof else None,
+           "valu_busy": prof.get("valu_busy"), "lds_bank_conflict_ratio": prof.get("lds_bank_conflict_ratio")}
+    if extra:
+        out.update(extra)
+    return out
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "affinity": avail, "model": model}
+
+
+def cpu_baseline_long(coll, pairs, t, sample_all: int, sample_one: int, seed: int = 11):
+    """The oracle (each candidate verified by the reference's merge, once) on a seeded
+    sample of the join's queries, each against the whole collection: all the host
+    threads this process may use (at most 16, the host share of one GPU) on
+    `sample_all` queries and one thread on the first `sample_one` of them, each 1
+    warm-up + median of 5.  The GPU's pairs of the sampled queries must equal the
+    oracle's bit for bit."""
     import numpy as np
 
     import oracle
     oracle.build()
     ids, lens, pos, tok = coll.arrays()
     sets = oracle.Sets.from_arrays(ids, lens, pos, tok, coll.info.n_terms)
-    threads = max(1, min(16, os.cpu_count() or 1))
-    cand = 0
-    exact = True
-    counters = []
+    qs = np.sort(np.random.default_rng(seed).choice(sets.n, size=min(sample_all, sets.n), replace=False))
+    cpu = host_cpu()
+    threads = max(1, min(16, cpu["affinity"]))
+    res = {}
+    for name, nth, q in (("all", threads, qs), ("one", 1, np.sort(qs[:sample_one]))):
+        runs = []
+        for r in range(6):
+            tm = []
+            tri, ctr = oracle.selfjoin(sets, t, nthreads=nth, queries=q, checks=False, times=tm)
+            if r > 0:
+                runs.append(tm)
+        runs.sort(key=lambda x: x[1])
+        med = runs[len(runs) // 2]
+        res[name] = {"threads": nth, "queries": int(q.size), "index_s": med[0], "query_s": med[1],
+                     "candidates": ctr["candidates"], "candidates_per_s": ctr["candidates"] / med[1], "ctr": ctr,
+                     "tri": tri}
+        log(f"cpu baseline: {nth} thread(s), {q.size} queries: {med[1]:.2f} s (+ index {med[0]:.2f} s), "
+            f"{ctr['candidates'] / med[1]:.3g} candidates/s")
+    got = np.stack([pairs.q, pairs.s, pairs.overlap], axis=1) if len(pairs) else np.zeros((0, 3), np.int32)
+    keep = np.isin(got[:, 0], qs) if len(got) else np.zeros(0, bool)
+    exact = bool(np.array_equal(got[keep], res["all"]["tri"]))
+    a, o = res["all"], res["one"]
+    n = sets.n
+    whole_s = a["index_s"] + a["query_s"] * n / a["queries"]
+    out = {"value": a["candidates_per_s"], "unit": "candidates/s", "cores": a["threads"], "kind": "port",
+           "sample": f"{a['queries']} random queries (seed {seed}) of the {n}-record t={t} join, each against the "
+                     f"whole collection; oracle/fgss_oracle.c closed form, each candidate verified by the reference's "
+                     f"merge once; query phase, median of 5 after 1 warm-up",
+           "one_core": {"value": o["candidates_per_s"], "queries": o["queries"], "query_s": o["query_s"]},
+           "all_cores": {"threads": a["threads"], "queries": a["queries"], "query_s": a["query_s"],
+                         "index_build_s": a["index_s"]},
+           "host": cpu, "whole_join_estimate_s": whole_s}
+    return out, exact, a["ctr"]
+
+
+def long_leg(fg, args):
+    """The headline: n long records (seed 3), q=3, t=0.8, tokenized on the GPU."""
+    import numpy as np
+    t = float(np.float32(LONG_T))
     t0 = time.perf_counter()
-    for t, p in zip(THRESHOLDS, gpu_pairs):
-        tri, ctr = oracle.selfjoin(sets, t, nthreads=threads)
-        counters.append(ctr)
-        cand += ctr["candidates"]
-        got = np.stack([p.q, p.s, p.overlap], axis=1) if len(p) else np.zeros((0, 3), np.int32)
-        exact &= got.shape == tri.shape and bool(np.array_equal(got, tri)) and p.stats["candidates"] == ctr["candidates"]
-    dt = time.perf_counter() - t0
-    return {"value": cand / dt, "unit": "candidates/s", "cores": threads, "kind": "port",
-            "sample": f"full step: {N_RECORDS} titles x t={list(THRESHOLDS)}, {dt:.1f} s, "
-                      f"oracle/fgss_oracle.c closed form, {threads} threads"}, exact, counters
+    text = fg.generate_text(fg.GEN_LONG, args.long_records, LONG_SEED)
+    t1 = time.perf_counter()
+    coll = fg.Collection.from_text_bytes(text, 3, device=True)
+    tok_s = time.perf_counter() - t1
+    del text
+    info = coll.info
+    log(f"long: {info.n_sets} records, {info.n_tokens} tokens, {info.n_terms} terms (GPU tokenizer {tok_s:.2f} s, "
+        f"set-up {time.perf_counter() - t0:.1f} s)")
+    coll.upload()
+    coll.selfjoin(t)   # sizes the workspace
+    for i in range(args.warmup):
+        coll.selfjoin(t)
+        log(f"warmup {i + 1}/{args.warmup} done")
+    fg.device_synchronize()
+    runs = []
+    with gc_paused():
+        a = time.perf_counter()
+        for i in range(args.steps):
+            runs.append(coll.selfjoin(t))
+        fg.device_synchronize()
+        dt = time.perf_counter() - a
+    log(f"timed {args.steps} steps in {dt:.3f} s")
+    p = runs[-1]
+    st = p.stats
+    ms_per_step = dt * 1e3 / args.steps
+    kernel_ms = sum(r.stats["ms_kernel"] for r in runs) / len(runs)
+    out = {"records": int(info.n_sets), "tokens": int(info.n_tokens), "terms": int(info.n_terms),
+           "tokenize_s": tok_s, "ms_per_step": ms_per_step, "join_ms": st["ms_total"], "index_ms": st["ms_index"],
+           "kernel_ms": kernel_ms, "pairs": len(p), "candidates": st["candidates"],
+           "postings": st["postings_scanned"], "probe_tokens": st["probe_tokens"],
+           "index_entries": st["index_entries"], "signature_checks": st["signature_checks"],
+           "signature_checks_128": st["signature_checks_128"], "verified": st["verified"],
+           "verify_tokens": st["verify_tokens"], "kernel_launches": st["kernel_launches"],
+           "phase_cycles": st["phase_cycles"], "window_retries": st["window_retries"],
+           "repeatable": all(np.array_equal(r.q, p.q) and np.array_equal(r.s, p.s) for r in runs)}
+    cb, ctr = None, None
+    if not args.no_cpu:
+        log("long: cpu baseline on sampled queries ...")
+        cb, exact, ctr = cpu_baseline_long(coll, p, t, args.cpu_sample, args.cpu_sample_one)
+        out["bit_exact_vs_oracle_on_sample"] = exact
+        cb["speedup"] = cb["whole_join_estimate_s"] / (ms_per_step / 1e3)
+        log(f"long: sample bit-exact {exact}; CPU whole-join estimate {cb['whole_join_estimate_s']:.0f} s "
+            f"({cb['speedup']:.0f}x)")
+    extra = {}
+    if ctr is not None:
+        # the reference algorithm's bytes: its merge reads scale with the candidates, so
+        # the sample's count is scaled to the whole join; the other terms are exact
+        m_est = ctr["m_reads"] * st["candidates"] / max(ctr["candidates"], 1)
+        rb = 4 * st["probe_tokens"] + 8 * st["postings_scanned"] + 4 * m_est + 8 * st["results"]
+        extra = {"reference_bytes_per_launch": rb,
+                 "reference_equivalent_gbs": rb / (kernel_ms / 1e3) / 1e9,
+                 "reference_equivalent_note": "the reference algorithm's bytes (merge reads estimated from the "
+                                              "oracle's sample) per kernel second: a work-equivalent rate, not "
+                                              "bandwidth; the signature bound skips most of those reads"}
+    roof = roofline("long", [r.stats for r in runs], kernel_ms, extra)
+    coll.close()
+    return out, roof, cb
+
+
+def titles_leg(fg, args):
+    """100k titles (seed 2) at t = 0.6, 0.75, 0.9: one step = the three joins."""
+    import numpy as np
+
+    import oracle
+    coll = fg.Collection.generate(fg.GEN_TITLES, TITLES_RECORDS, TITLES_SEED, device=True)
+    coll.upload()
+
+    def step():
+        return [coll.selfjoin(t) for t in TITLES_T]
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    fg.device_synchronize()
+    kms, last = [], None
+    with gc_paused():
+        t0 = time.perf_counter()
+        for _ in range(args.titles_steps):
+            last = step()
+            kms.append([p.stats["ms_kernel"] for p in last])
+        fg.device_synchronize()
+        dt = time.perf_counter() - t0
+    ms = dt * 1e3 / args.titles_steps
+    kms_t = [sum(k[i] for k in kms) / len(kms) for i in range(len(TITLES_T))]
+    cand = sum(p.stats["candidates"] for p in last)
+    per_t = {str(t): {"join_ms": round(p.stats["ms_total"], 3), "kernel_ms": round(km, 3),
+                      "index_ms": round(p.stats["ms_index"], 3), "pairs": len(p),
+                      "candidates": p.stats["candidates"], "postings": p.stats["postings_scanned"],
+                      "sig_checks": p.stats["signature_checks"], "verified": p.stats["verified"],
+                      "phase_cycles": p.stats["phase_cycles"]}
+             for t, p, km in zip(TITLES_T, last, kms_t)}
+    out = {"workload": f"plain self-join, {TITLES_RECORDS} synthetic titles, q=3, t={list(TITLES_T)}; a step is the "
+                       "three joins", "ms_per_step": ms, "candidates_per_s": cand / (ms / 1e3),
+           "per_threshold": per_t}
+    # roofline over the three launches of a step: bytes of the step / kernel time of the step
+    kb = sum(design_bytes(p.stats) for p in last)
+    ksum = sum(kms_t)
+    roof = roofline("titles", [p.stats for p in last], ksum / len(TITLES_T))
+    roof["achieved"] = kb / (ksum / 1e3) / 1e9
+    roof["frac"] = roof["achieved"] / HBM_PEAK_GBS
+    roof["bytes_per_launch"] = kb / len(TITLES_T)
+    out["roofline"] = roof
+    if not args.no_cpu:
+        oracle.build()
+        ids, lens, pos, tok = coll.arrays()
+        sets = oracle.Sets.from_arrays(ids, lens, pos, tok, coll.info.n_terms)
+        threads = max(1, min(16, host_cpu()["affinity"]))
+        exact, cpu_s, ccand = True, 0.0, 0
+        for t, p in zip(TITLES_T, last):
+            tm = []
+            tri, ctr = oracle.selfjoin(sets, t, nthreads=threads, checks=False, times=tm)
+            cpu_s += tm[0] + tm[1]
+            ccand += ctr["candidates"]
+            got = np.stack([p.q, p.s, p.overlap], axis=1) if len(p) else np.zeros((0, 3), np.int32)
+            exact &= bool(np.array_equal(got, tri)) and p.stats["candidates"] == ctr["candidates"]
+        out["bit_exact_vs_oracle"] = exact
+        out["cpu_oracle"] = {"threads": threads, "step_s": cpu_s, "candidates_per_s": ccand / cpu_s}
+    coll.close()
+    return out
 
 
 def batched_leg(fg, n_jobs: int, reps: int):
@@ -155,87 +332,6 @@ def batched_leg(fg, n_jobs: int, reps: int):
             "plain_sequential_ms": plain_s * 1e3, "batched_ms": batch_s * 1e3,
             "plain_joins_per_s": n_jobs / plain_s, "batched_joins_per_s": n_jobs / batch_s,
             "speedup": plain_s / batch_s, "autotune": tune, "bit_exact_vs_plain": bool(exact)}
-
-
-def large_cpu_baseline(coll, pairs, t, sample: int, seed: int = 11):
-    """The oracle on a uniform sample of queries of the large join (each against the
-    whole collection): the CPU rate, the GPU's pairs for those queries bit-exact, and
-    the sample's merge-read count for the algorithmic bytes."""
-    import numpy as np
-
-    import oracle
-    oracle.build()
-    ids, lens, pos, tok = coll.arrays()
-    sets = oracle.Sets.from_arrays(ids, lens, pos, tok, coll.info.n_terms)
-    qs = np.sort(np.random.default_rng(seed).choice(sets.n, size=min(sample, sets.n), replace=False))
-    threads = max(1, min(16, os.cpu_count() or 1))
-    t0 = time.perf_counter()
-    tri, ctr = oracle.selfjoin(sets, t, nthreads=threads, queries=qs)
-    dt = time.perf_counter() - t0
-    got = np.stack([pairs.q, pairs.s, pairs.overlap], axis=1) if len(pairs) else np.zeros((0, 3), np.int32)
-    got = got[np.isin(got[:, 0], qs)] if len(got) else got
-    exact = got.shape == tri.shape and bool(np.array_equal(got, tri))
-    return ({"value": ctr["candidates"] / dt, "unit": "candidates/s", "cores": threads, "kind": "port",
-             "sample": f"{qs.size} random queries (seed {seed}) of the {sets.n}-record join against the whole "
-                       f"collection, {dt:.1f} s, oracle/fgss_oracle.c closed form, {threads} threads",
-             "sample_s": dt, "sample_queries": int(qs.size)}, exact, ctr)
-
-
-def large_leg(fg, n: int, sample: int, steps: int = 2, t: float = 0.8):
-    """Plain large self-join: n long records (200-1200 chars of Zipf words, 5%
-    near-duplicates with 1-10 edits, seed 3), q=3, t=0.8, tokenized on the GPU."""
-    import numpy as np
-    t = float(np.float32(t))
-    t0 = time.perf_counter()
-    text = fg.generate_text(fg.GEN_LONG, n, 3)
-    t1 = time.perf_counter()
-    coll = fg.Collection.from_text_bytes(text, 3, device=True)
-    tok_s = time.perf_counter() - t1
-    del text
-    info = coll.info
-    log(f"large: {info.n_sets} long records, {info.n_tokens} tokens (GPU tokenizer {tok_s:.2f} s, "
-        f"set-up {time.perf_counter() - t0:.1f} s)")
-    coll.upload()
-    coll.selfjoin(t)   # sizes the workspace
-    runs = []
-    for _ in range(steps):
-        with gc_paused():
-            a = time.perf_counter()
-            p = coll.selfjoin(t)
-            runs.append((time.perf_counter() - a, p))
-    runs.sort(key=lambda r: r[0])
-    wall_s, p = runs[len(runs) // 2]
-    st = p.stats
-    log(f"large: join {st['ms_total']:.0f} ms (kernel {st['ms_kernel']:.0f} ms), {len(p)} pairs, "
-        f"{st['candidates']} candidates")
-    out = {"workload": f"plain self-join, {info.n_sets} synthetic long records (200-1200 chars), q=3, t=0.8",
-           "records": int(info.n_sets), "tokens": int(info.n_tokens), "terms": int(info.n_terms),
-           "tokenize_s": tok_s, "join_ms": st["ms_total"], "kernel_ms": st["ms_kernel"], "wall_ms": wall_s * 1e3,
-           "pairs": len(p), "candidates": st["candidates"], "candidates_per_s": st["candidates"] / (st["ms_total"] / 1e3),
-           "postings": st["postings_scanned"], "probe_tokens": st["probe_tokens"],
-           "index_entries": st["index_entries"], "signature_checks": st["signature_checks"],
-           "verified": st["verified"], "kernel_launches": st["kernel_launches"]}
-    if sample > 0:
-        log(f"large: cpu baseline on {sample} sampled queries ...")
-        cb, exact, c = large_cpu_baseline(coll, p, t, sample)
-        out["cpu_baseline"] = cb
-        out["bit_exact_vs_oracle_on_sample"] = exact
-        # merge reads of the reference scale with the candidates: estimate the whole
-        # join's from the sample, the other terms are the GPU's exact counts
-        m_est = c["m_reads"] * st["candidates"] / max(c["candidates"], 1)
-        kbytes = 4 * st["probe_tokens"] + 8 * st["postings_scanned"] + 4 * m_est + 8 * st["results"]
-        achieved = kbytes / (st["ms_kernel"] / 1e3) / 1e9
-        out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": achieved / HBM_PEAK_GBS, "kernel": "k_join", "bytes_per_launch": kbytes,
-                           "basis": "algorithmic bytes of the reference algorithm; merge reads estimated from the "
-                                    "oracle's sample (scaled by candidates), the rest counted exactly"}
-        cpu_full_s = cb["sample_s"] * info.n_sets / max(cb["sample_queries"], 1)
-        out["cpu_full_estimate_s"] = cpu_full_s
-        out["speedup_vs_cpu_estimate"] = cpu_full_s / (st["ms_total"] / 1e3)
-        log(f"large: sample bit-exact {exact}; CPU estimate {cpu_full_s:.0f} s for the whole join "
-            f"({out['speedup_vs_cpu_estimate']:.0f}x); roofline {achieved:.0f} GB/s")
-    coll.close()
-    return out
 
 
 def _edit(line: bytes, rng, n_edits: int) -> bytes:
@@ -412,20 +508,53 @@ def served_measure(fg, idx, reqs, ts, slots: int, server_kw: dict):
             "bit_exact_vs_plain": bool(exact)}
 
 
+def calibrate_copy(fg, nbytes: int = 1 << 30, reps: int = 3):
+    """`reps` device-to-device copies of `nbytes` (the HIP runtime's copy kernel, after
+    one fill of the source): the reference point for the PMC byte counters, whose
+    per-dispatch FETCH_SIZE and WRITE_SIZE should read nbytes / 1024 KiB each."""
+    import ctypes as C
+    fg.lib()
+    path = next((ln.split()[-1] for ln in open("/proc/self/maps") if "libamdhip64.so" in ln), None)
+    if path is None:
+        raise RuntimeError("the HIP runtime is not mapped")
+    h = C.CDLL(path)
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    h.hipFree.argtypes = [C.c_void_p]
+    src, dst = C.c_void_p(), C.c_void_p()
+    if h.hipMalloc(C.byref(src), nbytes) or h.hipMalloc(C.byref(dst), nbytes):
+        raise RuntimeError("hipMalloc failed")
+    h.hipMemset(src, 7, nbytes)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        if h.hipMemcpy(dst, src, nbytes, 3):   # device to device
+            raise RuntimeError("hipMemcpy failed")
+    fg.device_synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    h.hipFree(src)
+    h.hipFree(dst)
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps, "ms_per_copy": dt * 1e3,
+                      "gbs": 2 * nbytes / dt / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=5, help="timed joins of the headline workload")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--table-slots", type=int, default=0)
-    ap.add_argument("--load-pct", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle legs")
+    ap.add_argument("--long-records", type=int, default=LONG_RECORDS,
+                    help="records of the headline workload (0 skips it: profiling runs of the other legs)")
+    ap.add_argument("--cpu-sample", type=int, default=200, help="queries of the all-threads CPU baseline")
+    ap.add_argument("--cpu-sample-one", type=int, default=30, help="queries of the one-thread CPU baseline")
+    ap.add_argument("--titles-steps", type=int, default=5, help="steps of the 100k-title leg (0 skips it)")
     ap.add_argument("--batch-jobs", type=int, default=512, help="jobs of the batched leg (0 skips it)")
     ap.add_argument("--serve-index", type=int, default=2_000_000, help="indexed titles of the served leg")
     ap.add_argument("--serve-requests", type=int, default=1000, help="requests of the served leg (0 skips it)")
     ap.add_argument("--serve-slots", type=int, default=2)
-    ap.add_argument("--large-records", type=int, default=1_000_000, help="records of the large leg (0 skips it)")
-    ap.add_argument("--large-sample", type=int, default=1000, help="oracle-checked queries of the large leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 1 GiB device-to-device copies (the byte counters' reference point)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the join does not shard across GPUs "
@@ -433,98 +562,40 @@ def main():
         sys.exit(2)
 
     import new_fgssjoin_amd as fg
+    if args.calibrate_copy:
+        calibrate_copy(fg)
+        return
     # fgss_selfjoin returns only after its stream has drained and the pairs are in
     # host memory, so each timed step is already bracketed by device synchronization.
     # torch is deliberately not imported: it would load a second HIP runtime.
-    sync = fg.device_synchronize
-
-    t0 = time.perf_counter()
-    coll = fg.Collection.generate(fg.GEN_TITLES, N_RECORDS, SEED, device=True)
-    info = coll.info
-    log(f"generated+tokenized {info.n_sets} records, {info.n_tokens} tokens, {info.n_terms} terms "
-        f"in {time.perf_counter() - t0:.1f} s")
-    coll.upload()
-
-    def step():
-        return [coll.selfjoin(t, table_slots=args.table_slots, load_pct=args.load_pct) for t in THRESHOLDS]
-
-    for i in range(args.warmup):
-        step()
-        log(f"warmup {i + 1}/{args.warmup} done")
-    sync()
-    last = None
-    kms = []
-    with gc_paused():
-        t0 = time.perf_counter()
-        for i in range(args.steps):
-            last = step()
-            kms.append([p.stats["ms_kernel"] for p in last])
-        sync()
-        dt = time.perf_counter() - t0
-    log(f"timed {args.steps} steps in {dt:.3f} s")
-    cand = sum(p.stats["candidates"] for p in last)
-    ms_per_step = dt * 1e3 / args.steps
-    # k_join launch duration per threshold: mean over the timed steps (HIP events on
-    # the join's own stream around the launch)
-    kms_t = [sum(k[i] for k in kms) / len(kms) for i in range(len(THRESHOLDS))]
-    per_t = {str(t): {"join_ms": round(p.stats["ms_total"], 3), "kernel_ms": round(km, 3),
-                      "index_ms": round(p.stats["ms_index"], 3), "pairs": len(p),
-                      "candidates": p.stats["candidates"], "postings": p.stats["postings_scanned"],
-                      "verify_tokens": p.stats["verify_tokens"], "sig_checks": p.stats["signature_checks"],
-                      "verified": p.stats["verified"], "phase_cycles": p.stats["phase_cycles"]}
-             for t, p, km in zip(THRESHOLDS, last, kms_t)}
+    if args.long_records > 0:
+        long, roof, cb = long_leg(fg, args)
+    else:   # profiling runs of the other legs alone
+        long, roof, cb = {"records": 0, "candidates": 0, "ms_per_step": None}, None, None
     out = {
         "metric": "join ms + candidates verified/s (bit-exact); batched & served p99 vs plain",
-        "value": cand / (ms_per_step / 1e3),
+        "value": long["candidates"] / (long["ms_per_step"] / 1e3) if long["ms_per_step"] else None,
         "unit": "candidates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": long["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": f"plain self-join, {N_RECORDS} synthetic titles, q=3, t={list(THRESHOLDS)}",
-                   "records": info.n_sets, "tokens": info.n_tokens, "terms": info.n_terms,
-                   "per_threshold": per_t},
+        "config": dict({"workload": f"plain self-join, {long['records']} synthetic long records (200-1200 chars, "
+                                    f"seed {LONG_SEED}), q=3, t={LONG_T}; a step is one join"}, **long),
+        "roofline": roof,
+        "cpu_baseline": cb,
     }
-    counters = None
-    if not args.no_cpu:
-        log("cpu baseline: running the oracle on the same step ...")
-        cb, exact, counters = cpu_baseline(coll, last)
-        out["cpu_baseline"] = cb
-        out["bit_exact_vs_oracle"] = exact
-    else:
-        out["cpu_baseline"] = None
-    kernel_s = sum(kms_t) / 1e3
-    n_launch = len(THRESHOLDS)
-    if counters is not None:
-        kbytes = sum(alg_bytes_join_kernel(c) for c in counters)
-        basis = "algorithmic bytes of the reference algorithm, counted by the oracle"
-        join_s = sum(p.stats["ms_total"] for p in last) / 1e3
-        join_frac = sum(alg_bytes_join(c) for c in counters) / join_s / (HBM_PEAK_GBS * 1e9)
-    else:
-        kbytes = sum(design_bytes(p.stats) for p in last)
-        basis = "bytes this build's kernel reads (oracle skipped)"
-        join_frac = None
-    achieved = kbytes / kernel_s / 1e9 if kernel_s > 0 else 0.0
-    prof = pmc_summary()
-    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK_GBS, "traffic": prof.get("traffic_bytes_per_launch"),
-                       "kernel": "k_join", "launches_per_step": n_launch,
-                       "bytes_per_launch": kbytes / n_launch, "kernel_ms_per_launch": kernel_s * 1e3 / n_launch,
-                       "basis": basis, "join_frac": join_frac,
-                       "lds_bank_conflict_ratio": prof.get("lds_bank_conflict_ratio"),
-                       "valu_busy": prof.get("valu_busy_max"),
-                       "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) if prof else None}
+    if args.titles_steps > 0:
+        out["titles"] = titles_leg(fg, args)
     if args.batch_jobs > 0:
         out["batched"] = batched_leg(fg, args.batch_jobs, reps=3)
     if args.serve_requests > 0:
         out["served"] = served_leg(fg, args.serve_index, args.serve_requests, args.serve_slots)
-    if args.large_records > 0:
-        out["large"] = large_leg(fg, args.large_records, 0 if args.no_cpu else args.large_sample)
     print(json.dumps(out), flush=True)
 
 

@@ -66,7 +66,7 @@ def lib():
                                            C.POINTER(i32p), C.POINTER(Counters)]
         L.orc_selfjoin_ex.restype = C.c_int64
         L.orc_selfjoin_ex.argtypes = [C.c_int64, i32p, i64p, i32p, C.c_int32, f, C.c_int, i64p, C.c_int64, C.c_int,
-                                      C.POINTER(i32p), C.POINTER(Counters)]
+                                      C.POINTER(i32p), C.POINTER(Counters), C.POINTER(C.c_double)]
         L.orc_selfjoin_literal.restype = C.c_int64
         L.orc_selfjoin_literal.argtypes = [C.c_int64, i32p, i64p, i32p, C.c_int32, f, C.c_int64,
                                            C.POINTER(i32p), C.POINTER(Counters)]
@@ -148,11 +148,12 @@ class Sets:
 
 
 def selfjoin(sets: Sets, t: float, nthreads: int = 0, mode: str = "closed", block_size: int = 0, queries=None,
-             checks: bool = True):
+             checks: bool = True, times: list | None = None):
     """-> (triples int32 [R,3] of (q, s, overlap) internal indices sorted, counters dict).
     queries: restrict the closed form to these query indices.  checks=False verifies
     each candidate with the reference's merge alone (no closed-form cross-check; the
-    CPU baseline's timed pass)."""
+    CPU baseline's timed pass); then `times`, if a list, receives the seconds of the
+    index build and of the query phase."""
     L = lib()
     if nthreads <= 0:
         nthreads = os.cpu_count() or 1
@@ -160,8 +161,12 @@ def selfjoin(sets: Sets, t: float, nthreads: int = 0, mode: str = "closed", bloc
     ctr = Counters()
     if not checks and mode == "closed":
         qs = np.ascontiguousarray(queries if queries is not None else np.zeros(0), np.int64)
+        secs = (C.c_double * 2)()
         n = L.orc_selfjoin_ex(sets.n, _p32(sets.len), _p64(sets.pos), _p32(sets.tok), sets.n_terms, t, nthreads,
-                              _p64(qs) if queries is not None else None, qs.size, 0, C.byref(out), C.byref(ctr))
+                              _p64(qs) if queries is not None else None, qs.size, 0, C.byref(out), C.byref(ctr),
+                              secs)
+        if times is not None:
+            times[:] = [secs[0], secs[1]]
     elif queries is not None:
         qs = np.ascontiguousarray(queries, np.int64)
         n = L.orc_selfjoin_queries(sets.n, _p32(sets.len), _p64(sets.pos), _p32(sets.tok), sets.n_terms, t,

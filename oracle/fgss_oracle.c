@@ -18,6 +18,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 /* ------------------------------------------------------------------------ */
 /* Bound expressions: include/fgssjoin/similarity.cuh:9-82                   */
@@ -449,8 +450,11 @@ static void *cf_run(void *arg)
     int32_t *lastj = (int32_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
     int32_t *lasts = (int32_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
     int32_t *touched = (int32_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
-    const int64_t CH = 32;
     const int64_t nq = sh->queries ? sh->nq : n;
+    /* queries per claim: 32 over a whole collection, 1 over a subset (a sampled
+     * subset of a few hundred queries of very different cost would otherwise leave
+     * most threads idle) */
+    const int64_t CH = sh->queries ? 1 : 32;
     for (;;) {
         int64_t q0 = atomic_fetch_add(&sh->next, CH);
         if (q0 >= nq) break;
@@ -523,15 +527,25 @@ static void *cf_run(void *arg)
     return NULL;
 }
 
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
 static int64_t selfjoin_closed(int64_t n_sets, const int32_t *len, const int64_t *pos,
                                const int32_t *tok, int32_t n_terms, float t, int nthreads,
-                               const int64_t *queries, int64_t nq, int checks, int32_t **out, orc_counters *ctr)
+                               const int64_t *queries, int64_t nq, int checks, int32_t **out, orc_counters *ctr,
+                               double *secs)
 {
     if (nthreads < 1) nthreads = 1;
     orc_index ix;
     orc_counters total;
     memset(&total, 0, sizeof(total));
+    const double t0 = now_s();
     build_index(n_sets, len, pos, tok, n_terms, t, &ix, &total.e_idx);
+    const double t1 = now_s();
     cf_shared sh;
     sh.n = n_sets; sh.len = len; sh.pos = pos; sh.tok = tok; sh.t = t; sh.ix = &ix;
     sh.queries = queries; sh.nq = nq; sh.checks = checks;
@@ -551,10 +565,12 @@ static int64_t selfjoin_closed(int64_t n_sets, const int32_t *len, const int64_t
         total.v_reads += ws[i].c.v_reads; total.candidates += ws[i].c.candidates;
         total.intersections += ws[i].c.intersections; total.results += ws[i].c.results;
     }
+    const double t2 = now_s();
     qsort(res, (size_t)R, 3 * sizeof(int32_t), cmp_tri);
     free(ws); free(th); free_index(&ix);
     *out = res;
     if (ctr) *ctr = total;
+    if (secs) { secs[0] = t1 - t0; secs[1] = t2 - t1; }
     return R;
 }
 
@@ -562,17 +578,17 @@ int64_t orc_selfjoin(int64_t n_sets, const int32_t *len, const int64_t *pos,
                      const int32_t *tok, int32_t n_terms, float t, int nthreads,
                      int32_t **out, orc_counters *ctr)
 {
-    return selfjoin_closed(n_sets, len, pos, tok, n_terms, t, nthreads, NULL, 0, 1, out, ctr);
+    return selfjoin_closed(n_sets, len, pos, tok, n_terms, t, nthreads, NULL, 0, 1, out, ctr, NULL);
 }
 
 int64_t orc_selfjoin_ex(int64_t n_sets, const int32_t *len, const int64_t *pos, const int32_t *tok,
                         int32_t n_terms, float t, int nthreads, const int64_t *queries, int64_t n_queries,
-                        int checks, int32_t **out, orc_counters *ctr)
+                        int checks, int32_t **out, orc_counters *ctr, double *secs)
 {
     for (int64_t k = 0; queries && k < n_queries; k++)
         if (queries[k] < 0 || queries[k] >= n_sets) return -1;
     return selfjoin_closed(n_sets, len, pos, tok, n_terms, t, nthreads, queries, queries ? n_queries : 0, checks,
-                           out, ctr);
+                           out, ctr, secs);
 }
 
 int64_t orc_selfjoin_queries(int64_t n_sets, const int32_t *len, const int64_t *pos,
@@ -581,7 +597,7 @@ int64_t orc_selfjoin_queries(int64_t n_sets, const int32_t *len, const int64_t *
 {
     for (int64_t k = 0; k < n_queries; k++)
         if (queries[k] < 0 || queries[k] >= n_sets) return -1;
-    return selfjoin_closed(n_sets, len, pos, tok, n_terms, t, nthreads, queries, n_queries, 1, out, ctr);
+    return selfjoin_closed(n_sets, len, pos, tok, n_terms, t, nthreads, queries, n_queries, 1, out, ctr, NULL);
 }
 
 /* ------------------------------------------------------------------------ */

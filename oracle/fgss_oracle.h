@@ -107,10 +107,11 @@ int64_t orc_selfjoin_queries(int64_t n_sets, const int32_t *len, const int64_t *
 /* orc_selfjoin (queries null) or orc_selfjoin_queries, with checks = 0: each
  * candidate is verified by the reference's merge alone (checking.cu:93-175, the
  * score it stores), without the closed-form overlap it is checked against and the
- * suffix read count (v_reads stays 0).  The CPU baseline's timed pass. */
+ * suffix read count (v_reads stays 0).  The CPU baseline's timed pass.  secs (may
+ * be null) receives the wall seconds of the index build and of the query phase. */
 int64_t orc_selfjoin_ex(int64_t n_sets, const int32_t *len, const int64_t *pos, const int32_t *tok,
                         int32_t n_terms, float t, int nthreads, const int64_t *queries, int64_t n_queries,
-                        int checks, int32_t **out, orc_counters *ctr);
+                        int checks, int32_t **out, orc_counters *ctr, double *secs);
 int64_t orc_selfjoin_literal(int64_t n_sets, const int32_t *len, const int64_t *pos,
                              const int32_t *tok, int32_t n_terms, float t,
                              int64_t block_size, int32_t **out, orc_counters *ctr);
