@@ -131,6 +131,8 @@ typedef struct {
     int64_t window_retries;    /* streamed windows of several parts whose table overflowed and that
                                   ran again part by part (streamed items join a growing number of
                                   consecutive parts at a time while the table stays sparse)  */
+    int64_t dense_parts;       /* dense parts joined (windows full enough for a table indexed by the
+                                  source; see fgss_join_opts.dense_pct)                        */
 } fgss_join_stats;
 
 typedef struct {
@@ -142,7 +144,10 @@ typedef struct {
                                   the boundary table                                   */
     int32_t bounds_cap;        /* precomputed split boundaries at most, 0 = default (2^28); the
                                   parts of queries beyond it locate their posting ranges by search */
-    int32_t reserved;
+    int32_t dense_pct;         /* queries whose source window holds at least this many postings per
+                                  100 sources (over several dense parts of 2 x table_slots sources)
+                                  are joined part after part in a table indexed by the source; 0 =
+                                  default (25), < 0 = never                                      */
     int64_t query_begin;       /* self-joins: only the queries q in [query_begin, query_end) (internal
                                   indices), each against every s > q; 0, 0 = all (another empty range:
                                   none).  One GPU's share of

@@ -67,7 +67,7 @@ class JoinStats(C.Structure):
                 ("ms_copy", C.c_double), ("ms_total", C.c_double),
                 ("ms_kernel", C.c_double), ("kernel_launches", C.c_int64),
                 ("signature_checks", C.c_int64), ("signature_checks_128", C.c_int64), ("verified", C.c_int64),
-                ("phase_cycles", C.c_int64 * 6), ("window_retries", C.c_int64)]
+                ("phase_cycles", C.c_int64 * 6), ("window_retries", C.c_int64), ("dense_parts", C.c_int64)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}
@@ -77,7 +77,7 @@ class JoinStats(C.Structure):
 
 class JoinOpts(C.Structure):
     _fields_ = [("load_pct", C.c_int32), ("table_slots", C.c_int32), ("stream_parts", C.c_int32),
-                ("bounds_cap", C.c_int32), ("reserved", C.c_int32), ("query_begin", C.c_int64),
+                ("bounds_cap", C.c_int32), ("dense_pct", C.c_int32), ("query_begin", C.c_int64),
                 ("query_end", C.c_int64), ("stream", C.c_void_p)]
 
 
@@ -315,12 +315,13 @@ class Collection:
         _check(lib().fgss_collection_upload(self._h))
 
     def selfjoin(self, threshold: float, table_slots: int = 0, load_pct: int = 0, bounds_cap: int = 0,
-                 stream_parts: int = 0, queries: tuple | None = None, stream: int = 0) -> Pairs:
+                 stream_parts: int = 0, queries: tuple | None = None, stream: int = 0, dense_pct: int = 0) -> Pairs:
         """queries=(begin, end): only those queries (internal indices), each against every
-        later record; stream: a hipStream_t (as int) for the call's device work."""
+        later record; stream: a hipStream_t (as int) for the call's device work;
+        dense_pct: see fgss_join_opts (-1: no dense parts)."""
         L = lib()
         qb, qe = queries if queries is not None else (0, 0)
-        opts = JoinOpts(load_pct, table_slots, stream_parts, bounds_cap, 0, qb, qe, stream or None)
+        opts = JoinOpts(load_pct, table_slots, stream_parts, bounds_cap, dense_pct, qb, qe, stream or None)
         out = C.POINTER(_Pairs)()
         _check(L.fgss_selfjoin(self._h, C.c_float(threshold), C.byref(opts), C.byref(out)))
         return _take_pairs(out)

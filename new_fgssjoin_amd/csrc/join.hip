@@ -91,12 +91,13 @@ enum Ctr {
     C_CYC_D,
     C_CYC_RANGES,    // part of C_CYC_A spent locating posting ranges
     C_RETRY,         // streamed windows of several parts that overflowed and ran again part by part
+    C_DENSE,         // dense parts joined
     C_COUNT
 };
 
 struct Item {
     uint32_t q, a, b;   // query and source sub-window [a, b)
-    uint32_t pad;       // expected useful postings (sizes the LDS table) | kWholeWindow | kPlanned | kStreamed
+    uint32_t pad;       // expected useful postings (sizes the LDS table) | kWholeWindow | kPlanned | kStreamed | kDense
     uint32_t part, nparts;  // planned items: part index and part count of the query
     uint32_t bofs;          // planned items: first boundary position of the query in the bounds table
     uint32_t job;           // the query's job
@@ -122,7 +123,18 @@ constexpr uint32_t kMidProbe = 0x10000u;
 constexpr uint32_t kWholeWindow = 0x80000000u;   // the item covers the query's whole window
 constexpr uint32_t kPlanned = 0x40000000u;       // part of a planned split: ranges come from the bounds table
 constexpr uint32_t kStreamed = 0x20000000u;      // the whole window, walked part by part by one wavefront
-constexpr uint32_t kPadMask = 0x1FFFFFFFu;
+constexpr uint32_t kDense = 0x10000000u;         // consecutive dense parts of the window, walked in order
+constexpr uint32_t kPadMask = 0x0FFFFFFFu;
+// nparts flag of a query joined in dense parts: 2 x table-slots consecutive sources each,
+// counted in a table indexed by the source (no hashing); the count is that of dense parts
+constexpr uint32_t kDenseParts = 0x80000000u;
+#ifndef FGSS_DENSE_CHUNK
+#define FGSS_DENSE_CHUNK 32
+#endif
+#ifndef FGSS_DENSE_LOGK_MIN
+#define FGSS_DENSE_LOGK_MIN 3
+#endif
+constexpr uint32_t kDenseChunk = FGSS_DENSE_CHUNK;   // dense parts per work item
 // streamed items keep one posting cursor per probe token in the wavefront's LDS
 // (the range arrays of the other items)
 constexpr uint32_t kStreamQp = 256;
@@ -148,9 +160,10 @@ constexpr uint32_t kMaxSpan = FGSS_MAX_SPAN;     // parts per window of a stream
 constexpr uint32_t kSpanGrowPct = FGSS_SPAN_GROW_PCT, kSpanShrinkPct = FGSS_SPAN_SHRINK_PCT;
 
 // How the items of a query with np parts read their posting ranges.
-enum ItemMode : uint32_t { IM_NONE = 0, IM_WHOLE, IM_PLANNED, IM_STREAMED, IM_SEARCHED };
+enum ItemMode : uint32_t { IM_NONE = 0, IM_WHOLE, IM_PLANNED, IM_STREAMED, IM_SEARCHED, IM_DENSE };
 __device__ __forceinline__ uint32_t item_mode(uint32_t np, uint32_t qp, uint64_t bcnt, uint64_t bofs, uint32_t cap_bounds)
 {
+    if (np & kDenseParts) return IM_DENSE;
     if (np == 0) return IM_NONE;
     if (np == 1) return IM_WHOLE;
     if (bcnt > 0 && bofs + bcnt <= cap_bounds) return IM_PLANNED;
@@ -193,7 +206,7 @@ struct fgss_device_sets {
     // timing events and the counter copies enqueued between and after them
     hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
     hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};
-    std::array<int64_t, 14> graph_key{};   // what the captured stages were built for (graph_key_of)
+    std::array<int64_t, 16> graph_key{};   // what the captured stages were built for (graph_key_of)
     int cap_gen = 0;              // bumped whenever a buffer the pipeline uses is reallocated
     bool sized = false;           // a run has completed (first-run buffer sizing is over)
     int32_t q_begin = 0, q_end = 0;   // the run's query range (internal indices; 0, 0: every record)
@@ -622,7 +635,8 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
                                               uint32_t heavy_target,
                                               uint2 *__restrict__ rng, uint32_t *__restrict__ hi_out,
                                               uint32_t *__restrict__ nparts, uint32_t *__restrict__ eq_out,
-                                              uint64_t *__restrict__ bcnt, uint32_t stream_parts)
+                                              uint64_t *__restrict__ bcnt, uint32_t stream_parts, uint32_t dense_w,
+                                              uint32_t dense_pct)
 {
     const int64_t ngroups = (int64_t)gridDim.x * (blockDim.x / kPlanGroup);
     const uint32_t gl = (uint32_t)lane_id() & (kPlanGroup - 1);   // lane within the group
@@ -664,10 +678,18 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
             // a query of more than kHeavyParts parts fills its parts fuller: fewer parts
             // (boundary rows, streamed rounds) outweigh the longer probe chains there
             if (np > kHeavyParts) np = (e + heavy_target - 1) / heavy_target;
-            nparts[q] = np;
+            // dense parts: a window of several parts whose postings fill dense parts of
+            // dense_w consecutive sources to dense_pct% of their slots on average, for a
+            // probe that fits the cursor slots (windows this full would need hash-table
+            // parts of a few postings per probe token)
+            const uint32_t win = wend - (uint32_t)q - 1u;
+            const uint32_t ndp = dense_w ? (win + dense_w - 1u) / dense_w : 0u;
+            const bool dense = ndp > 1 && np > 1 && qp <= kStreamQp &&
+                               (uint64_t)e * 100u >= (uint64_t)ndp * dense_w * dense_pct;
+            nparts[q] = dense ? (ndp | kDenseParts) : np;
             // internal boundaries x probe tokens, for queries whose parts read a planned
             // boundary table (heavier ones are streamed when their probe fits a cursor slot)
-            bcnt[q] = np > 1 && (np <= stream_parts || qp > kStreamQp) ? (uint64_t)(np - 1) * qp : 0ull;
+            bcnt[q] = !dense && np > 1 && (np <= stream_parts || qp > kStreamQp) ? (uint64_t)(np - 1) * qp : 0ull;
         }
     }
 }
@@ -745,7 +767,8 @@ __global__ void k_nitems(const uint32_t *__restrict__ nparts, const uint32_t *__
     if (q >= n) return;
     const uint32_t np = nparts[q];
     const uint32_t m = item_mode(np, probelen[q], bcnt[q], bofs[q], cap_bounds);
-    nitems[q] = m == IM_NONE ? 0u : m == IM_WHOLE ? 1u : m == IM_STREAMED ? (np + kStreamChunk - 1) / kStreamChunk : np;
+    nitems[q] = m == IM_NONE ? 0u : m == IM_WHOLE ? 1u : m == IM_STREAMED ? (np + kStreamChunk - 1) / kStreamChunk
+              : m == IM_DENSE ? ((np & ~kDenseParts) + kDenseChunk - 1) / kDenseChunk : np;
 }
 
 // wlo: window start per query (null: q + 1, the self-join window).  rs: the queries
@@ -764,7 +787,7 @@ __global__ __launch_bounds__(256) void k_items(int rs, const uint32_t *__restric
                                                const float *__restrict__ tj, const uint32_t *__restrict__ len,
                                                const uint32_t *__restrict__ pos, const uint32_t *__restrict__ poff,
                                                int64_t n, const uint32_t *__restrict__ n_items_dev,
-                                               Item *__restrict__ items, uint32_t cap_items)
+                                               Item *__restrict__ items, uint32_t cap_items, uint32_t dense_w)
 {
     const uint32_t total = *n_items_dev;
     if (total > cap_items) return;   // too small: the host resizes and reruns
@@ -780,14 +803,20 @@ __global__ __launch_bounds__(256) void k_items(int rs, const uint32_t *__restric
         const uint32_t np = nparts[q];
         const uint32_t mode = item_mode(np, probelen[q], bcnt[q], bofs[q], cap_bounds);
         const uint64_t a0 = wlo ? (uint64_t)wlo[q] : (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
-        const uint32_t per = (eq[q] + np - 1) / np;   // expected postings per part: sizes its LDS table
+        const uint32_t npc = np & ~kDenseParts;
+        const uint32_t per = min((eq[q] + npc - 1) / npc, kPadMask);   // expected postings per part: sizes its LDS table
         const int64_t r = rs ? q >> 1 : q;           // the record whose tokens probe
         const uint32_t flag = rs && (q & 1) == 0 ? kMidProbe : 0u;
         const uint32_t fl = mode == IM_WHOLE ? kWholeWindow : mode == IM_PLANNED ? kPlanned
-                          : mode == IM_STREAMED ? kStreamed : 0u;
+                          : mode == IM_STREAMED ? kStreamed : mode == IM_DENSE ? kDense : 0u;
         Item it;
         it.q = (uint32_t)r;
-        if (mode == IM_STREAMED) {   // chunk p: parts [p * kStreamChunk, ...) of the query's np
+        if (mode == IM_DENSE) {      // chunk p: dense parts [p * kDenseChunk, ...) of dense_w sources each
+            const uint32_t p0 = p * kDenseChunk, p1 = min(p0 + kDenseChunk, npc);
+            it.a = (uint32_t)(a0 + (uint64_t)p0 * dense_w);
+            it.b = (uint32_t)min<uint64_t>(a0 + (uint64_t)p1 * dense_w, (uint64_t)hi[q]);
+            it.nparts = p1 - p0;
+        } else if (mode == IM_STREAMED) {   // chunk p: parts [p * kStreamChunk, ...) of the query's np
             const uint32_t p0 = p * kStreamChunk, p1 = min(p0 + kStreamChunk, np);
             it.a = part_start(a0, W, p0, np);
             it.b = part_start(a0, W, p1, np);
@@ -946,6 +975,23 @@ __device__ __forceinline__ bool table_insert(const uint2 (&pe)[N], uint32_t valm
     return __ballot(failm != 0) != 0;
 }
 
+// The dense counterpart of table_insert, for a dense part of sources [wa, wa + 2 x CAP):
+// a source's slot is its offset from wa (no hashing or probing, so nothing overflows),
+// its match count is 16 bits, two slots to a word of dcnt, and dlast keeps its last
+// (query position, source position).  Lanes without work aim their updates at scr[lane].
+template <int N>
+__device__ __forceinline__ void dense_insert(const uint2 (&pe)[N], uint32_t valm, const uint32_t (&jpos)[N], uint32_t wa,
+                                             uint32_t *dlast, uint32_t *dcnt, uint32_t *scr, int lane)
+{
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const bool v = (valm >> k) & 1u;
+        const uint32_t h = pe[k].x - wa;
+        atomicAdd(v ? &dcnt[h >> 1] : &scr[lane], 1u << ((h & 1u) << 4));
+        atomicMax(v ? &dlast[h] : &scr[lane], (jpos[k] << 16) | (pe[k].y & 0xFFFFu));
+    }
+}
+
 // Work item = (query q, source window [a, b)).  Phases, all inside one wavefront:
 //  A  candidate generation: the postings of the probe-prefix tokens whose record id
 //     lies in the window, flattened over all tokens, kUnroll per lane in flight; per
@@ -976,8 +1022,8 @@ constexpr int kUnroll = FGSS_UNROLL;
 constexpr int kQStage = 160;   // query tokens staged in LDS for verification (longer queries stay in HBM)
 // per-wave counters (lane k of one 64-bit register holds counter k): totals,
 // shader-clock phases, the current job's counters
-enum WaveAcc { W_ESCAN = 0, W_CAND, W_SIG, W_VER, W_SIG2, W_INTER, W_VTOK, W_RETRY, W_CYC0, W_JOB0 = W_CYC0 + 6,
-               kAccSlots = W_JOB0 + kJobCtr };
+enum WaveAcc { W_ESCAN = 0, W_CAND, W_SIG, W_VER, W_SIG2, W_INTER, W_VTOK, W_RETRY, W_DENSE, W_CYC0,
+               W_JOB0 = W_CYC0 + 6, kAccSlots = W_JOB0 + kJobCtr };
 // per-wavefront slot in global memory at the end of a launch: the kAccSlots
 // counters and the wavefront's last job
 constexpr int kWaveSlot = 32;
@@ -1122,14 +1168,28 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         // token (LDS), starting at the token's first posting inside the item's window
         uint32_t *cur = rlo;
         static_assert(2 * kTokBatch + 1 >= (int)kStreamQp, "cursors live in rlo and rpre");
-        if (streamed) {
+        // A dense item walks its nparts dense parts of DW consecutive sources in order,
+        // with cursors as a streamed item.  Its table is indexed by the source's offset in
+        // the part: dlast (the key and count words of the hash table) and dcnt (16-bit
+        // counts, the last-match words); nsid (the last kStreamQp words of the list) holds
+        // the record id at each token's cursor, so that a part rounds only over the
+        // tokens with postings in it.  The hash table's empty state is restored at the end.
+        constexpr uint32_t DW = 2u * CAP;
+        constexpr uint32_t kDenseList = CAP > kStreamQp ? CAP - kStreamQp : 0u;   // candidates listed at once
+        const bool dense = CAP >= 512 && !RS && (item.pad & kDense) != 0;
+        uint32_t *const dlast = tkey, *const dcnt = tlast, *const nsid = clist + kDenseList;
+        if (streamed || dense) {
             for (uint32_t j = lane; j < qp; j += kWave) {
                 const uint2 r = qrng[j];
-                cur[j] = item.part == 0 ? r.x : lower_bound_sid(A.psid, A.fence, r.x, r.y, item.a);
+                const uint32_t c = item.part == 0 ? r.x : lower_bound_sid(A.psid, A.fence, r.x, r.y, item.a);
+                cur[j] = c;
+                if (dense) nsid[j] = c < r.y ? A.psid[c] : kEmpty;
             }
+            if (dense)
+                for (uint32_t i = lane; i < DW; i += kWave) dlast[i] = 0u;
             wave_sync();
         }
-        const uint32_t nwin = streamed ? item.nparts : 1u;
+        const uint32_t nwin = streamed || dense ? item.nparts : 1u;
         PHASE_MARK(0);
 
         // a streamed item joins its parts a window of `span` consecutive parts at a
@@ -1141,7 +1201,11 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         for (uint32_t p0 = 0, p1 = 0; p0 < nwin; p0 = p1) {
         p1 = min(p0 + span, nwin);
         uint32_t wa = item.a, wb = item.b;
-        if (streamed) {
+        if (dense) {
+            p1 = p0 + 1;
+            wa = item.a + p0 * DW;
+            wb = min(wa + DW, item.b);
+        } else if (streamed) {
             const uint64_t W = (uint64_t)item.b - item.a;
             wa = part_start(item.a, W, p0, nwin);
             wb = part_start(item.a, W, p1, nwin);
@@ -1150,7 +1214,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         // ---- A: candidate generation (filtering.cu:94-139) ----
         bool overflow = false;
         uint32_t escan = 0, nlist = 0, jdone = 0;
-        for (uint32_t jb = 0; !streamed && jb < qp && !overflow; jb += kTokBatch) {
+        for (uint32_t jb = 0; !streamed && !dense && jb < qp && !overflow; jb += kTokBatch) {
             const uint32_t nb = min((uint32_t)kTokBatch, qp - jb);
             PHASE_START(r0);
             for (uint32_t jj = lane; jj < nb; jj += kWave) {
@@ -1268,12 +1332,13 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             }
             wave_sync();
         }
-        if (streamed) {
+        if (streamed || dense) {
             // A streamed part [wa, wb): every probe token with postings left takes the
             // next K of them from its cursor (independent loads, no search).  Those
             // before wb are this part's; the group of K lanes of a token moves its
             // cursor past them, and a token whose K were all before wb takes K more
-            // (K doubling up to 64) in the next round.
+            // (K doubling up to 64) in the next round.  A dense part takes only the
+            // tokens whose cursor record lies before wb.
             uint8_t *toks = (uint8_t *)qstage;          // active token ids (bytes)
             uint32_t *satm = qstage + kStreamQp / 4;    // kStreamQp bits: tokens with postings left
             uint32_t *sscr = satm + kStreamQp / 32;     // a word per lane for updates without work
@@ -1281,7 +1346,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             uint32_t nact = 0;
             for (uint32_t j0 = 0; j0 < qp; j0 += kWave) {
                 const uint32_t j = j0 + lane;
-                const bool act = j < qp && cur[j] < qrng[j].y;
+                const bool act = j < qp && (dense ? nsid[j] < wb : cur[j] < qrng[j].y);
                 const uint64_t m = __ballot(act);
                 if (act) toks[nact + (uint32_t)__popcll(m & lanemask_lt())] = (uint8_t)j;
                 nact += (uint32_t)__popcll(m);
@@ -1290,8 +1355,13 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             // first round: K = 8 lanes per token.  A round waits for the cursor moves of
             // the one before it, so fewer rounds beat fewer idle lanes up to a point
             // (1M long records at t = 0.8, about 2.4 postings per token and part:
-            // K = 4 2.11 s, 8 2.02 s, 16 2.51 s; starting at 1 2.59 s)
+            // K = 4 2.11 s, 8 2.02 s, 16 2.51 s; starting at 1 2.59 s).  A dense part
+            // starts at the power of two at or above its expected postings per active token.
             uint32_t logK = 3;
+            if (dense && nact > 0) {
+                const uint32_t per_tok = ((item.pad & kPadMask) + nact - 1) / nact;
+                logK = (uint32_t)min(6, max(FGSS_DENSE_LOGK_MIN, ceil_log2(per_tok)));
+            }
             while (nact > 0 && !overflow) {
                 if (lane < (int)(kStreamQp / 32)) satm[lane] = 0;
                 wave_sync();
@@ -1330,12 +1400,20 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                         const bool lead = e < total && ((uint32_t)lane & (K - 1)) == 0 && cnt > 0;
                         *(lead ? &cur[jv[k]] : &sscr[lane]) = curj[k] + cnt;
                         atomicOr(lead && cnt == K ? &satm[jv[k] >> 5] : &sscr[lane], 1u << (jv[k] & 31));
+                        // dense: the first lane of the group past the part holds the record at
+                        // the token's new cursor (none when the list ended there)
+                        const bool next = dense && e < total && ((uint32_t)lane & (K - 1)) == cnt;
+                        *(next ? &nsid[jv[k]] : &sscr[lane]) = idx[k] < end[k] ? pe[k].x : kEmpty;
                         uint32_t v = below;
                         if (RS) {   // the source's own prefix at t
                             const uint32_t ls = pe[k].y >> 16;
                             v &= (uint32_t)((pe[k].y & 0xFFFFu) < (midprobe ? fgss_max_prefix(ls, t) : fgss_mid_prefix(ls, t)));
                         }
                         valm |= v << k;
+                    }
+                    if (dense) {
+                        dense_insert(pe, valm, jv, wa, dlast, dcnt, sscr, lane);
+                        return false;
                     }
                     return table_insert(pe, valm, jv, tkey, tcnt, tlast, clist, sscr, lane, tlog, tsize, nlist);
                 };
@@ -1399,44 +1477,33 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         }
         PHASE_MARK(1);
 
-        // ---- B: positional filter over the distinct sources ----
-        // (branch-free like phase A: lanes past the list work on its last entry and
-        // aim their stores at a word of their own in the idle query stage)
-        uint32_t ncand = 0, cand = 0;
-        for (uint32_t base = 0; base < nlist; base += kWave) {
-            const uint32_t k = base + lane;
-            const bool in = k < nlist;
-            const uint32_t ent = clist[min(k, nlist - 1u)];
-            const uint32_t h = ent & 0xFFFFu;
-            const uint32_t ls = ent >> 16;
-            const uint32_t m = tcnt[h];
-            const uint32_t last = tlast[h];
-            const float bound = fgss_min_overlap(lq, ls, t);
-            const uint32_t js = last >> 16, ss = last & 0xFFFFu;
-            const uint32_t rq = lq - js, rs = ls - ss;
-            const uint32_t minrem = rq < rs ? rq : rs;
-            const bool pass = in && !((float)(m + minrem) < bound);   // filtering.cu:127-133 at the last match
-            cand += (uint32_t)__popcll(__ballot(pass));
-            // overlap can still reach floor(bound): m + min(|Q|-js-1, |S|-ss-1)
-            const bool keep = pass && (int)(m + minrem - 1) >= (int)floorf(bound);
-            const bool drop = in && !keep;
-            *(drop ? &tkey[h] : &qstage[lane]) = kEmpty;
-            *(drop ? &tcnt[h] : &qstage[lane]) = 0;
-            *(drop ? &tlast[h] : &qstage[lane]) = 0;
-            const uint64_t bal = __ballot(keep);
-            wave_sync();
-            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-            *(keep ? &clist[ncand + below] : &qstage[lane]) = ent;
-            ncand += (uint32_t)__popcll(bal);
-        }
-        wave_sync();
-        PHASE_MARK(2);
+        // table slot accessors of phases C and D: a hash slot (its key is the source) or a
+        // dense slot (the source is wa + slot); the branches on `dense` are wave-uniform
+        auto src_of = [&](uint32_t h) -> uint32_t { return dense ? wa + h : tkey[h]; };
+        auto cnt_of = [&](uint32_t h) -> uint32_t {
+            return dense ? (dcnt[h >> 1] >> ((h & 1u) << 4)) & 0xFFFFu : tcnt[h];
+        };
+        auto last_of = [&](uint32_t h) -> uint32_t { return dense ? dlast[h] : tlast[h]; };
+        // empties slot h when drop is set (the other lanes write a word of their own in the
+        // idle query stage: no branch on a per-lane condition)
+        auto drop_slot = [&](bool drop, uint32_t h) {
+            if (dense) {
+                *(drop ? &dlast[h] : &qstage[lane]) = 0u;
+                atomicAnd(drop ? &dcnt[h >> 1] : &qstage[lane], (h & 1u) ? 0x0000FFFFu : 0xFFFF0000u);
+            } else {
+                *(drop ? &tkey[h] : &qstage[lane]) = kEmpty;
+                *(drop ? &tcnt[h] : &qstage[lane]) = 0;
+                *(drop ? &tlast[h] : &qstage[lane]) = 0;
+            }
+        };
+        uint32_t cand = 0, nsig = 0, nver = 0, inter = 0, vtok = 0, nres = 0;
 
-        // ---- C: signature bound, one lane per candidate: 64 buckets, then 128 for the rest ----
+        // ---- C: signature bound over clist[0, nc), one lane per candidate: 64 buckets,
+        // then 128 for the rest; the survivors are compacted to the front of the list and
+        // the others' slots emptied.  Returns the survivors.
         // (the source's bad-signature flag and its 64-bucket line are read together)
-        const uint32_t nsig = ncand;
-        uint32_t nver = 0;
-        {
+        auto phase_c = [&](const uint32_t nsig) -> uint32_t {
+            uint32_t nver = 0;
             const uint4 *qs4 = (const uint4 *)(A.qsig + (size_t)q * kSigWords);
             const uint4 *qh4 = (const uint4 *)(A.qsig64 + (size_t)q * kSig64Words);
             const bool qbad = A.qsigbad[q] != 0;
@@ -1451,7 +1518,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                 for (int u = 0; u < kSigPer; u++) {
                     const uint32_t c = base + (uint32_t)(u * kWave) + lane;
                     ent[u] = clist[min(c, nsig - 1u)];
-                    s[u] = tkey[ent[u] & 0xFFFFu];
+                    s[u] = src_of(ent[u] & 0xFFFFu);
                 }
 #pragma unroll
                 for (int u = 0; u < kSigPer; u++) {
@@ -1502,10 +1569,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                 for (int u = 0; u < kSigPer; u++) {
                     const bool in = base + (uint32_t)(u * kWave) + lane < nsig;
                     const uint32_t h = ent[u] & 0xFFFFu;
-                    const bool drop = in && !keep[u];
-                    *(drop ? &tkey[h] : &qstage[lane]) = kEmpty;
-                    *(drop ? &tcnt[h] : &qstage[lane]) = 0;
-                    *(drop ? &tlast[h] : &qstage[lane]) = 0;
+                    drop_slot(in && !keep[u], h);
                 }
                 wave_sync();
 #pragma unroll
@@ -1517,69 +1581,153 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                 }
             }
             wave_sync();
-        }
-        PHASE_MARK(3);
+            return nver;
+        };
 
-        // ---- D: verification (checking.cu:93-175): overlap of S after the last prefix match ----
-        uint32_t inter = 0, vtok = 0, nres = 0;
-        const bool staged = lq <= (uint32_t)kQStage;
-        if (nver > 0 && staged) {
-            for (uint32_t i = lane; i < lq; i += kWave) qstage[i] = A.qtok[qpos + i];
-            wave_sync();
-        }
-        const uint32_t *Qv = staged ? qstage : A.qtok + qpos;
-        for (uint32_t c = 0; c < nver; c++) {
-            const uint32_t ent = clist[c];
-            const uint32_t h = ent & 0xFFFFu, ls = ent >> 16;
-            const uint32_t s = tkey[h];
-            const uint32_t m = tcnt[h];
-            const uint32_t js = tlast[h] >> 16, ss = tlast[h] & 0xFFFFu;
-            const uint32_t spos = A.pos[s];
-            const float bound = fgss_min_overlap(lq, ls, t);
-            const int floorb = (int)floorf(bound);
-            uint32_t ov = m;
-            uint32_t base = ss + 1;
-            for (; base < ls; base += kWave) {
-                const uint32_t i = base + lane;
-                bool hit = false;
-                if (i < ls) {
-                    // tokens of S after position ss exceed Q[js]: search Q(js, lq)
-                    const uint32_t tk = A.tok[spos + i];
-                    uint32_t lo = js + 1, hi = lq;
-                    while (lo < hi) {
-                        uint32_t mid = (lo + hi) >> 1;
-                        if (Qv[mid] < tk) lo = mid + 1;
-                        else hi = mid;
-                    }
-                    hit = lo < lq && Qv[lo] == tk;
-                }
-                ov += (uint32_t)__popcll(__ballot(hit));
-                const int rem = (int)ls - (int)(base + kWave);
-                if ((int)ov + (rem > 0 ? rem : 0) < floorb) { base += kWave; break; }
+        // ---- D: verification (checking.cu:93-175) of clist[0, nver): overlap of S after
+        // the last prefix match; the slots are emptied afterwards ----
+        auto phase_d = [&](const uint32_t nver) {
+            const bool staged = lq <= (uint32_t)kQStage;
+            if (nver > 0 && staged) {
+                for (uint32_t i = lane; i < lq; i += kWave) qstage[i] = A.qtok[qpos + i];
+                wave_sync();
             }
-            vtok += min(base, ls) - (ss + 1);
-            if ((int)ov >= floorb) {
-                inter++;
-                if (fgss_similarity(lq, ls, ov) >= t) {            // block.cu:189-192
-                    nres++;
-                    if (lane == 0) {
-                        uint32_t r = (uint32_t)atomicAdd(&A.ctr[C_RESULTS], 1ull);
-                        if (r < A.out_cap) {
-                            A.out[3 * (size_t)r] = (int32_t)q;
-                            A.out[3 * (size_t)r + 1] = (int32_t)s;
-                            A.out[3 * (size_t)r + 2] = (int32_t)ov;
+            const uint32_t *Qv = staged ? qstage : A.qtok + qpos;
+            for (uint32_t c = 0; c < nver; c++) {
+                const uint32_t ent = clist[c];
+                const uint32_t h = ent & 0xFFFFu, ls = ent >> 16;
+                const uint32_t s = src_of(h);
+                const uint32_t m = cnt_of(h);
+                const uint32_t lastm = last_of(h);
+                const uint32_t js = lastm >> 16, ss = lastm & 0xFFFFu;
+                const uint32_t spos = A.pos[s];
+                const float bound = fgss_min_overlap(lq, ls, t);
+                const int floorb = (int)floorf(bound);
+                uint32_t ov = m;
+                uint32_t base = ss + 1;
+                for (; base < ls; base += kWave) {
+                    const uint32_t i = base + lane;
+                    bool hit = false;
+                    if (i < ls) {
+                        // tokens of S after position ss exceed Q[js]: search Q(js, lq)
+                        const uint32_t tk = A.tok[spos + i];
+                        uint32_t lo = js + 1, hi = lq;
+                        while (lo < hi) {
+                            uint32_t mid = (lo + hi) >> 1;
+                            if (Qv[mid] < tk) lo = mid + 1;
+                            else hi = mid;
+                        }
+                        hit = lo < lq && Qv[lo] == tk;
+                    }
+                    ov += (uint32_t)__popcll(__ballot(hit));
+                    const int rem = (int)ls - (int)(base + kWave);
+                    if ((int)ov + (rem > 0 ? rem : 0) < floorb) { base += kWave; break; }
+                }
+                vtok += min(base, ls) - (ss + 1);
+                if ((int)ov >= floorb) {
+                    inter++;
+                    if (fgss_similarity(lq, ls, ov) >= t) {            // block.cu:189-192
+                        nres++;
+                        if (lane == 0) {
+                            uint32_t r = (uint32_t)atomicAdd(&A.ctr[C_RESULTS], 1ull);
+                            if (r < A.out_cap) {
+                                A.out[3 * (size_t)r] = (int32_t)q;
+                                A.out[3 * (size_t)r + 1] = (int32_t)s;
+                                A.out[3 * (size_t)r + 2] = (int32_t)ov;
+                            }
                         }
                     }
                 }
             }
+            wave_sync();
+            for (uint32_t c = lane; c < nver; c += kWave) drop_slot(true, clist[c] & 0xFFFFu);
+            wave_sync();
+        };
+
+        if (!dense) {
+            // ---- B: positional filter over the distinct sources ----
+            // (branch-free like phase A: lanes past the list work on its last entry and
+            // aim their stores at a word of their own in the idle query stage)
+            uint32_t ncand = 0;
+            for (uint32_t base = 0; base < nlist; base += kWave) {
+                const uint32_t k = base + lane;
+                const bool in = k < nlist;
+                const uint32_t ent = clist[min(k, nlist - 1u)];
+                const uint32_t h = ent & 0xFFFFu;
+                const uint32_t ls = ent >> 16;
+                const uint32_t m = tcnt[h];
+                const uint32_t last = tlast[h];
+                const float bound = fgss_min_overlap(lq, ls, t);
+                const uint32_t js = last >> 16, ss = last & 0xFFFFu;
+                const uint32_t rq = lq - js, rs = ls - ss;
+                const uint32_t minrem = rq < rs ? rq : rs;
+                const bool pass = in && !((float)(m + minrem) < bound);   // filtering.cu:127-133 at the last match
+                cand += (uint32_t)__popcll(__ballot(pass));
+                // overlap can still reach floor(bound): m + min(|Q|-js-1, |S|-ss-1)
+                const bool keep = pass && (int)(m + minrem - 1) >= (int)floorf(bound);
+                const bool drop = in && !keep;
+                *(drop ? &tkey[h] : &qstage[lane]) = kEmpty;
+                *(drop ? &tcnt[h] : &qstage[lane]) = 0;
+                *(drop ? &tlast[h] : &qstage[lane]) = 0;
+                const uint64_t bal = __ballot(keep);
+                wave_sync();
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                *(keep ? &clist[ncand + below] : &qstage[lane]) = ent;
+                ncand += (uint32_t)__popcll(bal);
+            }
+            wave_sync();
+            wave_sync();
+            PHASE_MARK(2);
+            nsig = ncand;
+            nver = phase_c(ncand);
+            PHASE_MARK(3);
+            phase_d(nver);
+            PHASE_MARK(4);
+        } else if constexpr (kDenseList >= (uint32_t)kWave) {
+            // ---- B (dense): the positional filter over every slot of the part, 64 at a
+            // time; the candidates that can still reach floor(bound) are listed with their
+            // source length, the others' slots emptied.  A full list goes through phases
+            // C and D before the scan goes on. ----
+            const uint32_t nslots = wb - wa;   // <= DW, and slot0 < nslots keeps every slot below DW
+            for (uint32_t slot0 = 0; slot0 < nslots;) {
+                uint32_t nl = 0;
+                for (; slot0 < nslots && nl + (uint32_t)kWave <= kDenseList; slot0 += kWave) {
+                    const uint32_t h = slot0 + lane;
+                    const bool in = h < nslots;
+                    const uint32_t cw = dcnt[h >> 1];
+                    const uint32_t m = in ? (cw >> ((h & 1u) << 4)) & 0xFFFFu : 0u;
+                    const bool has = m != 0;
+                    if (__ballot(has) == 0) continue;
+                    const uint32_t ls = A.len[wa + min(h, nslots - 1u)];   // coalesced
+                    const uint32_t last = dlast[h];
+                    const float bound = fgss_min_overlap(lq, ls, t);
+                    const uint32_t js = last >> 16, ss = last & 0xFFFFu;
+                    const uint32_t rq = lq - js, rs = ls - ss;
+                    const uint32_t minrem = rq < rs ? rq : rs;
+                    const bool pass = has && !((float)(m + minrem) < bound);   // filtering.cu:127-133
+                    cand += (uint32_t)__popcll(__ballot(pass));
+                    const bool keep = pass && (int)(m + minrem - 1) >= (int)floorf(bound);
+                    // the others lose their last match and their half of the count word
+                    // (the even lane of each pair rewrites the word)
+                    *(has && !keep ? &dlast[h] : &qstage[lane]) = 0u;
+                    const uint32_t keepnb = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)keep, 0xB1, 0xF, 0xF, false);
+                    const uint32_t neww = (keep ? cw & 0xFFFFu : 0u) | (keepnb ? cw & 0xFFFF0000u : 0u);
+                    *((h & 1u) == 0 && cw != 0 ? &dcnt[h >> 1] : &qstage[lane]) = neww;
+                    const uint64_t bal = __ballot(keep);
+                    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                    *(keep ? &clist[nl + below] : &qstage[lane]) = h | (ls << 16);
+                    nl += (uint32_t)__popcll(bal);
+                }
+                wave_sync();
+                PHASE_MARK(2);
+                nsig += nl;
+                const uint32_t nv = phase_c(nl);
+                PHASE_MARK(3);
+                nver += nv;
+                phase_d(nv);
+                PHASE_MARK(4);
+            }
         }
-        wave_sync();
-        for (uint32_t c = lane; c < nver; c += kWave) {
-            const uint32_t h = clist[c] & 0xFFFFu;
-            tkey[h] = kEmpty; tcnt[h] = 0; tlast[h] = 0;
-        }
-        wave_sync();
-        PHASE_MARK(4);
         WACC(W_JOB0 + 0, escan);
         WACC(W_JOB0 + 1, cand);
         WACC(W_JOB0 + 2, inter);
@@ -1590,9 +1738,17 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         WACC(W_VER, nver);
         WACC(W_INTER, inter);
         WACC(W_VTOK, vtok);
+        if (dense) {
+            WACC(W_DENSE, 1);
+            continue;
+        }
         if (nlist * 100 <= tsize * kSpanGrowPct) span = min(span * 2, kMaxSpan);
         else if (nlist * 100 > tsize * kSpanShrinkPct) span = max(span / 2, 1u);
         }   // windows
+        if (dense) {   // every dense slot is zero again: the hash table's empty keys back
+            for (uint32_t i = lane; i < CAP; i += kWave) tkey[i] = kEmpty;
+            wave_sync();
+        }
     }
     // this wavefront's totals and its last job's pending counters to its own slot:
     // one coalesced store, no atomics (thousands of wavefronts adding to the same
@@ -1612,7 +1768,7 @@ __global__ __launch_bounds__(256) void k_wave_reduce(const unsigned long long *_
                                                      unsigned long long *__restrict__ ctr_job)
 {
     constexpr int kMain = W_JOB0;   // global counters per slot
-    const int dst[kMain] = {C_ESCAN, C_CAND, C_SIGCHK, C_VERIFIED, C_SIG2, C_INTER, C_VTOK, C_RETRY,
+    const int dst[kMain] = {C_ESCAN, C_CAND, C_SIGCHK, C_VERIFIED, C_SIG2, C_INTER, C_VTOK, C_RETRY, C_DENSE,
                             C_CYC_SETUP, C_CYC_SETUP + 1, C_CYC_SETUP + 2, C_CYC_SETUP + 3, C_CYC_SETUP + 4,
                             C_CYC_RANGES};
     const uint32_t wv = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1714,12 +1870,17 @@ static uint64_t misc_nb(const fgss_device_sets *d)
 #ifndef FGSS_LOAD_MAX
 #define FGSS_LOAD_MAX 100
 #endif
+#ifndef FGSS_DENSE_PCT
+#define FGSS_DENSE_PCT 25
+#endif
 struct RunCfg {
     int cap = 512;          // LDS table slots per wave
     uint32_t target = 332;  // postings per work item
     uint32_t heavy_target = 409;   // postings per work item of a query of more than kHeavyParts parts
     uint32_t bounds = 1u << 28;   // boundary-table entries at most
     uint32_t stream_parts = kStreamParts;   // queries split into more parts are streamed
+    uint32_t dense_w = 1024;      // sources per dense part (2 x cap; 0: no dense parts)
+    uint32_t dense_pct = FGSS_DENSE_PCT;   // dense parts for windows whose postings fill them this full (%)
 };
 
 static RunCfg make_cfg(const fgss_join_opts *opts)
@@ -1734,6 +1895,9 @@ static RunCfg make_cfg(const fgss_join_opts *opts)
     c.heavy_target = opts && opts->load_pct > 0 ? c.target : std::max<uint32_t>(1u, (uint32_t)(c.cap * 80 / 100));
     if (opts && opts->bounds_cap > 0) c.bounds = std::min<uint32_t>((uint32_t)opts->bounds_cap, c.bounds);
     if (opts && opts->stream_parts > 0) c.stream_parts = (uint32_t)opts->stream_parts;
+    // dense parts need the list room left after the cursor records: 512 table slots or more
+    c.dense_w = c.cap >= 512 && !(opts && opts->dense_pct < 0) ? 2u * (uint32_t)c.cap : 0u;
+    if (opts && opts->dense_pct > 0) c.dense_pct = (uint32_t)std::min(opts->dense_pct, 1000);
     return c;
 }
 
@@ -1913,7 +2077,8 @@ static int enqueue_stage_plan(fgss_device_sets *d, const RunCfg &cfg)
     // 6-8: plan work items
     hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(grid_for(n * kPlanGroup, 256), 4096)), dim3(256), 0, st,
                        d->tok, d->pos, d->len, n, d->job, d->tj, d->jend, probelen, poff, loff, psid, fence, cfg.target,
-                       cfg.heavy_target, P<uint2>(d->b_rng), hi, nparts, eq, bcnt, cfg.stream_parts);
+                       cfg.heavy_target, P<uint2>(d->b_rng), hi, nparts, eq, bcnt, cfg.stream_parts, cfg.dense_w,
+                       cfg.dense_pct);
     HIPCK(hipGetLastError());
     const uint32_t blim = std::min(d->cap_bounds, cfg.bounds);
     uint32_t *nitems = P<uint32_t>(d->b_nitems);
@@ -1927,7 +2092,7 @@ static int enqueue_stage_plan(fgss_device_sets *d, const RunCfg &cfg)
     HIPCK(hipGetLastError());
     hipLaunchKernelGGL(k_items, dim3(kItemsGrid), dim3(256), 0, st, 0, (const uint32_t *)nullptr, hi, nparts,
                        probelen, eq, partoff, bofs, bcnt, blim, d->job, d->tj,
-                       d->len, d->pos, poff, n, misc + M_NITEMS, P<Item>(d->b_items), d->cap_items);
+                       d->len, d->pos, poff, n, misc + M_NITEMS, P<Item>(d->b_items), d->cap_items, cfg.dense_w);
     HIPCK(hipGetLastError());
     return FGSS_OK;
 }
@@ -2011,10 +2176,11 @@ struct RunTotals {
 // (a reallocation of any buffer bumps cap_gen): the configuration, the capacities
 // the kernels test against and the record counts.  A replay is valid while it is
 // unchanged; a capacity can grow inside a buffer's slack without a reallocation.
-static std::array<int64_t, 14> graph_key_of(const fgss_device_sets *d, const RunCfg &cfg)
+static std::array<int64_t, 16> graph_key_of(const fgss_device_sets *d, const RunCfg &cfg)
 {
     return {cfg.cap, cfg.target, cfg.heavy_target, cfg.bounds, cfg.stream_parts, d->cap_gen,
-            d->cap_items, d->cap_out, d->cap_ovf, d->cap_bounds, d->n, d->njobs, d->q_begin, d->q_end};
+            d->cap_items, d->cap_out, d->cap_ovf, d->cap_bounds, d->n, d->njobs, d->q_begin, d->q_end,
+            cfg.dense_w, cfg.dense_pct};
 }
 
 // For the length of one call: the record set's work on the caller's stream (opts->stream)
@@ -2166,7 +2332,7 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
         if (hipEventElapsedTime(&ms, d->ev[1], d->ev[5]) == hipSuccess) T.ms_join = ms;
         if (hipEventElapsedTime(&ms, d->ev[4], d->ev[5]) == hipSuccess) T.ms_kernel = ms;
     }
-    for (int k = C_ESCAN; k <= C_RETRY; k++) T.acc[k] += d->h_ctr[k];
+    for (int k = C_ESCAN; k <= C_DENSE; k++) T.acc[k] += d->h_ctr[k];
     uint64_t nres = d->h_ctr[C_RESULTS], novf = d->h_ctr[C_OVERFLOW];
     tri.resize(3 * (size_t)nres);
     if (nres && nres <= d->out_copied) memcpy(tri.data(), d->h_out, (size_t)nres * 12);
@@ -2233,7 +2399,7 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
             if (rc) return rc;
             d->cap_gen++;
         }
-        for (int k = C_ESCAN; k <= C_RETRY; k++) T.acc[k] += d->h_ctr[k];
+        for (int k = C_ESCAN; k <= C_DENSE; k++) T.acc[k] += d->h_ctr[k];
         nres = d->h_ctr[C_RESULTS];
         novf = d->h_ctr[C_OVERFLOW];
         const size_t old = tri.size();
@@ -2407,6 +2573,7 @@ static void fill_stats(const RunTotals &T, size_t job, fgss_join_stats &st)
     st.verified = (int64_t)T.acc[C_VERIFIED];
     for (int k = 0; k < 6; k++) st.phase_cycles[k] = (int64_t)T.acc[C_CYC_SETUP + k];
     st.window_retries = (int64_t)T.acc[C_RETRY];
+    st.dense_parts = (int64_t)T.acc[C_DENSE];
     st.ms_index = T.ms_index;
     st.ms_join = T.ms_join;
     st.ms_kernel = T.ms_kernel;
@@ -3033,7 +3200,7 @@ static int enqueue_rs(fgss_server *srv, ServeSlot &sl, bool timing)
     HIPCK(hipGetLastError());
     hipLaunchKernelGGL(k_items, dim3(kItemsGrid), dim3(256), 0, st, 1, wlo, hi, nparts, probelen, eq, partoff,
                        bofs, bcnt, blim, d->job, d->tj, d->len, d->pos, poff, v, misc + M_NITEMS, P<Item>(d->b_items),
-                       d->cap_items);
+                       d->cap_items, 0u);
     HIPCK(hipGetLastError());
     if (timing) HIPCK(hipEventRecord(d->ev[4], st));
     int rc = launch_join_cap(d, rs_args(srv, sl), cfg.cap, false, true);
@@ -3842,7 +4009,7 @@ extern "C" int fgss_selfjoin_noalloc(const int32_t *tokens, const int32_t *lengt
                          (unsigned long long)written);
     }
     uint64_t acc[C_COUNT] = {0};
-    for (int k = C_ESCAN; k <= C_RETRY; k++) acc[k] += h_ctr[k];
+    for (int k = C_ESCAN; k <= C_DENSE; k++) acc[k] += h_ctr[k];
 
     // 3. overflowed work items: split four ways and run again, in launches of at most
     //    kNoAllocSplit items; the items that overflow again wait in the other buffer
@@ -3883,7 +4050,7 @@ extern "C" int fgss_selfjoin_noalloc(const int32_t *tokens, const int32_t *lengt
                                  (long long)out_cap, (long long)*n_out);
             }
             written += h_ctr[C_RESULTS];
-            for (int k = C_ESCAN; k <= C_RETRY; k++) acc[k] += h_ctr[k];
+            for (int k = C_ESCAN; k <= C_DENSE; k++) acc[k] += h_ctr[k];
             const uint32_t o = (uint32_t)h_ctr[C_OVERFLOW];
             if ((uint64_t)nnext + o > kNoAllocPending)
                 return fgss_fail(FGSS_ERR_UNSUPPORTED, "overflowed work items exceed the no-allocation scratch's %u",
@@ -3914,6 +4081,7 @@ extern "C" int fgss_selfjoin_noalloc(const int32_t *tokens, const int32_t *lengt
         stats->verified = (int64_t)acc[C_VERIFIED];
         for (int k = 0; k < 6; k++) stats->phase_cycles[k] = (int64_t)acc[C_CYC_SETUP + k];
         stats->window_retries = (int64_t)acc[C_RETRY];
+        stats->dense_parts = (int64_t)acc[C_DENSE];
         stats->kernel_launches = launches;
         stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }

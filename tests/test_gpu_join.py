@@ -287,3 +287,24 @@ def test_resplit_round_larger_than_the_result_buffer(fg, oracle_mod):
     assert_join_equal(oracle_mod, sets, p, 0.5, counters=want)
     _say("resplit retry: the same collection again (buffers now sized)")
     assert_join_equal(oracle_mod, sets, coll.selfjoin(0.5, table_slots=256), 0.5, counters=want)
+
+
+@pytest.mark.parametrize("dense_pct", [1, 0, -1])
+def test_dense_parts(fg, oracle_mod, dense_pct):
+    """Windows joined in dense parts (a table indexed by the source, 2 x table-slots
+    sources a part) give the oracle's pairs and counters, as hash-table parts do: dense
+    parts forced wherever a window spans two parts (1), the default rule (0), none (-1)."""
+    cases = [(fg.Collection.generate(fg.GEN_LONG, 20000, 13), (0.5, 0.8), (512, 1024)),
+             (fg.Collection.generate(fg.GEN_TITLES, 30000, 14), (0.5, 0.7), (512,))]
+    for coll, ts, slots in cases:
+        sets = oracle_sets(oracle_mod, coll)
+        for t in ts:
+            want = oracle_mod.selfjoin(sets, t)
+            for cap in slots:
+                _say(f"dense_pct={dense_pct} n={coll.info.n_sets} t={t} slots={cap}")
+                p = coll.selfjoin(t, dense_pct=dense_pct, table_slots=cap)
+                if dense_pct == 1:
+                    assert p.stats["dense_parts"] > 0
+                if dense_pct == -1:
+                    assert p.stats["dense_parts"] == 0
+                assert_join_equal(oracle_mod, sets, p, t, counters=want)
