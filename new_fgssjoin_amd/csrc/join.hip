@@ -84,6 +84,7 @@ enum Ctr {
     C_RESULTS,       // pairs emitted (may exceed the output capacity)
     C_OVERFLOW,      // work items whose LDS table overflowed
     C_QUEUE,         // work-queue head
+    C_QUEUE_D,       // work-queue head of the dense join kernel
     C_CYC_SETUP,     // shader-clock cycles summed over waves, per k_join phase
     C_CYC_A,
     C_CYC_B,
@@ -129,12 +130,36 @@ constexpr uint32_t kPadMask = 0x0FFFFFFFu;
 // counted in a table indexed by the source (no hashing); the count is that of dense parts
 constexpr uint32_t kDenseParts = 0x80000000u;
 #ifndef FGSS_DENSE_CHUNK
-#define FGSS_DENSE_CHUNK 32
+#define FGSS_DENSE_CHUNK 64
 #endif
 #ifndef FGSS_DENSE_LOGK_MIN
 #define FGSS_DENSE_LOGK_MIN 3
 #endif
-constexpr uint32_t kDenseChunk = FGSS_DENSE_CHUNK;   // dense parts per work item
+// dense match counts: 8 bits, four slots a word (1), or 16 bits, two slots a word (0);
+// list ends of a dense item's tokens read from LDS (1, needs 8-bit counts) or from the
+// posting ranges in global memory (0)
+#ifndef FGSS_DENSE_CNT8
+#define FGSS_DENSE_CNT8 1
+#endif
+#ifndef FGSS_DENSE_END_LDS
+#define FGSS_DENSE_END_LDS 1
+#endif
+constexpr uint32_t kDcShift = FGSS_DENSE_CNT8 ? 2u : 1u;   // log2 slots per count word
+constexpr uint32_t kDcSlots = 1u << kDcShift;
+constexpr uint32_t kDcBits = 32u >> kDcShift;
+constexpr uint32_t kDcMask = (1u << kDcBits) - 1u;
+constexpr bool kDenseEndLds = FGSS_DENSE_END_LDS && FGSS_DENSE_CNT8;
+constexpr uint32_t kDenseChunk = FGSS_DENSE_CHUNK;   // dense parts per work item at most
+constexpr uint64_t kDenseItems = 16384;              // dense work items wanted per launch (4x the waves of its role)
+
+// dense parts per work item: up to kDenseChunk (fewer items, less set-up), but small
+// enough that all dense parts of the run make kDenseItems items (a few heavy queries
+// then spread over the grid instead of running one long item per wave)
+__device__ __forceinline__ uint32_t dense_chunk(unsigned long long total_parts)
+{
+    const unsigned long long c = (total_parts + kDenseItems - 1) / kDenseItems;
+    return (uint32_t)(c < 1 ? 1 : c > kDenseChunk ? kDenseChunk : c);
+}
 // streamed items keep one posting cursor per probe token in the wavefront's LDS
 // (the range arrays of the other items)
 constexpr uint32_t kStreamQp = 256;
@@ -190,7 +215,7 @@ struct fgss_device_sets {
     float *tj = nullptr;        // per job: threshold of the current run
     DevBuf b_psid, b_fence, b_bcnt, b_bofs, b_bounds, b_probelen, b_poff, b_rng, b_eq, b_midlen, b_midoff, b_ka,
         b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_pl_of, b_post, b_loff, b_hi, b_nparts, b_partoff, b_scanscr, b_items,
-        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc, b_nitems, b_ovfp, b_wctr, b_jsave;
+        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc, b_nitems, b_ovfp, b_wctr, b_jsave, b_ditems, b_dnum, b_doff;
     uint32_t cap_items = 0, cap_bounds = 0, cap_out = 0;   // current capacities (elements)
     uint32_t cap_ovf = 0;         // overflow records (a streamed item can record one per part)
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -258,7 +283,7 @@ void fgss_device_release(fgss_collection *c)
                       &d->b_rng, &d->b_eq, &d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib,
                       &d->b_sortscr, &d->b_sid_of, &d->b_pl_of, &d->b_post, &d->b_loff, &d->b_hi, &d->b_nparts, &d->b_partoff,
                       &d->b_scanscr, &d->b_items, &d->b_items2, &d->b_ovf, &d->b_out, &d->b_ctr, &d->b_jctr,
-                      &d->b_nitems, &d->b_ovfp, &d->b_wctr, &d->b_jsave,
+                      &d->b_nitems, &d->b_ovfp, &d->b_wctr, &d->b_jsave, &d->b_ditems, &d->b_dnum, &d->b_doff,
                       &d->b_misc};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -636,8 +661,9 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
                                               uint2 *__restrict__ rng, uint32_t *__restrict__ hi_out,
                                               uint32_t *__restrict__ nparts, uint32_t *__restrict__ eq_out,
                                               uint64_t *__restrict__ bcnt, uint32_t stream_parts, uint32_t dense_w,
-                                              uint32_t dense_pct)
+                                              uint32_t dense_pct, unsigned long long *__restrict__ ndparts)
 {
+    uint32_t dparts = 0;   // dense parts of this lane's queries
     const int64_t ngroups = (int64_t)gridDim.x * (blockDim.x / kPlanGroup);
     const uint32_t gl = (uint32_t)lane_id() & (kPlanGroup - 1);   // lane within the group
     // every group of a wavefront runs the same number of iterations (the loop bound is
@@ -684,14 +710,19 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
             // parts of a few postings per probe token)
             const uint32_t win = wend - (uint32_t)q - 1u;
             const uint32_t ndp = dense_w ? (win + dense_w - 1u) / dense_w : 0u;
-            const bool dense = ndp > 1 && np > 1 && qp <= kStreamQp &&
+            const bool dense = ndp > 1 && np > 1 && qp < kStreamQp &&
                                (uint64_t)e * 100u >= (uint64_t)ndp * dense_w * dense_pct;
             nparts[q] = dense ? (ndp | kDenseParts) : np;
+            dparts += dense ? ndp : 0u;
             // internal boundaries x probe tokens, for queries whose parts read a planned
             // boundary table (heavier ones are streamed when their probe fits a cursor slot)
             bcnt[q] = !dense && np > 1 && (np <= stream_parts || qp > kStreamQp) ? (uint64_t)(np - 1) * qp : 0ull;
         }
     }
+    unsigned long long w = dparts;
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) w += __shfl_xor(w, d, kWave);
+    if (lane_id() == 0 && w) atomicAdd(ndparts, w);
 }
 
 // sid of the start of part p of a window [a0, a0 + W) split into np parts
@@ -759,16 +790,23 @@ __global__ __launch_bounds__(256) void k_bounds(int64_t n, const uint32_t *__res
 
 // Items per query (before the scan of their offsets): one streamed item, np planned
 // or searched parts, one whole-window item, or none.
+// dnum (null: no dense items): the dense items among them, whose scan places each
+// query's dense items in the list that the dense lanes of k_join walk.
 __global__ void k_nitems(const uint32_t *__restrict__ nparts, const uint32_t *__restrict__ probelen,
                          const uint64_t *__restrict__ bofs, const uint64_t *__restrict__ bcnt, uint32_t cap_bounds,
-                         int64_t n, uint32_t *__restrict__ nitems)
+                         int64_t n, uint32_t *__restrict__ nitems, uint32_t *__restrict__ dnum,
+                         const unsigned long long *__restrict__ ndparts)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    const uint32_t np = nparts[q];
-    const uint32_t m = item_mode(np, probelen[q], bcnt[q], bofs[q], cap_bounds);
-    nitems[q] = m == IM_NONE ? 0u : m == IM_WHOLE ? 1u : m == IM_STREAMED ? (np + kStreamChunk - 1) / kStreamChunk
-              : m == IM_DENSE ? ((np & ~kDenseParts) + kDenseChunk - 1) / kDenseChunk : np;
+    const uint32_t chunk = dense_chunk(*ndparts);
+    if (q < n) {
+        const uint32_t np = nparts[q];
+        const uint32_t m = item_mode(np, probelen[q], bcnt[q], bofs[q], cap_bounds);
+        const uint32_t ni = m == IM_NONE ? 0u : m == IM_WHOLE ? 1u : m == IM_STREAMED ? (np + kStreamChunk - 1) / kStreamChunk
+                          : m == IM_DENSE ? ((np & ~kDenseParts) + chunk - 1) / chunk : np;
+        nitems[q] = ni;
+        if (dnum) dnum[q] = m == IM_DENSE ? ni : 0u;
+    }
 }
 
 // wlo: window start per query (null: q + 1, the self-join window).  rs: the queries
@@ -787,9 +825,12 @@ __global__ __launch_bounds__(256) void k_items(int rs, const uint32_t *__restric
                                                const float *__restrict__ tj, const uint32_t *__restrict__ len,
                                                const uint32_t *__restrict__ pos, const uint32_t *__restrict__ poff,
                                                int64_t n, const uint32_t *__restrict__ n_items_dev,
-                                               Item *__restrict__ items, uint32_t cap_items, uint32_t dense_w)
+                                               Item *__restrict__ items, uint32_t cap_items, uint32_t dense_w,
+                                               const unsigned long long *__restrict__ ndparts,
+                                               const uint32_t *__restrict__ doff, uint32_t *__restrict__ ditems)
 {
     const uint32_t total = *n_items_dev;
+    const uint32_t chunk = dense_chunk(*ndparts);
     if (total > cap_items) return;   // too small: the host resizes and reruns
     for (uint32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < total; o += gridDim.x * blockDim.x) {
         int64_t lo = 0, top = n - 1;   // partoff[0] = 0 <= o
@@ -811,8 +852,8 @@ __global__ __launch_bounds__(256) void k_items(int rs, const uint32_t *__restric
                           : mode == IM_STREAMED ? kStreamed : mode == IM_DENSE ? kDense : 0u;
         Item it;
         it.q = (uint32_t)r;
-        if (mode == IM_DENSE) {      // chunk p: dense parts [p * kDenseChunk, ...) of dense_w sources each
-            const uint32_t p0 = p * kDenseChunk, p1 = min(p0 + kDenseChunk, npc);
+        if (mode == IM_DENSE) {      // chunk p: dense parts [p * chunk, ...) of dense_w sources each
+            const uint32_t p0 = p * chunk, p1 = min(p0 + chunk, npc);
             it.a = (uint32_t)(a0 + (uint64_t)p0 * dense_w);
             it.b = (uint32_t)min<uint64_t>(a0 + (uint64_t)p1 * dense_w, (uint64_t)hi[q]);
             it.nparts = p1 - p0;
@@ -835,6 +876,8 @@ __global__ __launch_bounds__(256) void k_items(int rs, const uint32_t *__restric
         it.qpos = pos[r];
         it.qpoff = poff[q];
         items[o] = it;
+        // the dense items' indices, listed apart (in item order) for the lanes that join them
+        if (mode == IM_DENSE) ditems[doff[q] + p] = o;
     }
 }
 
@@ -884,8 +927,11 @@ struct JoinArgs {
     const uint32_t *poff;       // per query: first entry of its probe tokens in rng
     const uint32_t *bounds;     // planned split boundaries (see k_bounds)
     const Item *items;
+    const uint32_t *ditems;     // indices of the dense items in items (any order; null: none)
     const uint32_t *n_items_dev;  // number of items (device); the launch is void if above n_items_cap
     uint32_t n_items_cap;
+    const uint32_t *n_dense_dev;  // dense items among them (device; null: none)
+    uint32_t dense_blocks;      // blocks of the launch that join the dense items (the first ones)
     const uint32_t *job;        // per record: job id
     const float *tj;            // per job: Jaccard threshold
     unsigned long long *ctr_job;  // per job: kJobCtr counters
@@ -977,8 +1023,12 @@ __device__ __forceinline__ bool table_insert(const uint2 (&pe)[N], uint32_t valm
 
 // The dense counterpart of table_insert, for a dense part of sources [wa, wa + 2 x CAP):
 // a source's slot is its offset from wa (no hashing or probing, so nothing overflows),
-// its match count is 16 bits, two slots to a word of dcnt, and dlast keeps its last
-// (query position, source position).  Lanes without work aim their updates at scr[lane].
+// its match count is 8 bits, four slots to a word of dcnt (probes of dense items hold at
+// most 255 tokens), and dlast keeps query position << 24 | source position << 16 | |S|:
+// both positions are below 256 (a source is no longer than its query, so its position
+// lies inside the query's probe length), and |S| is the same in every update of a slot,
+// so the maximum word holds the last match and the source length.  Lanes without work
+// aim their updates at scr[lane].
 template <int N>
 __device__ __forceinline__ void dense_insert(const uint2 (&pe)[N], uint32_t valm, const uint32_t (&jpos)[N], uint32_t wa,
                                              uint32_t *dlast, uint32_t *dcnt, uint32_t *scr, int lane)
@@ -987,8 +1037,8 @@ __device__ __forceinline__ void dense_insert(const uint2 (&pe)[N], uint32_t valm
     for (int k = 0; k < N; k++) {
         const bool v = (valm >> k) & 1u;
         const uint32_t h = pe[k].x - wa;
-        atomicAdd(v ? &dcnt[h >> 1] : &scr[lane], 1u << ((h & 1u) << 4));
-        atomicMax(v ? &dlast[h] : &scr[lane], (jpos[k] << 16) | (pe[k].y & 0xFFFFu));
+        atomicAdd(v ? &dcnt[h >> kDcShift] : &scr[lane], 1u << ((h & (kDcSlots - 1u)) * kDcBits));
+        atomicMax(v ? &dlast[h] : &scr[lane], (jpos[k] << 24) | ((pe[k].y & 0xFFu) << 16) | (pe[k].y >> 16));
     }
 }
 
@@ -1035,10 +1085,15 @@ constexpr uint32_t join_wave_words(int cap)
 }
 
 // RS: an R-S join against an index holding max_prefix(|S|, t_min) tokens per source;
-// postings past max_prefix(|S|, t) of the item's threshold are skipped.
-template <int CAP, bool RS>
-__global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
+// postings past max_prefix(|S|, t) of the item's threshold are skipped.  DENSE: the
+// blocks of this role join the dense items only, the others every other item (both walk
+// the one item list, each role with its own queue), so that the registers of each loop
+// hold one kind of item.  rblock / rblocks: the block's index among its role's blocks
+// and their number.
+template <int CAP, bool RS, bool DENSE>
+__device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, uint32_t rblocks)
 {
+    static_assert(!DENSE || (CAP >= 512 && !RS), "dense items: self-joins with 512 table slots or more");
     constexpr int LOG2CAP = CAP == 256 ? 8 : CAP == 512 ? 9 : CAP == 1024 ? 10 : CAP == 2048 ? 11 : 12;
     static_assert((1 << LOG2CAP) == CAP, "CAP must be a power of two");
     extern __shared__ uint32_t smem[];
@@ -1054,7 +1109,9 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
     // a launch whose item list or boundary table did not fit does nothing: the host
     // sees the totals, grows the buffers and runs the pipeline again
     if (*A.n_items_dev > A.n_items_cap) return;
-    const uint32_t n_items = *A.n_items_dev;
+    // the dense role walks the list of dense item indices, the other role every item
+    // (skipping the dense ones)
+    const uint32_t n_items = DENSE ? (A.n_dense_dev == nullptr || A.ditems == nullptr ? 0u : *A.n_dense_dev) : *A.n_items_dev;
 
     for (uint32_t i = lane; i < CAP; i += kWave) { tkey[i] = kEmpty; tcnt[i] = 0; tlast[i] = 0; }
     wave_sync();
@@ -1091,10 +1148,10 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
     // items (a small served batch) spreads over every wavefront without atomics; the
     // items past the static chunks are claimed kQueueBatch at a time, one global
     // atomic per claim (claims of fewer items contend on the counter).
-    const uint32_t nwaves_grid = gridDim.x * (blockDim.x / kWave);
+    const uint32_t nwaves_grid = rblocks * (blockDim.x / kWave);
     const uint32_t g0 = min((n_items + nwaves_grid - 1) / nwaves_grid, (uint32_t)kQueueBatch);
     const uint32_t qbase = nwaves_grid * g0;   // first item of the dynamic claims
-    const uint32_t gw = blockIdx.x * (blockDim.x / kWave) + (uint32_t)w;
+    const uint32_t gw = rblock * (blockDim.x / kWave) + (uint32_t)w;
     uint32_t batch_next = min(gw * g0, n_items), batch_end = min(gw * g0 + g0, n_items);
     uint32_t cur_job = kEmpty;
     // the next item of the claimed batch is loaded while the current one runs
@@ -1104,7 +1161,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         if (batch_next >= batch_end) {
             if (qbase >= n_items) break;   // every item was in a static chunk
             uint32_t b0 = 0;
-            if (lane == 0) b0 = atomicAdd((unsigned int *)&A.ctr[C_QUEUE], (unsigned)kQueueBatch);
+            if (lane == 0) b0 = atomicAdd((unsigned int *)&A.ctr[DENSE ? C_QUEUE_D : C_QUEUE], (unsigned)kQueueBatch);
             b0 = qbase + __builtin_amdgcn_readfirstlane(b0);
             batch_next = min(b0, n_items);
             batch_end = min(b0 + (uint32_t)kQueueBatch, n_items);
@@ -1114,9 +1171,9 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         const uint32_t it = batch_next++;
         // an item is kItemWords words, lane k holding word k (one register for the
         // prefetched item instead of kItemWords)
-        const uint32_t ldw = have_nxt ? nxtw : item_word(A.items, it, lane);
+        const uint32_t ldw = have_nxt ? nxtw : item_word(A.items, DENSE ? A.ditems[it] : it, lane);
         have_nxt = batch_next < batch_end;
-        if (have_nxt) nxtw = item_word(A.items, batch_next, lane);
+        if (have_nxt) nxtw = item_word(A.items, DENSE ? A.ditems[batch_next] : batch_next, lane);
         // the item is the same for every lane: its fields live in scalar registers
         Item item;
         item.q = __builtin_amdgcn_readlane(ldw, 0);
@@ -1131,6 +1188,7 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         item.lq = __builtin_amdgcn_readlane(ldw, 9);
         item.qpos = __builtin_amdgcn_readlane(ldw, 10);
         item.qpoff = __builtin_amdgcn_readlane(ldw, 11);
+        if (((item.pad & kDense) != 0) != DENSE) continue;   // the other kernel's item
         const uint32_t q = item.q;
         const uint32_t jq = item.job;
         const float t = item.t;
@@ -1170,20 +1228,25 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         static_assert(2 * kTokBatch + 1 >= (int)kStreamQp, "cursors live in rlo and rpre");
         // A dense item walks its nparts dense parts of DW consecutive sources in order,
         // with cursors as a streamed item.  Its table is indexed by the source's offset in
-        // the part: dlast (the key and count words of the hash table) and dcnt (16-bit
-        // counts, the last-match words); nsid (the last kStreamQp words of the list) holds
-        // the record id at each token's cursor, so that a part rounds only over the
-        // tokens with postings in it.  The hash table's empty state is restored at the end.
+        // the part: dlast (the key and count words of the hash table) and dcnt (8-bit
+        // counts, the first quarter of the last-match words); tend (the next quarter)
+        // holds each token's list end inside the window and nsid (the last kStreamQp words
+        // of the list) the record at its cursor, so that a part rounds only over the tokens
+        // with postings in it.  The hash table's empty state is restored at the end.
         constexpr uint32_t DW = 2u * CAP;
         constexpr uint32_t kDenseList = CAP > kStreamQp ? CAP - kStreamQp : 0u;   // candidates listed at once
-        const bool dense = CAP >= 512 && !RS && (item.pad & kDense) != 0;
-        uint32_t *const dlast = tkey, *const dcnt = tlast, *const nsid = clist + kDenseList;
+        static_assert(CAP < 512 || CAP / 2 >= kStreamQp, "list ends fit the last-match words");
+        constexpr bool dense = DENSE;
+        uint32_t *const dlast = tkey, *const dcnt = tlast, *const tend = tlast + CAP / 2, *const nsid = clist + kDenseList;
         if (streamed || dense) {
             for (uint32_t j = lane; j < qp; j += kWave) {
                 const uint2 r = qrng[j];
                 const uint32_t c = item.part == 0 ? r.x : lower_bound_sid(A.psid, A.fence, r.x, r.y, item.a);
                 cur[j] = c;
-                if (dense) nsid[j] = c < r.y ? A.psid[c] : kEmpty;
+                if (dense) {
+                    nsid[j] = c < r.y ? A.psid[c] : kEmpty;
+                    if (kDenseEndLds) tend[j] = r.y;
+                }
             }
             if (dense)
                 for (uint32_t i = lane; i < DW; i += kWave) dlast[i] = 0u;
@@ -1337,109 +1400,115 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
             // next K of them from its cursor (independent loads, no search).  Those
             // before wb are this part's; the group of K lanes of a token moves its
             // cursor past them, and a token whose K were all before wb takes K more
-            // (K doubling up to 64) in the next round.  A dense part takes only the
-            // tokens whose cursor record lies before wb.
-            uint8_t *toks = (uint8_t *)qstage;          // active token ids (bytes)
-            uint32_t *satm = qstage + kStreamQp / 4;    // kStreamQp bits: tokens with postings left
+            // (K doubling up to 64) in the next round.
+            uint8_t *toks = (uint8_t *)qstage;          // tokens of a round (bytes)
+            uint32_t *satm = qstage + kStreamQp / 4;    // kStreamQp bits: tokens going on to the next round
             uint32_t *sscr = satm + kStreamQp / 32;     // a word per lane for updates without work
             static_assert(kStreamQp / 4 + kStreamQp / 32 + kWave <= kQStage, "token list fits the query stage");
-            uint32_t nact = 0;
-            for (uint32_t j0 = 0; j0 < qp; j0 += kWave) {
-                const uint32_t j = j0 + lane;
-                const bool act = j < qp && (dense ? nsid[j] < wb : cur[j] < qrng[j].y);
-                const uint64_t m = __ballot(act);
-                if (act) toks[nact + (uint32_t)__popcll(m & lanemask_lt())] = (uint8_t)j;
-                nact += (uint32_t)__popcll(m);
-            }
-            wave_sync();
-            // first round: K = 8 lanes per token.  A round waits for the cursor moves of
-            // the one before it, so fewer rounds beat fewer idle lanes up to a point
-            // (1M long records at t = 0.8, about 2.4 postings per token and part:
-            // K = 4 2.11 s, 8 2.02 s, 16 2.51 s; starting at 1 2.59 s).  A dense part
-            // starts at the power of two at or above its expected postings per active token.
-            uint32_t logK = 3;
-            if (dense && nact > 0) {
-                const uint32_t per_tok = ((item.pad & kPadMask) + nact - 1) / nact;
-                logK = (uint32_t)min(6, max(FGSS_DENSE_LOGK_MIN, ceil_log2(per_tok)));
-            }
-            while (nact > 0 && !overflow) {
-                if (lane < (int)(kStreamQp / 32)) satm[lane] = 0;
-                wave_sync();
-                const uint32_t K = 1u << logK, total = nact << logK;
-                // rows per step as in the flattened loop (a round's total is nact x K)
-                auto sstep = [&](auto rows, uint32_t e0) -> bool {
-                    constexpr int N = decltype(rows)::value;
-                    // every lane reads a posting (lanes past the end the last group's;
-                    // reads past a token's list land in the next list or the spare
-                    // postings) together with its token's list end, and then tests it
-                    uint2 pe[N];
-                    uint32_t jv[N], curj[N], idx[N], end[N];
+            // one pass over the tokens toks[0, total / K), K = 1 << logK lanes each
+            uint32_t logK = 3, K = 8, total = 0;
+            // rows per step as in the flattened loop
+            auto sstep = [&](auto rows, uint32_t e0) -> bool {
+                constexpr int N = decltype(rows)::value;
+                // every lane reads a posting (lanes past the end the last group's;
+                // reads past a token's list land in the next list or the spare
+                // postings) together with its token's list end, and then tests it
+                uint2 pe[N];
+                uint32_t jv[N], curj[N], idx[N], end[N];
 #pragma unroll
-                    for (int k = 0; k < N; k++) {
-                        const uint32_t ec = min(e0 + (uint32_t)(k * kWave) + lane, total - 1u);
-                        jv[k] = toks[ec >> logK];
-                        curj[k] = cur[jv[k]];
-                        idx[k] = curj[k] + (ec & (K - 1));
-                        pe[k] = A.post[idx[k]];
-                        end[k] = qrng[jv[k]].y;
-                    }
-                    uint32_t valm = 0;
+                for (int k = 0; k < N; k++) {
+                    const uint32_t ec = min(e0 + (uint32_t)(k * kWave) + lane, total - 1u);
+                    jv[k] = toks[ec >> logK];
+                    curj[k] = cur[jv[k]];
+                    idx[k] = curj[k] + (ec & (K - 1));
+                    pe[k] = A.post[idx[k]];
+                    end[k] = dense && kDenseEndLds ? tend[jv[k]] : qrng[jv[k]].y;
+                }
+                uint32_t valm = 0;
 #pragma unroll
-                    for (int k = 0; k < N; k++) {
-                        const uint32_t e = e0 + (uint32_t)(k * kWave) + lane;
-                        const bool below = e < total && idx[k] < end[k] && pe[k].x < wb;
-                        // below is a prefix of each token's K lanes (postings ascend by
-                        // record): its length is the count, taken by the group's first lane
-                        const uint64_t bm = __ballot(below);
-                        escan += (uint32_t)__popcll(bm);
-                        const uint32_t gs = (uint32_t)lane & ~(K - 1);
-                        const uint64_t gmask = K >= 64 ? ~0ull : ((1ull << K) - 1ull);
-                        const uint32_t cnt = (uint32_t)__popcll((bm >> gs) & gmask);
-                        // each token has one group per round, in one row k: the groups
-                        // of a row move different cursors
-                        const bool lead = e < total && ((uint32_t)lane & (K - 1)) == 0 && cnt > 0;
-                        *(lead ? &cur[jv[k]] : &sscr[lane]) = curj[k] + cnt;
-                        atomicOr(lead && cnt == K ? &satm[jv[k] >> 5] : &sscr[lane], 1u << (jv[k] & 31));
-                        // dense: the first lane of the group past the part holds the record at
-                        // the token's new cursor (none when the list ended there)
-                        const bool next = dense && e < total && ((uint32_t)lane & (K - 1)) == cnt;
-                        *(next ? &nsid[jv[k]] : &sscr[lane]) = idx[k] < end[k] ? pe[k].x : kEmpty;
-                        uint32_t v = below;
-                        if (RS) {   // the source's own prefix at t
-                            const uint32_t ls = pe[k].y >> 16;
-                            v &= (uint32_t)((pe[k].y & 0xFFFFu) < (midprobe ? fgss_max_prefix(ls, t) : fgss_mid_prefix(ls, t)));
-                        }
-                        valm |= v << k;
+                for (int k = 0; k < N; k++) {
+                    const uint32_t e = e0 + (uint32_t)(k * kWave) + lane;
+                    const bool below = e < total && idx[k] < end[k] && pe[k].x < wb;
+                    // below is a prefix of each token's K lanes (postings ascend by
+                    // record): its length is the count, taken by the group's first lane
+                    const uint64_t bm = __ballot(below);
+                    escan += (uint32_t)__popcll(bm);
+                    const uint32_t gs = (uint32_t)lane & ~(K - 1);
+                    const uint64_t gmask = K >= 64 ? ~0ull : ((1ull << K) - 1ull);
+                    const uint32_t cnt = (uint32_t)__popcll((bm >> gs) & gmask);
+                    // each token has one group per pass, in one row k: the groups of a
+                    // row move different cursors
+                    const bool lead = e < total && ((uint32_t)lane & (K - 1)) == 0 && cnt > 0;
+                    *(lead ? &cur[jv[k]] : &sscr[lane]) = curj[k] + cnt;
+                    atomicOr(lead && cnt == K ? &satm[jv[k] >> 5] : &sscr[lane], 1u << (jv[k] & 31));
+                    // dense: the first lane of the group past the part holds the record at
+                    // the token's new cursor (none when the list ended there)
+                    const bool next = dense && e < total && ((uint32_t)lane & (K - 1)) == cnt;
+                    *(next ? &nsid[jv[k]] : &sscr[lane]) = idx[k] < end[k] ? pe[k].x : kEmpty;
+                    uint32_t v = below;
+                    if (RS) {   // the source's own prefix at t
+                        const uint32_t ls = pe[k].y >> 16;
+                        v &= (uint32_t)((pe[k].y & 0xFFFFu) < (midprobe ? fgss_max_prefix(ls, t) : fgss_mid_prefix(ls, t)));
                     }
-                    if (dense) {
-                        dense_insert(pe, valm, jv, wa, dlast, dcnt, sscr, lane);
-                        return false;
-                    }
-                    return table_insert(pe, valm, jv, tkey, tcnt, tlast, clist, sscr, lane, tlog, tsize, nlist);
-                };
+                    valm |= v << k;
+                }
+                if (dense) {
+                    dense_insert(pe, valm, jv, wa, dlast, dcnt, sscr, lane);
+                    return false;
+                }
+                return table_insert(pe, valm, jv, tkey, tcnt, tlast, clist, sscr, lane, tlog, tsize, nlist);
+            };
+            auto pass = [&]() -> bool {   // true when the hash table filled up
+#ifdef FGSS_DEBUG_FILL
+                if (dense) WACC(W_RETRY, total);
+#endif
                 for (uint32_t e0 = 0; e0 < total;) {
                     const uint32_t rem = total - e0;   // wave-uniform
                     bool full;
                     if (rem > 3u * kWave) { full = sstep(std::integral_constant<int, kUnroll>(), e0); e0 += kWave * kUnroll; }
                     else if (rem > (uint32_t)kWave) { full = sstep(std::integral_constant<int, 2>(), e0); e0 += 2 * kWave; }
                     else { full = sstep(std::integral_constant<int, 1>(), e0); e0 += kWave; }
-                    if (full) {
-                        overflow = true;
-                        break;
-                    }
+                    if (full) return true;
                 }
-                wave_sync();
-                nact = 0;
+                return false;
+            };
+            {
+                uint32_t nact = 0;
                 for (uint32_t j0 = 0; j0 < qp; j0 += kWave) {
                     const uint32_t j = j0 + lane;
-                    const bool act = j < qp && ((satm[j >> 5] >> (j & 31)) & 1u);
+                    const bool act = j < qp && (dense ? nsid[j] < wb : cur[j] < qrng[j].y);
                     const uint64_t m = __ballot(act);
-                    wave_sync();
                     if (act) toks[nact + (uint32_t)__popcll(m & lanemask_lt())] = (uint8_t)j;
                     nact += (uint32_t)__popcll(m);
                 }
                 wave_sync();
-                if (logK < 6) logK++;
+                // first round: K = 8 lanes per token.  A round waits for the cursor moves of
+                // the one before it, so fewer rounds beat fewer idle lanes up to a point
+                // (1M long records at t = 0.8, about 2.4 postings per token and part:
+                // K = 4 2.11 s, 8 2.02 s, 16 2.51 s; starting at 1 2.59 s).  A dense part
+                // starts at the power of two at or above its expected postings per token.
+                logK = 3;
+                if (dense && nact > 0)
+                    logK = (uint32_t)min(6, max(3, ceil_log2(((item.pad & kPadMask) + nact - 1) / nact)));
+                while (nact > 0 && !overflow) {
+                    if (lane < (int)(kStreamQp / 32)) satm[lane] = 0;
+                    wave_sync();
+                    K = 1u << logK;
+                    total = nact << logK;
+                    overflow = pass();
+                    wave_sync();
+                    nact = 0;
+                    for (uint32_t j0 = 0; j0 < qp; j0 += kWave) {
+                        const uint32_t j = j0 + lane;
+                        const bool act = j < qp && ((satm[j >> 5] >> (j & 31)) & 1u);
+                        const uint64_t m = __ballot(act);
+                        wave_sync();
+                        if (act) toks[nact + (uint32_t)__popcll(m & lanemask_lt())] = (uint8_t)j;
+                        nact += (uint32_t)__popcll(m);
+                    }
+                    wave_sync();
+                    if (logK < 6) logK++;
+                }
             }
         }
         if (overflow) {
@@ -1481,15 +1550,19 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         // dense slot (the source is wa + slot); the branches on `dense` are wave-uniform
         auto src_of = [&](uint32_t h) -> uint32_t { return dense ? wa + h : tkey[h]; };
         auto cnt_of = [&](uint32_t h) -> uint32_t {
-            return dense ? (dcnt[h >> 1] >> ((h & 1u) << 4)) & 0xFFFFu : tcnt[h];
+            return dense ? (dcnt[h >> kDcShift] >> ((h & (kDcSlots - 1u)) * kDcBits)) & kDcMask : tcnt[h];
         };
-        auto last_of = [&](uint32_t h) -> uint32_t { return dense ? dlast[h] : tlast[h]; };
+        auto last_of = [&](uint32_t h) -> uint32_t {   // query position << 16 | source position
+            if (!dense) return tlast[h];
+            const uint32_t w = dlast[h];
+            return (w >> 24) << 16 | ((w >> 16) & 0xFFu);
+        };
         // empties slot h when drop is set (the other lanes write a word of their own in the
         // idle query stage: no branch on a per-lane condition)
         auto drop_slot = [&](bool drop, uint32_t h) {
             if (dense) {
                 *(drop ? &dlast[h] : &qstage[lane]) = 0u;
-                atomicAnd(drop ? &dcnt[h >> 1] : &qstage[lane], (h & 1u) ? 0x0000FFFFu : 0xFFFF0000u);
+                atomicAnd(drop ? &dcnt[h >> kDcShift] : &qstage[lane], ~(kDcMask << ((h & (kDcSlots - 1u)) * kDcBits)));
             } else {
                 *(drop ? &tkey[h] : &qstage[lane]) = kEmpty;
                 *(drop ? &tcnt[h] : &qstage[lane]) = 0;
@@ -1694,26 +1767,27 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
                 for (; slot0 < nslots && nl + (uint32_t)kWave <= kDenseList; slot0 += kWave) {
                     const uint32_t h = slot0 + lane;
                     const bool in = h < nslots;
-                    const uint32_t cw = dcnt[h >> 1];
-                    const uint32_t m = in ? (cw >> ((h & 1u) << 4)) & 0xFFFFu : 0u;
+                    const uint32_t cw = dcnt[h >> kDcShift];
+                    const uint32_t m = in ? (cw >> ((h & (kDcSlots - 1u)) * kDcBits)) & kDcMask : 0u;
                     const bool has = m != 0;
                     if (__ballot(has) == 0) continue;
-                    const uint32_t ls = A.len[wa + min(h, nslots - 1u)];   // coalesced
-                    const uint32_t last = dlast[h];
+                    const uint32_t w = dlast[h];
+                    const uint32_t ls = w & 0xFFFFu, js = w >> 24, ss = (w >> 16) & 0xFFu;
                     const float bound = fgss_min_overlap(lq, ls, t);
-                    const uint32_t js = last >> 16, ss = last & 0xFFFFu;
                     const uint32_t rq = lq - js, rs = ls - ss;
                     const uint32_t minrem = rq < rs ? rq : rs;
                     const bool pass = has && !((float)(m + minrem) < bound);   // filtering.cu:127-133
                     cand += (uint32_t)__popcll(__ballot(pass));
                     const bool keep = pass && (int)(m + minrem - 1) >= (int)floorf(bound);
-                    // the others lose their last match and their half of the count word
-                    // (the even lane of each pair rewrites the word)
+                    // the others lose their last match and their part of the count word
+                    // (the first lane of the word's slots rewrites it)
                     *(has && !keep ? &dlast[h] : &qstage[lane]) = 0u;
-                    const uint32_t keepnb = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)keep, 0xB1, 0xF, 0xF, false);
-                    const uint32_t neww = (keep ? cw & 0xFFFFu : 0u) | (keepnb ? cw & 0xFFFF0000u : 0u);
-                    *((h & 1u) == 0 && cw != 0 ? &dcnt[h >> 1] : &qstage[lane]) = neww;
                     const uint64_t bal = __ballot(keep);
+                    const uint32_t qb = (uint32_t)(bal >> ((uint32_t)lane & ~(kDcSlots - 1u))) & ((1u << kDcSlots) - 1u);
+                    uint32_t keepbits = 0;
+#pragma unroll
+                    for (uint32_t i = 0; i < kDcSlots; i++) keepbits |= (qb >> i) & 1u ? kDcMask << (i * kDcBits) : 0u;
+                    *((h & (kDcSlots - 1u)) == 0 && cw != 0 ? &dcnt[h >> kDcShift] : &qstage[lane]) = cw & keepbits;
                     const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
                     *(keep ? &clist[nl + below] : &qstage[lane]) = h | (ls << 16);
                     nl += (uint32_t)__popcll(bal);
@@ -1745,8 +1819,11 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
         if (nlist * 100 <= tsize * kSpanGrowPct) span = min(span * 2, kMaxSpan);
         else if (nlist * 100 > tsize * kSpanShrinkPct) span = max(span / 2, 1u);
         }   // windows
-        if (dense) {   // every dense slot is zero again: the hash table's empty keys back
+        if (dense) {   // every dense slot is zero again: the hash table's empty keys and its
+                       // last-match words under the list ends back
             for (uint32_t i = lane; i < CAP; i += kWave) tkey[i] = kEmpty;
+            if (kDenseEndLds)
+                for (uint32_t i = lane; i < CAP / 2; i += kWave) tend[i] = 0u;
             wave_sync();
         }
     }
@@ -1757,6 +1834,21 @@ __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
     unsigned long long *slot = A.wctr + ((size_t)blockIdx.x * join_waves(CAP) + (size_t)w) * kWaveSlot;
     if (lane < kAccSlots) slot[lane] = wv;
     if (lane == kAccSlots) slot[lane] = cur_job;
+}
+
+// One launch, two roles: the first A.dense_blocks blocks join the dense items (they are
+// dispatched first, and when there are few dense items their idle waves leave at once
+// and free their CUs for the others), the remaining blocks every other item.
+template <int CAP, bool RS>
+__global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
+{
+    if constexpr (CAP >= 512 && !RS) {
+        if (blockIdx.x < A.dense_blocks) {
+            join_items<CAP, RS, true>(A, blockIdx.x, A.dense_blocks);
+            return;
+        }
+    }
+    join_items<CAP, RS, false>(A, blockIdx.x - A.dense_blocks, gridDim.x - A.dense_blocks);
 }
 
 // Sum the per-wavefront slots of a k_join launch into the global counters and the
@@ -1807,8 +1899,10 @@ static size_t join_smem_bytes()
 }
 
 // persistent grid: as many blocks as fit on the device, each wave loops over items
+// persistent grid: as many blocks as fit on the device, each wave loops over items;
+// with dense items possible, twice that: the dense role's blocks, then the other's
 template <int CAP, bool RS>
-static int launch_join(fgss_device_sets *d, const JoinArgs &A, bool dry = false)
+static int launch_join(fgss_device_sets *d, const JoinArgs &A0, bool dry, bool with_dense)
 {
     static thread_local int cached[2] = {-1, -1};   // blocks, device
     const size_t smem = join_smem_bytes<CAP>();
@@ -1821,44 +1915,50 @@ static int launch_join(fgss_device_sets *d, const JoinArgs &A, bool dry = false)
         cached[1] = d->device;
     }
     if (dry) return FGSS_OK;
-    if ((uint64_t)cached[0] * join_waves(CAP) * kWaveSlot * 8 > d->b_wctr.cap)
+    JoinArgs A = A0;
+    A.dense_blocks = with_dense && CAP >= 512 && !RS && A.n_dense_dev != nullptr ? (uint32_t)cached[0] : 0u;
+    const uint32_t blocks = (uint32_t)cached[0] + A.dense_blocks;
+    const uint32_t nw = blocks * join_waves(CAP);
+    if ((uint64_t)nw * kWaveSlot * 8 > d->b_wctr.cap)
         return fgss_fail(FGSS_ERR_DEVICE, "internal: wavefront counter slots too small");
-    hipLaunchKernelGGL((k_join<CAP, RS>), dim3(cached[0]), dim3(join_waves(CAP) * kWave), smem, d->st, A);
+    hipLaunchKernelGGL((k_join<CAP, RS>), dim3(blocks), dim3(join_waves(CAP) * kWave), smem, d->st, A);
     HIPCK(hipGetLastError());
-    const uint32_t nw = (uint32_t)(cached[0] * join_waves(CAP));
     hipLaunchKernelGGL(k_wave_reduce, dim3((nw + 255) / 256), dim3(256), 0, d->st, (const unsigned long long *)A.wctr,
                        nw, A.ctr, A.ctr_job);
     HIPCK(hipGetLastError());
     return FGSS_OK;
 }
 
-static int launch_join_cap(fgss_device_sets *d, const JoinArgs &A, int cap, bool dry = false, bool rs = false)
+// with_dense: the launch's items may be dense (re-split items never are)
+static int launch_join_cap(fgss_device_sets *d, const JoinArgs &A, int cap, bool dry = false, bool rs = false,
+                           bool with_dense = true)
 {
     if (rs) {
         switch (cap) {
-        case 256: return launch_join<256, true>(d, A, dry);
-        case 512: return launch_join<512, true>(d, A, dry);
-        case 1024: return launch_join<1024, true>(d, A, dry);
-        case 2048: return launch_join<2048, true>(d, A, dry);
-        case 4096: return launch_join<4096, true>(d, A, dry);
+        case 256: return launch_join<256, true>(d, A, dry, false);
+        case 512: return launch_join<512, true>(d, A, dry, false);
+        case 1024: return launch_join<1024, true>(d, A, dry, false);
+        case 2048: return launch_join<2048, true>(d, A, dry, false);
+        case 4096: return launch_join<4096, true>(d, A, dry, false);
         default: break;
         }
     }
     switch (cap) {
-    case 256: return launch_join<256, false>(d, A, dry);
-    case 512: return launch_join<512, false>(d, A, dry);
-    case 1024: return launch_join<1024, false>(d, A, dry);
-    case 2048: return launch_join<2048, false>(d, A, dry);
-    case 4096: return launch_join<4096, false>(d, A, dry);
+    case 256: return launch_join<256, false>(d, A, dry, with_dense);
+    case 512: return launch_join<512, false>(d, A, dry, with_dense);
+    case 1024: return launch_join<1024, false>(d, A, dry, with_dense);
+    case 2048: return launch_join<2048, false>(d, A, dry, with_dense);
+    case 4096: return launch_join<4096, false>(d, A, dry, with_dense);
     default: return fgss_fail(FGSS_ERR_ARG, "table_slots must be 256, 512, 1024, 2048 or 4096");
     }
 }
 
 static inline unsigned grid_for(int64_t n, int threads) { return (unsigned)std::max<int64_t>((n + threads - 1) / threads, 1); }
 
-// device totals written by the scans (indices into b_misc)
+// device totals written by the scans (indices into b_misc); M_NDP64 (two words) = dense
+// parts of the run
 // device-side counts; M_NB64 (two words, 8-byte aligned) = boundary-table rows wanted
-enum Misc { M_NE = 0, M_NITEMS, M_NP, M_NRESPLIT, M_NB64 = 6, M_COUNT = 16 };
+enum Misc { M_NE = 0, M_NITEMS, M_NP, M_NRESPLIT, M_NDENSE, M_NB64 = 6, M_NDP64 = 8, M_COUNT = 16 };
 constexpr uint32_t kBoundsBudget = 1u << 28;   // boundary-table entries at most (1 GiB)
 constexpr unsigned kBoundsGrid = 2048;          // k_bounds blocks (the table size is known on the device only)
 
@@ -1923,7 +2023,7 @@ static int wave_slots(fgss_device_sets *d)
 {
     int cus = 0;
     HIPCK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device));
-    return grow(d, d->b_wctr, (size_t)std::max(cus, 1) * 32 * kWaveSlot * 8);
+    return grow(d, d->b_wctr, (size_t)std::max(cus, 1) * 2 * 32 * kWaveSlot * 8);   // the two roles of a join launch
 }
 
 // Every buffer the pipeline touches, sized for upper bounds where one exists (index
@@ -1960,8 +2060,11 @@ static int prepare_buffers(fgss_device_sets *d)
     GROW(d->b_bcnt, (size_t)n * 8);
     GROW(d->b_bofs, (size_t)(n + 1) * 8);
     GROW(d->b_nitems, (size_t)n * 4);
+    GROW(d->b_dnum, (size_t)n * 4);
+    GROW(d->b_doff, (size_t)(n + 1) * 4);
     GROW(d->b_bounds, (size_t)d->cap_bounds * 4);
     GROW(d->b_items, (size_t)d->cap_items * sizeof(Item));
+    GROW(d->b_ditems, (size_t)d->cap_items * 4);
     d->cap_ovf = std::max(d->cap_ovf, d->cap_items);
     GROW(d->b_ovf, (size_t)d->cap_ovf * 4);
     GROW(d->b_ovfp, (size_t)d->cap_ovf * 4);
@@ -1985,8 +2088,9 @@ static JoinArgs join_args(fgss_device_sets *d)
     A.sig = d->sig; A.sig64 = d->sig64; A.sigbad = d->sigbad;
     A.qtok = d->tok; A.qsig = d->sig; A.qsig64 = d->sig64; A.qsigbad = d->sigbad;
     A.rng = P<uint2>(d->b_rng); A.poff = P<uint32_t>(d->b_poff); A.bounds = P<uint32_t>(d->b_bounds);
-    A.items = P<Item>(d->b_items);
+    A.items = P<Item>(d->b_items); A.ditems = P<uint32_t>(d->b_ditems);
     A.n_items_dev = P<uint32_t>(d->b_misc) + M_NITEMS; A.n_items_cap = d->cap_items;
+    A.n_dense_dev = P<uint32_t>(d->b_misc) + M_NDENSE; A.dense_blocks = 0;
     A.job = d->job; A.tj = d->tj; A.ctr_job = P<unsigned long long>(d->b_jctr);
     A.out = P<int32_t>(d->b_out); A.out_cap = d->cap_out; A.ovf = P<uint32_t>(d->b_ovf);
     A.ovfp = P<uint32_t>(d->b_ovfp); A.ovf_cap = d->cap_ovf;
@@ -2078,21 +2182,27 @@ static int enqueue_stage_plan(fgss_device_sets *d, const RunCfg &cfg)
     hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(grid_for(n * kPlanGroup, 256), 4096)), dim3(256), 0, st,
                        d->tok, d->pos, d->len, n, d->job, d->tj, d->jend, probelen, poff, loff, psid, fence, cfg.target,
                        cfg.heavy_target, P<uint2>(d->b_rng), hi, nparts, eq, bcnt, cfg.stream_parts, cfg.dense_w,
-                       cfg.dense_pct);
+                       cfg.dense_pct, (unsigned long long *)(misc + M_NDP64));
     HIPCK(hipGetLastError());
     const uint32_t blim = std::min(d->cap_bounds, cfg.bounds);
     uint32_t *nitems = P<uint32_t>(d->b_nitems);
     HIPCK(exclusive_scan<uint64_t>(bcnt, bofs, n, (uint64_t *)(misc + M_NB64), scr64, st));
-    hipLaunchKernelGGL(k_nitems, dim3(grid_for(n, 256)), dim3(256), 0, st, nparts, probelen, bofs, bcnt, blim, n, nitems);
+    // dense items (only with dense parts on): their count per query, scanned into the
+    // offsets of each query's entries in the dense item list; M_NDENSE their total
+    uint32_t *dnum = cfg.dense_w ? P<uint32_t>(d->b_dnum) : nullptr, *doff = P<uint32_t>(d->b_doff);
+    hipLaunchKernelGGL(k_nitems, dim3(grid_for(n, 256)), dim3(256), 0, st, nparts, probelen, bofs, bcnt, blim, n, nitems,
+                       dnum, (const unsigned long long *)(misc + M_NDP64));
     HIPCK(hipGetLastError());
     HIPCK(exclusive_scan<uint32_t>(nitems, partoff, n, misc + M_NITEMS, scr, st));
+    if (dnum) HIPCK(exclusive_scan<uint32_t>(dnum, doff, n, misc + M_NDENSE, scr, st));
     hipLaunchKernelGGL(k_bounds, dim3(kBoundsGrid), dim3(256), 0, st, n, probelen, poff, P<uint2>(d->b_rng),
                        (const uint32_t *)nullptr, hi, nparts, bofs, bcnt, (const uint64_t *)(misc + M_NB64), psid,
                        fence, P<uint32_t>(d->b_bounds), blim);
     HIPCK(hipGetLastError());
     hipLaunchKernelGGL(k_items, dim3(kItemsGrid), dim3(256), 0, st, 0, (const uint32_t *)nullptr, hi, nparts,
                        probelen, eq, partoff, bofs, bcnt, blim, d->job, d->tj,
-                       d->len, d->pos, poff, n, misc + M_NITEMS, P<Item>(d->b_items), d->cap_items, cfg.dense_w);
+                       d->len, d->pos, poff, n, misc + M_NITEMS, P<Item>(d->b_items), d->cap_items, cfg.dense_w,
+                       (const unsigned long long *)(misc + M_NDP64), (const uint32_t *)doff, P<uint32_t>(d->b_ditems));
     HIPCK(hipGetLastError());
     return FGSS_OK;
 }
@@ -2384,7 +2494,7 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
             A.ovfp = P<uint32_t>(d->b_ovfp);
             A.ovf_cap = cnt;   // a re-split item overflows at most once
             HIPCK(hipEventRecord(d->ev[4], st));
-            rc = launch_join_cap(d, A, cfg.cap, false, ops.rs);
+            rc = launch_join_cap(d, A, cfg.cap, false, ops.rs, false);   // re-split items are never dense
             if (rc) return rc;
             HIPCK(hipEventRecord(d->ev[5], st));
             HIPCK(hipMemcpyAsync(d->h_ctr, ctr, C_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
@@ -3191,7 +3301,8 @@ static int enqueue_rs(fgss_server *srv, ServeSlot &sl, bool timing)
     const uint32_t blim = std::min(d->cap_bounds, cfg.bounds);
     uint32_t *nitems = P<uint32_t>(d->b_nitems);
     HIPCK(exclusive_scan<uint64_t>(bcnt, bofs, v, (uint64_t *)(misc + M_NB64), scr64, st));
-    hipLaunchKernelGGL(k_nitems, dim3(grid_for(v, 256)), dim3(256), 0, st, nparts, probelen, bofs, bcnt, blim, v, nitems);
+    hipLaunchKernelGGL(k_nitems, dim3(grid_for(v, 256)), dim3(256), 0, st, nparts, probelen, bofs, bcnt, blim, v, nitems,
+                       (uint32_t *)nullptr, (const unsigned long long *)(misc + M_NDP64));
     HIPCK(hipGetLastError());
     HIPCK(exclusive_scan<uint32_t>(nitems, partoff, v, misc + M_NITEMS, scr, st));
     hipLaunchKernelGGL(k_bounds, dim3(kBoundsGrid), dim3(256), 0, st, v, probelen, poff, P<uint2>(d->b_rng), wlo, hi,
@@ -3200,7 +3311,8 @@ static int enqueue_rs(fgss_server *srv, ServeSlot &sl, bool timing)
     HIPCK(hipGetLastError());
     hipLaunchKernelGGL(k_items, dim3(kItemsGrid), dim3(256), 0, st, 1, wlo, hi, nparts, probelen, eq, partoff,
                        bofs, bcnt, blim, d->job, d->tj, d->len, d->pos, poff, v, misc + M_NITEMS, P<Item>(d->b_items),
-                       d->cap_items, 0u);
+                       d->cap_items, 0u, (const unsigned long long *)(misc + M_NDP64), (const uint32_t *)nullptr,
+                       (uint32_t *)nullptr);
     HIPCK(hipGetLastError());
     if (timing) HIPCK(hipEventRecord(d->ev[4], st));
     int rc = launch_join_cap(d, rs_args(srv, sl), cfg.cap, false, true);
@@ -3822,11 +3934,14 @@ static size_t noalloc_carve(fgss_device_sets *d, uint8_t *base, int64_t n, int64
     buf(d->b_bcnt, (size_t)N * 8);
     buf(d->b_bofs, (size_t)(N + 1) * 8);
     buf(d->b_nitems, (size_t)N * 4);
+    buf(d->b_dnum, (size_t)N * 4);
+    buf(d->b_doff, (size_t)(N + 1) * 4);
     buf(d->b_bounds, (size_t)d->cap_bounds * 4);
     buf(d->b_items, (size_t)d->cap_items * sizeof(Item));
+    buf(d->b_ditems, (size_t)d->cap_items * 4);
     buf(d->b_ovf, (size_t)d->cap_ovf * 4);
     buf(d->b_ovfp, (size_t)d->cap_ovf * 4);
-    buf(d->b_wctr, (size_t)std::max(cus, 1) * 32 * kWaveSlot * 8);
+    buf(d->b_wctr, (size_t)std::max(cus, 1) * 2 * 32 * kWaveSlot * 8);   // the two roles of a join launch
     buf(d->b_ctr, C_COUNT * sizeof(uint64_t));
     buf(d->b_jctr, kJobCtr * sizeof(uint64_t));
     buf(d->b_misc, M_COUNT * 4);
@@ -4039,7 +4154,7 @@ extern "C" int fgss_selfjoin_noalloc(const int32_t *tokens, const int32_t *lengt
             A.out = out + 3 * written;
             A.out_cap = (uint32_t)std::min<uint64_t>((uint64_t)out_cap - written, UINT32_MAX / 16);
             A.ovf_cap = cnt;
-            rc = launch_join_cap(&d, A, cfg.cap, false, false);
+            rc = launch_join_cap(&d, A, cfg.cap, false, false, false);   // re-split items are never dense
             if (rc) return rc;
             launches++;
             HIPCK(hipMemcpyAsync(h_ctr, d.b_ctr.p, C_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
