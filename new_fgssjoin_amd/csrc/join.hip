@@ -1112,6 +1112,18 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
     // the dense role walks the list of dense item indices, the other role every item
     // (skipping the dense ones)
     const uint32_t n_items = DENSE ? (A.n_dense_dev == nullptr || A.ditems == nullptr ? 0u : *A.n_dense_dev) : *A.n_items_dev;
+    {   // a wavefront with no static chunk and no queue to claim from (the dense role of a
+        // run with few or no dense items) writes a zero counter slot and leaves at once
+        const uint32_t nwg = rblocks * (blockDim.x / kWave);
+        const uint32_t g = min((n_items + nwg - 1) / nwg, (uint32_t)kQueueBatch);
+        const uint32_t gwi = rblock * (blockDim.x / kWave) + (uint32_t)w;
+        if ((uint64_t)gwi * g >= n_items && (uint64_t)nwg * g >= n_items) {
+            unsigned long long *slot = A.wctr + ((size_t)blockIdx.x * join_waves(CAP) + (size_t)w) * kWaveSlot;
+            if (lane < kAccSlots) slot[lane] = 0ull;
+            if (lane == kAccSlots) slot[lane] = kEmpty;
+            return;
+        }
+    }
 
     for (uint32_t i = lane; i < CAP; i += kWave) { tkey[i] = kEmpty; tcnt[i] = 0; tlast[i] = 0; }
     wave_sync();
@@ -2209,7 +2221,7 @@ static int enqueue_stage_plan(fgss_device_sets *d, const RunCfg &cfg)
 
 static int enqueue_stage_join(fgss_device_sets *d, const RunCfg &cfg)
 {
-    return launch_join_cap(d, join_args(d), cfg.cap);   // 9: the join
+    return launch_join_cap(d, join_args(d), cfg.cap, false, false, cfg.dense_w != 0);   // 9: the join
 }
 
 // The first out_hint result triples into pinned memory, behind the join on its stream
