@@ -42,6 +42,17 @@ static int read_all(const char *path, std::vector<char> &buf)
     return FGSS_OK;
 }
 
+// the handles of a run, released on every return path
+struct Handles {
+    fgss_collection *c = nullptr;
+    fgss_pairs *p = nullptr;
+    ~Handles()
+    {
+        fgss_pairs_free(p);
+        fgss_collection_free(c);
+    }
+};
+
 static int usage(const char *argv0)
 {
     fprintf(stdout, "Usage:\n./%s <ppjoin bin file> <threshold> [verbose]\n", argv0);
@@ -91,7 +102,8 @@ int main(int argc, char **argv)
     fprintf(stderr, "Algorithm: %s\n", "fgssjoin");
     fprintf(stderr, "Threshold: Jaccard %g\n", threshold);
     fprintf(stderr, "... LOADING DATASET ...\n");
-    fgss_collection *c = nullptr;
+    Handles h;
+    fgss_collection *&c = h.c;
     int rc;
     if (text && q <= 8 && !host_tok) {   // the device tokenizer packs q-grams of up to 8 bytes
         std::vector<char> buf;
@@ -146,7 +158,7 @@ int main(int argc, char **argv)
         fprintf(stderr, "%s\n", fgss_last_error());
         return 1;
     }
-    fgss_pairs *p = nullptr;
+    fgss_pairs *&p = h.p;
     rc = fgss_selfjoin(c, threshold, nullptr, &p);
     if (rc != FGSS_OK) {
         fprintf(stderr, "%s\n", fgss_last_error());
@@ -169,7 +181,5 @@ int main(int argc, char **argv)
     double t1 = now_ms();
     fprintf(stderr, "===  END JOIN (TIMER STOPPED)  ===\n");
     fprintf(stderr, "Total Running Time: %g s\n", (t1 - t0) / 1000.0);
-    fgss_pairs_free(p);
-    fgss_collection_free(c);
     return 0;
 }

@@ -157,7 +157,8 @@ extern "C" int fgss_collection_from_bin_buffer(const int32_t *ints, int64_t n_in
         int64_t s = starts[n - 1 - k];
         c->id[k] = ints[s];
         c->len[k] = ints[s + 1];
-        memcpy(c->tok.data() + p, ints + s + 2, (size_t)ints[s + 1] * sizeof(int32_t));
+        // (an empty record copies nothing: tok may still be empty, its data() null)
+        if (ints[s + 1] > 0) memcpy(c->tok.data() + p, ints + s + 2, (size_t)ints[s + 1] * sizeof(int32_t));
         p += ints[s + 1];
     }
     const int rc = fgss_finish_collection(c);

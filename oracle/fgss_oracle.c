@@ -680,7 +680,7 @@ int64_t orc_selfjoin_literal(int64_t n_sets, const int32_t *len, const int64_t *
     }
     for (int64_t q = 0; q < n_sets; q++) c.p_q += prefix_clamp(orc_max_prefix((uint32_t)len[q], t), (uint32_t)len[q]);
     free(scores); free(cand); free_index(&ix);
-    qsort(res.v, (size_t)res.n, 3 * sizeof(int32_t), cmp_tri);
+    if (res.n > 1) qsort(res.v, (size_t)res.n, 3 * sizeof(int32_t), cmp_tri);   /* res.v is null when empty */
     *out = res.v ? res.v : (int32_t *)malloc(sizeof(int32_t));
     if (ctr) *ctr = c;
     return res.n;

@@ -1,7 +1,8 @@
-"""The device tokenizer equals the host tokenizer (itself pinned to the reference's
-fixture, the oracle and the reference's tokenizer sources) field for field: ids,
-lengths, tokens, q-gram offsets, term count and the dictionary (checked through
-probes tokenized against it)."""
+"""The device tokenizer equals the oracle's tokenizer (oracle/fgss_oracle.c, pinned to the
+reference's fixture bytes and to the reference's own tokenizer sources) byte for byte in
+the .bin and .loc it writes, and the host tokenizer field for field: ids, lengths,
+tokens, q-gram offsets, term count and the dictionary (checked through probes
+tokenized against it)."""
 import sys
 
 import numpy as np
@@ -37,6 +38,14 @@ def _loc(c, tmp_path, name):
     return open(p, "rb").read(), open(p + ".loc", "rb").read()
 
 
+def _assert_oracle(oracle_mod, dev, text, q, tmp_path):
+    """the device collection's .bin and .loc bytes equal the oracle tokenizer's"""
+    b, l = oracle_mod.tokenize(text, q)
+    db, dl = _loc(dev, tmp_path, "o.bin")
+    assert db == b.astype("<i4").tobytes(), "bin"
+    assert dl == l.astype("<i4").tobytes(), "loc"
+
+
 def test_fixture_bytes(fg, tmp_path):
     text = open(golden_path("small.txt"), "rb").read()
     dev = fg.Collection.from_text_bytes(text, 3, device=True)
@@ -46,21 +55,23 @@ def test_fixture_bytes(fg, tmp_path):
 
 
 @pytest.mark.parametrize("q", [1, 2, 3, 4, 5, 7, 8])
-def test_dblp_all_q(fg, tmp_path, q):
+def test_dblp_all_q(fg, oracle_mod, tmp_path, q):
     text = dblp_text()
     _say(f"dblp q={q}")
     host = fg.Collection.from_text_bytes(text, q)
     dev = fg.Collection.from_text_bytes(text, q, device=True)
+    _assert_oracle(oracle_mod, dev, text, q, tmp_path)
     _assert_same(fg, host, dev, probe_text=text[:20000] + b"\nnever seen zqxj words here\n")
     assert _loc(host, tmp_path, "h.bin") == _loc(dev, tmp_path, "d.bin")
 
 
 @pytest.mark.parametrize("kind,n", [(0, 200000), (1, 20000)])
-def test_generated(fg, kind, n):
+def test_generated(fg, oracle_mod, tmp_path, kind, n):
     text = fg.generate_text(kind, n, 31)
     _say(f"generated kind={kind} n={n}")
     host = fg.Collection.from_text_bytes(text, 3)
     dev = fg.Collection.from_text_bytes(text, 3, device=True)
+    _assert_oracle(oracle_mod, dev, text, 3, tmp_path)
     _assert_same(fg, host, dev, probe_text=fg.generate_text(kind, 500, 99))
 
 
@@ -82,10 +93,11 @@ EDGE_TEXTS = [
 
 @pytest.mark.parametrize("k", range(len(EDGE_TEXTS)))
 @pytest.mark.parametrize("q", [1, 3, 6])
-def test_edge_texts(fg, k, q):
+def test_edge_texts(fg, oracle_mod, tmp_path, k, q):
     text = EDGE_TEXTS[k]
     host = fg.Collection.from_text_bytes(text, q)
     dev = fg.Collection.from_text_bytes(text, q, device=True)
+    _assert_oracle(oracle_mod, dev, text, q, tmp_path)
     _assert_same(fg, host, dev, probe_text=b"probe line\n" + text[:100])
 
 
