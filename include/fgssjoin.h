@@ -174,7 +174,10 @@ typedef struct {
  * first use.  Pairs are returned sorted by (q, s).  Replaces inverted_index
  * (include/fgssjoin/index.cuh:99-100) + process_blocks
  * (include/fgssjoin/block.cuh:13-14), whose results were printed instead of
- * returned.  opts may be null. */
+ * returned.  opts may be null.  Records may come in any order (as ppjoin_format
+ * reads them): the earlier internal index of a pair is its query, and a collection
+ * whose lengths are not non-increasing joins every query against every later record
+ * through the 4-way size filter (slower than a size-ascending .bin, same rules). */
 int fgss_selfjoin(fgss_collection *c, float threshold, const fgss_join_opts *opts, fgss_pairs **out);
 void fgss_pairs_free(fgss_pairs *p);
 
@@ -182,7 +185,7 @@ void fgss_pairs_free(fgss_pairs *p);
  * cannot allocate device or host memory -- code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc.  The records are already in
  * device memory in the layout fgss_collection_arrays describes (internal order,
- * lengths non-increasing, tokens strictly ascending in [0, n_terms), offsets the
+ * any length order, tokens strictly ascending in [0, n_terms), offsets the
  * exclusive scan of the lengths; int32 each); every workspace buffer lies in one
  * static device array of FGSS_NOALLOC_SCRATCH_BYTES per GPU, held until the process
  * ends, so collections above the limits below fail with FGSS_ERR_UNSUPPORTED.

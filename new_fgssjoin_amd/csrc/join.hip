@@ -121,6 +121,12 @@ __device__ __forceinline__ uint32_t item_word(const Item *__restrict__ items, ui
 // R-S items over the sources longer than the probe: the probe contributes its mid
 // prefix and the sources their max prefix (the longer record is the query)
 constexpr uint32_t kMidProbe = 0x10000u;
+// an item of a job whose lengths are not non-increasing: every posting of the window
+// passes the 4-way size filter itself (a sorted job's window holds only partners)
+constexpr uint32_t kSizeFilter = 0x20000u;
+// jend[job] | kJobUnsorted: the job's record lengths are not non-increasing (its
+// queries' windows then run to the job's end)
+constexpr uint32_t kJobUnsorted = 0x80000000u;
 constexpr uint32_t kWholeWindow = 0x80000000u;   // the item covers the query's whole window
 constexpr uint32_t kPlanned = 0x40000000u;       // part of a planned split: ranges come from the bounds table
 constexpr uint32_t kStreamed = 0x20000000u;      // the whole window, walked part by part by one wavefront
@@ -359,19 +365,6 @@ static int validate_collection(const fgss_collection *c)
     if ((int64_t)c->tok.size() >= (int64_t)UINT32_MAX)
         return fgss_fail(FGSS_ERR_UNSUPPORTED, "more than 2^32-1 tokens (the reference's unsigned offsets)");
     if (c->max_len >= 65536) return fgss_fail(FGSS_ERR_UNSUPPORTED, "records longer than 65535 tokens");
-    {
-        // lengths non-increasing within each job (the whole collection is one job)
-        int64_t start = 0;
-        const size_t nj = c->jend.empty() ? 1 : c->jend.size();
-        for (size_t j = 0; j < nj; j++) {
-            const int64_t end = c->jend.empty() ? n : (int64_t)c->jend[j];
-            for (int64_t i = start + 1; i < end; i++)
-                if (c->len[(size_t)i] > c->len[(size_t)i - 1])
-                    return fgss_fail(FGSS_ERR_UNSUPPORTED,
-                                     "record lengths must be non-increasing in internal order (a size-ascending .bin)");
-            start = end;
-        }
-    }
     if (c->n_terms < 0 || c->n_terms >= INT32_MAX)
         return fgss_fail(FGSS_ERR_UNSUPPORTED, "token ids must be below 2^31-1");
     for (int64_t i = 0; i < n; i++) {
@@ -418,15 +411,22 @@ static int upload_records(fgss_device_sets **out, const fgss_collection *c)
         hipMalloc(&d->tj, jb) != hipSuccess) {
         return fgss_fail(FGSS_ERR_NOMEM, "device allocation for the collection failed");
     }
-    if (c->jend.empty()) {
-        HIPCK(fill<uint32_t>(d->job, d->n, 0u, d->st));
-        const uint32_t e = (uint32_t)d->n;
-        HIPCK(hipMemcpyAsync(d->jend, &e, 4, hipMemcpyHostToDevice, d->st));
-        HIPCK(hipStreamSynchronize(d->st));   // e lives on this stack frame
-    } else {
-        HIPCK(hipMemcpyAsync(d->job, c->job.data(), (size_t)d->n * 4, hipMemcpyHostToDevice, d->st));
-        HIPCK(hipMemcpyAsync(d->jend, c->jend.data(), jb, hipMemcpyHostToDevice, d->st));
+    // each job's end, flagged when its lengths are not non-increasing in internal order
+    // (a .bin in any record order: its queries' windows run to the job's end)
+    std::vector<uint32_t> je(d->njobs);
+    {
+        int64_t start = 0;
+        for (uint32_t j = 0; j < d->njobs; j++) {
+            const int64_t end = c->jend.empty() ? d->n : (int64_t)c->jend[j];
+            bool sorted = true;
+            for (int64_t i = start + 1; i < end && sorted; i++) sorted = c->len[(size_t)i] <= c->len[(size_t)i - 1];
+            je[j] = (uint32_t)end | (sorted ? 0u : kJobUnsorted);
+            start = end;
+        }
     }
+    if (c->jend.empty()) HIPCK(fill<uint32_t>(d->job, d->n, 0u, d->st));
+    else HIPCK(hipMemcpyAsync(d->job, c->job.data(), (size_t)d->n * 4, hipMemcpyHostToDevice, d->st));
+    HIPCK(hipMemcpyAsync(d->jend, je.data(), jb, hipMemcpyHostToDevice, d->st));   // je lives until the sync below
     std::vector<uint32_t> pos32((size_t)d->n);
     for (int64_t i = 0; i < d->n; i++) pos32[(size_t)i] = (uint32_t)c->pos[(size_t)i];
     HIPCK(hipMemcpyAsync(d->tok, c->tok.data(), (size_t)d->ntok * 4, hipMemcpyHostToDevice, d->st));
@@ -546,6 +546,19 @@ __device__ __forceinline__ uint32_t partner_lmin(uint32_t l, float t)
         uint32_t mid = (lo + hi) >> 1;
         if (fgss_size_ok(l, mid, t)) hi = mid;
         else lo = mid + 1;
+    }
+    return lo;
+}
+
+// The largest partner length ls >= l (below 2^16) with size_ok(l, ls): the other end of
+// the interval partner_lmin starts.
+__device__ __forceinline__ uint32_t partner_lmax(uint32_t l, float t)
+{
+    uint32_t lo = l, hi = 65535u;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (fgss_size_ok(l, mid, t)) lo = mid;
+        else hi = mid - 1;
     }
     return lo;
 }
@@ -675,10 +688,14 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
         const uint32_t lq = len[q];
         const uint32_t qp = live ? probelen[q] : 0u;
         const uint32_t jb = job[q];
-        // window: the s > q of the same job with len[s] >= the smallest partner length;
-        // lengths are non-increasing within a job
+        // window: the s > q of the same job with len[s] >= the smallest partner length
+        // when lengths are non-increasing within the job; every s > q of the job
+        // otherwise (its items then apply the size filter posting by posting)
+        const uint32_t je = jend[jb];
+        const bool unsorted = (je & kJobUnsorted) != 0;
         const uint32_t need = partner_lmin(lq, tj[jb]);
-        uint32_t lo = (uint32_t)q + 1, hi = jend[jb];   // first s with len[s] < need
+        uint32_t lo = (uint32_t)q + 1, hi = je & ~kJobUnsorted;   // first s with len[s] < need
+        if (unsorted) lo = hi;
         while (lo < hi) {
             uint32_t mid = (lo + hi) >> 1;
             if (len[mid] >= need) lo = mid + 1;
@@ -710,7 +727,7 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
             // parts of a few postings per probe token)
             const uint32_t win = wend - (uint32_t)q - 1u;
             const uint32_t ndp = dense_w ? (win + dense_w - 1u) / dense_w : 0u;
-            const bool dense = ndp > 1 && np > 1 && qp < kStreamQp &&
+            const bool dense = !unsorted && ndp > 1 && np > 1 && qp < kStreamQp &&
                                (uint64_t)e * 100u >= (uint64_t)ndp * dense_w * dense_pct;
             nparts[q] = dense ? (ndp | kDenseParts) : np;
             dparts += dense ? ndp : 0u;
@@ -827,7 +844,8 @@ __global__ __launch_bounds__(256) void k_items(int rs, const uint32_t *__restric
                                                int64_t n, const uint32_t *__restrict__ n_items_dev,
                                                Item *__restrict__ items, uint32_t cap_items, uint32_t dense_w,
                                                const unsigned long long *__restrict__ ndparts,
-                                               const uint32_t *__restrict__ doff, uint32_t *__restrict__ ditems)
+                                               const uint32_t *__restrict__ doff, uint32_t *__restrict__ ditems,
+                                               const uint32_t *__restrict__ jend)
 {
     const uint32_t total = *n_items_dev;
     const uint32_t chunk = dense_chunk(*ndparts);
@@ -872,7 +890,7 @@ __global__ __launch_bounds__(256) void k_items(int rs, const uint32_t *__restric
         it.bofs = mode == IM_PLANNED ? (uint32_t)bofs[q] : 0u;
         it.job = job[r];
         it.t = tj[it.job];
-        it.lq = len[r] | flag;
+        it.lq = len[r] | flag | (jend != nullptr && (jend[it.job] & kJobUnsorted) != 0 ? kSizeFilter : 0u);
         it.qpos = pos[r];
         it.qpoff = poff[q];
         items[o] = it;
@@ -1218,6 +1236,14 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
         }
         const uint32_t lq = item.lq & 0xFFFFu;
         const bool midprobe = RS && (item.lq & kMidProbe) != 0;
+        // a job in any record order: the partner lengths that pass the 4-way size filter,
+        // an interval [szlo, szhi] (each of the filter's bounds is monotone in |S|)
+        const bool sizefilt = !RS && (item.lq & kSizeFilter) != 0;
+        uint32_t szlo = 0, szhi = 0xFFFFu;
+        if (sizefilt) {
+            szlo = partner_lmin(lq, item.t);
+            szhi = partner_lmax(lq, item.t);
+        }
         const uint32_t qpos = item.qpos;
         uint32_t qp = midprobe ? fgss_mid_prefix(lq, t) : fgss_max_prefix(lq, t);
         if (qp > lq) qp = lq;
@@ -1389,6 +1415,11 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                         const uint32_t ls = pe[k].y >> 16;
                         v &= (uint32_t)((pe[k].y & 0xFFFFu) < (midprobe ? fgss_max_prefix(ls, t) : fgss_mid_prefix(ls, t)));
                     }
+                    if (sizefilt) {   // postings of non-partners are neither scanned nor counted
+                        const uint32_t ok = (uint32_t)((pe[k].y >> 16) - szlo <= szhi - szlo);
+                        escan -= (uint32_t)__popcll(__ballot(v & (ok ^ 1u)));
+                        v &= ok;
+                    }
                     valm |= v << k;
                     jpos[k] = jb + seg[k];
                 }
@@ -1461,6 +1492,11 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                     if (RS) {   // the source's own prefix at t
                         const uint32_t ls = pe[k].y >> 16;
                         v &= (uint32_t)((pe[k].y & 0xFFFFu) < (midprobe ? fgss_max_prefix(ls, t) : fgss_mid_prefix(ls, t)));
+                    }
+                    if (sizefilt) {   // the cursor still moves past non-partners (bm above)
+                        const uint32_t ok = (uint32_t)((pe[k].y >> 16) - szlo <= szhi - szlo);
+                        escan -= (uint32_t)__popcll(__ballot(v & (ok ^ 1u)));
+                        v &= ok;
                     }
                     valm |= v << k;
                 }
@@ -2214,7 +2250,8 @@ static int enqueue_stage_plan(fgss_device_sets *d, const RunCfg &cfg)
     hipLaunchKernelGGL(k_items, dim3(kItemsGrid), dim3(256), 0, st, 0, (const uint32_t *)nullptr, hi, nparts,
                        probelen, eq, partoff, bofs, bcnt, blim, d->job, d->tj,
                        d->len, d->pos, poff, n, misc + M_NITEMS, P<Item>(d->b_items), d->cap_items, cfg.dense_w,
-                       (const unsigned long long *)(misc + M_NDP64), (const uint32_t *)doff, P<uint32_t>(d->b_ditems));
+                       (const unsigned long long *)(misc + M_NDP64), (const uint32_t *)doff, P<uint32_t>(d->b_ditems),
+                       (const uint32_t *)d->jend);
     HIPCK(hipGetLastError());
     return FGSS_OK;
 }
@@ -3324,7 +3361,7 @@ static int enqueue_rs(fgss_server *srv, ServeSlot &sl, bool timing)
     hipLaunchKernelGGL(k_items, dim3(kItemsGrid), dim3(256), 0, st, 1, wlo, hi, nparts, probelen, eq, partoff,
                        bofs, bcnt, blim, d->job, d->tj, d->len, d->pos, poff, v, misc + M_NITEMS, P<Item>(d->b_items),
                        d->cap_items, 0u, (const unsigned long long *)(misc + M_NDP64), (const uint32_t *)nullptr,
-                       (uint32_t *)nullptr);
+                       (uint32_t *)nullptr, (const uint32_t *)nullptr);
     HIPCK(hipGetLastError());
     if (timing) HIPCK(hipEventRecord(d->ev[4], st));
     int rc = launch_join_cap(d, rs_args(srv, sl), cfg.cap, false, true);
@@ -3623,6 +3660,10 @@ extern "C" int fgss_server_create(const fgss_collection *indexed, float t_min, i
         opts->table_slots != 1024 && opts->table_slots != 2048 && opts->table_slots != 4096)
         return fgss_fail(FGSS_ERR_ARG, "table_slots must be 256, 512, 1024, 2048 or 4096");
     if (!indexed->jend.empty()) return fgss_fail(FGSS_ERR_ARG, "the indexed collection must be a plain collection");
+    // its size bands are contiguous id ranges: lengths must be non-increasing
+    if (!indexed->sorted_desc)
+        return fgss_fail(FGSS_ERR_UNSUPPORTED,
+                         "the indexed collection's lengths must be non-increasing in internal order (a size-ascending .bin)");
     int rc0 = check_range(opts, (int64_t)indexed->len.size(), false);
     if (rc0) return rc0;
     auto *srv = new fgss_server();
@@ -3971,8 +4012,9 @@ static size_t noalloc_carve(fgss_device_sets *d, uint8_t *base, int64_t n, int64
 }
 
 // Checks of caller-given records before any kernel indexes with them: lengths below
-// 2^16 and non-increasing, offsets the exclusive scan of the lengths, tokens strictly
-// ascending and in [0, n_terms).  A thread per record; flags or-ed into *err.
+// 2^16, offsets the exclusive scan of the lengths, tokens strictly ascending and in
+// [0, n_terms); flag 2 marks lengths that are not non-increasing (allowed: the job is
+// then joined in any record order).  A thread per record; flags or-ed into *err.
 __global__ void k_validate_records(const uint32_t *__restrict__ tok, const uint32_t *__restrict__ len,
                                    const uint32_t *__restrict__ pos, int64_t n, int64_t ntok, uint32_t n_terms,
                                    uint32_t *__restrict__ err)
@@ -4107,15 +4149,13 @@ extern "C" int fgss_selfjoin_noalloc(const int32_t *tokens, const int32_t *lengt
     HIPCK(hipMemcpyAsync(&h_err, nb.err, 4, hipMemcpyDeviceToHost, st));
     HIPCK(hipStreamSynchronize(st));
     if (h_err & 1u) return fgss_fail(FGSS_ERR_UNSUPPORTED, "records longer than 65535 tokens");
-    if (h_err & 2u)
-        return fgss_fail(FGSS_ERR_UNSUPPORTED, "record lengths must be non-increasing (a size-ascending .bin)");
     if (h_err & 4u) return fgss_fail(FGSS_ERR_ARG, "offsets are not the exclusive scan of the lengths");
     if (h_err & 8u) return fgss_fail(FGSS_ERR_UNSUPPORTED, "token ids must lie in [0, n_terms)");
     if (h_err & 16u) return fgss_fail(FGSS_ERR_UNSUPPORTED, "tokens must be strictly ascending in each record");
 
     // 2. the record set's job table, threshold and signatures, then the pipeline
     HIPCK(fill<uint32_t>(d.job, n_sets, 0u, st));
-    HIPCK(fill<uint32_t>(d.jend, 1, (uint32_t)n_sets, st));
+    HIPCK(fill<uint32_t>(d.jend, 1, (uint32_t)n_sets | ((h_err & 2u) ? kJobUnsorted : 0u), st));
     HIPCK(fill<float>(d.tj, 1, threshold, st));
     hipLaunchKernelGGL(k_signature, dim3((unsigned)std::min<int64_t>((n_sets + 3) / 4, 65536)), dim3(256), 0, st, d.tok,
                        d.pos, d.len, n_sets, d.sig, d.sig64, d.sigbad);

@@ -57,6 +57,29 @@ def records_to_bin(records, ids=None) -> np.ndarray:
     return np.array(out, np.int32)
 
 
+def bin_records(bin_ints):
+    """(id, tokens) per record of a .bin int32 stream, in file order."""
+    v = np.asarray(bin_ints, np.int32)
+    out, i = [], 0
+    while i < v.size:
+        n = int(v[i + 1])
+        out.append((int(v[i]), v[i + 2:i + 2 + n].copy()))
+        i += 2 + n
+    return out
+
+
+def shuffle_bin(bin_ints, seed: int) -> np.ndarray:
+    """The same records as a .bin stream in a seeded random file order (any order is a
+    valid .bin: ppjoin_format reads records as they come, src/fgssjoin/data.cu:31-126)."""
+    recs = bin_records(bin_ints)
+    order = np.random.default_rng(seed).permutation(len(recs))
+    out = []
+    for k in order:
+        rid, tok = recs[int(k)]
+        out += [rid, int(tok.size)] + tok.tolist()
+    return np.array(out, np.int32)
+
+
 def hip_runtime():
     """The HIP runtime that libfgssjoin.so links (already mapped in this process), for
     tests that create their own streams or device buffers."""

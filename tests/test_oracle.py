@@ -165,6 +165,41 @@ def test_closed_form_equals_literal(oracle_mod, t):
             assert lc[k] == ctr[k], (B, k)
 
 
+@pytest.mark.parametrize("t", [0.5, 0.7, 0.9])
+def test_any_record_order(oracle_mod, t):
+    """A .bin in any record order (ppjoin_format, data.cu:31-126, reads records as they
+    come; the earlier internal index is the query): the closed form equals the reference's
+    tile loop at block size 1 -- whose tile skip (block.cu:121-130) then drops only what the
+    4-way size filter drops -- on pairs and every counter, and lies within brute force."""
+    from _util import shuffle_bin
+    b, _ = oracle_mod.tokenize(_titles(300, 61), 3)
+    s = oracle_mod.Sets(shuffle_bin(b, 5))
+    assert np.any(np.diff(s.len) > 0)   # lengths really out of order
+    tri, ctr = oracle_mod.selfjoin(s, t)
+    lit, lc = oracle_mod.selfjoin(s, t, mode="literal", block_size=1)
+    assert np.array_equal(lit, tri)
+    for k in ("e_idx", "p_q", "e_scan", "candidates", "intersections", "results", "m_reads"):
+        assert lc[k] == ctr[k], k
+    br, _ = oracle_mod.selfjoin(s, t, mode="brute")
+    assert {tuple(x) for x in tri.tolist()} <= {tuple(x) for x in br.tolist()}
+    assert len(tri) > 0
+
+
+def test_tile_skip_drops_pairs_of_unsorted_input(oracle_mod):
+    """The reference's tile skip (block.cu:121-130) compares the last query's length with
+    the first source's and assumes lengths non-increasing in internal order: on a .bin in
+    another order a whole tile is skipped although it holds a pair.  Internal order
+    {1,2}, {1,2,3}, {10..19} at t = 0.5: one block of 3 skips everything (10 > floor(2 / 0.5));
+    block size 1, the closed form and this build find (0, 1) with overlap 2."""
+    from _util import records_to_bin
+    s = oracle_mod.Sets(records_to_bin([[1, 2], [1, 2, 3], list(range(10, 20))]))
+    t = float(np.float32(0.5))
+    tri, _ = oracle_mod.selfjoin(s, t)
+    assert tri.tolist() == [[0, 1, 2]]
+    assert oracle_mod.selfjoin(s, t, mode="literal", block_size=1)[0].tolist() == [[0, 1, 2]]
+    assert oracle_mod.selfjoin(s, t, mode="literal", block_size=3)[0].shape[0] == 0
+
+
 @pytest.mark.parametrize("t", [0.3, 0.5, 0.7, 0.8, 0.9])
 def test_subset_of_brute_force(oracle_mod, t):
     b, _ = oracle_mod.tokenize(_titles(600, 33), 3)

@@ -119,3 +119,15 @@ def test_noalloc_null_stream_long_records_and_small_output(fg):
     with pytest.raises(fg.FgssError) as e:
         fg.selfjoin_noalloc(dt.p.value, dl.p.value, dp.p.value, 2, 4, 4, 0.5, 0, 0)
     assert e.value.status == fg.FGSS_ERR_UNSUPPORTED
+
+
+def test_noalloc_any_record_order(fg):
+    """records whose lengths are out of order: the device-side check lets them through and
+    the join runs each query against every later record, as fgss_selfjoin does"""
+    from _util import shuffle_bin
+    hip = _hip()
+    coll = fg.Collection.from_bin_ints(shuffle_bin(fg.Collection.generate(fg.GEN_TITLES, 3000, 34).bin_ints(), 9))
+    assert coll.info.sorted_desc == 0
+    _say("noalloc shuffled titles n=3000 t=0.6")
+    tri, st = _noalloc_join(fg, hip, coll, 0.6, 0)
+    _equal(fg, coll, 0.6, tri, st)
