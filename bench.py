@@ -310,14 +310,15 @@ def batched_leg(fg, n_jobs: int, reps: int):
         plain_s = time.perf_counter() - t0
     log(f"batched: {n_jobs} sequential plain joins in {plain_s * 1e3:.1f} ms")
     b = fg.Batch(colls, ts)
-    tune = b.autotune(reps=2)
-    b.run(graph=True, table_slots=tune["table_slots"], load_pct=tune["load_pct"])
+    tune = b.autotune(reps=2)   # table slots x load, then index groups and record blocks
+    kw = {k: tune[k] for k in ("table_slots", "load_pct", "index_groups", "block_records")}
+    b.run(graph=True, **kw)
     times = []
     exact = True
     for r in range(reps):
         with gc_paused():
             t0 = time.perf_counter()
-            b.run(graph=True, table_slots=tune["table_slots"], load_pct=tune["load_pct"])
+            b.run(graph=True, **kw)
             got = b.all_pairs()
             times.append(time.perf_counter() - t0)
         if r == 0:
@@ -331,7 +332,9 @@ def batched_leg(fg, n_jobs: int, reps: int):
     return {"jobs": n_jobs, "records": int(sizes.sum()),
             "plain_sequential_ms": plain_s * 1e3, "batched_ms": batch_s * 1e3,
             "plain_joins_per_s": n_jobs / plain_s, "batched_joins_per_s": n_jobs / batch_s,
-            "speedup": plain_s / batch_s, "autotune": tune, "bit_exact_vs_plain": bool(exact)}
+            "speedup": plain_s / batch_s, "autotune": tune,
+            "partitioning": {"index_groups": max(1, tune["index_groups"]), "block_records": tune["block_records"]},
+            "bit_exact_vs_plain": bool(exact)}
 
 
 def _edit(line: bytes, rng, n_edits: int) -> bytes:

@@ -157,6 +157,14 @@ typedef struct {
     void *stream;              /* hipStream_t carrying the call's device work (the reference's
                                   per-thread default stream, include/util.cuh:4); null = the handle's
                                   own stream.  The call still returns after that stream has drained. */
+    int32_t index_groups;      /* batches: the jobs split into this many groups of consecutive jobs
+                                  (balanced by records), each indexed and joined by a pipeline run of
+                                  its own -- a smaller index, item list and working set per run (the
+                                  reference partitions its index by record block instead,
+                                  src/fgssjoin/block.cu:273-434, index.cu:114-162); 0 or 1 = one */
+    int32_t block_records;     /* a work item spans at most this many consecutive source records
+                                  (the reference's record-block size, src/fgssjoin/block.cu:50-81,
+                                  as an upper bound on a part's window); 0 = no limit */
 } fgss_join_opts;
 
 typedef struct {
@@ -248,9 +256,17 @@ int fgss_batch_pairs(fgss_batch *b, int32_t job, fgss_pairs **out);
  * each job's counters; out->stats is not filled.  No reference counterpart (the
  * reference joins one collection per run). */
 int fgss_batch_pairs_all(fgss_batch *b, fgss_pairs **out, int64_t *job_first, fgss_join_stats *job_stats);
-/* Autotuner: time the batch for each LDS table size x work-item load and return
- * the fastest options (graph replays, median of reps). */
+/* Autotuner: graph replays of the batch, median of reps each, over LDS table sizes
+ * {512, 1024} x work-item loads {50, 65, 80}%, then, from the fastest, index groups
+ * {2, 4, 8} and record blocks {4096, 32768}; returns the fastest options. */
 int fgss_batch_autotune(fgss_batch *b, int32_t reps, fgss_join_opts *best, double *best_ms);
+/* The autotuner with a tuning cache: cache_path, a text file (created when missing),
+ * holds one line per tuned workload, keyed by its jobs, records, tokens, thresholds,
+ * device and library version.  A workload found there gets its stored choice without
+ * timing (*from_cache = 1; *best_ms the stored time); any other is tuned and its line
+ * appended.  No reference counterpart. */
+int fgss_batch_autotune_cached(fgss_batch *b, int32_t reps, const char *cache_path, fgss_join_opts *best,
+                               double *best_ms, int32_t *from_cache);
 void fgss_batch_free(fgss_batch *b);
 
 /* ------------------------------------------------------------------ */

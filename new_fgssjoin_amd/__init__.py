@@ -42,7 +42,7 @@ EXPORTS = (
     "fgss_selfjoin", "fgss_selfjoin_noalloc", "fgss_pairs_free", "fgss_pairs_print", "fgss_collection_print_index",
     "fgss_generate_text", "fgss_free",
     "fgss_batch_create", "fgss_batch_set_thresholds", "fgss_batch_run", "fgss_batch_pairs",
-    "fgss_batch_pairs_all", "fgss_batch_autotune", "fgss_batch_free",
+    "fgss_batch_pairs_all", "fgss_batch_autotune", "fgss_batch_autotune_cached", "fgss_batch_free",
     "fgss_server_create", "fgss_server_submit", "fgss_server_poll", "fgss_server_done", "fgss_server_wait",
     "fgss_server_result", "fgss_server_hold", "fgss_server_free",
 )
@@ -78,7 +78,8 @@ class JoinStats(C.Structure):
 class JoinOpts(C.Structure):
     _fields_ = [("load_pct", C.c_int32), ("table_slots", C.c_int32), ("stream_parts", C.c_int32),
                 ("bounds_cap", C.c_int32), ("dense_pct", C.c_int32), ("query_begin", C.c_int64),
-                ("query_end", C.c_int64), ("stream", C.c_void_p)]
+                ("query_end", C.c_int64), ("stream", C.c_void_p), ("index_groups", C.c_int32),
+                ("block_records", C.c_int32)]
 
 
 class BatchStats(C.Structure):
@@ -143,6 +144,8 @@ def lib():
         "fgss_batch_pairs": [vp, C.c_int32, C.POINTER(C.POINTER(_Pairs))],
         "fgss_batch_pairs_all": [vp, C.POINTER(C.POINTER(_Pairs)), C.POINTER(C.c_int64), C.POINTER(JoinStats)],
         "fgss_batch_autotune": [vp, C.c_int32, C.POINTER(JoinOpts), C.POINTER(C.c_double)],
+        "fgss_batch_autotune_cached": [vp, C.c_int32, C.c_char_p, C.POINTER(JoinOpts), C.POINTER(C.c_double),
+                                       i32p],
         "fgss_server_create": [vp, C.c_float, C.c_int32, C.c_int64, C.POINTER(JoinOpts), C.POINTER(vp)],
         "fgss_server_submit": [vp, vp, C.c_float, i64p],
         "fgss_server_poll": [vp, C.c_int32, i32p, i32p],
@@ -315,13 +318,15 @@ class Collection:
         _check(lib().fgss_collection_upload(self._h))
 
     def selfjoin(self, threshold: float, table_slots: int = 0, load_pct: int = 0, bounds_cap: int = 0,
-                 stream_parts: int = 0, queries: tuple | None = None, stream: int = 0, dense_pct: int = 0) -> Pairs:
+                 stream_parts: int = 0, queries: tuple | None = None, stream: int = 0, dense_pct: int = 0,
+                 block_records: int = 0) -> Pairs:
         """queries=(begin, end): only those queries (internal indices), each against every
         later record; stream: a hipStream_t (as int) for the call's device work;
-        dense_pct: see fgss_join_opts (-1: no dense parts)."""
+        dense_pct, block_records: see fgss_join_opts (dense_pct -1: no dense parts)."""
         L = lib()
         qb, qe = queries if queries is not None else (0, 0)
-        opts = JoinOpts(load_pct, table_slots, stream_parts, bounds_cap, dense_pct, qb, qe, stream or None)
+        opts = JoinOpts(load_pct, table_slots, stream_parts, bounds_cap, dense_pct, qb, qe, stream or None, 0,
+                        block_records)
         out = C.POINTER(_Pairs)()
         _check(L.fgss_selfjoin(self._h, C.c_float(threshold), C.byref(opts), C.byref(out)))
         return _take_pairs(out)
@@ -378,9 +383,10 @@ class Batch:
         ts = (C.c_float * self.njobs)(*[float(t) for t in thresholds])
         _check(lib().fgss_batch_set_thresholds(self._h, ts))
 
-    def run(self, graph: bool = True, table_slots: int = 0, load_pct: int = 0, stream: int = 0) -> dict:
+    def run(self, graph: bool = True, table_slots: int = 0, load_pct: int = 0, stream: int = 0,
+            index_groups: int = 0, block_records: int = 0) -> dict:
         st = BatchStats()
-        opts = JoinOpts(load_pct, table_slots, 0, 0, 0, 0, 0, stream or None)
+        opts = JoinOpts(load_pct, table_slots, 0, 0, 0, 0, 0, stream or None, index_groups, block_records)
         _check(lib().fgss_batch_run(self._h, C.byref(opts), 1 if graph else 0, C.byref(st)))
         return st.as_dict()
 
@@ -409,11 +415,19 @@ class Batch:
         return [Pairs(q[a:b], s[a:b], rid[a:b], sid[a:b], ov[a:b], sim[a:b], stats[j].as_dict())
                 for j, (a, b) in enumerate(zip(f[:-1], f[1:]))]
 
-    def autotune(self, reps: int = 3):
+    def autotune(self, reps: int = 3, cache_path: str | None = None):
+        """The fastest options (fgss_batch_autotune); with cache_path, through the tuning
+        cache there (fgss_batch_autotune_cached: a stored choice is returned untimed)."""
         best = JoinOpts()
         ms = C.c_double()
-        _check(lib().fgss_batch_autotune(self._h, reps, C.byref(best), C.byref(ms)))
-        return {"table_slots": best.table_slots, "load_pct": best.load_pct, "ms": ms.value}
+        hit = C.c_int32(0)
+        if cache_path is None:
+            _check(lib().fgss_batch_autotune(self._h, reps, C.byref(best), C.byref(ms)))
+        else:
+            _check(lib().fgss_batch_autotune_cached(self._h, reps, os.fsencode(cache_path), C.byref(best),
+                                                    C.byref(ms), C.byref(hit)))
+        return {"table_slots": best.table_slots, "load_pct": best.load_pct, "index_groups": best.index_groups,
+                "block_records": best.block_records, "ms": ms.value, "from_cache": bool(hit.value)}
 
     def close(self):
         if self._h is not None and self._h.value:

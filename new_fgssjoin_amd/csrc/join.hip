@@ -221,7 +221,7 @@ struct fgss_device_sets {
     float *tj = nullptr;        // per job: threshold of the current run
     DevBuf b_psid, b_fence, b_bcnt, b_bofs, b_bounds, b_probelen, b_poff, b_rng, b_eq, b_midlen, b_midoff, b_ka,
         b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_pl_of, b_post, b_loff, b_hi, b_nparts, b_partoff, b_scanscr, b_items,
-        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc, b_nitems, b_ovfp, b_wctr, b_jsave, b_ditems, b_dnum, b_doff;
+        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc, b_nitems, b_ovfp, b_wctr, b_jsave, b_ditems, b_dnum, b_doff, b_grp;
     uint32_t cap_items = 0, cap_bounds = 0, cap_out = 0;   // current capacities (elements)
     uint32_t cap_ovf = 0;         // overflow records (a streamed item can record one per part)
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -237,10 +237,13 @@ struct fgss_device_sets {
     // timing events and the counter copies enqueued between and after them
     hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
     hipGraphExec_t graph_exec[3] = {nullptr, nullptr, nullptr};
-    std::array<int64_t, 16> graph_key{};   // what the captured stages were built for (graph_key_of)
+    std::array<int64_t, 17> graph_key{};   // what the captured stages were built for (graph_key_of)
     int cap_gen = 0;              // bumped whenever a buffer the pipeline uses is reallocated
     bool sized = false;           // a run has completed (first-run buffer sizing is over)
     int32_t q_begin = 0, q_end = 0;   // the run's query range (internal indices; 0, 0: every record)
+    // a batch run's job group: only records [grp_begin, grp_end) are indexed and probe
+    // (0, 0: every record); written to b_grp before each run, outside any captured graph
+    uint32_t grp_begin = 0, grp_end = 0;
     // Buffers replaced by larger ones (device, pinned host): freed when the record set is
     // released.  hipFree and hipHostFree wait for the whole device, which would stall the
     // runs of other record sets (a server's other slots, another thread's collection).
@@ -289,7 +292,7 @@ void fgss_device_release(fgss_collection *c)
                       &d->b_rng, &d->b_eq, &d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib,
                       &d->b_sortscr, &d->b_sid_of, &d->b_pl_of, &d->b_post, &d->b_loff, &d->b_hi, &d->b_nparts, &d->b_partoff,
                       &d->b_scanscr, &d->b_items, &d->b_items2, &d->b_ovf, &d->b_out, &d->b_ctr, &d->b_jctr,
-                      &d->b_nitems, &d->b_ovfp, &d->b_wctr, &d->b_jsave, &d->b_ditems, &d->b_dnum, &d->b_doff,
+                      &d->b_nitems, &d->b_ovfp, &d->b_wctr, &d->b_jsave, &d->b_ditems, &d->b_dnum, &d->b_doff, &d->b_grp,
                       &d->b_misc};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -460,9 +463,12 @@ extern "C" int fgss_collection_upload(fgss_collection *c)
 // at the smallest threshold it will serve) instead of the self-join mid-prefix.
 // Records outside the query range [qb, qe) (unless qb = qe = 0: every record) probe with
 // no tokens: they stay sources of the queries inside it, which are joined as usual.
+// grp (null: none): a job group [grp[0], grp[1]) of a batch run (grp[1] = 0: every
+// record); records outside it are neither indexed nor probe.
 __global__ void k_midlen(const uint32_t *__restrict__ len, const uint32_t *__restrict__ job,
                          const float *__restrict__ tj, uint32_t *__restrict__ midlen,
-                         uint32_t *__restrict__ probelen, int64_t n, int rs, int64_t qb, int64_t qe)
+                         uint32_t *__restrict__ probelen, int64_t n, int rs, int64_t qb, int64_t qe,
+                         const uint32_t *__restrict__ grp)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
@@ -470,8 +476,17 @@ __global__ void k_midlen(const uint32_t *__restrict__ len, const uint32_t *__res
         uint32_t l = len[i];
         uint32_t mp = rs ? fgss_max_prefix(l, t) : fgss_mid_prefix(l, t);
         uint32_t qp = fgss_max_prefix(l, t);
-        midlen[i] = mp < l ? mp : l;
-        probelen[i] = (qb != 0 || qe != 0) && (i < qb || i >= qe) ? 0u : (qp < l ? qp : l);
+        const bool out = grp != nullptr && grp[1] != 0u && (i < (int64_t)grp[0] || i >= (int64_t)grp[1]);
+        midlen[i] = out ? 0u : (mp < l ? mp : l);
+        probelen[i] = out || ((qb != 0 || qe != 0) && (i < qb || i >= qe)) ? 0u : (qp < l ? qp : l);
+    }
+}
+
+__global__ void k_set_pair(uint32_t *__restrict__ p, uint32_t a, uint32_t b)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        p[0] = a;
+        p[1] = b;
     }
 }
 
@@ -674,7 +689,8 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
                                               uint2 *__restrict__ rng, uint32_t *__restrict__ hi_out,
                                               uint32_t *__restrict__ nparts, uint32_t *__restrict__ eq_out,
                                               uint64_t *__restrict__ bcnt, uint32_t stream_parts, uint32_t dense_w,
-                                              uint32_t dense_pct, unsigned long long *__restrict__ ndparts)
+                                              uint32_t dense_pct, unsigned long long *__restrict__ ndparts,
+                                              uint32_t block_records)
 {
     uint32_t dparts = 0;   // dense parts of this lane's queries
     const int64_t ngroups = (int64_t)gridDim.x * (blockDim.x / kPlanGroup);
@@ -696,6 +712,7 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
         const uint32_t need = partner_lmin(lq, tj[jb]);
         uint32_t lo = (uint32_t)q + 1, hi = je & ~kJobUnsorted;   // first s with len[s] < need
         if (unsorted) lo = hi;
+        if (qp == 0) hi = lo;   // no probe (outside the run's queries or job group): no window
         while (lo < hi) {
             uint32_t mid = (lo + hi) >> 1;
             if (len[mid] >= need) lo = mid + 1;
@@ -721,6 +738,8 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
             // a query of more than kHeavyParts parts fills its parts fuller: fewer parts
             // (boundary rows, streamed rounds) outweigh the longer probe chains there
             if (np > kHeavyParts) np = (e + heavy_target - 1) / heavy_target;
+            // a part spans at most block_records consecutive sources
+            if (block_records != 0 && e != 0) np = max(np, (wend - (uint32_t)q - 1u + block_records - 1u) / block_records);
             // dense parts: a window of several parts whose postings fill dense parts of
             // dense_w consecutive sources to dense_pct% of their slots on average, for a
             // probe that fits the cursor slots (windows this full would need hash-table
@@ -2029,6 +2048,8 @@ struct RunCfg {
     uint32_t stream_parts = kStreamParts;   // queries split into more parts are streamed
     uint32_t dense_w = 1024;      // sources per dense part (2 x cap; 0: no dense parts)
     uint32_t dense_pct = FGSS_DENSE_PCT;   // dense parts for windows whose postings fill them this full (%)
+    uint32_t block_records = 0;   // sources per work item at most (0: no limit)
+    int groups = 1;               // batches: job groups, one pipeline run each
 };
 
 static RunCfg make_cfg(const fgss_join_opts *opts)
@@ -2046,6 +2067,8 @@ static RunCfg make_cfg(const fgss_join_opts *opts)
     // dense parts need the list room left after the cursor records: 512 table slots or more
     c.dense_w = c.cap >= 512 && !(opts && opts->dense_pct < 0) ? 2u * (uint32_t)c.cap : 0u;
     if (opts && opts->dense_pct > 0) c.dense_pct = (uint32_t)std::min(opts->dense_pct, 1000);
+    if (opts && opts->block_records > 0) c.block_records = (uint32_t)opts->block_records;
+    if (opts && opts->index_groups > 1) c.groups = opts->index_groups;
     return c;
 }
 
@@ -2113,6 +2136,7 @@ static int prepare_buffers(fgss_device_sets *d)
     GROW(d->b_bounds, (size_t)d->cap_bounds * 4);
     GROW(d->b_items, (size_t)d->cap_items * sizeof(Item));
     GROW(d->b_ditems, (size_t)d->cap_items * 4);
+    GROW(d->b_grp, 8);
     d->cap_ovf = std::max(d->cap_ovf, d->cap_items);
     GROW(d->b_ovf, (size_t)d->cap_ovf * 4);
     GROW(d->b_ovfp, (size_t)d->cap_ovf * 4);
@@ -2163,7 +2187,7 @@ static int enqueue_index(fgss_device_sets *d, int rs)
     uint32_t *psid = P<uint32_t>(d->b_psid), *fence = P<uint32_t>(d->b_fence);
     uint2 *post = P<uint2>(d->b_post);
     hipLaunchKernelGGL(k_midlen, dim3(grid_for(n, 256)), dim3(256), 0, st, d->len, d->job, d->tj, midlen, probelen, n, rs,
-                       (int64_t)d->q_begin, (int64_t)d->q_end);
+                       (int64_t)d->q_begin, (int64_t)d->q_end, rs ? (const uint32_t *)nullptr : P<uint32_t>(d->b_grp));
     HIPCK(hipGetLastError());
     HIPCK(exclusive_scan<uint32_t>(midlen, midoff, n, misc + M_NE, scr, st));
     HIPCK(exclusive_scan<uint32_t>(probelen, poff, n, misc + M_NP, scr, st));
@@ -2230,7 +2254,7 @@ static int enqueue_stage_plan(fgss_device_sets *d, const RunCfg &cfg)
     hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(grid_for(n * kPlanGroup, 256), 4096)), dim3(256), 0, st,
                        d->tok, d->pos, d->len, n, d->job, d->tj, d->jend, probelen, poff, loff, psid, fence, cfg.target,
                        cfg.heavy_target, P<uint2>(d->b_rng), hi, nparts, eq, bcnt, cfg.stream_parts, cfg.dense_w,
-                       cfg.dense_pct, (unsigned long long *)(misc + M_NDP64));
+                       cfg.dense_pct, (unsigned long long *)(misc + M_NDP64), cfg.block_records);
     HIPCK(hipGetLastError());
     const uint32_t blim = std::min(d->cap_bounds, cfg.bounds);
     uint32_t *nitems = P<uint32_t>(d->b_nitems);
@@ -2335,11 +2359,11 @@ struct RunTotals {
 // (a reallocation of any buffer bumps cap_gen): the configuration, the capacities
 // the kernels test against and the record counts.  A replay is valid while it is
 // unchanged; a capacity can grow inside a buffer's slack without a reallocation.
-static std::array<int64_t, 16> graph_key_of(const fgss_device_sets *d, const RunCfg &cfg)
+static std::array<int64_t, 17> graph_key_of(const fgss_device_sets *d, const RunCfg &cfg)
 {
     return {cfg.cap, cfg.target, cfg.heavy_target, cfg.bounds, cfg.stream_parts, d->cap_gen,
             d->cap_items, d->cap_out, d->cap_ovf, d->cap_bounds, d->n, d->njobs, d->q_begin, d->q_end,
-            cfg.dense_w, cfg.dense_pct};
+            cfg.dense_w, cfg.dense_pct, cfg.block_records};
 }
 
 // For the length of one call: the record set's work on the caller's stream (opts->stream)
@@ -2418,6 +2442,10 @@ static int launch_once(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &op
 {
     int rc = ops.prepare();
     if (rc) return rc;
+    if (!ops.rs) {   // the job group of this run, ahead of the (possibly captured) stages
+        hipLaunchKernelGGL(k_set_pair, dim3(1), dim3(64), 0, d->st, P<uint32_t>(d->b_grp), d->grp_begin, d->grp_end);
+        HIPCK(hipGetLastError());
+    }
     if (use_graph && ops.graph_ok) {
         rc = launch_join_cap(d, ops.args(), cfg.cap, true, ops.rs);   // launch configuration, outside the capture
         if (rc) return rc;
@@ -2985,12 +3013,47 @@ extern "C" int fgss_batch_run(fgss_batch *b, const fgss_join_opts *opts, int32_t
     CallScope scope(d, opts, false);
     rc = set_thresholds(d, b->t.data(), (uint32_t)b->t.size());
     if (rc) return rc;
-    rc = run_pipeline(d, cfg, use_graph != 0, b->tri, b->totals);
-    if (rc) return rc;
+    const size_t nj = b->colls.size();
+    const int groups = (int)std::min<size_t>((size_t)std::max(cfg.groups, 1), std::max<size_t>(nj, 1));
+    if (groups <= 1) {
+        rc = run_pipeline(d, cfg, use_graph != 0, b->tri, b->totals);
+        if (rc) return rc;
+    } else {
+        // job groups of about equal record counts, each indexed and joined by a run of
+        // its own over the same captured stages (the group's range is read on the device)
+        b->tri.clear();
+        b->totals = RunTotals();
+        b->totals.jctr.assign(nj * kJobCtr, 0);
+        std::vector<int32_t> tri;
+        RunTotals T;
+        const int64_t total = b->rec_begin[nj];
+        size_t j0 = 0;
+        for (int g = 0; g < groups && j0 < nj; g++) {
+            size_t j1 = j0 + 1;
+            const int64_t want = total * (g + 1) / groups;
+            while (j1 < nj && (b->rec_begin[j1] < want || g == groups - 1)) j1++;
+            d->grp_begin = (uint32_t)b->rec_begin[j0];
+            d->grp_end = (uint32_t)b->rec_begin[j1];
+            rc = run_pipeline(d, cfg, use_graph != 0, tri, T);
+            d->grp_begin = d->grp_end = 0;
+            if (rc) return rc;
+            b->tri.insert(b->tri.end(), tri.begin(), tri.end());
+            RunTotals &A = b->totals;
+            for (int k = 0; k < C_COUNT; k++) A.acc[k] += T.acc[k];
+            for (size_t k = 0; k < A.jctr.size(); k++) A.jctr[k] += T.jctr[k];
+            A.index_entries += T.index_entries;
+            A.probe_tokens += T.probe_tokens;
+            A.ms_index += T.ms_index;
+            A.ms_join += T.ms_join;
+            A.ms_kernel += T.ms_kernel;
+            A.launches += T.launches;
+            A.graph = T.graph;
+            j0 = j1;
+        }
+    }
     auto t1 = std::chrono::steady_clock::now();
     sort_pairs_by_key(b->tri, b->order);
     // jobs are contiguous record ranges, so the sorted triples are grouped by job
-    const size_t nj = b->colls.size();
     b->job_first.assign(nj + 1, b->order.size());
     size_t k = 0;
     for (size_t j = 0; j < nj; j++) {
@@ -3077,34 +3140,138 @@ extern "C" int fgss_batch_pairs_all(fgss_batch *b, fgss_pairs **out, int64_t *jo
 
 // Autotuner: time the batch under each (table slots, item load) choice, replaying
 // captured graphs, and return the fastest.  reps runs per choice, the median counts.
+// median of reps graph-replayed runs of the batch with options o (a first run captures
+// the graphs and warms the buffers)
+static int batch_time(fgss_batch *b, const fgss_join_opts &o, int32_t reps, double &ms)
+{
+    fgss_batch_stats st;
+    int rc = fgss_batch_run(b, &o, 1, &st);
+    if (rc) return rc;
+    std::vector<double> v;
+    for (int r = 0; r < reps; r++) {
+        rc = fgss_batch_run(b, &o, 1, &st);
+        if (rc) return rc;
+        v.push_back(st.ms_total);
+    }
+    std::sort(v.begin(), v.end());
+    ms = v[v.size() / 2];
+    return FGSS_OK;
+}
+
 extern "C" int fgss_batch_autotune(fgss_batch *b, int32_t reps, fgss_join_opts *best, double *best_ms)
 {
     if (!b || !best) return fgss_fail(FGSS_ERR_ARG, "null argument");
     if (reps < 1) reps = 3;
-    const int slots[] = {512, 1024};
-    const int loads[] = {50, 65, 80};
     double bms = 1e300;
     fgss_join_opts bo{};
-    for (int s : slots)
-        for (int l : loads) {
+    auto consider = [&](const fgss_join_opts &o) -> int {
+        double ms = 0;
+        const int rc = batch_time(b, o, reps, ms);
+        if (rc) return rc;
+        if (ms < bms) { bms = ms; bo = o; }
+        return FGSS_OK;
+    };
+    // LDS table size x work-item load, then index groups and record blocks from the best
+    for (int s : {512, 1024})
+        for (int l : {50, 65, 80}) {
             fgss_join_opts o{};
             o.table_slots = s;
             o.load_pct = l;
-            fgss_batch_stats st;
-            int rc = fgss_batch_run(b, &o, 1, &st);   // capture (and warm up)
+            int rc = consider(o);
             if (rc) return rc;
-            std::vector<double> ms;
-            for (int r = 0; r < reps; r++) {
-                rc = fgss_batch_run(b, &o, 1, &st);
-                if (rc) return rc;
-                ms.push_back(st.ms_total);
-            }
-            std::sort(ms.begin(), ms.end());
-            const double med = ms[ms.size() / 2];
-            if (med < bms) { bms = med; bo = o; }
         }
+    const fgss_join_opts base = bo;
+    for (int g : {2, 4, 8}) {
+        if ((size_t)g > b->colls.size()) break;
+        fgss_join_opts o = base;
+        o.index_groups = g;
+        int rc = consider(o);
+        if (rc) return rc;
+    }
+    const fgss_join_opts base2 = bo;
+    for (int r : {4096, 32768}) {
+        fgss_join_opts o = base2;
+        o.block_records = r;
+        int rc = consider(o);
+        if (rc) return rc;
+    }
     *best = bo;
     if (best_ms) *best_ms = bms;
+    return FGSS_OK;
+}
+
+// The tuning cache's key of a batch: its shape, thresholds, device and library version.
+static std::string batch_tune_key(const fgss_batch *b)
+{
+    uint64_t h = 1469598103934665603ull;   // FNV-1a over the thresholds and job sizes
+    auto mix = [&h](uint64_t v) {
+        for (int k = 0; k < 8; k++) {
+            h ^= (v >> (8 * k)) & 0xFFu;
+            h *= 1099511628211ull;
+        }
+    };
+    int64_t toks = 0;
+    for (size_t j = 0; j < b->colls.size(); j++) {
+        uint32_t tb;
+        memcpy(&tb, &b->t[j], 4);
+        mix(tb);
+        mix((uint64_t)b->colls[j]->len.size());
+        toks += (int64_t)b->colls[j]->tok.size();
+    }
+    hipDeviceProp_t prop{};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) prop.gcnArchName[0] = 0;
+    char buf[256];
+    snprintf(buf, sizeof(buf), "jobs=%zu,records=%lld,tokens=%lld,h=%016llx,dev=%s/%d,lib=%s", b->colls.size(),
+             (long long)b->rec_begin.back(), (long long)toks, (unsigned long long)h, prop.gcnArchName,
+             prop.multiProcessorCount, fgss_version());
+    for (char *c = buf; *c; c++)
+        if (*c == ' ' || *c == '\t' || *c == '\n') *c = '_';
+    return buf;
+}
+
+extern "C" int fgss_batch_autotune_cached(fgss_batch *b, int32_t reps, const char *cache_path, fgss_join_opts *best,
+                                          double *best_ms, int32_t *from_cache)
+{
+    if (!b || !best || !cache_path) return fgss_fail(FGSS_ERR_ARG, "null argument");
+    if (from_cache) *from_cache = 0;
+    const std::string key = batch_tune_key(b);
+    // a line: <key> table_slots load_pct index_groups block_records ms
+    if (FILE *f = fopen(cache_path, "r")) {
+        char line[512];
+        bool hit = false;
+        while (!hit && fgets(line, sizeof(line), f)) {
+            char k[300];
+            int s = 0, l = 0, g = 0, r = 0;
+            double ms = 0;
+            if (sscanf(line, "%299s %d %d %d %d %lf", k, &s, &l, &g, &r, &ms) == 6 && key == k &&
+                (s == 256 || s == 512 || s == 1024 || s == 2048 || s == 4096) && l >= 0 && l <= 100 && g >= 0 &&
+                r >= 0) {
+                fgss_join_opts o{};
+                o.table_slots = s;
+                o.load_pct = l;
+                o.index_groups = g;
+                o.block_records = r;
+                *best = o;
+                if (best_ms) *best_ms = ms;
+                hit = true;
+            }
+        }
+        fclose(f);
+        if (hit) {
+            if (from_cache) *from_cache = 1;
+            return FGSS_OK;
+        }
+    }
+    double ms = 0;
+    int rc = fgss_batch_autotune(b, reps, best, &ms);
+    if (rc) return rc;
+    if (best_ms) *best_ms = ms;
+    FILE *f = fopen(cache_path, "a");
+    if (!f) return fgss_fail(FGSS_ERR_IO, "cannot append to the tuning cache \"%s\"", cache_path);
+    const bool ok = fprintf(f, "%s %d %d %d %d %.6f\n", key.c_str(), best->table_slots, best->load_pct,
+                            best->index_groups, best->block_records, ms) > 0;
+    if (fclose(f) != 0 || !ok) return fgss_fail(FGSS_ERR_IO, "cannot append to the tuning cache \"%s\"", cache_path);
     return FGSS_OK;
 }
 
@@ -3992,6 +4159,7 @@ static size_t noalloc_carve(fgss_device_sets *d, uint8_t *base, int64_t n, int64
     buf(d->b_bounds, (size_t)d->cap_bounds * 4);
     buf(d->b_items, (size_t)d->cap_items * sizeof(Item));
     buf(d->b_ditems, (size_t)d->cap_items * 4);
+    buf(d->b_grp, 8);
     buf(d->b_ovf, (size_t)d->cap_ovf * 4);
     buf(d->b_ovfp, (size_t)d->cap_ovf * 4);
     buf(d->b_wctr, (size_t)std::max(cus, 1) * 2 * 32 * kWaveSlot * 8);   // the two roles of a join launch
@@ -4156,6 +4324,7 @@ extern "C" int fgss_selfjoin_noalloc(const int32_t *tokens, const int32_t *lengt
     // 2. the record set's job table, threshold and signatures, then the pipeline
     HIPCK(fill<uint32_t>(d.job, n_sets, 0u, st));
     HIPCK(fill<uint32_t>(d.jend, 1, (uint32_t)n_sets | ((h_err & 2u) ? kJobUnsorted : 0u), st));
+    HIPCK(fill<uint32_t>(P<uint32_t>(d.b_grp), 2, 0u, st));   // one job group: every record
     HIPCK(fill<float>(d.tj, 1, threshold, st));
     hipLaunchKernelGGL(k_signature, dim3((unsigned)std::min<int64_t>((n_sets + 3) / 4, 65536)), dim3(256), 0, st, d.tok,
                        d.pos, d.len, n_sets, d.sig, d.sig64, d.sigbad);

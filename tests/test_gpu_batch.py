@@ -90,11 +90,55 @@ def test_graph_replays_follow_growing_capacities(fg):
             assert _same(b.pairs(j), c.selfjoin(tk[j])), (k, j)
 
 
-def test_autotune_returns_a_valid_choice(fg):
+def test_autotune_returns_a_valid_choice(fg, tmp_path):
     colls, ts = _jobs(fg)
     b = fg.Batch(colls[:6], ts[:6])
     best = b.autotune(reps=2)
     assert best["table_slots"] in (512, 1024) and best["load_pct"] in (50, 65, 80) and best["ms"] > 0
-    b.run(graph=True, table_slots=best["table_slots"], load_pct=best["load_pct"])
+    assert best["index_groups"] in (0, 2, 4) and best["block_records"] in (0, 4096, 32768)
+    b.run(graph=True, table_slots=best["table_slots"], load_pct=best["load_pct"],
+          index_groups=best["index_groups"], block_records=best["block_records"])
     for j in range(6):
         assert _same(b.pairs(j), colls[j].selfjoin(ts[j]))
+    # the tuning cache: tuned and stored once, then returned untimed; another workload misses
+    cache = str(tmp_path / "tune.txt")
+    first = b.autotune(reps=1, cache_path=cache)
+    assert not first["from_cache"]
+    again = b.autotune(reps=1, cache_path=cache)
+    assert again["from_cache"]
+    for k in ("table_slots", "load_pct", "index_groups", "block_records"):
+        assert again[k] == first[k], k
+    assert len(open(cache).read().splitlines()) == 1
+    other = fg.Batch(colls[:5], ts[:5])
+    assert not other.autotune(reps=1, cache_path=cache)["from_cache"]
+    assert len(open(cache).read().splitlines()) == 2
+
+
+@pytest.mark.parametrize("groups,block", [(2, 0), (3, 0), (5, 0), (16, 0), (1, 64), (1, 1000), (4, 4096)])
+def test_partitioning_choices_give_identical_pairs(fg, oracle_mod, groups, block):
+    """index groups (one index and run per group of jobs) and record blocks (items span at
+    most that many sources): every choice yields each job's plain pairs and counters"""
+    colls, ts = _jobs(fg)
+    plain = [c.selfjoin(t) for c, t in zip(colls, ts)]
+    b = fg.Batch(colls, ts)
+    for graph in (False, True, True):
+        _say(f"batch groups={groups} block_records={block} graph={graph}")
+        st = b.run(graph=graph, index_groups=groups, block_records=block)
+        assert st["n_pairs"] == sum(len(p) for p in plain)
+        every = b.all_pairs()
+        for j, p in enumerate(plain):
+            assert _same(every[j], p), f"job {j}"
+            for k in ("candidates", "intersections", "results", "postings_scanned"):
+                assert every[j].stats[k] == p.stats[k], (j, k)
+    tri, ctr = oracle_mod.selfjoin(oracle_sets(oracle_mod, colls[3]), ts[3])
+    got = every[3]
+    assert np.array_equal(np.stack([got.q, got.s, got.overlap], 1) if len(got) else np.zeros((0, 3), np.int32), tri)
+
+
+@pytest.mark.parametrize("block", [16, 300, 5000])
+def test_record_blocks_in_plain_joins(fg, oracle_mod, block):
+    from _util import assert_join_equal
+    coll = fg.Collection.generate(fg.GEN_TITLES, 20000, 47)
+    sets = oracle_sets(oracle_mod, coll)
+    _say(f"titles n=20000 block_records={block}")
+    assert_join_equal(oracle_mod, sets, coll.selfjoin(0.6, block_records=block), 0.6)
