@@ -138,6 +138,12 @@ constexpr uint32_t kDenseParts = 0x80000000u;
 #ifndef FGSS_DENSE_CHUNK
 #define FGSS_DENSE_CHUNK 64
 #endif
+#ifndef FGSS_DENSE_UNROLL
+#define FGSS_DENSE_UNROLL 4   // rows of kWave postings per step of a dense part's rounds
+#endif
+#ifndef FGSS_DENSE_LOGK_BIAS
+#define FGSS_DENSE_LOGK_BIAS 0   // a dense part's first K: 2^bias x its expected postings per token
+#endif
 #ifndef FGSS_DENSE_LOGK_MIN
 #define FGSS_DENSE_LOGK_MIN 3
 #endif
@@ -221,7 +227,7 @@ struct fgss_device_sets {
     float *tj = nullptr;        // per job: threshold of the current run
     DevBuf b_psid, b_fence, b_bcnt, b_bofs, b_bounds, b_probelen, b_poff, b_rng, b_eq, b_midlen, b_midoff, b_ka,
         b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_pl_of, b_post, b_loff, b_hi, b_nparts, b_partoff, b_scanscr, b_items,
-        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc, b_nitems, b_ovfp, b_wctr, b_jsave, b_ditems, b_dnum, b_doff, b_grp;
+        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc, b_nitems, b_ovfp, b_wctr, b_jsave, b_ditems, b_dnum, b_doff, b_grp, b_pden;
     uint32_t cap_items = 0, cap_bounds = 0, cap_out = 0;   // current capacities (elements)
     uint32_t cap_ovf = 0;         // overflow records (a streamed item can record one per part)
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -292,7 +298,7 @@ void fgss_device_release(fgss_collection *c)
                       &d->b_rng, &d->b_eq, &d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib,
                       &d->b_sortscr, &d->b_sid_of, &d->b_pl_of, &d->b_post, &d->b_loff, &d->b_hi, &d->b_nparts, &d->b_partoff,
                       &d->b_scanscr, &d->b_items, &d->b_items2, &d->b_ovf, &d->b_out, &d->b_ctr, &d->b_jctr,
-                      &d->b_nitems, &d->b_ovfp, &d->b_wctr, &d->b_jsave, &d->b_ditems, &d->b_dnum, &d->b_doff, &d->b_grp,
+                      &d->b_nitems, &d->b_ovfp, &d->b_wctr, &d->b_jsave, &d->b_ditems, &d->b_dnum, &d->b_doff, &d->b_grp, &d->b_pden,
                       &d->b_misc};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -524,17 +530,22 @@ __global__ __launch_bounds__(256) void k_entries(const uint32_t *__restrict__ to
 }
 
 // postings in term order, packed: x = record id, y = token position | record length << 16
+// pden (null: none): the 4-byte postings that dense parts read, record << 8 | position
+// (records below 2^24; a posting a dense part reads has its position below 256, the
+// longer ones are clamped and never read)
 __global__ void k_postings(const uint32_t *__restrict__ sidx, const uint32_t *__restrict__ sid_of,
                            const uint32_t *__restrict__ pl_of, const uint32_t *__restrict__ nE_dev,
-                           uint2 *__restrict__ post, uint32_t *__restrict__ psid, uint32_t *__restrict__ fence)
+                           uint2 *__restrict__ post, uint32_t *__restrict__ psid, uint32_t *__restrict__ fence,
+                           uint32_t *__restrict__ pden)
 {
     const uint32_t nE = *nE_dev;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nE) return;
     const uint32_t e = sidx[i];
-    const uint32_t s = sid_of[e];
-    post[i] = make_uint2(s, pl_of[e]);
+    const uint32_t s = sid_of[e], pl = pl_of[e];
+    post[i] = make_uint2(s, pl);
     psid[i] = s;
+    if (pden) pden[i] = s << 8 | min(pl & 0xFFFFu, 255u);
     if ((i & (kFence - 1)) == 0) fence[i / kFence] = s;
 }
 
@@ -955,6 +966,7 @@ struct JoinArgs {
     const uint8_t *qsigbad;
     const uint32_t *loff;
     const uint2 *post;          // postings (record id, token position | length << 16), grouped by term
+    const uint32_t *pden;       // the same postings as record << 8 | position, for dense parts (null: none)
     const uint32_t *psid;       // record ids of the postings alone
     const uint32_t *fence;      // psid[f * kFence]
     const uint32_t *sig;        // per record: kSigWords words of byte counts per token bucket
@@ -1061,11 +1073,11 @@ __device__ __forceinline__ bool table_insert(const uint2 (&pe)[N], uint32_t valm
 // The dense counterpart of table_insert, for a dense part of sources [wa, wa + 2 x CAP):
 // a source's slot is its offset from wa (no hashing or probing, so nothing overflows),
 // its match count is 8 bits, four slots to a word of dcnt (probes of dense items hold at
-// most 255 tokens), and dlast keeps query position << 24 | source position << 16 | |S|:
-// both positions are below 256 (a source is no longer than its query, so its position
-// lies inside the query's probe length), and |S| is the same in every update of a slot,
-// so the maximum word holds the last match and the source length.  Lanes without work
-// aim their updates at scr[lane].
+// most 255 tokens), and dlast keeps query position << 24 | source position << 16: both
+// positions are below 256 (a source is no longer than its query, so its position lies
+// inside the query's probe length), so the maximum word holds the last match.  pe[k] is
+// (record, source position) from the 4-byte postings.  Lanes without work aim their
+// updates at scr[lane].
 template <int N>
 __device__ __forceinline__ void dense_insert(const uint2 (&pe)[N], uint32_t valm, const uint32_t (&jpos)[N], uint32_t wa,
                                              uint32_t *dlast, uint32_t *dcnt, uint32_t *scr, int lane)
@@ -1075,7 +1087,7 @@ __device__ __forceinline__ void dense_insert(const uint2 (&pe)[N], uint32_t valm
         const bool v = (valm >> k) & 1u;
         const uint32_t h = pe[k].x - wa;
         atomicAdd(v ? &dcnt[h >> kDcShift] : &scr[lane], 1u << ((h & (kDcSlots - 1u)) * kDcBits));
-        atomicMax(v ? &dlast[h] : &scr[lane], (jpos[k] << 24) | ((pe[k].y & 0xFFu) << 16) | (pe[k].y >> 16));
+        atomicMax(v ? &dlast[h] : &scr[lane], (jpos[k] << 24) | (pe[k].y << 16));
     }
 }
 
@@ -1115,10 +1127,13 @@ enum WaveAcc { W_ESCAN = 0, W_CAND, W_SIG, W_VER, W_SIG2, W_INTER, W_VTOK, W_RET
 // counters and the wavefront's last job
 constexpr int kWaveSlot = 32;
 static_assert(kAccSlots + 1 <= kWaveSlot && kAccSlots + 1 <= kWave, "slot layout");
-// LDS words per wavefront: table (key, count, last match), list, token ranges, query stage
+// length samples of a dense part: the first source of each 32nd of the part, and its last
+constexpr uint32_t kDenseSamples = 33;
+// LDS words per wavefront: table (key, count, last match), list, token ranges, query
+// stage, a dense part's length samples
 constexpr uint32_t join_wave_words(int cap)
 {
-    return 4u * cap + 2u * kTokBatch + 1u + kQStage;
+    return 4u * cap + 2u * kTokBatch + 1u + kQStage + kDenseSamples;
 }
 
 // RS: an R-S join against an index holding max_prefix(|S|, t_min) tokens per source;
@@ -1321,10 +1336,15 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
         for (uint32_t p0 = 0, p1 = 0; p0 < nwin; p0 = p1) {
         p1 = min(p0 + span, nwin);
         uint32_t wa = item.a, wb = item.b;
+        // a dense part's source lengths are non-increasing (dense parts are for jobs in
+        // size order): lane k < kDenseSamples samples the length at k x DW / 32 (the last
+        // one at the part's end).  The load is issued here and stored after the rounds.
+        uint32_t lsamp = 0;
         if (dense) {
             p1 = p0 + 1;
             wa = item.a + p0 * DW;
             wb = min(wa + DW, item.b);
+            if (lane < (int)kDenseSamples) lsamp = A.len[wa + min((uint32_t)lane * (DW / 32u), wb - wa - 1u)];
         } else if (streamed) {
             const uint64_t W = (uint64_t)item.b - item.a;
             wa = part_start(item.a, W, p0, nwin);
@@ -1483,7 +1503,12 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                     jv[k] = toks[ec >> logK];
                     curj[k] = cur[jv[k]];
                     idx[k] = curj[k] + (ec & (K - 1));
-                    pe[k] = A.post[idx[k]];
+                    if constexpr (DENSE) {   // 4-byte postings: record, position (no length)
+                        const uint32_t w = A.pden[idx[k]];
+                        pe[k] = make_uint2(w >> 8, w & 0xFFu);
+                    } else {
+                        pe[k] = A.post[idx[k]];
+                    }
                     end[k] = dense && kDenseEndLds ? tend[jv[k]] : qrng[jv[k]].y;
                 }
                 uint32_t valm = 0;
@@ -1529,10 +1554,13 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
 #ifdef FGSS_DEBUG_FILL
                 if (dense) WACC(W_RETRY, total);
 #endif
+                // rows per full step: a dense part's rounds wait on their posting loads, so
+                // its steps may hold more rows in flight than the hash path's
+                constexpr int NS = DENSE ? FGSS_DENSE_UNROLL : kUnroll;
                 for (uint32_t e0 = 0; e0 < total;) {
                     const uint32_t rem = total - e0;   // wave-uniform
                     bool full;
-                    if (rem > 3u * kWave) { full = sstep(std::integral_constant<int, kUnroll>(), e0); e0 += kWave * kUnroll; }
+                    if (rem > (uint32_t)(NS - 1) * kWave) { full = sstep(std::integral_constant<int, NS>(), e0); e0 += kWave * NS; }
                     else if (rem > (uint32_t)kWave) { full = sstep(std::integral_constant<int, 2>(), e0); e0 += 2 * kWave; }
                     else { full = sstep(std::integral_constant<int, 1>(), e0); e0 += kWave; }
                     if (full) return true;
@@ -1556,7 +1584,7 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                 // starts at the power of two at or above its expected postings per token.
                 logK = 3;
                 if (dense && nact > 0)
-                    logK = (uint32_t)min(6, max(3, ceil_log2(((item.pad & kPadMask) + nact - 1) / nact)));
+                    logK = (uint32_t)min(6, max(FGSS_DENSE_LOGK_MIN, ceil_log2(((item.pad & kPadMask) + nact - 1) / nact) + FGSS_DENSE_LOGK_BIAS));
                 while (nact > 0 && !overflow) {
                     if (lane < (int)(kStreamQp / 32)) satm[lane] = 0;
                     wave_sync();
@@ -1610,6 +1638,11 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                 }
             }
             continue;
+        }
+        uint32_t *const dsamp = qstage + kQStage;
+        if (dense) {
+            if (lane < (int)kDenseSamples) dsamp[lane] = lsamp;
+            wave_sync();
         }
         PHASE_MARK(1);
 
@@ -1839,7 +1872,10 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                     const bool has = m != 0;
                     if (__ballot(has) == 0) continue;
                     const uint32_t w = dlast[h];
-                    const uint32_t ls = w & 0xFFFFu, js = w >> 24, ss = (w >> 16) & 0xFFu;
+                    // the source's length: the samples around it when they agree (lengths are
+                    // monotone between them), else from the records
+                    const uint32_t sa = dsamp[h / (DW / 32u)], sb = dsamp[h / (DW / 32u) + 1u];
+                    const uint32_t ls = sa == sb ? sa : (has ? A.len[wa + h] : 0u), js = w >> 24, ss = (w >> 16) & 0xFFu;
                     const float bound = fgss_min_overlap(lq, ls, t);
                     const uint32_t rq = lq - js, rs = ls - ss;
                     const uint32_t minrem = rq < rs ? rq : rs;
@@ -2099,6 +2135,9 @@ static int wave_slots(fgss_device_sets *d)
 
 // Every buffer the pipeline touches, sized for upper bounds where one exists (index
 // entries and probe tokens <= tokens) and for the current capacities otherwise.
+// dense parts need record ids below 2^24 (their 4-byte postings)
+static bool dense_ok(const fgss_device_sets *d) { return d->n < ((int64_t)1 << 24); }
+
 static int prepare_buffers(fgss_device_sets *d)
 {
     const int64_t n = std::max<int64_t>(d->n, 1), E = std::max<int64_t>(d->ntok, 1);
@@ -2119,6 +2158,7 @@ static int prepare_buffers(fgss_device_sets *d)
     // kWave spare postings: a streamed round reads up to 63 postings past the end of a
     // token's list before it tests them against the end
     GROW(d->b_post, (size_t)(E + kWave) * 8);
+    if (dense_ok(d)) GROW(d->b_pden, (size_t)(E + kWave) * 4);
     GROW(d->b_psid, (size_t)(E + 1) * 4);
     GROW(d->b_fence, (size_t)(E / kFence + 2) * 4);
     GROW(d->b_loff, ((size_t)d->n_terms + 2) * 4);
@@ -2160,6 +2200,7 @@ static JoinArgs join_args(fgss_device_sets *d)
     A.sig = d->sig; A.sig64 = d->sig64; A.sigbad = d->sigbad;
     A.qtok = d->tok; A.qsig = d->sig; A.qsig64 = d->sig64; A.qsigbad = d->sigbad;
     A.rng = P<uint2>(d->b_rng); A.poff = P<uint32_t>(d->b_poff); A.bounds = P<uint32_t>(d->b_bounds);
+    A.pden = dense_ok(d) ? P<uint32_t>(d->b_pden) : nullptr;
     A.items = P<Item>(d->b_items); A.ditems = P<uint32_t>(d->b_ditems);
     A.n_items_dev = P<uint32_t>(d->b_misc) + M_NITEMS; A.n_items_cap = d->cap_items;
     A.n_dense_dev = P<uint32_t>(d->b_misc) + M_NDENSE; A.dense_blocks = 0;
@@ -2201,7 +2242,7 @@ static int enqueue_index(fgss_device_sets *d, int rs)
     const uint32_t *skeys = which ? kb : ka;
     const uint32_t *sidx = which ? ib : ia;
     hipLaunchKernelGGL(k_postings, dim3(grid_for(E, 256)), dim3(256), 0, st, sidx, sid_of, P<uint32_t>(d->b_pl_of),
-                       misc + M_NE, post, psid, fence);
+                       misc + M_NE, post, psid, fence, rs || !dense_ok(d) ? (uint32_t *)nullptr : P<uint32_t>(d->b_pden));
     HIPCK(hipGetLastError());
     hipLaunchKernelGGL(k_list_offsets, dim3(grid_for(E + 1, 256)), dim3(256), 0, st, skeys, misc + M_NE, d->n_terms, loff);
     HIPCK(hipGetLastError());
@@ -2253,15 +2294,17 @@ static int enqueue_stage_plan(fgss_device_sets *d, const RunCfg &cfg)
     // 6-8: plan work items
     hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(grid_for(n * kPlanGroup, 256), 4096)), dim3(256), 0, st,
                        d->tok, d->pos, d->len, n, d->job, d->tj, d->jend, probelen, poff, loff, psid, fence, cfg.target,
-                       cfg.heavy_target, P<uint2>(d->b_rng), hi, nparts, eq, bcnt, cfg.stream_parts, cfg.dense_w,
-                       cfg.dense_pct, (unsigned long long *)(misc + M_NDP64), cfg.block_records);
+                       cfg.heavy_target, P<uint2>(d->b_rng), hi, nparts, eq, bcnt, cfg.stream_parts,
+                       dense_ok(d) ? cfg.dense_w : 0u, cfg.dense_pct, (unsigned long long *)(misc + M_NDP64),
+                       cfg.block_records);
     HIPCK(hipGetLastError());
     const uint32_t blim = std::min(d->cap_bounds, cfg.bounds);
     uint32_t *nitems = P<uint32_t>(d->b_nitems);
     HIPCK(exclusive_scan<uint64_t>(bcnt, bofs, n, (uint64_t *)(misc + M_NB64), scr64, st));
     // dense items (only with dense parts on): their count per query, scanned into the
     // offsets of each query's entries in the dense item list; M_NDENSE their total
-    uint32_t *dnum = cfg.dense_w ? P<uint32_t>(d->b_dnum) : nullptr, *doff = P<uint32_t>(d->b_doff);
+    const uint32_t dense_w = dense_ok(d) ? cfg.dense_w : 0u;
+    uint32_t *dnum = dense_w ? P<uint32_t>(d->b_dnum) : nullptr, *doff = P<uint32_t>(d->b_doff);
     hipLaunchKernelGGL(k_nitems, dim3(grid_for(n, 256)), dim3(256), 0, st, nparts, probelen, bofs, bcnt, blim, n, nitems,
                        dnum, (const unsigned long long *)(misc + M_NDP64));
     HIPCK(hipGetLastError());
@@ -2273,7 +2316,7 @@ static int enqueue_stage_plan(fgss_device_sets *d, const RunCfg &cfg)
     HIPCK(hipGetLastError());
     hipLaunchKernelGGL(k_items, dim3(kItemsGrid), dim3(256), 0, st, 0, (const uint32_t *)nullptr, hi, nparts,
                        probelen, eq, partoff, bofs, bcnt, blim, d->job, d->tj,
-                       d->len, d->pos, poff, n, misc + M_NITEMS, P<Item>(d->b_items), d->cap_items, cfg.dense_w,
+                       d->len, d->pos, poff, n, misc + M_NITEMS, P<Item>(d->b_items), d->cap_items, dense_w,
                        (const unsigned long long *)(misc + M_NDP64), (const uint32_t *)doff, P<uint32_t>(d->b_ditems),
                        (const uint32_t *)d->jend);
     HIPCK(hipGetLastError());
@@ -2282,7 +2325,7 @@ static int enqueue_stage_plan(fgss_device_sets *d, const RunCfg &cfg)
 
 static int enqueue_stage_join(fgss_device_sets *d, const RunCfg &cfg)
 {
-    return launch_join_cap(d, join_args(d), cfg.cap, false, false, cfg.dense_w != 0);   // 9: the join
+    return launch_join_cap(d, join_args(d), cfg.cap, false, false, dense_ok(d) && cfg.dense_w != 0);   // 9: the join
 }
 
 // The first out_hint result triples into pinned memory, behind the join on its stream
@@ -4142,6 +4185,7 @@ static size_t noalloc_carve(fgss_device_sets *d, uint8_t *base, int64_t n, int64
     buf(d->b_sid_of, (size_t)E * 4);
     buf(d->b_pl_of, (size_t)E * 4);
     buf(d->b_post, (size_t)(E + kWave) * 8);
+    buf(d->b_pden, (size_t)(E + kWave) * 4);
     buf(d->b_psid, (size_t)(E + 1) * 4);
     buf(d->b_fence, (size_t)(E / kFence + 2) * 4);
     buf(d->b_loff, ((size_t)n_terms + 2) * 4);
