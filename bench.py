@@ -52,11 +52,14 @@ def log(msg):
 
 def design_bytes(st):
     """Bytes one join's k_join launch must read and write as designed: 4 B per probe
-    token, 8 B per useful posting (record id, position | length), one 64-byte signature
-    line per candidate tested and a 128-byte one per candidate tested again, 4 B per
-    verified source token, 12 B per result triple."""
-    return (4 * st["probe_tokens"] + 8 * st["postings_scanned"] + 64 * st["signature_checks"]
-            + 128 * st["signature_checks_128"] + 4 * st["verify_tokens"] + 12 * st["results"])
+    token, 8 B per useful posting of a hash part (record id, position | length) and 4 B
+    per posting of a dense part (record << 8 | position), one 64-byte signature line per
+    candidate tested and a 128-byte one per candidate tested again, 4 B per verified
+    source token, 12 B per result triple."""
+    dense = st.get("dense_postings", 0)
+    return (4 * st["probe_tokens"] + 8 * (st["postings_scanned"] - dense) + 4 * dense
+            + 64 * st["signature_checks"] + 128 * st["signature_checks_128"] + 4 * st["verify_tokens"]
+            + 12 * st["results"])
 
 
 def pmc_summary(workload):
@@ -175,7 +178,10 @@ def long_leg(fg, args):
     info = coll.info
     log(f"long: {info.n_sets} records, {info.n_tokens} tokens, {info.n_terms} terms (GPU tokenizer {tok_s:.2f} s, "
         f"set-up {time.perf_counter() - t0:.1f} s)")
+    tu = time.perf_counter()
     coll.upload()
+    fg.device_synchronize()
+    upload_s = time.perf_counter() - tu
     coll.selfjoin(t)   # sizes the workspace
     for i in range(args.warmup):
         coll.selfjoin(t)
@@ -194,13 +200,18 @@ def long_leg(fg, args):
     ms_per_step = dt * 1e3 / args.steps
     kernel_ms = sum(r.stats["ms_kernel"] for r in runs) / len(runs)
     out = {"records": int(info.n_sets), "tokens": int(info.n_tokens), "terms": int(info.n_terms),
-           "tokenize_s": tok_s, "ms_per_step": ms_per_step, "join_ms": st["ms_total"], "index_ms": st["ms_index"],
+           "tokenize_s": tok_s, "upload_s": upload_s,
+           "outside_step": "tokenizing and the upload (host-to-device copy of the records and their "
+                           "token-bucket signatures, k_signature: about 1 ms at 1M records) precede the "
+                           "timed steps; a step starts from the HBM-resident records",
+           "ms_per_step": ms_per_step, "join_ms": st["ms_total"], "index_ms": st["ms_index"],
            "kernel_ms": kernel_ms, "pairs": len(p), "candidates": st["candidates"],
            "postings": st["postings_scanned"], "probe_tokens": st["probe_tokens"],
            "index_entries": st["index_entries"], "signature_checks": st["signature_checks"],
            "signature_checks_128": st["signature_checks_128"], "verified": st["verified"],
            "verify_tokens": st["verify_tokens"], "kernel_launches": st["kernel_launches"],
            "phase_cycles": st["phase_cycles"], "window_retries": st["window_retries"],
+           "dense_parts": st["dense_parts"], "dense_postings": st["dense_postings"],
            "repeatable": all(np.array_equal(r.q, p.q) and np.array_equal(r.s, p.s) for r in runs)}
     cb, ctr = None, None
     if not args.no_cpu:
@@ -255,6 +266,7 @@ def titles_leg(fg, args):
                       "index_ms": round(p.stats["ms_index"], 3), "pairs": len(p),
                       "candidates": p.stats["candidates"], "postings": p.stats["postings_scanned"],
                       "sig_checks": p.stats["signature_checks"], "verified": p.stats["verified"],
+                      "dense_parts": p.stats["dense_parts"], "dense_postings": p.stats["dense_postings"],
                       "phase_cycles": p.stats["phase_cycles"]}
              for t, p, km in zip(TITLES_T, last, kms_t)}
     out = {"workload": f"plain self-join, {TITLES_RECORDS} synthetic titles, q=3, t={list(TITLES_T)}; a step is the "

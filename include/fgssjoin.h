@@ -133,6 +133,8 @@ typedef struct {
                                   consecutive parts at a time while the table stays sparse)  */
     int64_t dense_parts;       /* dense parts joined (windows full enough for a table indexed by the
                                   source; see fgss_join_opts.dense_pct)                        */
+    int64_t dense_postings;    /* of postings_scanned, those read by dense parts (4 bytes each;
+                                  the others are 8-byte postings)                              */
 } fgss_join_stats;
 
 typedef struct {

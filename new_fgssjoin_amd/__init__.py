@@ -67,7 +67,8 @@ class JoinStats(C.Structure):
                 ("ms_copy", C.c_double), ("ms_total", C.c_double),
                 ("ms_kernel", C.c_double), ("kernel_launches", C.c_int64),
                 ("signature_checks", C.c_int64), ("signature_checks_128", C.c_int64), ("verified", C.c_int64),
-                ("phase_cycles", C.c_int64 * 6), ("window_retries", C.c_int64), ("dense_parts", C.c_int64)]
+                ("phase_cycles", C.c_int64 * 6), ("window_retries", C.c_int64), ("dense_parts", C.c_int64),
+                ("dense_postings", C.c_int64)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}

@@ -93,6 +93,7 @@ enum Ctr {
     C_CYC_RANGES,    // part of C_CYC_A spent locating posting ranges
     C_RETRY,         // streamed windows of several parts that overflowed and ran again part by part
     C_DENSE,         // dense parts joined
+    C_DPOST,         // postings scanned by dense parts (4-byte postings)
     C_COUNT
 };
 
@@ -132,46 +133,18 @@ constexpr uint32_t kPlanned = 0x40000000u;       // part of a planned split: ran
 constexpr uint32_t kStreamed = 0x20000000u;      // the whole window, walked part by part by one wavefront
 constexpr uint32_t kDense = 0x10000000u;         // consecutive dense parts of the window, walked in order
 constexpr uint32_t kPadMask = 0x0FFFFFFFu;
-// nparts flag of a query joined in dense parts: 2 x table-slots consecutive sources each,
-// counted in a table indexed by the source (no hashing); the count is that of dense parts
+// nparts flag of a query joined in dense parts: the parts of a global grid of 2 x
+// table-slots records that meet its window, each counted in a table indexed by the
+// source (no hashing); the count is that of its dense parts
 constexpr uint32_t kDenseParts = 0x80000000u;
-#ifndef FGSS_DENSE_CHUNK
-#define FGSS_DENSE_CHUNK 64
-#endif
 #ifndef FGSS_DENSE_UNROLL
-#define FGSS_DENSE_UNROLL 4   // rows of kWave postings per step of a dense part's rounds
+#define FGSS_DENSE_UNROLL 4   // rows of kWave postings per step of a dense part's walk
 #endif
-#ifndef FGSS_DENSE_LOGK_BIAS
-#define FGSS_DENSE_LOGK_BIAS 0   // a dense part's first K: 2^bias x its expected postings per token
-#endif
-#ifndef FGSS_DENSE_LOGK_MIN
-#define FGSS_DENSE_LOGK_MIN 3
-#endif
-// dense match counts: 8 bits, four slots a word (1), or 16 bits, two slots a word (0);
-// list ends of a dense item's tokens read from LDS (1, needs 8-bit counts) or from the
-// posting ranges in global memory (0)
-#ifndef FGSS_DENSE_CNT8
-#define FGSS_DENSE_CNT8 1
-#endif
-#ifndef FGSS_DENSE_END_LDS
-#define FGSS_DENSE_END_LDS 1
-#endif
-constexpr uint32_t kDcShift = FGSS_DENSE_CNT8 ? 2u : 1u;   // log2 slots per count word
+// dense match counts: 8 bits, four slots a word
+constexpr uint32_t kDcShift = 2u;   // log2 slots per count word
 constexpr uint32_t kDcSlots = 1u << kDcShift;
 constexpr uint32_t kDcBits = 32u >> kDcShift;
 constexpr uint32_t kDcMask = (1u << kDcBits) - 1u;
-constexpr bool kDenseEndLds = FGSS_DENSE_END_LDS && FGSS_DENSE_CNT8;
-constexpr uint32_t kDenseChunk = FGSS_DENSE_CHUNK;   // dense parts per work item at most
-constexpr uint64_t kDenseItems = 16384;              // dense work items wanted per launch (4x the waves of its role)
-
-// dense parts per work item: up to kDenseChunk (fewer items, less set-up), but small
-// enough that all dense parts of the run make kDenseItems items (a few heavy queries
-// then spread over the grid instead of running one long item per wave)
-__device__ __forceinline__ uint32_t dense_chunk(unsigned long long total_parts)
-{
-    const unsigned long long c = (total_parts + kDenseItems - 1) / kDenseItems;
-    return (uint32_t)(c < 1 ? 1 : c > kDenseChunk ? kDenseChunk : c);
-}
 // streamed items keep one posting cursor per probe token in the wavefront's LDS
 // (the range arrays of the other items)
 constexpr uint32_t kStreamQp = 256;
@@ -227,7 +200,9 @@ struct fgss_device_sets {
     float *tj = nullptr;        // per job: threshold of the current run
     DevBuf b_psid, b_fence, b_bcnt, b_bofs, b_bounds, b_probelen, b_poff, b_rng, b_eq, b_midlen, b_midoff, b_ka,
         b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_pl_of, b_post, b_loff, b_hi, b_nparts, b_partoff, b_scanscr, b_items,
-        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc, b_nitems, b_ovfp, b_wctr, b_jsave, b_ditems, b_dnum, b_doff, b_grp, b_pden;
+        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc, b_nitems, b_ovfp, b_wctr, b_jsave, b_ditems, b_dnum, b_doff, b_grp, b_pden,
+        b_doffs, b_dq, b_dtpre;
+    const uint32_t *skeys = nullptr;   // the last index build's sorted entry terms, one per posting
     uint32_t cap_items = 0, cap_bounds = 0, cap_out = 0;   // current capacities (elements)
     uint32_t cap_ovf = 0;         // overflow records (a streamed item can record one per part)
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -299,7 +274,7 @@ void fgss_device_release(fgss_collection *c)
                       &d->b_sortscr, &d->b_sid_of, &d->b_pl_of, &d->b_post, &d->b_loff, &d->b_hi, &d->b_nparts, &d->b_partoff,
                       &d->b_scanscr, &d->b_items, &d->b_items2, &d->b_ovf, &d->b_out, &d->b_ctr, &d->b_jctr,
                       &d->b_nitems, &d->b_ovfp, &d->b_wctr, &d->b_jsave, &d->b_ditems, &d->b_dnum, &d->b_doff, &d->b_grp, &d->b_pden,
-                      &d->b_misc};
+                      &d->b_doffs, &d->b_dq, &d->b_dtpre, &d->b_misc};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     void *arrs[] = {d->tok, d->len, d->pos, d->sig, d->sig64, d->sigbad, d->job, d->jend, d->tj};
@@ -561,6 +536,85 @@ __global__ void k_list_offsets(const uint32_t *__restrict__ skeys, const uint32_
     for (int64_t tt = prev + 1; tt <= cur; tt++) loff[tt] = i;
 }
 
+// Dense parts follow a global grid of dense_w records: grid part g holds the records
+// [g * dense_w, (g + 1) * dense_w).  The offset table gives, for every term t and grid part
+// g <= ng, the first posting of t's list whose record lies at or past g * dense_w:
+// offs[t * (ng + 1) + g] (the list's end when there is none), so that the postings of t in
+// grid part g are offs[t][g] .. offs[t][g + 1] with no search.  A thread per posting
+// writes the entries of the grid parts that end at it (those past the previous posting's
+// part, up to its own), the last posting of a list also those after it; a thread per term
+// fills the table row of an empty list.  Nothing to do when the run planned no dense parts.
+__global__ __launch_bounds__(256) void k_dense_offs(const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ psid,
+                                                    const uint32_t *__restrict__ loff, const uint32_t *__restrict__ nE_dev,
+                                                    uint32_t n_terms, uint32_t dense_w, uint32_t ng,
+                                                    const unsigned long long *__restrict__ ndparts,
+                                                    uint32_t *__restrict__ offs)
+{
+    if (*ndparts == 0) return;
+    const uint32_t nE = *nE_dev;
+    const size_t row = (size_t)ng + 1;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nE; i += gridDim.x * blockDim.x) {
+        const uint32_t t = skeys[i], g = psid[i] / dense_w;
+        const bool first = i == 0 || skeys[i - 1] != t;
+        const uint32_t g0 = first ? 0u : psid[i - 1] / dense_w + 1u;   // grid parts whose first posting is i
+        uint32_t *o = offs + (size_t)t * row;
+        for (uint32_t gg = g0; gg <= g; gg++) o[gg] = i;
+        if (i + 1 == nE || skeys[i + 1] != t)
+            for (uint32_t gg = g + 1; gg <= ng; gg++) o[gg] = i + 1;
+    }
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n_terms; t += gridDim.x * blockDim.x) {
+        const uint32_t l0 = loff[t];
+        if (l0 != loff[t + 1]) continue;
+        uint32_t *o = offs + (size_t)t * row;
+        for (uint32_t gg = 0; gg <= ng; gg++) o[gg] = l0;
+    }
+}
+constexpr unsigned kDenseOffsGrid = 8192;   // k_dense_offs blocks (grid-stride)
+
+// the dense queries in ascending order: dl[doff[q]] = q where dflag[q] (doff: their scan)
+__global__ void k_dense_list(const uint32_t *__restrict__ dflag, const uint32_t *__restrict__ doff, int64_t n,
+                             const unsigned long long *__restrict__ ndparts, uint32_t *__restrict__ dl)
+{
+    if (*ndparts == 0) return;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n && dflag[q]) dl[doff[q]] = (uint32_t)q;
+}
+
+// Dense work units, in grid-part-major order so that the wavefronts running at a time
+// share one grid part's postings and signatures in L2.  dl lists the ndq dense queries in
+// ascending order (a scan of k_plan's flags); their window ends are non-decreasing (every
+// dense job is in size order, and a job's windows end at its own end), so the dense
+// queries whose window meets grid part g are the entries [lo_g, end_g) of dl: lo_g the
+// first whose window ends past g * dense_w, end_g the first at or past (g + 1) * dense_w
+// - 1 (a query lies before its window).  Each part's entries are cut into tiles of
+// kDenseTile queries: qlo[g] = lo_g and ntl[g] = its tiles (scanned into the tile offsets).
+constexpr uint32_t kDenseTile = 16;   // queries per dense tile (one queue claim)
+__global__ __launch_bounds__(256) void k_dense_tiles(const uint32_t *__restrict__ dl, const uint32_t *__restrict__ ndq_dev,
+                                                     const uint32_t *__restrict__ hi, uint32_t ng, uint32_t dense_w,
+                                                     const unsigned long long *__restrict__ ndparts,
+                                                     uint32_t *__restrict__ qlo, uint64_t *__restrict__ ntl)
+{
+    const uint32_t ndq = *ndparts == 0 ? 0u : *ndq_dev;   // no dense parts: no tiles
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g <= ng; g += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, top = ndq;   // first entry whose window ends past g * dense_w
+        const uint64_t start = (uint64_t)g * dense_w;
+        while (lo < top) {
+            const uint32_t mid = (lo + top) >> 1;
+            if ((uint64_t)hi[dl[mid]] > start) top = mid;
+            else lo = mid + 1;
+        }
+        uint32_t e = lo, etop = ndq;   // first entry at or past (g + 1) * dense_w - 1
+        const uint64_t stop = (uint64_t)(g + 1u) * dense_w - 1u;
+        while (e < etop) {
+            const uint32_t mid = (e + etop) >> 1;
+            if ((uint64_t)dl[mid] >= stop) etop = mid;
+            else e = mid + 1;
+        }
+        qlo[g] = lo;
+        ntl[g] = g < ng ? (e - lo + kDenseTile - 1u) / kDenseTile : 0u;
+    }
+}
+
 // The smallest partner length ls <= l with size_ok(l, ls).  The partner lengths that
 // pass the 4-way size filter form an interval containing l (each of its four bounds
 // is monotone in ls), so on [0, l] the filter is false below the answer and true from
@@ -701,7 +755,7 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
                                               uint32_t *__restrict__ nparts, uint32_t *__restrict__ eq_out,
                                               uint64_t *__restrict__ bcnt, uint32_t stream_parts, uint32_t dense_w,
                                               uint32_t dense_pct, unsigned long long *__restrict__ ndparts,
-                                              uint32_t block_records)
+                                              uint32_t block_records, uint32_t *__restrict__ dflag)
 {
     uint32_t dparts = 0;   // dense parts of this lane's queries
     const int64_t ngroups = (int64_t)gridDim.x * (blockDim.x / kPlanGroup);
@@ -755,12 +809,15 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
             // dense_w consecutive sources to dense_pct% of their slots on average, for a
             // probe that fits the cursor slots (windows this full would need hash-table
             // parts of a few postings per probe token)
+            // (dense parts follow the global grid of dense_w records: the window's first
+            // and last parts are clipped to it)
             const uint32_t win = wend - (uint32_t)q - 1u;
-            const uint32_t ndp = dense_w ? (win + dense_w - 1u) / dense_w : 0u;
+            const uint32_t ndp = dense_w && win ? (wend - 1u) / dense_w - ((uint32_t)q + 1u) / dense_w + 1u : 0u;
             const bool dense = !unsorted && ndp > 1 && np > 1 && qp < kStreamQp &&
                                (uint64_t)e * 100u >= (uint64_t)ndp * dense_w * dense_pct;
             nparts[q] = dense ? (ndp | kDenseParts) : np;
             dparts += dense ? ndp : 0u;
+            if (dflag) dflag[q] = dense ? 1u : 0u;   // scanned into the list of dense queries
             // internal boundaries x probe tokens, for queries whose parts read a planned
             // boundary table (heavier ones are streamed when their probe fits a cursor slot)
             bcnt[q] = !dense && np > 1 && (np <= stream_parts || qp > kStreamQp) ? (uint64_t)(np - 1) * qp : 0ull;
@@ -835,24 +892,20 @@ __global__ __launch_bounds__(256) void k_bounds(int64_t n, const uint32_t *__res
     }
 }
 
-// Items per query (before the scan of their offsets): one streamed item, np planned
-// or searched parts, one whole-window item, or none.
-// dnum (null: no dense items): the dense items among them, whose scan places each
-// query's dense items in the list that the dense lanes of k_join walk.
+// Items per query (before the scan of their offsets): one streamed item per chunk of
+// parts, np planned or searched parts, one whole-window item, or none.  Queries joined in
+// dense parts have no items: the dense role of the join walks their (grid part, query)
+// units instead.
 __global__ void k_nitems(const uint32_t *__restrict__ nparts, const uint32_t *__restrict__ probelen,
                          const uint64_t *__restrict__ bofs, const uint64_t *__restrict__ bcnt, uint32_t cap_bounds,
-                         int64_t n, uint32_t *__restrict__ nitems, uint32_t *__restrict__ dnum,
-                         const unsigned long long *__restrict__ ndparts)
+                         int64_t n, uint32_t *__restrict__ nitems)
 {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t chunk = dense_chunk(*ndparts);
     if (q < n) {
         const uint32_t np = nparts[q];
         const uint32_t m = item_mode(np, probelen[q], bcnt[q], bofs[q], cap_bounds);
-        const uint32_t ni = m == IM_NONE ? 0u : m == IM_WHOLE ? 1u : m == IM_STREAMED ? (np + kStreamChunk - 1) / kStreamChunk
-                          : m == IM_DENSE ? ((np & ~kDenseParts) + chunk - 1) / chunk : np;
-        nitems[q] = ni;
-        if (dnum) dnum[q] = m == IM_DENSE ? ni : 0u;
+        nitems[q] = m == IM_NONE || m == IM_DENSE ? 0u : m == IM_WHOLE ? 1u
+                  : m == IM_STREAMED ? (np + kStreamChunk - 1) / kStreamChunk : np;
     }
 }
 
@@ -872,13 +925,10 @@ __global__ __launch_bounds__(256) void k_items(int rs, const uint32_t *__restric
                                                const float *__restrict__ tj, const uint32_t *__restrict__ len,
                                                const uint32_t *__restrict__ pos, const uint32_t *__restrict__ poff,
                                                int64_t n, const uint32_t *__restrict__ n_items_dev,
-                                               Item *__restrict__ items, uint32_t cap_items, uint32_t dense_w,
-                                               const unsigned long long *__restrict__ ndparts,
-                                               const uint32_t *__restrict__ doff, uint32_t *__restrict__ ditems,
+                                               Item *__restrict__ items, uint32_t cap_items,
                                                const uint32_t *__restrict__ jend)
 {
     const uint32_t total = *n_items_dev;
-    const uint32_t chunk = dense_chunk(*ndparts);
     if (total > cap_items) return;   // too small: the host resizes and reruns
     for (uint32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < total; o += gridDim.x * blockDim.x) {
         int64_t lo = 0, top = n - 1;   // partoff[0] = 0 <= o
@@ -892,20 +942,14 @@ __global__ __launch_bounds__(256) void k_items(int rs, const uint32_t *__restric
         const uint32_t np = nparts[q];
         const uint32_t mode = item_mode(np, probelen[q], bcnt[q], bofs[q], cap_bounds);
         const uint64_t a0 = wlo ? (uint64_t)wlo[q] : (uint64_t)q + 1, W = (uint64_t)hi[q] - a0;
-        const uint32_t npc = np & ~kDenseParts;
-        const uint32_t per = min((eq[q] + npc - 1) / npc, kPadMask);   // expected postings per part: sizes its LDS table
+        const uint32_t per = min((eq[q] + np - 1) / np, kPadMask);   // expected postings per part: sizes its LDS table
         const int64_t r = rs ? q >> 1 : q;           // the record whose tokens probe
         const uint32_t flag = rs && (q & 1) == 0 ? kMidProbe : 0u;
         const uint32_t fl = mode == IM_WHOLE ? kWholeWindow : mode == IM_PLANNED ? kPlanned
-                          : mode == IM_STREAMED ? kStreamed : mode == IM_DENSE ? kDense : 0u;
+                          : mode == IM_STREAMED ? kStreamed : 0u;
         Item it;
         it.q = (uint32_t)r;
-        if (mode == IM_DENSE) {      // chunk p: dense parts [p * chunk, ...) of dense_w sources each
-            const uint32_t p0 = p * chunk, p1 = min(p0 + chunk, npc);
-            it.a = (uint32_t)(a0 + (uint64_t)p0 * dense_w);
-            it.b = (uint32_t)min<uint64_t>(a0 + (uint64_t)p1 * dense_w, (uint64_t)hi[q]);
-            it.nparts = p1 - p0;
-        } else if (mode == IM_STREAMED) {   // chunk p: parts [p * kStreamChunk, ...) of the query's np
+        if (mode == IM_STREAMED) {   // chunk p: parts [p * kStreamChunk, ...) of the query's np
             const uint32_t p0 = p * kStreamChunk, p1 = min(p0 + kStreamChunk, np);
             it.a = part_start(a0, W, p0, np);
             it.b = part_start(a0, W, p1, np);
@@ -924,8 +968,6 @@ __global__ __launch_bounds__(256) void k_items(int rs, const uint32_t *__restric
         it.qpos = pos[r];
         it.qpoff = poff[q];
         items[o] = it;
-        // the dense items' indices, listed apart (in item order) for the lanes that join them
-        if (mode == IM_DENSE) ditems[doff[q] + p] = o;
     }
 }
 
@@ -967,6 +1009,16 @@ struct JoinArgs {
     const uint32_t *loff;
     const uint2 *post;          // postings (record id, token position | length << 16), grouped by term
     const uint32_t *pden;       // the same postings as record << 8 | position, for dense parts (null: none)
+    const uint32_t *doffs;      // dense parts: per term, the first posting of each grid part (k_dense_offs)
+    uint32_t dng;               // grid parts of the dense offset table (its rows hold dng + 1 entries)
+    const uint32_t *dl;         // the dense queries in ascending order (k_dense_list)
+    const uint32_t *ndq_dev;    // their number (device)
+    const uint32_t *dqlo;       // per grid part: the first entry of dl in its dense tiles (k_dense_tiles)
+    const uint64_t *dtpre;      // per grid part: dense tiles of the parts before it
+    const unsigned long long *n_tiles_dev;   // dense tiles of kDenseTile units (device; null: none)
+    const uint32_t *hi;         // per query: the end of its window
+    const uint32_t *nparts;     // per query: its parts (kDenseParts: joined in dense units)
+    int64_t n;                  // records (the queries of a self-join)
     const uint32_t *psid;       // record ids of the postings alone
     const uint32_t *fence;      // psid[f * kFence]
     const uint32_t *sig;        // per record: kSigWords words of byte counts per token bucket
@@ -976,11 +1028,9 @@ struct JoinArgs {
     const uint32_t *poff;       // per query: first entry of its probe tokens in rng
     const uint32_t *bounds;     // planned split boundaries (see k_bounds)
     const Item *items;
-    const uint32_t *ditems;     // indices of the dense items in items (any order; null: none)
     const uint32_t *n_items_dev;  // number of items (device); the launch is void if above n_items_cap
     uint32_t n_items_cap;
-    const uint32_t *n_dense_dev;  // dense items among them (device; null: none)
-    uint32_t dense_blocks;      // blocks of the launch that join the dense items (the first ones)
+    uint32_t dense_blocks;      // blocks of the launch that join the dense units (the first ones)
     const uint32_t *job;        // per record: job id
     const float *tj;            // per job: Jaccard threshold
     unsigned long long *ctr_job;  // per job: kJobCtr counters
@@ -1121,7 +1171,7 @@ constexpr int kUnroll = FGSS_UNROLL;
 constexpr int kQStage = 160;   // query tokens staged in LDS for verification (longer queries stay in HBM)
 // per-wave counters (lane k of one 64-bit register holds counter k): totals,
 // shader-clock phases, the current job's counters
-enum WaveAcc { W_ESCAN = 0, W_CAND, W_SIG, W_VER, W_SIG2, W_INTER, W_VTOK, W_RETRY, W_DENSE, W_CYC0,
+enum WaveAcc { W_ESCAN = 0, W_CAND, W_SIG, W_VER, W_SIG2, W_INTER, W_VTOK, W_RETRY, W_DENSE, W_DPOST, W_CYC0,
                W_JOB0 = W_CYC0 + 6, kAccSlots = W_JOB0 + kJobCtr };
 // per-wavefront slot in global memory at the end of a launch: the kAccSlots
 // counters and the wavefront's last job
@@ -1161,15 +1211,16 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
     // a launch whose item list or boundary table did not fit does nothing: the host
     // sees the totals, grows the buffers and runs the pipeline again
     if (*A.n_items_dev > A.n_items_cap) return;
-    // the dense role walks the list of dense item indices, the other role every item
-    // (skipping the dense ones)
-    const uint32_t n_items = DENSE ? (A.n_dense_dev == nullptr || A.ditems == nullptr ? 0u : *A.n_dense_dev) : *A.n_items_dev;
+    // the dense role walks the dense work units (grid part, query), the other role the
+    // item list
+    const uint64_t n_items = DENSE ? (A.n_tiles_dev == nullptr ? 0ull : *A.n_tiles_dev * kDenseTile) : (uint64_t)*A.n_items_dev;
+    const uint32_t ndq = DENSE && A.n_tiles_dev != nullptr ? *A.ndq_dev : 0u;
     {   // a wavefront with no static chunk and no queue to claim from (the dense role of a
-        // run with few or no dense items) writes a zero counter slot and leaves at once
-        const uint32_t nwg = rblocks * (blockDim.x / kWave);
-        const uint32_t g = min((n_items + nwg - 1) / nwg, (uint32_t)kQueueBatch);
-        const uint32_t gwi = rblock * (blockDim.x / kWave) + (uint32_t)w;
-        if ((uint64_t)gwi * g >= n_items && (uint64_t)nwg * g >= n_items) {
+        // run with few or no dense parts) writes a zero counter slot and leaves at once
+        const uint64_t nwg = (uint64_t)rblocks * (blockDim.x / kWave);
+        const uint64_t g = min((n_items + nwg - 1) / nwg, (uint64_t)kQueueBatch);
+        const uint64_t gwi = (uint64_t)rblock * (blockDim.x / kWave) + (uint32_t)w;
+        if (gwi * g >= n_items && nwg * g >= n_items) {
             unsigned long long *slot = A.wctr + ((size_t)blockIdx.x * join_waves(CAP) + (size_t)w) * kWaveSlot;
             if (lane < kAccSlots) slot[lane] = 0ull;
             if (lane == kAccSlots) slot[lane] = kEmpty;
@@ -1177,7 +1228,13 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
         }
     }
 
-    for (uint32_t i = lane; i < CAP; i += kWave) { tkey[i] = kEmpty; tcnt[i] = 0; tlast[i] = 0; }
+    // the hash table starts empty; the dense role keeps its LDS in the dense state for
+    // good (every part leaves its slots zero again)
+    for (uint32_t i = lane; i < CAP; i += kWave) {
+        tkey[i] = DENSE ? 0u : kEmpty;
+        tcnt[i] = 0;
+        tlast[i] = 0;
+    }
     wave_sync();
 // shader-clock time per phase, summed per wavefront (FGSS_PHASE_CLOCKS=0 builds
 // without the clock reads and their LDS updates)
@@ -1212,47 +1269,88 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
     // items (a small served batch) spreads over every wavefront without atomics; the
     // items past the static chunks are claimed kQueueBatch at a time, one global
     // atomic per claim (claims of fewer items contend on the counter).
-    const uint32_t nwaves_grid = rblocks * (blockDim.x / kWave);
-    const uint32_t g0 = min((n_items + nwaves_grid - 1) / nwaves_grid, (uint32_t)kQueueBatch);
-    const uint32_t qbase = nwaves_grid * g0;   // first item of the dynamic claims
-    const uint32_t gw = rblock * (blockDim.x / kWave) + (uint32_t)w;
-    uint32_t batch_next = min(gw * g0, n_items), batch_end = min(gw * g0 + g0, n_items);
+    const uint64_t nwaves_grid = (uint64_t)rblocks * (blockDim.x / kWave);
+    const uint64_t g0 = min((n_items + nwaves_grid - 1) / nwaves_grid, (uint64_t)kQueueBatch);
+    const uint64_t qbase = nwaves_grid * g0;   // first item of the dynamic claims
+    const uint64_t gw = (uint64_t)rblock * (blockDim.x / kWave) + (uint32_t)w;
+    uint64_t batch_next = min(gw * g0, n_items), batch_end = min(gw * g0 + g0, n_items);
     uint32_t cur_job = kEmpty;
     // the next item of the claimed batch is loaded while the current one runs
     uint32_t nxtw = 0;
     bool have_nxt = false;
+    // dense units: the grid part whose tiles were looked up last, and its tile range
+    uint32_t ugp = 0;
+    uint64_t utlo = 1, uthi = 0;
+    const uint32_t dng = DENSE ? (uint32_t)((A.n + 2 * CAP - 1) / (2 * CAP)) : 0u;   // grid parts
     for (;;) {
         if (batch_next >= batch_end) {
             if (qbase >= n_items) break;   // every item was in a static chunk
-            uint32_t b0 = 0;
-            if (lane == 0) b0 = atomicAdd((unsigned int *)&A.ctr[DENSE ? C_QUEUE_D : C_QUEUE], (unsigned)kQueueBatch);
-            b0 = qbase + __builtin_amdgcn_readfirstlane(b0);
-            batch_next = min(b0, n_items);
-            batch_end = min(b0 + (uint32_t)kQueueBatch, n_items);
+            unsigned long long b0 = 0;
+            if (lane == 0) b0 = atomicAdd(&A.ctr[DENSE ? C_QUEUE_D : C_QUEUE], (unsigned long long)kQueueBatch);
+            b0 = qbase + (((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(b0 >> 32)) << 32) |
+                          (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)b0));
+            batch_next = min((uint64_t)b0, n_items);
+            batch_end = min((uint64_t)b0 + kQueueBatch, n_items);
             have_nxt = false;
         }
         if (batch_next >= n_items) break;
-        const uint32_t it = batch_next++;
-        // an item is kItemWords words, lane k holding word k (one register for the
-        // prefetched item instead of kItemWords)
-        const uint32_t ldw = have_nxt ? nxtw : item_word(A.items, DENSE ? A.ditems[it] : it, lane);
-        have_nxt = batch_next < batch_end;
-        if (have_nxt) nxtw = item_word(A.items, DENSE ? A.ditems[batch_next] : batch_next, lane);
-        // the item is the same for every lane: its fields live in scalar registers
+        const uint64_t it = batch_next++;
         Item item;
-        item.q = __builtin_amdgcn_readlane(ldw, 0);
-        item.a = __builtin_amdgcn_readlane(ldw, 1);
-        item.b = __builtin_amdgcn_readlane(ldw, 2);
-        item.pad = __builtin_amdgcn_readlane(ldw, 3);
-        item.part = __builtin_amdgcn_readlane(ldw, 4);
-        item.nparts = __builtin_amdgcn_readlane(ldw, 5);
-        item.bofs = __builtin_amdgcn_readlane(ldw, 6);
-        item.job = __builtin_amdgcn_readlane(ldw, 7);
-        item.t = __int_as_float(__builtin_amdgcn_readlane(ldw, 8));
-        item.lq = __builtin_amdgcn_readlane(ldw, 9);
-        item.qpos = __builtin_amdgcn_readlane(ldw, 10);
-        item.qpoff = __builtin_amdgcn_readlane(ldw, 11);
-        if (((item.pad & kDense) != 0) != DENSE) continue;   // the other kernel's item
+        if constexpr (DENSE) {
+            // unit it: dense query it % kDenseTile of tile it / kDenseTile in part-major
+            // order (k_dense_tiles); the grid part holding the tile is searched only when
+            // the tile leaves the last part's range.  The tile's last units may lie past
+            // the part's queries: their windows do not meet it.
+            const uint64_t tile = it / kDenseTile;
+            if (tile < utlo || tile >= uthi) {
+                uint32_t lo = 0, hi = dng;   // the last part g with tpre[g] <= tile
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (A.dtpre[mid] <= tile) lo = mid;
+                    else hi = mid;
+                }
+                ugp = lo;
+                utlo = A.dtpre[lo];
+                uthi = A.dtpre[lo + 1];
+            }
+            const uint64_t di = (uint64_t)A.dqlo[ugp] + (tile - utlo) * kDenseTile + it % kDenseTile;
+            if (di >= ndq) continue;
+            const uint32_t qd = A.dl[di];
+            constexpr uint32_t DWu = 2u * CAP;
+            const uint32_t ua = max(qd + 1u, ugp * DWu), ub = min(A.hi[qd], (ugp + 1u) * DWu);
+            if (ua >= ub) continue;   // the query's window does not meet this grid part
+            item.q = qd;
+            item.a = ua;
+            item.b = ub;
+            item.pad = kDense;
+            item.part = 0;
+            item.nparts = 1;
+            item.bofs = 0;
+            item.job = A.job[qd];
+            item.t = A.tj[item.job];
+            item.lq = A.len[qd];
+            item.qpos = A.pos[qd];
+            item.qpoff = 0;
+        } else {
+            // an item is kItemWords words, lane k holding word k (one register for the
+            // prefetched item instead of kItemWords)
+            const uint32_t ldw = have_nxt ? nxtw : item_word(A.items, (uint32_t)it, lane);
+            have_nxt = batch_next < batch_end;
+            if (have_nxt) nxtw = item_word(A.items, (uint32_t)batch_next, lane);
+            // the item is the same for every lane: its fields live in scalar registers
+            item.q = __builtin_amdgcn_readlane(ldw, 0);
+            item.a = __builtin_amdgcn_readlane(ldw, 1);
+            item.b = __builtin_amdgcn_readlane(ldw, 2);
+            item.pad = __builtin_amdgcn_readlane(ldw, 3);
+            item.part = __builtin_amdgcn_readlane(ldw, 4);
+            item.nparts = __builtin_amdgcn_readlane(ldw, 5);
+            item.bofs = __builtin_amdgcn_readlane(ldw, 6);
+            item.job = __builtin_amdgcn_readlane(ldw, 7);
+            item.t = __int_as_float(__builtin_amdgcn_readlane(ldw, 8));
+            item.lq = __builtin_amdgcn_readlane(ldw, 9);
+            item.qpos = __builtin_amdgcn_readlane(ldw, 10);
+            item.qpoff = __builtin_amdgcn_readlane(ldw, 11);
+        }
         const uint32_t q = item.q;
         const uint32_t jq = item.job;
         const float t = item.t;
@@ -1298,33 +1396,24 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
         // token (LDS), starting at the token's first posting inside the item's window
         uint32_t *cur = rlo;
         static_assert(2 * kTokBatch + 1 >= (int)kStreamQp, "cursors live in rlo and rpre");
-        // A dense item walks its nparts dense parts of DW consecutive sources in order,
-        // with cursors as a streamed item.  Its table is indexed by the source's offset in
-        // the part: dlast (the key and count words of the hash table) and dcnt (8-bit
-        // counts, the first quarter of the last-match words); tend (the next quarter)
-        // holds each token's list end inside the window and nsid (the last kStreamQp words
-        // of the list) the record at its cursor, so that a part rounds only over the tokens
-        // with postings in it.  The hash table's empty state is restored at the end.
+        // A dense unit joins its query against one part of the global grid of DW records
+        // (its window's part [a, b)).  Its table is indexed by the source's offset in the
+        // part: dlast (the key and count words of the hash table) and dcnt (8-bit counts,
+        // the first quarter of the last-match words); the dense role keeps them zero
+        // between units.
         constexpr uint32_t DW = 2u * CAP;
         constexpr uint32_t kDenseList = CAP > kStreamQp ? CAP - kStreamQp : 0u;   // candidates listed at once
-        static_assert(CAP < 512 || CAP / 2 >= kStreamQp, "list ends fit the last-match words");
+        static_assert(CAP < 512 || DW <= 0x3FFFu, "dense part slots fit 14 bits");
         constexpr bool dense = DENSE;
-        uint32_t *const dlast = tkey, *const dcnt = tlast, *const tend = tlast + CAP / 2, *const nsid = clist + kDenseList;
-        if (streamed || dense) {
+        uint32_t *const dlast = tkey, *const dcnt = tlast;
+        if (streamed) {
             for (uint32_t j = lane; j < qp; j += kWave) {
                 const uint2 r = qrng[j];
-                const uint32_t c = item.part == 0 ? r.x : lower_bound_sid(A.psid, A.fence, r.x, r.y, item.a);
-                cur[j] = c;
-                if (dense) {
-                    nsid[j] = c < r.y ? A.psid[c] : kEmpty;
-                    if (kDenseEndLds) tend[j] = r.y;
-                }
+                cur[j] = item.part == 0 ? r.x : lower_bound_sid(A.psid, A.fence, r.x, r.y, item.a);
             }
-            if (dense)
-                for (uint32_t i = lane; i < DW; i += kWave) dlast[i] = 0u;
             wave_sync();
         }
-        const uint32_t nwin = streamed || dense ? item.nparts : 1u;
+        const uint32_t nwin = streamed ? item.nparts : 1u;
         PHASE_MARK(0);
 
         // a streamed item joins its parts a window of `span` consecutive parts at a
@@ -1340,10 +1429,7 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
         // size order): lane k < kDenseSamples samples the length at k x DW / 32 (the last
         // one at the part's end).  The load is issued here and stored after the rounds.
         uint32_t lsamp = 0;
-        if (dense) {
-            p1 = p0 + 1;
-            wa = item.a + p0 * DW;
-            wb = min(wa + DW, item.b);
+        if (dense) {   // the unit's part of its grid part, [a, b)
             if (lane < (int)kDenseSamples) lsamp = A.len[wa + min((uint32_t)lane * (DW / 32u), wb - wa - 1u)];
         } else if (streamed) {
             const uint64_t W = (uint64_t)item.b - item.a;
@@ -1477,7 +1563,105 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
             }
             wave_sync();
         }
-        if (streamed || dense) {
+        if constexpr (DENSE) {
+            // ---- A (dense): a probe token's postings in the part are offs[t][g] ..
+            // offs[t][g + 1] of the grid's offset table (k_dense_offs), from the query's
+            // successor on in its window's first part (a search inside them), so every
+            // count is known.  The counts are scanned (tokens lane * PER + k, in token
+            // order) and the part's postings walked flattened as in the hash path:
+            // segment marks and a max-scan hand each posting its token, and cur holds
+            // each token's posting index of flattened position 0.  Postings at or past wb
+            // (the window's end inside its last grid part) are read but take no part. ----
+            constexpr int PER = kStreamQp / kWave;
+            const uint32_t g = wa / DW;   // the part's grid index
+            const uint32_t *orow = A.doffs + g;
+            const size_t ostride = (size_t)dng + 1u;
+            uint32_t dc[PER], dbeg[PER], dlo[PER], sum = 0;
+#pragma unroll
+            for (int k = 0; k < PER; k++) {
+                const uint32_t j = (uint32_t)lane * PER + k;
+                uint32_t lo = 0, hi = 0;
+                if (j < qp) {
+                    const uint32_t *o = orow + (size_t)A.qtok[qpos + j] * ostride;
+                    lo = o[0];
+                    hi = o[1];
+                }
+                dlo[k] = lo;
+                dc[k] = hi - lo;
+            }
+            if (wa > g * DW) {   // the window's first part: from the query's successor on
+#pragma unroll
+                for (int k = 0; k < PER; k++) {
+                    if (dc[k] != 0) {
+                        const uint32_t hi = dlo[k] + dc[k];
+                        dlo[k] = lower_bound_sid(A.psid, A.fence, dlo[k], hi, wa);
+                        dc[k] = hi - dlo[k];
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < PER; k++) sum += dc[k];
+            const uint32_t inc = wave_inclusive_scan_dpp(sum);
+            uint32_t run0 = inc - sum;
+#pragma unroll
+            for (int k = 0; k < PER; k++) {
+                dbeg[k] = run0;
+                run0 += dc[k];
+                cur[(uint32_t)lane * PER + k] = dlo[k] - dbeg[k];
+            }
+            const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
+#ifdef FGSS_DEBUG_FILL
+            WACC(W_RETRY, (total + kWave - 1u) & ~(kWave - 1u));   // debug builds: lanes issued
+#endif
+            wave_sync();
+            uint32_t *scr = qstage;                         // a word per lane for updates without work
+            uint8_t *smark = (uint8_t *)(qstage + kWave);   // a step's segment marks (a byte per position)
+            static_assert(FGSS_DENSE_UNROLL * kWave <= 4 * (kQStage - kWave), "a step's marks fit the query stage");
+            static_assert(kStreamQp <= 256, "segment indices fit a byte");
+            uint32_t segc = 0;   // segment of the last posting of the previous step
+            auto dstep = [&](auto rows, uint32_t e0) {
+                constexpr int N = decltype(rows)::value;
+                uint32_t e[N], seg[N];
+                uint2 pe[N];
+#pragma unroll
+                for (int k = 0; k < N; k++) e[k] = min(e0 + (uint32_t)(k * kWave) + lane, total - 1u);
+#pragma unroll
+                for (int i = 0; i < N / 4 + (N % 4 != 0); i++) ((uint32_t *)smark)[i * kWave + lane] = 0u;
+#pragma unroll
+                for (int k = 0; k < PER; k++) {
+                    const uint32_t at = dbeg[k] - e0;   // wraps above the step for starts before e0
+                    const bool mk = dc[k] != 0 && at < (uint32_t)(N * kWave);
+                    *(mk ? smark + at : (uint8_t *)(scr + lane)) = (uint8_t)(lane * PER + k);
+                }
+                wave_sync();
+                uint32_t c = segc;
+#pragma unroll
+                for (int k = 0; k < N; k++) {
+                    const uint32_t m = wave_max_scan((uint32_t)smark[k * kWave + lane]);
+                    seg[k] = max(m, c);
+                    c = max(c, (uint32_t)__builtin_amdgcn_readlane((int)m, kWave - 1));
+                }
+                segc = c;
+                uint32_t valm = 0;
+#pragma unroll
+                for (int k = 0; k < N; k++) {
+                    const uint32_t w = A.pden[cur[seg[k]] + e[k]];   // 4-byte postings: record, position
+                    pe[k] = make_uint2(w >> 8, w & 0xFFu);
+                    const bool v = e0 + (uint32_t)(k * kWave) + (uint32_t)lane < total && pe[k].x < wb;
+                    escan += (uint32_t)__popcll(__ballot(v));
+                    valm |= (uint32_t)v << k;
+                }
+                dense_insert(pe, valm, seg, wa, dlast, dcnt, scr, lane);
+            };
+            constexpr int NS = FGSS_DENSE_UNROLL;
+            for (uint32_t e0 = 0; e0 < total;) {
+                const uint32_t rem = total - e0;   // wave-uniform
+                if (rem > (uint32_t)(NS - 1) * kWave) { dstep(std::integral_constant<int, NS>(), e0); e0 += kWave * NS; }
+                else if (rem > (uint32_t)kWave) { dstep(std::integral_constant<int, 2>(), e0); e0 += 2 * kWave; }
+                else { dstep(std::integral_constant<int, 1>(), e0); e0 += kWave; }
+            }
+            wave_sync();
+        } else if (streamed) {
             // A streamed part [wa, wb): every probe token with postings left takes the
             // next K of them from its cursor (independent loads, no search).  Those
             // before wb are this part's; the group of K lanes of a token moves its
@@ -1503,13 +1687,8 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                     jv[k] = toks[ec >> logK];
                     curj[k] = cur[jv[k]];
                     idx[k] = curj[k] + (ec & (K - 1));
-                    if constexpr (DENSE) {   // 4-byte postings: record, position (no length)
-                        const uint32_t w = A.pden[idx[k]];
-                        pe[k] = make_uint2(w >> 8, w & 0xFFu);
-                    } else {
-                        pe[k] = A.post[idx[k]];
-                    }
-                    end[k] = dense && kDenseEndLds ? tend[jv[k]] : qrng[jv[k]].y;
+                    pe[k] = A.post[idx[k]];
+                    end[k] = qrng[jv[k]].y;
                 }
                 uint32_t valm = 0;
 #pragma unroll
@@ -1528,10 +1707,6 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                     const bool lead = e < total && ((uint32_t)lane & (K - 1)) == 0 && cnt > 0;
                     *(lead ? &cur[jv[k]] : &sscr[lane]) = curj[k] + cnt;
                     atomicOr(lead && cnt == K ? &satm[jv[k] >> 5] : &sscr[lane], 1u << (jv[k] & 31));
-                    // dense: the first lane of the group past the part holds the record at
-                    // the token's new cursor (none when the list ended there)
-                    const bool next = dense && e < total && ((uint32_t)lane & (K - 1)) == cnt;
-                    *(next ? &nsid[jv[k]] : &sscr[lane]) = idx[k] < end[k] ? pe[k].x : kEmpty;
                     uint32_t v = below;
                     if (RS) {   // the source's own prefix at t
                         const uint32_t ls = pe[k].y >> 16;
@@ -1544,19 +1719,10 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                     }
                     valm |= v << k;
                 }
-                if (dense) {
-                    dense_insert(pe, valm, jv, wa, dlast, dcnt, sscr, lane);
-                    return false;
-                }
                 return table_insert(pe, valm, jv, tkey, tcnt, tlast, clist, sscr, lane, tlog, tsize, nlist);
             };
             auto pass = [&]() -> bool {   // true when the hash table filled up
-#ifdef FGSS_DEBUG_FILL
-                if (dense) WACC(W_RETRY, total);
-#endif
-                // rows per full step: a dense part's rounds wait on their posting loads, so
-                // its steps may hold more rows in flight than the hash path's
-                constexpr int NS = DENSE ? FGSS_DENSE_UNROLL : kUnroll;
+                constexpr int NS = kUnroll;
                 for (uint32_t e0 = 0; e0 < total;) {
                     const uint32_t rem = total - e0;   // wave-uniform
                     bool full;
@@ -1571,7 +1737,7 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                 uint32_t nact = 0;
                 for (uint32_t j0 = 0; j0 < qp; j0 += kWave) {
                     const uint32_t j = j0 + lane;
-                    const bool act = j < qp && (dense ? nsid[j] < wb : cur[j] < qrng[j].y);
+                    const bool act = j < qp && cur[j] < qrng[j].y;
                     const uint64_t m = __ballot(act);
                     if (act) toks[nact + (uint32_t)__popcll(m & lanemask_lt())] = (uint8_t)j;
                     nact += (uint32_t)__popcll(m);
@@ -1580,11 +1746,8 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                 // first round: K = 8 lanes per token.  A round waits for the cursor moves of
                 // the one before it, so fewer rounds beat fewer idle lanes up to a point
                 // (1M long records at t = 0.8, about 2.4 postings per token and part:
-                // K = 4 2.11 s, 8 2.02 s, 16 2.51 s; starting at 1 2.59 s).  A dense part
-                // starts at the power of two at or above its expected postings per token.
+                // K = 4 2.11 s, 8 2.02 s, 16 2.51 s; starting at 1 2.59 s).
                 logK = 3;
-                if (dense && nact > 0)
-                    logK = (uint32_t)min(6, max(FGSS_DENSE_LOGK_MIN, ceil_log2(((item.pad & kPadMask) + nact - 1) / nact) + FGSS_DENSE_LOGK_BIAS));
                 while (nact > 0 && !overflow) {
                     if (lane < (int)(kStreamQp / 32)) satm[lane] = 0;
                     wave_sync();
@@ -1595,7 +1758,7 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                     nact = 0;
                     for (uint32_t j0 = 0; j0 < qp; j0 += kWave) {
                         const uint32_t j = j0 + lane;
-                        const bool act = j < qp && ((satm[j >> 5] >> (j & 31)) & 1u);
+                        const bool act = j < qp && ((satm[j >> 5] >> (j & 31)) & 1u) != 0;
                         const uint64_t m = __ballot(act);
                         wave_sync();
                         if (act) toks[nact + (uint32_t)__popcll(m & lanemask_lt())] = (uint8_t)j;
@@ -1633,7 +1796,7 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
             if (lane == 0) {
                 uint32_t k = (uint32_t)atomicAdd(&A.ctr[C_OVERFLOW], 1ull);
                 if (k < A.ovf_cap) {
-                    A.ovf[k] = it;
+                    A.ovf[k] = (uint32_t)it;
                     A.ovfp[k] = streamed ? p0 : kEmpty;
                 }
             }
@@ -1917,18 +2080,12 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
         WACC(W_VTOK, vtok);
         if (dense) {
             WACC(W_DENSE, 1);
+            WACC(W_DPOST, escan);
             continue;
         }
         if (nlist * 100 <= tsize * kSpanGrowPct) span = min(span * 2, kMaxSpan);
         else if (nlist * 100 > tsize * kSpanShrinkPct) span = max(span / 2, 1u);
         }   // windows
-        if (dense) {   // every dense slot is zero again: the hash table's empty keys and its
-                       // last-match words under the list ends back
-            for (uint32_t i = lane; i < CAP; i += kWave) tkey[i] = kEmpty;
-            if (kDenseEndLds)
-                for (uint32_t i = lane; i < CAP / 2; i += kWave) tend[i] = 0u;
-            wave_sync();
-        }
     }
     // this wavefront's totals and its last job's pending counters to its own slot:
     // one coalesced store, no atomics (thousands of wavefronts adding to the same
@@ -1964,7 +2121,7 @@ __global__ __launch_bounds__(256) void k_wave_reduce(const unsigned long long *_
 {
     constexpr int kMain = W_JOB0;   // global counters per slot
     const int dst[kMain] = {C_ESCAN, C_CAND, C_SIGCHK, C_VERIFIED, C_SIG2, C_INTER, C_VTOK, C_RETRY, C_DENSE,
-                            C_CYC_SETUP, C_CYC_SETUP + 1, C_CYC_SETUP + 2, C_CYC_SETUP + 3, C_CYC_SETUP + 4,
+                            C_DPOST, C_CYC_SETUP, C_CYC_SETUP + 1, C_CYC_SETUP + 2, C_CYC_SETUP + 3, C_CYC_SETUP + 4,
                             C_CYC_RANGES};
     const uint32_t wv = blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = wv < nwaves;
@@ -2019,7 +2176,7 @@ static int launch_join(fgss_device_sets *d, const JoinArgs &A0, bool dry, bool w
     }
     if (dry) return FGSS_OK;
     JoinArgs A = A0;
-    A.dense_blocks = with_dense && CAP >= 512 && !RS && A.n_dense_dev != nullptr ? (uint32_t)cached[0] : 0u;
+    A.dense_blocks = with_dense && CAP >= 512 && !RS && A.n_tiles_dev != nullptr ? (uint32_t)cached[0] : 0u;
     const uint32_t blocks = (uint32_t)cached[0] + A.dense_blocks;
     const uint32_t nw = blocks * join_waves(CAP);
     if ((uint64_t)nw * kWaveSlot * 8 > d->b_wctr.cap)
@@ -2061,7 +2218,7 @@ static inline unsigned grid_for(int64_t n, int threads) { return (unsigned)std::
 // device totals written by the scans (indices into b_misc); M_NDP64 (two words) = dense
 // parts of the run
 // device-side counts; M_NB64 (two words, 8-byte aligned) = boundary-table rows wanted
-enum Misc { M_NE = 0, M_NITEMS, M_NP, M_NRESPLIT, M_NDENSE, M_NB64 = 6, M_NDP64 = 8, M_COUNT = 16 };
+enum Misc { M_NE = 0, M_NITEMS, M_NP, M_NRESPLIT, M_NDENSE, M_NB64 = 6, M_NDP64 = 8, M_NDU64 = 10, M_COUNT = 16 };
 constexpr uint32_t kBoundsBudget = 1u << 28;   // boundary-table entries at most (1 GiB)
 constexpr unsigned kBoundsGrid = 2048;          // k_bounds blocks (the table size is known on the device only)
 
@@ -2135,8 +2292,22 @@ static int wave_slots(fgss_device_sets *d)
 
 // Every buffer the pipeline touches, sized for upper bounds where one exists (index
 // entries and probe tokens <= tokens) and for the current capacities otherwise.
-// dense parts need record ids below 2^24 (their 4-byte postings)
+// dense parts need record ids below 2^24 (their 4-byte postings) and an offset table of
+// terms x grid parts words (k_dense_offs) within kDenseTableBudget at the finest grid
 static bool dense_ok(const fgss_device_sets *d) { return d->n < ((int64_t)1 << 24); }
+constexpr uint32_t kDenseGridMin = 1024;                   // records per grid part at 512 table slots
+constexpr int64_t kDenseTableBudget = (int64_t)2 << 30;   // bytes
+static int64_t dense_grid_parts(const fgss_device_sets *d, uint32_t dw) { return (d->n + dw - 1) / dw; }
+static bool dense_table_fits(const fgss_device_sets *d)
+{
+    return (int64_t)d->n_terms * (dense_grid_parts(d, kDenseGridMin) + 1) * 4 <= kDenseTableBudget;
+}
+// the run's dense grid (records per grid part; 0: no dense parts)
+static uint32_t dense_w_of(const fgss_device_sets *d, const RunCfg &cfg)
+{
+    return dense_ok(d) && dense_table_fits(d) ? cfg.dense_w : 0u;
+}
+
 
 static int prepare_buffers(fgss_device_sets *d)
 {
@@ -2159,6 +2330,12 @@ static int prepare_buffers(fgss_device_sets *d)
     // token's list before it tests them against the end
     GROW(d->b_post, (size_t)(E + kWave) * 8);
     if (dense_ok(d)) GROW(d->b_pden, (size_t)(E + kWave) * 4);
+    if (dense_ok(d) && dense_table_fits(d)) {   // sized for the finest grid (the largest table)
+        const int64_t ng = dense_grid_parts(d, kDenseGridMin);
+        GROW(d->b_doffs, std::max<size_t>((size_t)d->n_terms * (size_t)(ng + 1) * 4, 16));
+        GROW(d->b_dq, (size_t)(ng + 1) * 4);
+        GROW(d->b_dtpre, (size_t)(ng + 1) * 8);
+    }
     GROW(d->b_psid, (size_t)(E + 1) * 4);
     GROW(d->b_fence, (size_t)(E / kFence + 2) * 4);
     GROW(d->b_loff, ((size_t)d->n_terms + 2) * 4);
@@ -2175,7 +2352,7 @@ static int prepare_buffers(fgss_device_sets *d)
     GROW(d->b_doff, (size_t)(n + 1) * 4);
     GROW(d->b_bounds, (size_t)d->cap_bounds * 4);
     GROW(d->b_items, (size_t)d->cap_items * sizeof(Item));
-    GROW(d->b_ditems, (size_t)d->cap_items * 4);
+    GROW(d->b_ditems, (size_t)std::max<int64_t>(d->cap_items, n) * 4);   // the dense query list
     GROW(d->b_grp, 8);
     d->cap_ovf = std::max(d->cap_ovf, d->cap_items);
     GROW(d->b_ovf, (size_t)d->cap_ovf * 4);
@@ -2201,9 +2378,13 @@ static JoinArgs join_args(fgss_device_sets *d)
     A.qtok = d->tok; A.qsig = d->sig; A.qsig64 = d->sig64; A.qsigbad = d->sigbad;
     A.rng = P<uint2>(d->b_rng); A.poff = P<uint32_t>(d->b_poff); A.bounds = P<uint32_t>(d->b_bounds);
     A.pden = dense_ok(d) ? P<uint32_t>(d->b_pden) : nullptr;
-    A.items = P<Item>(d->b_items); A.ditems = P<uint32_t>(d->b_ditems);
+    A.doffs = P<uint32_t>(d->b_doffs); A.dqlo = P<uint32_t>(d->b_dq); A.dtpre = P<uint64_t>(d->b_dtpre);
+    A.dl = P<uint32_t>(d->b_ditems); A.ndq_dev = P<uint32_t>(d->b_misc) + M_NDENSE;
+    A.n_tiles_dev = d->b_doffs.p ? (const unsigned long long *)(P<uint32_t>(d->b_misc) + M_NDU64) : nullptr;
+    A.hi = P<uint32_t>(d->b_hi); A.nparts = P<uint32_t>(d->b_nparts); A.n = d->n;
+    A.items = P<Item>(d->b_items);
     A.n_items_dev = P<uint32_t>(d->b_misc) + M_NITEMS; A.n_items_cap = d->cap_items;
-    A.n_dense_dev = P<uint32_t>(d->b_misc) + M_NDENSE; A.dense_blocks = 0;
+    A.dense_blocks = 0;
     A.job = d->job; A.tj = d->tj; A.ctr_job = P<unsigned long long>(d->b_jctr);
     A.out = P<int32_t>(d->b_out); A.out_cap = d->cap_out; A.ovf = P<uint32_t>(d->b_ovf);
     A.ovfp = P<uint32_t>(d->b_ovfp); A.ovf_cap = d->cap_ovf;
@@ -2241,6 +2422,7 @@ static int enqueue_index(fgss_device_sets *d, int rs)
     HIPCK(radix_sort_pairs(ka, ia, kb, ib, E, misc + M_NE, bits, P<uint32_t>(d->b_sortscr), st, &which));
     const uint32_t *skeys = which ? kb : ka;
     const uint32_t *sidx = which ? ib : ia;
+    d->skeys = skeys;
     hipLaunchKernelGGL(k_postings, dim3(grid_for(E, 256)), dim3(256), 0, st, sidx, sid_of, P<uint32_t>(d->b_pl_of),
                        misc + M_NE, post, psid, fence, rs || !dense_ok(d) ? (uint32_t *)nullptr : P<uint32_t>(d->b_pden));
     HIPCK(hipGetLastError());
@@ -2291,33 +2473,47 @@ static int enqueue_stage_plan(fgss_device_sets *d, const RunCfg &cfg)
     uint32_t *hi = P<uint32_t>(d->b_hi), *eq = P<uint32_t>(d->b_eq), *nparts = P<uint32_t>(d->b_nparts),
              *partoff = P<uint32_t>(d->b_partoff);
     uint64_t *bcnt = P<uint64_t>(d->b_bcnt), *bofs = P<uint64_t>(d->b_bofs), *scr64 = P<uint64_t>(d->b_scanscr);
-    // 6-8: plan work items
+    // 6-8: plan work items; with dense parts on, k_plan flags the dense queries
+    const uint32_t dense_w = dense_w_of(d, cfg);
+    const uint32_t ng = dense_w ? (uint32_t)dense_grid_parts(d, dense_w) : 0u;
+    uint32_t *dflag = P<uint32_t>(d->b_dnum), *doff = P<uint32_t>(d->b_doff), *dl = P<uint32_t>(d->b_ditems);
     hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(grid_for(n * kPlanGroup, 256), 4096)), dim3(256), 0, st,
                        d->tok, d->pos, d->len, n, d->job, d->tj, d->jend, probelen, poff, loff, psid, fence, cfg.target,
                        cfg.heavy_target, P<uint2>(d->b_rng), hi, nparts, eq, bcnt, cfg.stream_parts,
-                       dense_ok(d) ? cfg.dense_w : 0u, cfg.dense_pct, (unsigned long long *)(misc + M_NDP64),
-                       cfg.block_records);
+                       dense_w, cfg.dense_pct, (unsigned long long *)(misc + M_NDP64), cfg.block_records,
+                       dense_w ? dflag : (uint32_t *)nullptr);
     HIPCK(hipGetLastError());
+    if (dense_w) {   // the dense offset table, query list and work units (nothing when none were planned)
+        const unsigned long long *ndp = (const unsigned long long *)(misc + M_NDP64);
+        hipLaunchKernelGGL(k_dense_offs, dim3((unsigned)std::min<int64_t>(grid_for(std::max<int64_t>(d->ntok, 1), 256),
+                                                                           kDenseOffsGrid)),
+                           dim3(256), 0, st, d->skeys, psid, loff, misc + M_NE, d->n_terms, dense_w, ng, ndp,
+                           P<uint32_t>(d->b_doffs));
+        HIPCK(hipGetLastError());
+        HIPCK(exclusive_scan<uint32_t>(dflag, doff, n, misc + M_NDENSE, scr, st));
+        hipLaunchKernelGGL(k_dense_list, dim3(grid_for(n, 256)), dim3(256), 0, st, (const uint32_t *)dflag,
+                           (const uint32_t *)doff, n, ndp, dl);
+        HIPCK(hipGetLastError());
+        uint64_t *tpre = P<uint64_t>(d->b_dtpre);
+        hipLaunchKernelGGL(k_dense_tiles, dim3(grid_for((int64_t)ng + 1, 256)), dim3(256), 0, st, (const uint32_t *)dl,
+                           misc + M_NDENSE, hi, ng, dense_w, ndp, P<uint32_t>(d->b_dq), tpre);
+        HIPCK(hipGetLastError());
+        HIPCK(exclusive_scan<uint64_t>(tpre, tpre, (int64_t)ng + 1, (uint64_t *)(misc + M_NDU64), scr64, st));
+    }
     const uint32_t blim = std::min(d->cap_bounds, cfg.bounds);
     uint32_t *nitems = P<uint32_t>(d->b_nitems);
     HIPCK(exclusive_scan<uint64_t>(bcnt, bofs, n, (uint64_t *)(misc + M_NB64), scr64, st));
-    // dense items (only with dense parts on): their count per query, scanned into the
-    // offsets of each query's entries in the dense item list; M_NDENSE their total
-    const uint32_t dense_w = dense_ok(d) ? cfg.dense_w : 0u;
-    uint32_t *dnum = dense_w ? P<uint32_t>(d->b_dnum) : nullptr, *doff = P<uint32_t>(d->b_doff);
-    hipLaunchKernelGGL(k_nitems, dim3(grid_for(n, 256)), dim3(256), 0, st, nparts, probelen, bofs, bcnt, blim, n, nitems,
-                       dnum, (const unsigned long long *)(misc + M_NDP64));
+    // (dense queries have no items: the dense role of the join walks their units)
+    hipLaunchKernelGGL(k_nitems, dim3(grid_for(n, 256)), dim3(256), 0, st, nparts, probelen, bofs, bcnt, blim, n, nitems);
     HIPCK(hipGetLastError());
     HIPCK(exclusive_scan<uint32_t>(nitems, partoff, n, misc + M_NITEMS, scr, st));
-    if (dnum) HIPCK(exclusive_scan<uint32_t>(dnum, doff, n, misc + M_NDENSE, scr, st));
     hipLaunchKernelGGL(k_bounds, dim3(kBoundsGrid), dim3(256), 0, st, n, probelen, poff, P<uint2>(d->b_rng),
                        (const uint32_t *)nullptr, hi, nparts, bofs, bcnt, (const uint64_t *)(misc + M_NB64), psid,
                        fence, P<uint32_t>(d->b_bounds), blim);
     HIPCK(hipGetLastError());
     hipLaunchKernelGGL(k_items, dim3(kItemsGrid), dim3(256), 0, st, 0, (const uint32_t *)nullptr, hi, nparts,
                        probelen, eq, partoff, bofs, bcnt, blim, d->job, d->tj,
-                       d->len, d->pos, poff, n, misc + M_NITEMS, P<Item>(d->b_items), d->cap_items, dense_w,
-                       (const unsigned long long *)(misc + M_NDP64), (const uint32_t *)doff, P<uint32_t>(d->b_ditems),
+                       d->len, d->pos, poff, n, misc + M_NITEMS, P<Item>(d->b_items), d->cap_items,
                        (const uint32_t *)d->jend);
     HIPCK(hipGetLastError());
     return FGSS_OK;
@@ -2325,7 +2521,7 @@ static int enqueue_stage_plan(fgss_device_sets *d, const RunCfg &cfg)
 
 static int enqueue_stage_join(fgss_device_sets *d, const RunCfg &cfg)
 {
-    return launch_join_cap(d, join_args(d), cfg.cap, false, false, dense_ok(d) && cfg.dense_w != 0);   // 9: the join
+    return launch_join_cap(d, join_args(d), cfg.cap, false, false, dense_w_of(d, cfg) != 0);   // 9: the join
 }
 
 // The first out_hint result triples into pinned memory, behind the join on its stream
@@ -2562,7 +2758,7 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
         if (hipEventElapsedTime(&ms, d->ev[1], d->ev[5]) == hipSuccess) T.ms_join = ms;
         if (hipEventElapsedTime(&ms, d->ev[4], d->ev[5]) == hipSuccess) T.ms_kernel = ms;
     }
-    for (int k = C_ESCAN; k <= C_DENSE; k++) T.acc[k] += d->h_ctr[k];
+    for (int k = C_ESCAN; k <= C_DPOST; k++) T.acc[k] += d->h_ctr[k];
     uint64_t nres = d->h_ctr[C_RESULTS], novf = d->h_ctr[C_OVERFLOW];
     tri.resize(3 * (size_t)nres);
     if (nres && nres <= d->out_copied) memcpy(tri.data(), d->h_out, (size_t)nres * 12);
@@ -2629,7 +2825,7 @@ static int finish_run(fgss_device_sets *d, const RunCfg &cfg, const PipeOps &ops
             if (rc) return rc;
             d->cap_gen++;
         }
-        for (int k = C_ESCAN; k <= C_DENSE; k++) T.acc[k] += d->h_ctr[k];
+        for (int k = C_ESCAN; k <= C_DPOST; k++) T.acc[k] += d->h_ctr[k];
         nres = d->h_ctr[C_RESULTS];
         novf = d->h_ctr[C_OVERFLOW];
         const size_t old = tri.size();
@@ -2804,6 +3000,7 @@ static void fill_stats(const RunTotals &T, size_t job, fgss_join_stats &st)
     for (int k = 0; k < 6; k++) st.phase_cycles[k] = (int64_t)T.acc[C_CYC_SETUP + k];
     st.window_retries = (int64_t)T.acc[C_RETRY];
     st.dense_parts = (int64_t)T.acc[C_DENSE];
+    st.dense_postings = (int64_t)T.acc[C_DPOST];
     st.ms_index = T.ms_index;
     st.ms_join = T.ms_join;
     st.ms_kernel = T.ms_kernel;
@@ -3560,8 +3757,7 @@ static int enqueue_rs(fgss_server *srv, ServeSlot &sl, bool timing)
     const uint32_t blim = std::min(d->cap_bounds, cfg.bounds);
     uint32_t *nitems = P<uint32_t>(d->b_nitems);
     HIPCK(exclusive_scan<uint64_t>(bcnt, bofs, v, (uint64_t *)(misc + M_NB64), scr64, st));
-    hipLaunchKernelGGL(k_nitems, dim3(grid_for(v, 256)), dim3(256), 0, st, nparts, probelen, bofs, bcnt, blim, v, nitems,
-                       (uint32_t *)nullptr, (const unsigned long long *)(misc + M_NDP64));
+    hipLaunchKernelGGL(k_nitems, dim3(grid_for(v, 256)), dim3(256), 0, st, nparts, probelen, bofs, bcnt, blim, v, nitems);
     HIPCK(hipGetLastError());
     HIPCK(exclusive_scan<uint32_t>(nitems, partoff, v, misc + M_NITEMS, scr, st));
     hipLaunchKernelGGL(k_bounds, dim3(kBoundsGrid), dim3(256), 0, st, v, probelen, poff, P<uint2>(d->b_rng), wlo, hi,
@@ -3570,8 +3766,7 @@ static int enqueue_rs(fgss_server *srv, ServeSlot &sl, bool timing)
     HIPCK(hipGetLastError());
     hipLaunchKernelGGL(k_items, dim3(kItemsGrid), dim3(256), 0, st, 1, wlo, hi, nparts, probelen, eq, partoff,
                        bofs, bcnt, blim, d->job, d->tj, d->len, d->pos, poff, v, misc + M_NITEMS, P<Item>(d->b_items),
-                       d->cap_items, 0u, (const unsigned long long *)(misc + M_NDP64), (const uint32_t *)nullptr,
-                       (uint32_t *)nullptr, (const uint32_t *)nullptr);
+                       d->cap_items, (const uint32_t *)nullptr);
     HIPCK(hipGetLastError());
     if (timing) HIPCK(hipEventRecord(d->ev[4], st));
     int rc = launch_join_cap(d, rs_args(srv, sl), cfg.cap, false, true);
@@ -4313,7 +4508,8 @@ extern "C" int fgss_selfjoin_noalloc(const int32_t *tokens, const int32_t *lengt
         return fgss_fail(FGSS_ERR_ARG, "table_slots must be 256, 512, 1024, 2048 or 4096");
     if (n_sets == 0) return FGSS_OK;
     auto t0 = std::chrono::steady_clock::now();
-    const RunCfg cfg = make_cfg(opts);
+    RunCfg cfg = make_cfg(opts);
+    cfg.dense_w = 0;   // no dense parts: their offset table would not fit the scratch
     const hipStream_t st = (hipStream_t)stream;
     int dev = 0, cus = 0;
     HIPCK(hipGetDevice(&dev));
@@ -4389,7 +4585,7 @@ extern "C" int fgss_selfjoin_noalloc(const int32_t *tokens, const int32_t *lengt
                          (unsigned long long)written);
     }
     uint64_t acc[C_COUNT] = {0};
-    for (int k = C_ESCAN; k <= C_DENSE; k++) acc[k] += h_ctr[k];
+    for (int k = C_ESCAN; k <= C_DPOST; k++) acc[k] += h_ctr[k];
 
     // 3. overflowed work items: split four ways and run again, in launches of at most
     //    kNoAllocSplit items; the items that overflow again wait in the other buffer
@@ -4430,7 +4626,7 @@ extern "C" int fgss_selfjoin_noalloc(const int32_t *tokens, const int32_t *lengt
                                  (long long)out_cap, (long long)*n_out);
             }
             written += h_ctr[C_RESULTS];
-            for (int k = C_ESCAN; k <= C_DENSE; k++) acc[k] += h_ctr[k];
+            for (int k = C_ESCAN; k <= C_DPOST; k++) acc[k] += h_ctr[k];
             const uint32_t o = (uint32_t)h_ctr[C_OVERFLOW];
             if ((uint64_t)nnext + o > kNoAllocPending)
                 return fgss_fail(FGSS_ERR_UNSUPPORTED, "overflowed work items exceed the no-allocation scratch's %u",
@@ -4462,6 +4658,7 @@ extern "C" int fgss_selfjoin_noalloc(const int32_t *tokens, const int32_t *lengt
         for (int k = 0; k < 6; k++) stats->phase_cycles[k] = (int64_t)acc[C_CYC_SETUP + k];
         stats->window_retries = (int64_t)acc[C_RETRY];
         stats->dense_parts = (int64_t)acc[C_DENSE];
+        stats->dense_postings = (int64_t)acc[C_DPOST];
         stats->kernel_launches = launches;
         stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }

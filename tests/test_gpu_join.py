@@ -307,4 +307,7 @@ def test_dense_parts(fg, oracle_mod, dense_pct):
                     assert p.stats["dense_parts"] > 0
                 if dense_pct == -1:
                     assert p.stats["dense_parts"] == 0
+                # the dense parts' postings are part of the postings scanned
+                assert 0 <= p.stats["dense_postings"] <= p.stats["postings_scanned"]
+                assert (p.stats["dense_postings"] > 0) == (p.stats["dense_parts"] > 0)
                 assert_join_equal(oracle_mod, sets, p, t, counters=want)
