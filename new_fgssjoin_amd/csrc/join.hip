@@ -84,7 +84,7 @@ enum Ctr {
     C_RESULTS,       // pairs emitted (may exceed the output capacity)
     C_OVERFLOW,      // work items whose LDS table overflowed
     C_QUEUE,         // work-queue head
-    C_QUEUE_D,       // work-queue head of the dense join kernel
+    C_QUEUE_D,       // work-queue head of the dense role (dense units)
     C_CYC_SETUP,     // shader-clock cycles summed over waves, per k_join phase
     C_CYC_A,
     C_CYC_B,
@@ -138,7 +138,7 @@ constexpr uint32_t kPadMask = 0x0FFFFFFFu;
 // source (no hashing); the count is that of its dense parts
 constexpr uint32_t kDenseParts = 0x80000000u;
 #ifndef FGSS_DENSE_UNROLL
-#define FGSS_DENSE_UNROLL 4   // rows of kWave postings per step of a dense part's walk
+#define FGSS_DENSE_UNROLL 6   // rows of kWave postings per step of a dense part's walk
 #endif
 // dense match counts: 8 bits, four slots a word
 constexpr uint32_t kDcShift = 2u;   // log2 slots per count word
@@ -200,7 +200,7 @@ struct fgss_device_sets {
     float *tj = nullptr;        // per job: threshold of the current run
     DevBuf b_psid, b_fence, b_bcnt, b_bofs, b_bounds, b_probelen, b_poff, b_rng, b_eq, b_midlen, b_midoff, b_ka,
         b_ia, b_kb, b_ib, b_sortscr, b_sid_of, b_pl_of, b_post, b_loff, b_hi, b_nparts, b_partoff, b_scanscr, b_items,
-        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc, b_nitems, b_ovfp, b_wctr, b_jsave, b_ditems, b_dnum, b_doff, b_grp, b_pden,
+        b_items2, b_ovf, b_out, b_ctr, b_jctr, b_misc, b_nitems, b_ovfp, b_wctr, b_jsave, b_dlist, b_dflag, b_doff, b_grp, b_pden,
         b_doffs, b_dq, b_dtpre;
     const uint32_t *skeys = nullptr;   // the last index build's sorted entry terms, one per posting
     uint32_t cap_items = 0, cap_bounds = 0, cap_out = 0;   // current capacities (elements)
@@ -273,7 +273,7 @@ void fgss_device_release(fgss_collection *c)
                       &d->b_rng, &d->b_eq, &d->b_midlen, &d->b_midoff, &d->b_ka, &d->b_ia, &d->b_kb, &d->b_ib,
                       &d->b_sortscr, &d->b_sid_of, &d->b_pl_of, &d->b_post, &d->b_loff, &d->b_hi, &d->b_nparts, &d->b_partoff,
                       &d->b_scanscr, &d->b_items, &d->b_items2, &d->b_ovf, &d->b_out, &d->b_ctr, &d->b_jctr,
-                      &d->b_nitems, &d->b_ovfp, &d->b_wctr, &d->b_jsave, &d->b_ditems, &d->b_dnum, &d->b_doff, &d->b_grp, &d->b_pden,
+                      &d->b_nitems, &d->b_ovfp, &d->b_wctr, &d->b_jsave, &d->b_dlist, &d->b_dflag, &d->b_doff, &d->b_grp, &d->b_pden,
                       &d->b_doffs, &d->b_dq, &d->b_dtpre, &d->b_misc};
     for (DevBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -1122,7 +1122,7 @@ __device__ __forceinline__ bool table_insert(const uint2 (&pe)[N], uint32_t valm
 
 // The dense counterpart of table_insert, for a dense part of sources [wa, wa + 2 x CAP):
 // a source's slot is its offset from wa (no hashing or probing, so nothing overflows),
-// its match count is 8 bits, four slots to a word of dcnt (probes of dense items hold at
+// its match count is 8 bits, four slots to a word of dcnt (probes of dense units hold at
 // most 255 tokens), and dlast keeps query position << 24 | source position << 16: both
 // positions are below 256 (a source is no longer than its query, so its position lies
 // inside the query's probe length), so the maximum word holds the last match.  pe[k] is
@@ -1188,14 +1188,13 @@ constexpr uint32_t join_wave_words(int cap)
 
 // RS: an R-S join against an index holding max_prefix(|S|, t_min) tokens per source;
 // postings past max_prefix(|S|, t) of the item's threshold are skipped.  DENSE: the
-// blocks of this role join the dense items only, the others every other item (both walk
-// the one item list, each role with its own queue), so that the registers of each loop
-// hold one kind of item.  rblock / rblocks: the block's index among its role's blocks
-// and their number.
+// blocks of this role join the dense units (grid part, query), the others the item list,
+// each role with its own queue, so that the registers of each loop hold one kind of
+// work.  rblock / rblocks: the block's index among its role's blocks and their number.
 template <int CAP, bool RS, bool DENSE>
 __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, uint32_t rblocks)
 {
-    static_assert(!DENSE || (CAP >= 512 && !RS), "dense items: self-joins with 512 table slots or more");
+    static_assert(!DENSE || (CAP >= 512 && !RS), "dense units: self-joins with 512 table slots or more");
     constexpr int LOG2CAP = CAP == 256 ? 8 : CAP == 512 ? 9 : CAP == 1024 ? 10 : CAP == 2048 ? 11 : 12;
     static_assert((1 << LOG2CAP) == CAP, "CAP must be a power of two");
     extern __shared__ uint32_t smem[];
@@ -1275,12 +1274,16 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
     const uint64_t gw = (uint64_t)rblock * (blockDim.x / kWave) + (uint32_t)w;
     uint64_t batch_next = min(gw * g0, n_items), batch_end = min(gw * g0 + g0, n_items);
     uint32_t cur_job = kEmpty;
+    float cur_t = 0.0f;   // the threshold of cur_job
     // the next item of the claimed batch is loaded while the current one runs
     uint32_t nxtw = 0;
     bool have_nxt = false;
-    // dense units: the grid part whose tiles were looked up last, and its tile range
+    // dense units: the grid part whose tiles were looked up last, its tile range, and the
+    // loaded tile: lane u < kDenseTile holds its unit u's query (kEmpty: none), window end,
+    // length, offset and job
     uint32_t ugp = 0;
-    uint64_t utlo = 1, uthi = 0;
+    uint64_t utlo = 1, uthi = 0, utile = ~0ull;
+    uint32_t tq = kEmpty, thi = 0, tlen = 0, tpos = 0, tjob = 0;
     const uint32_t dng = DENSE ? (uint32_t)((A.n + 2 * CAP - 1) / (2 * CAP)) : 0u;   // grid parts
     for (;;) {
         if (batch_next >= batch_end) {
@@ -1313,11 +1316,22 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                 utlo = A.dtpre[lo];
                 uthi = A.dtpre[lo + 1];
             }
-            const uint64_t di = (uint64_t)A.dqlo[ugp] + (tile - utlo) * kDenseTile + it % kDenseTile;
-            if (di >= ndq) continue;
-            const uint32_t qd = A.dl[di];
+            if (tile != utile) {   // the tile's queries, one lane each, loaded together
+                utile = tile;
+                const uint64_t di = (uint64_t)A.dqlo[ugp] + (tile - utlo) * kDenseTile + (uint32_t)lane;
+                tq = lane < (int)kDenseTile && di < ndq ? A.dl[di] : kEmpty;
+                if (tq != kEmpty) {
+                    thi = A.hi[tq];
+                    tlen = A.len[tq];
+                    tpos = A.pos[tq];
+                    tjob = A.job[tq];
+                }
+            }
+            const uint32_t u = (uint32_t)(it % kDenseTile);
+            const uint32_t qd = __builtin_amdgcn_readlane(tq, u);
+            if (qd == kEmpty) continue;
             constexpr uint32_t DWu = 2u * CAP;
-            const uint32_t ua = max(qd + 1u, ugp * DWu), ub = min(A.hi[qd], (ugp + 1u) * DWu);
+            const uint32_t ua = max(qd + 1u, ugp * DWu), ub = min(__builtin_amdgcn_readlane(thi, u), (ugp + 1u) * DWu);
             if (ua >= ub) continue;   // the query's window does not meet this grid part
             item.q = qd;
             item.a = ua;
@@ -1326,10 +1340,10 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
             item.part = 0;
             item.nparts = 1;
             item.bofs = 0;
-            item.job = A.job[qd];
-            item.t = A.tj[item.job];
-            item.lq = A.len[qd];
-            item.qpos = A.pos[qd];
+            item.job = __builtin_amdgcn_readlane(tjob, u);
+            item.t = item.job == cur_job ? cur_t : A.tj[item.job];
+            item.lq = __builtin_amdgcn_readlane(tlen, u);
+            item.qpos = __builtin_amdgcn_readlane(tpos, u);
             item.qpoff = 0;
         } else {
             // an item is kItemWords words, lane k holding word k (one register for the
@@ -1365,6 +1379,7 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
             }
             wv = lane >= W_JOB0 && lane < W_JOB0 + kJobCtr ? 0ull : wv;
             cur_job = jq;
+            cur_t = t;
         }
         const uint32_t lq = item.lq & 0xFFFFu;
         const bool midprobe = RS && (item.lq & kMidProbe) != 0;
@@ -1610,9 +1625,6 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                 cur[(uint32_t)lane * PER + k] = dlo[k] - dbeg[k];
             }
             const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
-#ifdef FGSS_DEBUG_FILL
-            WACC(W_RETRY, (total + kWave - 1u) & ~(kWave - 1u));   // debug builds: lanes issued
-#endif
             wave_sync();
             uint32_t *scr = qstage;                         // a word per lane for updates without work
             uint8_t *smark = (uint8_t *)(qstage + kWave);   // a step's segment marks (a byte per position)
@@ -2096,9 +2108,9 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
     if (lane == kAccSlots) slot[lane] = cur_job;
 }
 
-// One launch, two roles: the first A.dense_blocks blocks join the dense items (they are
-// dispatched first, and when there are few dense items their idle waves leave at once
-// and free their CUs for the others), the remaining blocks every other item.
+// One launch, two roles: the first A.dense_blocks blocks join the dense units (they are
+// dispatched first, and when there are few dense units their idle waves leave at once
+// and free their CUs for the others), the remaining blocks the item list.
 template <int CAP, bool RS>
 __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
 {
@@ -2160,7 +2172,7 @@ static size_t join_smem_bytes()
 
 // persistent grid: as many blocks as fit on the device, each wave loops over items
 // persistent grid: as many blocks as fit on the device, each wave loops over items;
-// with dense items possible, twice that: the dense role's blocks, then the other's
+// with dense units possible, twice that: the dense role's blocks, then the other's
 template <int CAP, bool RS>
 static int launch_join(fgss_device_sets *d, const JoinArgs &A0, bool dry, bool with_dense)
 {
@@ -2348,11 +2360,11 @@ static int prepare_buffers(fgss_device_sets *d)
     GROW(d->b_bcnt, (size_t)n * 8);
     GROW(d->b_bofs, (size_t)(n + 1) * 8);
     GROW(d->b_nitems, (size_t)n * 4);
-    GROW(d->b_dnum, (size_t)n * 4);
+    GROW(d->b_dflag, (size_t)n * 4);
     GROW(d->b_doff, (size_t)(n + 1) * 4);
     GROW(d->b_bounds, (size_t)d->cap_bounds * 4);
     GROW(d->b_items, (size_t)d->cap_items * sizeof(Item));
-    GROW(d->b_ditems, (size_t)std::max<int64_t>(d->cap_items, n) * 4);   // the dense query list
+    GROW(d->b_dlist, (size_t)n * 4);   // the dense query list
     GROW(d->b_grp, 8);
     d->cap_ovf = std::max(d->cap_ovf, d->cap_items);
     GROW(d->b_ovf, (size_t)d->cap_ovf * 4);
@@ -2379,7 +2391,7 @@ static JoinArgs join_args(fgss_device_sets *d)
     A.rng = P<uint2>(d->b_rng); A.poff = P<uint32_t>(d->b_poff); A.bounds = P<uint32_t>(d->b_bounds);
     A.pden = dense_ok(d) ? P<uint32_t>(d->b_pden) : nullptr;
     A.doffs = P<uint32_t>(d->b_doffs); A.dqlo = P<uint32_t>(d->b_dq); A.dtpre = P<uint64_t>(d->b_dtpre);
-    A.dl = P<uint32_t>(d->b_ditems); A.ndq_dev = P<uint32_t>(d->b_misc) + M_NDENSE;
+    A.dl = P<uint32_t>(d->b_dlist); A.ndq_dev = P<uint32_t>(d->b_misc) + M_NDENSE;
     A.n_tiles_dev = d->b_doffs.p ? (const unsigned long long *)(P<uint32_t>(d->b_misc) + M_NDU64) : nullptr;
     A.hi = P<uint32_t>(d->b_hi); A.nparts = P<uint32_t>(d->b_nparts); A.n = d->n;
     A.items = P<Item>(d->b_items);
@@ -2476,7 +2488,7 @@ static int enqueue_stage_plan(fgss_device_sets *d, const RunCfg &cfg)
     // 6-8: plan work items; with dense parts on, k_plan flags the dense queries
     const uint32_t dense_w = dense_w_of(d, cfg);
     const uint32_t ng = dense_w ? (uint32_t)dense_grid_parts(d, dense_w) : 0u;
-    uint32_t *dflag = P<uint32_t>(d->b_dnum), *doff = P<uint32_t>(d->b_doff), *dl = P<uint32_t>(d->b_ditems);
+    uint32_t *dflag = P<uint32_t>(d->b_dflag), *doff = P<uint32_t>(d->b_doff), *dl = P<uint32_t>(d->b_dlist);
     hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(grid_for(n * kPlanGroup, 256), 4096)), dim3(256), 0, st,
                        d->tok, d->pos, d->len, n, d->job, d->tj, d->jend, probelen, poff, loff, psid, fence, cfg.target,
                        cfg.heavy_target, P<uint2>(d->b_rng), hi, nparts, eq, bcnt, cfg.stream_parts,
@@ -4393,11 +4405,11 @@ static size_t noalloc_carve(fgss_device_sets *d, uint8_t *base, int64_t n, int64
     buf(d->b_bcnt, (size_t)N * 8);
     buf(d->b_bofs, (size_t)(N + 1) * 8);
     buf(d->b_nitems, (size_t)N * 4);
-    buf(d->b_dnum, (size_t)N * 4);
+    buf(d->b_dflag, (size_t)N * 4);
     buf(d->b_doff, (size_t)(N + 1) * 4);
     buf(d->b_bounds, (size_t)d->cap_bounds * 4);
     buf(d->b_items, (size_t)d->cap_items * sizeof(Item));
-    buf(d->b_ditems, (size_t)d->cap_items * 4);
+    buf(d->b_dlist, (size_t)N * 4);
     buf(d->b_grp, 8);
     buf(d->b_ovf, (size_t)d->cap_ovf * 4);
     buf(d->b_ovfp, (size_t)d->cap_ovf * 4);
