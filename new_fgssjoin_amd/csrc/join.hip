@@ -137,6 +137,9 @@ constexpr uint32_t kPadMask = 0x0FFFFFFFu;
 // table-slots records that meet its window, each counted in a table indexed by the
 // source (no hashing); the count is that of its dense parts
 constexpr uint32_t kDenseParts = 0x80000000u;
+#ifndef FGSS_DENSE_SIGPER
+#define FGSS_DENSE_SIGPER 1   // candidates per lane and step of a dense part's signature bound
+#endif
 #ifndef FGSS_DENSE_UNROLL
 #define FGSS_DENSE_UNROLL 6   // rows of kWave postings per step of a dense part's walk
 #endif
@@ -1125,19 +1128,19 @@ __device__ __forceinline__ bool table_insert(const uint2 (&pe)[N], uint32_t valm
 // its match count is 8 bits, four slots to a word of dcnt (probes of dense units hold at
 // most 255 tokens), and dlast keeps query position << 24 | source position << 16: both
 // positions are below 256 (a source is no longer than its query, so its position lies
-// inside the query's probe length), so the maximum word holds the last match.  pe[k] is
-// (record, source position) from the 4-byte postings.  Lanes without work aim their
-// updates at scr[lane].
+// inside the query's probe length), so the maximum word holds the last match.  pw[k] is a
+// 4-byte posting, record << 8 | source position.  Lanes without work aim their updates at
+// scr[lane].
 template <int N>
-__device__ __forceinline__ void dense_insert(const uint2 (&pe)[N], uint32_t valm, const uint32_t (&jpos)[N], uint32_t wa,
+__device__ __forceinline__ void dense_insert(const uint32_t (&pw)[N], uint32_t valm, const uint32_t (&jpos)[N], uint32_t wa,
                                              uint32_t *dlast, uint32_t *dcnt, uint32_t *scr, int lane)
 {
 #pragma unroll
     for (int k = 0; k < N; k++) {
         const bool v = (valm >> k) & 1u;
-        const uint32_t h = pe[k].x - wa;
+        const uint32_t h = (pw[k] >> 8) - wa;
         atomicAdd(v ? &dcnt[h >> kDcShift] : &scr[lane], 1u << ((h & (kDcSlots - 1u)) * kDcBits));
-        atomicMax(v ? &dlast[h] : &scr[lane], (jpos[k] << 24) | (pe[k].y << 16));
+        atomicMax(v ? &dlast[h] : &scr[lane], (jpos[k] << 24) | ((pw[k] & 0xFFu) << 16));
     }
 }
 
@@ -1192,7 +1195,7 @@ constexpr uint32_t join_wave_words(int cap)
 // each role with its own queue, so that the registers of each loop hold one kind of
 // work.  rblock / rblocks: the block's index among its role's blocks and their number.
 template <int CAP, bool RS, bool DENSE>
-__device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, uint32_t rblocks)
+__device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, uint32_t rblocks, uint32_t sblock)
 {
     static_assert(!DENSE || (CAP >= 512 && !RS), "dense units: self-joins with 512 table slots or more");
     constexpr int LOG2CAP = CAP == 256 ? 8 : CAP == 512 ? 9 : CAP == 1024 ? 10 : CAP == 2048 ? 11 : 12;
@@ -1220,7 +1223,7 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
         const uint64_t g = min((n_items + nwg - 1) / nwg, (uint64_t)kQueueBatch);
         const uint64_t gwi = (uint64_t)rblock * (blockDim.x / kWave) + (uint32_t)w;
         if (gwi * g >= n_items && nwg * g >= n_items) {
-            unsigned long long *slot = A.wctr + ((size_t)blockIdx.x * join_waves(CAP) + (size_t)w) * kWaveSlot;
+            unsigned long long *slot = A.wctr + ((size_t)sblock * join_waves(CAP) + (size_t)w) * kWaveSlot;
             if (lane < kAccSlots) slot[lane] = 0ull;
             if (lane == kAccSlots) slot[lane] = kEmpty;
             return;
@@ -1633,10 +1636,7 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
             uint32_t segc = 0;   // segment of the last posting of the previous step
             auto dstep = [&](auto rows, uint32_t e0) {
                 constexpr int N = decltype(rows)::value;
-                uint32_t e[N], seg[N];
-                uint2 pe[N];
-#pragma unroll
-                for (int k = 0; k < N; k++) e[k] = min(e0 + (uint32_t)(k * kWave) + lane, total - 1u);
+                uint32_t seg[N], pw[N];
 #pragma unroll
                 for (int i = 0; i < N / 4 + (N % 4 != 0); i++) ((uint32_t *)smark)[i * kWave + lane] = 0u;
 #pragma unroll
@@ -1654,16 +1654,17 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                     c = max(c, (uint32_t)__builtin_amdgcn_readlane((int)m, kWave - 1));
                 }
                 segc = c;
+                // lanes past the end read the last posting (a real one)
+#pragma unroll
+                for (int k = 0; k < N; k++) pw[k] = A.pden[cur[seg[k]] + min(e0 + (uint32_t)(k * kWave) + lane, total - 1u)];
                 uint32_t valm = 0;
 #pragma unroll
                 for (int k = 0; k < N; k++) {
-                    const uint32_t w = A.pden[cur[seg[k]] + e[k]];   // 4-byte postings: record, position
-                    pe[k] = make_uint2(w >> 8, w & 0xFFu);
-                    const bool v = e0 + (uint32_t)(k * kWave) + (uint32_t)lane < total && pe[k].x < wb;
+                    const bool v = e0 + (uint32_t)(k * kWave) + (uint32_t)lane < total && (pw[k] >> 8) < wb;
                     escan += (uint32_t)__popcll(__ballot(v));
                     valm |= (uint32_t)v << k;
                 }
-                dense_insert(pe, valm, seg, wa, dlast, dcnt, scr, lane);
+                dense_insert(pw, valm, seg, wa, dlast, dcnt, scr, lane);
             };
             constexpr int NS = FGSS_DENSE_UNROLL;
             for (uint32_t e0 = 0; e0 < total;) {
@@ -1857,7 +1858,7 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
             const bool qbad = A.qsigbad[q] != 0;
             // kSigPer candidates per lane and step: their signature lines are gathered
             // together, so a step waits for memory once
-            constexpr int kSigPer = 2;
+            constexpr int kSigPer = DENSE ? FGSS_DENSE_SIGPER : 2;
             for (uint32_t base = 0; base < nsig; base += kWave * kSigPer) {
                 uint32_t ent[kSigPer], s[kSigPer];
                 uint8_t sbad[kSigPer];
@@ -2103,26 +2104,26 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
     // one coalesced store, no atomics (thousands of wavefronts adding to the same
     // counters at the end of the launch serialize); k_wave_reduce sums the slots
     wave_sync();
-    unsigned long long *slot = A.wctr + ((size_t)blockIdx.x * join_waves(CAP) + (size_t)w) * kWaveSlot;
+    unsigned long long *slot = A.wctr + ((size_t)sblock * join_waves(CAP) + (size_t)w) * kWaveSlot;
     if (lane < kAccSlots) slot[lane] = wv;
     if (lane == kAccSlots) slot[lane] = cur_job;
 }
 
-// One launch, two roles: the first A.dense_blocks blocks join the dense units (they are
-// dispatched first, and when there are few dense units their idle waves leave at once
-// and free their CUs for the others), the remaining blocks the item list.
+// Two kernels on the join's stream, one per role, so that each keeps its registers for
+// one kind of work (one kernel with both roles spilled the dense loop's registers to
+// scratch): k_join_dense walks the dense units, then k_join the item list.  Their
+// wavefronts write disjoint counter slots: k_join's blocks follow the dense kernel's
+// A.dense_blocks blocks.
+template <int CAP>
+__global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join_dense(JoinArgs A)
+{
+    if constexpr (CAP >= 512) join_items<CAP, false, true>(A, blockIdx.x, gridDim.x, blockIdx.x);
+}
 template <int CAP, bool RS>
 __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
 {
-    if constexpr (CAP >= 512 && !RS) {
-        if (blockIdx.x < A.dense_blocks) {
-            join_items<CAP, RS, true>(A, blockIdx.x, A.dense_blocks);
-            return;
-        }
-    }
-    join_items<CAP, RS, false>(A, blockIdx.x - A.dense_blocks, gridDim.x - A.dense_blocks);
+    join_items<CAP, RS, false>(A, blockIdx.x, gridDim.x, blockIdx.x + A.dense_blocks);
 }
-
 // Sum the per-wavefront slots of a k_join launch into the global counters and the
 // per-job counters: a thread per slot.  Global counters: a wavefront sum, then one
 // atomic per wavefront.  Per-job counters: consecutive slots usually share a job,
@@ -2176,23 +2177,32 @@ static size_t join_smem_bytes()
 template <int CAP, bool RS>
 static int launch_join(fgss_device_sets *d, const JoinArgs &A0, bool dry, bool with_dense)
 {
-    static thread_local int cached[2] = {-1, -1};   // blocks, device
+    static thread_local int cached[3] = {-1, -1, -1};   // blocks, dense-kernel blocks, device
     const size_t smem = join_smem_bytes<CAP>();
-    if (cached[0] < 0 || cached[1] != d->device) {   // runtime queries: never inside a graph capture
+    if (cached[0] < 0 || cached[2] != d->device) {   // runtime queries: never inside a graph capture
+        int per_cu = 0, per_cu_d = 0, cus = 0;
         HIPCK(hipFuncSetAttribute((const void *)k_join<CAP, RS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-        int per_cu = 0, cus = 0;
         HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_join<CAP, RS>, join_waves(CAP) * kWave, smem));
+        if (CAP >= 512 && !RS) {
+            HIPCK(hipFuncSetAttribute((const void *)k_join_dense<CAP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+            HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_d, k_join_dense<CAP>, join_waves(CAP) * kWave, smem));
+        }
         HIPCK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device));
         cached[0] = std::max(per_cu, 1) * cus;
-        cached[1] = d->device;
+        cached[1] = std::max(per_cu_d, 1) * cus;
+        cached[2] = d->device;
     }
     if (dry) return FGSS_OK;
     JoinArgs A = A0;
-    A.dense_blocks = with_dense && CAP >= 512 && !RS && A.n_tiles_dev != nullptr ? (uint32_t)cached[0] : 0u;
-    const uint32_t blocks = (uint32_t)cached[0] + A.dense_blocks;
-    const uint32_t nw = blocks * join_waves(CAP);
+    A.dense_blocks = with_dense && CAP >= 512 && !RS && A.n_tiles_dev != nullptr ? (uint32_t)cached[1] : 0u;
+    const uint32_t blocks = (uint32_t)cached[0];
+    const uint32_t nw = (blocks + A.dense_blocks) * join_waves(CAP);
     if ((uint64_t)nw * kWaveSlot * 8 > d->b_wctr.cap)
         return fgss_fail(FGSS_ERR_DEVICE, "internal: wavefront counter slots too small");
+    if (A.dense_blocks) {
+        hipLaunchKernelGGL((k_join_dense<CAP>), dim3(A.dense_blocks), dim3(join_waves(CAP) * kWave), smem, d->st, A);
+        HIPCK(hipGetLastError());
+    }
     hipLaunchKernelGGL((k_join<CAP, RS>), dim3(blocks), dim3(join_waves(CAP) * kWave), smem, d->st, A);
     HIPCK(hipGetLastError());
     hipLaunchKernelGGL(k_wave_reduce, dim3((nw + 255) / 256), dim3(256), 0, d->st, (const unsigned long long *)A.wctr,
