@@ -51,7 +51,7 @@ def log(msg):
 
 
 def design_bytes(st):
-    """Bytes one join's k_join launch must read and write as designed: 4 B per probe
+    """Bytes one join's kernels (k_join_dense and k_join) must read and write as designed: 4 B per probe
     token, 8 B per useful posting of a hash part (record id, position | length) and 4 B
     per posting of a dense part (record << 8 | position), one 64-byte signature line per
     candidate tested and a 128-byte one per candidate tested again, 4 B per verified
@@ -79,19 +79,20 @@ def pmc_summary(workload):
 
 
 def roofline(workload, st_list, kernel_ms, extra=None):
-    """Roofline of k_join over launches with stats st_list and mean duration kernel_ms
-    (HIP events around the launch on its stream)."""
+    """Roofline of a join's kernels (k_join_dense, then k_join, on one stream) over joins with
+    stats st_list and mean duration kernel_ms (HIP events on their stream around the two
+    launches)."""
     kbytes = sum(design_bytes(st) for st in st_list) / len(st_list)
     achieved = kbytes / (kernel_ms / 1e3) / 1e9 if kernel_ms > 0 else 0.0
     prof = pmc_summary(workload)
     fetch, write = prof.get("fetch_bytes_per_launch"), prof.get("write_bytes_per_launch")
     traffic = fetch + write if fetch is not None and write is not None else None
     out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_join",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_join_dense + k_join",
            "bytes_per_launch": kbytes, "kernel_ms_per_launch": kernel_ms,
-           "basis": "bytes this kernel must move as designed (probe tokens, useful postings, signature lines, "
-                    "verified tokens, results) / HIP-event launch time",
-           "traffic_basis": "unscaled FETCH_SIZE + WRITE_SIZE per k_join launch from the same-build PMC summary",
+           "basis": "bytes the join kernels must move as designed (probe tokens, useful postings, signature "
+                    "lines, verified tokens, results) / HIP-event time of the two launches",
+           "traffic_basis": "unscaled FETCH_SIZE + WRITE_SIZE of a join's two kernels from the same-build PMC summary",
            "traffic_frac": (traffic / (kernel_ms / 1e3) / 1e9 / HBM_PEAK_GBS) if traffic and kernel_ms > 0 else None,
            "traffic_over_design": (traffic / kbytes) if traffic and kbytes else None,
            "traffic_source": os.path.relpath(PMC[workload], ROOT) if prof else None,
