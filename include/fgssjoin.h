@@ -283,24 +283,31 @@ typedef struct {
     double ms_latency;        /* submit -> its results were on the host          */
     int32_t batch_requests;   /* requests that shared that run (itself included) */
     int32_t reserved;
+    int64_t charged_bytes;    /* its charge against the server's budget          */
+    int64_t inflight_bytes;   /* charges in flight, over all slots, once its run
+                                 was admitted (its own included)                  */
 } fgss_request_stats;
 
 /* Index `indexed` (a plain collection in internal order, lengths non-increasing)
  * once for every threshold >= t_min: each record's first max_prefix(|s|, t_min)
  * tokens.  The server keeps a private device copy; the caller keeps `indexed`
  * alive while results are read (record ids come from it).  `slots` runs may be in
- * flight at once, each on its own HIP stream; `budget_bytes` (<= 0: 16 GiB) caps
- * the work admitted in flight, estimated at 16 B per posting a request's probe
- * prefixes may reach (prefix tokens x the index's mean list length).  Larger
- * budgets batch more requests per run: more throughput, longer queueing.  Each slot's
- * workspace is allocated here, for runs of up to 65,536 probes of the index's mean
- * length, so that serving does not allocate; larger runs grow it. */
+ * flight at once, each on its own HIP stream; `budget_bytes` (<= 0: 256 MiB) caps
+ * the device memory of the runs' work buffers in flight: each request is charged
+ * the bytes it occupies in a run -- its staged records, the plan arrays and posting
+ * ranges of its probes, and its work items with their boundary rows, estimated from
+ * the postings its probe prefixes may reach (prefix tokens x the index's mean list
+ * length); the candidates themselves are counted in the join kernel's LDS.  Requests
+ * are admitted while their charges fit.  Larger budgets batch more requests per run:
+ * more throughput, longer queueing.  Each slot's workspace is allocated here, for runs
+ * of up to 65,536 probes of the index's mean length and the work items of the whole
+ * budget, so that serving does not allocate; larger runs grow it. */
 int fgss_server_create(const fgss_collection *indexed, float t_min, int32_t slots, int64_t budget_bytes,
                        const fgss_join_opts *opts, fgss_server **out);
 /* Queue an R-S join of `probes` (a plain collection in the index's token space,
  * e.g. from fgss_collection_from_text_dict; any length order) at threshold
  * t in [t_min, 1).  Pending requests are admitted, first come first served, into
- * idle slots, several per run, while their estimates fit the budget.  The caller
+ * idle slots, several per run, while their charges fit the budget.  The caller
  * keeps `probes` alive until fgss_server_result. */
 int fgss_server_submit(fgss_server *srv, const fgss_collection *probes, float t, int64_t *request_id);
 /* Retire completed runs and admit pending requests; with block set, first wait for

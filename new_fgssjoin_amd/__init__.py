@@ -93,7 +93,7 @@ class BatchStats(C.Structure):
 
 class RequestStats(C.Structure):
     _fields_ = [("ms_queue", C.c_double), ("ms_latency", C.c_double), ("batch_requests", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("reserved", C.c_int32), ("charged_bytes", C.c_int64), ("inflight_bytes", C.c_int64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}

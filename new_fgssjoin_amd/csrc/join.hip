@@ -3654,18 +3654,41 @@ __global__ __launch_bounds__(256) void k_plan_rs(const uint32_t *__restrict__ pt
 
 // host-side request and server state
 constexpr uint32_t kReserveJobs = 4096;   // requests per run a slot is sized for at creation
+constexpr int64_t kServeBudget = (int64_t)256 << 20;   // default budget: device bytes of the runs in flight
+
+// Device memory a request occupies in a run's work buffers, charged against the server's
+// budget: its staged records (4 B a token, 12 B a probe), the plan arrays of its two
+// half-probes (48 B each), the posting ranges of their prefix tokens (8 B each), and its
+// estimated work items (48 B each) with their boundary rows (4 B per item and probe-prefix
+// token of an average probe).  The items are estimated from the postings the probe
+// prefixes may reach (prefix tokens x the index's mean list length) in parts of `target`
+// postings, plus one per half-probe.  The candidates themselves live in LDS: the work
+// items and boundary rows are the buffers that grow with them.
+static int64_t serve_request_bytes(const fgss_collection *probes, float t, double mean_list, uint32_t target)
+{
+    double tok = 0, ptok = 0;
+    for (int32_t l : probes->len) {
+        tok += (double)l;
+        ptok += (double)std::min<uint32_t>(fgss_max_prefix((uint32_t)l, t), (uint32_t)l);
+    }
+    const double np = (double)probes->len.size();
+    const double items = ptok * mean_list / std::max<uint32_t>(target, 1u) + 2.0 * np;
+    const double qp = np > 0 ? ptok / np : 0.0;
+    return (int64_t)(4.0 * tok + 12.0 * np + 96.0 * np + 16.0 * ptok + 48.0 * items + 4.0 * items * qp) + 4096;
+}
 
 struct ServeRequest {
     int64_t id = 0;
     const fgss_collection *probes = nullptr;
     float t = 0.f;
-    int64_t est = 0;                    // work estimate (16 B per posting the probe prefixes may reach)
+    int64_t est = 0;                    // device bytes charged against the budget (serve_request_bytes)
     bool done = false;
     int status = FGSS_OK;
     std::vector<int32_t> tri;           // (probe, source, overlap), probe indices local to the request
     uint64_t cnt[kJobCtr] = {0, 0, 0, 0};
     int32_t launches = 0;               // join-kernel launches of its run
     int32_t batch_requests = 0;         // requests that shared its run
+    int64_t inflight = 0;               // charges in flight once its run was admitted
     std::chrono::steady_clock::time_point t_submit, t_start, t_done;
 };
 
@@ -4062,6 +4085,7 @@ static int server_progress(fgss_server *srv, bool block)
             if (!fits) break;
         }
         if (sl.reqs.empty()) break;
+        for (ServeRequest *r : sl.reqs) r->inflight = srv->inflight;
         sl.seq = srv->launch_seq++;
         int rc = slot_launch(srv, sl);
         if (rc) {
@@ -4096,7 +4120,7 @@ extern "C" int fgss_server_create(const fgss_collection *indexed, float t_min, i
     auto *srv = new fgss_server();
     srv->coll = indexed;
     srv->t_min = t_min;
-    srv->budget = budget_bytes > 0 ? budget_bytes : (int64_t)16 << 30;
+    srv->budget = budget_bytes > 0 ? budget_bytes : kServeBudget;
     srv->cfg = make_cfg(opts);
     int rc = upload_records(&srv->idx, indexed);
     if (rc) { delete srv; return rc; }
@@ -4145,9 +4169,9 @@ extern "C" int fgss_server_create(const fgss_collection *indexed, float t_min, i
         // (at most 2^24 tokens) and kReserveJobs requests
         if (!srv->empty) {
             const int64_t mean_len = (x->ntok + x->n - 1) / x->n;
-            // work items: the budget's postings (16 B each) in parts of cfg.target, plus
-            // the two half-probes of every probe (about 3.4M items, 180 MB, at 16 GiB)
-            const int64_t items = srv->budget / 16 / std::max<uint32_t>(srv->cfg.target, 1u) + 2 * srv->max_probes + 4096;
+            // work items: as many as the budget's bytes would hold (48 B each), plus the two
+            // half-probes of every probe (about 5.7M items, 270 MB, at the default 256 MiB)
+            const int64_t items = srv->budget / (int64_t)sizeof(Item) + 2 * srv->max_probes + 4096;
             rc = slot_reserve(sl, srv->max_probes, std::min<int64_t>(srv->max_probes * mean_len, 1ll << 24), kReserveJobs,
                               std::min<int64_t>(items, 1ll << 23));
             if (rc) { fgss_server_free(srv); return rc; }
@@ -4176,11 +4200,7 @@ extern "C" int fgss_server_submit(fgss_server *srv, const fgss_collection *probe
     r->id = srv->next_id++;
     r->probes = probes;
     r->t = t;
-    // work estimate: the postings the probe prefixes may reach (prefix tokens x the
-    // index's mean list length, before the size filter), at 16 B each
-    double ptok = 0;
-    for (int32_t l : probes->len) ptok += (double)std::min<uint32_t>(fgss_max_prefix((uint32_t)l, t), (uint32_t)l);
-    r->est = (int64_t)(16.0 * ptok * srv->mean_list) + 4096;
+    r->est = serve_request_bytes(probes, t, srv->mean_list, srv->cfg.target);
     r->t_submit = std::chrono::steady_clock::now();
     srv->reqs[r->id] = r;
     srv->pending.push_back(r);
@@ -4286,6 +4306,8 @@ extern "C" int fgss_server_result(fgss_server *srv, int64_t request_id, fgss_pai
         stats->ms_queue = std::chrono::duration<double, std::milli>(r->t_start - r->t_submit).count();
         stats->ms_latency = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
         stats->batch_requests = r->batch_requests;
+        stats->charged_bytes = r->est;
+        stats->inflight_bytes = r->inflight;
         stats->reserved = 0;
     }
     *out = p;

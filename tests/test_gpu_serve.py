@@ -113,9 +113,15 @@ def test_batched_equals_alone(fg, world, slots, budget):
         srv.wait(ids[0])                                  # held and nothing running
     srv.hold(False)
     shared = 0
+    cap = budget if budget > 0 else 256 << 20
     for k in np.random.default_rng(slots).permutation(len(ids)):   # collect in a shuffled order
         pairs, st = srv.result(ids[k])
         shared = max(shared, st["batch_requests"])
+        # admission: the charges in flight fit the budget, or the request ran alone
+        assert st["charged_bytes"] > 0
+        assert st["inflight_bytes"] >= st["charged_bytes"]
+        assert st["inflight_bytes"] <= cap or (
+            st["batch_requests"] == 1 and st["inflight_bytes"] == st["charged_bytes"]), (k, st)
         assert _same(pairs, alone[k]), f"request {k}"
         for key in ("candidates", "intersections", "results", "postings_scanned"):
             assert pairs.stats[key] == alone[k].stats[key], (k, key)
