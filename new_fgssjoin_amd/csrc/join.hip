@@ -99,7 +99,7 @@ enum Ctr {
 
 struct Item {
     uint32_t q, a, b;   // query and source sub-window [a, b)
-    uint32_t pad;       // expected useful postings (sizes the LDS table) | kWholeWindow | kPlanned | kStreamed | kDense
+    uint32_t pad;       // expected useful postings (sizes the LDS table) | kWholeWindow | kPlanned | kStreamed
     uint32_t part, nparts;  // planned items: part index and part count of the query
     uint32_t bofs;          // planned items: first boundary position of the query in the bounds table
     uint32_t job;           // the query's job
@@ -131,23 +131,11 @@ constexpr uint32_t kJobUnsorted = 0x80000000u;
 constexpr uint32_t kWholeWindow = 0x80000000u;   // the item covers the query's whole window
 constexpr uint32_t kPlanned = 0x40000000u;       // part of a planned split: ranges come from the bounds table
 constexpr uint32_t kStreamed = 0x20000000u;      // the whole window, walked part by part by one wavefront
-constexpr uint32_t kDense = 0x10000000u;         // consecutive dense parts of the window, walked in order
 constexpr uint32_t kPadMask = 0x0FFFFFFFu;
 // nparts flag of a query joined in dense parts: the parts of a global grid of 2 x
 // table-slots records that meet its window, each counted in a table indexed by the
 // source (no hashing); the count is that of its dense parts
 constexpr uint32_t kDenseParts = 0x80000000u;
-#ifndef FGSS_DENSE_SIGPER
-#define FGSS_DENSE_SIGPER 1   // candidates per lane and step of a dense part's signature bound
-#endif
-#ifndef FGSS_DENSE_UNROLL
-#define FGSS_DENSE_UNROLL 6   // rows of kWave postings per step of a dense part's walk
-#endif
-// dense match counts: 8 bits, four slots a word
-constexpr uint32_t kDcShift = 2u;   // log2 slots per count word
-constexpr uint32_t kDcSlots = 1u << kDcShift;
-constexpr uint32_t kDcBits = 32u >> kDcShift;
-constexpr uint32_t kDcMask = (1u << kDcBits) - 1u;
 // streamed items keep one posting cursor per probe token in the wavefront's LDS
 // (the range arrays of the other items)
 constexpr uint32_t kStreamQp = 256;
@@ -542,11 +530,12 @@ __global__ void k_list_offsets(const uint32_t *__restrict__ skeys, const uint32_
 // Dense parts follow a global grid of dense_w records: grid part g holds the records
 // [g * dense_w, (g + 1) * dense_w).  The offset table gives, for every term t and grid part
 // g <= ng, the first posting of t's list whose record lies at or past g * dense_w:
-// offs[t * (ng + 1) + g] (the list's end when there is none), so that the postings of t in
-// grid part g are offs[t][g] .. offs[t][g + 1] with no search.  A thread per posting
-// writes the entries of the grid parts that end at it (those past the previous posting's
-// part, up to its own), the last posting of a list also those after it; a thread per term
-// fills the table row of an empty list.  Nothing to do when the run planned no dense parts.
+// offs[g * n_terms + t] (the list's end when there is none), so that the postings of t in
+// grid part g are offs[g][t] .. offs[g + 1][t] with no search.  Rows by grid part: the dense
+// units run part by part, so the rows in use stay in L2.  A thread per posting writes the
+// entries of the grid parts that end at it (those past the previous posting's part, up to
+// its own), the last posting of a list also those after it; a thread per term fills the
+// entries of an empty list.  Nothing to do when the run planned no dense parts.
 __global__ __launch_bounds__(256) void k_dense_offs(const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ psid,
                                                     const uint32_t *__restrict__ loff, const uint32_t *__restrict__ nE_dev,
                                                     uint32_t n_terms, uint32_t dense_w, uint32_t ng,
@@ -555,21 +544,18 @@ __global__ __launch_bounds__(256) void k_dense_offs(const uint32_t *__restrict__
 {
     if (*ndparts == 0) return;
     const uint32_t nE = *nE_dev;
-    const size_t row = (size_t)ng + 1;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nE; i += gridDim.x * blockDim.x) {
         const uint32_t t = skeys[i], g = psid[i] / dense_w;
         const bool first = i == 0 || skeys[i - 1] != t;
         const uint32_t g0 = first ? 0u : psid[i - 1] / dense_w + 1u;   // grid parts whose first posting is i
-        uint32_t *o = offs + (size_t)t * row;
-        for (uint32_t gg = g0; gg <= g; gg++) o[gg] = i;
+        for (uint32_t gg = g0; gg <= g; gg++) offs[(size_t)gg * n_terms + t] = i;
         if (i + 1 == nE || skeys[i + 1] != t)
-            for (uint32_t gg = g + 1; gg <= ng; gg++) o[gg] = i + 1;
+            for (uint32_t gg = g + 1; gg <= ng; gg++) offs[(size_t)gg * n_terms + t] = i + 1;
     }
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n_terms; t += gridDim.x * blockDim.x) {
         const uint32_t l0 = loff[t];
         if (l0 != loff[t + 1]) continue;
-        uint32_t *o = offs + (size_t)t * row;
-        for (uint32_t gg = 0; gg <= ng; gg++) o[gg] = l0;
+        for (uint32_t gg = 0; gg <= ng; gg++) offs[(size_t)gg * n_terms + t] = l0;
     }
 }
 constexpr unsigned kDenseOffsGrid = 8192;   // k_dense_offs blocks (grid-stride)
@@ -1012,8 +998,10 @@ struct JoinArgs {
     const uint32_t *loff;
     const uint2 *post;          // postings (record id, token position | length << 16), grouped by term
     const uint32_t *pden;       // the same postings as record << 8 | position, for dense parts (null: none)
-    const uint32_t *doffs;      // dense parts: per term, the first posting of each grid part (k_dense_offs)
-    uint32_t dng;               // grid parts of the dense offset table (its rows hold dng + 1 entries)
+    const uint32_t *doffs;      // dense parts: per grid part g <= dng and term t, the first posting of t
+                                // at or after record g x the grid width (row g, dterms entries; k_dense_offs)
+    uint32_t dng;               // grid parts of the dense offset table (dng + 1 rows)
+    uint32_t dterms;            // terms (the offset table's row length)
     const uint32_t *dl;         // the dense queries in ascending order (k_dense_list)
     const uint32_t *ndq_dev;    // their number (device)
     const uint32_t *dqlo;       // per grid part: the first entry of dl in its dense tiles (k_dense_tiles)
@@ -1123,27 +1111,6 @@ __device__ __forceinline__ bool table_insert(const uint2 (&pe)[N], uint32_t valm
     return __ballot(failm != 0) != 0;
 }
 
-// The dense counterpart of table_insert, for a dense part of sources [wa, wa + 2 x CAP):
-// a source's slot is its offset from wa (no hashing or probing, so nothing overflows),
-// its match count is 8 bits, four slots to a word of dcnt (probes of dense units hold at
-// most 255 tokens), and dlast keeps query position << 24 | source position << 16: both
-// positions are below 256 (a source is no longer than its query, so its position lies
-// inside the query's probe length), so the maximum word holds the last match.  pw[k] is a
-// 4-byte posting, record << 8 | source position.  Lanes without work aim their updates at
-// scr[lane].
-template <int N>
-__device__ __forceinline__ void dense_insert(const uint32_t (&pw)[N], uint32_t valm, const uint32_t (&jpos)[N], uint32_t wa,
-                                             uint32_t *dlast, uint32_t *dcnt, uint32_t *scr, int lane)
-{
-#pragma unroll
-    for (int k = 0; k < N; k++) {
-        const bool v = (valm >> k) & 1u;
-        const uint32_t h = (pw[k] >> 8) - wa;
-        atomicAdd(v ? &dcnt[h >> kDcShift] : &scr[lane], 1u << ((h & (kDcSlots - 1u)) * kDcBits));
-        atomicMax(v ? &dlast[h] : &scr[lane], (jpos[k] << 24) | ((pw[k] & 0xFFu) << 16));
-    }
-}
-
 // Work item = (query q, source window [a, b)).  Phases, all inside one wavefront:
 //  A  candidate generation: the postings of the probe-prefix tokens whose record id
 //     lies in the window, flattened over all tokens, kUnroll per lane in flight; per
@@ -1180,24 +1147,19 @@ enum WaveAcc { W_ESCAN = 0, W_CAND, W_SIG, W_VER, W_SIG2, W_INTER, W_VTOK, W_RET
 // counters and the wavefront's last job
 constexpr int kWaveSlot = 32;
 static_assert(kAccSlots + 1 <= kWaveSlot && kAccSlots + 1 <= kWave, "slot layout");
-// length samples of a dense part: the first source of each 32nd of the part, and its last
-constexpr uint32_t kDenseSamples = 33;
-// LDS words per wavefront: table (key, count, last match), list, token ranges, query
-// stage, a dense part's length samples
+// LDS words per wavefront: table (key, count, last match), list, token ranges, query stage
 constexpr uint32_t join_wave_words(int cap)
 {
-    return 4u * cap + 2u * kTokBatch + 1u + kQStage + kDenseSamples;
+    return 4u * cap + 2u * kTokBatch + 1u + kQStage;
 }
 
 // RS: an R-S join against an index holding max_prefix(|S|, t_min) tokens per source;
-// postings past max_prefix(|S|, t) of the item's threshold are skipped.  DENSE: the
-// blocks of this role join the dense units (grid part, query), the others the item list,
-// each role with its own queue, so that the registers of each loop hold one kind of
-// work.  rblock / rblocks: the block's index among its role's blocks and their number.
-template <int CAP, bool RS, bool DENSE>
+// postings past max_prefix(|S|, t) of the item's threshold are skipped.  rblock /
+// rblocks: the block's index among the launch's blocks and their number; sblock: its
+// counter slot (after the dense kernel's blocks).
+template <int CAP, bool RS>
 __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, uint32_t rblocks, uint32_t sblock)
 {
-    static_assert(!DENSE || (CAP >= 512 && !RS), "dense units: self-joins with 512 table slots or more");
     constexpr int LOG2CAP = CAP == 256 ? 8 : CAP == 512 ? 9 : CAP == 1024 ? 10 : CAP == 2048 ? 11 : 12;
     static_assert((1 << LOG2CAP) == CAP, "CAP must be a power of two");
     extern __shared__ uint32_t smem[];
@@ -1213,12 +1175,9 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
     // a launch whose item list or boundary table did not fit does nothing: the host
     // sees the totals, grows the buffers and runs the pipeline again
     if (*A.n_items_dev > A.n_items_cap) return;
-    // the dense role walks the dense work units (grid part, query), the other role the
-    // item list
-    const uint64_t n_items = DENSE ? (A.n_tiles_dev == nullptr ? 0ull : *A.n_tiles_dev * kDenseTile) : (uint64_t)*A.n_items_dev;
-    const uint32_t ndq = DENSE && A.n_tiles_dev != nullptr ? *A.ndq_dev : 0u;
-    {   // a wavefront with no static chunk and no queue to claim from (the dense role of a
-        // run with few or no dense parts) writes a zero counter slot and leaves at once
+    const uint64_t n_items = (uint64_t)*A.n_items_dev;
+    {   // a wavefront with no static chunk and no queue to claim from writes a zero
+        // counter slot and leaves at once
         const uint64_t nwg = (uint64_t)rblocks * (blockDim.x / kWave);
         const uint64_t g = min((n_items + nwg - 1) / nwg, (uint64_t)kQueueBatch);
         const uint64_t gwi = (uint64_t)rblock * (blockDim.x / kWave) + (uint32_t)w;
@@ -1230,10 +1189,9 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
         }
     }
 
-    // the hash table starts empty; the dense role keeps its LDS in the dense state for
-    // good (every part leaves its slots zero again)
+    // the hash table starts empty
     for (uint32_t i = lane; i < CAP; i += kWave) {
-        tkey[i] = DENSE ? 0u : kEmpty;
+        tkey[i] = kEmpty;
         tcnt[i] = 0;
         tlast[i] = 0;
     }
@@ -1258,9 +1216,11 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
     // update (of a wave-uniform value) is a compare and a select, no branch or LDS
     // round trip.
     unsigned long long wv = 0;
-#define WACC(k, v)                                                             \
-    do {                                                                       \
-        wv += (uint32_t)lane == (uint32_t)(k) ? (unsigned long long)(v) : 0ull; \
+// (v is evaluated on every lane before the select: a ballot inside it must see them all)
+#define WACC(k, v)                                                        \
+    do {                                                                  \
+        const unsigned long long wacc_v_ = (unsigned long long)(v);       \
+        wv += (uint32_t)lane == (uint32_t)(k) ? wacc_v_ : 0ull;           \
     } while (0)
 
     unsigned long long c0 = clock64(), c1 = 0;
@@ -1281,18 +1241,11 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
     // the next item of the claimed batch is loaded while the current one runs
     uint32_t nxtw = 0;
     bool have_nxt = false;
-    // dense units: the grid part whose tiles were looked up last, its tile range, and the
-    // loaded tile: lane u < kDenseTile holds its unit u's query (kEmpty: none), window end,
-    // length, offset and job
-    uint32_t ugp = 0;
-    uint64_t utlo = 1, uthi = 0, utile = ~0ull;
-    uint32_t tq = kEmpty, thi = 0, tlen = 0, tpos = 0, tjob = 0;
-    const uint32_t dng = DENSE ? (uint32_t)((A.n + 2 * CAP - 1) / (2 * CAP)) : 0u;   // grid parts
     for (;;) {
         if (batch_next >= batch_end) {
             if (qbase >= n_items) break;   // every item was in a static chunk
             unsigned long long b0 = 0;
-            if (lane == 0) b0 = atomicAdd(&A.ctr[DENSE ? C_QUEUE_D : C_QUEUE], (unsigned long long)kQueueBatch);
+            if (lane == 0) b0 = atomicAdd(&A.ctr[C_QUEUE], (unsigned long long)kQueueBatch);
             b0 = qbase + (((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(b0 >> 32)) << 32) |
                           (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)b0));
             batch_next = min((uint64_t)b0, n_items);
@@ -1302,72 +1255,24 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
         if (batch_next >= n_items) break;
         const uint64_t it = batch_next++;
         Item item;
-        if constexpr (DENSE) {
-            // unit it: dense query it % kDenseTile of tile it / kDenseTile in part-major
-            // order (k_dense_tiles); the grid part holding the tile is searched only when
-            // the tile leaves the last part's range.  The tile's last units may lie past
-            // the part's queries: their windows do not meet it.
-            const uint64_t tile = it / kDenseTile;
-            if (tile < utlo || tile >= uthi) {
-                uint32_t lo = 0, hi = dng;   // the last part g with tpre[g] <= tile
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (A.dtpre[mid] <= tile) lo = mid;
-                    else hi = mid;
-                }
-                ugp = lo;
-                utlo = A.dtpre[lo];
-                uthi = A.dtpre[lo + 1];
-            }
-            if (tile != utile) {   // the tile's queries, one lane each, loaded together
-                utile = tile;
-                const uint64_t di = (uint64_t)A.dqlo[ugp] + (tile - utlo) * kDenseTile + (uint32_t)lane;
-                tq = lane < (int)kDenseTile && di < ndq ? A.dl[di] : kEmpty;
-                if (tq != kEmpty) {
-                    thi = A.hi[tq];
-                    tlen = A.len[tq];
-                    tpos = A.pos[tq];
-                    tjob = A.job[tq];
-                }
-            }
-            const uint32_t u = (uint32_t)(it % kDenseTile);
-            const uint32_t qd = __builtin_amdgcn_readlane(tq, u);
-            if (qd == kEmpty) continue;
-            constexpr uint32_t DWu = 2u * CAP;
-            const uint32_t ua = max(qd + 1u, ugp * DWu), ub = min(__builtin_amdgcn_readlane(thi, u), (ugp + 1u) * DWu);
-            if (ua >= ub) continue;   // the query's window does not meet this grid part
-            item.q = qd;
-            item.a = ua;
-            item.b = ub;
-            item.pad = kDense;
-            item.part = 0;
-            item.nparts = 1;
-            item.bofs = 0;
-            item.job = __builtin_amdgcn_readlane(tjob, u);
-            item.t = item.job == cur_job ? cur_t : A.tj[item.job];
-            item.lq = __builtin_amdgcn_readlane(tlen, u);
-            item.qpos = __builtin_amdgcn_readlane(tpos, u);
-            item.qpoff = 0;
-        } else {
-            // an item is kItemWords words, lane k holding word k (one register for the
-            // prefetched item instead of kItemWords)
-            const uint32_t ldw = have_nxt ? nxtw : item_word(A.items, (uint32_t)it, lane);
-            have_nxt = batch_next < batch_end;
-            if (have_nxt) nxtw = item_word(A.items, (uint32_t)batch_next, lane);
-            // the item is the same for every lane: its fields live in scalar registers
-            item.q = __builtin_amdgcn_readlane(ldw, 0);
-            item.a = __builtin_amdgcn_readlane(ldw, 1);
-            item.b = __builtin_amdgcn_readlane(ldw, 2);
-            item.pad = __builtin_amdgcn_readlane(ldw, 3);
-            item.part = __builtin_amdgcn_readlane(ldw, 4);
-            item.nparts = __builtin_amdgcn_readlane(ldw, 5);
-            item.bofs = __builtin_amdgcn_readlane(ldw, 6);
-            item.job = __builtin_amdgcn_readlane(ldw, 7);
-            item.t = __int_as_float(__builtin_amdgcn_readlane(ldw, 8));
-            item.lq = __builtin_amdgcn_readlane(ldw, 9);
-            item.qpos = __builtin_amdgcn_readlane(ldw, 10);
-            item.qpoff = __builtin_amdgcn_readlane(ldw, 11);
-        }
+        // an item is kItemWords words, lane k holding word k (one register for the
+        // prefetched item instead of kItemWords)
+        const uint32_t ldw = have_nxt ? nxtw : item_word(A.items, (uint32_t)it, lane);
+        have_nxt = batch_next < batch_end;
+        if (have_nxt) nxtw = item_word(A.items, (uint32_t)batch_next, lane);
+        // the item is the same for every lane: its fields live in scalar registers
+        item.q = __builtin_amdgcn_readlane(ldw, 0);
+        item.a = __builtin_amdgcn_readlane(ldw, 1);
+        item.b = __builtin_amdgcn_readlane(ldw, 2);
+        item.pad = __builtin_amdgcn_readlane(ldw, 3);
+        item.part = __builtin_amdgcn_readlane(ldw, 4);
+        item.nparts = __builtin_amdgcn_readlane(ldw, 5);
+        item.bofs = __builtin_amdgcn_readlane(ldw, 6);
+        item.job = __builtin_amdgcn_readlane(ldw, 7);
+        item.t = __int_as_float(__builtin_amdgcn_readlane(ldw, 8));
+        item.lq = __builtin_amdgcn_readlane(ldw, 9);
+        item.qpos = __builtin_amdgcn_readlane(ldw, 10);
+        item.qpoff = __builtin_amdgcn_readlane(ldw, 11);
         const uint32_t q = item.q;
         const uint32_t jq = item.job;
         const float t = item.t;
@@ -1414,16 +1319,6 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
         // token (LDS), starting at the token's first posting inside the item's window
         uint32_t *cur = rlo;
         static_assert(2 * kTokBatch + 1 >= (int)kStreamQp, "cursors live in rlo and rpre");
-        // A dense unit joins its query against one part of the global grid of DW records
-        // (its window's part [a, b)).  Its table is indexed by the source's offset in the
-        // part: dlast (the key and count words of the hash table) and dcnt (8-bit counts,
-        // the first quarter of the last-match words); the dense role keeps them zero
-        // between units.
-        constexpr uint32_t DW = 2u * CAP;
-        constexpr uint32_t kDenseList = CAP > kStreamQp ? CAP - kStreamQp : 0u;   // candidates listed at once
-        static_assert(CAP < 512 || DW <= 0x3FFFu, "dense part slots fit 14 bits");
-        constexpr bool dense = DENSE;
-        uint32_t *const dlast = tkey, *const dcnt = tlast;
         if (streamed) {
             for (uint32_t j = lane; j < qp; j += kWave) {
                 const uint2 r = qrng[j];
@@ -1443,13 +1338,7 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
         for (uint32_t p0 = 0, p1 = 0; p0 < nwin; p0 = p1) {
         p1 = min(p0 + span, nwin);
         uint32_t wa = item.a, wb = item.b;
-        // a dense part's source lengths are non-increasing (dense parts are for jobs in
-        // size order): lane k < kDenseSamples samples the length at k x DW / 32 (the last
-        // one at the part's end).  The load is issued here and stored after the rounds.
-        uint32_t lsamp = 0;
-        if (dense) {   // the unit's part of its grid part, [a, b)
-            if (lane < (int)kDenseSamples) lsamp = A.len[wa + min((uint32_t)lane * (DW / 32u), wb - wa - 1u)];
-        } else if (streamed) {
+        if (streamed) {
             const uint64_t W = (uint64_t)item.b - item.a;
             wa = part_start(item.a, W, p0, nwin);
             wb = part_start(item.a, W, p1, nwin);
@@ -1458,7 +1347,7 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
         // ---- A: candidate generation (filtering.cu:94-139) ----
         bool overflow = false;
         uint32_t escan = 0, nlist = 0, jdone = 0;
-        for (uint32_t jb = 0; !streamed && !dense && jb < qp && !overflow; jb += kTokBatch) {
+        for (uint32_t jb = 0; !streamed && jb < qp && !overflow; jb += kTokBatch) {
             const uint32_t nb = min((uint32_t)kTokBatch, qp - jb);
             PHASE_START(r0);
             for (uint32_t jj = lane; jj < nb; jj += kWave) {
@@ -1581,100 +1470,7 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
             }
             wave_sync();
         }
-        if constexpr (DENSE) {
-            // ---- A (dense): a probe token's postings in the part are offs[t][g] ..
-            // offs[t][g + 1] of the grid's offset table (k_dense_offs), from the query's
-            // successor on in its window's first part (a search inside them), so every
-            // count is known.  The counts are scanned (tokens lane * PER + k, in token
-            // order) and the part's postings walked flattened as in the hash path:
-            // segment marks and a max-scan hand each posting its token, and cur holds
-            // each token's posting index of flattened position 0.  Postings at or past wb
-            // (the window's end inside its last grid part) are read but take no part. ----
-            constexpr int PER = kStreamQp / kWave;
-            const uint32_t g = wa / DW;   // the part's grid index
-            const uint32_t *orow = A.doffs + g;
-            const size_t ostride = (size_t)dng + 1u;
-            uint32_t dc[PER], dbeg[PER], dlo[PER], sum = 0;
-#pragma unroll
-            for (int k = 0; k < PER; k++) {
-                const uint32_t j = (uint32_t)lane * PER + k;
-                uint32_t lo = 0, hi = 0;
-                if (j < qp) {
-                    const uint32_t *o = orow + (size_t)A.qtok[qpos + j] * ostride;
-                    lo = o[0];
-                    hi = o[1];
-                }
-                dlo[k] = lo;
-                dc[k] = hi - lo;
-            }
-            if (wa > g * DW) {   // the window's first part: from the query's successor on
-#pragma unroll
-                for (int k = 0; k < PER; k++) {
-                    if (dc[k] != 0) {
-                        const uint32_t hi = dlo[k] + dc[k];
-                        dlo[k] = lower_bound_sid(A.psid, A.fence, dlo[k], hi, wa);
-                        dc[k] = hi - dlo[k];
-                    }
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < PER; k++) sum += dc[k];
-            const uint32_t inc = wave_inclusive_scan_dpp(sum);
-            uint32_t run0 = inc - sum;
-#pragma unroll
-            for (int k = 0; k < PER; k++) {
-                dbeg[k] = run0;
-                run0 += dc[k];
-                cur[(uint32_t)lane * PER + k] = dlo[k] - dbeg[k];
-            }
-            const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1);
-            wave_sync();
-            uint32_t *scr = qstage;                         // a word per lane for updates without work
-            uint8_t *smark = (uint8_t *)(qstage + kWave);   // a step's segment marks (a byte per position)
-            static_assert(FGSS_DENSE_UNROLL * kWave <= 4 * (kQStage - kWave), "a step's marks fit the query stage");
-            static_assert(kStreamQp <= 256, "segment indices fit a byte");
-            uint32_t segc = 0;   // segment of the last posting of the previous step
-            auto dstep = [&](auto rows, uint32_t e0) {
-                constexpr int N = decltype(rows)::value;
-                uint32_t seg[N], pw[N];
-#pragma unroll
-                for (int i = 0; i < N / 4 + (N % 4 != 0); i++) ((uint32_t *)smark)[i * kWave + lane] = 0u;
-#pragma unroll
-                for (int k = 0; k < PER; k++) {
-                    const uint32_t at = dbeg[k] - e0;   // wraps above the step for starts before e0
-                    const bool mk = dc[k] != 0 && at < (uint32_t)(N * kWave);
-                    *(mk ? smark + at : (uint8_t *)(scr + lane)) = (uint8_t)(lane * PER + k);
-                }
-                wave_sync();
-                uint32_t c = segc;
-#pragma unroll
-                for (int k = 0; k < N; k++) {
-                    const uint32_t m = wave_max_scan((uint32_t)smark[k * kWave + lane]);
-                    seg[k] = max(m, c);
-                    c = max(c, (uint32_t)__builtin_amdgcn_readlane((int)m, kWave - 1));
-                }
-                segc = c;
-                // lanes past the end read the last posting (a real one)
-#pragma unroll
-                for (int k = 0; k < N; k++) pw[k] = A.pden[cur[seg[k]] + min(e0 + (uint32_t)(k * kWave) + lane, total - 1u)];
-                uint32_t valm = 0;
-#pragma unroll
-                for (int k = 0; k < N; k++) {
-                    const bool v = e0 + (uint32_t)(k * kWave) + (uint32_t)lane < total && (pw[k] >> 8) < wb;
-                    escan += (uint32_t)__popcll(__ballot(v));
-                    valm |= (uint32_t)v << k;
-                }
-                dense_insert(pw, valm, seg, wa, dlast, dcnt, scr, lane);
-            };
-            constexpr int NS = FGSS_DENSE_UNROLL;
-            for (uint32_t e0 = 0; e0 < total;) {
-                const uint32_t rem = total - e0;   // wave-uniform
-                if (rem > (uint32_t)(NS - 1) * kWave) { dstep(std::integral_constant<int, NS>(), e0); e0 += kWave * NS; }
-                else if (rem > (uint32_t)kWave) { dstep(std::integral_constant<int, 2>(), e0); e0 += 2 * kWave; }
-                else { dstep(std::integral_constant<int, 1>(), e0); e0 += kWave; }
-            }
-            wave_sync();
-        } else if (streamed) {
+        if (streamed) {
             // A streamed part [wa, wb): every probe token with postings left takes the
             // next K of them from its cursor (independent loads, no search).  Those
             // before wb are this part's; the group of K lanes of a token moves its
@@ -1815,35 +1611,18 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
             }
             continue;
         }
-        uint32_t *const dsamp = qstage + kQStage;
-        if (dense) {
-            if (lane < (int)kDenseSamples) dsamp[lane] = lsamp;
-            wave_sync();
-        }
         PHASE_MARK(1);
 
-        // table slot accessors of phases C and D: a hash slot (its key is the source) or a
-        // dense slot (the source is wa + slot); the branches on `dense` are wave-uniform
-        auto src_of = [&](uint32_t h) -> uint32_t { return dense ? wa + h : tkey[h]; };
-        auto cnt_of = [&](uint32_t h) -> uint32_t {
-            return dense ? (dcnt[h >> kDcShift] >> ((h & (kDcSlots - 1u)) * kDcBits)) & kDcMask : tcnt[h];
-        };
-        auto last_of = [&](uint32_t h) -> uint32_t {   // query position << 16 | source position
-            if (!dense) return tlast[h];
-            const uint32_t w = dlast[h];
-            return (w >> 24) << 16 | ((w >> 16) & 0xFFu);
-        };
+        // table slot accessors of phases C and D
+        auto src_of = [&](uint32_t h) -> uint32_t { return tkey[h]; };
+        auto cnt_of = [&](uint32_t h) -> uint32_t { return tcnt[h]; };
+        auto last_of = [&](uint32_t h) -> uint32_t { return tlast[h]; };   // query position << 16 | source position
         // empties slot h when drop is set (the other lanes write a word of their own in the
         // idle query stage: no branch on a per-lane condition)
         auto drop_slot = [&](bool drop, uint32_t h) {
-            if (dense) {
-                *(drop ? &dlast[h] : &qstage[lane]) = 0u;
-                atomicAnd(drop ? &dcnt[h >> kDcShift] : &qstage[lane], ~(kDcMask << ((h & (kDcSlots - 1u)) * kDcBits)));
-            } else {
-                *(drop ? &tkey[h] : &qstage[lane]) = kEmpty;
-                *(drop ? &tcnt[h] : &qstage[lane]) = 0;
-                *(drop ? &tlast[h] : &qstage[lane]) = 0;
-            }
+            *(drop ? &tkey[h] : &qstage[lane]) = kEmpty;
+            *(drop ? &tcnt[h] : &qstage[lane]) = 0;
+            *(drop ? &tlast[h] : &qstage[lane]) = 0;
         };
         uint32_t cand = 0, nsig = 0, nver = 0, inter = 0, vtok = 0, nres = 0;
 
@@ -1852,13 +1631,13 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
         // the others' slots emptied.  Returns the survivors.
         // (the source's bad-signature flag and its 64-bucket line are read together)
         auto phase_c = [&](const uint32_t nsig) -> uint32_t {
-            uint32_t nver = 0;
+            uint32_t nver = 0, nex = 0;   // survivors; candidates that reach the 128-bucket bound
             const uint4 *qs4 = (const uint4 *)(A.qsig + (size_t)q * kSigWords);
             const uint4 *qh4 = (const uint4 *)(A.qsig64 + (size_t)q * kSig64Words);
             const bool qbad = A.qsigbad[q] != 0;
             // kSigPer candidates per lane and step: their signature lines are gathered
             // together, so a step waits for memory once
-            constexpr int kSigPer = DENSE ? FGSS_DENSE_SIGPER : 2;
+            constexpr int kSigPer = 2;
             for (uint32_t base = 0; base < nsig; base += kWave * kSigPer) {
                 uint32_t ent[kSigPer], s[kSigPer];
                 uint8_t sbad[kSigPer];
@@ -1895,8 +1674,9 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                     // sum over buckets of min(cQ, cS) = (|Q| + |S| - SAD) / 2 >= |Q n S|
                     const bool exact = in && !bad && ((int)(lq + ls) - (int)sad) / 2 >= floorb;
                     keep[u] = in && bad;
-                    if (__ballot(exact)) {
-                        WACC(W_SIG2, __popcll(__ballot(exact)));
+                    const uint64_t exm = __ballot(exact);
+                    nex += (uint32_t)__popcll(exm);
+                    if (exm) {
                         if (exact) {
                             const uint4 *ss4 = (const uint4 *)(A.sig + (size_t)s[u] * kSigWords);
                             uint32_t sad2 = 0;
@@ -1929,6 +1709,7 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                     nver += (uint32_t)__popcll(bal);
                 }
             }
+            WACC(W_SIG2, nex);
             wave_sync();
             return nver;
         };
@@ -1993,7 +1774,7 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
             wave_sync();
         };
 
-        if (!dense) {
+        {
             // ---- B: positional filter over the distinct sources ----
             // (branch-free like phase A: lanes past the list work on its last entry and
             // aim their stores at a word of their own in the idle query stage)
@@ -2032,54 +1813,6 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
             PHASE_MARK(3);
             phase_d(nver);
             PHASE_MARK(4);
-        } else if constexpr (kDenseList >= (uint32_t)kWave) {
-            // ---- B (dense): the positional filter over every slot of the part, 64 at a
-            // time; the candidates that can still reach floor(bound) are listed with their
-            // source length, the others' slots emptied.  A full list goes through phases
-            // C and D before the scan goes on. ----
-            const uint32_t nslots = wb - wa;   // <= DW, and slot0 < nslots keeps every slot below DW
-            for (uint32_t slot0 = 0; slot0 < nslots;) {
-                uint32_t nl = 0;
-                for (; slot0 < nslots && nl + (uint32_t)kWave <= kDenseList; slot0 += kWave) {
-                    const uint32_t h = slot0 + lane;
-                    const bool in = h < nslots;
-                    const uint32_t cw = dcnt[h >> kDcShift];
-                    const uint32_t m = in ? (cw >> ((h & (kDcSlots - 1u)) * kDcBits)) & kDcMask : 0u;
-                    const bool has = m != 0;
-                    if (__ballot(has) == 0) continue;
-                    const uint32_t w = dlast[h];
-                    // the source's length: the samples around it when they agree (lengths are
-                    // monotone between them), else from the records
-                    const uint32_t sa = dsamp[h / (DW / 32u)], sb = dsamp[h / (DW / 32u) + 1u];
-                    const uint32_t ls = sa == sb ? sa : (has ? A.len[wa + h] : 0u), js = w >> 24, ss = (w >> 16) & 0xFFu;
-                    const float bound = fgss_min_overlap(lq, ls, t);
-                    const uint32_t rq = lq - js, rs = ls - ss;
-                    const uint32_t minrem = rq < rs ? rq : rs;
-                    const bool pass = has && !((float)(m + minrem) < bound);   // filtering.cu:127-133
-                    cand += (uint32_t)__popcll(__ballot(pass));
-                    const bool keep = pass && (int)(m + minrem - 1) >= (int)floorf(bound);
-                    // the others lose their last match and their part of the count word
-                    // (the first lane of the word's slots rewrites it)
-                    *(has && !keep ? &dlast[h] : &qstage[lane]) = 0u;
-                    const uint64_t bal = __ballot(keep);
-                    const uint32_t qb = (uint32_t)(bal >> ((uint32_t)lane & ~(kDcSlots - 1u))) & ((1u << kDcSlots) - 1u);
-                    uint32_t keepbits = 0;
-#pragma unroll
-                    for (uint32_t i = 0; i < kDcSlots; i++) keepbits |= (qb >> i) & 1u ? kDcMask << (i * kDcBits) : 0u;
-                    *((h & (kDcSlots - 1u)) == 0 && cw != 0 ? &dcnt[h >> kDcShift] : &qstage[lane]) = cw & keepbits;
-                    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                    *(keep ? &clist[nl + below] : &qstage[lane]) = h | (ls << 16);
-                    nl += (uint32_t)__popcll(bal);
-                }
-                wave_sync();
-                PHASE_MARK(2);
-                nsig += nl;
-                const uint32_t nv = phase_c(nl);
-                PHASE_MARK(3);
-                nver += nv;
-                phase_d(nv);
-                PHASE_MARK(4);
-            }
         }
         WACC(W_JOB0 + 0, escan);
         WACC(W_JOB0 + 1, cand);
@@ -2091,11 +1824,6 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
         WACC(W_VER, nver);
         WACC(W_INTER, inter);
         WACC(W_VTOK, vtok);
-        if (dense) {
-            WACC(W_DENSE, 1);
-            WACC(W_DPOST, escan);
-            continue;
-        }
         if (nlist * 100 <= tsize * kSpanGrowPct) span = min(span * 2, kMaxSpan);
         else if (nlist * 100 > tsize * kSpanShrinkPct) span = max(span / 2, 1u);
         }   // windows
@@ -2109,6 +1837,520 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
     if (lane == kAccSlots) slot[lane] = cur_job;
 }
 
+// ---- Dense units (k_join_dense) ----
+// A dense unit joins one dense query against one part of the global grid of DW = 2 x CAP
+// records.  Its table is indexed by the source's offset in the grid part (record mod DW):
+// dcnt holds 8-bit match counts, four slots to a word, and dlast the last match as query
+// position << 24 | source position << 16 (both below 256: the probes of dense queries
+// hold at most 255 tokens, and a source's mid-prefix position lies inside its partner's
+// probe).  Both tables are zero between units: the positional filter clears every word
+// it reads.  A wavefront's LDS: the count tables of the block's wavefronts first (wave w
+// at byte w x DW, so that a slot's count word is an OR of its offset and that base), then
+// the last-match tables, then each wavefront's candidate list (two words an entry), its
+// probe-token table, a step's start marks and the part's length samples.
+constexpr uint32_t kDList = 256;                          // candidates listed at once
+constexpr uint32_t kDenseSamples = 33;                    // length samples: each 32nd of the part, and its end
+constexpr int kDRows = 6;                                 // rows of kWave postings per full walk step
+constexpr uint32_t kDMarkWords = kDRows * kWave / 4;      // start marks of a step, a byte per position
+constexpr uint32_t kDOwnWords = 2 * kDList + 2 * kStreamQp + kDMarkWords + kDenseSamples + 1;
+static_assert(kDOwnWords % 2 == 0 && 2 * kDList >= (uint32_t)kWave, "entries stay 8-byte aligned; a word per lane in the list");
+constexpr uint32_t kDBad = 0x100u;                        // list entry: the source's signature overflowed
+constexpr uint32_t dense_block_words(int cap) { return (uint32_t)join_waves(cap) * (2u * cap / 4u + 2u * cap + kDOwnWords); }
+#ifndef FGSS_DENSE_SIGPER
+#define FGSS_DENSE_SIGPER 2
+#endif
+
+template <int CAP>
+__device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, uint32_t rblocks)
+{
+    constexpr uint32_t DW = 2u * CAP;
+    constexpr int LOG2DW = CAP == 512 ? 10 : CAP == 1024 ? 11 : CAP == 2048 ? 12 : 13;
+    static_assert((1u << LOG2DW) == DW && DW <= 0xFFFFu, "grid parts of a power of two records, slots in 16 bits");
+    constexpr int NW = join_waves(CAP);
+    constexpr int PER = kStreamQp / kWave;   // probe tokens per lane
+    extern __shared__ uint32_t smem[];
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int lane = lane_id();
+    const uint32_t cbase = w * DW;   // byte address of the wave's count words (a multiple of DW)
+    uint32_t *const dlast = smem + (size_t)NW * (DW / 4u) + (size_t)w * DW;
+    uint32_t *const own = smem + (size_t)NW * (DW / 4u + DW) + (size_t)w * kDOwnWords;
+    uint2 *const list = (uint2 *)own;
+    uint32_t *const cur4 = own + 2 * kDList;   // per non-empty probe token, in token order: 4 x (first posting - flattened start)
+    uint32_t *const jw = cur4 + kStreamQp;     // ... and its query position
+    uint32_t *const mw = jw + kStreamQp;       // a step's start marks (words, for clearing)
+    uint8_t *const smark = (uint8_t *)mw;
+    uint32_t *const dsamp = mw + kDMarkWords;
+    uint32_t *const dump = own;                // during the walk (the list is empty): a word per lane for updates without work
+    auto cnt_word = [&](uint32_t h) -> uint32_t * {   // the count word of slot h
+        return (uint32_t *)((char *)smem + ((h & (DW - 4u)) | cbase));
+    };
+
+    const uint64_t n_items = A.n_tiles_dev == nullptr || *A.n_items_dev > A.n_items_cap ? 0ull : *A.n_tiles_dev * kDenseTile;
+    const uint32_t ndq = n_items ? *A.ndq_dev : 0u;
+    unsigned long long *const slot = A.wctr + ((size_t)rblock * NW + w) * kWaveSlot;
+    const uint64_t nwaves_grid = (uint64_t)rblocks * NW;
+    const uint64_t g0 = min((n_items + nwaves_grid - 1) / nwaves_grid, (uint64_t)kQueueBatch);
+    const uint64_t gw = (uint64_t)rblock * NW + w;
+    if (gw * g0 >= n_items && nwaves_grid * g0 >= n_items) {   // nothing to claim
+        if (lane < kAccSlots) slot[lane] = 0ull;
+        if (lane == kAccSlots) slot[lane] = kEmpty;
+        return;
+    }
+    for (uint32_t i = lane; i < DW / 4u; i += kWave) *cnt_word(i * 4u) = 0u;
+    for (uint32_t i = lane; i < DW; i += kWave) dlast[i] = 0u;
+    wave_sync();
+
+    unsigned long long wv = 0;   // counter k in lane k (see join_items)
+    unsigned long long c0 = clock64(), c1 = 0;
+    (void)c0;
+    (void)c1;
+    const uint64_t qbase = nwaves_grid * g0;
+    uint64_t batch_next = min(gw * g0, n_items), batch_end = min(gw * g0 + g0, n_items);
+    uint32_t cur_job = kEmpty;
+    float cur_t = 0.0f;
+    uint32_t ugp = 0;
+    uint64_t utlo = 1, uthi = 0, utile = ~0ull;
+    uint32_t tq = kEmpty, thi = 0, tlen = 0, tpos = 0, tjob = 0;
+    const uint32_t dng = (uint32_t)((A.n + DW - 1) / DW);
+    // the next unit's probe tokens and offset-table entries, loaded ahead (pf_it: the unit)
+    uint64_t pf_it = ~0ull;
+    uint32_t pf_qp = 0, pf_tok[PER], pf_lo[PER], pf_hi[PER];
+    for (;;) {
+        if (batch_next >= batch_end) {
+            if (qbase >= n_items) break;
+            unsigned long long b0 = 0;
+            if (lane == 0) b0 = atomicAdd(&A.ctr[C_QUEUE_D], (unsigned long long)kQueueBatch);
+            b0 = qbase + (((unsigned long long)__builtin_amdgcn_readfirstlane((uint32_t)(b0 >> 32)) << 32) |
+                          (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)b0));
+            batch_next = min((uint64_t)b0, n_items);
+            batch_end = min((uint64_t)b0 + kQueueBatch, n_items);
+        }
+        if (batch_next >= n_items) break;
+        const uint64_t it = batch_next++;
+        // unit it: dense query it % kDenseTile of tile it / kDenseTile in part-major order
+        // (k_dense_tiles); the grid part holding the tile is searched only when the tile
+        // leaves the last part's range.  The tile's last units may lie past the part's
+        // queries: their windows do not meet it.
+        const uint64_t tile = it / kDenseTile;
+        if (tile < utlo || tile >= uthi) {
+            uint32_t lo = 0, hi = dng;   // the last part g with tpre[g] <= tile
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (A.dtpre[mid] <= tile) lo = mid;
+                else hi = mid;
+            }
+            ugp = lo;
+            utlo = A.dtpre[lo];
+            uthi = A.dtpre[lo + 1];
+        }
+        if (tile != utile) {   // the tile's queries, one lane each, loaded together
+            utile = tile;
+            const uint64_t di = (uint64_t)A.dqlo[ugp] + (tile - utlo) * kDenseTile + (uint32_t)lane;
+            tq = lane < (int)kDenseTile && di < ndq ? A.dl[di] : kEmpty;
+            if (tq != kEmpty) {
+                thi = A.hi[tq];
+                tlen = A.len[tq];
+                tpos = A.pos[tq];
+                tjob = A.job[tq];
+            }
+        }
+        const uint32_t u = (uint32_t)(it % kDenseTile);
+        const uint32_t q = __builtin_amdgcn_readlane(tq, u);
+        if (q == kEmpty) continue;
+        const uint32_t gbase = ugp * DW;   // the grid part's first record
+        const uint32_t wa = max(q + 1u, gbase), wb = min(__builtin_amdgcn_readlane(thi, u), gbase + DW);
+        if (wa >= wb) continue;   // the query's window does not meet this grid part
+        const uint32_t jq = __builtin_amdgcn_readlane(tjob, u);
+        const float t = jq == cur_job ? cur_t : A.tj[jq];
+        const uint32_t lq = __builtin_amdgcn_readlane(tlen, u) & 0xFFFFu;
+        const uint32_t qpos = __builtin_amdgcn_readlane(tpos, u);
+        if (jq != cur_job) {   // per-job counters: flush when the job changes
+            if (cur_job != kEmpty) {
+#pragma unroll
+                for (int k = 0; k < kJobCtr; k++) {
+                    const unsigned long long v = (unsigned long long)__builtin_amdgcn_readlane((uint32_t)wv, W_JOB0 + k) |
+                                                 ((unsigned long long)__builtin_amdgcn_readlane((uint32_t)(wv >> 32), W_JOB0 + k) << 32);
+                    if (lane == 0 && v) atomicAdd(&A.ctr_job[(size_t)cur_job * kJobCtr + k], v);
+                }
+            }
+            wv = lane >= W_JOB0 && lane < W_JOB0 + kJobCtr ? 0ull : wv;
+            cur_job = jq;
+            cur_t = t;
+        }
+        const uint32_t qp = min(fgss_max_prefix(lq, t), lq);
+        // length samples: lane k < kDenseSamples reads the record at gbase + k DW / 32,
+        // clamped into [wa, wb) (lengths are non-increasing, so a source between two equal
+        // samples has their length); the load is stored after the walk
+        uint32_t lsamp = 0;
+        if (lane < (int)kDenseSamples) lsamp = A.len[min(max(gbase + (uint32_t)lane * (DW / 32u), wa), wb - 1u)];
+        PHASE_MARK(0);
+
+        // ---- A: the postings of the probe tokens in [wa, wb).  A token's postings in the
+        // grid part are offs[t][g] .. offs[t][g + 1] (k_dense_offs); in the window's first
+        // or last part they are narrowed to [wa, wb) by search.  The non-empty tokens are
+        // ranked in token order and their postings flattened: a step marks where each
+        // token starts, a row's marks become a ballot, and each lane's token is the number
+        // of starts at or before its position (mbcnt), so no posting needs a test. ----
+        PHASE_START(r0);
+        const uint32_t *const row0 = A.doffs + (size_t)ugp * A.dterms, *const row1 = row0 + A.dterms;
+        uint32_t lo[PER], dc[PER];
+        {
+            if (pf_it == it) {   // prefetched by the unit before
+#pragma unroll
+                for (int k = 0; k < PER; k++) {
+                    lo[k] = pf_lo[k];
+                    dc[k] = pf_hi[k];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < PER; k++) {
+                    const uint32_t j = (uint32_t)lane * PER + k;
+                    uint32_t a = 0, b = 0;
+                    if (j < qp) {
+                        const uint32_t tk = A.qtok[qpos + j];
+                        a = row0[tk];
+                        b = row1[tk];
+                    }
+                    lo[k] = a;
+                    dc[k] = b;
+                }
+            }
+            // the next unit of the tile (the same grid part): its probe tokens now, their
+            // offset-table entries after the walk
+            pf_it = ~0ull;
+            if (u + 1u < kDenseTile && it + 1 < batch_end) {
+                const uint32_t q2 = __builtin_amdgcn_readlane(tq, u + 1u);
+                if (q2 != kEmpty) {
+                    const uint32_t jq2 = __builtin_amdgcn_readlane(tjob, u + 1u);
+                    const float t2 = jq2 == jq ? t : A.tj[jq2];
+                    const uint32_t lq2 = __builtin_amdgcn_readlane(tlen, u + 1u) & 0xFFFFu;
+                    const uint32_t qpos2 = __builtin_amdgcn_readlane(tpos, u + 1u);
+                    pf_qp = min(fgss_max_prefix(lq2, t2), lq2);
+#pragma unroll
+                    for (int k = 0; k < PER; k++) {
+                        const uint32_t j = (uint32_t)lane * PER + k;
+                        pf_tok[k] = j < pf_qp ? A.qtok[qpos2 + j] : 0u;
+                    }
+                    pf_it = it + 1;
+                }
+            }
+            if (wa > gbase || wb < gbase + DW) {   // the window's first or last part
+#pragma unroll
+                for (int k = 0; k < PER; k++) {
+                    if (dc[k] > lo[k]) {
+                        uint32_t a, b;
+                        lower_bound_sid2(A.psid, A.fence, lo[k], dc[k], wa, lo[k], dc[k], wb, a, b);
+                        lo[k] = a;
+                        dc[k] = b;
+                    }
+                }
+            }
+        }
+        uint32_t sdc = 0, nz = 0;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            dc[k] -= lo[k];
+            sdc += dc[k];
+            nz += dc[k] != 0 ? 1u : 0u;
+        }
+        // one scan carries both the flattened starts (low 22 bits: at most 255 x 8192
+        // postings) and the ranks of the non-empty tokens
+        const uint32_t pk = sdc | nz << 22;
+        const uint32_t inc = wave_inclusive_scan_dpp(pk);
+        uint32_t run = (inc - pk) & 0x3FFFFFu, rk = (inc - pk) >> 22;
+        uint32_t dbeg[PER];   // flattened start of each token (~0: empty)
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const bool ne = dc[k] != 0;
+            *(ne ? &cur4[rk] : &dump[lane]) = (lo[k] - run) * 4u;
+            *(ne ? &jw[rk] : &dump[lane]) = (uint32_t)lane * PER + k;
+            dbeg[k] = ne ? run : ~0u;
+            rk += ne ? 1u : 0u;
+            run += dc[k];
+        }
+        const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)inc, kWave - 1) & 0x3FFFFFu;
+        PHASE_ADD(5, r0);
+        const char *const pdb = (const char *)A.pden;
+        uint32_t rb = 0;   // non-empty tokens starting before the current row
+        // a step of N rows from flattened position e0; CHECK: the last step, whose rows
+        // past nrows are skipped and whose lanes past the end neither load nor update
+        auto wstep = [&](auto rows, auto check, uint32_t e0, uint32_t nrows) {
+            constexpr int N = decltype(rows)::value;
+            constexpr bool CHECK = decltype(check)::value;
+            wave_sync();
+#pragma unroll
+            for (uint32_t i = 0; i < (uint32_t)N * 16u; i += kWave)
+                if (i + lane < (uint32_t)N * 16u) mw[i + lane] = 0u;
+            wave_sync();
+#pragma unroll
+            for (int k = 0; k < PER; k++) {
+                const uint32_t at = dbeg[k] - e0;   // wraps above the step for starts before e0
+                *(at < (uint32_t)(N * kWave) ? smark + at : (uint8_t *)(dump + lane)) = 1u;
+            }
+            wave_sync();
+            uint32_t pw[N], jv[N];
+#pragma unroll
+            for (int r = 0; r < N; r++) {
+                if (CHECK && (uint32_t)r >= nrows) break;
+                const uint64_t M = __ballot(smark[r * kWave + lane] != 0);
+                const uint64_t X = M >> 1;
+                const uint32_t acc = rb + (uint32_t)(M & 1ull) - 1u;
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(X >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)X, acc));
+                rb += (uint32_t)__popcll(M);
+                const uint32_t c4 = cur4[rank];
+                jv[r] = jw[rank];
+                uint32_t p = e0 + (uint32_t)(r * kWave) + (uint32_t)lane;
+                if (CHECK) p = min(p, total - 1u);
+                pw[r] = *(const uint32_t *)(pdb + (c4 + 4u * p));
+            }
+#pragma unroll
+            for (int r = 0; r < N; r++) {
+                if (CHECK && (uint32_t)r >= nrows) break;
+                const uint32_t h = __builtin_amdgcn_ubfe(pw[r], 8u, (uint32_t)LOG2DW);
+                const uint32_t add = 1u << (h << 3);   // the shift count's low five bits: (h & 3) x 8
+                const uint32_t last = __builtin_amdgcn_perm(jv[r], pw[r], 0x04000C0Cu);
+                if (CHECK) {
+                    const bool v = e0 + (uint32_t)(r * kWave) + (uint32_t)lane < total;
+                    atomicAdd(v ? cnt_word(h) : &dump[lane], add);
+                    atomicMax(v ? &dlast[h] : &dump[lane], last);
+                } else {
+                    atomicAdd(cnt_word(h), add);
+                    atomicMax(&dlast[h], last);
+                }
+            }
+        };
+        uint32_t e0 = 0;
+        for (; e0 + (uint32_t)(kDRows * kWave) <= total; e0 += kDRows * kWave)
+            wstep(std::integral_constant<int, kDRows>(), std::false_type(), e0, 0u);
+        if (e0 < total) {
+            const uint32_t nrows = (total - e0 + kWave - 1u) / kWave;   // 1 .. kDRows
+            if (nrows > 2u) wstep(std::integral_constant<int, kDRows>(), std::true_type(), e0, nrows);
+            else wstep(std::integral_constant<int, 2>(), std::true_type(), e0, nrows);
+        }
+        if (pf_it == it + 1) {
+#pragma unroll
+            for (int k = 0; k < PER; k++) {
+                const bool in = (uint32_t)lane * PER + k < pf_qp;
+                pf_lo[k] = in ? row0[pf_tok[k]] : 0u;
+                pf_hi[k] = in ? row1[pf_tok[k]] : 0u;
+            }
+        }
+        wave_sync();
+        if (lane < (int)kDenseSamples) dsamp[lane] = lsamp;
+        wave_sync();
+        PHASE_MARK(1);
+
+        uint32_t cand = 0, nsig = 0, nver = 0, inter = 0, vtok = 0, nres = 0, nl = 0;
+        // ---- C, C2 and D over list[0, nl) ----
+        auto flush = [&]() {
+            wave_sync();
+            PHASE_MARK(2);
+            nsig += nl;
+            // C: 64 buckets, FGSS_DENSE_SIGPER candidates per lane and step; the query's
+            // counts are read once into scalar registers.  Survivors (and sources whose
+            // counts overflowed, marked kDBad) are compacted to the front of the list.
+#ifndef FGSS_DENSE_QSCALAR
+#define FGSS_DENSE_QSCALAR 1
+#endif
+#if FGSS_DENSE_QSCALAR
+            typedef __attribute__((address_space(4))) const uint32_t cu32;
+            const cu32 *qhp = (const cu32 *)(A.qsig64 + (size_t)q * kSig64Words);
+#else
+            const uint32_t *qhp = A.qsig64 + (size_t)q * kSig64Words;
+#endif
+            uint32_t qh[kSig64Words];
+#pragma unroll
+            for (int k = 0; k < kSig64Words; k++) qh[k] = qhp[k];
+            const bool qbad = A.qsigbad[q] != 0;
+            constexpr int SP = FGSS_DENSE_SIGPER;
+            uint32_t nv = 0, nex = 0;
+            for (uint32_t base = 0; base < nl; base += kWave * SP) {
+                uint2 ent[SP];
+                uint32_t s[SP];
+                uint8_t sbad[SP];
+                uint4 b4[SP][kSig64Words / 4];
+#pragma unroll
+                for (int v = 0; v < SP; v++) {
+                    ent[v] = list[min(base + (uint32_t)(v * kWave) + (uint32_t)lane, nl - 1u)];
+                    s[v] = gbase + (ent[v].x & 0xFFFFu);
+                }
+#pragma unroll
+                for (int v = 0; v < SP; v++) {
+                    sbad[v] = A.sigbad[s[v]];
+                    const uint4 *sh4 = (const uint4 *)(A.sig64 + (size_t)s[v] * kSig64Words);
+#pragma unroll
+                    for (int k = 0; k < kSig64Words / 4; k++) b4[v][k] = sh4[k];
+                }
+                bool surv[SP];
+#pragma unroll
+                for (int v = 0; v < SP; v++) {
+                    const bool in = base + (uint32_t)(v * kWave) + (uint32_t)lane < nl;
+                    const uint32_t ls = ent[v].x >> 16;
+                    const int floorb = (int)floorf(fgss_min_overlap(lq, ls, t));
+                    uint32_t sad = 0;
+#pragma unroll
+                    for (int k = 0; k < kSig64Words / 4; k++) {
+                        sad = __builtin_amdgcn_sad_u8(b4[v][k].x, qh[4 * k], sad);
+                        sad = __builtin_amdgcn_sad_u8(b4[v][k].y, qh[4 * k + 1], sad);
+                        sad = __builtin_amdgcn_sad_u8(b4[v][k].z, qh[4 * k + 2], sad);
+                        sad = __builtin_amdgcn_sad_u8(b4[v][k].w, qh[4 * k + 3], sad);
+                    }
+                    const bool bad = qbad || sbad[v] != 0;
+                    // sum over buckets of min(cQ, cS) = (|Q| + |S| - SAD) / 2 >= |Q n S|
+                    const bool ok = ((int)(lq + ls) - (int)sad) / 2 >= floorb;
+                    nex += (uint32_t)__popcll(__ballot(in && !bad && ok));
+                    surv[v] = in && (bad || ok);
+                    ent[v].y |= bad ? kDBad : 0u;
+                }
+                // every entry of this step is in registers: compact in place (the writes
+                // land at or below the entries just read)
+                wave_sync();
+#pragma unroll
+                for (int v = 0; v < SP; v++) {
+                    const uint64_t bal = __ballot(surv[v]);
+                    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                    if (surv[v]) list[nv + below] = ent[v];
+                    nv += (uint32_t)__popcll(bal);
+                }
+            }
+            WACC(W_SIG2, nex);
+            wave_sync();
+            // C2: the survivors with good counts against the 128-bucket counts, one per lane
+            uint32_t nv2 = 0;
+            {
+                const uint4 *qs4 = (const uint4 *)(A.qsig + (size_t)q * kSigWords);
+                for (uint32_t base = 0; base < nv; base += kWave) {
+                    const bool in = base + (uint32_t)lane < nv;
+                    const uint2 ent = list[min(base + (uint32_t)lane, nv - 1u)];
+                    const bool bad = (ent.y & kDBad) != 0;
+                    bool keep = in;
+                    if (__ballot(in && !bad) != 0) {
+                        if (in && !bad) {
+                            const uint32_t s = gbase + (ent.x & 0xFFFFu), ls = ent.x >> 16;
+                            const uint4 *ss4 = (const uint4 *)(A.sig + (size_t)s * kSigWords);
+                            uint32_t sad2 = 0;
+#pragma unroll
+                            for (int k = 0; k < kSigWords / 4; k++) {
+                                const uint4 a4 = qs4[k], c4 = ss4[k];
+                                sad2 = __builtin_amdgcn_sad_u8(a4.x, c4.x, sad2);
+                                sad2 = __builtin_amdgcn_sad_u8(a4.y, c4.y, sad2);
+                                sad2 = __builtin_amdgcn_sad_u8(a4.z, c4.z, sad2);
+                                sad2 = __builtin_amdgcn_sad_u8(a4.w, c4.w, sad2);
+                            }
+                            keep = ((int)(lq + ls) - (int)sad2) / 2 >= (int)floorf(fgss_min_overlap(lq, ls, t));
+                        }
+                    }
+                    wave_sync();
+                    const uint64_t bal = __ballot(keep);
+                    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                    if (keep) list[nv2 + below] = ent;
+                    nv2 += (uint32_t)__popcll(bal);
+                }
+            }
+            wave_sync();
+            nver += nv2;
+            PHASE_MARK(3);
+            // D: verification (checking.cu:93-175): the source tokens after the last
+            // prefix match, each searched in the query after its last match
+            const uint32_t *Qv = A.qtok + qpos;
+            for (uint32_t c = 0; c < nv2; c++) {
+                const uint2 ent = list[c];
+                const uint32_t s = gbase + (ent.x & 0xFFFFu), ls = ent.x >> 16;
+                const uint32_t m = ent.y & 0xFFu, js = ent.y >> 24, ss = (ent.y >> 16) & 0xFFu;
+                const uint32_t spos = A.pos[s];
+                const int floorb = (int)floorf(fgss_min_overlap(lq, ls, t));
+                uint32_t ov = m;
+                uint32_t base = ss + 1;
+                for (; base < ls; base += kWave) {
+                    const uint32_t i = base + lane;
+                    bool hit = false;
+                    if (i < ls) {
+                        const uint32_t tk = A.tok[spos + i];
+                        uint32_t a = js + 1, b = lq;
+                        while (a < b) {
+                            const uint32_t mid = (a + b) >> 1;
+                            if (Qv[mid] < tk) a = mid + 1;
+                            else b = mid;
+                        }
+                        hit = a < lq && Qv[a] == tk;
+                    }
+                    ov += (uint32_t)__popcll(__ballot(hit));
+                    const int rem = (int)ls - (int)(base + kWave);
+                    if ((int)ov + (rem > 0 ? rem : 0) < floorb) { base += kWave; break; }
+                }
+                vtok += min(base, ls) - (ss + 1);
+                if ((int)ov >= floorb) {
+                    inter++;
+                    if (fgss_similarity(lq, ls, ov) >= t) {   // block.cu:189-192
+                        nres++;
+                        if (lane == 0) {
+                            const uint32_t r = (uint32_t)atomicAdd(&A.ctr[C_RESULTS], 1ull);
+                            if (r < A.out_cap) {
+                                A.out[3 * (size_t)r] = (int32_t)q;
+                                A.out[3 * (size_t)r + 1] = (int32_t)s;
+                                A.out[3 * (size_t)r + 2] = (int32_t)ov;
+                            }
+                        }
+                    }
+                }
+            }
+            PHASE_MARK(4);
+            nl = 0;
+        };
+
+        // ---- B: the positional filter (filtering.cu:127-133) over the unit's slots, 64
+        // at a time: every word read is cleared; the candidates that can still reach
+        // floor(bound) are listed with their source length, count and last match.  A
+        // full list goes through C, C2 and D before the scan goes on. ----
+        const uint32_t slo = wa - gbase, shi = wb - gbase;
+        const uint32_t csh = ((uint32_t)lane & 3u) * 8u;
+        for (uint32_t h0 = slo & ~(uint32_t)(kWave - 1); h0 < shi; h0 += kWave) {
+            const uint32_t h = h0 + (uint32_t)lane;
+            const uint32_t cw = *cnt_word(h);
+            const uint32_t m = (cw >> csh) & 0xFFu;
+            const bool has = m != 0;
+            if (__ballot(has) == 0) continue;   // slots outside [wa, wb) are never touched
+            const uint32_t wl = dlast[h];
+            const uint32_t sg = h >> (LOG2DW - 5);
+            const uint32_t sa = dsamp[sg], sb = dsamp[sg + 1u];
+            uint32_t ls = sa;
+            if (sa != sb && has) ls = A.len[gbase + h];
+            dlast[h] = 0u;
+            if ((lane & 3) == 0) *cnt_word(h) = 0u;
+            const uint32_t js = wl >> 24, ss = (wl >> 16) & 0xFFu;
+            const float bound = fgss_min_overlap(lq, ls, t);
+            const uint32_t rq = lq - js, rs = ls - ss;
+            const uint32_t x = m + (rq < rs ? rq : rs);
+            const bool pass = has && !((float)x < bound);
+            cand += (uint32_t)__popcll(__ballot(pass));
+            // overlap can still reach floor(bound): m + min(|Q|-js-1, |S|-ss-1)
+            const bool keep = pass && (int)(x - 1u) >= (int)floorf(bound);
+            const uint64_t bal = __ballot(keep);
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            if (keep) list[nl + below] = make_uint2(h | ls << 16, m | (wl & 0xFFFF0000u));
+            nl += (uint32_t)__popcll(bal);
+            if (nl + (uint32_t)kWave > kDList) flush();
+        }
+        if (nl) flush();
+        wave_sync();
+        WACC(W_JOB0 + 0, total);
+        WACC(W_JOB0 + 1, cand);
+        WACC(W_JOB0 + 2, inter);
+        WACC(W_JOB0 + 3, nres);
+        WACC(W_ESCAN, total);
+        WACC(W_CAND, cand);
+        WACC(W_SIG, nsig);
+        WACC(W_VER, nver);
+        WACC(W_INTER, inter);
+        WACC(W_VTOK, vtok);
+        WACC(W_DENSE, 1);
+        WACC(W_DPOST, total);
+    }
+    wave_sync();
+    if (lane < kAccSlots) slot[lane] = wv;
+    if (lane == kAccSlots) slot[lane] = cur_job;
+}
+
 // Two kernels on the join's stream, one per role, so that each keeps its registers for
 // one kind of work (one kernel with both roles spilled the dense loop's registers to
 // scratch): k_join_dense walks the dense units, then k_join the item list.  Their
@@ -2117,12 +2359,12 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
 template <int CAP>
 __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join_dense(JoinArgs A)
 {
-    if constexpr (CAP >= 512) join_items<CAP, false, true>(A, blockIdx.x, gridDim.x, blockIdx.x);
+    if constexpr (CAP >= 512) dense_units<CAP>(A, blockIdx.x, gridDim.x);
 }
 template <int CAP, bool RS>
 __global__ __launch_bounds__(join_waves(CAP) * kWave, 4) void k_join(JoinArgs A)
 {
-    join_items<CAP, RS, false>(A, blockIdx.x, gridDim.x, blockIdx.x + A.dense_blocks);
+    join_items<CAP, RS>(A, blockIdx.x, gridDim.x, blockIdx.x + A.dense_blocks);
 }
 // Sum the per-wavefront slots of a k_join launch into the global counters and the
 // per-job counters: a thread per slot.  Global counters: a wavefront sum, then one
@@ -2178,14 +2420,14 @@ template <int CAP, bool RS>
 static int launch_join(fgss_device_sets *d, const JoinArgs &A0, bool dry, bool with_dense)
 {
     static thread_local int cached[3] = {-1, -1, -1};   // blocks, dense-kernel blocks, device
-    const size_t smem = join_smem_bytes<CAP>();
+    const size_t smem = join_smem_bytes<CAP>(), smem_d = (size_t)dense_block_words(CAP) * sizeof(uint32_t);
     if (cached[0] < 0 || cached[2] != d->device) {   // runtime queries: never inside a graph capture
         int per_cu = 0, per_cu_d = 0, cus = 0;
         HIPCK(hipFuncSetAttribute((const void *)k_join<CAP, RS>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
         HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_join<CAP, RS>, join_waves(CAP) * kWave, smem));
         if (CAP >= 512 && !RS) {
-            HIPCK(hipFuncSetAttribute((const void *)k_join_dense<CAP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-            HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_d, k_join_dense<CAP>, join_waves(CAP) * kWave, smem));
+            HIPCK(hipFuncSetAttribute((const void *)k_join_dense<CAP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem_d));
+            HIPCK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_d, k_join_dense<CAP>, join_waves(CAP) * kWave, smem_d));
         }
         HIPCK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->device));
         cached[0] = std::max(per_cu, 1) * cus;
@@ -2200,7 +2442,7 @@ static int launch_join(fgss_device_sets *d, const JoinArgs &A0, bool dry, bool w
     if ((uint64_t)nw * kWaveSlot * 8 > d->b_wctr.cap)
         return fgss_fail(FGSS_ERR_DEVICE, "internal: wavefront counter slots too small");
     if (A.dense_blocks) {
-        hipLaunchKernelGGL((k_join_dense<CAP>), dim3(A.dense_blocks), dim3(join_waves(CAP) * kWave), smem, d->st, A);
+        hipLaunchKernelGGL((k_join_dense<CAP>), dim3(A.dense_blocks), dim3(join_waves(CAP) * kWave), smem_d, d->st, A);
         HIPCK(hipGetLastError());
     }
     hipLaunchKernelGGL((k_join<CAP, RS>), dim3(blocks), dim3(join_waves(CAP) * kWave), smem, d->st, A);
@@ -2316,7 +2558,8 @@ static int wave_slots(fgss_device_sets *d)
 // entries and probe tokens <= tokens) and for the current capacities otherwise.
 // dense parts need record ids below 2^24 (their 4-byte postings) and an offset table of
 // terms x grid parts words (k_dense_offs) within kDenseTableBudget at the finest grid
-static bool dense_ok(const fgss_device_sets *d) { return d->n < ((int64_t)1 << 24); }
+// dense units: record ids in 24 bits of a dense posting, and posting byte offsets in 32 bits
+static bool dense_ok(const fgss_device_sets *d) { return d->n < ((int64_t)1 << 24) && d->ntok + kWave < ((int64_t)1 << 30); }
 constexpr uint32_t kDenseGridMin = 1024;                   // records per grid part at 512 table slots
 constexpr int64_t kDenseTableBudget = (int64_t)2 << 30;   // bytes
 static int64_t dense_grid_parts(const fgss_device_sets *d, uint32_t dw) { return (d->n + dw - 1) / dw; }
@@ -2400,7 +2643,7 @@ static JoinArgs join_args(fgss_device_sets *d)
     A.qtok = d->tok; A.qsig = d->sig; A.qsig64 = d->sig64; A.qsigbad = d->sigbad;
     A.rng = P<uint2>(d->b_rng); A.poff = P<uint32_t>(d->b_poff); A.bounds = P<uint32_t>(d->b_bounds);
     A.pden = dense_ok(d) ? P<uint32_t>(d->b_pden) : nullptr;
-    A.doffs = P<uint32_t>(d->b_doffs); A.dqlo = P<uint32_t>(d->b_dq); A.dtpre = P<uint64_t>(d->b_dtpre);
+    A.doffs = P<uint32_t>(d->b_doffs); A.dterms = (uint32_t)d->n_terms; A.dqlo = P<uint32_t>(d->b_dq); A.dtpre = P<uint64_t>(d->b_dtpre);
     A.dl = P<uint32_t>(d->b_dlist); A.ndq_dev = P<uint32_t>(d->b_misc) + M_NDENSE;
     A.n_tiles_dev = d->b_doffs.p ? (const unsigned long long *)(P<uint32_t>(d->b_misc) + M_NDU64) : nullptr;
     A.hi = P<uint32_t>(d->b_hi); A.nparts = P<uint32_t>(d->b_nparts); A.n = d->n;

@@ -94,3 +94,74 @@ def hip_runtime():
     h.hipStreamDestroy.argtypes = [C.c_void_p]
     h.hipStreamSynchronize.argtypes = [C.c_void_p]
     return h
+
+
+def filter_counts(coll, t):
+    """This build's candidate filters restated on the host for a collection in size order
+    (lengths non-increasing): (candidates, positional-filter survivors, survivors whose
+    64-bucket counts pass the overlap bound).  The survivors are the candidates whose
+    count plus the tokens left after the last prefix match can still reach floor(bound);
+    the bound (|Q| + |S| - SAD) / 2 over 64 byte counts, bucket (token x 0x9E3779B1) >> 26
+    of 32 bits, is skipped (the candidate kept) when a count of either record exceeds 255.
+    The join reports them as candidates, signature_checks and signature_checks_128."""
+    _, lens, pos, tok = coll.arrays()
+    lens = lens.astype(np.int64)
+    pos = pos.astype(np.int64)
+    tok = tok.astype(np.int64)
+    n = len(lens)
+    f32 = np.float32
+    t = f32(t)
+    r = t / (f32(1) + t)
+    lf = lens.astype(f32)
+    max_pre = np.minimum((lf - np.ceil(lf * t) + f32(1)).astype(np.int64), lens)
+    mid_pre = np.minimum((lf - np.ceil(r * (lf + lf)) + f32(1)).astype(np.int64), lens)
+    fmin = np.floor(lf * t).astype(np.int64)
+    fmax = np.floor(lf / t).astype(np.int64)
+    # 64-bucket counts per record
+    b64 = ((tok * 0x9E3779B1) & 0xFFFFFFFF) >> 26
+    rec = np.repeat(np.arange(n), lens)
+    sig = np.zeros((n, 64), np.int64)
+    np.add.at(sig, (rec, b64), 1)
+    bad = (sig > 255).any(axis=1)
+    # index of the mid-prefix tokens: (token, record, position), grouped by token
+    src = np.repeat(np.arange(n), mid_pre)
+    off = np.arange(len(src)) - np.repeat(np.cumsum(mid_pre) - mid_pre, mid_pre)
+    ptok = tok[pos[src] + off]
+    order = np.lexsort((src, ptok))
+    ptok, psrc, ppos = ptok[order], src[order], off[order]
+    starts = np.searchsorted(ptok, np.arange(int(tok.max(initial=0)) + 2))
+    cand = keep = pass64 = 0
+    for q in range(n):
+        lq = int(lens[q])
+        qt = tok[pos[q]:pos[q] + max_pre[q]]
+        if len(qt) == 0:
+            continue
+        a, b = starts[qt], starts[qt + 1]
+        cnt = b - a
+        if cnt.sum() == 0:
+            continue
+        idx = np.repeat(a - np.cumsum(cnt) + cnt, cnt) + np.arange(cnt.sum())
+        js = np.repeat(np.arange(len(qt)), cnt)
+        s, ss = psrc[idx], ppos[idx]
+        ls = lens[s]
+        ok = (s > q) & ~((ls < fmin[q]) | (ls > fmax[q]) | (lq < fmin[s]) | (lq > fmax[s]))
+        s, ss, js = s[ok], ss[ok], js[ok]
+        if len(s) == 0:
+            continue
+        o = np.lexsort((js, s))   # per source, its last match last
+        s, ss, js = s[o], ss[o], js[o]
+        last = np.r_[s[1:] != s[:-1], True]
+        m = np.bincount(np.searchsorted(np.unique(s), s))
+        s, ss, js = s[last], ss[last], js[last]
+        ls = lens[s]
+        bound = r * (f32(lq) + ls.astype(f32)).astype(f32)
+        x = m + np.minimum(lq - js, ls - ss)
+        ps = ~(x.astype(f32) < bound)
+        kp = ps & (x - 1 >= np.floor(bound).astype(np.int64))
+        cand += int(ps.sum())
+        keep += int(kp.sum())
+        sad = np.abs((sig[s[kp]] & 255) - (sig[q] & 255)).sum(axis=1)
+        fb = np.floor(bound[kp]).astype(np.int64)
+        good = ~(bad[q] | bad[s[kp]])
+        pass64 += int((good & ((lq + ls[kp] - sad) // 2 >= fb)).sum())
+    return cand, keep, pass64

@@ -10,7 +10,7 @@ import sys
 import numpy as np
 import pytest
 
-from _util import assert_join_equal, dblp_text, golden_path, oracle_sets
+from _util import assert_join_equal, dblp_text, filter_counts, golden_path, oracle_sets
 
 pytestmark = pytest.mark.gpu
 
@@ -287,6 +287,22 @@ def test_resplit_round_larger_than_the_result_buffer(fg, oracle_mod):
     assert_join_equal(oracle_mod, sets, p, 0.5, counters=want)
     _say("resplit retry: the same collection again (buffers now sized)")
     assert_join_equal(oracle_mod, sets, coll.selfjoin(0.5, table_slots=256), 0.5, counters=want)
+
+
+@pytest.mark.parametrize("dense_pct", [-1, 1])
+def test_filter_counters(fg, dense_pct):
+    """The counters of the filters after the reference's candidates -- positional-filter
+    survivors (signature_checks) and those passing the 64-bucket bound
+    (signature_checks_128) -- equal their host restatement, in hash-table parts (-1) and
+    in dense parts (1)."""
+    cases = [(fg.Collection.generate(fg.GEN_TITLES, 4096, 1), 0.6), (fg.Collection.generate(fg.GEN_LONG, 3000, 13), 0.8)]
+    for coll, t in cases:
+        t = float(np.float32(t))
+        cand, keep, pass64 = filter_counts(coll, t)
+        _say(f"filter counters dense_pct={dense_pct} n={coll.info.n_sets} t={t}")
+        st = coll.selfjoin(t, dense_pct=dense_pct).stats
+        assert (st["candidates"], st["signature_checks"], st["signature_checks_128"]) == (cand, keep, pass64)
+        assert st["verified"] <= pass64
 
 
 @pytest.mark.parametrize("dense_pct", [1, 0, -1])

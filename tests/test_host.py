@@ -7,7 +7,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from _util import dblp_text, golden_path
+from _util import dblp_text, filter_counts, golden_path, oracle_sets
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -103,6 +103,19 @@ def test_generator_is_seeded_and_shaped(fg):
     long = fg.generate_text(fg.GEN_LONG, 200, 3).split(b"\n")[:-1]
     ll = np.array([len(x) for x in long])
     assert 180 <= ll.min() and ll.max() <= 1215 and 550 < ll.mean() < 850
+
+
+@pytest.mark.parametrize("kind,n,seed,t", [(0, 4096, 1, 0.6), (1, 3000, 13, 0.8), (0, 3000, 5, 0.9)])
+def test_filter_restatement_candidates_match_oracle(fg, oracle_mod, kind, n, seed, t):
+    """The host restatement of the join's filters (the checker of signature_checks and
+    signature_checks_128 in the GPU tests) counts the oracle's candidates, and its later
+    filters only narrow them."""
+    coll = fg.Collection.generate(kind, n, seed)
+    t = float(np.float32(t))
+    cand, keep, pass64 = filter_counts(coll, t)
+    _, ctr = oracle_mod.selfjoin(oracle_sets(oracle_mod, coll), t)
+    assert cand == ctr["candidates"]
+    assert ctr["intersections"] <= pass64 <= keep <= cand
 
 
 def test_errors_are_status_codes(fg, tmp_path):
