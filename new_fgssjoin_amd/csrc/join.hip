@@ -535,14 +535,13 @@ __global__ void k_list_offsets(const uint32_t *__restrict__ skeys, const uint32_
 // units run part by part, so the rows in use stay in L2.  A thread per posting writes the
 // entries of the grid parts that end at it (those past the previous posting's part, up to
 // its own), the last posting of a list also those after it; a thread per term fills the
-// entries of an empty list.  Nothing to do when the run planned no dense parts.
+// entries of an empty list.  It runs before the planning (k_plan counts the postings of
+// the queries that may be dense in it).
 __global__ __launch_bounds__(256) void k_dense_offs(const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ psid,
                                                     const uint32_t *__restrict__ loff, const uint32_t *__restrict__ nE_dev,
                                                     uint32_t n_terms, uint32_t dense_w, uint32_t ng,
-                                                    const unsigned long long *__restrict__ ndparts,
                                                     uint32_t *__restrict__ offs)
 {
-    if (*ndparts == 0) return;
     const uint32_t nE = *nE_dev;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nE; i += gridDim.x * blockDim.x) {
         const uint32_t t = skeys[i], g = psid[i] / dense_w;
@@ -744,7 +743,8 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
                                               uint32_t *__restrict__ nparts, uint32_t *__restrict__ eq_out,
                                               uint64_t *__restrict__ bcnt, uint32_t stream_parts, uint32_t dense_w,
                                               uint32_t dense_pct, unsigned long long *__restrict__ ndparts,
-                                              uint32_t block_records, uint32_t *__restrict__ dflag)
+                                              uint32_t block_records, uint32_t *__restrict__ dflag,
+                                              const uint32_t *__restrict__ doffs, uint32_t dterms)
 {
     uint32_t dparts = 0;   // dense parts of this lane's queries
     const int64_t ngroups = (int64_t)gridDim.x * (blockDim.x / kPlanGroup);
@@ -774,36 +774,55 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
         }
         const uint32_t wend = lo;   // window is [q+1, wend)
         const uint32_t ro = poff[q];
+        // parts of e postings: ceil(e / target); a query of more than kHeavyParts parts
+        // fills its parts fuller (fewer parts -- boundary rows, streamed rounds -- outweigh
+        // the longer probe chains there); a part spans at most block_records sources
+        auto parts_of = [&](uint32_t e) -> uint32_t {
+            uint32_t np = e == 0 ? 0u : (e + target - 1) / target;
+            if (np > kHeavyParts) np = (e + heavy_target - 1) / heavy_target;
+            if (block_records != 0 && e != 0) np = max(np, (wend - (uint32_t)q - 1u + block_records - 1u) / block_records);
+            return np;
+        };
+        // dense parts: a window of several parts whose postings fill dense parts of
+        // dense_w consecutive sources to dense_pct% of their slots on average, for a probe
+        // that fits the cursor slots (windows this full would need hash-table parts of a
+        // few postings per probe token).  Dense parts follow the global grid of dense_w
+        // records (the window's first and last parts are clipped to it), and the postings
+        // counted for the test are those of the grid parts the window meets, two entries
+        // of the per-part offset table per probe token: a dense query needs no searches
+        // here (its units narrow the first and last part themselves).
+        const uint32_t win = wend - (uint32_t)q - 1u;
+        const uint32_t g0 = dense_w ? ((uint32_t)q + 1u) / dense_w : 0u, g1 = dense_w && win ? (wend - 1u) / dense_w : 0u;
+        const uint32_t ndp = dense_w && win ? g1 - g0 + 1u : 0u;
+        bool dense = false;
         uint32_t e = 0;
-        for (uint32_t j = gl; j < qp; j += kPlanGroup) {
-            uint32_t term = tok[pos[q] + j];
-            uint32_t L0 = loff[term], L1 = loff[term + 1];
-            uint32_t a, b;   // window [q + 1, wend), wend >= q + 1
-            lower_bound_sid2(psid, fence, L0, L1, (uint32_t)q + 1, L0, L1, wend, a, b);
-            rng[ro + j] = make_uint2(a, b);
-            e += b - a;
-        }
+        if (!unsorted && ndp > 1 && qp < kStreamQp) {   // uniform over the group
+            const uint32_t *r0 = doffs + (size_t)g0 * dterms, *r1 = doffs + (size_t)(g1 + 1u) * dterms;
+            for (uint32_t j = gl; j < qp; j += kPlanGroup) {
+                const uint32_t term = tok[pos[q] + j];
+                e += r1[term] - r0[term];
+            }
 #pragma unroll
-        for (int d = kPlanGroup / 2; d > 0; d >>= 1) e += __shfl_xor(e, d, kPlanGroup);
+            for (int d = kPlanGroup / 2; d > 0; d >>= 1) e += __shfl_xor(e, d, kPlanGroup);
+            dense = parts_of(e) > 1 && (uint64_t)e * 100u >= (uint64_t)ndp * dense_w * dense_pct;
+        }
+        if (!dense) {   // the posting range of each probe token inside the window
+            e = 0;
+            for (uint32_t j = gl; j < qp; j += kPlanGroup) {
+                uint32_t term = tok[pos[q] + j];
+                uint32_t L0 = loff[term], L1 = loff[term + 1];
+                uint32_t a, b;   // window [q + 1, wend), wend >= q + 1
+                lower_bound_sid2(psid, fence, L0, L1, (uint32_t)q + 1, L0, L1, wend, a, b);
+                rng[ro + j] = make_uint2(a, b);
+                e += b - a;
+            }
+#pragma unroll
+            for (int d = kPlanGroup / 2; d > 0; d >>= 1) e += __shfl_xor(e, d, kPlanGroup);
+        }
         if (live && gl == 0) {
             hi_out[q] = wend;
             eq_out[q] = e;
-            uint32_t np = e == 0 ? 0u : (e + target - 1) / target;
-            // a query of more than kHeavyParts parts fills its parts fuller: fewer parts
-            // (boundary rows, streamed rounds) outweigh the longer probe chains there
-            if (np > kHeavyParts) np = (e + heavy_target - 1) / heavy_target;
-            // a part spans at most block_records consecutive sources
-            if (block_records != 0 && e != 0) np = max(np, (wend - (uint32_t)q - 1u + block_records - 1u) / block_records);
-            // dense parts: a window of several parts whose postings fill dense parts of
-            // dense_w consecutive sources to dense_pct% of their slots on average, for a
-            // probe that fits the cursor slots (windows this full would need hash-table
-            // parts of a few postings per probe token)
-            // (dense parts follow the global grid of dense_w records: the window's first
-            // and last parts are clipped to it)
-            const uint32_t win = wend - (uint32_t)q - 1u;
-            const uint32_t ndp = dense_w && win ? (wend - 1u) / dense_w - ((uint32_t)q + 1u) / dense_w + 1u : 0u;
-            const bool dense = !unsorted && ndp > 1 && np > 1 && qp < kStreamQp &&
-                               (uint64_t)e * 100u >= (uint64_t)ndp * dense_w * dense_pct;
+            const uint32_t np = dense ? 0u : parts_of(e);
             nparts[q] = dense ? (ndp | kDenseParts) : np;
             dparts += dense ? ndp : 0u;
             if (dflag) dflag[q] = dense ? 1u : 0u;   // scanned into the list of dense queries
@@ -1846,14 +1865,18 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
 // probe).  Both tables are zero between units: the positional filter clears every word
 // it reads.  A wavefront's LDS: the count tables of the block's wavefronts first (wave w
 // at byte w x DW, so that a slot's count word is an OR of its offset and that base), then
-// the last-match tables, then each wavefront's candidate list (two words an entry), its
-// probe-token table, a step's start marks and the part's length samples.
+// the last-match tables, then each wavefront's own area: its candidate list (two words an
+// entry) -- which during the walk, while the list is empty, holds the probe-token table
+// instead -- a word per lane for updates without work, a step's start marks and the
+// part's length samples.  Sharing the list's words with the token table keeps a block
+// of four wavefronts at 512 table slots under 32 KiB, so that five blocks (as many as the
+// kernel's registers allow) fit a CU's 160 KiB of LDS.
 constexpr uint32_t kDList = 256;                          // candidates listed at once
 constexpr uint32_t kDenseSamples = 33;                    // length samples: each 32nd of the part, and its end
 constexpr int kDRows = 6;                                 // rows of kWave postings per full walk step
 constexpr uint32_t kDMarkWords = kDRows * kWave / 4;      // start marks of a step, a byte per position
-constexpr uint32_t kDOwnWords = 2 * kDList + 2 * kStreamQp + kDMarkWords + kDenseSamples + 1;
-static_assert(kDOwnWords % 2 == 0 && 2 * kDList >= (uint32_t)kWave, "entries stay 8-byte aligned; a word per lane in the list");
+constexpr uint32_t kDOwnWords = 2 * kDList + kWave + kDMarkWords + kDenseSamples + 1;
+static_assert(kDOwnWords % 2 == 0 && 2 * kDList >= 2 * kStreamQp, "entries stay 8-byte aligned; the token table fits the list");
 constexpr uint32_t kDBad = 0x100u;                        // list entry: the source's signature overflowed
 constexpr uint32_t dense_block_words(int cap) { return (uint32_t)join_waves(cap) * (2u * cap / 4u + 2u * cap + kDOwnWords); }
 #ifndef FGSS_DENSE_SIGPER
@@ -1875,12 +1898,12 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
     uint32_t *const dlast = smem + (size_t)NW * (DW / 4u) + (size_t)w * DW;
     uint32_t *const own = smem + (size_t)NW * (DW / 4u + DW) + (size_t)w * kDOwnWords;
     uint2 *const list = (uint2 *)own;
-    uint32_t *const cur4 = own + 2 * kDList;   // per non-empty probe token, in token order: 4 x (first posting - flattened start)
+    uint32_t *const cur4 = own;                // walk only: per non-empty probe token, in token order: 4 x (first posting - flattened start)
     uint32_t *const jw = cur4 + kStreamQp;     // ... and its query position
-    uint32_t *const mw = jw + kStreamQp;       // a step's start marks (words, for clearing)
+    uint32_t *const dump = own + 2 * kDList;   // a word per lane for updates without work
+    uint32_t *const mw = dump + kWave;         // a step's start marks (words, for clearing)
     uint8_t *const smark = (uint8_t *)mw;
     uint32_t *const dsamp = mw + kDMarkWords;
-    uint32_t *const dump = own;                // during the walk (the list is empty): a word per lane for updates without work
     auto cnt_word = [&](uint32_t h) -> uint32_t * {   // the count word of slot h
         return (uint32_t *)((char *)smem + ((h & (DW - 4u)) | cbase));
     };
@@ -2742,19 +2765,22 @@ static int enqueue_stage_plan(fgss_device_sets *d, const RunCfg &cfg)
     const uint32_t dense_w = dense_w_of(d, cfg);
     const uint32_t ng = dense_w ? (uint32_t)dense_grid_parts(d, dense_w) : 0u;
     uint32_t *dflag = P<uint32_t>(d->b_dflag), *doff = P<uint32_t>(d->b_doff), *dl = P<uint32_t>(d->b_dlist);
+    if (dense_w) {   // the dense offset table: k_plan counts the postings of dense candidates in it
+        hipLaunchKernelGGL(k_dense_offs, dim3((unsigned)std::min<int64_t>(grid_for(std::max<int64_t>(d->ntok, 1), 256),
+                                                                           kDenseOffsGrid)),
+                           dim3(256), 0, st, d->skeys, psid, loff, misc + M_NE, d->n_terms, dense_w, ng,
+                           P<uint32_t>(d->b_doffs));
+        HIPCK(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_plan, dim3((unsigned)std::min<int64_t>(grid_for(n * kPlanGroup, 256), 4096)), dim3(256), 0, st,
                        d->tok, d->pos, d->len, n, d->job, d->tj, d->jend, probelen, poff, loff, psid, fence, cfg.target,
                        cfg.heavy_target, P<uint2>(d->b_rng), hi, nparts, eq, bcnt, cfg.stream_parts,
                        dense_w, cfg.dense_pct, (unsigned long long *)(misc + M_NDP64), cfg.block_records,
-                       dense_w ? dflag : (uint32_t *)nullptr);
+                       dense_w ? dflag : (uint32_t *)nullptr, dense_w ? (const uint32_t *)P<uint32_t>(d->b_doffs) : nullptr,
+                       (uint32_t)d->n_terms);
     HIPCK(hipGetLastError());
-    if (dense_w) {   // the dense offset table, query list and work units (nothing when none were planned)
+    if (dense_w) {   // the dense query list and work units (nothing when none were planned)
         const unsigned long long *ndp = (const unsigned long long *)(misc + M_NDP64);
-        hipLaunchKernelGGL(k_dense_offs, dim3((unsigned)std::min<int64_t>(grid_for(std::max<int64_t>(d->ntok, 1), 256),
-                                                                           kDenseOffsGrid)),
-                           dim3(256), 0, st, d->skeys, psid, loff, misc + M_NE, d->n_terms, dense_w, ng, ndp,
-                           P<uint32_t>(d->b_doffs));
-        HIPCK(hipGetLastError());
         HIPCK(exclusive_scan<uint32_t>(dflag, doff, n, misc + M_NDENSE, scr, st));
         hipLaunchKernelGGL(k_dense_list, dim3(grid_for(n, 256)), dim3(256), 0, st, (const uint32_t *)dflag,
                            (const uint32_t *)doff, n, ndp, dl);
