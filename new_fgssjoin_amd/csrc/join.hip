@@ -1651,8 +1651,11 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
         // (the source's bad-signature flag and its 64-bucket line are read together)
         auto phase_c = [&](const uint32_t nsig) -> uint32_t {
             uint32_t nver = 0, nex = 0;   // survivors; candidates that reach the 128-bucket bound
-            const uint4 *qs4 = (const uint4 *)(A.qsig + (size_t)q * kSigWords);
-            const uint4 *qh4 = (const uint4 *)(A.qsig64 + (size_t)q * kSig64Words);
+            // the query's counts are read through the scalar cache (wave-uniform operands of
+            // the byte SADs), not gathered by every lane
+            typedef __attribute__((address_space(4))) const uint32_t cu32q;
+            const cu32q *qsp = (const cu32q *)(A.qsig + (size_t)q * kSigWords);
+            const cu32q *qhp = (const cu32q *)(A.qsig64 + (size_t)q * kSig64Words);
             const bool qbad = A.qsigbad[q] != 0;
             // kSigPer candidates per lane and step: their signature lines are gathered
             // together, so a step waits for memory once
@@ -1683,11 +1686,10 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                     uint32_t sad = 0;
 #pragma unroll
                     for (int k = 0; k < kSig64Words / 4; k++) {
-                        const uint4 a4 = qh4[k];
-                        sad = __builtin_amdgcn_sad_u8(a4.x, b4[u][k].x, sad);
-                        sad = __builtin_amdgcn_sad_u8(a4.y, b4[u][k].y, sad);
-                        sad = __builtin_amdgcn_sad_u8(a4.z, b4[u][k].z, sad);
-                        sad = __builtin_amdgcn_sad_u8(a4.w, b4[u][k].w, sad);
+                        sad = __builtin_amdgcn_sad_u8(b4[u][k].x, qhp[4 * k], sad);
+                        sad = __builtin_amdgcn_sad_u8(b4[u][k].y, qhp[4 * k + 1], sad);
+                        sad = __builtin_amdgcn_sad_u8(b4[u][k].z, qhp[4 * k + 2], sad);
+                        sad = __builtin_amdgcn_sad_u8(b4[u][k].w, qhp[4 * k + 3], sad);
                     }
                     const bool bad = qbad || sbad[u] != 0;
                     // sum over buckets of min(cQ, cS) = (|Q| + |S| - SAD) / 2 >= |Q n S|
@@ -1701,11 +1703,11 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                             uint32_t sad2 = 0;
 #pragma unroll
                             for (int k = 0; k < kSigWords / 4; k++) {
-                                const uint4 a4 = qs4[k], c4 = ss4[k];
-                                sad2 = __builtin_amdgcn_sad_u8(a4.x, c4.x, sad2);
-                                sad2 = __builtin_amdgcn_sad_u8(a4.y, c4.y, sad2);
-                                sad2 = __builtin_amdgcn_sad_u8(a4.z, c4.z, sad2);
-                                sad2 = __builtin_amdgcn_sad_u8(a4.w, c4.w, sad2);
+                                const uint4 c4 = ss4[k];
+                                sad2 = __builtin_amdgcn_sad_u8(c4.x, qsp[4 * k], sad2);
+                                sad2 = __builtin_amdgcn_sad_u8(c4.y, qsp[4 * k + 1], sad2);
+                                sad2 = __builtin_amdgcn_sad_u8(c4.z, qsp[4 * k + 2], sad2);
+                                sad2 = __builtin_amdgcn_sad_u8(c4.w, qsp[4 * k + 3], sad2);
                             }
                             keep[u] = ((int)(lq + ls) - (int)sad2) / 2 >= floorb;
                         }
@@ -1880,7 +1882,7 @@ static_assert(kDOwnWords % 2 == 0 && 2 * kDList >= 2 * kStreamQp, "entries stay 
 constexpr uint32_t kDBad = 0x100u;                        // list entry: the source's signature overflowed
 constexpr uint32_t dense_block_words(int cap) { return (uint32_t)join_waves(cap) * (2u * cap / 4u + 2u * cap + kDOwnWords); }
 #ifndef FGSS_DENSE_SIGPER
-#define FGSS_DENSE_SIGPER 2
+#define FGSS_DENSE_SIGPER 1
 #endif
 
 template <int CAP>
@@ -2239,9 +2241,11 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
             WACC(W_SIG2, nex);
             wave_sync();
             // C2: the survivors with good counts against the 128-bucket counts, one per lane
+            // (the query's counts read through the scalar cache, as in C)
             uint32_t nv2 = 0;
             {
-                const uint4 *qs4 = (const uint4 *)(A.qsig + (size_t)q * kSigWords);
+                typedef __attribute__((address_space(4))) const uint32_t cu32q;
+                const cu32q *qsp = (const cu32q *)(A.qsig + (size_t)q * kSigWords);
                 for (uint32_t base = 0; base < nv; base += kWave) {
                     const bool in = base + (uint32_t)lane < nv;
                     const uint2 ent = list[min(base + (uint32_t)lane, nv - 1u)];
@@ -2254,11 +2258,11 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
                             uint32_t sad2 = 0;
 #pragma unroll
                             for (int k = 0; k < kSigWords / 4; k++) {
-                                const uint4 a4 = qs4[k], c4 = ss4[k];
-                                sad2 = __builtin_amdgcn_sad_u8(a4.x, c4.x, sad2);
-                                sad2 = __builtin_amdgcn_sad_u8(a4.y, c4.y, sad2);
-                                sad2 = __builtin_amdgcn_sad_u8(a4.z, c4.z, sad2);
-                                sad2 = __builtin_amdgcn_sad_u8(a4.w, c4.w, sad2);
+                                const uint4 c4 = ss4[k];
+                                sad2 = __builtin_amdgcn_sad_u8(c4.x, qsp[4 * k], sad2);
+                                sad2 = __builtin_amdgcn_sad_u8(c4.y, qsp[4 * k + 1], sad2);
+                                sad2 = __builtin_amdgcn_sad_u8(c4.z, qsp[4 * k + 2], sad2);
+                                sad2 = __builtin_amdgcn_sad_u8(c4.w, qsp[4 * k + 3], sad2);
                             }
                             keep = ((int)(lq + ls) - (int)sad2) / 2 >= (int)floorf(fgss_min_overlap(lq, ls, t));
                         }
