@@ -496,13 +496,14 @@ __global__ __launch_bounds__(256) void k_entries(const uint32_t *__restrict__ to
 }
 
 // postings in term order, packed: x = record id, y = token position | record length << 16
-// pden (null: none): the 4-byte postings that dense parts read, record << 8 | position
-// (records below 2^24; a posting a dense part reads has its position below 256, the
-// longer ones are clamped and never read)
+// pden (null: none): the 4-byte postings that dense parts of dense_w records read,
+// position << 24 | (record mod dense_w) << (24 - log2 dense_w) | record length (a posting
+// a dense part reads has its position below 256, the longer ones are clamped and never
+// read; dense parts are on only for lengths that fit their field)
 __global__ void k_postings(const uint32_t *__restrict__ sidx, const uint32_t *__restrict__ sid_of,
                            const uint32_t *__restrict__ pl_of, const uint32_t *__restrict__ nE_dev,
                            uint2 *__restrict__ post, uint32_t *__restrict__ psid, uint32_t *__restrict__ fence,
-                           uint32_t *__restrict__ pden)
+                           uint32_t *__restrict__ pden, uint32_t dense_w, uint32_t dshift)
 {
     const uint32_t nE = *nE_dev;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -511,7 +512,7 @@ __global__ void k_postings(const uint32_t *__restrict__ sidx, const uint32_t *__
     const uint32_t s = sid_of[e], pl = pl_of[e];
     post[i] = make_uint2(s, pl);
     psid[i] = s;
-    if (pden) pden[i] = s << 8 | min(pl & 0xFFFFu, 255u);
+    if (pden) pden[i] = min(pl & 0xFFFFu, 255u) << 24 | (s & (dense_w - 1u)) << dshift | pl >> 16;
     if ((i & (kFence - 1)) == 0) fence[i / kFence] = s;
 }
 
@@ -1016,7 +1017,7 @@ struct JoinArgs {
     const uint8_t *qsigbad;
     const uint32_t *loff;
     const uint2 *post;          // postings (record id, token position | length << 16), grouped by term
-    const uint32_t *pden;       // the same postings as record << 8 | position, for dense parts (null: none)
+    const uint32_t *pden;       // the same postings as record | position << 24, for dense parts (null: none)
     const uint32_t *doffs;      // dense parts: per grid part g <= dng and term t, the first posting of t
                                 // at or after record g x the grid width (row g, dterms entries; k_dense_offs)
     uint32_t dng;               // grid parts of the dense offset table (dng + 1 rows)
@@ -1403,7 +1404,7 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
 #pragma unroll
             for (int k = 0; k < PER; k++) {
                 uint32_t jj = (uint32_t)lane * PER + k;
-                sbeg[k] = run;
+                sbeg[k] = slen[k] != 0 ? run : ~0u;   // an empty segment marks nothing
                 if (jj < nb) rlo[jj] -= run;
                 run += slen[k];
             }
@@ -1420,7 +1421,7 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
             // segment marks of a step, one byte per flattened position, after the
             // per-lane words in the query stage
             uint8_t *smark = (uint8_t *)(qstage + kWave);
-            static_assert(kUnroll * kWave <= 4 * (kQStage - kWave), "a step's marks fit the query stage");
+            static_assert(kUnroll * kWave + 1 <= 4 * (kQStage - kWave), "a step's marks and a spare byte fit the query stage");
             uint32_t segc = 0;   // segment of the last posting of the previous step
             // A step takes N postings per lane (rows of kWave): full steps of kUnroll
             // rows, then a tail of 2 or 1 rows, so that a short range (most items hold
@@ -1441,11 +1442,8 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
                 static_assert(kTokBatch <= 256, "segment indices fit a byte");
                 ((uint32_t *)smark)[lane] = 0u;
 #pragma unroll
-                for (int k = 0; k < PER; k++) {
-                    const uint32_t at = sbeg[k] - e0;   // wraps above the range for starts before e0
-                    const bool mk = slen[k] != 0 && at < (uint32_t)(N * kWave);
-                    *(mk ? smark + at : (uint8_t *)(scr + lane)) = (uint8_t)(lane * PER + k);
-                }
+                for (int k = 0; k < PER; k++)   // starts outside the step, and empty segments, mark a spare byte
+                    smark[min(sbeg[k] - e0, (uint32_t)(N * kWave))] = (uint8_t)(lane * PER + k);
                 wave_sync();
                 uint32_t c = segc;
 #pragma unroll
@@ -1864,20 +1862,20 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
 // dcnt holds 8-bit match counts, four slots to a word, and dlast the last match as query
 // position << 24 | source position << 16 (both below 256: the probes of dense queries
 // hold at most 255 tokens, and a source's mid-prefix position lies inside its partner's
-// probe).  Both tables are zero between units: the positional filter clears every word
-// it reads.  A wavefront's LDS: the count tables of the block's wavefronts first (wave w
-// at byte w x DW, so that a slot's count word is an OR of its offset and that base), then
-// the last-match tables, then each wavefront's own area: its candidate list (two words an
-// entry) -- which during the walk, while the list is empty, holds the probe-token table
-// instead -- a word per lane for updates without work, a step's start marks and the
-// part's length samples.  Sharing the list's words with the token table keeps a block
+// probe) | the low 16 bits of the source's postings, which hold its length (the same in
+// every posting of a source, so the maximum keeps them).  Both tables are zero between
+// units: the positional filter clears every word it reads.  A wavefront's LDS: the count
+// tables of the block's wavefronts first (wave w at byte w x DW, so that a slot's count
+// word is an OR of its offset and that base), then the last-match tables, then each
+// wavefront's own area: its candidate list (two words an entry) -- which during the walk,
+// while the list is empty, holds the probe-token table instead -- a word per lane for
+// updates without work and a step's start marks.  Sharing the list's words with the token table keeps a block
 // of four wavefronts at 512 table slots under 32 KiB, so that five blocks (as many as the
 // kernel's registers allow) fit a CU's 160 KiB of LDS.
 constexpr uint32_t kDList = 256;                          // candidates listed at once
-constexpr uint32_t kDenseSamples = 33;                    // length samples: each 32nd of the part, and its end
 constexpr int kDRows = 6;                                 // rows of kWave postings per full walk step
-constexpr uint32_t kDMarkWords = kDRows * kWave / 4;      // start marks of a step, a byte per position
-constexpr uint32_t kDOwnWords = 2 * kDList + kWave + kDMarkWords + kDenseSamples + 1;
+constexpr uint32_t kDMarkWords = kDRows * kWave / 4 + 1;  // start marks of a step, a byte per position, and a spare word
+constexpr uint32_t kDOwnWords = 2 * kDList + kWave + kDMarkWords + 1;
 static_assert(kDOwnWords % 2 == 0 && 2 * kDList >= 2 * kStreamQp, "entries stay 8-byte aligned; the token table fits the list");
 constexpr uint32_t kDBad = 0x100u;                        // list entry: the source's signature overflowed
 constexpr uint32_t dense_block_words(int cap) { return (uint32_t)join_waves(cap) * (2u * cap / 4u + 2u * cap + kDOwnWords); }
@@ -1891,6 +1889,8 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
     constexpr uint32_t DW = 2u * CAP;
     constexpr int LOG2DW = CAP == 512 ? 10 : CAP == 1024 ? 11 : CAP == 2048 ? 12 : 13;
     static_assert((1u << LOG2DW) == DW && DW <= 0xFFFFu, "grid parts of a power of two records, slots in 16 bits");
+    constexpr int LSH = 24 - LOG2DW;   // a dense posting: position << 24 | slot << LSH | length
+    static_assert(LSH >= 11 && LSH <= 16, "the length field holds 2047 tokens or more, inside a last-match word's low 16 bits");
     constexpr int NW = join_waves(CAP);
     constexpr int PER = kStreamQp / kWave;   // probe tokens per lane
     extern __shared__ uint32_t smem[];
@@ -1905,7 +1905,6 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
     uint32_t *const dump = own + 2 * kDList;   // a word per lane for updates without work
     uint32_t *const mw = dump + kWave;         // a step's start marks (words, for clearing)
     uint8_t *const smark = (uint8_t *)mw;
-    uint32_t *const dsamp = mw + kDMarkWords;
     auto cnt_word = [&](uint32_t h) -> uint32_t * {   // the count word of slot h
         return (uint32_t *)((char *)smem + ((h & (DW - 4u)) | cbase));
     };
@@ -2003,11 +2002,6 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
             cur_t = t;
         }
         const uint32_t qp = min(fgss_max_prefix(lq, t), lq);
-        // length samples: lane k < kDenseSamples reads the record at gbase + k DW / 32,
-        // clamped into [wa, wb) (lengths are non-increasing, so a source between two equal
-        // samples has their length); the load is stored after the walk
-        uint32_t lsamp = 0;
-        if (lane < (int)kDenseSamples) lsamp = A.len[min(max(gbase + (uint32_t)lane * (DW / 32u), wa), wb - 1u)];
         PHASE_MARK(0);
 
         // ---- A: the postings of the probe tokens in [wa, wb).  A token's postings in the
@@ -2107,11 +2101,10 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
             for (uint32_t i = 0; i < (uint32_t)N * 16u; i += kWave)
                 if (i + lane < (uint32_t)N * 16u) mw[i + lane] = 0u;
             wave_sync();
+            // a start outside the step (before e0 its offset wraps above) marks the spare
+            // byte after the step's marks
 #pragma unroll
-            for (int k = 0; k < PER; k++) {
-                const uint32_t at = dbeg[k] - e0;   // wraps above the step for starts before e0
-                *(at < (uint32_t)(N * kWave) ? smark + at : (uint8_t *)(dump + lane)) = 1u;
-            }
+            for (int k = 0; k < PER; k++) smark[min(dbeg[k] - e0, (uint32_t)(N * kWave))] = 1u;
             wave_sync();
             uint32_t pw[N], jv[N];
 #pragma unroll
@@ -2131,9 +2124,11 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
 #pragma unroll
             for (int r = 0; r < N; r++) {
                 if (CHECK && (uint32_t)r >= nrows) break;
-                const uint32_t h = __builtin_amdgcn_ubfe(pw[r], 8u, (uint32_t)LOG2DW);
-                const uint32_t add = 1u << (h << 3);   // the shift count's low five bits: (h & 3) x 8
-                const uint32_t last = __builtin_amdgcn_perm(jv[r], pw[r], 0x04000C0Cu);
+                const uint32_t h = __builtin_amdgcn_ubfe(pw[r], (uint32_t)LSH, (uint32_t)LOG2DW);
+                const uint32_t add = 1u << ((h << 3) & 31u);   // the count byte of slot h in its word
+                // j << 24 | pos << 16 | the posting's low 16 bits (the source's length and
+                // slot bits, the same in every posting of the source)
+                const uint32_t last = __builtin_amdgcn_perm(jv[r], pw[r], 0x04030100u);
                 if (CHECK) {
                     const bool v = e0 + (uint32_t)(r * kWave) + (uint32_t)lane < total;
                     atomicAdd(v ? cnt_word(h) : &dump[lane], add);
@@ -2160,8 +2155,6 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
                 pf_hi[k] = in ? row1[pf_tok[k]] : 0u;
             }
         }
-        wave_sync();
-        if (lane < (int)kDenseSamples) dsamp[lane] = lsamp;
         wave_sync();
         PHASE_MARK(1);
 
@@ -2338,10 +2331,7 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
             const bool has = m != 0;
             if (__ballot(has) == 0) continue;   // slots outside [wa, wb) are never touched
             const uint32_t wl = dlast[h];
-            const uint32_t sg = h >> (LOG2DW - 5);
-            const uint32_t sa = dsamp[sg], sb = dsamp[sg + 1u];
-            uint32_t ls = sa;
-            if (sa != sb && has) ls = A.len[gbase + h];
+            const uint32_t ls = wl & ((1u << LSH) - 1u);   // the source's length, from its postings
             dlast[h] = 0u;
             if ((lane & 3) == 0) *cnt_word(h) = 0u;
             const uint32_t js = wl >> 24, ss = (wl >> 16) & 0xFFu;
@@ -2583,10 +2573,11 @@ static int wave_slots(fgss_device_sets *d)
 
 // Every buffer the pipeline touches, sized for upper bounds where one exists (index
 // entries and probe tokens <= tokens) and for the current capacities otherwise.
-// dense parts need record ids below 2^24 (their 4-byte postings) and an offset table of
-// terms x grid parts words (k_dense_offs) within kDenseTableBudget at the finest grid
-// dense units: record ids in 24 bits of a dense posting, and posting byte offsets in 32 bits
+// dense parts need an offset table of terms x grid parts words (k_dense_offs) within
+// kDenseTableBudget at the finest grid, record ids below 2^24 and posting byte offsets in
+// 32 bits; a dense posting holds a record's length in 24 - log2(grid width) bits
 static bool dense_ok(const fgss_device_sets *d) { return d->n < ((int64_t)1 << 24) && d->ntok + kWave < ((int64_t)1 << 30); }
+static uint32_t log2u(uint32_t x) { uint32_t k = 0; while ((2u << k) <= x) k++; return k; }
 constexpr uint32_t kDenseGridMin = 1024;                   // records per grid part at 512 table slots
 constexpr int64_t kDenseTableBudget = (int64_t)2 << 30;   // bytes
 static int64_t dense_grid_parts(const fgss_device_sets *d, uint32_t dw) { return (d->n + dw - 1) / dw; }
@@ -2597,7 +2588,8 @@ static bool dense_table_fits(const fgss_device_sets *d)
 // the run's dense grid (records per grid part; 0: no dense parts)
 static uint32_t dense_w_of(const fgss_device_sets *d, const RunCfg &cfg)
 {
-    return dense_ok(d) && dense_table_fits(d) ? cfg.dense_w : 0u;
+    if (cfg.dense_w == 0 || !dense_ok(d) || !dense_table_fits(d)) return 0u;
+    return (int64_t)d->max_len < ((int64_t)1 << (24 - log2u(cfg.dense_w))) ? cfg.dense_w : 0u;
 }
 
 
@@ -2689,7 +2681,8 @@ static JoinArgs join_args(fgss_device_sets *d)
 // mid_prefix at the record's job threshold; rs: max_prefix at tj[0], the smallest
 // threshold an R-S index will serve) and the probe prefix, the stable sort of the
 // entries by term, packed postings, the id-only copy, the fence and list offsets.
-static int enqueue_index(fgss_device_sets *d, int rs)
+// dense_w: the run's dense grid width (0: no dense postings)
+static int enqueue_index(fgss_device_sets *d, int rs, uint32_t dense_w)
 {
     hipStream_t st = d->st;
     const int64_t n = d->n, E = std::max<int64_t>(d->ntok, 1);
@@ -2716,7 +2709,8 @@ static int enqueue_index(fgss_device_sets *d, int rs)
     const uint32_t *sidx = which ? ib : ia;
     d->skeys = skeys;
     hipLaunchKernelGGL(k_postings, dim3(grid_for(E, 256)), dim3(256), 0, st, sidx, sid_of, P<uint32_t>(d->b_pl_of),
-                       misc + M_NE, post, psid, fence, rs || !dense_ok(d) ? (uint32_t *)nullptr : P<uint32_t>(d->b_pden));
+                       misc + M_NE, post, psid, fence, rs || !dense_w ? (uint32_t *)nullptr : P<uint32_t>(d->b_pden),
+                       dense_w, dense_w ? 24u - log2u(dense_w) : 0u);
     HIPCK(hipGetLastError());
     hipLaunchKernelGGL(k_list_offsets, dim3(grid_for(E + 1, 256)), dim3(256), 0, st, skeys, misc + M_NE, d->n_terms, loff);
     HIPCK(hipGetLastError());
@@ -2749,10 +2743,10 @@ static hipError_t clear_run(fgss_device_sets *d, hipStream_t st)
 // that each can be captured into a HIP graph): the index build (counters cleared
 // first), the planning of work items, and the join kernel; then the device->host
 // copies of all counters into pinned memory.
-static int enqueue_stage_index(fgss_device_sets *d)
+static int enqueue_stage_index(fgss_device_sets *d, const RunCfg &cfg)
 {
     HIPCK(clear_run(d, d->st));
-    return enqueue_index(d, 0);
+    return enqueue_index(d, 0, dense_w_of(d, cfg));
 }
 
 static int enqueue_stage_plan(fgss_device_sets *d, const RunCfg &cfg)
@@ -2856,7 +2850,7 @@ static int enqueue_pipeline(fgss_device_sets *d, const RunCfg &cfg, bool timing)
 {
     hipStream_t st = d->st;
     if (timing) HIPCK(hipEventRecord(d->ev[0], st));
-    int rc = enqueue_stage_index(d);
+    int rc = enqueue_stage_index(d, cfg);
     if (rc) return rc;
     if (timing) HIPCK(hipEventRecord(d->ev[1], st));
     rc = enqueue_stage_plan(d, cfg);
@@ -2939,7 +2933,7 @@ static int capture_graph(fgss_device_sets *d, const RunCfg &cfg)
     }
     for (int k = 0; k < 3; k++) {
         HIPCK(hipStreamBeginCapture(d->st, hipStreamCaptureModeThreadLocal));
-        int rc = k == 0 ? enqueue_stage_index(d) : k == 1 ? enqueue_stage_plan(d, cfg) : enqueue_stage_join(d, cfg);
+        int rc = k == 0 ? enqueue_stage_index(d, cfg) : k == 1 ? enqueue_stage_plan(d, cfg) : enqueue_stage_join(d, cfg);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(d->st, &g);
         if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -3398,7 +3392,7 @@ extern "C" int fgss_collection_print_index(fgss_collection *c, float threshold, 
     uint32_t *misc = P<uint32_t>(d->b_misc);
     HIPCK(fill<uint32_t>(misc, M_COUNT, 0u, d->st));
     if (d->n > 0) {
-        rc = enqueue_index(d, 0);
+        rc = enqueue_index(d, 0, 0u);
         if (rc) return rc;
     }
     HIPCK(hipMemcpyAsync(d->h_misc, misc, M_COUNT * 4, hipMemcpyDeviceToHost, d->st));
@@ -4410,7 +4404,7 @@ extern "C" int fgss_server_create(const fgss_collection *indexed, float t_min, i
         if (!rc && !srv->empty) {
             uint32_t *misc = P<uint32_t>(x->b_misc);
             if (fill<uint32_t>(misc, M_COUNT, 0u, x->st) != hipSuccess) rc = fgss_fail(FGSS_ERR_DEVICE, "fill failed");
-            if (!rc) rc = enqueue_index(x, 1);
+            if (!rc) rc = enqueue_index(x, 1, 0u);
             if (!rc && hipMemcpyAsync(x->h_misc, misc, M_COUNT * 4, hipMemcpyDeviceToHost, x->st) != hipSuccess)
                 rc = fgss_fail(FGSS_ERR_DEVICE, "copy of the index counters failed");
             if (!rc && hipStreamSynchronize(x->st) != hipSuccess) rc = fgss_fail(FGSS_ERR_DEVICE, "index build failed");
