@@ -1059,6 +1059,14 @@ __device__ __forceinline__ uint32_t slot_hash(uint32_t s, int log2cap)
     return (s * 0x9E3779B1u) >> (32 - log2cap);
 }
 
+// the probe step of key s (odd, so that a key's probes visit every slot of a power-of-two
+// table): double hashing keeps the probe chains of an item's keys apart, where linear
+// probing grows clusters whose longest chain sets the rounds of a wavefront's insert
+__device__ __forceinline__ uint32_t slot_step(uint32_t s, int log2cap)
+{
+    return ((s * 0x85EBCA6Bu) >> (32 - log2cap)) | 1u;
+}
+
 // Order LDS traffic between lanes of one wavefront: a compiler barrier plus a
 // wavefront-scope fence (the hardware executes a wave's LDS operations in order).
 __device__ __forceinline__ void wave_sync()
@@ -1075,9 +1083,10 @@ __device__ __forceinline__ int ceil_log2(uint32_t x)
 // Phase A's table update for the kUnroll postings a lane holds (pe[k], query position
 // jpos[k]; bit k of valm set when posting k takes part).  The probes of all kUnroll
 // postings are issued together and wait once; collisions are resolved for all of them
-// in one loop (linear probing stays exact whatever the order in which keys are placed,
-// as no slot is emptied during the phase, and every pending probe advances one slot per
-// round, so one wave-uniform round counter bounds them by tsize slots).  New keys are
+// in one loop (open addressing with double hashing stays exact whatever the order in
+// which keys are placed, as no slot is emptied during the phase, and every pending probe
+// moves to a new slot of its key's cycle each round, so one wave-uniform round counter
+// bounds them by tsize slots).  New keys are
 // appended to clist.  Lanes without work aim their LDS updates at scr[lane].  Returns
 // true on any lane when some posting found no free slot.
 template <int N>
@@ -1086,7 +1095,7 @@ __device__ __forceinline__ bool table_insert(const uint2 (&pe)[N], uint32_t valm
                                              uint32_t *scr, int lane, int tlog, uint32_t tsize, uint32_t &nlist)
 {
     uint32_t newm = 0, pendm = 0, failm = 0;
-    uint32_t h[N], prev[N];
+    uint32_t h[N], prev[N], hs[N];
 #pragma unroll
     for (int k = 0; k < N; k++) h[k] = slot_hash(pe[k].x, tlog);
 #pragma unroll
@@ -1097,12 +1106,16 @@ __device__ __forceinline__ bool table_insert(const uint2 (&pe)[N], uint32_t valm
         newm |= (vk & (uint32_t)(prev[k] == kEmpty)) << k;
         pendm |= (vk & (uint32_t)(prev[k] != kEmpty) & (uint32_t)(prev[k] != pe[k].x)) << k;
     }
+    if (__ballot(pendm != 0)) {
+#pragma unroll
+        for (int k = 0; k < N; k++) hs[k] = slot_step(pe[k].x, tlog);
+    }
     for (uint32_t round = 1; __ballot(pendm != 0); round++) {
         if (round >= tsize) { failm |= pendm; break; }
 #pragma unroll
         for (int k = 0; k < N; k++) {
             const uint32_t pk = (pendm >> k) & 1u;
-            h[k] = (h[k] + pk) & (tsize - 1u);
+            h[k] = (h[k] + (pk ? hs[k] : 0u)) & (tsize - 1u);
             prev[k] = atomicCAS(pk ? &tkey[h[k]] : &scr[lane], kEmpty, pe[k].x);
         }
 #pragma unroll
