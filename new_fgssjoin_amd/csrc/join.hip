@@ -2171,7 +2171,7 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
         wave_sync();
         PHASE_MARK(1);
 
-        uint32_t cand = 0, nsig = 0, nver = 0, inter = 0, vtok = 0, nres = 0, nl = 0;
+        uint32_t nsig = 0, nver = 0, inter = 0, vtok = 0, nres = 0, nl = 0;
         // ---- C, C2 and D over list[0, nl) ----
         auto flush = [&]() {
             wave_sync();
@@ -2331,36 +2331,50 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
             nl = 0;
         };
 
-        // ---- B: the positional filter (filtering.cu:127-133) over the unit's slots, 64
-        // at a time: every word read is cleared; the candidates that can still reach
-        // floor(bound) are listed with their source length, count and last match.  A
-        // full list goes through C, C2 and D before the scan goes on. ----
+        // ---- B: the positional filter (filtering.cu:127-133) over the unit's slots, two
+        // per lane and 128 a step (a lane's two count bytes share a half of a count word,
+        // its two last-match words are adjacent): every word read is cleared; the
+        // candidates that can still reach floor(bound) are listed with their source
+        // length, count and last match, in slot order.  A list that could not take
+        // another step goes through C, C2 and D before the scan goes on. ----
         const uint32_t slo = wa - gbase, shi = wb - gbase;
-        const uint32_t csh = ((uint32_t)lane & 3u) * 8u;
-        for (uint32_t h0 = slo & ~(uint32_t)(kWave - 1); h0 < shi; h0 += kWave) {
-            const uint32_t h = h0 + (uint32_t)lane;
-            const uint32_t cw = *cnt_word(h);
-            const uint32_t m = (cw >> csh) & 0xFFu;
-            const bool has = m != 0;
-            if (__ballot(has) == 0) continue;   // slots outside [wa, wb) are never touched
-            const uint32_t wl = dlast[h];
-            const uint32_t ls = wl & ((1u << LSH) - 1u);   // the source's length, from its postings
-            dlast[h] = 0u;
-            if ((lane & 3) == 0) *cnt_word(h) = 0u;
-            const uint32_t js = wl >> 24, ss = (wl >> 16) & 0xFFu;
-            const float bound = fgss_min_overlap(lq, ls, t);
-            const uint32_t rq = lq - js, rs = ls - ss;
-            const uint32_t x = m + (rq < rs ? rq : rs);
-            const bool pass = has && !((float)x < bound);
-            cand += (uint32_t)__popcll(__ballot(pass));
-            // overlap can still reach floor(bound): m + min(|Q|-js-1, |S|-ss-1)
-            const bool keep = pass && (int)(x - 1u) >= (int)floorf(bound);
-            const uint64_t bal = __ballot(keep);
-            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-            if (keep) list[nl + below] = make_uint2(h | ls << 16, m | (wl & 0xFFFF0000u));
-            nl += (uint32_t)__popcll(bal);
-            if (nl + (uint32_t)kWave > kDList) flush();
+        const uint32_t csh2 = ((uint32_t)lane & 1u) * 16u;   // the lane's count bytes in its word
+        uint32_t candl = 0;   // candidates counted per lane, summed after the scan
+        for (uint32_t h0 = slo & ~(uint32_t)(2 * kWave - 1); h0 < shi; h0 += 2u * kWave) {
+            const uint32_t hp = h0 + 2u * (uint32_t)lane;   // slots hp, hp + 1 (hp + 1 < DW)
+            const uint32_t c2 = (*cnt_word(hp) >> csh2) & 0xFFFFu;
+            if (__ballot(c2 != 0) == 0) continue;   // slots outside [wa, wb) are never touched
+            uint2 *const wl2p = (uint2 *)&dlast[hp];
+            const uint2 wl2 = *wl2p;
+            *wl2p = make_uint2(0u, 0u);
+            *(uint16_t *)((char *)cnt_word(hp) + (csh2 >> 3)) = 0;
+            uint32_t kb = 0;
+            uint2 e[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const uint32_t m = (c2 >> (8 * i)) & 0xFFu;
+                const uint32_t wl = i ? wl2.y : wl2.x;
+                const uint32_t ls = wl & ((1u << LSH) - 1u);   // the source's length, from its postings
+                const uint32_t js = wl >> 24, ss = (wl >> 16) & 0xFFu;
+                const float bound = fgss_min_overlap(lq, ls, t);
+                const uint32_t rq = lq - js, rs = ls - ss;
+                const uint32_t x = m + (rq < rs ? rq : rs);
+                const bool pass = m != 0 && !((float)x < bound);
+                candl += pass ? 1u : 0u;
+                // overlap can still reach floor(bound): m + min(|Q|-js-1, |S|-ss-1)
+                const bool keep = pass && (int)(x - 1u) >= (int)floorf(bound);
+                kb |= (keep ? 1u : 0u) << i;
+                e[i] = make_uint2((hp + (uint32_t)i) | ls << 16, m | (wl & 0xFFFF0000u));
+            }
+            const uint32_t nk = (kb & 1u) + (kb >> 1);
+            const uint32_t incl = wave_inclusive_scan_dpp(nk);
+            uint32_t at = nl + incl - nk;
+            if (kb & 1u) list[at++] = e[0];
+            if (kb & 2u) list[at] = e[1];
+            nl += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+            if (nl + 2u * (uint32_t)kWave > kDList) flush();
         }
+        const uint32_t cand = (uint32_t)__builtin_amdgcn_readlane((int)wave_inclusive_scan_dpp(candl), kWave - 1);
         if (nl) flush();
         wave_sync();
         WACC(W_JOB0 + 0, total);
