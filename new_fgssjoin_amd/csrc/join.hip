@@ -1887,6 +1887,11 @@ __device__ __forceinline__ void join_items(const JoinArgs &A, uint32_t rblock, u
 // kernel's registers allow) fit a CU's 160 KiB of LDS.
 constexpr uint32_t kDList = 256;                          // candidates listed at once
 constexpr int kDRows = 6;                                 // rows of kWave postings per full walk step
+#ifndef FGSS_DENSE_SLOTS
+#define FGSS_DENSE_SLOTS 4
+#endif
+constexpr uint32_t kDSlots = FGSS_DENSE_SLOTS;            // slots per lane in the positional filter (2 or 4)
+static_assert(kDSlots == 2 || kDSlots == 4, "a lane's count bytes lie in one count word");
 constexpr uint32_t kDMarkWords = kDRows * kWave / 4 + 1;  // start marks of a step, a byte per position, and a spare word
 constexpr uint32_t kDOwnWords = 2 * kDList + kWave + kDMarkWords + 1;
 static_assert(kDOwnWords % 2 == 0 && 2 * kDList >= 2 * kStreamQp, "entries stay 8-byte aligned; the token table fits the list");
@@ -2331,29 +2336,39 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
             nl = 0;
         };
 
-        // ---- B: the positional filter (filtering.cu:127-133) over the unit's slots, two
-        // per lane and 128 a step (a lane's two count bytes share a half of a count word,
-        // its two last-match words are adjacent): every word read is cleared; the
-        // candidates that can still reach floor(bound) are listed with their source
-        // length, count and last match, in slot order.  A list that could not take
-        // another step goes through C, C2 and D before the scan goes on. ----
+        // ---- B: the positional filter (filtering.cu:127-133) over the unit's slots,
+        // kDSlots per lane (a lane's count bytes share a count word, its last-match words
+        // are adjacent): every word read is cleared; the candidates that can still reach
+        // floor(bound) are listed with their source length, count and last match, in slot
+        // order.  A step's entries are counted before they are written: a list that cannot
+        // take them goes through C, C2 and D first. ----
         const uint32_t slo = wa - gbase, shi = wb - gbase;
-        const uint32_t csh2 = ((uint32_t)lane & 1u) * 16u;   // the lane's count bytes in its word
+        constexpr uint32_t SL = kDSlots, STEP = SL * kWave;
+        const uint32_t cshs = ((uint32_t)lane & (4u / SL - 1u)) * 8u * SL;   // the lane's count bytes in its word
         uint32_t candl = 0;   // candidates counted per lane, summed after the scan
-        for (uint32_t h0 = slo & ~(uint32_t)(2 * kWave - 1); h0 < shi; h0 += 2u * kWave) {
-            const uint32_t hp = h0 + 2u * (uint32_t)lane;   // slots hp, hp + 1 (hp + 1 < DW)
-            const uint32_t c2 = (*cnt_word(hp) >> csh2) & 0xFFFFu;
-            if (__ballot(c2 != 0) == 0) continue;   // slots outside [wa, wb) are never touched
-            uint2 *const wl2p = (uint2 *)&dlast[hp];
-            const uint2 wl2 = *wl2p;
-            *wl2p = make_uint2(0u, 0u);
-            *(uint16_t *)((char *)cnt_word(hp) + (csh2 >> 3)) = 0;
+        for (uint32_t h0 = slo & ~(STEP - 1u); h0 < shi; h0 += STEP) {
+            const uint32_t hp = h0 + SL * (uint32_t)lane;   // slots hp .. hp + SL - 1 (< DW)
+            const uint32_t cs = SL == 4 ? *cnt_word(hp) : (*cnt_word(hp) >> cshs) & ((1u << (8 * SL)) - 1u);
+            if (__ballot(cs != 0) == 0) continue;   // slots outside [wa, wb) are never touched
+            uint32_t wlv[SL];
+            if constexpr (SL == 4) {
+                uint4 *const p4 = (uint4 *)&dlast[hp];
+                const uint4 v = *p4;
+                *p4 = make_uint4(0u, 0u, 0u, 0u);
+                wlv[0] = v.x; wlv[1] = v.y; wlv[2] = v.z; wlv[3] = v.w;
+                *cnt_word(hp) = 0u;
+            } else {
+                uint2 *const p2 = (uint2 *)&dlast[hp];
+                const uint2 v = *p2;
+                *p2 = make_uint2(0u, 0u);
+                wlv[0] = v.x; wlv[1] = v.y;
+                *(uint16_t *)((char *)cnt_word(hp) + (cshs >> 3)) = 0;
+            }
             uint32_t kb = 0;
-            uint2 e[2];
 #pragma unroll
-            for (int i = 0; i < 2; i++) {
-                const uint32_t m = (c2 >> (8 * i)) & 0xFFu;
-                const uint32_t wl = i ? wl2.y : wl2.x;
+            for (int i = 0; i < (int)SL; i++) {
+                const uint32_t m = (cs >> (8 * i)) & 0xFFu;
+                const uint32_t wl = wlv[i];
                 const uint32_t ls = wl & ((1u << LSH) - 1u);   // the source's length, from its postings
                 const uint32_t js = wl >> 24, ss = (wl >> 16) & 0xFFu;
                 const float bound = fgss_min_overlap(lq, ls, t);
@@ -2364,15 +2379,21 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
                 // overlap can still reach floor(bound): m + min(|Q|-js-1, |S|-ss-1)
                 const bool keep = pass && (int)(x - 1u) >= (int)floorf(bound);
                 kb |= (keep ? 1u : 0u) << i;
-                e[i] = make_uint2((hp + (uint32_t)i) | ls << 16, m | (wl & 0xFFFF0000u));
             }
-            const uint32_t nk = (kb & 1u) + (kb >> 1);
+            const uint32_t nk = (uint32_t)__popc(kb);
             const uint32_t incl = wave_inclusive_scan_dpp(nk);
+            const uint32_t add = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+            if (nl + add > kDList) flush();   // wave-uniform; the step's entries are still in registers
             uint32_t at = nl + incl - nk;
-            if (kb & 1u) list[at++] = e[0];
-            if (kb & 2u) list[at] = e[1];
-            nl += (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
-            if (nl + 2u * (uint32_t)kWave > kDList) flush();
+#pragma unroll
+            for (int i = 0; i < (int)SL; i++) {
+                if ((kb >> i) & 1u) {
+                    const uint32_t wl = wlv[i];
+                    list[at++] = make_uint2((hp + (uint32_t)i) | (wl & ((1u << LSH) - 1u)) << 16,
+                                            ((cs >> (8 * i)) & 0xFFu) | (wl & 0xFFFF0000u));
+                }
+            }
+            nl += add;
         }
         const uint32_t cand = (uint32_t)__builtin_amdgcn_readlane((int)wave_inclusive_scan_dpp(candl), kWave - 1);
         if (nl) flush();
