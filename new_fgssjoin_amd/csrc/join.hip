@@ -1897,9 +1897,11 @@ constexpr uint32_t kDOwnWords = 2 * kDList + kWave + kDMarkWords + 1;
 static_assert(kDOwnWords % 2 == 0 && 2 * kDList >= 2 * kStreamQp, "entries stay 8-byte aligned; the token table fits the list");
 constexpr uint32_t kDBad = 0x100u;                        // list entry: the source's signature overflowed
 constexpr uint32_t dense_block_words(int cap) { return (uint32_t)join_waves(cap) * (2u * cap / 4u + 2u * cap + kDOwnWords); }
-#ifndef FGSS_DENSE_SIGPER
-#define FGSS_DENSE_SIGPER 1
+#ifndef FGSS_DENSE_LANES
+#define FGSS_DENSE_LANES 2
 #endif
+constexpr int kDLanes = FGSS_DENSE_LANES;   // lanes per candidate in the 64-bucket signature test (1, 2 or 4)
+static_assert(kDLanes == 1 || kDLanes == 2 || kDLanes == 4, "a candidate's lanes share a quad");
 
 template <int CAP>
 __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, uint32_t rblocks)
@@ -2182,72 +2184,66 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
             wave_sync();
             PHASE_MARK(2);
             nsig += nl;
-            // C: 64 buckets, FGSS_DENSE_SIGPER candidates per lane and step; the query's
-            // counts are read once into scalar registers.  Survivors (and sources whose
-            // counts overflowed, marked kDBad) are compacted to the front of the list.
-#ifndef FGSS_DENSE_QSCALAR
-#define FGSS_DENSE_QSCALAR 1
-#endif
-#if FGSS_DENSE_QSCALAR
+            // C: 64 buckets, kDLanes lanes per candidate: each reads its part of the
+            // candidate's 64-byte line (adjacent lanes, adjacent bytes: one instruction
+            // touches kWave / kDLanes lines, not kWave), sums its part of the byte SADs
+            // against the query's counts, and the parts are added across the lanes (DPP).
+            // The query's counts are read once through the scalar cache.  Survivors (and
+            // sources whose counts overflowed, marked kDBad) are compacted to the front of
+            // the list by the first lane of each candidate.
+            constexpr int LPC = kDLanes, WPL = kSig64Words / LPC, CPR = kWave / LPC;
             typedef __attribute__((address_space(4))) const uint32_t cu32;
             const cu32 *qhp = (const cu32 *)(A.qsig64 + (size_t)q * kSig64Words);
-#else
-            const uint32_t *qhp = A.qsig64 + (size_t)q * kSig64Words;
-#endif
             uint32_t qh[kSig64Words];
 #pragma unroll
             for (int k = 0; k < kSig64Words; k++) qh[k] = qhp[k];
+            const uint32_t sub = (uint32_t)lane % LPC;
+            uint32_t qv[WPL];   // the query's words of this lane's part
+#pragma unroll
+            for (int k = 0; k < WPL; k++) {
+                uint32_t v = qh[k];
+#pragma unroll
+                for (int j = 1; j < LPC; j++) v = sub == (uint32_t)j ? qh[j * WPL + k] : v;
+                qv[k] = v;
+            }
             const bool qbad = A.qsigbad[q] != 0;
-            constexpr int SP = FGSS_DENSE_SIGPER;
+            const bool lead = sub == 0;
             uint32_t nv = 0, nex = 0;
-            for (uint32_t base = 0; base < nl; base += kWave * SP) {
-                uint2 ent[SP];
-                uint32_t s[SP];
-                uint8_t sbad[SP];
-                uint4 b4[SP][kSig64Words / 4];
+            for (uint32_t base = 0; base < nl; base += CPR) {
+                const uint32_t c = base + (uint32_t)lane / LPC;
+                const bool in = c < nl;
+                uint2 ent = list[min(c, nl - 1u)];
+                const uint32_t s = gbase + (ent.x & 0xFFFFu);
+                const uint8_t sbad = A.sigbad[s];
+                const uint4 *sh4 = (const uint4 *)(A.sig64 + (size_t)s * kSig64Words) + sub * (WPL / 4);
+                uint4 b4[WPL / 4];
 #pragma unroll
-                for (int v = 0; v < SP; v++) {
-                    ent[v] = list[min(base + (uint32_t)(v * kWave) + (uint32_t)lane, nl - 1u)];
-                    s[v] = gbase + (ent[v].x & 0xFFFFu);
+                for (int k = 0; k < WPL / 4; k++) b4[k] = sh4[k];
+                const uint32_t ls = ent.x >> 16;
+                const int floorb = (int)floorf(fgss_min_overlap(lq, ls, t));
+                uint32_t sad = 0;
+#pragma unroll
+                for (int k = 0; k < WPL / 4; k++) {
+                    sad = __builtin_amdgcn_sad_u8(b4[k].x, qv[4 * k], sad);
+                    sad = __builtin_amdgcn_sad_u8(b4[k].y, qv[4 * k + 1], sad);
+                    sad = __builtin_amdgcn_sad_u8(b4[k].z, qv[4 * k + 2], sad);
+                    sad = __builtin_amdgcn_sad_u8(b4[k].w, qv[4 * k + 3], sad);
                 }
-#pragma unroll
-                for (int v = 0; v < SP; v++) {
-                    sbad[v] = A.sigbad[s[v]];
-                    const uint4 *sh4 = (const uint4 *)(A.sig64 + (size_t)s[v] * kSig64Words);
-#pragma unroll
-                    for (int k = 0; k < kSig64Words / 4; k++) b4[v][k] = sh4[k];
-                }
-                bool surv[SP];
-#pragma unroll
-                for (int v = 0; v < SP; v++) {
-                    const bool in = base + (uint32_t)(v * kWave) + (uint32_t)lane < nl;
-                    const uint32_t ls = ent[v].x >> 16;
-                    const int floorb = (int)floorf(fgss_min_overlap(lq, ls, t));
-                    uint32_t sad = 0;
-#pragma unroll
-                    for (int k = 0; k < kSig64Words / 4; k++) {
-                        sad = __builtin_amdgcn_sad_u8(b4[v][k].x, qh[4 * k], sad);
-                        sad = __builtin_amdgcn_sad_u8(b4[v][k].y, qh[4 * k + 1], sad);
-                        sad = __builtin_amdgcn_sad_u8(b4[v][k].z, qh[4 * k + 2], sad);
-                        sad = __builtin_amdgcn_sad_u8(b4[v][k].w, qh[4 * k + 3], sad);
-                    }
-                    const bool bad = qbad || sbad[v] != 0;
-                    // sum over buckets of min(cQ, cS) = (|Q| + |S| - SAD) / 2 >= |Q n S|
-                    const bool ok = ((int)(lq + ls) - (int)sad) / 2 >= floorb;
-                    nex += (uint32_t)__popcll(__ballot(in && !bad && ok));
-                    surv[v] = in && (bad || ok);
-                    ent[v].y |= bad ? kDBad : 0u;
-                }
+                if (LPC >= 2) sad += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)sad, 0xB1, 0xf, 0xf, false);   // quad_perm 1,0,3,2
+                if (LPC >= 4) sad += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)sad, 0x4E, 0xf, 0xf, false);   // quad_perm 2,3,0,1
+                const bool bad = qbad || sbad != 0;
+                // sum over buckets of min(cQ, cS) = (|Q| + |S| - SAD) / 2 >= |Q n S|
+                const bool ok = ((int)(lq + ls) - (int)sad) / 2 >= floorb;
+                nex += (uint32_t)__popcll(__ballot(lead && in && !bad && ok));
+                const bool surv = lead && in && (bad || ok);
+                ent.y |= bad ? kDBad : 0u;
                 // every entry of this step is in registers: compact in place (the writes
                 // land at or below the entries just read)
                 wave_sync();
-#pragma unroll
-                for (int v = 0; v < SP; v++) {
-                    const uint64_t bal = __ballot(surv[v]);
-                    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                    if (surv[v]) list[nv + below] = ent[v];
-                    nv += (uint32_t)__popcll(bal);
-                }
+                const uint64_t bal = __ballot(surv);
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                if (surv) list[nv + below] = ent;
+                nv += (uint32_t)__popcll(bal);
             }
             WACC(W_SIG2, nex);
             wave_sync();
