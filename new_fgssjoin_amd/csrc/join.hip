@@ -2247,33 +2247,46 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
             }
             WACC(W_SIG2, nex);
             wave_sync();
-            // C2: the survivors with good counts against the 128-bucket counts, one per lane
-            // (the query's counts read through the scalar cache, as in C)
+            // C2: the survivors with good counts against the 128-bucket counts, two lanes per
+            // candidate as in C: each lane reads one 64-byte half of the candidate's line and
+            // takes the SADs of its words against both halves of the query's counts (scalar
+            // registers), keeping the one of its half -- twice the SADs of one lane per
+            // candidate, half the lines per load instruction
             uint32_t nv2 = 0;
             {
                 typedef __attribute__((address_space(4))) const uint32_t cu32q;
                 const cu32q *qsp = (const cu32q *)(A.qsig + (size_t)q * kSigWords);
-                for (uint32_t base = 0; base < nv; base += kWave) {
-                    const bool in = base + (uint32_t)lane < nv;
-                    const uint2 ent = list[min(base + (uint32_t)lane, nv - 1u)];
+                constexpr int HW = kSigWords / 2;   // words per half line
+                const uint32_t sub2 = (uint32_t)lane & 1u;
+                for (uint32_t base = 0; base < nv; base += kWave / 2) {
+                    const uint32_t c = base + (uint32_t)lane / 2u;
+                    const bool in = c < nv;
+                    const uint2 ent = list[min(c, nv - 1u)];
                     const bool bad = (ent.y & kDBad) != 0;
                     bool keep = in;
                     if (__ballot(in && !bad) != 0) {
+                        const uint32_t s = gbase + (ent.x & 0xFFFFu), ls = ent.x >> 16;
+                        const uint4 *ss4 = (const uint4 *)(A.sig + (size_t)s * kSigWords) + sub2 * (HW / 4);
+                        uint32_t sa = 0, sb = 0;
                         if (in && !bad) {
-                            const uint32_t s = gbase + (ent.x & 0xFFFFu), ls = ent.x >> 16;
-                            const uint4 *ss4 = (const uint4 *)(A.sig + (size_t)s * kSigWords);
-                            uint32_t sad2 = 0;
 #pragma unroll
-                            for (int k = 0; k < kSigWords / 4; k++) {
+                            for (int k = 0; k < HW / 4; k++) {
                                 const uint4 c4 = ss4[k];
-                                sad2 = __builtin_amdgcn_sad_u8(c4.x, qsp[4 * k], sad2);
-                                sad2 = __builtin_amdgcn_sad_u8(c4.y, qsp[4 * k + 1], sad2);
-                                sad2 = __builtin_amdgcn_sad_u8(c4.z, qsp[4 * k + 2], sad2);
-                                sad2 = __builtin_amdgcn_sad_u8(c4.w, qsp[4 * k + 3], sad2);
+                                sa = __builtin_amdgcn_sad_u8(c4.x, qsp[4 * k], sa);
+                                sa = __builtin_amdgcn_sad_u8(c4.y, qsp[4 * k + 1], sa);
+                                sa = __builtin_amdgcn_sad_u8(c4.z, qsp[4 * k + 2], sa);
+                                sa = __builtin_amdgcn_sad_u8(c4.w, qsp[4 * k + 3], sa);
+                                sb = __builtin_amdgcn_sad_u8(c4.x, qsp[HW + 4 * k], sb);
+                                sb = __builtin_amdgcn_sad_u8(c4.y, qsp[HW + 4 * k + 1], sb);
+                                sb = __builtin_amdgcn_sad_u8(c4.z, qsp[HW + 4 * k + 2], sb);
+                                sb = __builtin_amdgcn_sad_u8(c4.w, qsp[HW + 4 * k + 3], sb);
                             }
-                            keep = ((int)(lq + ls) - (int)sad2) / 2 >= (int)floorf(fgss_min_overlap(lq, ls, t));
                         }
+                        uint32_t sad2 = sub2 ? sb : sa;
+                        sad2 += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)sad2, 0xB1, 0xf, 0xf, false);   // quad_perm 1,0,3,2
+                        if (in && !bad) keep = ((int)(lq + ls) - (int)sad2) / 2 >= (int)floorf(fgss_min_overlap(lq, ls, t));
                     }
+                    keep = keep && sub2 == 0u;
                     wave_sync();
                     const uint64_t bal = __ballot(keep);
                     const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
