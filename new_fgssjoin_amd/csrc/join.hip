@@ -292,6 +292,9 @@ void fgss_device_release(fgss_collection *c)
 // which v_sad_u8 evaluates four buckets per instruction.  Built once per upload.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t sig_bucket(uint32_t tok) { return (tok * 0x9E3779B1u) >> (32 - 7); }
+// zero bytes past a collection's signature-overflow flags: a whole grid part of dense units
+// (at most 2 x 4096 records) can be read past the last record
+constexpr size_t kSigbadPad = 8192;
 
 __global__ __launch_bounds__(256) void k_signature(const uint32_t *__restrict__ tok, const uint32_t *__restrict__ pos,
                                                    const uint32_t *__restrict__ len, int64_t n,
@@ -381,7 +384,7 @@ static int upload_records(fgss_device_sets **out, const fgss_collection *c)
     const size_t jb = (size_t)d->njobs * 4;
     if (hipMalloc(&d->tok, tb) != hipSuccess || hipMalloc(&d->len, nb) != hipSuccess ||
         hipMalloc(&d->pos, nb) != hipSuccess || hipMalloc(&d->sig, nb * kSigWords) != hipSuccess ||
-        hipMalloc(&d->sig64, nb * kSig64Words) != hipSuccess || hipMalloc(&d->sigbad, nb) != hipSuccess ||
+        hipMalloc(&d->sig64, nb * kSig64Words) != hipSuccess || hipMalloc(&d->sigbad, nb + kSigbadPad) != hipSuccess ||
         hipMalloc(&d->job, nb) != hipSuccess || hipMalloc(&d->jend, jb) != hipSuccess ||
         hipMalloc(&d->tj, jb) != hipSuccess) {
         return fgss_fail(FGSS_ERR_NOMEM, "device allocation for the collection failed");
@@ -407,6 +410,8 @@ static int upload_records(fgss_device_sets **out, const fgss_collection *c)
     HIPCK(hipMemcpyAsync(d->tok, c->tok.data(), (size_t)d->ntok * 4, hipMemcpyHostToDevice, d->st));
     HIPCK(hipMemcpyAsync(d->len, c->len.data(), (size_t)d->n * 4, hipMemcpyHostToDevice, d->st));
     HIPCK(hipMemcpyAsync(d->pos, pos32.data(), (size_t)d->n * 4, hipMemcpyHostToDevice, d->st));
+    // the overflow bytes past the last record read as zeros (dense units read whole grid parts)
+    HIPCK(hipMemsetAsync(d->sigbad + d->n, 0, kSigbadPad, d->st));
     if (d->n > 0) {
         const int64_t blocks = std::min<int64_t>((d->n + 3) / 4, 65536);
         hipLaunchKernelGGL(k_signature, dim3((unsigned)blocks), dim3(256), 0, d->st, d->tok, d->pos, d->len, d->n,
@@ -1909,6 +1914,7 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
     constexpr uint32_t DW = 2u * CAP;
     constexpr int LOG2DW = CAP == 512 ? 10 : CAP == 1024 ? 11 : CAP == 2048 ? 12 : 13;
     static_assert((1u << LOG2DW) == DW && DW <= 0xFFFFu, "grid parts of a power of two records, slots in 16 bits");
+    static_assert(DW <= kSigbadPad && DW % (16u * kWave) == 0, "a grid part's overflow bytes are read 16 per lane");
     constexpr int LSH = 24 - LOG2DW;   // a dense posting: position << 24 | slot << LSH | length
     static_assert(LSH >= 11 && LSH <= 16, "the length field holds 2047 tokens or more, inside a last-match word's low 16 bits");
     constexpr int NW = join_waves(CAP);
@@ -1955,6 +1961,7 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
     uint32_t ugp = 0;
     uint64_t utlo = 1, uthi = 0, utile = ~0ull;
     uint32_t tq = kEmpty, thi = 0, tlen = 0, tpos = 0, tjob = 0;
+    bool gclean = false;   // no source of the tile's grid part has an overflowed signature
     const uint32_t dng = (uint32_t)((A.n + DW - 1) / DW);
     // the next unit's probe tokens and offset-table entries, loaded ahead (pf_it: the unit)
     uint64_t pf_it = ~0ull;
@@ -1997,6 +2004,16 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
                 tpos = A.pos[tq];
                 tjob = A.job[tq];
             }
+            // the grid part's signature-overflow bytes, 16 per lane (the byte array is
+            // padded to whole grid parts with zeros): when none is set, the signature test
+            // reads no flag per candidate
+            uint32_t anyb = 0;
+#pragma unroll
+            for (uint32_t i = 0; i < DW / (16u * kWave); i++) {
+                const uint4 b = ((const uint4 *)(A.sigbad + (size_t)ugp * DW))[i * kWave + (uint32_t)lane];
+                anyb |= b.x | b.y | b.z | b.w;
+            }
+            gclean = __ballot(anyb != 0) == 0;
         }
         const uint32_t u = (uint32_t)(it % kDenseTile);
         const uint32_t q = __builtin_amdgcn_readlane(tq, u);
@@ -2214,7 +2231,7 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
                 const bool in = c < nl;
                 uint2 ent = list[min(c, nl - 1u)];
                 const uint32_t s = gbase + (ent.x & 0xFFFFu);
-                const uint8_t sbad = A.sigbad[s];
+                const uint8_t sbad = gclean ? (uint8_t)0 : A.sigbad[s];
                 const uint4 *sh4 = (const uint4 *)(A.sig64 + (size_t)s * kSig64Words) + sub * (WPL / 4);
                 uint4 b4[WPL / 4];
 #pragma unroll

@@ -327,3 +327,59 @@ def test_dense_parts(fg, oracle_mod, dense_pct):
                 assert 0 <= p.stats["dense_postings"] <= p.stats["postings_scanned"]
                 assert (p.stats["dense_postings"] > 0) == (p.stats["dense_parts"] > 0)
                 assert_join_equal(oracle_mod, sets, p, t, counters=want)
+
+
+def _overflow_collection(n_bad=1200, n_clean=1300, seed=23):
+    """Records of 300 tokens: the first n_bad hold 260 tokens of one 64-bucket (a count
+    above 255: an overflowed signature, which the signature bounds must not use), the
+    others spread over the buckets; a tenth of each group are near-copies of another
+    record of the group.  Tokens ascending in each record, all records the same length."""
+    rng = np.random.default_rng(seed)
+    ids = np.arange(60000, dtype=np.int64)
+    b64 = ((ids * 0x9E3779B1) & 0xFFFFFFFF) >> 26
+    pool_a = ids[b64 == 0][:280]
+    other = ids[b64 != 0]
+    pool_b, pool_c = other[:300], other[300:700]
+    recs = []
+    for g, n in ((0, n_bad), (1, n_clean)):
+        first = len(recs)
+        for i in range(n):
+            if i > 10 and rng.random() < 0.1:
+                r = recs[first + int(rng.integers(0, i))].copy()
+                pool = pool_a if g == 0 else pool_c
+                free = np.setdiff1d(pool, r)
+                k = int(rng.integers(0, r.size))
+                r[k] = free[int(rng.integers(0, free.size))]
+                r = np.sort(r)
+            elif g == 0:
+                r = np.sort(np.concatenate([rng.choice(pool_a, 260, replace=False), rng.choice(pool_b, 40, replace=False)]))
+            else:
+                r = np.sort(rng.choice(pool_c, 300, replace=False))
+            recs.append(r.astype(np.int32))
+    return recs
+
+
+@pytest.mark.parametrize("dense_pct", [1, 0])
+def test_dense_signature_overflow(fg, oracle_mod, dense_pct):
+    """Sources whose 64-bucket counts overflow a byte (their signature bounds are skipped)
+    in grid parts beside parts without any: pairs and counters equal the oracle's, and the
+    filter counters their host restatement."""
+    from _util import records_to_bin
+    coll = fg.Collection.from_bin_ints(records_to_bin(_overflow_collection()))
+    sets = oracle_sets(oracle_mod, coll)
+    _, lens, pos, tok = coll.arrays()
+    b64 = ((tok.astype(np.int64) * 0x9E3779B1) & 0xFFFFFFFF) >> 26
+    rec = np.repeat(np.arange(len(lens)), lens)
+    per = np.zeros((len(lens), 64), np.int64)
+    np.add.at(per, (rec, b64), 1)
+    nbad = int((per > 255).any(axis=1).sum())
+    assert 0 < nbad < len(lens)
+    t = float(np.float32(0.8))
+    _say(f"signature overflow dense_pct={dense_pct}: {nbad} of {len(lens)} records overflow")
+    p = coll.selfjoin(t, dense_pct=dense_pct)
+    if dense_pct == 1:
+        assert p.stats["dense_parts"] > 0
+    assert len(p) > 0
+    assert_join_equal(oracle_mod, sets, p, t)
+    cand, keep, pass64 = filter_counts(coll, t)
+    assert (p.stats["candidates"], p.stats["signature_checks"], p.stats["signature_checks_128"]) == (cand, keep, pass64)
