@@ -537,30 +537,38 @@ __global__ void k_list_offsets(const uint32_t *__restrict__ skeys, const uint32_
 // [g * dense_w, (g + 1) * dense_w).  The offset table gives, for every term t and grid part
 // g <= ng, the first posting of t's list whose record lies at or past g * dense_w:
 // offs[g * n_terms + t] (the list's end when there is none), so that the postings of t in
-// grid part g are offs[g][t] .. offs[g + 1][t] with no search.  Rows by grid part: the dense
-// units run part by part, so the rows in use stay in L2.  A thread per posting writes the
-// entries of the grid parts that end at it (those past the previous posting's part, up to
-// its own), the last posting of a list also those after it; a thread per term fills the
-// entries of an empty list.  It runs before the planning (k_plan counts the postings of
-// the queries that may be dense in it).
+// grid part g are offs[g][t] .. offs[g + 1][t] with no search.  The table holds the two
+// together: entry (g, t) = (offs[g][t], offs[g + 1][t]) for g < ng, one 8-byte read per
+// probe token.  Rows by grid part: the dense units run part by part, so the rows in use
+// stay in L2.  A thread per posting writes offs[.][t] of the grid parts that end at it
+// (those past the previous posting's part, up to its own) -- the first half of their
+// entries and the second half of the entries before them -- the last posting of a list
+// also those after it; a thread per term fills the entries of an empty list.  It runs
+// before the planning (k_plan counts the postings of the queries that may be dense in it).
+__device__ __forceinline__ void dense_off_set(uint32_t *__restrict__ offs2, uint32_t n_terms, uint32_t ng, uint32_t gg,
+                                              uint32_t t, uint32_t v)
+{
+    if (gg < ng) offs2[2 * ((size_t)gg * n_terms + t)] = v;
+    if (gg > 0) offs2[2 * ((size_t)(gg - 1) * n_terms + t) + 1] = v;
+}
 __global__ __launch_bounds__(256) void k_dense_offs(const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ psid,
                                                     const uint32_t *__restrict__ loff, const uint32_t *__restrict__ nE_dev,
                                                     uint32_t n_terms, uint32_t dense_w, uint32_t ng,
-                                                    uint32_t *__restrict__ offs)
+                                                    uint32_t *__restrict__ offs2)
 {
     const uint32_t nE = *nE_dev;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nE; i += gridDim.x * blockDim.x) {
         const uint32_t t = skeys[i], g = psid[i] / dense_w;
         const bool first = i == 0 || skeys[i - 1] != t;
         const uint32_t g0 = first ? 0u : psid[i - 1] / dense_w + 1u;   // grid parts whose first posting is i
-        for (uint32_t gg = g0; gg <= g; gg++) offs[(size_t)gg * n_terms + t] = i;
+        for (uint32_t gg = g0; gg <= g; gg++) dense_off_set(offs2, n_terms, ng, gg, t, i);
         if (i + 1 == nE || skeys[i + 1] != t)
-            for (uint32_t gg = g + 1; gg <= ng; gg++) offs[(size_t)gg * n_terms + t] = i + 1;
+            for (uint32_t gg = g + 1; gg <= ng; gg++) dense_off_set(offs2, n_terms, ng, gg, t, i + 1);
     }
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n_terms; t += gridDim.x * blockDim.x) {
         const uint32_t l0 = loff[t];
         if (l0 != loff[t + 1]) continue;
-        for (uint32_t gg = 0; gg <= ng; gg++) offs[(size_t)gg * n_terms + t] = l0;
+        for (uint32_t gg = 0; gg < ng; gg++) ((uint2 *)offs2)[(size_t)gg * n_terms + t] = make_uint2(l0, l0);
     }
 }
 constexpr unsigned kDenseOffsGrid = 8192;   // k_dense_offs blocks (grid-stride)
@@ -750,7 +758,7 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
                                               uint64_t *__restrict__ bcnt, uint32_t stream_parts, uint32_t dense_w,
                                               uint32_t dense_pct, unsigned long long *__restrict__ ndparts,
                                               uint32_t block_records, uint32_t *__restrict__ dflag,
-                                              const uint32_t *__restrict__ doffs, uint32_t dterms)
+                                              const uint2 *__restrict__ doffs, uint32_t dterms)
 {
     uint32_t dparts = 0;   // dense parts of this lane's queries
     const int64_t ngroups = (int64_t)gridDim.x * (blockDim.x / kPlanGroup);
@@ -803,10 +811,10 @@ __global__ __launch_bounds__(256) void k_plan(const uint32_t *__restrict__ tok, 
         bool dense = false;
         uint32_t e = 0;
         if (!unsorted && ndp > 1 && qp < kStreamQp) {   // uniform over the group
-            const uint32_t *r0 = doffs + (size_t)g0 * dterms, *r1 = doffs + (size_t)(g1 + 1u) * dterms;
+            const uint2 *r0 = doffs + (size_t)g0 * dterms, *r1 = doffs + (size_t)g1 * dterms;
             for (uint32_t j = gl; j < qp; j += kPlanGroup) {
                 const uint32_t term = tok[pos[q] + j];
-                e += r1[term] - r0[term];
+                e += r1[term].y - r0[term].x;
             }
 #pragma unroll
             for (int d = kPlanGroup / 2; d > 0; d >>= 1) e += __shfl_xor(e, d, kPlanGroup);
@@ -1023,9 +1031,9 @@ struct JoinArgs {
     const uint32_t *loff;
     const uint2 *post;          // postings (record id, token position | length << 16), grouped by term
     const uint32_t *pden;       // the same postings as record | position << 24, for dense parts (null: none)
-    const uint32_t *doffs;      // dense parts: per grid part g <= dng and term t, the first posting of t
-                                // at or after record g x the grid width (row g, dterms entries; k_dense_offs)
-    uint32_t dng;               // grid parts of the dense offset table (dng + 1 rows)
+    const uint2 *doffs;         // dense parts: per grid part g < dng and term t, the first postings of t at
+                                // or after records g and g + 1 x the grid width (row g, dterms entries; k_dense_offs)
+    uint32_t dng;               // grid parts of the dense offset table (its rows)
     uint32_t dterms;            // terms (the offset table's row length)
     const uint32_t *dl;         // the dense queries in ascending order (k_dense_list)
     const uint32_t *ndq_dev;    // their number (device)
@@ -2048,7 +2056,7 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
         // token starts, a row's marks become a ballot, and each lane's token is the number
         // of starts at or before its position (mbcnt), so no posting needs a test. ----
         PHASE_START(r0);
-        const uint32_t *const row0 = A.doffs + (size_t)ugp * A.dterms, *const row1 = row0 + A.dterms;
+        const uint2 *const row = A.doffs + (size_t)ugp * A.dterms;
         uint32_t lo[PER], dc[PER];
         {
             if (pf_it == it) {   // prefetched by the unit before
@@ -2063,9 +2071,9 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
                     const uint32_t j = (uint32_t)lane * PER + k;
                     uint32_t a = 0, b = 0;
                     if (j < qp) {
-                        const uint32_t tk = A.qtok[qpos + j];
-                        a = row0[tk];
-                        b = row1[tk];
+                        const uint2 ab = row[A.qtok[qpos + j]];
+                        a = ab.x;
+                        b = ab.y;
                     }
                     lo[k] = a;
                     dc[k] = b;
@@ -2188,8 +2196,9 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
 #pragma unroll
             for (int k = 0; k < PER; k++) {
                 const bool in = (uint32_t)lane * PER + k < pf_qp;
-                pf_lo[k] = in ? row0[pf_tok[k]] : 0u;
-                pf_hi[k] = in ? row1[pf_tok[k]] : 0u;
+                const uint2 ab = in ? row[pf_tok[k]] : make_uint2(0u, 0u);
+                pf_lo[k] = ab.x;
+                pf_hi[k] = ab.y;
             }
         }
         wave_sync();
@@ -2657,7 +2666,7 @@ constexpr int64_t kDenseTableBudget = (int64_t)2 << 30;   // bytes
 static int64_t dense_grid_parts(const fgss_device_sets *d, uint32_t dw) { return (d->n + dw - 1) / dw; }
 static bool dense_table_fits(const fgss_device_sets *d)
 {
-    return (int64_t)d->n_terms * (dense_grid_parts(d, kDenseGridMin) + 1) * 4 <= kDenseTableBudget;
+    return (int64_t)d->n_terms * dense_grid_parts(d, kDenseGridMin) * 8 <= kDenseTableBudget;
 }
 // the run's dense grid (records per grid part; 0: no dense parts)
 static uint32_t dense_w_of(const fgss_device_sets *d, const RunCfg &cfg)
@@ -2690,7 +2699,7 @@ static int prepare_buffers(fgss_device_sets *d)
     if (dense_ok(d)) GROW(d->b_pden, (size_t)(E + kWave) * 4);
     if (dense_ok(d) && dense_table_fits(d)) {   // sized for the finest grid (the largest table)
         const int64_t ng = dense_grid_parts(d, kDenseGridMin);
-        GROW(d->b_doffs, std::max<size_t>((size_t)d->n_terms * (size_t)(ng + 1) * 4, 16));
+        GROW(d->b_doffs, std::max<size_t>((size_t)d->n_terms * (size_t)ng * 8, 16));
         GROW(d->b_dq, (size_t)(ng + 1) * 4);
         GROW(d->b_dtpre, (size_t)(ng + 1) * 8);
     }
@@ -2736,7 +2745,7 @@ static JoinArgs join_args(fgss_device_sets *d)
     A.qtok = d->tok; A.qsig = d->sig; A.qsig64 = d->sig64; A.qsigbad = d->sigbad;
     A.rng = P<uint2>(d->b_rng); A.poff = P<uint32_t>(d->b_poff); A.bounds = P<uint32_t>(d->b_bounds);
     A.pden = dense_ok(d) ? P<uint32_t>(d->b_pden) : nullptr;
-    A.doffs = P<uint32_t>(d->b_doffs); A.dterms = (uint32_t)d->n_terms; A.dqlo = P<uint32_t>(d->b_dq); A.dtpre = P<uint64_t>(d->b_dtpre);
+    A.doffs = P<uint2>(d->b_doffs); A.dterms = (uint32_t)d->n_terms; A.dqlo = P<uint32_t>(d->b_dq); A.dtpre = P<uint64_t>(d->b_dtpre);
     A.dl = P<uint32_t>(d->b_dlist); A.ndq_dev = P<uint32_t>(d->b_misc) + M_NDENSE;
     A.n_tiles_dev = d->b_doffs.p ? (const unsigned long long *)(P<uint32_t>(d->b_misc) + M_NDU64) : nullptr;
     A.hi = P<uint32_t>(d->b_hi); A.nparts = P<uint32_t>(d->b_nparts); A.n = d->n;
@@ -2848,7 +2857,7 @@ static int enqueue_stage_plan(fgss_device_sets *d, const RunCfg &cfg)
                        d->tok, d->pos, d->len, n, d->job, d->tj, d->jend, probelen, poff, loff, psid, fence, cfg.target,
                        cfg.heavy_target, P<uint2>(d->b_rng), hi, nparts, eq, bcnt, cfg.stream_parts,
                        dense_w, cfg.dense_pct, (unsigned long long *)(misc + M_NDP64), cfg.block_records,
-                       dense_w ? dflag : (uint32_t *)nullptr, dense_w ? (const uint32_t *)P<uint32_t>(d->b_doffs) : nullptr,
+                       dense_w ? dflag : (uint32_t *)nullptr, dense_w ? (const uint2 *)P<uint2>(d->b_doffs) : nullptr,
                        (uint32_t)d->n_terms);
     HIPCK(hipGetLastError());
     if (dense_w) {   // the dense query list and work units (nothing when none were planned)
