@@ -2057,6 +2057,9 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
         // of starts at or before its position (mbcnt), so no posting needs a test. ----
         PHASE_START(r0);
         const uint2 *const row = A.doffs + (size_t)ugp * A.dterms;
+        // probe token k x kWave + lane is this lane's k-th (each load reads 64 consecutive
+        // tokens); the flattened order below is (lane, k), and every posting carries its
+        // token's query position, so no step needs the tokens in query order
         uint32_t lo[PER], dc[PER];
         {
             if (pf_it == it) {   // prefetched by the unit before
@@ -2068,7 +2071,7 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
             } else {
 #pragma unroll
                 for (int k = 0; k < PER; k++) {
-                    const uint32_t j = (uint32_t)lane * PER + k;
+                    const uint32_t j = (uint32_t)k * kWave + (uint32_t)lane;
                     uint32_t a = 0, b = 0;
                     if (j < qp) {
                         const uint2 ab = row[A.qtok[qpos + j]];
@@ -2092,7 +2095,7 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
                     pf_qp = min(fgss_max_prefix(lq2, t2), lq2);
 #pragma unroll
                     for (int k = 0; k < PER; k++) {
-                        const uint32_t j = (uint32_t)lane * PER + k;
+                        const uint32_t j = (uint32_t)k * kWave + (uint32_t)lane;
                         pf_tok[k] = j < pf_qp ? A.qtok[qpos2 + j] : 0u;
                     }
                     pf_it = it + 1;
@@ -2127,7 +2130,7 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
         for (int k = 0; k < PER; k++) {
             const bool ne = dc[k] != 0;
             *(ne ? &cur4[rk] : &dump[lane]) = (lo[k] - run) * 4u;
-            *(ne ? &jw[rk] : &dump[lane]) = (uint32_t)lane * PER + k;
+            *(ne ? &jw[rk] : &dump[lane]) = (uint32_t)k * kWave + (uint32_t)lane;
             dbeg[k] = ne ? run : ~0u;
             rk += ne ? 1u : 0u;
             run += dc[k];
@@ -2195,7 +2198,7 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
         if (pf_it == it + 1) {
 #pragma unroll
             for (int k = 0; k < PER; k++) {
-                const bool in = (uint32_t)lane * PER + k < pf_qp;
+                const bool in = (uint32_t)k * kWave + (uint32_t)lane < pf_qp;
                 const uint2 ab = in ? row[pf_tok[k]] : make_uint2(0u, 0u);
                 pf_lo[k] = ab.x;
                 pf_hi[k] = ab.y;
