@@ -127,7 +127,8 @@ typedef struct {
     int64_t phase_cycles[6];   /* join kernel shader cycles summed over wavefronts: item setup,
                                   candidate generation, positional filter, signature bound,
                                   token verification, and the part of candidate generation
-                                  spent locating posting ranges                        */
+                                  spent locating posting ranges (zeros unless the library is
+                                  built with -DFGSS_PHASE_CLOCKS=1)                    */
     int64_t window_retries;    /* streamed windows of several parts whose table overflowed and that
                                   ran again part by part (streamed items join a growing number of
                                   consecutive parts at a time while the table stays sparse)  */

@@ -1178,7 +1178,7 @@ __device__ __forceinline__ bool table_insert(const uint2 (&pe)[N], uint32_t valm
 #define FGSS_UNROLL 4
 #endif
 #ifndef FGSS_PHASE_CLOCKS
-#define FGSS_PHASE_CLOCKS 1
+#define FGSS_PHASE_CLOCKS 0   // 1: the per-phase shader clocks (phase_cycles; they cost 1-4% of the join)
 #endif
 #ifndef FGSS_TABLE_FACTOR
 #define FGSS_TABLE_FACTOR 2
@@ -1962,6 +1962,19 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
     unsigned long long c0 = clock64(), c1 = 0;
     (void)c0;
     (void)c1;
+    auto lane_u64 = [&](int k) -> unsigned long long {   // counter k, wave-uniform
+        return (unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)wv, k) |
+               ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(wv >> 32), k) << 32);
+    };
+    // the current job's postings, candidates and intersections into the run totals (the
+    // job counters hold every unit's, the totals none until the job's counters are left)
+    auto fold_job = [&]() {
+        const unsigned long long e = lane_u64(W_JOB0 + 0), cd = lane_u64(W_JOB0 + 1), in = lane_u64(W_JOB0 + 2);
+        WACC(W_ESCAN, e);
+        WACC(W_DPOST, e);
+        WACC(W_CAND, cd);
+        WACC(W_INTER, in);
+    };
     const uint64_t qbase = nwaves_grid * g0;
     uint64_t batch_next = min(gw * g0, n_items), batch_end = min(gw * g0 + g0, n_items);
     uint32_t cur_job = kEmpty;
@@ -2037,11 +2050,11 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
             if (cur_job != kEmpty) {
 #pragma unroll
                 for (int k = 0; k < kJobCtr; k++) {
-                    const unsigned long long v = (unsigned long long)__builtin_amdgcn_readlane((uint32_t)wv, W_JOB0 + k) |
-                                                 ((unsigned long long)__builtin_amdgcn_readlane((uint32_t)(wv >> 32), W_JOB0 + k) << 32);
+                    const unsigned long long v = lane_u64(W_JOB0 + k);
                     if (lane == 0 && v) atomicAdd(&A.ctr_job[(size_t)cur_job * kJobCtr + k], v);
                 }
             }
+            fold_job();
             wv = lane >= W_JOB0 && lane < W_JOB0 + kJobCtr ? 0ull : wv;
             cur_job = jq;
             cur_t = t;
@@ -2436,19 +2449,18 @@ __device__ __forceinline__ void dense_units(const JoinArgs &A, uint32_t rblock, 
         const uint32_t cand = (uint32_t)__builtin_amdgcn_readlane((int)wave_inclusive_scan_dpp(candl), kWave - 1);
         if (nl) flush();
         wave_sync();
+        // (the totals of postings, candidates and intersections are the job counters'
+        // sums: fold_job adds them when a job's counters are flushed and at the end)
         WACC(W_JOB0 + 0, total);
         WACC(W_JOB0 + 1, cand);
         WACC(W_JOB0 + 2, inter);
         WACC(W_JOB0 + 3, nres);
-        WACC(W_ESCAN, total);
-        WACC(W_CAND, cand);
         WACC(W_SIG, nsig);
         WACC(W_VER, nver);
-        WACC(W_INTER, inter);
         WACC(W_VTOK, vtok);
         WACC(W_DENSE, 1);
-        WACC(W_DPOST, total);
     }
+    fold_job();
     wave_sync();
     if (lane < kAccSlots) slot[lane] = wv;
     if (lane == kAccSlots) slot[lane] = cur_job;
